@@ -1,0 +1,178 @@
+/*
+ * vgcut — the per-frame GPU-driven cut of three-virtual-geometry on MI355X (gfx950), as a C ABI.
+ *
+ * The reference runs this path as TSL compute nodes dispatched by VirtualGeometry.update()
+ * (src/core/runtime/VirtualGeometry.ts:392-507) over the storage buffers of a GeometryPool
+ * (src/core/runtime/GeometryPool.ts:568-1092). Each entry point below names the reference interface
+ * it replaces. Buffers crossing this boundary keep the reference's layouts:
+ *   meshlet bounds   16 f32 per meshlet (buildVirtualMesh.ts:85-91)
+ *   meshlet info     4 u32 per meshlet  (buildVirtualMesh.ts:92-96)
+ *   instance matrix  16 f32 column-major per instance (VirtualMesh.ts:32-37)
+ *   counters         GeometryPool.ts:598-640 (12 globals, per (cut, slot, part) count/start, 8 state words)
+ *   draw args        5 u32 per (cut, slot, part), drawIndexedIndirect (GeometryPool.ts:605-606, 976-995)
+ *   draw list        uvec4 per drawn meshlet (instance, meshlet, firstIndex, triCount | lo << 8 | hi << 16)
+ *   HZB              68-float header + mips (OcclusionCulling.ts:30-34, 92-113)
+ * Instance ids in the draw list are GPU slots: the library stores each mesh's instances in Morton
+ * order (VirtualMesh.ts:171-174); vg_instances_set returns the order.
+ *
+ * Errors are negative return codes; vg_last_error() describes the last one. Calls on one context are
+ * not thread safe and are stream-ordered on the context's stream.
+ */
+#ifndef VGCUT_H
+#define VGCUT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VG_OK 0
+#define VG_E_INVALID (-1)
+#define VG_E_NOMEM (-2)
+#define VG_E_CAPACITY (-3)
+#define VG_E_DEVICE (-4)
+
+/* Per-context capacities (VirtualGeometryOptions, VirtualGeometry.ts:62-75, 191-198). */
+typedef struct vg_config {
+  uint32_t max_triangles;        /* camera triangles per view per frame (default 10M) */
+  uint32_t max_shadow_triangles; /* shadow triangles per view per frame (default 4M) */
+  uint32_t max_meshlets;         /* drawn meshlets per cut per view (default 1 << 20, at most 1 << 24) */
+  uint32_t max_views;            /* view slots: controller state, HZB and outputs are per slot */
+  uint32_t mesh_slots;           /* mesh slot capacity of a scene (GeometryPool meshSlotCapacity, default 64) */
+  uint32_t hzb_max_size;         /* largest HZB side (OcclusionCulling maxSize, default 1024) */
+  uint32_t budget_bins;          /* thresholds of the triangle-budget histogram (default 256) */
+  int32_t device;                /* HIP device ordinal; negative: host-only context (pool layout, no cuts) */
+} vg_config;
+
+/* The VirtualMeshData fields the cut reads (buildVirtualMesh.ts:75-138). */
+typedef struct vg_mesh_desc {
+  uint32_t meshlet_count;
+  const float* meshlet_bounds;      /* 16 per meshlet */
+  const uint32_t* meshlet_info;     /* 4 per meshlet */
+  uint32_t level_count;
+  const uint32_t* level_ranges;     /* 2 per level */
+  const float* level_errors;        /* 2 per level */
+  const float* level_guard_errors;  /* 2 per level */
+  float bounding_sphere[4];
+  float lod_bounds_sphere[4];
+  float cull_bounds_sphere[4];
+} vg_mesh_desc;
+
+/*
+ * One view of a cut: the uniforms VirtualGeometry.update() sets (VirtualGeometry.ts:78-183, 400-437).
+ * `slot` picks the view slot whose controller state, HZB and outputs this view uses.
+ */
+typedef struct vg_view {
+  float view_matrix[16];        /* camera.matrixWorldInverse, column-major */
+  float projection_matrix[16];  /* unjittered projection, column-major (occlusion test) */
+  float frustum_planes[24];     /* 6 x (normal xyz, constant), Frustum.setFromProjectionMatrix(P * V) */
+  float shadow_sweep[3];
+  float proj_scale;             /* projection[5] * viewport height / 2 */
+  float near_plane;
+  float error_threshold;        /* pixels */
+  float shadow_error_scale;     /* default 3 */
+  float lod_blend_max;          /* max(1, lodBlend), default 2 */
+  float lod_scale_limit;        /* max(1, maxErrorThreshold / errorThreshold) */
+  float min_pixel_radius;       /* default 0.7 */
+  uint32_t frustum_culling;     /* 1: on */
+  uint32_t occlusion;           /* 1: occlusion culling wanted this frame (needs a valid HZB in the slot) */
+  uint32_t slot;                /* view slot, < max_views */
+  uint32_t triangle_budget;     /* 0: off; else the error threshold is chosen on the GPU (see vg_cut) */
+  float threshold_range[2];     /* budget: smallest and largest threshold (VirtualGeometry.thresholdRange) */
+  float max_error_threshold;    /* budget: maxErrorThreshold (VirtualGeometry.ts:124) */
+} vg_view;
+
+/* Work metrics of the last cut of a view slot (each per cut: camera, shadow). */
+typedef struct vg_metrics {
+  uint64_t cells_tested;
+  uint64_t instances_tested;
+  uint64_t work_items[2];
+  uint64_t clusters_tested[2];  /* meshlet-pass range lengths + fast-path meshlets, retries included */
+  uint64_t occlusion_tests;
+  float error_threshold;        /* threshold the cut used (the budget's pick in budget mode) */
+  uint32_t budget_bin;          /* budget mode: chosen bin */
+} vg_metrics;
+
+typedef struct vg_ctx vg_ctx;
+typedef struct vg_scene vg_scene;
+
+/* VirtualGeometry constructor (VirtualGeometry.ts:191-198). NULL config: defaults. */
+int vg_create(const vg_config* config, vg_ctx** out);
+void vg_destroy(vg_ctx* ctx);
+const char* vg_last_error(const vg_ctx* ctx);
+
+/* A resident scene = one GeometryPool (GeometryPool.ts:260-280). */
+int vg_scene_create(vg_ctx* ctx, vg_scene** out);
+void vg_scene_destroy(vg_scene* scene);
+/* Device bytes a scene holds (admission by HBM footprint). */
+uint64_t vg_scene_bytes(const vg_scene* scene);
+
+/* GeometryPool.add geometry part (GeometryPool.ts:302-355, 386-460): returns the mesh slot. */
+int vg_mesh_add(vg_scene* scene, const vg_mesh_desc* mesh, uint32_t* slot);
+/*
+ * Instances of a mesh (GeometryPool.writeInstances + computeCell, GeometryPool.ts:484-545; Morton order
+ * VirtualMesh.ts:171-174, GeometryPool.ts:1263-1311). `matrices`: 16 f32 per instance; `colors`: 4 f32
+ * per instance or NULL. `gpu_to_input` (n entries, may be NULL) receives the caller instance of each slot.
+ */
+int vg_instances_set(vg_scene* scene, uint32_t slot, const float* matrices, const float* colors, uint32_t count,
+                     uint32_t* gpu_to_input);
+/* GeometryPool.writeFlags (GeometryPool.ts:463-477). max_draw_distance >= 1e30: none. */
+int vg_mesh_flags(vg_scene* scene, uint32_t slot, float max_draw_distance, float min_pixel_radius, uint32_t enabled,
+                  uint32_t shadow, uint32_t occlusion);
+
+/* OcclusionCulling HZB copy + reduce (OcclusionCulling.ts:275-352) from a host depth image (row 0 at top). */
+int vg_hzb_set(vg_scene* scene, uint32_t view_slot, const float* depth, uint32_t width, uint32_t height);
+int vg_hzb_invalidate(vg_scene* scene, uint32_t view_slot);
+/* The slot's HZB buffer (OcclusionCulling hzbAttribute): `size_floats` receives its length; dst may be NULL. */
+int vg_hzb_read(vg_scene* scene, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats);
+
+/*
+ * The compute part of VirtualGeometry.update() (VirtualGeometry.ts:462-491): reset, cells, instances,
+ * meshlets, prefix, capacity retry rounds and steer for `count` views in one launch sequence, enqueued
+ * on the context's stream. Views must use distinct slots. With `triangle_budget` > 0 a view's error
+ * threshold is the smallest of `budget_bins` log-spaced thresholds over threshold_range whose plain cut
+ * has at most that many triangles (projected-error histogram), else the largest.
+ */
+int vg_cut(vg_scene* scene, const vg_view* views, uint32_t count);
+/* Same as vg_cut, replayed from a captured HIP graph when the view count matches the last capture. */
+int vg_cut_graph(vg_scene* scene, const vg_view* views, uint32_t count);
+/* Waits for the context's stream. */
+int vg_sync(vg_scene* scene);
+
+/*
+ * Reads a view slot's results after vg_sync (GeometryPool.readCounters, GeometryPool.ts:1135-1151):
+ * counters: 12 + 8 * mesh_slots + 8 words; draw_args: 20 * mesh_slots words; draw_list: up to
+ * `max_entries` uvec4 per cut (camera entries, then shadow entries at offset max_entries). Any pointer
+ * may be NULL.
+ */
+int vg_read(vg_scene* scene, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list,
+            uint32_t max_entries, vg_metrics* metrics);
+/* Capacity-controller state words of a view slot (8 u32, x10000). */
+int vg_state_get(vg_scene* scene, uint32_t view_slot, uint32_t* state);
+int vg_state_set(vg_scene* scene, uint32_t view_slot, const uint32_t* state);
+
+/*
+ * The pool's host-side buffers in the reference layout (GeometryPool.ts:221-231, the typed arrays behind
+ * its GrowBuffers): what a debugger or CPU consumer of the pool reads. `size_bytes` receives the size;
+ * `dst` may be NULL to query it. Instances are in GPU (Morton) order: 4 matrix columns, then
+ * (tint rgb = 1, mesh slot).
+ */
+#define VG_POOL_MESH_TABLE 0     /* f32: 24 per mesh slot, then the level tables (8 per level) */
+#define VG_POOL_INSTANCES 1      /* f32: 20 per instance */
+#define VG_POOL_CELLS 2          /* f32: 8 per cell */
+#define VG_POOL_MESHLET_BOUNDS 3 /* f32: 16 per meshlet (pool-global ids) */
+#define VG_POOL_MESHLET_INFO 4   /* u32: 4 per meshlet, firstIndex rebased to the pool */
+int vg_scene_pool_read(const vg_scene* scene, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes);
+/* Highest used mesh slot + 1 (GeometryPool.meshSlots). */
+uint32_t vg_scene_mesh_slots(const vg_scene* scene);
+/* First GPU instance slot and instance count of a mesh (PoolEntry.instanceBase/instanceCount, GeometryPool.ts:161-162). */
+int vg_mesh_instance_base(const vg_scene* scene, uint32_t slot, uint32_t* base, uint32_t* count);
+
+/* Layout helpers: counter words and draw-args words for the context's mesh slot capacity. */
+uint32_t vg_counter_words(const vg_ctx* ctx);
+uint32_t vg_draw_args_words(const vg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,249 @@
+"""ORACLE — test infrastructure only: ctypes access to the C restatement (vgoracle.c).
+
+Only tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use this module, as the
+checker. Inputs are the pool buffers in the reference layout (GeometryPool.ts:221-231) and a view
+record (VirtualGeometry.ts:78-183).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "liboracle.so")
+
+
+class OrScene(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("mesh_slots", ctypes.c_uint32),
+        ("mesh_table", ctypes.c_void_p),
+        ("instances", ctypes.c_void_p),
+        ("instance_count", ctypes.c_uint32),
+        ("cells", ctypes.c_void_p),
+        ("cell_count", ctypes.c_uint32),
+        ("bounds", ctypes.c_void_p),
+        ("info", ctypes.c_void_p),
+        ("meshlet_count", ctypes.c_uint32),
+        ("max_meshlets", ctypes.c_uint32),
+        ("cut_capacity", ctypes.c_uint32 * 2),
+    ]
+
+
+class OrView(ctypes.Structure):
+    _fields_ = [
+        ("view", ctypes.c_float * 16),
+        ("proj", ctypes.c_float * 16),
+        ("planes", ctypes.c_float * 24),
+        ("sweep", ctypes.c_float * 3),
+        ("proj_scale", ctypes.c_float),
+        ("near_plane", ctypes.c_float),
+        ("error_threshold", ctypes.c_float),
+        ("shadow_error_scale", ctypes.c_float),
+        ("lod_blend_max", ctypes.c_float),
+        ("lod_scale_limit", ctypes.c_float),
+        ("min_pixel_radius", ctypes.c_float),
+        ("frustum_culling", ctypes.c_uint32),
+        ("occlusion_active", ctypes.c_uint32),
+        ("hzb", ctypes.c_void_p),
+    ]
+
+
+class OrStats(ctypes.Structure):
+    _fields_ = [
+        ("cells_tested", ctypes.c_uint64),
+        ("instances_tested", ctypes.c_uint64),
+        ("work_items", ctypes.c_uint64 * 2),
+        ("clusters_tested", ctypes.c_uint64 * 2),
+        ("occlusion_tests", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64 * 2),
+        ("lod_checked", ctypes.c_uint64),
+        ("near_threshold", ctypes.c_uint64),
+        ("mismatch_near", ctypes.c_uint64),
+        ("mismatch_hard", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so with the recipe in oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            build()
+        lib = ctypes.CDLL(LIBRARY)
+        lib.or_counter_words.argtypes = [ctypes.c_uint32]
+        lib.or_counter_words.restype = ctypes.c_uint32
+        lib.or_cut.argtypes = [ctypes.POINTER(OrScene), ctypes.POINTER(OrView), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrStats)]
+        lib.or_cut.restype = ctypes.c_int
+        lib.or_hzb_size.argtypes = [ctypes.c_uint32]
+        lib.or_hzb_size.restype = ctypes.c_uint32
+        lib.or_hzb_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        lib.or_hzb_build.restype = ctypes.c_int
+        lib.or_occluded.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+        lib.or_occluded.restype = ctypes.c_int
+        lib.or_logf.argtypes = [ctypes.c_float]
+        lib.or_logf.restype = ctypes.c_float
+        lib.or_budget_histogram.argtypes = [ctypes.POINTER(OrScene), ctypes.POINTER(OrView), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_uint32, ctypes.c_void_p]
+        lib.or_budget_histogram.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+@dataclass
+class Pool:
+    """Pool buffers in the reference layout (what the oracle reads)."""
+
+    mesh_table: np.ndarray  # f32
+    instances: np.ndarray  # f32, 20 per instance
+    cells: np.ndarray  # f32, 8 per cell
+    bounds: np.ndarray  # f32, 16 per meshlet
+    info: np.ndarray  # u32, 4 per meshlet
+    slots: int
+    mesh_slots: int
+    max_meshlets: int
+    cut_capacity: tuple
+
+    def struct(self) -> OrScene:
+        s = OrScene()
+        s.slots = self.slots
+        s.mesh_slots = self.mesh_slots
+        s.mesh_table = self.mesh_table.ctypes.data
+        s.instances = self.instances.ctypes.data
+        s.instance_count = len(self.instances) // 20
+        s.cells = self.cells.ctypes.data
+        s.cell_count = len(self.cells) // 8
+        s.bounds = self.bounds.ctypes.data
+        s.info = self.info.ctypes.data
+        s.meshlet_count = len(self.info) // 4
+        s.max_meshlets = self.max_meshlets
+        s.cut_capacity[0] = self.cut_capacity[0]
+        s.cut_capacity[1] = self.cut_capacity[1]
+        return s
+
+
+def pool_from_scene(scene, max_meshlets: int, max_triangles: int, max_shadow_triangles: int) -> Pool:
+    """Reads a vgcut scene's host-side pool buffers (vg_scene_pool_read)."""
+    from vgcut import native
+
+    def f(b, dtype):
+        a = scene.pool(b)
+        return np.ascontiguousarray(a, dtype=dtype)
+
+    return Pool(
+        mesh_table=f(native.VG_POOL_MESH_TABLE, np.float32),
+        instances=f(native.VG_POOL_INSTANCES, np.float32),
+        cells=f(native.VG_POOL_CELLS, np.float32),
+        bounds=f(native.VG_POOL_MESHLET_BOUNDS, np.float32),
+        info=f(native.VG_POOL_MESHLET_INFO, np.uint32),
+        slots=scene.ctx.config.mesh_slots,
+        mesh_slots=scene.mesh_slots,
+        max_meshlets=max_meshlets,
+        cut_capacity=(max_triangles * 3, max_shadow_triangles * 3),
+    )
+
+
+def view_struct(v, hzb: Optional[np.ndarray]) -> OrView:
+    """An oracle view from a vgcut View (the same f32 uniforms)."""
+    o = OrView()
+    for i in range(16):
+        o.view[i] = v.view_matrix[i]
+        o.proj[i] = v.projection_matrix[i]
+    for i in range(24):
+        o.planes[i] = v.frustum_planes[i]
+    for i in range(3):
+        o.sweep[i] = v.shadow_sweep[i]
+    o.proj_scale = v.proj_scale
+    o.near_plane = v.near_plane
+    o.error_threshold = v.error_threshold
+    o.shadow_error_scale = v.shadow_error_scale
+    o.lod_blend_max = v.lod_blend_max
+    o.lod_scale_limit = v.lod_scale_limit
+    o.min_pixel_radius = v.min_pixel_radius
+    o.frustum_culling = v.frustum_culling
+    o.occlusion_active = 1 if (v.occlusion and hzb is not None) else 0
+    o.hzb = hzb.ctypes.data if hzb is not None else None
+    return o
+
+
+@dataclass
+class CutResult:
+    counters: np.ndarray
+    args: np.ndarray
+    draws: np.ndarray  # (2, max_meshlets, 4)
+    state: np.ndarray
+    stats: OrStats
+
+
+def initial_state() -> np.ndarray:
+    """Controller words at pool creation: scale 1 for both cuts (GeometryPool.ts:602-603)."""
+    s = np.zeros(8, dtype=np.uint32)
+    s[0] = s[1] = 10000
+    return s
+
+
+def cut(pool: Pool, view, state: Optional[np.ndarray] = None, hzb: Optional[np.ndarray] = None, fp64: bool = False) -> CutResult:
+    lib = load()
+    st = initial_state() if state is None else np.ascontiguousarray(state, dtype=np.uint32).copy()
+    counters = np.zeros(lib.or_counter_words(pool.slots), dtype=np.uint32)
+    args = np.zeros(pool.slots * 20, dtype=np.uint32)
+    draws = np.zeros((2, pool.max_meshlets, 4), dtype=np.uint32)
+    stats = OrStats()
+    sc = pool.struct()
+    hz = None if hzb is None else np.ascontiguousarray(hzb, dtype=np.float32)
+    vs = view_struct(view, hz)
+    rc = lib.or_cut(ctypes.byref(sc), ctypes.byref(vs), st.ctypes.data, counters.ctypes.data, args.ctypes.data, draws.ctypes.data,
+                    1 if fp64 else 0, ctypes.byref(stats))
+    if rc != 0:
+        raise RuntimeError(f"or_cut failed ({rc})")
+    return CutResult(np.concatenate([counters, st]), args, draws, st, stats)
+
+
+def hzb_build(depth: np.ndarray, max_size: int = 1024) -> np.ndarray:
+    lib = load()
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    h, w = d.shape
+    out = np.zeros(lib.or_hzb_size(max_size), dtype=np.float32)
+    rc = lib.or_hzb_build(d.ctypes.data, w, h, max_size, out.ctypes.data)
+    if rc != 0:
+        raise ValueError("or_hzb_build: depth larger than max_size")
+    return out
+
+
+def occluded(proj, near: float, hzb: np.ndarray, view_center, radius: float) -> bool:
+    lib = load()
+    p = np.ascontiguousarray(proj, dtype=np.float32)
+    c = np.ascontiguousarray(view_center, dtype=np.float32)
+    return bool(lib.or_occluded(p.ctypes.data, near, hzb.ctypes.data, c.ctypes.data, radius))
+
+
+def logf(x: float) -> float:
+    return float(load().or_logf(x))
+
+
+def budget_histogram(pool: Pool, view, thresholds: np.ndarray, state: Optional[np.ndarray] = None,
+                     hzb: Optional[np.ndarray] = None) -> np.ndarray:
+    lib = load()
+    st = initial_state() if state is None else np.ascontiguousarray(state, dtype=np.uint32)
+    t = np.ascontiguousarray(thresholds, dtype=np.float32)
+    out = np.zeros(len(t), dtype=np.uint64)
+    sc = pool.struct()
+    hz = None if hzb is None else np.ascontiguousarray(hzb, dtype=np.float32)
+    vs = view_struct(view, hz)
+    rc = lib.or_budget_histogram(ctypes.byref(sc), ctypes.byref(vs), st.ctypes.data, t.ctypes.data, len(t), out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("or_budget_histogram failed")
+    return out

@@ -1,0 +1,177 @@
+"""GPU parity: the HIP cut through the C ABI against the oracle on the same inputs.
+
+Bar: counters, controller state and draw args bit-exact; drawn-cluster sets (instance, meshlet,
+triangles | blend range) identical after sorting; per (slot, part) the firstIndex ranges tile the
+requested index count; work metrics (cells, instances, work items, cluster tests, occlusion tests)
+equal.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import compare_cut, ground_camera, load_gpu, small_ruins, tiling_ok
+from oracle import oracle
+from vgcut import camera as cam
+from vgcut import native, scenegen, scenes
+from vgcut.virtual_geometry import make_view
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_arithmetic_matches_host(gpu_available):
+    """Division, sqrt, the shared log, round-half-even and u32 -> f32 give the host's bits on gfx950."""
+    rng = np.random.default_rng(1)
+    n = 1 << 18
+    a = (rng.standard_normal(n) * np.exp(rng.uniform(-30, 30, n))).astype(np.float32)
+    b = (rng.standard_normal(n) * np.exp(rng.uniform(-30, 30, n))).astype(np.float32)
+    a[:8] = [0.5, 1.5, 2.5, -0.5, 1.0, 3.0, 1e-30, 2e30]
+    b[b == 0] = 1.0
+    out = native.arith_probe(a, b)
+    with np.errstate(all="ignore"):
+        assert np.array_equal(out[:, 0].view(np.uint32), (a / b).astype(np.float32).view(np.uint32)), "division"
+        assert np.array_equal(out[:, 1].view(np.uint32), np.sqrt(np.abs(a)).view(np.uint32)), "sqrt"
+    host_log = np.array([oracle.logf(float(x)) for x in np.abs(a[:4096])], dtype=np.float32)
+    assert np.array_equal(out[:4096, 2].view(np.uint32), host_log.view(np.uint32)), "log"
+    assert np.array_equal(out[:, 3], np.rint(a)), "rint"
+    assert np.array_equal(out[:, 4], (b.view(np.uint32) >> 3).astype(np.float32)), "u32 -> f32"
+
+
+def _run(loaded, views, hzbs=None, fp64=False):
+    loaded.scene.cut(views)
+    loaded.scene.sync()
+    results = []
+    for v in views:
+        gpu = loaded.scene.read(v.slot, max_entries=loaded.ctx.config.max_meshlets)
+        results.append(gpu)
+    return results
+
+
+def test_config0_icosphere_single_view(gpu_available):
+    spec, camera, height = scenes.config0()
+    loaded = load_gpu(spec, max_meshlets=1 << 20, max_triangles=10_000_000, max_shadow=4_000_000)
+    view = make_view(camera, height)
+    gpu, = _run(loaded, [view])
+    ref = oracle.cut(loaded.pool, view, fp64=True)
+    compare_cut(gpu, ref, 1 << 20, "config0")
+    assert ref.stats.mismatch_hard == 0 and ref.stats.near_threshold == 0
+    n = int(ref.counters[2])
+    assert n > 500 and tiling_ok(gpu[2][0], n, gpu[0], 0, 64, loaded.slot_of_instance)
+
+
+@pytest.mark.parametrize("threshold,shadows,seed", [(1.0, False, 3), (0.5, True, 4), (2.0, True, 5), (4.0, False, 6)])
+def test_ruins_views(gpu_available, threshold, shadows, seed):
+    """Many instances and meshes: frustum, draw distance, min pixel radius, shadow cut with a sweep, blend band."""
+    spec = small_ruins(seed=seed, shadows=shadows)
+    loaded = load_gpu(spec)
+    rng = np.random.default_rng(seed)
+    for k in range(3):
+        eye = (rng.uniform(-40, 40), rng.uniform(1.5, 30), rng.uniform(-40, 40))
+        target = (rng.uniform(-60, 60), 0.0, rng.uniform(-60, 60))
+        camera = ground_camera(eye, target)
+        sweep = (0.0, 0.0, 0.0) if not shadows else (30.0, -60.0, 10.0)
+        view = make_view(camera, 1080, error_threshold=threshold, shadow_sweep=sweep)
+        gpu, = _run(loaded, [view])
+        state_before = oracle.initial_state() if k == 0 else state
+        ref = oracle.cut(loaded.pool, view, state=state_before, fp64=True)
+        state = ref.state
+        compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"ruins t={threshold} view {k}")
+        assert ref.stats.mismatch_hard == 0
+        for cut in (0, 1):
+            n = min(int(ref.counters[2 + cut * 5]), loaded.ctx.config.max_meshlets)
+            assert tiling_ok(gpu[2][cut], n, gpu[0], cut, 64, loaded.slot_of_instance)
+        assert int(ref.counters[2]) > 0
+
+
+def test_capacity_retry_and_steer_over_frames(gpu_available):
+    """Tiny draw buffers: overflowing cuts are redone coarser in the same frame; the scale glides across frames."""
+    spec = small_ruins(seed=21, shadows=True)
+    loaded = load_gpu(spec, max_meshlets=1 << 14, max_triangles=60_000, max_shadow=30_000)
+    state = oracle.initial_state()
+    retried = 0
+    for f in range(6):
+        camera = ground_camera((5.0 * f - 10, 3.0, -20.0), (0.0, 0.0, 40.0))
+        view = make_view(camera, 1080, error_threshold=0.5, shadow_sweep=(10.0, -30.0, 0.0))
+        gpu, = _run(loaded, [view])
+        ref = oracle.cut(loaded.pool, view, state=state)
+        state = ref.state
+        retried += int(ref.stats.retries[0] + ref.stats.retries[1])
+        if max(int(ref.counters[2]), int(ref.counters[7])) <= loaded.ctx.config.max_meshlets:
+            compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"capacity frame {f}")
+        else:
+            assert np.array_equal(gpu[0][-8:], ref.counters[-8:]), "controller state"
+    assert retried > 0, "the test must exercise the retry rounds"
+
+
+def test_occlusion_against_synthetic_hzb(gpu_available):
+    """Two views, each with its own HZB from a rasterized noise heightfield; HZB pyramid equal to the oracle's."""
+    spec = small_ruins(seed=31, instances=3000)
+    loaded = load_gpu(spec, max_views=2, occlusion=True)
+    views, hzbs = [], []
+    for slot, (eye, target) in enumerate([((0.0, 2.0, -70.0), (0.0, 0.0, 60.0)), ((50.0, 2.0, 10.0), (-60.0, 2.0, 0.0))]):
+        camera = ground_camera(eye, target)
+        w, h = 480, 270
+        vp = cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse)
+        depth = scenegen.heightfield_depth(vp, w, h, (0.0, 0.0), 120.0, 128, 6.0, 1 / 25.0, 77 + slot)
+        loaded.scene.set_hzb(slot, depth)
+        hzb = oracle.hzb_build(depth, 1024)
+        got = loaded.scene.hzb(slot)
+        assert np.array_equal(got[:len(hzb)], hzb), "HZB pyramid differs from the oracle"
+        views.append(make_view(camera, 540, occlusion=True, slot=slot))
+        hzbs.append(hzb)
+    results = _run(loaded, views)
+    occluded = 0
+    for v, gpu, hzb in zip(views, results, hzbs):
+        ref = oracle.cut(loaded.pool, v, hzb=hzb)
+        compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"occlusion slot {v.slot}")
+        occluded += int(ref.counters[1])
+    assert occluded > 0, "the HZB must hide something"
+
+
+def test_batched_views_equal_plain_launches(gpu_available):
+    """One launch of 6 views == 6 single-view launches == the oracle per view (graph replay included)."""
+    spec = small_ruins(seed=41, shadows=True)
+    loaded = load_gpu(spec, max_views=6)
+    rng = np.random.default_rng(41)
+    views = []
+    for slot in range(6):
+        if slot < 4:
+            camera = ground_camera((rng.uniform(-50, 50), rng.uniform(2, 40), rng.uniform(-50, 50)),
+                                   (rng.uniform(-50, 50), 0.0, rng.uniform(-50, 50)))
+            views.append(make_view(camera, [540, 720, 1080, 2160][slot], slot=slot, error_threshold=[1, 0.5, 2, 1][slot],
+                                   shadow_sweep=(20.0, -40.0, 0.0)))
+        else:
+            ortho = cam.orthographic_camera(-60, 60, 60, -60, 0.1, 500).look_at((30.0, 200.0, 20.0), (0.0, 0.0, 0.0))
+            views.append(make_view(ortho, 2048, slot=slot))
+    plain = []
+    for v in views:
+        loaded.scene.cut([v])
+        loaded.scene.sync()
+        plain.append(loaded.scene.read(v.slot))
+        # reset the slot's controller so the batched run starts from the same state
+        loaded.scene.set_state(v.slot, oracle.initial_state())
+    refs = [oracle.cut(loaded.pool, v) for v in views]
+    for graph in (False, True, True):
+        loaded.scene.cut(views, graph=graph)
+        loaded.scene.sync()
+        for v, p, ref in zip(views, plain, refs):
+            got = loaded.scene.read(v.slot)
+            compare_cut(got, ref, loaded.ctx.config.max_meshlets, f"batched slot {v.slot} graph={graph}")
+            compare_cut(p, ref, loaded.ctx.config.max_meshlets, f"plain slot {v.slot}")
+            loaded.scene.set_state(v.slot, oracle.initial_state())
+
+
+def test_graph_replays_match_eager(gpu_available):
+    """A captured launch sequence replayed several times, with eager work in between, equals eager runs."""
+    spec = small_ruins(seed=51)
+    loaded = load_gpu(spec)
+    state = oracle.initial_state()
+    for f in range(4):
+        camera = ground_camera((-30.0 + 7 * f, 2.0, -30.0), (10.0, 0.0, 30.0))
+        view = make_view(camera, 1080)
+        loaded.scene.cut([view], graph=(f % 2 == 1))
+        loaded.scene.sync()
+        gpu = loaded.scene.read(0)
+        ref = oracle.cut(loaded.pool, view, state=state)
+        state = ref.state
+        compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"graph frame {f}")

@@ -1,0 +1,922 @@
+// Host runtime of the cut behind include/vgcut.h: contexts, resident scenes (GeometryPool.ts:213-512),
+// launch sequences and HIP-graph replay (VirtualGeometry.ts:392-507), HZB builds
+// (OcclusionCulling.ts:244-352). One translation unit with the kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vgcut.h"
+#include "vgcut_kernels.hip"
+
+using namespace vg;
+
+namespace {
+
+constexpr uint32_t kInstanceStride = 16;  // floats per caller matrix
+constexpr uint32_t kMaxMeshletsPerScene = 1u << 24;  // ids travel as f32 in the level tables and cells
+constexpr uint32_t kMaxInstancesPerScene = 1u << 24;
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct MeshHost {
+  bool used = false;
+  uint32_t meshlet_base = 0, meshlet_count = 0;
+  uint32_t level_base = 0, level_count = 0;  // level_base in float4 units of the mesh table
+  float bounding[4], lod_bounds[4], cull_bounds[4];
+  uint32_t instance_base = 0, instance_count = 0;
+  std::vector<float4> cell_sphere, cell_rec;
+};
+
+}  // namespace
+
+struct vg_ctx {
+  vg_config cfg;
+  hipStream_t stream = nullptr;
+  int cus = 256;
+  std::string error;
+};
+
+struct vg_scene {
+  vg_ctx* ctx = nullptr;
+  uint32_t slots = 64;
+  std::vector<MeshHost> meshes;
+  uint32_t mesh_slots = 0;  // highest used slot + 1
+  uint32_t level_used = 0;  // float4 entries of level tables
+  // host mirrors (device layout)
+  std::vector<float4> mesh_table;
+  std::vector<float4> m_cull, m_lod, m_parent;
+  std::vector<float2> m_err;
+  std::vector<uint32_t> m_tris;
+  std::vector<uint32_t> m_info;     // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
+  uint64_t index_used = 0;          // indices of the meshes added so far
+  std::vector<float4> i_col[4];
+  std::vector<float> i_slot;        // mesh slot of each instance
+  std::vector<float4> c_sphere, c_rec;
+  bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
+  // device scene
+  DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
+  // per view slot
+  DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs;
+  std::vector<DevBuf> d_draws, d_hzbs;
+  std::vector<uint8_t> hzb_valid;
+  std::vector<uint4*> h_draw_ptrs;
+  std::vector<float*> h_hzb_ptrs;
+  uint32_t counter_stride = 0, args_stride = 0, hzb_floats = 0;
+  // launch scratch
+  DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_counter, d_flags;
+  DevBuf d_hzb_stage;
+  uint32_t view_capacity = 0;
+  size_t cell_queue_capacity = 0;
+  uint32_t work_capacity = 0;
+  ViewDev* h_views = nullptr;  // pinned staging
+  hipEvent_t views_copied = nullptr;
+  bool views_copy_pending = false;
+  uint64_t layout_version = 1;
+  // captured launch sequence
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_views = 0;
+  uint64_t graph_version = 0;
+  uint64_t device_bytes = 0;
+};
+
+namespace {
+
+int fail(vg_ctx* ctx, int code, const std::string& message) {
+  if (ctx) ctx->error = message;
+  return code;
+}
+
+int hip_fail(vg_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, VG_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define VG_HIP(ctx, expr)                                 \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+// Grows `b` to at least `bytes` (contents are not kept). Returns 0 or an error code.
+int ensure(vg_scene* s, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return 0;
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    s->device_bytes -= b.bytes;
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t n = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, n);
+  if (e != hipSuccess) return fail(s->ctx, VG_E_NOMEM, std::string("hipMalloc of ") + std::to_string(n) + " bytes failed");
+  b.bytes = n;
+  s->device_bytes += n;
+  s->layout_version++;
+  return 0;
+}
+
+void release(vg_scene* s, DevBuf& b) {
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    s->device_bytes -= b.bytes;
+  }
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+template <typename T>
+int upload(vg_scene* s, DevBuf& b, const std::vector<T>& v) {
+  if (int rc = ensure(s, b, sizeof(T) * std::max<size_t>(v.size(), 1))) return rc;
+  if (!v.empty()) VG_HIP(s->ctx, hipMemcpyAsync(b.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s->ctx->stream));
+  return 0;
+}
+
+// instanceSphere, GeometryPool.ts:1171-1180 (fp64; column lengths as sqrt of the sum of squares).
+void instance_sphere(const float* m, const float* sphere, double out[4]) {
+  const double sx = sphere[0], sy = sphere[1], sz = sphere[2], sr = sphere[3];
+  out[0] = static_cast<double>(m[0]) * sx + static_cast<double>(m[4]) * sy + static_cast<double>(m[8]) * sz + m[12];
+  out[1] = static_cast<double>(m[1]) * sx + static_cast<double>(m[5]) * sy + static_cast<double>(m[9]) * sz + m[13];
+  out[2] = static_cast<double>(m[2]) * sx + static_cast<double>(m[6]) * sy + static_cast<double>(m[10]) * sz + m[14];
+  auto col = [&](int k) {
+    const double x = m[k * 4], y = m[k * 4 + 1], z = m[k * 4 + 2];
+    return std::sqrt(x * x + y * y + z * z);
+  };
+  out[3] = sr * std::max(col(0), std::max(col(1), col(2)));
+}
+
+uint32_t spread_bits10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// mortonOrder + radixOrder, GeometryPool.ts:1263-1311: caller index of each GPU slot.
+std::vector<uint32_t> morton_order(const float* matrices, uint32_t n, const float* sphere) {
+  std::vector<double> centers(static_cast<size_t>(n) * 3);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double s[4];
+  for (uint32_t i = 0; i < n; i++) {
+    instance_sphere(matrices + static_cast<size_t>(i) * kInstanceStride, sphere, s);
+    for (int a = 0; a < 3; a++) {
+      centers[i * 3 + a] = s[a];
+      lo[a] = std::min(lo[a], s[a]);
+      hi[a] = std::max(hi[a], s[a]);
+    }
+  }
+  const double extent = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), std::max(hi[2] - lo[2], 1e-9));
+  std::vector<uint32_t> codes(n);
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+      double v = std::floor(((centers[i * 3 + a] - lo[a]) / extent) * 1024);
+      q[a] = static_cast<uint32_t>(std::min(1023.0, std::max(0.0, v)));
+    }
+    codes[i] = spread_bits10(q[0]) | (spread_bits10(q[1]) << 1) | (spread_bits10(q[2]) << 2);
+  }
+  std::vector<uint32_t> order(n), next(n), counts(1u << 15);
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  for (uint32_t shift : {0u, 15u}) {
+    std::fill(counts.begin(), counts.end(), 0u);
+    for (uint32_t i = 0; i < n; i++) counts[(codes[i] >> shift) & 0x7fffu]++;
+    uint32_t sum = 0;
+    for (auto& c : counts) {
+      uint32_t v = c;
+      c = sum;
+      sum += v;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+      uint32_t index = order[i];
+      next[counts[(codes[index] >> shift) & 0x7fffu]++] = index;
+    }
+    order.swap(next);
+  }
+  return order;
+}
+
+// computeCell, GeometryPool.ts:519-545 (fp64, stored as f32).
+void compute_cell(const std::vector<float>& gpu_matrices, uint32_t first, uint32_t end, const float* sphere, uint32_t slot,
+                  uint32_t instance_base, float4& out_sphere, float4& out_rec) {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, s[4];
+  double max_radius = 0;
+  for (uint32_t g = first; g < end; g++) {
+    instance_sphere(&gpu_matrices[static_cast<size_t>(g) * 16], sphere, s);
+    for (int a = 0; a < 3; a++) {
+      lo[a] = std::min(lo[a], s[a] - s[3]);
+      hi[a] = std::max(hi[a], s[a] + s[3]);
+    }
+    max_radius = std::max(max_radius, s[3]);
+  }
+  const double cx = (lo[0] + hi[0]) / 2, cy = (lo[1] + hi[1]) / 2, cz = (lo[2] + hi[2]) / 2;
+  double radius = 0;
+  for (uint32_t g = first; g < end; g++) {
+    instance_sphere(&gpu_matrices[static_cast<size_t>(g) * 16], sphere, s);
+    const double dx = s[0] - cx, dy = s[1] - cy, dz = s[2] - cz;
+    radius = std::max(radius, std::sqrt(dx * dx + dy * dy + dz * dz) + s[3]);
+  }
+  out_sphere = make_float4(static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz),
+                           static_cast<float>(radius * (1 + 1e-4) + 1e-3));
+  out_rec = make_float4(static_cast<float>(max_radius * (1 + 1e-4) + 1e-4), static_cast<float>(slot),
+                        static_cast<float>(instance_base + first), static_cast<float>(end - first));
+}
+
+void write_mesh_record(vg_scene* s, uint32_t slot) {
+  const MeshHost& m = s->meshes[slot];
+  float4* r = &s->mesh_table[slot * kMeshStride];
+  r[0] = make_float4(m.bounding[0], m.bounding[1], m.bounding[2], m.bounding[3]);
+  r[1] = make_float4(m.lod_bounds[0], m.lod_bounds[1], m.lod_bounds[2], m.lod_bounds[3]);
+  r[2] = make_float4(m.cull_bounds[0], m.cull_bounds[1], m.cull_bounds[2], m.cull_bounds[3]);
+  r[4] = make_float4(1, 1, 0, 0);
+}
+
+int upload_scene(vg_scene* s) {
+  int rc = 0;
+  if (s->dirty_meshlets) {
+    if ((rc = upload(s, s->d_cull, s->m_cull)) || (rc = upload(s, s->d_lod, s->m_lod)) || (rc = upload(s, s->d_parent, s->m_parent)) ||
+        (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)))
+      return rc;
+    s->dirty_meshlets = false;
+  }
+  if (s->dirty_table) {
+    if ((rc = upload(s, s->d_mesh_table, s->mesh_table))) return rc;
+    s->dirty_table = false;
+  }
+  if (s->dirty_instances) {
+    for (int k = 0; k < 4; k++)
+      if ((rc = upload(s, s->d_col[k], s->i_col[k]))) return rc;
+    s->dirty_instances = false;
+  }
+  if (s->dirty_cells) {
+    s->c_sphere.clear();
+    s->c_rec.clear();
+    for (const MeshHost& m : s->meshes) {
+      if (!m.used) continue;
+      s->c_sphere.insert(s->c_sphere.end(), m.cell_sphere.begin(), m.cell_sphere.end());
+      s->c_rec.insert(s->c_rec.end(), m.cell_rec.begin(), m.cell_rec.end());
+    }
+    if ((rc = upload(s, s->d_csphere, s->c_sphere)) || (rc = upload(s, s->d_crec, s->c_rec))) return rc;
+    s->dirty_cells = false;
+  }
+  return 0;
+}
+
+int ensure_slot(vg_scene* s, uint32_t slot) {
+  vg_ctx* ctx = s->ctx;
+  if (!s->d_draws[slot].ptr) {
+    if (int rc = ensure(s, s->d_draws[slot], sizeof(uint4) * 2 * static_cast<size_t>(ctx->cfg.max_meshlets))) return rc;
+    VG_HIP(ctx, hipMemsetAsync(s->d_draws[slot].ptr, 0, s->d_draws[slot].bytes, ctx->stream));
+    s->h_draw_ptrs[slot] = static_cast<uint4*>(s->d_draws[slot].ptr);
+    VG_HIP(ctx, hipMemcpyAsync(static_cast<uint4**>(s->d_draw_ptrs.ptr) + slot, &s->h_draw_ptrs[slot], sizeof(uint4*),
+                               hipMemcpyHostToDevice, ctx->stream));
+    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return 0;
+}
+
+Params make_params(vg_scene* s, uint32_t view_count) {
+  vg_ctx* ctx = s->ctx;
+  Params p;
+  std::memset(&p, 0, sizeof(p));
+  p.mesh_table = static_cast<const float4*>(s->d_mesh_table.ptr);
+  p.m_cull = static_cast<const float4*>(s->d_cull.ptr);
+  p.m_lod = static_cast<const float4*>(s->d_lod.ptr);
+  p.m_parent = static_cast<const float4*>(s->d_parent.ptr);
+  p.m_err = static_cast<const float2*>(s->d_err.ptr);
+  p.m_tris = static_cast<const uint32_t*>(s->d_tris.ptr);
+  p.i_col0 = static_cast<const float4*>(s->d_col[0].ptr);
+  p.i_col1 = static_cast<const float4*>(s->d_col[1].ptr);
+  p.i_col2 = static_cast<const float4*>(s->d_col[2].ptr);
+  p.i_col3 = static_cast<const float4*>(s->d_col[3].ptr);
+  p.c_sphere = static_cast<const float4*>(s->d_csphere.ptr);
+  p.c_rec = static_cast<const float4*>(s->d_crec.ptr);
+  p.cell_count = static_cast<uint32_t>(s->c_sphere.size());
+  p.mesh_slots = s->mesh_slots;
+  p.slots = s->slots;
+  p.max_meshlets = ctx->cfg.max_meshlets;
+  p.cut_capacity[0] = ctx->cfg.max_triangles * 3;
+  p.cut_capacity[1] = ctx->cfg.max_shadow_triangles * 3;
+  p.counters = static_cast<uint32_t*>(s->d_counters.ptr);
+  p.counter_stride = s->counter_stride;
+  p.args = static_cast<uint32_t*>(s->d_args.ptr);
+  p.args_stride = s->args_stride;
+  p.draws = static_cast<uint4* const*>(s->d_draw_ptrs.ptr);
+  p.hzbs = static_cast<float* const*>(s->d_hzb_ptrs.ptr);
+  p.metrics = static_cast<unsigned long long*>(s->d_metrics.ptr);
+  p.views = static_cast<ViewDev*>(s->d_views.ptr);
+  p.view_count = view_count;
+  p.cell_queue = static_cast<uint2*>(s->d_cell_queue.ptr);
+  p.cell_queue_count = static_cast<uint32_t*>(s->d_cell_queue_count.ptr);
+  p.work = static_cast<uint4*>(s->d_work.ptr);
+  p.work_start = static_cast<unsigned long long*>(s->d_work_start.ptr);
+  p.work_counter = static_cast<unsigned long long*>(s->d_work_counter.ptr);
+  p.work_capacity = s->work_capacity;
+  p.flags = static_cast<uint32_t*>(s->d_flags.ptr);
+  return p;
+}
+
+// The launch sequence of one cut (GeometryPool.ts:1081-1092): begin, cells, instances, meshlets,
+// prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer).
+int enqueue_cut(vg_scene* s, uint32_t view_count) {
+  vg_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  Params p = make_params(s, view_count);
+  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * 4;
+  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * 8;
+  hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
+  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, view_count), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
+  hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 0u, 1u);
+  hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 1u);
+  hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 0u, 2u);
+  hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 2u);
+  hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 1u, 0u);
+  VG_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_cut: host-only context (device < 0)");
+  if (!views || count == 0) return fail(ctx, VG_E_INVALID, "vg_cut: no views");
+  if (count > ctx->cfg.max_views || count >= (1u << 14)) return fail(ctx, VG_E_INVALID, "vg_cut: more views than max_views");
+  std::vector<uint8_t> seen(ctx->cfg.max_views, 0);
+  for (uint32_t i = 0; i < count; i++) {
+    if (views[i].slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_cut: view slot out of range");
+    if (seen[views[i].slot]++) return fail(ctx, VG_E_INVALID, "vg_cut: two views share a slot");
+    if (views[i].triangle_budget) return fail(ctx, VG_E_INVALID, "vg_cut: triangle budget is not available in this build");
+    if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0))
+      return fail(ctx, VG_E_INVALID, "vg_cut: proj_scale and near_plane must be positive");
+  }
+  if (int rc = upload_scene(s)) return rc;
+  for (uint32_t i = 0; i < count; i++)
+    if (int rc = ensure_slot(s, views[i].slot)) return rc;
+  // Scratch: every (view, cell) can be queued once; every (view, instance, cut) can be a work item.
+  const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
+  if (s->cell_queue_capacity < cells * count) {
+    if (int rc = ensure(s, s->d_cell_queue, sizeof(uint2) * cells * count)) return rc;
+    s->cell_queue_capacity = cells * count;
+  }
+  const uint64_t want = std::min<uint64_t>(static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64, kMaxWorkItems);
+  if (s->work_capacity < want) {
+    if (int rc = ensure(s, s->d_work, sizeof(uint4) * want)) return rc;
+    if (int rc = ensure(s, s->d_work_start, sizeof(unsigned long long) * want)) return rc;
+    s->work_capacity = static_cast<uint32_t>(want);
+  }
+  if (s->view_capacity < count) {
+    if (int rc = ensure(s, s->d_views, sizeof(ViewDev) * count)) return rc;
+    if (s->h_views) (void)hipHostFree(s->h_views);
+    s->h_views = nullptr;
+    VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_views), sizeof(ViewDev) * count, hipHostMallocDefault));
+    s->view_capacity = count;
+  }
+  if (s->views_copy_pending) {
+    VG_HIP(ctx, hipEventSynchronize(s->views_copied));
+    s->views_copy_pending = false;
+  }
+  for (uint32_t i = 0; i < count; i++) {
+    const vg_view& in = views[i];
+    ViewDev& v = s->h_views[i];
+    std::memset(&v, 0, sizeof(v));
+    std::memcpy(v.view, in.view_matrix, sizeof(v.view));
+    std::memcpy(v.proj, in.projection_matrix, sizeof(v.proj));
+    std::memcpy(v.planes, in.frustum_planes, sizeof(v.planes));
+    std::memcpy(v.sweep, in.shadow_sweep, sizeof(float) * 3);
+    v.proj_scale = in.proj_scale;
+    v.near_plane = in.near_plane;
+    v.error_threshold = in.error_threshold;
+    v.shadow_error_scale = in.shadow_error_scale;
+    v.lod_blend_max = in.lod_blend_max;
+    v.lod_scale_limit = in.lod_scale_limit;
+    v.min_pixel_radius = in.min_pixel_radius;
+    v.max_error_threshold = in.max_error_threshold;
+    v.frustum = in.frustum_culling;
+    v.occlusion = in.occlusion;
+    v.slot = in.slot;
+    v.budget = in.triangle_budget;
+    v.range_lo = in.threshold_range[0];
+    v.range_hi = in.threshold_range[1];
+    v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
+  }
+  VG_HIP(ctx, hipMemcpyAsync(s->d_views.ptr, s->h_views, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
+  VG_HIP(ctx, hipEventRecord(s->views_copied, ctx->stream));
+  s->views_copy_pending = true;
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int vg_create(const vg_config* config, vg_ctx** out) {
+  if (!out) return VG_E_INVALID;
+  *out = nullptr;
+  vg_config c;
+  if (config) {
+    c = *config;
+  } else {
+    std::memset(&c, 0, sizeof(c));
+  }
+  if (!c.max_triangles) c.max_triangles = 10000000;
+  if (!c.max_shadow_triangles) c.max_shadow_triangles = 4000000;
+  if (!c.max_meshlets) c.max_meshlets = 1u << 20;
+  if (!c.max_views) c.max_views = 1;
+  if (!c.mesh_slots) c.mesh_slots = 64;
+  if (!c.hzb_max_size) c.hzb_max_size = 1024;
+  if (!c.budget_bins) c.budget_bins = 256;
+  if (c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views >= (1u << 14) || c.hzb_max_size > 4096 ||
+      c.max_triangles > 0x55555555u || c.max_shadow_triangles > 0x55555555u)
+    return VG_E_INVALID;
+  auto* ctx = new (std::nothrow) vg_ctx();
+  if (!ctx) return VG_E_NOMEM;
+  ctx->cfg = c;
+  if (c.device < 0) {  // host-only: scenes lay out their buffers, nothing runs
+    *out = ctx;
+    return VG_OK;
+  }
+  if (hipSetDevice(c.device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return VG_E_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c.device) == hipSuccess && prop.multiProcessorCount > 0) ctx->cus = prop.multiProcessorCount;
+  *out = ctx;
+  return VG_OK;
+}
+
+void vg_destroy(vg_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+const char* vg_last_error(const vg_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+uint32_t vg_counter_words(const vg_ctx* ctx) { return state_base(ctx->cfg.mesh_slots) + kStateWords; }
+uint32_t vg_draw_args_words(const vg_ctx* ctx) { return ctx->cfg.mesh_slots * 20; }
+
+int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
+  if (!ctx || !out) return VG_E_INVALID;
+  *out = nullptr;
+  const bool device = ctx->cfg.device >= 0;
+  if (device) VG_HIP(ctx, hipSetDevice(ctx->cfg.device));
+  auto* s = new (std::nothrow) vg_scene();
+  if (!s) return fail(ctx, VG_E_NOMEM, "out of host memory");
+  s->ctx = ctx;
+  s->slots = ctx->cfg.mesh_slots;
+  s->meshes.resize(s->slots);
+  s->mesh_table.assign(static_cast<size_t>(s->slots) * kMeshStride, make_float4(0, 0, 0, 0));
+  const uint32_t views = ctx->cfg.max_views;
+  s->counter_stride = state_base(s->slots) + kStateWords;
+  s->args_stride = s->slots * 20;
+  s->d_draws.resize(views);
+  s->d_hzbs.resize(views);
+  s->hzb_valid.assign(views, 0);
+  s->h_draw_ptrs.assign(views, nullptr);
+  s->h_hzb_ptrs.assign(views, nullptr);
+  int rc = 0;
+  auto init = [&]() -> int {
+    if ((rc = ensure(s, s->d_counters, sizeof(uint32_t) * s->counter_stride * views)) ||
+        (rc = ensure(s, s->d_args, sizeof(uint32_t) * s->args_stride * views)) ||
+        (rc = ensure(s, s->d_metrics, sizeof(unsigned long long) * kMetricWords * views)) ||
+        (rc = ensure(s, s->d_draw_ptrs, sizeof(uint4*) * views)) || (rc = ensure(s, s->d_hzb_ptrs, sizeof(float*) * views)) ||
+        (rc = ensure(s, s->d_cell_queue_count, 64)) || (rc = ensure(s, s->d_work_counter, 64)) || (rc = ensure(s, s->d_flags, 64)))
+      return rc;
+    // Controller state: scale 1 for both cuts (GeometryPool.ts:602-603), everything else 0.
+    std::vector<uint32_t> counters(static_cast<size_t>(s->counter_stride) * views, 0u);
+    for (uint32_t v = 0; v < views; v++) {
+      counters[static_cast<size_t>(v) * s->counter_stride + state_base(s->slots) + kStateScale] = 10000;
+      counters[static_cast<size_t>(v) * s->counter_stride + state_base(s->slots) + kStateScale + 1] = 10000;
+    }
+    VG_HIP(ctx, hipMemcpy(s->d_counters.ptr, counters.data(), sizeof(uint32_t) * counters.size(), hipMemcpyHostToDevice));
+    VG_HIP(ctx, hipMemset(s->d_args.ptr, 0, s->d_args.bytes));
+    VG_HIP(ctx, hipMemset(s->d_metrics.ptr, 0, s->d_metrics.bytes));
+    VG_HIP(ctx, hipMemset(s->d_draw_ptrs.ptr, 0, s->d_draw_ptrs.bytes));
+    VG_HIP(ctx, hipMemset(s->d_hzb_ptrs.ptr, 0, s->d_hzb_ptrs.bytes));
+    VG_HIP(ctx, hipMemset(s->d_flags.ptr, 0, s->d_flags.bytes));
+    VG_HIP(ctx, hipEventCreateWithFlags(&s->views_copied, hipEventDisableTiming));
+    return 0;
+  };
+  if (device && (rc = init())) {
+    vg_scene_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return VG_OK;
+}
+
+void vg_scene_destroy(vg_scene* s) {
+  if (!s) return;
+  if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  DevBuf* bufs[] = {&s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
+                    &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
+                    &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
+                    &s->d_work_start, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage};
+  for (DevBuf* b : bufs) release(s, *b);
+  for (DevBuf& b : s->d_draws) release(s, b);
+  for (DevBuf& b : s->d_hzbs) release(s, b);
+  if (s->h_views) (void)hipHostFree(s->h_views);
+  if (s->views_copied) (void)hipEventDestroy(s->views_copied);
+  delete s;
+}
+
+uint64_t vg_scene_bytes(const vg_scene* s) { return s ? s->device_bytes : 0; }
+
+int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
+  if (!s || !mesh || !slot_out) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  const uint32_t n = mesh->meshlet_count, L = mesh->level_count;
+  if (n == 0 || L == 0 || !mesh->meshlet_bounds || !mesh->meshlet_info || !mesh->level_ranges || !mesh->level_errors ||
+      !mesh->level_guard_errors)
+    return fail(ctx, VG_E_INVALID, "vg_mesh_add: empty or incomplete mesh");
+  if (s->m_cull.size() + n > kMaxMeshletsPerScene) return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: more than 2^24 meshlets in a scene");
+  uint32_t slot = 0;
+  while (slot < s->slots && s->meshes[slot].used) slot++;
+  if (slot >= s->slots) return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: no free mesh slot");
+  for (uint32_t l = 0; l < L; l++) {
+    const uint64_t end = static_cast<uint64_t>(mesh->level_ranges[l * 2]) + mesh->level_ranges[l * 2 + 1];
+    if (end > n) return fail(ctx, VG_E_INVALID, "vg_mesh_add: level range outside the meshlets");
+  }
+  MeshHost& m = s->meshes[slot];
+  m = MeshHost();
+  m.used = true;
+  m.meshlet_base = static_cast<uint32_t>(s->m_cull.size());
+  m.meshlet_count = n;
+  std::memcpy(m.bounding, mesh->bounding_sphere, sizeof(m.bounding));
+  std::memcpy(m.lod_bounds, mesh->lod_bounds_sphere, sizeof(m.lod_bounds));
+  std::memcpy(m.cull_bounds, mesh->cull_bounds_sphere, sizeof(m.cull_bounds));
+  for (uint32_t i = 0; i < n; i++) {
+    const float* b = mesh->meshlet_bounds + static_cast<size_t>(i) * 16;
+    s->m_lod.push_back(make_float4(b[0], b[1], b[2], b[3]));
+    s->m_parent.push_back(make_float4(b[4], b[5], b[6], b[7]));
+    s->m_err.push_back(make_float2(b[8], b[9]));
+    s->m_cull.push_back(make_float4(b[12], b[13], b[14], b[15]));
+    s->m_tris.push_back(mesh->meshlet_info[static_cast<size_t>(i) * 4]);
+    const uint32_t* info = mesh->meshlet_info + static_cast<size_t>(i) * 4;
+    s->m_info.push_back(info[0]);
+    s->m_info.push_back(static_cast<uint32_t>(info[1] + s->index_used));
+    s->m_info.push_back(info[2]);
+    s->m_info.push_back(info[3]);
+  }
+  uint64_t mesh_indices = 0;
+  for (uint32_t i = 0; i < n; i++) mesh_indices += static_cast<uint64_t>(mesh->meshlet_info[static_cast<size_t>(i) * 4]) * 3;
+  s->index_used += mesh_indices;
+  // Level table after the mesh records: (first meshlet, count, min own, max parent), (max own, min parent, 0, 0)
+  // (GeometryPool.ts:432-445).
+  m.level_count = L;
+  m.level_base = s->slots * kMeshStride + s->level_used;
+  s->level_used += L * 2;
+  s->mesh_table.resize(static_cast<size_t>(s->slots) * kMeshStride + s->level_used);
+  for (uint32_t l = 0; l < L; l++) {
+    s->mesh_table[m.level_base + l * 2] =
+        make_float4(static_cast<float>(mesh->level_ranges[l * 2] + m.meshlet_base), static_cast<float>(mesh->level_ranges[l * 2 + 1]),
+                    mesh->level_errors[l * 2], mesh->level_errors[l * 2 + 1]);
+    s->mesh_table[m.level_base + l * 2 + 1] = make_float4(mesh->level_guard_errors[l * 2], mesh->level_guard_errors[l * 2 + 1], 0, 0);
+  }
+  write_mesh_record(s, slot);
+  // writeMeshRecord defaults (GeometryPool.ts:350): no draw distance, min pixel radius 0, enabled, shadow, no occlusion.
+  s->mesh_table[slot * kMeshStride + 3] = make_float4(1e30f, 0.0f, static_cast<float>(L), static_cast<float>(m.level_base));
+  s->mesh_table[slot * kMeshStride + 5] = make_float4(1, 1, 0, 0);
+  s->mesh_slots = std::max(s->mesh_slots, slot + 1);
+  s->dirty_meshlets = s->dirty_table = true;
+  *slot_out = slot;
+  return VG_OK;
+}
+
+int vg_instances_set(vg_scene* s, uint32_t slot, const float* matrices, const float* colors, uint32_t count,
+                     uint32_t* gpu_to_input) {
+  (void)colors;  // tints are raster-side data (GeometryPool.ts:490-491); the cut does not read them
+  if (!s || slot >= s->slots || !s->meshes[slot].used || (!matrices && count)) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  MeshHost& m = s->meshes[slot];
+  const size_t total = s->i_col[0].size();
+  if (count != m.instance_count || m.instance_count == 0) {
+    if (total + count > kMaxInstancesPerScene) return fail(ctx, VG_E_CAPACITY, "vg_instances_set: more than 2^24 instances in a scene");
+    m.instance_base = static_cast<uint32_t>(total);
+    for (int k = 0; k < 4; k++) s->i_col[k].resize(total + count);
+    s->i_slot.resize(total + count, static_cast<float>(slot));
+  }
+  m.instance_count = count;
+  std::vector<uint32_t> order = morton_order(matrices, count, m.bounding);
+  std::vector<float> gpu(static_cast<size_t>(count) * 16);
+  for (uint32_t g = 0; g < count; g++) {
+    const float* src = matrices + static_cast<size_t>(order[g]) * 16;
+    std::memcpy(&gpu[static_cast<size_t>(g) * 16], src, sizeof(float) * 16);
+    for (int k = 0; k < 4; k++) s->i_col[k][m.instance_base + g] = make_float4(src[k * 4], src[k * 4 + 1], src[k * 4 + 2], src[k * 4 + 3]);
+  }
+  if (gpu_to_input) std::memcpy(gpu_to_input, order.data(), sizeof(uint32_t) * count);
+  const uint32_t cells = (count + kCellSize - 1) / kCellSize;
+  m.cell_sphere.resize(cells);
+  m.cell_rec.resize(cells);
+  for (uint32_t c = 0; c < cells; c++)
+    compute_cell(gpu, c * kCellSize, std::min(count, (c + 1) * kCellSize), m.bounding, slot, m.instance_base, m.cell_sphere[c],
+                 m.cell_rec[c]);
+  s->dirty_instances = s->dirty_cells = true;
+  return VG_OK;
+}
+
+int vg_mesh_flags(vg_scene* s, uint32_t slot, float max_draw_distance, float min_pixel_radius, uint32_t enabled, uint32_t shadow,
+                  uint32_t occlusion) {
+  if (!s || slot >= s->slots || !s->meshes[slot].used) return VG_E_INVALID;
+  float4& levers = s->mesh_table[slot * kMeshStride + 3];
+  levers.x = max_draw_distance;
+  levers.y = min_pixel_radius;
+  s->mesh_table[slot * kMeshStride + 5] = make_float4(enabled ? 1.0f : 0.0f, shadow ? 1.0f : 0.0f, occlusion ? 1.0f : 0.0f, 0.0f);
+  s->dirty_table = true;
+  return VG_OK;
+}
+
+int vg_hzb_set(vg_scene* s, uint32_t view_slot, const float* depth, uint32_t width, uint32_t height) {
+  if (!s || !depth || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_hzb_set: host-only context");
+  const uint32_t max_size = ctx->cfg.hzb_max_size;
+  if (width == 0 || height == 0 || width > max_size || height > max_size)
+    return fail(ctx, VG_E_INVALID, "vg_hzb_set: depth size must be within [1, hzb_max_size]");
+  // Layout: header + one region per level sized for max_size (OcclusionCulling.ts:92-113, 284-300).
+  std::vector<uint32_t> offsets;
+  uint32_t total = kHzbHeader;
+  for (uint32_t dim = max_size;;) {
+    offsets.push_back(total);
+    total += dim * dim;
+    if (dim == 1) break;
+    dim = (dim + 1) / 2;
+  }
+  std::vector<float> header(kHzbHeader, 0.0f);
+  uint32_t lw = width, lh = height, levels = 0;
+  std::vector<uint32_t> ws, hs;
+  for (size_t i = 0; i < offsets.size(); i++) {
+    header[i * 4] = static_cast<float>(offsets[i]);
+    header[i * 4 + 1] = static_cast<float>(lw);
+    header[i * 4 + 2] = static_cast<float>(lh);
+    ws.push_back(lw);
+    hs.push_back(lh);
+    levels++;
+    if (lw == 1 && lh == 1) break;
+    lw = lw > 1 ? (lw + 1) / 2 : 1;
+    lh = lh > 1 ? (lh + 1) / 2 : 1;
+  }
+  header[kHzbLevelCountSlot] = static_cast<float>(levels);
+  if (!s->d_hzbs[view_slot].ptr) {
+    if (int rc = ensure(s, s->d_hzbs[view_slot], sizeof(float) * total)) return rc;
+    s->h_hzb_ptrs[view_slot] = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+    VG_HIP(ctx, hipMemcpyAsync(static_cast<float**>(s->d_hzb_ptrs.ptr) + view_slot, &s->h_hzb_ptrs[view_slot], sizeof(float*),
+                               hipMemcpyHostToDevice, ctx->stream));
+  }
+  float* hzb = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+  VG_HIP(ctx, hipMemcpyAsync(hzb, header.data(), sizeof(float) * kHzbHeader, hipMemcpyHostToDevice, ctx->stream));
+  VG_HIP(ctx, hipMemcpyAsync(hzb + offsets[0], depth, sizeof(float) * width * height, hipMemcpyHostToDevice, ctx->stream));
+  for (uint32_t base = 0; base + 1 < levels; base += 5) {
+    dim3 grid((ws[base] + 31) / 32, (hs[base] + 31) / 32);
+    hipLaunchKernelGGL(k_hzb_pass, grid, dim3(256), 0, ctx->stream, hzb, base);
+  }
+  VG_HIP(ctx, hipGetLastError());
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->hzb_valid[view_slot] = 1;
+  return VG_OK;
+}
+
+int vg_hzb_read(vg_scene* s, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats) {
+  if (!s || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  const uint64_t n = s->d_hzbs[view_slot].ptr ? s->d_hzbs[view_slot].bytes / sizeof(float) : 0;
+  if (size_floats) *size_floats = n;
+  if (dst && n) {
+    if (capacity_floats < n) return VG_E_CAPACITY;
+    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    VG_HIP(ctx, hipMemcpy(dst, s->d_hzbs[view_slot].ptr, sizeof(float) * n, hipMemcpyDeviceToHost));
+  }
+  return VG_OK;
+}
+
+int vg_hzb_invalidate(vg_scene* s, uint32_t view_slot) {
+  if (!s || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  s->hzb_valid[view_slot] = 0;
+  return VG_OK;
+}
+
+int vg_cut(vg_scene* s, const vg_view* views, uint32_t count) {
+  if (!s) return VG_E_INVALID;
+  if (int rc = prepare_cut(s, views, count)) return rc;
+  return enqueue_cut(s, count);
+}
+
+int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (int rc = prepare_cut(s, views, count)) return rc;
+  if (!s->graph_exec || s->graph_views != count || s->graph_version != s->layout_version) {
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    VG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_cut(s, count);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    s->graph = graph;
+    VG_HIP(ctx, hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
+    s->graph_views = count;
+    s->graph_version = s->layout_version;
+  }
+  VG_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
+  return VG_OK;
+}
+
+int vg_sync(vg_scene* s) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->views_copy_pending = false;
+  uint32_t flags[4] = {0, 0, 0, 0};
+  VG_HIP(ctx, hipMemcpy(flags, s->d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
+  if (flags[3]) return fail(ctx, VG_E_CAPACITY, "vg_cut: work list capacity exceeded; results are incomplete");
+  return VG_OK;
+}
+
+int vg_read(vg_scene* s, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list, uint32_t max_entries,
+            vg_metrics* metrics) {
+  if (!s || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (counters)
+    VG_HIP(ctx, hipMemcpy(counters, static_cast<uint32_t*>(s->d_counters.ptr) + static_cast<size_t>(view_slot) * s->counter_stride,
+                          sizeof(uint32_t) * s->counter_stride, hipMemcpyDeviceToHost));
+  if (draw_args)
+    VG_HIP(ctx, hipMemcpy(draw_args, static_cast<uint32_t*>(s->d_args.ptr) + static_cast<size_t>(view_slot) * s->args_stride,
+                          sizeof(uint32_t) * s->args_stride, hipMemcpyDeviceToHost));
+  if (draw_list && max_entries) {
+    const uint32_t n = std::min(max_entries, ctx->cfg.max_meshlets);
+    if (!s->d_draws[view_slot].ptr) {
+      std::memset(draw_list, 0, sizeof(uint4) * 2 * static_cast<size_t>(max_entries));
+    } else {
+      const uint4* d = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
+      VG_HIP(ctx, hipMemcpy(draw_list, d, sizeof(uint4) * n, hipMemcpyDeviceToHost));
+      VG_HIP(ctx, hipMemcpy(draw_list + static_cast<size_t>(max_entries) * 4, d + ctx->cfg.max_meshlets, sizeof(uint4) * n,
+                            hipMemcpyDeviceToHost));
+    }
+  }
+  if (metrics) {
+    unsigned long long m[kMetricWords];
+    VG_HIP(ctx, hipMemcpy(m, static_cast<unsigned long long*>(s->d_metrics.ptr) + static_cast<size_t>(view_slot) * kMetricWords,
+                          sizeof(m), hipMemcpyDeviceToHost));
+    std::memset(metrics, 0, sizeof(*metrics));
+    metrics->cells_tested = m[M_CELLS];
+    metrics->instances_tested = m[M_INSTANCES];
+    metrics->work_items[0] = m[M_WORK0];
+    metrics->work_items[1] = m[M_WORK1];
+    metrics->clusters_tested[0] = m[M_CLUSTERS0];
+    metrics->clusters_tested[1] = m[M_CLUSTERS1];
+    metrics->occlusion_tests = m[M_OCCLUSION];
+  }
+  return VG_OK;
+}
+
+int vg_state_get(vg_scene* s, uint32_t view_slot, uint32_t* state) {
+  if (!s || !state || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  if (!s->ctx->stream) return fail(s->ctx, VG_E_DEVICE, "host-only context");
+  VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  VG_HIP(s->ctx, hipMemcpy(state, static_cast<uint32_t*>(s->d_counters.ptr) + static_cast<size_t>(view_slot) * s->counter_stride +
+                                      state_base(s->slots),
+                           sizeof(uint32_t) * kStateWords, hipMemcpyDeviceToHost));
+  return VG_OK;
+}
+
+int vg_state_set(vg_scene* s, uint32_t view_slot, const uint32_t* state) {
+  if (!s || !state || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  if (!s->ctx->stream) return fail(s->ctx, VG_E_DEVICE, "host-only context");
+  VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  VG_HIP(s->ctx, hipMemcpy(static_cast<uint32_t*>(s->d_counters.ptr) + static_cast<size_t>(view_slot) * s->counter_stride +
+                               state_base(s->slots),
+                           state, sizeof(uint32_t) * kStateWords, hipMemcpyHostToDevice));
+  return VG_OK;
+}
+
+int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes) {
+  if (!s || buffer > VG_POOL_MESHLET_INFO) return VG_E_INVALID;
+  std::vector<float> f;
+  const void* src = nullptr;
+  uint64_t bytes = 0;
+  switch (buffer) {
+    case VG_POOL_MESH_TABLE:
+      src = s->mesh_table.data();
+      bytes = sizeof(float4) * s->mesh_table.size();
+      break;
+    case VG_POOL_INSTANCES: {
+      const size_t n = s->i_col[0].size();
+      f.resize(n * 20);
+      for (size_t i = 0; i < n; i++) {
+        for (int k = 0; k < 4; k++) std::memcpy(&f[i * 20 + k * 4], &s->i_col[k][i], sizeof(float4));
+        f[i * 20 + 16] = f[i * 20 + 17] = f[i * 20 + 18] = 1.0f;
+        f[i * 20 + 19] = s->i_slot[i];
+      }
+      src = f.data();
+      bytes = sizeof(float) * f.size();
+      break;
+    }
+    case VG_POOL_CELLS: {
+      f.clear();
+      for (const MeshHost& m : s->meshes) {
+        if (!m.used) continue;
+        for (size_t c = 0; c < m.cell_sphere.size(); c++) {
+          const float4 a = m.cell_sphere[c], b = m.cell_rec[c];
+          const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+          f.insert(f.end(), v, v + 8);
+        }
+      }
+      src = f.data();
+      bytes = sizeof(float) * f.size();
+      break;
+    }
+    case VG_POOL_MESHLET_BOUNDS: {
+      const size_t n = s->m_cull.size();
+      f.resize(n * 16);
+      for (size_t i = 0; i < n; i++) {
+        std::memcpy(&f[i * 16], &s->m_lod[i], sizeof(float4));
+        std::memcpy(&f[i * 16 + 4], &s->m_parent[i], sizeof(float4));
+        f[i * 16 + 8] = s->m_err[i].x;
+        f[i * 16 + 9] = s->m_err[i].y;
+        f[i * 16 + 10] = f[i * 16 + 11] = 0.0f;
+        std::memcpy(&f[i * 16 + 12], &s->m_cull[i], sizeof(float4));
+      }
+      src = f.data();
+      bytes = sizeof(float) * f.size();
+      break;
+    }
+    default:
+      src = s->m_info.data();
+      bytes = sizeof(uint32_t) * s->m_info.size();
+      break;
+  }
+  if (size_bytes) *size_bytes = bytes;
+  if (dst) {
+    if (capacity_bytes < bytes) return VG_E_CAPACITY;
+    if (bytes) std::memcpy(dst, src, bytes);
+  }
+  return VG_OK;
+}
+
+uint32_t vg_scene_mesh_slots(const vg_scene* s) { return s ? s->mesh_slots : 0; }
+
+int vg_mesh_instance_base(const vg_scene* s, uint32_t slot, uint32_t* base, uint32_t* count) {
+  if (!s || slot >= s->slots || !s->meshes[slot].used) return VG_E_INVALID;
+  if (base) *base = s->meshes[slot].instance_base;
+  if (count) *count = s->meshes[slot].instance_count;
+  return VG_OK;
+}
+
+// Test hook: evaluates the arithmetic the decisions depend on for n input pairs (see k_arith_probe).
+int vg_arith_probe(const float* a, const float* b, float* out, uint32_t n) {
+  if (!a || !b || !out || n == 0) return VG_E_INVALID;
+  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  if (hipMalloc(&da, sizeof(float) * n) != hipSuccess) return VG_E_NOMEM;
+  if (hipMalloc(&db, sizeof(float) * n) != hipSuccess) {
+    (void)hipFree(da);
+    return VG_E_NOMEM;
+  }
+  if (hipMalloc(&dout, sizeof(float) * n * 5) != hipSuccess) {
+    (void)hipFree(da);
+    (void)hipFree(db);
+    return VG_E_NOMEM;
+  }
+  int rc = VG_OK;
+  if (hipMemcpy(da, a, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = VG_E_DEVICE;
+  if (rc == VG_OK) {
+    hipLaunchKernelGGL(k_arith_probe, dim3((n + 255) / 256), dim3(256), 0, 0, da, db, dout, n);
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(out, dout, sizeof(float) * n * 5, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = VG_E_DEVICE;
+  }
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dout);
+  return rc;
+}
+
+}  // extern "C"

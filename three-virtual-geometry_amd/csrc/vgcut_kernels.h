@@ -1,0 +1,255 @@
+// Device side of the cut: binary32 helpers in the reference kernels' expression order, the launch
+// parameter block and the per-view records the kernels read. Compiled with -ffp-contract=off and
+// correctly rounded division and square root, so every expression below rounds exactly as written.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vg {
+
+// GeometryPool.ts:96-140, constants.ts:2-20, lodBlend.ts:22-24
+constexpr uint32_t kMeshStride = 6;      // float4 per mesh record
+constexpr uint32_t kMaxLevels = 48;      // level-table entries read per mesh
+constexpr uint32_t kCellSize = 128;      // instances per cell
+constexpr uint32_t kCutCounters = 5;
+constexpr uint32_t kSelected = 0, kDrawn = 1, kSurvived = 2, kWork = 3, kRequested = 4;
+constexpr uint32_t kPerMeshCounters = 12;
+constexpr uint32_t kStateScale = 0, kStateFill = 2, kStateRetry = 4, kStateWords = 8;
+constexpr uint32_t kFastPathMax = 4;
+constexpr uint32_t kFadeSteps = 255;
+constexpr uint32_t kFullFadeBits = 255u << 16;
+constexpr uint32_t kHzbMaxLevels = 16;
+constexpr uint32_t kHzbHeader = kHzbMaxLevels * 4 + 4;
+constexpr uint32_t kHzbLevelCountSlot = kHzbMaxLevels * 4;
+// Work-queue counter: item count in the high 28 bits, task (cluster test) count in the low 36 bits.
+constexpr int kTaskBits = 36;
+constexpr uint64_t kTaskMask = (1ull << kTaskBits) - 1;
+constexpr uint64_t kTaskLimit = 1ull << 35;
+constexpr uint32_t kMaxWorkItems = (1u << 28) - 1;
+// Metrics words (u64) per view slot.
+constexpr uint32_t kMetricWords = 16;
+enum : uint32_t { M_CELLS = 0, M_INSTANCES = 1, M_WORK0 = 2, M_WORK1 = 3, M_CLUSTERS0 = 4, M_CLUSTERS1 = 5, M_OCCLUSION = 6 };
+
+// Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
+#define VG_INF_HALF ((float)(1e30 * 0.5))
+#define VG_INF ((float)1e30)
+#define VG_MARGIN_LO ((float)(1 - 1e-3))
+#define VG_MARGIN_HI ((float)(1 + 1e-3))
+
+// Per launched view, in device memory. The first part is copied from vg_view; `low`, `high`,
+// `fade_scale` (loadBands, GeometryPool.ts:652-665) are derived on the device from the slot's state.
+struct ViewDev {
+  float view[16];
+  float proj[16];
+  float planes[24];
+  float sweep[4];
+  float proj_scale, near_plane, error_threshold, shadow_error_scale;
+  float lod_blend_max, lod_scale_limit, min_pixel_radius, max_error_threshold;
+  uint32_t frustum, occlusion, slot, budget;
+  float range_lo, range_hi;
+  uint32_t hzb_valid, pad0;
+  // derived
+  float low[2], high[2];
+  float fade_scale;
+  uint32_t redo[2];  // retry flags (S_RETRY > 0.5) of the current round
+  uint32_t budget_bin;
+};
+
+struct Params {
+  // scene
+  const float4* __restrict__ mesh_table;  // slot records then level tables (GeometryPool.ts:432-460)
+  const float4* __restrict__ m_cull;      // per meshlet: culling sphere
+  const float4* __restrict__ m_lod;       // LOD sphere
+  const float4* __restrict__ m_parent;    // parent LOD sphere
+  const float2* __restrict__ m_err;       // (error, parentError)
+  const uint32_t* __restrict__ m_tris;    // triangle count
+  const float4* __restrict__ i_col0;      // instance matrix columns, SoA
+  const float4* __restrict__ i_col1;
+  const float4* __restrict__ i_col2;
+  const float4* __restrict__ i_col3;
+  const float4* __restrict__ c_sphere;    // per cell: bounding sphere
+  const float4* __restrict__ c_rec;       // (maxRadius, slot, firstInstance, count)
+  uint32_t cell_count;
+  uint32_t mesh_slots;   // highest used slot + 1
+  uint32_t slots;        // slot capacity
+  uint32_t max_meshlets;
+  uint32_t cut_capacity[2];
+  // per view slot
+  uint32_t* counters;    // counter_stride words per slot (state words at the end)
+  uint32_t counter_stride;
+  uint32_t* args;        // args_stride words per slot
+  uint32_t args_stride;
+  uint4* const* draws;   // per slot: 2 * max_meshlets uvec4
+  float* const* hzbs;    // per slot HZB buffer
+  unsigned long long* metrics;  // kMetricWords per slot
+  // launch
+  ViewDev* views;
+  uint32_t view_count;
+  uint2* cell_queue;     // (view, cell)
+  uint32_t* cell_queue_count;
+  uint4* work;           // (instance, first, last, tag)
+  unsigned long long* work_start;  // first task of each item
+  unsigned long long* work_counter;
+  uint32_t work_capacity;
+  uint32_t* flags;       // [0] any retry this round, [1] overflow
+  // triangle budget
+  const float* budget_thresholds;  // bins, ascending
+  unsigned long long* budget_hist; // per view: bins + 1
+  uint32_t budget_bins;
+};
+
+__host__ __device__ inline uint32_t state_base(uint32_t slots) { return kPerMeshCounters + 2u * slots * 4u; }
+
+// ------------------------------------------------------------------ arithmetic
+
+__device__ __forceinline__ float4 mat_mul(const float4& c0, const float4& c1, const float4& c2, const float4& c3, float x,
+                                          float y, float z, float w) {
+  float4 r;
+  r.x = ((c0.x * x + c1.x * y) + c2.x * z) + c3.x * w;
+  r.y = ((c0.y * x + c1.y * y) + c2.y * z) + c3.y * w;
+  r.z = ((c0.z * x + c1.z * y) + c2.z * z) + c3.z * w;
+  r.w = ((c0.w * x + c1.w * y) + c2.w * z) + c3.w * w;
+  return r;
+}
+
+__device__ __forceinline__ float4 mat_mul16(const float* m, float x, float y, float z, float w) {
+  float4 r;
+  r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
+  r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
+  r.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14] * w;
+  r.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15] * w;
+  return r;
+}
+
+// sqrt and / are correctly rounded: HIP's default -fhip-fp32-correctly-rounded-divide-sqrt lowers
+// llvm.sqrt.f32 and fdiv to IEEE-exact sequences on gfx950 (checked against the host by a GPU test).
+__device__ __forceinline__ float len3(float x, float y, float z) { return __builtin_sqrtf((x * x + y * y) + z * z); }
+__device__ __forceinline__ float dot3(const float* n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// fdlibm e_logf.c (Sun Microsystems, 1993): x = 2^k (1 + f), log(1 + f) = f - s (f - R(z)),
+// s = f / (2 + f). Basic operations only, so the host and device give the same bits.
+__host__ __device__ inline float vg_logf(float x) {
+  const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f, two25 = 3.355443200e+07f;
+  const float Lg1 = 6.6666668653e-01f, Lg2 = 4.0000000596e-01f, Lg3 = 2.8571429849e-01f, Lg4 = 2.2222198546e-01f,
+              Lg5 = 1.8183572590e-01f, Lg6 = 1.5313838422e-01f, Lg7 = 1.4798198640e-01f;
+  uint32_t ix = __builtin_bit_cast(uint32_t, x);
+  int32_t k = 0;
+  if (ix >= 0x7f800000u) return x + x;
+  if (ix < 0x00800000u) {
+    if (ix == 0) return -__builtin_inff();
+    k -= 25;
+    x *= two25;
+    ix = __builtin_bit_cast(uint32_t, x);
+  }
+  k += static_cast<int32_t>(ix >> 23) - 127;
+  ix &= 0x007fffffu;
+  uint32_t i = (ix + (0x95f64u << 3)) & 0x800000u;
+  float m = __builtin_bit_cast(float, ix | (i ^ 0x3f800000u));
+  k += static_cast<int32_t>(i >> 23);
+  float f = m - 1.0f;
+  float dk = static_cast<float>(k);
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float w = z * z;
+  float t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+  float t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+  float R = t2 + t1;
+  float hfsq = 0.5f * f * f;
+  return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
+// ------------------------------------------------------------------ wave64 helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+  return static_cast<T>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, 64);
+    if (static_cast<int>(lane_id()) >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t t = __shfl_up(v, d, 64);
+    if (static_cast<int>(lane_id()) >= d) v += t;
+  }
+  return v;
+}
+
+// atomicAdd of `value` for every lane with `active`, aggregated per distinct address: one atomic per
+// address per wave. Returns the lane's old value (the group base plus its exclusive prefix). All 64
+// lanes must call it together.
+__device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t value, bool active) {
+  uint32_t result = 0;
+  uint64_t pending = __ballot(active);
+  const uint32_t lane = lane_id();
+  const uint64_t a = reinterpret_cast<uint64_t>(addr);
+  while (pending) {
+    int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    uint32_t lo = __shfl(static_cast<uint32_t>(a), leader, 64);
+    uint32_t hi = __shfl(static_cast<uint32_t>(a >> 32), leader, 64);
+    uint64_t la = (static_cast<uint64_t>(hi) << 32) | lo;
+    bool mine = ((pending >> lane) & 1ull) && a == la;
+    uint64_t group = __ballot(mine);
+    uint32_t v = mine ? value : 0u;
+    uint32_t incl = wave_scan(v);
+    uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == static_cast<uint32_t>(leader) && total) base = atomicAdd(reinterpret_cast<uint32_t*>(la), total);
+    base = __shfl(base, leader, 64);
+    if (mine) result = base + incl - v;
+    pending &= ~group;
+  }
+  return result;
+}
+
+// One atomicAdd per wave on a wave-uniform address; returns each active lane's old value.
+__device__ __forceinline__ uint32_t wave_atomic_add_uniform(uint32_t* addr, uint32_t value, bool active) {
+  uint32_t v = active ? value : 0u;
+  uint32_t incl = wave_scan(v);
+  uint32_t total = __shfl(incl, 63, 64);
+  uint32_t base = 0;
+  if (lane_id() == 0 && total) base = atomicAdd(addr, total);
+  base = __shfl(base, 0, 64);
+  return base + incl - v;
+}
+
+// Keyed 64-bit counter add: one atomic per distinct address per wave.
+__device__ __forceinline__ void wave_count_add_keyed(unsigned long long* addr, uint64_t value, bool active) {
+  uint64_t pending = __ballot(active);
+  const uint32_t lane = lane_id();
+  const uint64_t a = reinterpret_cast<uint64_t>(addr);
+  while (pending) {
+    int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    uint32_t lo = __shfl(static_cast<uint32_t>(a), leader, 64);
+    uint32_t hi = __shfl(static_cast<uint32_t>(a >> 32), leader, 64);
+    uint64_t la = (static_cast<uint64_t>(hi) << 32) | lo;
+    bool mine = ((pending >> lane) & 1ull) && a == la;
+    uint64_t group = __ballot(mine);
+    uint64_t incl = wave_scan64(mine ? value : 0ull);
+    uint64_t total = __shfl(incl, 63, 64);
+    if (lane == static_cast<uint32_t>(leader) && total) atomicAdd(reinterpret_cast<unsigned long long*>(la), total);
+    pending &= ~group;
+  }
+}
+
+__device__ __forceinline__ void wave_count_add(unsigned long long* addr, uint64_t value, bool active) {
+  uint64_t v = active ? value : 0ull;
+  uint64_t incl = wave_scan64(v);
+  uint64_t total = __shfl(incl, 63, 64);
+  if (lane_id() == 0 && total) atomicAdd(addr, static_cast<unsigned long long>(total));
+}
+
+}  // namespace vg

@@ -1,0 +1,589 @@
+// Kernels of the per-frame cut (GeometryPool.ts:568-1092, OcclusionCulling.ts:119-170, 275-352).
+//
+// One launch sequence serves any number of views (the plain path is one view):
+//   k_begin      zero the views' counters, derive the threshold bands from the controller state
+//   k_cells      one thread per (view, cell): cell culls, wave-compacted append to a (view, cell) queue
+//   k_instances  one wave per queued (view, cell), 2 instances per lane: instance culls, occlusion,
+//                level ranges, fast path; work items appended with one 64-bit atomic per wave that
+//                returns both the item slot and the item's first cluster test, so the work list is
+//                a prefix sum of cluster tests in item order
+//   k_meshlets   persistent waves over equal shares of the flat cluster-test space: each wave finds
+//                its first item by a 64-ary search and walks the items with wave shuffles, so short
+//                and long ranges load-balance perfectly; drawn clusters are compacted with ballot and
+//                one atomic per (view, cut) per wave, firstIndex with one atomic per (view, cut, slot, part)
+//   k_prefix     one wave per (view, cut): region layout + indirect args; then the capacity check
+//                (retry rounds) or the steer, one thread per view
+// The hierarchical-Z pyramid is built by k_hzb_pass: 32 x 32 texel tiles reduced 5 levels at a time
+// in LDS (max of 2 x 2, edge clamped).
+#include "vgcut_kernels.h"
+
+namespace vg {
+
+__device__ __forceinline__ uint32_t* view_counters(const Params& p, uint32_t slot) {
+  return p.counters + static_cast<size_t>(slot) * p.counter_stride;
+}
+__device__ __forceinline__ uint32_t* counter_at(const Params& p, uint32_t slot, uint32_t cut, uint32_t k) {
+  return view_counters(p, slot) + 2 + cut * kCutCounters + k;
+}
+__device__ __forceinline__ uint32_t* mesh_count_at(const Params& p, uint32_t slot, uint32_t cut, uint32_t mesh, uint32_t part) {
+  return view_counters(p, slot) + kPerMeshCounters + cut * p.slots * 4 + mesh * 4 + part * 2;
+}
+__device__ __forceinline__ uint32_t* state_at(const Params& p, uint32_t slot) {
+  return view_counters(p, slot) + state_base(p.slots);
+}
+__device__ __forceinline__ unsigned long long* metric_at(const Params& p, uint32_t slot, uint32_t k) {
+  return p.metrics + static_cast<size_t>(slot) * kMetricWords + k;
+}
+
+__device__ __forceinline__ float get_state(const uint32_t* state, uint32_t i) {
+  return static_cast<float>(state[i]) / 10000.0f;
+}
+__device__ __forceinline__ void set_state(uint32_t* state, uint32_t i, float value) {
+  state[i] = static_cast<uint32_t>(rintf(value * 10000.0f));
+}
+
+// loadBands, GeometryPool.ts:652-665 (WGSL smoothstep).
+__device__ void load_bands(ViewDev& v, const uint32_t* state) {
+  float camera_scale = get_state(state, kStateScale);
+  float shadow_scale = get_state(state, kStateScale + 1);
+  float x = v.error_threshold * camera_scale;
+  float t = clampf((x - 1.25f) / (2.5f - 1.25f), 0.0f, 1.0f);
+  float smooth = (t * t) * (3.0f - 2.0f * t);
+  float width = 1.0f + (v.lod_blend_max - 1.0f) * smooth;
+  bool blending = width > 1.001f;
+  v.fade_scale = blending ? 255.0f / vg_logf(width) : 0.0f;
+  float t0 = v.error_threshold * camera_scale;
+  float t1 = (v.error_threshold * v.shadow_error_scale) * fmaxf(camera_scale, shadow_scale);
+  v.low[0] = t0;
+  v.high[0] = blending ? t0 * width : t0;
+  v.low[1] = t1;
+  v.high[1] = blending ? t1 * width : t1;
+}
+
+// frustumDistance, GeometryPool.ts:690-698. mode 0: plain, 1: swept (x 1), 2: swept x select(flag, 1, 0).
+__device__ __forceinline__ float frustum_distance(const ViewDev& v, float cx, float cy, float cz, int mode, bool flag) {
+  float best = 1e30f;
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    const float* n = &v.planes[p * 4];
+    float d = dot3(n, cx, cy, cz) + n[3];
+    if (mode == 1) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * 1.0f;
+    if (mode == 2) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * (flag ? 1.0f : 0.0f);
+    best = fminf(best, d);
+  }
+  return best;
+}
+
+// projectedError, GeometryPool.ts:678-683.
+__device__ __forceinline__ float projected_error(const ViewDev& v, const float4& c0, const float4& c1, const float4& c2,
+                                                 const float4& c3, float scale, const float4& s, float error) {
+  float4 q = mat_mul(c0, c1, c2, c3, s.x, s.y, s.z, 1.0f);
+  float4 vc = mat_mul16(v.view, q.x, q.y, q.z, q.w);
+  float distance = fmaxf(len3(vc.x, vc.y, vc.z) - s.w * scale, v.near_plane);
+  float pixels = ((error * scale) / distance) * v.proj_scale;
+  return error >= VG_INF_HALF ? VG_INF : pixels;
+}
+
+__device__ __forceinline__ uint32_t ceil_log2(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
+
+// occluded, OcclusionCulling.ts:119-170. `hzb` is only read when the sphere is in front of the near plane.
+__device__ bool occluded(const ViewDev& v, const float* __restrict__ hzb, float vx, float vy, float vz, float radius) {
+  float depth = -vz;
+  float near_depth = depth - radius;
+  float far_depth = depth + radius;
+  if (near_depth <= v.near_plane) return false;
+  const float* P = v.proj;
+  float x_lo = vx - radius, x_hi = vx + radius, y_lo = vy - radius, y_hi = vy + radius;
+  float sx = P[0], sy = P[5];
+  float min_x = fminf(x_lo / near_depth, x_lo / far_depth) * sx - P[8];
+  float max_x = fmaxf(x_hi / near_depth, x_hi / far_depth) * sx - P[8];
+  float min_y = fminf(y_lo / near_depth, y_lo / far_depth) * sy - P[9];
+  float max_y = fmaxf(y_hi / near_depth, y_hi / far_depth) * sy - P[9];
+  float min_z = P[14] / near_depth - P[10];
+  float W = hzb[1], H = hzb[2];
+  float x0 = clampf(floorf((min_x * 0.5f + 0.5f) * W), 0.0f, W - 1.0f);
+  float x1 = clampf(floorf((max_x * 0.5f + 0.5f) * W), 0.0f, W - 1.0f);
+  float y0 = clampf(floorf((0.5f - max_y * 0.5f) * H), 0.0f, H - 1.0f);
+  float y1 = clampf(floorf((0.5f - min_y * 0.5f) * H), 0.0f, H - 1.0f);
+  uint32_t extent = static_cast<uint32_t>(fmaxf(x1 - x0, y1 - y0)) + 1u;
+  uint32_t level = ceil_log2(extent);
+  uint32_t top = static_cast<uint32_t>(hzb[kHzbLevelCountSlot]) - 1u;
+  level = level > top ? top : level;
+  float scale = static_cast<float>(1u << level);
+  float offset = hzb[level * 4], lw = hzb[level * 4 + 1], lh = hzb[level * 4 + 2];
+  auto sample = [&](float fx, float fy) {
+    float tx = clampf(floorf(fx / scale), 0.0f, lw - 1.0f);
+    float ty = clampf(floorf(fy / scale), 0.0f, lh - 1.0f);
+    return hzb[static_cast<uint32_t>(offset + ty * lw + tx)];
+  };
+  float m = fmaxf(fmaxf(sample(x0, y0), sample(x1, y0)), fmaxf(sample(x0, y1), sample(x1, y1)));
+  return min_z < 1.0f && min_z > m + 1e-6f;
+}
+
+// occludedSphere, GeometryPool.ts:701-705.
+__device__ __forceinline__ bool occluded_sphere(const Params& p, const ViewDev& v, bool on, float cx, float cy, float cz,
+                                                float radius) {
+  if (!(on && v.hzb_valid)) return false;
+  float4 vc = mat_mul16(v.view, cx, cy, cz, 1.0f);
+  float margin = (fabsf(vc.z) * v.high[0]) / v.proj_scale;
+  return occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, radius + margin);
+}
+
+// ------------------------------------------------------------------ 0. begin (resetNode, GeometryPool.ts:714-725)
+
+__global__ __launch_bounds__(256) void k_begin(Params p) {
+  ViewDev& v = p.views[blockIdx.x];
+  uint32_t* c = view_counters(p, v.slot);
+  for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
+  if (threadIdx.x < kMetricWords) *metric_at(p, v.slot, threadIdx.x) = 0;
+  if (threadIdx.x == 0) {
+    load_bands(v, state_at(p, v.slot));
+    v.redo[0] = v.redo[1] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *p.cell_queue_count = 0;
+    *p.work_counter = 0;
+    p.flags[0] = p.flags[1] = p.flags[2] = p.flags[3] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ 1. cells (cellNode, GeometryPool.ts:728-760)
+
+__global__ __launch_bounds__(256) void k_cells(Params p) {
+  const uint32_t vi = blockIdx.y;
+  const ViewDev& v = p.views[vi];
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  if (cell < p.cell_count) {
+    float4 sphere = p.c_sphere[cell];
+    float4 record = p.c_rec[cell];
+    uint32_t slot = static_cast<uint32_t>(record.y);
+    float4 flags = p.mesh_table[slot * kMeshStride + 5];
+    if (flags.x != 0.0f) {
+      float radius = sphere.w;
+      bool out = v.frustum == 1 && frustum_distance(v, sphere.x, sphere.y, sphere.z, 2, flags.y > 0.5f) < -radius;
+      if (!out) {
+        float4 levers = p.mesh_table[slot * kMeshStride + 3];
+        float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
+        float max_radius = record.x;
+        float4 vc = mat_mul16(v.view, sphere.x, sphere.y, sphere.z, 1.0f);
+        float view_distance = len3(vc.x, vc.y, vc.z);
+        float gap = view_distance - radius;
+        bool small = gap > max_radius && (max_radius * v.proj_scale) / fmaxf(gap, 1e-6f) < min_pixels;
+        keep = !small && !(gap > levers.x);
+      }
+    }
+  }
+  uint32_t tested = cell < p.cell_count ? 1u : 0u;
+  wave_count_add(metric_at(p, v.slot, M_CELLS), tested, tested != 0);
+  uint32_t rank = wave_atomic_add_uniform(p.cell_queue_count, 1u, keep);
+  wave_atomic_add_uniform(view_counters(p, v.slot), 1u, keep);  // counters[0]: visible cells
+  if (keep) p.cell_queue[rank] = make_uint2(vi, cell);
+}
+
+// ------------------------------------------------------------------ 2. instances (instancePass + selectLevels, GeometryPool.ts:771-913)
+
+struct LevelResult {
+  uint32_t first, last, possible, guaranteed;
+};
+
+__device__ __forceinline__ LevelResult select_levels(const Params& p, uint32_t level_base, uint32_t level_count, float low,
+                                                     float high, float pmin, float pmax) {
+  LevelResult r{0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t i = 0; i < level_count; i++) {
+    float4 l0 = p.mesh_table[level_base + i * 2];
+    float4 l1 = p.mesh_table[level_base + i * 2 + 1];
+    uint32_t level_first = static_cast<uint32_t>(l0.x);
+    uint32_t size = static_cast<uint32_t>(l0.y);
+    bool possible = (l0.z * pmax <= high) && (l0.w >= VG_INF_HALF || l0.w * pmin > low);
+    bool sure = (l1.x * pmin <= low * VG_MARGIN_LO) && (l1.y >= VG_INF_HALF || l1.y * pmax > high * VG_MARGIN_HI);
+    if (possible) {
+      r.first = min(r.first, level_first);
+      r.last = max(r.last, level_first + size);
+      r.possible += size;
+      if (sure) r.guaranteed += size;
+    }
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
+  if (retry_round && !p.flags[retry_round - 1]) return;
+  const uint32_t lane = lane_id();
+  const uint32_t queue_count = *p.cell_queue_count;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t q = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < queue_count; q += waves) {
+    uint2 entry = p.cell_queue[q];
+    const uint32_t vi = uniform(entry.x);
+    const uint32_t cell = uniform(entry.y);
+    const ViewDev& v = p.views[vi];
+    bool redo_camera = true, redo_shadow = true;
+    if (retry_round) {
+      redo_camera = v.redo[0] != 0;
+      redo_shadow = v.redo[1] != 0;
+      if (!redo_camera && !redo_shadow) continue;
+    }
+    const float4 record = p.c_rec[cell];
+    const uint32_t slot = uniform(static_cast<uint32_t>(record.y));
+    const uint32_t first_instance = uniform(static_cast<uint32_t>(record.z));
+    const uint32_t count = uniform(static_cast<uint32_t>(record.w));
+    const float4 levers = p.mesh_table[slot * kMeshStride + 3];
+    const uint32_t level_count = min(static_cast<uint32_t>(levers.z), kMaxLevels);
+    const uint32_t level_base = static_cast<uint32_t>(levers.w);
+    const float4 object_sphere = p.mesh_table[slot * kMeshStride + 0];
+    const float4 lod_bounds = p.mesh_table[slot * kMeshStride + 1];
+    const float4 cull_bounds = p.mesh_table[slot * kMeshStride + 2];
+    const float4 flags = p.mesh_table[slot * kMeshStride + 5];
+    const bool shadow_on = flags.y > 0.5f, occlusion_on = flags.z > 0.5f;
+    const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
+    const float max_draw_distance = levers.x;
+    uint32_t* draws_base = reinterpret_cast<uint32_t*>(p.draws[v.slot]);
+
+    for (uint32_t r = 0; r < kCellSize; r += 64) {
+      const bool valid = r + lane < count;
+      const uint32_t instance = first_instance + r + lane;
+      float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = make_float4(0, 0, 0, 1);
+      if (valid) {
+        c0 = p.i_col0[instance];
+        c1 = p.i_col1[instance];
+        c2 = p.i_col2[instance];
+        c3 = p.i_col3[instance];
+      }
+      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+      const float radius = object_sphere.w * scale;
+      const bool no_frustum = v.frustum == 0;
+      bool in_camera = valid && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
+      bool in_shadow = valid && shadow_on && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, false) >= -radius);
+      in_camera = in_camera && redo_camera;
+      in_shadow = in_shadow && redo_shadow;
+      const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
+      const float view_distance = len3(vc.x, vc.y, vc.z);
+      const bool too_small = view_distance > radius && (radius * v.proj_scale) / view_distance < min_pixels;
+      const bool too_far = (view_distance - radius) > max_draw_distance;
+      const bool keep = (in_camera || in_shadow) && !too_small && !too_far;
+      wave_count_add(metric_at(p, v.slot, M_INSTANCES), 1, valid);
+      bool occ = false;
+      if (keep && in_camera && occlusion_on && v.hzb_valid) occ = occluded_sphere(p, v, true, center.x, center.y, center.z, radius);
+      wave_count_add(metric_at(p, v.slot, M_OCCLUSION), 1, keep && in_camera && occlusion_on && v.hzb_valid);
+      if (occ) in_camera = false;
+      wave_atomic_add_uniform(view_counters(p, v.slot) + 1, 1u, occ);
+
+      const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
+      const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
+      const float d = len3(lv.x, lv.y, lv.z);
+      const float rr = lod_bounds.w * scale;
+      const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
+      const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
+      const float4 cc = mat_mul(c0, c1, c2, c3, cull_bounds.x, cull_bounds.y, cull_bounds.z, 1.0f);
+      const float cull_radius = cull_bounds.w * scale;
+
+#pragma unroll
+      for (uint32_t cut = 0; cut < 2; cut++) {
+        const bool in_cut = keep && (cut == 0 ? in_camera : in_shadow);
+        LevelResult lr{0xffffffffu, 0u, 0u, 0u};
+        if (__ballot(in_cut)) lr = select_levels(p, level_base, level_count, v.low[cut], v.high[cut], pmin, pmax);
+        const bool survived = in_cut && lr.last > lr.first;
+        const bool fast = survived && lr.possible == lr.last - lr.first && lr.guaranteed == lr.possible &&
+                          lr.possible <= kFastPathMax &&
+                          (v.frustum == 0 || frustum_distance(v, cc.x, cc.y, cc.z, cut == 1 ? 1 : 0, false) >= cull_radius);
+        if (!__ballot(survived)) continue;
+        wave_atomic_add_uniform(counter_at(p, v.slot, cut, kSurvived), 1u, survived);
+        // Fast path: the lane's `possible` meshlets go straight to the draw list.
+        if (__ballot(fast)) {
+          const uint32_t base = wave_atomic_add_uniform(counter_at(p, v.slot, cut, kSelected), lr.possible, fast);
+          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), lr.possible, fast);
+          uint32_t tris[kFastPathMax];
+          uint32_t emit_sum = 0, emit_n = 0;
+#pragma unroll
+          for (uint32_t k = 0; k < kFastPathMax; k++) {
+            tris[k] = 0;
+            if (fast && k < lr.possible && base + k < p.max_meshlets) {
+              tris[k] = p.m_tris[lr.first + k];
+              emit_sum += tris[k] * 3;
+              emit_n = k + 1;
+            }
+          }
+          uint32_t index = wave_atomic_add_uniform(mesh_count_at(p, v.slot, cut, slot, 0), emit_sum, fast && emit_n > 0);
+          for (uint32_t k = 0; k < emit_n; k++) {
+            uint4 e = make_uint4(instance, lr.first + k, index, tris[k] | kFullFadeBits);
+            reinterpret_cast<uint4*>(draws_base)[base + k + cut * p.max_meshlets] = e;
+            index += tris[k] * 3;
+          }
+        }
+        // Work item: (instance, first, last, tag) plus the item's first cluster test.
+        const bool work = survived && !fast;
+        if (__ballot(work)) {
+          const uint32_t len = work ? lr.last - lr.first : 0u;
+          const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
+          const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
+          const uint32_t task_incl = wave_scan(len);
+          const uint32_t tasks = __shfl(task_incl, 63, 64);
+          unsigned long long old = 0;
+          if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
+          const uint32_t old_lo = __shfl(static_cast<uint32_t>(old), 0, 64);
+          const uint32_t old_hi = __shfl(static_cast<uint32_t>(old >> 32), 0, 64);
+          const uint64_t o = (static_cast<uint64_t>(old_hi) << 32) | old_lo;
+          const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
+          const uint64_t task_base = o & kTaskMask;
+          if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
+          if (work && item_base + rank < p.work_capacity) {
+            const uint32_t tag = slot | (cut << 16) | ((cut == 0 && occlusion_on ? 1u : 0u) << 17) | (vi << 18);
+            p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
+            p.work_start[item_base + rank] = task_base + task_incl - len;
+          }
+          wave_atomic_add_uniform(counter_at(p, v.slot, cut, kWork), 1u, work);
+          wave_count_add(metric_at(p, v.slot, M_WORK0 + cut), 1, work);
+          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), len, work);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 3. meshlets (meshletPass, GeometryPool.ts:926-968)
+
+__global__ __launch_bounds__(256) void k_meshlets(Params p) {
+  const unsigned long long packed = *p.work_counter;
+  const uint32_t items = min(static_cast<uint32_t>(packed >> kTaskBits), p.work_capacity);
+  const uint64_t total = packed & kTaskMask;
+  if (total == 0 || items == 0 || p.flags[3]) return;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+  const uint64_t seg_begin = total * wave / waves;
+  const uint64_t seg_end = total * (wave + 1) / waves;
+  if (seg_begin >= seg_end) return;
+
+  // The item holding seg_begin: the last item whose first task is <= seg_begin (64-ary search).
+  uint32_t lo = 0, hi = items;
+  while (hi - lo > 1) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t probe = lo + lane * step;
+    const bool le = probe < hi && p.work_start[probe] <= seg_begin;
+    const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(le)));  // >= 1: work_start[lo] <= seg_begin
+    const uint32_t nlo = lo + (n - 1) * step;
+    hi = min(hi, nlo + step);
+    lo = nlo;
+  }
+  uint32_t cur = uniform(lo);
+  uint64_t cur_start = p.work_start[cur];
+
+  for (uint64_t k0 = seg_begin; k0 < seg_end; k0 += 64) {
+    // Distances from k0 to the first tasks of the next 64 items (at least 1, clamped to 65).
+    const uint32_t next = cur + 1 + lane;
+    const uint64_t ns = next < items ? p.work_start[next] : ~0ull;
+    const uint32_t dist = ns - k0 > 65 ? 65u : static_cast<uint32_t>(ns - k0);
+    // pos = number of items after `cur` that start at or before task k0 + lane.
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t s = 32; s >= 1; s >>= 1) {
+      const uint32_t d = __shfl(dist, static_cast<int>(pos + s - 1), 64);
+      if (d <= lane) pos += s;
+    }
+    const uint32_t prev = __shfl(dist, static_cast<int>(pos > 0 ? pos - 1 : 0), 64);
+    const uint64_t k = k0 + lane;
+    const bool valid = k < seg_end;
+    const uint32_t item = cur + pos;
+    const uint64_t item_start = pos == 0 ? cur_start : k0 + prev;
+    uint4 w = make_uint4(0, 0, 0, 0);
+    if (valid) w = p.work[item];
+    const uint32_t meshlet = w.y + static_cast<uint32_t>(k - item_start);
+    const uint32_t tag = w.w;
+    const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
+    const bool occ_on = ((tag >> 17) & 1u) != 0;
+    const ViewDev& v = p.views[vi];
+
+    bool drawn = false, occ_test = false;
+    uint32_t packed_tris = 0, tris = 0;
+    if (valid) {
+      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float4 cs = p.m_cull[meshlet];
+      const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+      bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
+      occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
+      if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
+      const float2 err = p.m_err[meshlet];
+      const float own = projected_error(v, c0, c1, c2, c3, scale, p.m_lod[meshlet], err.x);
+      const float parent = projected_error(v, c0, c1, c2, c3, scale, p.m_parent[meshlet], err.y);
+      const float low = v.low[cut], high = v.high[cut], fade = v.fade_scale;
+      uint32_t fade_low = 0, fade_high = kFadeSteps;
+      if (fade > 0.0f) {
+        fade_low = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(own, 1e-30f) / low) * fade), 0.0f, 255.0f));
+        fade_high = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(parent, 1e-30f) / low) * fade), 0.0f, 255.0f));
+      }
+      drawn = visible && own <= high && parent > low && fade_high > fade_low;
+      if (drawn) {
+        tris = p.m_tris[meshlet];
+        packed_tris = tris | (fade_low << 8) | (fade_high << 16);
+      }
+    }
+    const uint32_t vslot = v.slot;
+    wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
+    uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
+    const bool write = drawn && at < p.max_meshlets;
+    const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
+    const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
+    if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(w.x, meshlet, first_index, packed_tris);
+
+    // The item holding task k0 + 64.
+    const uint32_t last_pos = __shfl(pos, 63, 64);
+    const uint32_t last_prev = __shfl(prev, 63, 64);
+    const uint32_t after = __shfl(dist, static_cast<int>(last_pos), 64);
+    if (last_pos > 0) {
+      cur_start = k0 + last_prev;
+      cur += last_pos;
+    }
+    if (after == 64) {
+      cur += 1;
+      cur_start = k0 + 64;
+    }
+    cur = uniform(cur);
+  }
+}
+
+// ------------------------------------------------------------------ 4. prefix + capacity (GeometryPool.ts:973-995, 1023-1069)
+
+__device__ void prefix_cut(const Params& p, uint32_t slot, uint32_t cut) {
+  const uint32_t lane = lane_id();
+  const uint32_t capacity = p.cut_capacity[cut];
+  uint32_t* args = p.args + static_cast<size_t>(slot) * p.args_stride;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < p.mesh_slots; base += 64) {
+    const uint32_t mesh = base + lane;
+    const bool valid = mesh < p.mesh_slots;
+    uint32_t c0 = valid ? *mesh_count_at(p, slot, cut, mesh, 0) : 0u;
+    uint32_t c1 = valid ? *mesh_count_at(p, slot, cut, mesh, 1) : 0u;
+    const uint32_t sum = c0 + c1;
+    const uint32_t incl = wave_scan(sum);
+    const uint32_t start0 = carry + incl - sum;
+    const uint32_t start1 = start0 + c0;
+    if (valid) {
+      const uint32_t starts[2] = {start0, start1};
+      const uint32_t req[2] = {c0, c1};
+      for (uint32_t part = 0; part < 2; part++) {
+        *(mesh_count_at(p, slot, cut, mesh, part) + 1) = starts[part];
+        const uint32_t room = starts[part] < capacity ? capacity - starts[part] : 0u;
+        uint32_t* a = args + cut * p.slots * 10 + part * 5 + mesh * 10;
+        a[0] = req[part] < room ? req[part] : room;
+        a[1] = 1;
+        a[2] = starts[part];
+        a[3] = 0;
+        a[4] = 0;
+      }
+    }
+    carry += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) {
+    *counter_at(p, slot, cut, kRequested) = carry;
+    const uint32_t selected = *counter_at(p, slot, cut, kSelected);
+    *counter_at(p, slot, cut, kDrawn) = selected < p.max_meshlets ? selected : p.max_meshlets;
+  }
+}
+
+__device__ float fill_of(const Params& p, uint32_t slot, uint32_t cut) {
+  const float requested = static_cast<float>(*counter_at(p, slot, cut, kRequested));
+  const float selected = static_cast<float>(*counter_at(p, slot, cut, kSelected));
+  return fmaxf(requested / static_cast<float>(p.cut_capacity[cut]), selected / static_cast<float>(p.max_meshlets));
+}
+
+// mode 0: prefix then checkNode of retry round `round` (1-based); mode 1: prefix then steerNode.
+__global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_t round) {
+  ViewDev& v = p.views[blockIdx.x];
+  const uint32_t slot = v.slot;
+  prefix_cut(p, slot, threadIdx.x / 64);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  uint32_t* state = state_at(p, slot);
+  if (mode == 0) {
+    if (blockIdx.x == 0) *p.work_counter = 0;  // the retry round's instance pass refills the work list
+    bool any = false;
+    for (uint32_t cut = 0; cut < 2; cut++) {
+      const float fill = fill_of(p, slot, cut);
+      const float scale = get_state(state, kStateScale + cut);
+      set_state(state, kStateRetry + cut, 0.0f);
+      v.redo[cut] = 0;
+      if (fill > 1.0f && scale < v.lod_scale_limit) {
+        set_state(state, kStateScale + cut, fminf(scale * clampf(sqrtf(fill / 0.82f), 1.1f, 4.0f), v.lod_scale_limit));
+        set_state(state, kStateRetry + cut, 1.0f);
+        v.redo[cut] = 1;
+        any = true;
+        for (uint32_t k = 0; k < kCutCounters; k++) *counter_at(p, slot, cut, k) = 0;
+        if (cut == 0) view_counters(p, slot)[1] = 0;
+        for (uint32_t i = 0; i < p.mesh_slots; i++) {
+          *mesh_count_at(p, slot, cut, i, 0) = 0;
+          *mesh_count_at(p, slot, cut, i, 1) = 0;
+        }
+      }
+    }
+    load_bands(v, state);
+    if (any) atomicOr(&p.flags[round - 1], 1u);
+  } else {
+    for (uint32_t cut = 0; cut < 2; cut++) {
+      const float fill = fill_of(p, slot, cut);
+      float scale = get_state(state, kStateScale + cut);
+      if (fill > 0.9f) scale = scale * fminf(sqrtf(fill / 0.82f), 1.03f);
+      else if (fill < 0.75f) scale = scale / 1.01f;
+      set_state(state, kStateScale + cut, fmaxf(fminf(scale, v.lod_scale_limit), 1.0f));
+      set_state(state, kStateFill + cut, fill);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ HZB (OcclusionCulling.ts:302-350)
+
+// Reduces levels base+1 .. base+5 (those that exist) over 32 x 32 texel tiles of level `base`.
+__global__ __launch_bounds__(256) void k_hzb_pass(float* hzb, uint32_t base) {
+  __shared__ float tile[2][32 * 32];
+  const uint32_t levels = static_cast<uint32_t>(hzb[kHzbLevelCountSlot]);
+  const uint32_t W = static_cast<uint32_t>(hzb[base * 4 + 1]), H = static_cast<uint32_t>(hzb[base * 4 + 2]);
+  const uint32_t src_offset = static_cast<uint32_t>(hzb[base * 4]);
+  const uint32_t tx = blockIdx.x * 32, ty = blockIdx.y * 32;
+  for (uint32_t i = threadIdx.x; i < 32 * 32; i += blockDim.x) {
+    const uint32_t x = min(tx + (i & 31), W - 1), y = min(ty + (i >> 5), H - 1);
+    tile[0][i] = hzb[src_offset + y * W + x];
+  }
+  __syncthreads();
+  uint32_t cur = 0;
+  for (uint32_t l = base + 1, side = 16; l < levels && l <= base + 5; l++, side >>= 1) {
+    const uint32_t sw = static_cast<uint32_t>(hzb[(l - 1) * 4 + 1]), sh = static_cast<uint32_t>(hzb[(l - 1) * 4 + 2]);
+    const uint32_t lw = static_cast<uint32_t>(hzb[l * 4 + 1]), lh = static_cast<uint32_t>(hzb[l * 4 + 2]);
+    const uint32_t off = static_cast<uint32_t>(hzb[l * 4]);
+    const uint32_t ox = tx >> (l - base), oy = ty >> (l - base);       // tile origin at level l
+    const uint32_t px = tx >> (l - 1 - base), py = ty >> (l - 1 - base);  // tile origin at level l - 1
+    for (uint32_t i = threadIdx.x; i < side * side; i += blockDim.x) {
+      const uint32_t lx = i % side, ly = i / side;
+      const uint32_t x = ox + lx, y = oy + ly;
+      float value = 0.0f;
+      if (x < lw && y < lh) {
+        const uint32_t x0 = x * 2, y0 = y * 2;
+        const uint32_t x1 = min(x0 + 1, sw - 1), y1 = min(y0 + 1, sh - 1);
+        const float* s = tile[cur];
+        const float a = fmaxf(s[(y0 - py) * 32 + (x0 - px)], s[(y0 - py) * 32 + (x1 - px)]);
+        const float b = fmaxf(s[(y1 - py) * 32 + (x0 - px)], s[(y1 - py) * 32 + (x1 - px)]);
+        value = fmaxf(a, b);
+        hzb[off + y * lw + x] = value;
+      }
+      tile[cur ^ 1][ly * 32 + lx] = value;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+// ------------------------------------------------------------------ arithmetic probe (tests)
+
+// Evaluates the operations the cut's decisions depend on, for comparison with the host.
+__global__ void k_arith_probe(const float* a, const float* b, float* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i], y = b[i];
+  out[i * 5 + 0] = x / y;
+  out[i * 5 + 1] = __builtin_sqrtf(fabsf(x));
+  out[i * 5 + 2] = vg_logf(fabsf(x));
+  out[i * 5 + 3] = rintf(x);
+  out[i * 5 + 4] = static_cast<float>(__float_as_uint(y) >> 3);
+}
+
+}  // namespace vg

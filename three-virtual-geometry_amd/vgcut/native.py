@@ -1,0 +1,328 @@
+"""ctypes bindings of the C ABI in include/vgcut.h (lib/libvgcut.so).
+
+The library is required: importing the bindings without it raises, so no caller can silently run
+without the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+
+VG_OK = 0
+VG_E_INVALID = -1
+VG_E_NOMEM = -2
+VG_E_CAPACITY = -3
+VG_E_DEVICE = -4
+
+VG_POOL_MESH_TABLE = 0
+VG_POOL_INSTANCES = 1
+VG_POOL_CELLS = 2
+VG_POOL_MESHLET_BOUNDS = 3
+VG_POOL_MESHLET_INFO = 4
+
+LIBRARY = _native.library_path("libvgcut.so")
+
+# Every entry point include/vgcut.h declares (the ABI test checks the library exports them).
+EXPORTS = (
+    "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
+    "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
+    "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
+    "vg_counter_words", "vg_draw_args_words",
+)
+
+
+class VgError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"vgcut error {code}: {message}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("max_triangles", ctypes.c_uint32),
+        ("max_shadow_triangles", ctypes.c_uint32),
+        ("max_meshlets", ctypes.c_uint32),
+        ("max_views", ctypes.c_uint32),
+        ("mesh_slots", ctypes.c_uint32),
+        ("hzb_max_size", ctypes.c_uint32),
+        ("budget_bins", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("meshlet_count", ctypes.c_uint32),
+        ("meshlet_bounds", ctypes.c_void_p),
+        ("meshlet_info", ctypes.c_void_p),
+        ("level_count", ctypes.c_uint32),
+        ("level_ranges", ctypes.c_void_p),
+        ("level_errors", ctypes.c_void_p),
+        ("level_guard_errors", ctypes.c_void_p),
+        ("bounding_sphere", ctypes.c_float * 4),
+        ("lod_bounds_sphere", ctypes.c_float * 4),
+        ("cull_bounds_sphere", ctypes.c_float * 4),
+    ]
+
+
+class View(ctypes.Structure):
+    _fields_ = [
+        ("view_matrix", ctypes.c_float * 16),
+        ("projection_matrix", ctypes.c_float * 16),
+        ("frustum_planes", ctypes.c_float * 24),
+        ("shadow_sweep", ctypes.c_float * 3),
+        ("proj_scale", ctypes.c_float),
+        ("near_plane", ctypes.c_float),
+        ("error_threshold", ctypes.c_float),
+        ("shadow_error_scale", ctypes.c_float),
+        ("lod_blend_max", ctypes.c_float),
+        ("lod_scale_limit", ctypes.c_float),
+        ("min_pixel_radius", ctypes.c_float),
+        ("frustum_culling", ctypes.c_uint32),
+        ("occlusion", ctypes.c_uint32),
+        ("slot", ctypes.c_uint32),
+        ("triangle_budget", ctypes.c_uint32),
+        ("threshold_range", ctypes.c_float * 2),
+        ("max_error_threshold", ctypes.c_float),
+    ]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [
+        ("cells_tested", ctypes.c_uint64),
+        ("instances_tested", ctypes.c_uint64),
+        ("work_items", ctypes.c_uint64 * 2),
+        ("clusters_tested", ctypes.c_uint64 * 2),
+        ("occlusion_tests", ctypes.c_uint64),
+        ("error_threshold", ctypes.c_float),
+        ("budget_bin", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Loads lib/libvgcut.so (raises when it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise RuntimeError(f"{LIBRARY} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(LIBRARY)
+    P, U32, I32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+    sig = {
+        "vg_create": ([ctypes.POINTER(Config), ctypes.POINTER(P)], ctypes.c_int),
+        "vg_destroy": ([P], None),
+        "vg_last_error": ([P], ctypes.c_char_p),
+        "vg_scene_create": ([P, ctypes.POINTER(P)], ctypes.c_int),
+        "vg_scene_destroy": ([P], None),
+        "vg_scene_bytes": ([P], U64),
+        "vg_mesh_add": ([P, ctypes.POINTER(MeshDesc), ctypes.POINTER(U32)], ctypes.c_int),
+        "vg_instances_set": ([P, U32, P, P, U32, P], ctypes.c_int),
+        "vg_mesh_flags": ([P, U32, ctypes.c_float, ctypes.c_float, U32, U32, U32], ctypes.c_int),
+        "vg_hzb_set": ([P, U32, P, U32, U32], ctypes.c_int),
+        "vg_hzb_invalidate": ([P, U32], ctypes.c_int),
+        "vg_hzb_read": ([P, U32, P, U64, ctypes.POINTER(U64)], ctypes.c_int),
+        "vg_cut": ([P, P, U32], ctypes.c_int),
+        "vg_cut_graph": ([P, P, U32], ctypes.c_int),
+        "vg_sync": ([P], ctypes.c_int),
+        "vg_read": ([P, U32, P, P, P, U32, ctypes.POINTER(Metrics)], ctypes.c_int),
+        "vg_state_get": ([P, U32, P], ctypes.c_int),
+        "vg_state_set": ([P, U32, P], ctypes.c_int),
+        "vg_scene_pool_read": ([P, U32, P, U64, ctypes.POINTER(U64)], ctypes.c_int),
+        "vg_scene_mesh_slots": ([P], U32),
+        "vg_mesh_instance_base": ([P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32)], ctypes.c_int),
+        "vg_counter_words": ([P], U32),
+        "vg_draw_args_words": ([P], U32),
+        "vg_arith_probe": ([P, P, P, U32], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    del I32
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """A vg_ctx: capacities and the HIP stream (VirtualGeometry constructor options)."""
+
+    def __init__(self, max_triangles=10_000_000, max_shadow_triangles=4_000_000, max_meshlets=1 << 20, max_views=1,
+                 mesh_slots=64, hzb_max_size=1024, budget_bins=256, device=0):
+        self.lib = load()
+        cfg = Config(max_triangles, max_shadow_triangles, max_meshlets, max_views, mesh_slots, hzb_max_size, budget_bins, device)
+        self.config = cfg
+        h = ctypes.c_void_p()
+        rc = self.lib.vg_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != VG_OK:
+            raise VgError(rc, "vg_create failed (no HIP device?)")
+        self.handle = h
+
+    def check(self, rc: int, what: str = ""):
+        if rc != VG_OK:
+            msg = self.lib.vg_last_error(self.handle)
+            raise VgError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @property
+    def counter_words(self) -> int:
+        return int(self.lib.vg_counter_words(self.handle))
+
+    @property
+    def draw_args_words(self) -> int:
+        return int(self.lib.vg_draw_args_words(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A vg_scene: one resident GeometryPool."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.vg_scene_create(ctx.handle, ctypes.byref(h)), "vg_scene_create")
+        self.handle = h
+
+    def add_mesh(self, data) -> int:
+        bounds = np.ascontiguousarray(data.meshletBounds, dtype=np.float32)
+        info = np.ascontiguousarray(data.meshletInfo, dtype=np.uint32)
+        ranges = np.ascontiguousarray(data.levelRanges, dtype=np.uint32)
+        errors = np.ascontiguousarray(data.levelErrors, dtype=np.float32)
+        guard = np.ascontiguousarray(data.levelGuardErrors, dtype=np.float32)
+        d = MeshDesc()
+        d.meshlet_count = data.meshletCount
+        d.meshlet_bounds = bounds.ctypes.data
+        d.meshlet_info = info.ctypes.data
+        d.level_count = len(ranges) // 2
+        d.level_ranges = ranges.ctypes.data
+        d.level_errors = errors.ctypes.data
+        d.level_guard_errors = guard.ctypes.data
+        for i in range(4):
+            d.bounding_sphere[i] = float(data.boundingSphere[i])
+            d.lod_bounds_sphere[i] = float(data.lodBoundsSphere[i])
+            d.cull_bounds_sphere[i] = float(data.cullBoundsSphere[i])
+        slot = ctypes.c_uint32()
+        self.ctx.check(self.lib.vg_mesh_add(self.handle, ctypes.byref(d), ctypes.byref(slot)), "vg_mesh_add")
+        return int(slot.value)
+
+    def set_instances(self, slot: int, matrices: np.ndarray, colors: Optional[np.ndarray] = None) -> np.ndarray:
+        m = np.ascontiguousarray(matrices, dtype=np.float32).reshape(-1, 16)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 4)
+        order = np.empty(len(m), dtype=np.uint32)
+        self.ctx.check(self.lib.vg_instances_set(self.handle, slot, _ptr(m), _ptr(c), len(m), _ptr(order)), "vg_instances_set")
+        return order
+
+    def set_flags(self, slot: int, max_draw_distance=1e30, min_pixel_radius=0.0, enabled=True, shadow=True, occlusion=False):
+        self.ctx.check(self.lib.vg_mesh_flags(self.handle, slot, float(max_draw_distance), float(min_pixel_radius), int(bool(enabled)),
+                                              int(bool(shadow)), int(bool(occlusion))), "vg_mesh_flags")
+
+    def set_hzb(self, view_slot: int, depth: np.ndarray):
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        h, w = d.shape
+        self.ctx.check(self.lib.vg_hzb_set(self.handle, view_slot, d.ctypes.data, w, h), "vg_hzb_set")
+
+    def hzb(self, view_slot: int) -> np.ndarray:
+        n = ctypes.c_uint64()
+        self.ctx.check(self.lib.vg_hzb_read(self.handle, view_slot, None, 0, ctypes.byref(n)), "vg_hzb_read")
+        out = np.empty(n.value, dtype=np.float32)
+        if n.value:
+            self.ctx.check(self.lib.vg_hzb_read(self.handle, view_slot, out.ctypes.data, n.value, None), "vg_hzb_read")
+        return out
+
+    def invalidate_hzb(self, view_slot: int):
+        self.ctx.check(self.lib.vg_hzb_invalidate(self.handle, view_slot), "vg_hzb_invalidate")
+
+    def cut(self, views, graph: bool = False):
+        arr = (View * len(views))(*views)
+        fn = self.lib.vg_cut_graph if graph else self.lib.vg_cut
+        self.ctx.check(fn(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views)), "vg_cut")
+        self._keep = arr
+
+    def sync(self):
+        self.ctx.check(self.lib.vg_sync(self.handle), "vg_sync")
+
+    def read(self, view_slot: int, max_entries: Optional[int] = None):
+        """Counters, draw args, draw lists (camera, shadow) and metrics of a view slot."""
+        if max_entries is None:
+            max_entries = self.ctx.config.max_meshlets
+        counters = np.empty(self.ctx.counter_words, dtype=np.uint32)
+        args = np.empty(self.ctx.draw_args_words, dtype=np.uint32)
+        draws = np.empty((2, max_entries, 4), dtype=np.uint32)
+        met = Metrics()
+        self.ctx.check(self.lib.vg_read(self.handle, view_slot, counters.ctypes.data, args.ctypes.data, draws.ctypes.data, max_entries,
+                                        ctypes.byref(met)), "vg_read")
+        return counters, args, draws, met
+
+    def state(self, view_slot: int) -> np.ndarray:
+        s = np.empty(8, dtype=np.uint32)
+        self.ctx.check(self.lib.vg_state_get(self.handle, view_slot, s.ctypes.data), "vg_state_get")
+        return s
+
+    def set_state(self, view_slot: int, state: np.ndarray):
+        s = np.ascontiguousarray(state, dtype=np.uint32)
+        self.ctx.check(self.lib.vg_state_set(self.handle, view_slot, s.ctypes.data), "vg_state_set")
+
+    def pool(self, buffer: int) -> np.ndarray:
+        size = ctypes.c_uint64()
+        self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, None, 0, ctypes.byref(size)), "vg_scene_pool_read")
+        dtype = np.uint32 if buffer == VG_POOL_MESHLET_INFO else np.float32
+        out = np.empty(size.value // 4, dtype=dtype)
+        self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, out.ctypes.data, size.value, None), "vg_scene_pool_read")
+        return out
+
+    @property
+    def mesh_slots(self) -> int:
+        return int(self.lib.vg_scene_mesh_slots(self.handle))
+
+    def instance_base(self, slot: int):
+        base, count = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx.check(self.lib.vg_mesh_instance_base(self.handle, slot, ctypes.byref(base), ctypes.byref(count)), "vg_mesh_instance_base")
+        return int(base.value), int(count.value)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.lib.vg_scene_bytes(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def arith_probe(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Runs the device arithmetic probe: per pair (a / b, sqrt|a|, log|a|, rint(a), float(bits(b) >> 3))."""
+    lib = load()
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.empty((len(a), 5), dtype=np.float32)
+    rc = lib.vg_arith_probe(a.ctypes.data, b.ctypes.data, out.ctypes.data, len(a))
+    if rc != VG_OK:
+        raise VgError(rc, "vg_arith_probe failed")
+    return out
