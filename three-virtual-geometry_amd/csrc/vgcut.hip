@@ -755,6 +755,7 @@ int vg_sync(vg_scene* s) {
   uint32_t flags[4] = {0, 0, 0, 0};
   VG_HIP(ctx, hipMemcpy(flags, s->d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[3]) return fail(ctx, VG_E_CAPACITY, "vg_cut: work list capacity exceeded; results are incomplete");
+  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut: meshlet pass lost its work item (internal error)");
   return VG_OK;
 }
 

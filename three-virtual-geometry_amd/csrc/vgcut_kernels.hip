@@ -361,7 +361,8 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
     const uint32_t step = (hi - lo + 63) / 64;
     const uint32_t probe = lo + lane * step;
     const bool le = probe < hi && p.work_start[probe] <= seg_begin;
-    const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(le)));  // >= 1: work_start[lo] <= seg_begin
+    uint32_t n = static_cast<uint32_t>(__popcll(__ballot(le)));  // >= 1: work_start[lo] <= seg_begin
+    n = n < 1 ? 1u : n;
     const uint32_t nlo = lo + (n - 1) * step;
     hi = min(hi, nlo + step);
     lo = nlo;
@@ -383,12 +384,17 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
     }
     const uint32_t prev = __shfl(dist, static_cast<int>(pos > 0 ? pos - 1 : 0), 64);
     const uint64_t k = k0 + lane;
-    const bool valid = k < seg_end;
     const uint32_t item = cur + pos;
     const uint64_t item_start = pos == 0 ? cur_start : k0 + prev;
+    bool valid = k < seg_end && item < items && k >= item_start;
     uint4 w = make_uint4(0, 0, 0, 0);
     if (valid) w = p.work[item];
     const uint32_t meshlet = w.y + static_cast<uint32_t>(k - item_start);
+    if (valid && meshlet >= w.z) {  // the walk lost its item: report instead of reading past the range
+      atomicOr(&p.flags[2], 1u);
+      valid = false;
+      w = make_uint4(0, 0, 0, 0);
+    }
     const uint32_t tag = w.w;
     const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
     const bool occ_on = ((tag >> 17) & 1u) != 0;
