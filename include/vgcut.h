@@ -168,6 +168,17 @@ uint32_t vg_scene_mesh_slots(const vg_scene* scene);
 /* First GPU instance slot and instance count of a mesh (PoolEntry.instanceBase/instanceCount, GeometryPool.ts:161-162). */
 int vg_mesh_instance_base(const vg_scene* scene, uint32_t slot, uint32_t* base, uint32_t* count);
 
+/*
+ * Measurement. vg_profile(scene, 1) times every kernel of eager launch sequences with HIP events on the
+ * context's stream; vg_profile_read returns the summed milliseconds per kernel position (11 positions,
+ * `names` receives their names), the number of sequences, and resets them. vg_footprint runs one cut
+ * (like vg_cut, then waits) and counts the distinct instances tested and distinct meshlet records in
+ * candidate ranges over all views (the unique inputs one launch must read).
+ */
+int vg_profile(vg_scene* scene, uint32_t enable);
+int vg_profile_read(vg_scene* scene, double* kernel_ms, uint32_t max_kernels, uint64_t* sequences, const char** names);
+int vg_footprint(vg_scene* scene, const vg_view* views, uint32_t count, uint64_t* instances, uint64_t* meshlets);
+
 /* Layout helpers: counter words and draw-args words for the context's mesh slot capacity. */
 uint32_t vg_counter_words(const vg_ctx* ctx);
 uint32_t vg_draw_args_words(const vg_ctx* ctx);

@@ -77,10 +77,23 @@ struct vg_scene {
   uint32_t view_capacity = 0;
   size_t cell_queue_capacity = 0;
   uint32_t work_capacity = 0;
-  ViewDev* h_views = nullptr;  // pinned staging
-  hipEvent_t views_copied = nullptr;
-  bool views_copy_pending = false;
+  // Pinned staging ring for view records: a launch reuses a buffer only after its earlier copy ran,
+  // so back-to-back launches do not wait for each other on the host.
+  static constexpr int kRing = 16;
+  ViewDev* h_ring[kRing] = {nullptr};
+  hipEvent_t ring_copied[kRing] = {nullptr};
+  bool ring_pending[kRing] = {false};
+  uint32_t ring_next = 0;
   uint64_t layout_version = 1;
+  // profiling: events around each kernel of eager launch sequences
+  bool profile = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<std::vector<hipEvent_t>> pending;  // per sequence: kKernelSlots + 1 events
+  double kernel_ms[16] = {0};
+  uint64_t profiled_sequences = 0;
+  // footprint bitmaps (measurement only)
+  DevBuf d_fp_instances, d_fp_meshlets, d_fp_counts;
+  bool footprint = false;
   // captured launch sequence
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -322,28 +335,70 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.work_counter = static_cast<unsigned long long*>(s->d_work_counter.ptr);
   p.work_capacity = s->work_capacity;
   p.flags = static_cast<uint32_t*>(s->d_flags.ptr);
+  if (s->footprint) {
+    p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
+    p.fp_meshlets = static_cast<uint32_t*>(s->d_fp_meshlets.ptr);
+  }
   return p;
+}
+
+constexpr int kKernelSlots = 11;
+const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells", "k_instances", "k_meshlets", "k_prefix(check 1)",
+                                                "k_instances(retry 1)", "k_meshlets(retry 1)", "k_prefix(check 2)",
+                                                "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)"};
+
+hipEvent_t take_event(vg_scene* s) {
+  if (!s->event_pool.empty()) {
+    hipEvent_t e = s->event_pool.back();
+    s->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
 }
 
 // The launch sequence of one cut (GeometryPool.ts:1081-1092): begin, cells, instances, meshlets,
 // prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer).
-int enqueue_cut(vg_scene* s, uint32_t view_count) {
+int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   vg_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   Params p = make_params(s, view_count);
   const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * 4;
   const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * 8;
+  std::vector<hipEvent_t> ev;
+  const bool timed = s->profile && !capturing;
+  if (timed)
+    for (int i = 0; i <= kKernelSlots; i++) ev.push_back(take_event(s));
+  int k = 0;
+  auto mark = [&]() {
+    if (timed && ev[k]) (void)hipEventRecord(ev[k], st);
+    k++;
+  };
+  mark();
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
+  mark();
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, view_count), dim3(256), 0, st, p);
+  mark();
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
+  mark();
   hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  mark();
   hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 0u, 1u);
+  mark();
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 1u);
+  mark();
   hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  mark();
   hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 0u, 2u);
+  mark();
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 2u);
+  mark();
   hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  mark();
   hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 1u, 0u);
+  mark();
+  if (timed) s->pending.push_back(ev);
   VG_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -377,19 +432,25 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
     s->work_capacity = static_cast<uint32_t>(want);
   }
   if (s->view_capacity < count) {
-    if (int rc = ensure(s, s->d_views, sizeof(ViewDev) * count)) return rc;
-    if (s->h_views) (void)hipHostFree(s->h_views);
-    s->h_views = nullptr;
-    VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_views), sizeof(ViewDev) * count, hipHostMallocDefault));
-    s->view_capacity = count;
+    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc = ensure(s, s->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
+    for (int r = 0; r < vg_scene::kRing; r++) {
+      if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
+      s->h_ring[r] = nullptr;
+      s->ring_pending[r] = false;
+      VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_ring[r]), sizeof(ViewDev) * ctx->cfg.max_views, hipHostMallocDefault));
+    }
+    s->view_capacity = ctx->cfg.max_views;
   }
-  if (s->views_copy_pending) {
-    VG_HIP(ctx, hipEventSynchronize(s->views_copied));
-    s->views_copy_pending = false;
+  const int ring = static_cast<int>(s->ring_next++ % vg_scene::kRing);
+  if (s->ring_pending[ring]) {
+    VG_HIP(ctx, hipEventSynchronize(s->ring_copied[ring]));
+    s->ring_pending[ring] = false;
   }
+  ViewDev* staging = s->h_ring[ring];
   for (uint32_t i = 0; i < count; i++) {
     const vg_view& in = views[i];
-    ViewDev& v = s->h_views[i];
+    ViewDev& v = staging[i];
     std::memset(&v, 0, sizeof(v));
     std::memcpy(v.view, in.view_matrix, sizeof(v.view));
     std::memcpy(v.proj, in.projection_matrix, sizeof(v.proj));
@@ -411,9 +472,9 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
     v.range_hi = in.threshold_range[1];
     v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
   }
-  VG_HIP(ctx, hipMemcpyAsync(s->d_views.ptr, s->h_views, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
-  VG_HIP(ctx, hipEventRecord(s->views_copied, ctx->stream));
-  s->views_copy_pending = true;
+  VG_HIP(ctx, hipMemcpyAsync(s->d_views.ptr, staging, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
+  VG_HIP(ctx, hipEventRecord(s->ring_copied[ring], ctx->stream));
+  s->ring_pending[ring] = true;
   return 0;
 }
 
@@ -512,7 +573,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
     VG_HIP(ctx, hipMemset(s->d_draw_ptrs.ptr, 0, s->d_draw_ptrs.bytes));
     VG_HIP(ctx, hipMemset(s->d_hzb_ptrs.ptr, 0, s->d_hzb_ptrs.bytes));
     VG_HIP(ctx, hipMemset(s->d_flags.ptr, 0, s->d_flags.bytes));
-    VG_HIP(ctx, hipEventCreateWithFlags(&s->views_copied, hipEventDisableTiming));
+    for (int r = 0; r < vg_scene::kRing; r++) VG_HIP(ctx, hipEventCreateWithFlags(&s->ring_copied[r], hipEventDisableTiming));
     return 0;
   };
   if (device && (rc = init())) {
@@ -531,12 +592,19 @@ void vg_scene_destroy(vg_scene* s) {
   DevBuf* bufs[] = {&s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
-                    &s->d_work_start, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage};
+                    &s->d_work_start, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
+                    &s->d_fp_meshlets, &s->d_fp_counts};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_hzbs) release(s, b);
-  if (s->h_views) (void)hipHostFree(s->h_views);
-  if (s->views_copied) (void)hipEventDestroy(s->views_copied);
+  for (int r = 0; r < vg_scene::kRing; r++) {
+    if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
+    if (s->ring_copied[r]) (void)hipEventDestroy(s->ring_copied[r]);
+  }
+  for (auto& seq : s->pending)
+    for (hipEvent_t e : seq)
+      if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
   delete s;
 }
 
@@ -732,7 +800,7 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
     s->graph_exec = nullptr;
     s->graph = nullptr;
     VG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_cut(s, count);
+    int rc = enqueue_cut(s, count, true);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (rc) return rc;
@@ -751,7 +819,7 @@ int vg_sync(vg_scene* s) {
   vg_ctx* ctx = s->ctx;
   if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
   VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  s->views_copy_pending = false;
+  for (bool& p : s->ring_pending) p = false;
   uint32_t flags[4] = {0, 0, 0, 0};
   VG_HIP(ctx, hipMemcpy(flags, s->d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[3]) return fail(ctx, VG_E_CAPACITY, "vg_cut: work list capacity exceeded; results are incomplete");
@@ -883,6 +951,66 @@ int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t c
 }
 
 uint32_t vg_scene_mesh_slots(const vg_scene* s) { return s ? s->mesh_slots : 0; }
+
+int vg_profile(vg_scene* s, uint32_t enable) {
+  if (!s) return VG_E_INVALID;
+  s->profile = enable != 0;
+  return VG_OK;
+}
+
+int vg_profile_read(vg_scene* s, double* kernel_ms, uint32_t max_kernels, uint64_t* sequences, const char** names) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& seq : s->pending) {
+    for (int i = 0; i < kKernelSlots; i++) {
+      float ms = 0;
+      if (seq[i] && seq[i + 1] && hipEventElapsedTime(&ms, seq[i], seq[i + 1]) == hipSuccess) s->kernel_ms[i] += ms;
+    }
+    for (hipEvent_t e : seq)
+      if (e) s->event_pool.push_back(e);
+    s->profiled_sequences++;
+  }
+  s->pending.clear();
+  for (uint32_t i = 0; i < max_kernels && i < static_cast<uint32_t>(kKernelSlots); i++) {
+    if (kernel_ms) kernel_ms[i] = s->kernel_ms[i];
+    if (names) names[i] = kKernelNames[i];
+  }
+  if (sequences) *sequences = s->profiled_sequences;
+  for (double& v : s->kernel_ms) v = 0;
+  s->profiled_sequences = 0;
+  return VG_OK;
+}
+
+int vg_footprint(vg_scene* s, const vg_view* views, uint32_t count, uint64_t* instances, uint64_t* meshlets) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (int rc = prepare_cut(s, views, count)) return rc;
+  const size_t iw = (s->i_col[0].size() + 31) / 32 + 1, mw = (s->m_cull.size() + 31) / 32 + 1;
+  int rc = 0;
+  if ((rc = ensure(s, s->d_fp_instances, iw * 4)) || (rc = ensure(s, s->d_fp_meshlets, mw * 4)) || (rc = ensure(s, s->d_fp_counts, 16)))
+    return rc;
+  VG_HIP(ctx, hipMemsetAsync(s->d_fp_instances.ptr, 0, iw * 4, ctx->stream));
+  VG_HIP(ctx, hipMemsetAsync(s->d_fp_meshlets.ptr, 0, mw * 4, ctx->stream));
+  VG_HIP(ctx, hipMemsetAsync(s->d_fp_counts.ptr, 0, 16, ctx->stream));
+  s->footprint = true;
+  rc = enqueue_cut(s, count);
+  s->footprint = false;
+  if (rc) return rc;
+  auto* counts = static_cast<unsigned long long*>(s->d_fp_counts.ptr);
+  hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, ctx->stream, static_cast<const uint32_t*>(s->d_fp_instances.ptr),
+                     static_cast<uint32_t>(iw), counts);
+  hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, ctx->stream, static_cast<const uint32_t*>(s->d_fp_meshlets.ptr),
+                     static_cast<uint32_t>(mw), counts + 1);
+  VG_HIP(ctx, hipGetLastError());
+  unsigned long long h[2];
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  VG_HIP(ctx, hipMemcpy(h, counts, sizeof(h), hipMemcpyDeviceToHost));
+  if (instances) *instances = h[0];
+  if (meshlets) *meshlets = h[1];
+  return vg_sync(s);
+}
 
 int vg_mesh_instance_base(const vg_scene* s, uint32_t slot, uint32_t* base, uint32_t* count) {
   if (!s || slot >= s->slots || !s->meshes[slot].used) return VG_E_INVALID;

@@ -92,6 +92,9 @@ struct Params {
   unsigned long long* work_counter;
   uint32_t work_capacity;
   uint32_t* flags;       // [0] any retry this round, [1] overflow
+  // footprint (measurement only): bit per instance tested / per meshlet in a candidate range
+  uint32_t* fp_instances;
+  uint32_t* fp_meshlets;
   // triangle budget
   const float* budget_thresholds;  // bins, ascending
   unsigned long long* budget_hist; // per view: bins + 1

@@ -207,6 +207,16 @@ __device__ __forceinline__ LevelResult select_levels(const Params& p, uint32_t l
   return r;
 }
 
+// Sets bits [first, last) of a bitmap (measurement only).
+__device__ void mark_range(uint32_t* bits, uint32_t first, uint32_t last) {
+  for (uint32_t w = first >> 5; w <= (last - 1) >> 5; w++) {
+    const uint32_t lo = w == (first >> 5) ? (first & 31) : 0u;
+    const uint32_t hi = w == ((last - 1) >> 5) ? ((last - 1) & 31) : 31u;
+    const uint32_t mask = (hi == 31 ? 0xffffffffu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+    atomicOr(&bits[w], mask);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
   if (retry_round && !p.flags[retry_round - 1]) return;
   const uint32_t lane = lane_id();
@@ -263,6 +273,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       const bool too_far = (view_distance - radius) > max_draw_distance;
       const bool keep = (in_camera || in_shadow) && !too_small && !too_far;
       wave_count_add(metric_at(p, v.slot, M_INSTANCES), 1, valid);
+      if (p.fp_instances && valid) atomicOr(&p.fp_instances[instance >> 5], 1u << (instance & 31));
       bool occ = false;
       if (keep && in_camera && occlusion_on && v.hzb_valid) occ = occluded_sphere(p, v, true, center.x, center.y, center.z, radius);
       wave_count_add(metric_at(p, v.slot, M_OCCLUSION), 1, keep && in_camera && occlusion_on && v.hzb_valid);
@@ -312,6 +323,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
           }
         }
         // Work item: (instance, first, last, tag) plus the item's first cluster test.
+        if (p.fp_meshlets && survived) mark_range(p.fp_meshlets, lr.first, lr.last);
         const bool work = survived && !fast;
         if (__ballot(work)) {
           const uint32_t len = work ? lr.last - lr.first : 0u;
@@ -576,6 +588,13 @@ __global__ __launch_bounds__(256) void k_hzb_pass(float* hzb, uint32_t base) {
     __syncthreads();
     cur ^= 1;
   }
+}
+
+// Population count of a bitmap into *out (measurement only).
+__global__ __launch_bounds__(256) void k_popcount(const uint32_t* bits, uint32_t words, unsigned long long* out) {
+  uint32_t n = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) n += __popc(bits[i]);
+  wave_count_add(out, n, n != 0);
 }
 
 // ------------------------------------------------------------------ arithmetic probe (tests)

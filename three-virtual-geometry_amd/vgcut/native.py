@@ -32,7 +32,7 @@ EXPORTS = (
     "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
     "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
-    "vg_counter_words", "vg_draw_args_words",
+    "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint",
 )
 
 
@@ -141,6 +141,9 @@ def load():
         "vg_counter_words": ([P], U32),
         "vg_draw_args_words": ([P], U32),
         "vg_arith_probe": ([P, P, P, U32], ctypes.c_int),
+        "vg_profile": ([P, U32], ctypes.c_int),
+        "vg_profile_read": ([P, P, U32, ctypes.POINTER(U64), P], ctypes.c_int),
+        "vg_footprint": ([P, P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -290,6 +293,27 @@ class Scene:
         out = np.empty(size.value // 4, dtype=dtype)
         self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, out.ctypes.data, size.value, None), "vg_scene_pool_read")
         return out
+
+    def profile(self, enable: bool = True):
+        self.ctx.check(self.lib.vg_profile(self.handle, 1 if enable else 0), "vg_profile")
+
+    def profile_read(self):
+        """Per kernel position: (name, summed ms); and the number of timed launch sequences."""
+        ms = (ctypes.c_double * 16)()
+        names = (ctypes.c_char_p * 16)()
+        seq = ctypes.c_uint64()
+        self.ctx.check(self.lib.vg_profile_read(self.handle, ctypes.cast(ms, ctypes.c_void_p), 16, ctypes.byref(seq),
+                                                ctypes.cast(names, ctypes.c_void_p)), "vg_profile_read")
+        out = [(names[i].decode(), float(ms[i])) for i in range(16) if names[i]]
+        return out, int(seq.value)
+
+    def footprint(self, views):
+        """(distinct instances tested, distinct meshlet records in candidate ranges) of one cut of `views`."""
+        arr = (View * len(views))(*views)
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.lib.vg_footprint(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views), ctypes.byref(a),
+                                             ctypes.byref(b)), "vg_footprint")
+        return int(a.value), int(b.value)
 
     @property
     def mesh_slots(self) -> int:
