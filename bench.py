@@ -1,0 +1,324 @@
+#!/usr/bin/env python3
+"""Cut benchmark on the 3M-instance world: views x clusters tested per second, plain vs batched.
+
+Workload: the world scene (64 procedural meshes of 0.1M-2M triangles, 3.0M instances by Poisson-disk
+sampling over a 6144 m x 6144 m heightfield, per-mesh draw distances 200-6000 m) cut for 64 mixed
+views: 4 main cameras (2 at 1920x1080, 2 at 3840x2160, device pixels), 16 orthographic shadow
+cascades at 2048^2, 24 cube-map faces at 512^2 (90 degrees, 60 m far) and 20 viewers at 1280x720.
+Perspective views with at least 720 lines are occlusion culled against the rasterized terrain.
+
+A step cuts all 64 views: the plain form launches one view per launch sequence (64 launches), the
+batched form cuts them in one launch sequence replayed from a HIP graph. `value` is the batched
+form's views x clusters tested per second with the scene resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "three-virtual-geometry_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Draw capacity per view: the reference's defaults are 10M / 4M triangles per pool (VirtualGeometry.ts:191-198);
+# the world's main views ask for up to ~60M triangles at 1 px, so the buffers are sized for MI355X instead
+# (no capacity retries at 1 px; a 300M-triangle index buffer is 3.6 GB).
+MAX_TRIANGLES = 300_000_000
+MAX_SHADOW_TRIANGLES = 100_000_000
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--instances", type=int, default=3_000_000)
+    ap.add_argument("--meshes", type=int, default=64)
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--latency-samples", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="a few batched launches for rocprofv3")
+    ap.add_argument("--check", action="store_true", help="compare the batched cut of every view with the oracle")
+    return ap.parse_args()
+
+
+def mixed_views(spec, count, vg_make_view, cam, scenegen, size=6144.0):
+    """The 64 mixed views (main, shadow cascades, cube faces, viewers) and their occluder depth."""
+    from vgcut import scenes
+
+    hf = spec.heightfield
+    path = scenes.flythrough(spec, 16, size, 30.0, seed=9)
+    views, depths = [], []
+    rng = np.random.default_rng(10)
+    slot = 0
+    sun = np.array([-0.4, -1.0, -0.3])
+    sun /= np.linalg.norm(sun)
+
+    def terrain_depth(camera, w, h):
+        vp = cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse)
+        return scenegen.heightfield_depth(vp, w, h, (0.0, 0.0), size / 2, 256, hf["amplitude"], hf["frequency"], hf["seed"])
+
+    mains = []
+    for i in range(4):
+        eye, target = path[i * 4]
+        width, height = (1920, 1080) if i < 2 else (3840, 2160)
+        camera = cam.perspective_camera(60, width / height, 0.1, 20000).look_at(eye, target)
+        mains.append((camera, eye, target))
+        hz = (960, 540) if i < 2 else (1024, 576)
+        views.append((camera, height, True, terrain_depth(camera, *hz)))
+    for camera, eye, target in mains:  # 4 cascades per main camera, centred ahead of it
+        fwd = np.asarray(target, dtype=np.float64) - np.asarray(eye, dtype=np.float64)
+        fwd /= np.linalg.norm(fwd)
+        for half in (20.0, 60.0, 180.0, 540.0):
+            center = np.asarray(eye) + fwd * half
+            ortho = cam.orthographic_camera(-half, half, half, -half, 0.1, 2000).look_at(center - sun * (half + 300), center)
+            views.append((ortho, 2048, False, None))
+    for camera, eye, target in mains:  # a reflection probe near each main camera: 6 faces
+        probe = np.asarray(eye) + np.array([5.0, -10.0, 5.0])
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            up = (0, 0, 1) if abs(d[1]) == 1 else (0, 1, 0)
+            face = cam.perspective_camera(90, 1.0, 0.1, 60.0).look_at(probe, probe + np.array(d, dtype=np.float64), up)
+            views.append((face, 512, False, None))
+    xz = rng.uniform(-0.45 * size, 0.45 * size, (count, 2)).astype(np.float32)
+    g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
+    k = 0
+    while len(views) < count:
+        yaw = rng.uniform(0, 2 * math.pi)
+        eye = np.array([xz[k, 0], g[k] + rng.uniform(2, 60), xz[k, 1]])
+        target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
+        camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
+        views.append((camera, 720, True, terrain_depth(camera, 640, 360)))
+        k += 1
+    views = views[:count]
+    out = []
+    for slot, (camera, height, occl, depth) in enumerate(views):
+        # Orthographic views use the reference's perspective LOD formula (projScale = P[5] * h / 2);
+        # they keep every instance regardless of its size on screen.
+        min_px = 0.7 if camera.isPerspectiveCamera else 0.0
+        out.append((vg_make_view(camera, height, slot=slot, occlusion=occl, min_pixel_radius=min_px), depth))
+    return out
+
+
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+    import torch
+
+    from vgcut import camera as cam
+    from vgcut import native, scenegen, scenes
+    from vgcut.virtual_geometry import make_view
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.zeros(1, device="cuda:0")
+    t0 = time.time()
+    spec = scenes.world(mesh_count=args.meshes, instances=args.instances)
+    log(f"world: {spec.instance_count} instances, {len(spec.meshes)} meshes, "
+        f"{sum(d.meshletCount for d in spec.meshes)} meshlets, {spec.full_detail_triangles / 1e9:.1f}B full-detail "
+        f"triangles ({time.time() - t0:.1f} s)")
+
+    views_n = args.views
+    ctx = native.Context(max_triangles=MAX_TRIANGLES, max_shadow_triangles=MAX_SHADOW_TRIANGLES, max_meshlets=1 << 20,
+                         max_views=views_n, mesh_slots=64, device=0)
+    scene = native.Scene(ctx)
+    t0 = time.time()
+    scenes.load_into(scene, spec, occlusion=True)
+    vd = mixed_views(spec, views_n, make_view, cam, scenegen)
+    views = [v for v, _ in vd]
+    for v, depth in vd:
+        if depth is not None:
+            scene.set_hzb(v.slot, depth)
+    log(f"scene loaded, {sum(d is not None for _, d in vd)} HZBs ({time.time() - t0:.1f} s)")
+
+    initial = np.zeros(8, dtype=np.uint32)
+    initial[0] = initial[1] = 10000
+
+    def reset_states():
+        for v in views:
+            scene.set_state(v.slot, initial)
+
+    # Warm-up (uploads, graph capture), then the work counts of one batched step.
+    for _ in range(max(1, args.warmup)):
+        scene.cut(views, graph=True)
+    scene.sync()
+    reset_states()
+    scene.cut(views, graph=True)
+    scene.sync()
+    tested = selected = 0
+    per_view = []
+    for v in views:
+        counters, _, _, met = scene.read(v.slot, max_entries=1)
+        k = int(met.clusters_tested[0] + met.clusters_tested[1])
+        s = int(counters[2]) + int(counters[7])
+        tested += k
+        selected += s
+        per_view.append((k, s, met))
+    log(f"one step: {tested} cluster tests, {selected} clusters drawn over {views_n} views")
+
+    if args.profile_only:
+        for _ in range(args.steps):
+            scene.cut(views, graph=False)
+        scene.sync()
+        print(json.dumps({"profile_only": True, "steps": args.steps, "views": views_n}))
+        return
+
+    # ---------------- batched: one launch sequence for all views (graph replay) ----------------
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        scene.cut(views, graph=True)
+    scene.sync()
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t0) / args.steps
+
+    # ---------------- plain: one view per launch sequence ----------------
+    plain_s = None
+    if not args.no_plain:
+        for v in views[:4]:
+            scene.cut([v])
+        scene.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for v in views:
+                scene.cut([v])
+        scene.sync()
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / args.steps
+
+    # ---------------- latency: launch to completion ----------------
+    lat_batched, lat_plain = [], []
+    for i in range(args.latency_samples):
+        t = time.perf_counter()
+        scene.cut(views, graph=True)
+        scene.sync()
+        lat_batched.append((time.perf_counter() - t) * 1e3)
+    if not args.no_plain:
+        for i in range(args.latency_samples):
+            v = views[i % views_n]
+            t = time.perf_counter()
+            scene.cut([v])
+            scene.sync()
+            lat_plain.append((time.perf_counter() - t) * 1e3)
+
+    # ---------------- per-kernel times (HIP events on the library's stream) ----------------
+    scene.profile(True)
+    prof_steps = max(3, args.steps // 2)
+    for _ in range(prof_steps):
+        scene.cut(views, graph=False)
+    scene.sync()
+    kernels, seqs = scene.profile_read()
+    scene.profile(False)
+    kernel_ms = {name: ms / max(1, seqs) for name, ms in kernels}
+    dominant = max(kernel_ms, key=kernel_ms.get)
+
+    # ---------------- algorithmic bytes (unique inputs per launch, outputs per view) ----------------
+    reset_states()
+    inst_unique, meshlet_unique = scene.footprint(views)
+    cells = len(scene.pool(native.VG_POOL_CELLS)) // 8
+    levels = sum(d.levelCount for d in spec.meshes)
+    W = sum(int(m.work_items[0] + m.work_items[1]) for _, _, m in per_view)
+    O = sum(int(m.occlusion_tests) for _, _, m in per_view)
+    slots = scene.mesh_slots
+    hzb_bytes = sum(4 * d.shape[0] * d.shape[1] * (2 + 4 / 3) for _, d in vd if d is not None)
+    per_kernel_bytes = {
+        "k_cells": 32 * cells * views_n,
+        "k_instances": 64 * inst_unique + 32 * levels + 16 * W,
+        "k_meshlets": 64 * meshlet_unique + 16 * W + 32 * selected + 16 * O,
+        "k_prefix(check 1)": 40 * slots * views_n,
+    }
+    launch_bytes = 32 * cells + 64 * inst_unique + 64 * meshlet_unique + 32 * levels + 32 * W + 32 * selected + 16 * O + 40 * slots * views_n
+    dom_bytes = per_kernel_bytes.get(dominant, launch_bytes)
+    dom_ms = kernel_ms[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    # ---------------- CPU baseline: the oracle, single thread, on a bounded sample of the views ----------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+
+        pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, MAX_TRIANGLES, MAX_SHADOW_TRIANGLES)
+        hzbs = {v.slot: oracle.hzb_build(d, 1024) for v, d in vd if d is not None}
+        done_tests = 0
+        used = 0
+        t0 = time.perf_counter()
+        for v in views:
+            r = oracle.cut(pool, v, hzb=hzbs.get(v.slot))
+            done_tests += int(r.stats.clusters_tested[0] + r.stats.clusters_tested[1])
+            used += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": done_tests / cpu_s, "unit": "views*clusters tested/s", "cores": 1, "kind": "port",
+               "sample": f"oracle cut of the first {used} of the {views_n} views (all culling, occlusion, retries), "
+                         f"{done_tests} cluster tests in {cpu_s:.1f} s"}
+        del pool
+
+    if args.check:
+        from oracle import oracle
+
+        pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, MAX_TRIANGLES, MAX_SHADOW_TRIANGLES)
+        reset_states()
+        scene.cut(views, graph=True)
+        scene.sync()
+        bad = 0
+        for v, d in vd:
+            ref = oracle.cut(pool, v, hzb=None if d is None else oracle.hzb_build(d, 1024))
+            counters, args_, draws, _ = scene.read(v.slot)
+            bad += int(not (np.array_equal(counters, ref.counters) and np.array_equal(args_, ref.args)))
+        log(f"check: {bad} of {views_n} views differ from the oracle")
+
+    value = tested / batched_s
+    out = {
+        "metric": "views*clusters tested/s (batched, 64 mixed views, 3M-instance world)",
+        "value": value,
+        "unit": "views*clusters tested/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batched_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"world {spec.instance_count} instances, {len(spec.meshes)} meshes, {views_n} mixed views",
+                   "instances": spec.instance_count, "views": views_n, "error_threshold_px": 1.0},
+        "plain": None if plain_s is None else {
+            "views_clusters_per_s": tested / plain_s, "ms_per_step": plain_s * 1e3, "launches_per_step": views_n,
+            "batched_speedup": plain_s / batched_s,
+            "latency_ms_p50": percentile(lat_plain, 50), "latency_ms_p99": percentile(lat_plain, 99)},
+        "batched": {"latency_ms_p50": percentile(lat_batched, 50), "latency_ms_p99": percentile(lat_batched, 99),
+                    "visible_clusters_per_s": selected / batched_s, "views_per_s": views_n / batched_s},
+        "work_per_step": {"cluster_tests": tested, "clusters_drawn": selected, "work_items": W, "occlusion_tests": O,
+                          "unique_instances": inst_unique, "unique_meshlets": meshlet_unique, "cells": cells},
+        "kernel_ms": kernel_ms,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "launch_algorithmic_bytes": launch_bytes, "kernel_ms": dom_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -49,25 +49,35 @@ def yaw_scale_translate(yaw: np.ndarray, scale: np.ndarray, pos: np.ndarray) -> 
     return m.astype(np.float32)
 
 
-def prop_mesh(rng: np.random.Generator, triangles: int, seed: int, kind: Optional[int] = None):
-    """A procedural prop of about `triangles` leaf triangles: box, column, arch or sphere, a few metres big."""
+def prop_params(rng: np.random.Generator, triangles: int, seed: int, kind: Optional[int] = None) -> dict:
+    """Build parameters of a procedural prop of about `triangles` leaf triangles: box, column, arch or sphere."""
     if kind is None:
         kind = int(rng.integers(0, 4))
     if kind == 0:
         size = (rng.uniform(0.5, 2.5), rng.uniform(0.5, 2.5), rng.uniform(0.5, 2.5))
-        return scenegen.build_mesh(scenegen.BOX, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.5,
-                                   target_triangles=triangles)
+        return dict(kind=scenegen.BOX, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.5, target_triangles=triangles)
     if kind == 1:
         size = (rng.uniform(0.3, 0.9), rng.uniform(3.0, 8.0), 0)
-        return scenegen.build_mesh(scenegen.COLUMN, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.5,
-                                   target_triangles=triangles)
+        return dict(kind=scenegen.COLUMN, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.5, target_triangles=triangles)
     if kind == 2:
         size = (rng.uniform(3.0, 6.0), rng.uniform(2.0, 5.0), rng.uniform(0.5, 1.0))
-        return scenegen.build_mesh(scenegen.ARCH, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.0,
-                                   target_triangles=triangles)
+        return dict(kind=scenegen.ARCH, seed=seed, size=size, noise_amplitude=0.05, noise_frequency=1.0, target_triangles=triangles)
     r = rng.uniform(0.8, 2.5)
-    return scenegen.build_mesh(scenegen.SPHERE, seed=seed, size=(r, 0, 0), noise_amplitude=0.05, noise_frequency=2.0 / r,
-                               target_triangles=triangles)
+    return dict(kind=scenegen.SPHERE, seed=seed, size=(r, 0, 0), noise_amplitude=0.05, noise_frequency=2.0 / r,
+                target_triangles=triangles)
+
+
+def prop_mesh(rng: np.random.Generator, triangles: int, seed: int, kind: Optional[int] = None):
+    return scenegen.build_mesh(**prop_params(rng, triangles, seed, kind))
+
+
+def build_props(rng: np.random.Generator, triangles, seed_base: int, threads: int = 16) -> list:
+    """Builds one prop per entry of `triangles` (parameters drawn in order, DAGs built in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    params = [prop_params(rng, int(t), seed_base + i) for i, t in enumerate(triangles)]
+    with ThreadPoolExecutor(max_workers=max(1, min(threads, len(params)))) as pool:
+        return list(pool.map(lambda p: scenegen.build_mesh(**p), params))
 
 
 def config0() -> Tuple[SceneSpec, cam.Camera, float]:
@@ -86,7 +96,7 @@ def ruins(mesh_count=48, tri_range=(200_000, 5_000_000), instances=113_106, half
     """Props with uniform positions over [-e, e]^2 on y = 0, yaw U[0, 2pi), scale U[0.5, 2]."""
     rng = np.random.default_rng(seed)
     tris = log_uniform(rng, tri_range[0], tri_range[1], mesh_count).astype(np.int64)
-    meshes = [prop_mesh(rng, int(t), seed * 1000 + i) for i, t in enumerate(tris)]
+    meshes = build_props(rng, tris, seed * 1000)
     irng = np.random.default_rng(seed + 1)
     which = irng.integers(0, mesh_count, instances)
     pos = np.zeros((instances, 3))
@@ -123,10 +133,10 @@ def world(mesh_count=64, tri_range=(100_000, 2_000_000), instances=3_000_000, si
     """Props by Bridson Poisson-disk sampling over a size x size noise heightfield (y = terrain height)."""
     rng = np.random.default_rng(seed)
     tris = log_uniform(rng, tri_range[0], tri_range[1], mesh_count).astype(np.int64)
-    meshes = [prop_mesh(rng, int(t), seed * 1000 + i) for i, t in enumerate(tris)]
-    # Disk radius for about 1.15x the wanted count (Bridson packs about 0.7 points per r^2), then an
-    # even random subsample to exactly `instances`.
-    radius = math.sqrt(0.7 * size * size / (instances * 1.15))
+    meshes = build_props(rng, tris, seed * 1000)
+    # Disk radius for about 1.1x the wanted count (Bridson with 12 candidates packs about 0.595 points per
+    # r^2), then an even random subsample to exactly `instances`.
+    radius = math.sqrt(0.595 * size * size / (instances * 1.1))
     xz = scenegen.poisson_disk(-size / 2, -size / 2, size, size, radius, seed + 1, candidates=12,
                                max_points=int(instances * 1.6) + 16)
     prng = np.random.default_rng(seed + 2)
