@@ -91,6 +91,7 @@ typedef struct vg_metrics {
   uint64_t occlusion_tests;
   float error_threshold;        /* threshold the cut used (the budget's pick in budget mode) */
   uint32_t budget_bin;          /* budget mode: chosen bin */
+  uint64_t budget_cluster_tests; /* budget mode: clusters binned by the histogram pass */
 } vg_metrics;
 
 typedef struct vg_ctx vg_ctx;
@@ -178,6 +179,12 @@ int vg_mesh_instance_base(const vg_scene* scene, uint32_t slot, uint32_t* base, 
 int vg_profile(vg_scene* scene, uint32_t enable);
 int vg_profile_read(vg_scene* scene, double* kernel_ms, uint32_t max_kernels, uint64_t* sequences, const char** names);
 int vg_footprint(vg_scene* scene, const vg_view* views, uint32_t count, uint64_t* instances, uint64_t* meshlets);
+
+/*
+ * The triangle budget's threshold table: bins values lo * (hi / lo)^(b / (bins - 1)) rounded to f32 (the
+ * last one exactly hi). A view's histogram bins clusters by these values (times the controller scale).
+ */
+int vg_budget_thresholds(float lo, float hi, uint32_t bins, float* out);
 
 /* Layout helpers: counter words and draw-args words for the context's mesh slot capacity. */
 uint32_t vg_counter_words(const vg_ctx* ctx);

@@ -247,3 +247,22 @@ def budget_histogram(pool: Pool, view, thresholds: np.ndarray, state: Optional[n
     if rc != 0:
         raise RuntimeError("or_budget_histogram failed")
     return out
+
+
+def budget_pick(pool: Pool, view, budget: int, thresholds: np.ndarray, state: Optional[np.ndarray] = None,
+                hzb: Optional[np.ndarray] = None):
+    """The budget's choice for a view: (bin, per-bin triangles, the view at the chosen threshold).
+
+    Bin b holds the triangles of the plain cut at thresholds[b]; the pick is the first bin at or under
+    the budget, else the last. The chosen view's lod_scale_limit is max(1, maxErrorThreshold / t) in f32.
+    """
+    tris = budget_histogram(pool, view, thresholds, state, hzb)
+    fits = np.nonzero(tris <= np.uint64(budget))[0]
+    b = int(fits[0]) if len(fits) else len(thresholds) - 1
+    chosen = type(view)()
+    ctypes.memmove(ctypes.addressof(chosen), ctypes.addressof(view), ctypes.sizeof(view))
+    t = np.float32(thresholds[b])
+    chosen.error_threshold = float(t)
+    chosen.lod_scale_limit = float(max(np.float32(1.0), np.float32(view.max_error_threshold) / t))
+    chosen.triangle_budget = 0
+    return b, tris, chosen

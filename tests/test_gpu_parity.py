@@ -175,3 +175,33 @@ def test_graph_replays_match_eager(gpu_available):
         ref = oracle.cut(loaded.pool, view, state=state)
         state = ref.state
         compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"graph frame {f}")
+
+
+def test_triangle_budget_pick_and_cut(gpu_available):
+    """Budget views pick the oracle's threshold on the GPU and cut exactly as the oracle does at it; a view
+    without a budget in the same launch is unaffected."""
+    spec = small_ruins(seed=61, shadows=True)
+    loaded = load_gpu(spec, max_views=3, max_triangles=20_000_000, max_shadow=10_000_000)
+    bins = loaded.ctx.config.budget_bins
+    t = native.budget_thresholds(0.5, 3.0, bins)
+    views = []
+    for slot, (eye, target, budget) in enumerate([((0.0, 4.0, -60.0), (0.0, 0.0, 40.0), 300_000),
+                                                  ((30.0, 15.0, 20.0), (-40.0, 0.0, -10.0), 50_000),
+                                                  ((-20.0, 3.0, 0.0), (40.0, 0.0, 5.0), 0)]):
+        camera = ground_camera(eye, target)
+        views.append(make_view(camera, 1080, slot=slot, triangle_budget=budget, threshold_range=(0.5, 3.0),
+                               shadow_sweep=(10.0, -30.0, 0.0)))
+    for graph in (False, True):
+        for v in views:
+            loaded.scene.set_state(v.slot, oracle.initial_state())
+        loaded.scene.cut(views, graph=graph)
+        loaded.scene.sync()
+        for v in views:
+            gpu = loaded.scene.read(v.slot)
+            if v.triangle_budget:
+                b, tris, chosen = oracle.budget_pick(loaded.pool, v, v.triangle_budget, t)
+                assert gpu[3].budget_bin == b and gpu[3].error_threshold == np.float32(t[b]), (gpu[3].budget_bin, b)
+                ref = oracle.cut(loaded.pool, chosen)
+            else:
+                ref = oracle.cut(loaded.pool, v)
+            compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"budget slot {v.slot} graph={graph}")

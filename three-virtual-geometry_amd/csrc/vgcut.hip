@@ -81,6 +81,7 @@ struct vg_scene {
   // so back-to-back launches do not wait for each other on the host.
   static constexpr int kRing = 16;
   ViewDev* h_ring[kRing] = {nullptr};
+  float* h_ring_thresholds[kRing] = {nullptr};
   hipEvent_t ring_copied[kRing] = {nullptr};
   bool ring_pending[kRing] = {false};
   uint32_t ring_next = 0;
@@ -91,6 +92,10 @@ struct vg_scene {
   std::vector<std::vector<hipEvent_t>> pending;  // per sequence: kKernelSlots + 1 events
   double kernel_ms[16] = {0};
   uint64_t profiled_sequences = 0;
+  // triangle budget: per view slot of a launch, its threshold table and histogram
+  DevBuf d_budget_thresholds, d_budget_hist;
+  bool launch_budget = false;
+  uint64_t graph_budget = 0;
   // footprint bitmaps (measurement only)
   DevBuf d_fp_instances, d_fp_meshlets, d_fp_counts;
   bool footprint = false;
@@ -101,6 +106,8 @@ struct vg_scene {
   uint64_t graph_version = 0;
   uint64_t device_bytes = 0;
 };
+
+extern "C" int vg_budget_thresholds(float lo, float hi, uint32_t bins, float* out);
 
 namespace {
 
@@ -335,6 +342,9 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.work_counter = static_cast<unsigned long long*>(s->d_work_counter.ptr);
   p.work_capacity = s->work_capacity;
   p.flags = static_cast<uint32_t*>(s->d_flags.ptr);
+  p.budget_thresholds = static_cast<const float*>(s->d_budget_thresholds.ptr);
+  p.budget_hist = static_cast<unsigned long long*>(s->d_budget_hist.ptr);
+  p.budget_bins = ctx->cfg.budget_bins;
   if (s->footprint) {
     p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
     p.fp_meshlets = static_cast<uint32_t*>(s->d_fp_meshlets.ptr);
@@ -343,7 +353,7 @@ Params make_params(vg_scene* s, uint32_t view_count) {
 }
 
 constexpr int kKernelSlots = 11;
-const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells", "k_instances", "k_meshlets", "k_prefix(check 1)",
+const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells (+ budget passes)", "k_instances", "k_meshlets", "k_prefix(check 1)",
                                                 "k_instances(retry 1)", "k_meshlets(retry 1)", "k_prefix(check 2)",
                                                 "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)"};
 
@@ -379,6 +389,11 @@ int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   mark();
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, view_count), dim3(256), 0, st, p);
+  if (s->launch_budget) {
+    hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(k_budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(k_budget_pick, dim3(view_count), dim3(64), 0, st, p);
+  }
   mark();
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
   mark();
@@ -412,7 +427,8 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   for (uint32_t i = 0; i < count; i++) {
     if (views[i].slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_cut: view slot out of range");
     if (seen[views[i].slot]++) return fail(ctx, VG_E_INVALID, "vg_cut: two views share a slot");
-    if (views[i].triangle_budget) return fail(ctx, VG_E_INVALID, "vg_cut: triangle budget is not available in this build");
+    if (views[i].triangle_budget && !(views[i].threshold_range[0] > 0 && views[i].threshold_range[1] >= views[i].threshold_range[0]))
+      return fail(ctx, VG_E_INVALID, "vg_cut: a triangle budget needs 0 < threshold_range[0] <= threshold_range[1]");
     if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0))
       return fail(ctx, VG_E_INVALID, "vg_cut: proj_scale and near_plane must be positive");
   }
@@ -436,10 +452,16 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
     if (int rc = ensure(s, s->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
     for (int r = 0; r < vg_scene::kRing; r++) {
       if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
+      if (s->h_ring_thresholds[r]) (void)hipHostFree(s->h_ring_thresholds[r]);
       s->h_ring[r] = nullptr;
+      s->h_ring_thresholds[r] = nullptr;
       s->ring_pending[r] = false;
       VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_ring[r]), sizeof(ViewDev) * ctx->cfg.max_views, hipHostMallocDefault));
+      VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_ring_thresholds[r]),
+                                sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins, hipHostMallocDefault));
     }
+    if (int rc = ensure(s, s->d_budget_thresholds, sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins)) return rc;
+    if (int rc = ensure(s, s->d_budget_hist, sizeof(unsigned long long) * ctx->cfg.max_views * (ctx->cfg.budget_bins + 1))) return rc;
     s->view_capacity = ctx->cfg.max_views;
   }
   const int ring = static_cast<int>(s->ring_next++ % vg_scene::kRing);
@@ -472,6 +494,16 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
     v.range_hi = in.threshold_range[1];
     v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
   }
+  s->launch_budget = false;
+  for (uint32_t i = 0; i < count; i++)
+    if (views[i].triangle_budget) {
+      s->launch_budget = true;
+      vg_budget_thresholds(views[i].threshold_range[0], views[i].threshold_range[1], ctx->cfg.budget_bins,
+                           s->h_ring_thresholds[ring] + static_cast<size_t>(i) * ctx->cfg.budget_bins);
+    }
+  if (s->launch_budget)
+    VG_HIP(ctx, hipMemcpyAsync(s->d_budget_thresholds.ptr, s->h_ring_thresholds[ring], sizeof(float) * count * ctx->cfg.budget_bins,
+                               hipMemcpyHostToDevice, ctx->stream));
   VG_HIP(ctx, hipMemcpyAsync(s->d_views.ptr, staging, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
   VG_HIP(ctx, hipEventRecord(s->ring_copied[ring], ctx->stream));
   s->ring_pending[ring] = true;
@@ -500,7 +532,7 @@ int vg_create(const vg_config* config, vg_ctx** out) {
   if (!c.mesh_slots) c.mesh_slots = 64;
   if (!c.hzb_max_size) c.hzb_max_size = 1024;
   if (!c.budget_bins) c.budget_bins = 256;
-  if (c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views >= (1u << 14) || c.hzb_max_size > 4096 ||
+  if (c.budget_bins > kMaxBudgetBins || c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views >= (1u << 14) || c.hzb_max_size > 4096 ||
       c.max_triangles > 0x55555555u || c.max_shadow_triangles > 0x55555555u)
     return VG_E_INVALID;
   auto* ctx = new (std::nothrow) vg_ctx();
@@ -593,12 +625,13 @@ void vg_scene_destroy(vg_scene* s) {
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
                     &s->d_work_start, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
-                    &s->d_fp_meshlets, &s->d_fp_counts};
+                    &s->d_fp_meshlets, &s->d_fp_counts, &s->d_budget_thresholds, &s->d_budget_hist};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_hzbs) release(s, b);
   for (int r = 0; r < vg_scene::kRing; r++) {
     if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
+    if (s->h_ring_thresholds[r]) (void)hipHostFree(s->h_ring_thresholds[r]);
     if (s->ring_copied[r]) (void)hipEventDestroy(s->ring_copied[r]);
   }
   for (auto& seq : s->pending)
@@ -794,7 +827,8 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
   if (!s) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
   if (int rc = prepare_cut(s, views, count)) return rc;
-  if (!s->graph_exec || s->graph_views != count || s->graph_version != s->layout_version) {
+  if (!s->graph_exec || s->graph_views != count || s->graph_version != s->layout_version ||
+      s->graph_budget != static_cast<uint64_t>(s->launch_budget)) {
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->graph_exec = nullptr;
@@ -809,6 +843,7 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
     VG_HIP(ctx, hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
     s->graph_views = count;
     s->graph_version = s->layout_version;
+    s->graph_budget = s->launch_budget;
   }
   VG_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
   return VG_OK;
@@ -862,6 +897,12 @@ int vg_read(vg_scene* s, uint32_t view_slot, uint32_t* counters, uint32_t* draw_
     metrics->clusters_tested[0] = m[M_CLUSTERS0];
     metrics->clusters_tested[1] = m[M_CLUSTERS1];
     metrics->occlusion_tests = m[M_OCCLUSION];
+    float t;
+    const uint32_t bits = static_cast<uint32_t>(m[M_THRESHOLD]);
+    std::memcpy(&t, &bits, 4);
+    metrics->error_threshold = t;
+    metrics->budget_bin = static_cast<uint32_t>(m[M_BUDGET_BIN]);
+    metrics->budget_cluster_tests = m[M_BUDGET_TESTS];
   }
   return VG_OK;
 }
@@ -951,6 +992,14 @@ int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t c
 }
 
 uint32_t vg_scene_mesh_slots(const vg_scene* s) { return s ? s->mesh_slots : 0; }
+
+int vg_budget_thresholds(float lo, float hi, uint32_t bins, float* out) {
+  if (!out || bins == 0 || !(lo > 0) || !(hi >= lo)) return VG_E_INVALID;
+  for (uint32_t b = 0; b < bins; b++)
+    out[b] = bins == 1 ? lo : static_cast<float>(static_cast<double>(lo) * std::pow(static_cast<double>(hi) / lo, static_cast<double>(b) / (bins - 1)));
+  out[bins - 1] = hi;
+  return VG_OK;
+}
 
 int vg_profile(vg_scene* s, uint32_t enable) {
   if (!s) return VG_E_INVALID;

@@ -28,7 +28,11 @@ constexpr uint64_t kTaskLimit = 1ull << 35;
 constexpr uint32_t kMaxWorkItems = (1u << 28) - 1;
 // Metrics words (u64) per view slot.
 constexpr uint32_t kMetricWords = 16;
-enum : uint32_t { M_CELLS = 0, M_INSTANCES = 1, M_WORK0 = 2, M_WORK1 = 3, M_CLUSTERS0 = 4, M_CLUSTERS1 = 5, M_OCCLUSION = 6 };
+enum : uint32_t {
+  M_CELLS = 0, M_INSTANCES = 1, M_WORK0 = 2, M_WORK1 = 3, M_CLUSTERS0 = 4, M_CLUSTERS1 = 5, M_OCCLUSION = 6,
+  M_BUDGET_TESTS = 7, M_THRESHOLD = 8, M_BUDGET_BIN = 9
+};
+constexpr uint32_t kMaxBudgetBins = 1024;
 
 // Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
 #define VG_INF_HALF ((float)(1e30 * 0.5))
@@ -53,6 +57,7 @@ struct ViewDev {
   float fade_scale;
   uint32_t redo[2];  // retry flags (S_RETRY > 0.5) of the current round
   uint32_t budget_bin;
+  float budget_scale, budget_low, budget_high;  // camera scale and band of the budget histogram
 };
 
 struct Params {

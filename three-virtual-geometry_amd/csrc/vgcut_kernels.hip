@@ -136,10 +136,23 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   uint32_t* c = view_counters(p, v.slot);
   for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
   if (threadIdx.x < kMetricWords) *metric_at(p, v.slot, threadIdx.x) = 0;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    load_bands(v, state_at(p, v.slot));
+    const uint32_t* state = state_at(p, v.slot);
+    load_bands(v, state);
     v.redo[0] = v.redo[1] = 0;
+    v.budget_bin = 0;
+    *reinterpret_cast<float*>(metric_at(p, v.slot, M_THRESHOLD)) = v.error_threshold;
+    if (v.budget) {
+      const float* t = p.budget_thresholds + static_cast<size_t>(blockIdx.x) * p.budget_bins;
+      v.budget_scale = get_state(state, kStateScale);
+      v.budget_low = t[0] * v.budget_scale;
+      v.budget_high = t[p.budget_bins - 1] * v.budget_scale;
+    }
   }
+  if (v.budget)
+    for (uint32_t i = threadIdx.x; i <= p.budget_bins; i += blockDim.x)
+      p.budget_hist[static_cast<size_t>(blockIdx.x) * (p.budget_bins + 1) + i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *p.cell_queue_count = 0;
     *p.work_counter = 0;
@@ -355,7 +368,11 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
 
 // ------------------------------------------------------------------ 3. meshlets (meshletPass, GeometryPool.ts:926-968)
 
-__global__ __launch_bounds__(256) void k_meshlets(Params p) {
+// Calls body(valid, item, w, meshlet) for 64 consecutive tasks at a time over this wave's equal share
+// of the flat task space (all lanes together). The wave finds its first item by a 64-ary search over
+// the items' first tasks, then walks forward with shuffles.
+template <typename Body>
+__device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
   const unsigned long long packed = *p.work_counter;
   const uint32_t items = min(static_cast<uint32_t>(packed >> kTaskBits), p.work_capacity);
   const uint64_t total = packed & kTaskMask;
@@ -407,6 +424,26 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
       valid = false;
       w = make_uint4(0, 0, 0, 0);
     }
+    body(valid, item, w, meshlet);
+
+    // The item holding task k0 + 64.
+    const uint32_t last_pos = __shfl(pos, 63, 64);
+    const uint32_t last_prev = __shfl(prev, 63, 64);
+    const uint32_t after = __shfl(dist, static_cast<int>(last_pos), 64);
+    if (last_pos > 0) {
+      cur_start = k0 + last_prev;
+      cur += last_pos;
+    }
+    if (after == 64) {
+      cur += 1;
+      cur_start = k0 + 64;
+    }
+    cur = uniform(cur);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_meshlets(Params p) {
+  for_each_task(p, [&](bool valid, uint32_t, const uint4& w, uint32_t meshlet) {
     const uint32_t tag = w.w;
     const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
     const bool occ_on = ((tag >> 17) & 1u) != 0;
@@ -444,20 +481,203 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
     const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
     const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
     if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(w.x, meshlet, first_index, packed_tris);
+  });
+}
 
-    // The item holding task k0 + 64.
-    const uint32_t last_pos = __shfl(pos, 63, 64);
-    const uint32_t last_prev = __shfl(prev, 63, 64);
-    const uint32_t after = __shfl(dist, static_cast<int>(last_pos), 64);
-    if (last_pos > 0) {
-      cur_start = k0 + last_prev;
-      cur += last_pos;
+// ------------------------------------------------------------------ triangle budget (projected-error histogram)
+//
+// For a view with a triangle budget, the threshold is chosen before its cut: over the clusters an
+// instance pass at the band [T[0], T[B-1]] (times the camera scale) would test, each visible cluster
+// adds its triangles to the thresholds b where it is in the plain cut, own <= T[b] < parent, i.e. to
+// bins [lower_bound(own), lower_bound(parent)) of the ascending table T. The smallest T[b] whose total
+// fits the budget becomes the view's error threshold (the largest when none fits).
+
+__device__ __forceinline__ uint32_t bin_of(const float* __restrict__ t, uint32_t bins, float scale, float x) {
+  uint32_t lo = 0, hi = bins;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (t[mid] * scale >= x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_budget_instances(Params p) {
+  const uint32_t lane = lane_id();
+  const uint32_t queue_count = *p.cell_queue_count;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t q = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < queue_count; q += waves) {
+    uint2 entry = p.cell_queue[q];
+    const uint32_t vi = uniform(entry.x);
+    const uint32_t cell = uniform(entry.y);
+    const ViewDev& v = p.views[vi];
+    if (!v.budget) continue;
+    const float4 record = p.c_rec[cell];
+    const uint32_t slot = uniform(static_cast<uint32_t>(record.y));
+    const uint32_t first_instance = uniform(static_cast<uint32_t>(record.z));
+    const uint32_t count = uniform(static_cast<uint32_t>(record.w));
+    const float4 levers = p.mesh_table[slot * kMeshStride + 3];
+    const uint32_t level_count = min(static_cast<uint32_t>(levers.z), kMaxLevels);
+    const uint32_t level_base = static_cast<uint32_t>(levers.w);
+    const float4 object_sphere = p.mesh_table[slot * kMeshStride + 0];
+    const float4 lod_bounds = p.mesh_table[slot * kMeshStride + 1];
+    const float4 flags = p.mesh_table[slot * kMeshStride + 5];
+    const bool occlusion_on = flags.z > 0.5f;
+    const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
+    for (uint32_t r = 0; r < kCellSize; r += 64) {
+      const bool valid = r + lane < count;
+      const uint32_t instance = first_instance + r + lane;
+      float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = make_float4(0, 0, 0, 1);
+      if (valid) {
+        c0 = p.i_col0[instance];
+        c1 = p.i_col1[instance];
+        c2 = p.i_col2[instance];
+        c3 = p.i_col3[instance];
+      }
+      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+      const float radius = object_sphere.w * scale;
+      bool keep = valid && (v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
+      const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
+      const float view_distance = len3(vc.x, vc.y, vc.z);
+      keep = keep && !(view_distance > radius && (radius * v.proj_scale) / view_distance < min_pixels);
+      keep = keep && !((view_distance - radius) > levers.x);
+      if (keep && occlusion_on && v.hzb_valid) {
+        const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
+        keep = !occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, radius + margin);
+      }
+      const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
+      const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
+      const float d = len3(lv.x, lv.y, lv.z);
+      const float rr = lod_bounds.w * scale;
+      const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
+      const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
+      LevelResult lr{0xffffffffu, 0u, 0u, 0u};
+      if (__ballot(keep)) lr = select_levels(p, level_base, level_count, v.budget_low, v.budget_high, pmin, pmax);
+      const bool work = keep && lr.last > lr.first;
+      if (!__ballot(work)) continue;
+      const uint32_t len = work ? lr.last - lr.first : 0u;
+      const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
+      const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
+      const uint32_t task_incl = wave_scan(len);
+      const uint32_t tasks = __shfl(task_incl, 63, 64);
+      unsigned long long old = 0;
+      if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
+      const uint32_t old_lo = __shfl(static_cast<uint32_t>(old), 0, 64);
+      const uint32_t old_hi = __shfl(static_cast<uint32_t>(old >> 32), 0, 64);
+      const uint64_t o = (static_cast<uint64_t>(old_hi) << 32) | old_lo;
+      const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
+      const uint64_t task_base = o & kTaskMask;
+      if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
+      if (work && item_base + rank < p.work_capacity) {
+        const uint32_t tag = slot | ((occlusion_on ? 1u : 0u) << 17) | (vi << 18);
+        p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
+        p.work_start[item_base + rank] = task_base + task_incl - len;
+      }
+      wave_count_add(metric_at(p, v.slot, M_BUDGET_TESTS), len, work);
     }
-    if (after == 64) {
-      cur += 1;
-      cur_start = k0 + 64;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
+  // Per-wave LDS histogram of the wave's current view, flushed to the view's global histogram when
+  // the view changes and at the end; lanes of another view add to global memory directly.
+  __shared__ unsigned long long hist[4][kMaxBudgetBins + 1];
+  const uint32_t wib = threadIdx.x / 64, lane = lane_id();
+  const uint32_t bins = p.budget_bins;
+  for (uint32_t i = lane; i <= bins; i += 64) hist[wib][i] = 0;
+  uint32_t current = 0xffffffffu;
+  auto flush = [&]() {
+    if (current != 0xffffffffu) {
+      unsigned long long* g = p.budget_hist + static_cast<size_t>(current) * (bins + 1);
+      for (uint32_t i = lane; i <= bins; i += 64) {
+        const unsigned long long h = hist[wib][i];
+        if (h) atomicAdd(&g[i], h);
+        hist[wib][i] = 0;
+      }
     }
-    cur = uniform(cur);
+  };
+  for_each_task(p, [&](bool valid, uint32_t, const uint4& w, uint32_t meshlet) {
+    const uint32_t vi = w.w >> 18;
+    const bool occ_on = ((w.w >> 17) & 1u) != 0;
+    const ViewDev& v = p.views[vi];
+    uint32_t a = 0, e = 0, tris = 0;
+    if (valid) {
+      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float4 cs = p.m_cull[meshlet];
+      const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+      bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -(cs.w * scale);
+      if (visible && occ_on && v.hzb_valid) {
+        const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
+        const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
+        visible = !occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, cs.w * scale + margin);
+      }
+      if (visible) {
+        const float2 err = p.m_err[meshlet];
+        const float own = projected_error(v, c0, c1, c2, c3, scale, p.m_lod[meshlet], err.x);
+        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.m_parent[meshlet], err.y);
+        const float* t = p.budget_thresholds + static_cast<size_t>(vi) * bins;
+        const float camera_scale = v.budget_scale;
+        a = bin_of(t, bins, camera_scale, own);
+        e = bin_of(t, bins, camera_scale, parent);
+        if (a < e) tris = p.m_tris[meshlet];
+      }
+    }
+    const bool add = tris != 0;
+    // Switch the wave's LDS histogram to the chunk's first contributing view when needed.
+    const uint64_t contrib = __ballot(add);
+    if (contrib) {
+      const uint32_t lead_view = __shfl(vi, __ffsll(static_cast<unsigned long long>(contrib)) - 1, 64);
+      if (__ballot(add && vi == current) == 0ull && lead_view != current) {
+        flush();
+        current = lead_view;
+      }
+    }
+    if (add) {
+      if (vi == current) {
+        atomicAdd(&hist[wib][a], static_cast<unsigned long long>(tris));
+        atomicAdd(&hist[wib][e], static_cast<unsigned long long>(0) - tris);
+      } else {
+        unsigned long long* g = p.budget_hist + static_cast<size_t>(vi) * (bins + 1);
+        atomicAdd(&g[a], static_cast<unsigned long long>(tris));
+        atomicAdd(&g[e], static_cast<unsigned long long>(0) - tris);
+      }
+    }
+  });
+  flush();
+}
+
+// One block per view with a budget: prefix over the bins, pick the threshold, redo the bands.
+__global__ __launch_bounds__(64) void k_budget_pick(Params p) {
+  ViewDev& v = p.views[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p.work_counter = 0;  // the regular instance pass refills the work list
+  if (!v.budget) return;
+  const uint32_t bins = p.budget_bins;
+  const unsigned long long* h = p.budget_hist + static_cast<size_t>(blockIdx.x) * (bins + 1);
+  const float* t = p.budget_thresholds + static_cast<size_t>(blockIdx.x) * bins;
+  const uint32_t lane = lane_id();
+  uint64_t carry = 0;
+  uint32_t pick = bins - 1;
+  bool found = false;
+  for (uint32_t base = 0; base < bins && !found; base += 64) {
+    const uint32_t b = base + lane;
+    const uint64_t d = b < bins ? h[b] : 0ull;
+    const uint64_t tri = carry + wave_scan64(d);  // triangles of the plain cut at T[b]
+    const uint64_t fits = __ballot(b < bins && tri <= v.budget);
+    if (fits) {
+      pick = base + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(fits)) - 1);
+      found = true;
+    }
+    carry = __shfl(tri, 63, 64);
+  }
+  if (threadIdx.x == 0) {
+    v.error_threshold = t[pick];
+    v.lod_scale_limit = fmaxf(1.0f, v.max_error_threshold / t[pick]);
+    v.budget_bin = pick;
+    load_bands(v, state_at(p, v.slot));
+    *reinterpret_cast<float*>(metric_at(p, v.slot, M_THRESHOLD)) = t[pick];
+    *metric_at(p, v.slot, M_BUDGET_BIN) = pick;
   }
 }
 

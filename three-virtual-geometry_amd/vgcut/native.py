@@ -32,7 +32,7 @@ EXPORTS = (
     "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
     "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
-    "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint",
+    "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
 )
 
 
@@ -101,6 +101,7 @@ class Metrics(ctypes.Structure):
         ("occlusion_tests", ctypes.c_uint64),
         ("error_threshold", ctypes.c_float),
         ("budget_bin", ctypes.c_uint32),
+        ("budget_cluster_tests", ctypes.c_uint64),
     ]
 
 
@@ -142,6 +143,7 @@ def load():
         "vg_draw_args_words": ([P], U32),
         "vg_arith_probe": ([P, P, P, U32], ctypes.c_int),
         "vg_profile": ([P, U32], ctypes.c_int),
+        "vg_budget_thresholds": ([ctypes.c_float, ctypes.c_float, U32, P], ctypes.c_int),
         "vg_profile_read": ([P, P, U32, ctypes.POINTER(U64), P], ctypes.c_int),
         "vg_footprint": ([P, P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64)], ctypes.c_int),
     }
@@ -338,6 +340,15 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def budget_thresholds(lo: float, hi: float, bins: int) -> np.ndarray:
+    """The budget histogram's threshold table (vg_budget_thresholds)."""
+    out = np.empty(bins, dtype=np.float32)
+    rc = load().vg_budget_thresholds(lo, hi, bins, out.ctypes.data)
+    if rc != VG_OK:
+        raise VgError(rc, "vg_budget_thresholds: need bins > 0 and 0 < lo <= hi")
+    return out
 
 
 def arith_probe(a: np.ndarray, b: np.ndarray) -> np.ndarray:
