@@ -503,12 +503,13 @@ static void meshlet_pass(ctx_t* c, int cut) {
       f4 center = mat_mul(m, point(cull));
       int visible = v->frustum_culling == 0 ||
                     frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, 0) >= -(cull[3] * scale);
-      if (cut == 0) visible = visible && !occluded_sphere(c, occlusion_on, center.x, center.y, center.z, cull[3] * scale);
       float own = projected_error(v, m, scale, b, b[8]);
       float parent = projected_error(v, m, scale, b + 4, b[9]);
       uint32_t fade_low = fade > 0.0f ? fade_step(own, low, fade) : 0u;
       uint32_t fade_high = fade > 0.0f ? fade_step(parent, low, fade) : LOD_FADE_STEPS;
       int lod = own <= high && parent > low && fade_high > fade_low;
+      /* drawn = visible && lod; the occlusion test (and its count) only matters for clusters in the cut */
+      if (cut == 0 && lod && visible) visible = !occluded_sphere(c, occlusion_on, center.x, center.y, center.z, cull[3] * scale);
       if (c->fp64) check64(c, cut, m, meshlet, lod);
       if (!(visible && lod)) continue;
       uint32_t at = (*counter(c, cut, C_SELECTED))++;

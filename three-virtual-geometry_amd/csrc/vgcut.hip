@@ -72,7 +72,7 @@ struct vg_scene {
   std::vector<float*> h_hzb_ptrs;
   uint32_t counter_stride = 0, args_stride = 0, hzb_floats = 0;
   // launch scratch
-  DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_counter, d_flags;
+  DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_scale, d_work_counter, d_flags;
   DevBuf d_hzb_stage;
   uint32_t view_capacity = 0;
   size_t cell_queue_capacity = 0;
@@ -339,6 +339,7 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.cell_queue_count = static_cast<uint32_t*>(s->d_cell_queue_count.ptr);
   p.work = static_cast<uint4*>(s->d_work.ptr);
   p.work_start = static_cast<unsigned long long*>(s->d_work_start.ptr);
+  p.work_scale = static_cast<float*>(s->d_work_scale.ptr);
   p.work_counter = static_cast<unsigned long long*>(s->d_work_counter.ptr);
   p.work_capacity = s->work_capacity;
   p.flags = static_cast<uint32_t*>(s->d_flags.ptr);
@@ -445,6 +446,7 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   if (s->work_capacity < want) {
     if (int rc = ensure(s, s->d_work, sizeof(uint4) * want)) return rc;
     if (int rc = ensure(s, s->d_work_start, sizeof(unsigned long long) * want)) return rc;
+    if (int rc = ensure(s, s->d_work_scale, sizeof(float) * want)) return rc;
     s->work_capacity = static_cast<uint32_t>(want);
   }
   if (s->view_capacity < count) {
@@ -624,7 +626,7 @@ void vg_scene_destroy(vg_scene* s) {
   DevBuf* bufs[] = {&s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
-                    &s->d_work_start, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
+                    &s->d_work_start, &s->d_work_scale, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
                     &s->d_fp_meshlets, &s->d_fp_counts, &s->d_budget_thresholds, &s->d_budget_hist};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);

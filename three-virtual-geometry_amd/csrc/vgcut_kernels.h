@@ -94,6 +94,7 @@ struct Params {
   uint32_t* cell_queue_count;
   uint4* work;           // (instance, first, last, tag)
   unsigned long long* work_start;  // first task of each item
+  float* work_scale;     // instance scale of each item (max column length, computed once)
   unsigned long long* work_counter;
   uint32_t work_capacity;
   uint32_t* flags;       // [0] any retry this round, [1] overflow

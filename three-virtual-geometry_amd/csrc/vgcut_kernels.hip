@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
             const uint32_t tag = slot | (cut << 16) | ((cut == 0 && occlusion_on ? 1u : 0u) << 17) | (vi << 18);
             p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
             p.work_start[item_base + rank] = task_base + task_incl - len;
+            p.work_scale[item_base + rank] = scale;
           }
           wave_atomic_add_uniform(counter_at(p, v.slot, cut, kWork), 1u, work);
           wave_count_add(metric_at(p, v.slot, M_WORK0 + cut), 1, work);
@@ -442,23 +443,60 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
   }
 }
 
+// The meshlet pass in two phases per wave. Phase 1 (every cluster test): the LOD decision, own <= high,
+// parent > low and the blend range. Clusters in the cut are queued in LDS; phase 2 runs on full waves of
+// queued clusters: frustum and occlusion tests (drawn = visible and in the cut, so the order does not
+// change any decision), then compaction into the draw list. The frustum and occlusion work is thus paid
+// per cluster in the cut, not per cluster tested, and no lane idles in phase 2.
 __global__ __launch_bounds__(256) void k_meshlets(Params p) {
-  for_each_task(p, [&](bool valid, uint32_t, const uint4& w, uint32_t meshlet) {
-    const uint32_t tag = w.w;
+  __shared__ uint4 queue[4][128];
+  const uint32_t wib = threadIdx.x / 64, lane = lane_id();
+  uint32_t queued = 0;  // wave-uniform
+
+  auto drain = [&](uint32_t n) {
+    const bool active = lane < n;
+    const uint4 c = active ? queue[wib][lane] : make_uint4(0, 0, 0, 0);
+    const uint4 w = active ? p.work[c.x] : make_uint4(0, 0, 0, 0);
+    const uint32_t meshlet = c.y, tag = w.w;
     const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
     const bool occ_on = ((tag >> 17) & 1u) != 0;
     const ViewDev& v = p.views[vi];
-
     bool drawn = false, occ_test = false;
-    uint32_t packed_tris = 0, tris = 0;
-    if (valid) {
+    uint32_t tris = 0;
+    if (active) {
       const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float scale = p.work_scale[c.x];
       const float4 cs = p.m_cull[meshlet];
       const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
       bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
       occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
       if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
+      drawn = visible;
+      if (drawn) tris = p.m_tris[meshlet];
+    }
+    const uint32_t packed_tris = tris | (c.z << 8);
+    const uint32_t vslot = v.slot;
+    wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
+    const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
+    const bool write = drawn && at < p.max_meshlets;
+    const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
+    const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
+    if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(w.x, meshlet, first_index, packed_tris);
+  };
+  auto wave_sync = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t meshlet) {
+    const uint32_t cut = (w.w >> 16) & 1u, vi = w.w >> 18;
+    const ViewDev& v = p.views[vi];
+    bool lod = false;
+    uint32_t fades = 0;
+    if (valid) {
+      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+      const float scale = p.work_scale[item];
       const float2 err = p.m_err[meshlet];
       const float own = projected_error(v, c0, c1, c2, c3, scale, p.m_lod[meshlet], err.x);
       const float parent = projected_error(v, c0, c1, c2, c3, scale, p.m_parent[meshlet], err.y);
@@ -468,20 +506,25 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
         fade_low = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(own, 1e-30f) / low) * fade), 0.0f, 255.0f));
         fade_high = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(parent, 1e-30f) / low) * fade), 0.0f, 255.0f));
       }
-      drawn = visible && own <= high && parent > low && fade_high > fade_low;
-      if (drawn) {
-        tris = p.m_tris[meshlet];
-        packed_tris = tris | (fade_low << 8) | (fade_high << 16);
-      }
+      lod = own <= high && parent > low && fade_high > fade_low;
+      fades = fade_low | (fade_high << 8);
     }
-    const uint32_t vslot = v.slot;
-    wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
-    uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
-    const bool write = drawn && at < p.max_meshlets;
-    const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
-    const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
-    if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(w.x, meshlet, first_index, packed_tris);
+    const uint64_t mask = __ballot(lod);
+    if (lod) queue[wib][queued + static_cast<uint32_t>(__popcll(mask & lanes_below()))] = make_uint4(item, meshlet, fades, 0u);
+    queued += static_cast<uint32_t>(__popcll(mask));
+    if (queued >= 64) {
+      wave_sync();
+      drain(64);
+      const uint4 rest = lane + 64 < queued ? queue[wib][lane + 64] : make_uint4(0, 0, 0, 0);
+      wave_sync();
+      if (lane + 64 < queued) queue[wib][lane] = rest;
+      queued -= 64;
+    }
   });
+  if (queued) {
+    wave_sync();
+    drain(queued);
+  }
 }
 
 // ------------------------------------------------------------------ triangle budget (projected-error histogram)
@@ -573,6 +616,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
         const uint32_t tag = slot | ((occlusion_on ? 1u : 0u) << 17) | (vi << 18);
         p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
         p.work_start[item_base + rank] = task_base + task_incl - len;
+        p.work_scale[item_base + rank] = scale;
       }
       wave_count_add(metric_at(p, v.slot, M_BUDGET_TESTS), len, work);
     }
