@@ -121,7 +121,7 @@ def percentile(a, q):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported (see DESIGN.md, sharding)"}))
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported: the cut of one scene runs on one GPU"}))
         sys.exit(2)
     import torch
 
@@ -176,10 +176,18 @@ def main():
     log(f"one step: {tested} cluster tests, {selected} clusters drawn over {views_n} views")
 
     if args.profile_only:
+        # A plain 1 GiB device copy first: its FETCH_SIZE / WRITE_SIZE calibrate the counters' units.
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
         for _ in range(args.steps):
             scene.cut(views, graph=False)
         scene.sync()
-        print(json.dumps({"profile_only": True, "steps": args.steps, "views": views_n}))
+        met = [scene.read(v.slot, max_entries=1)[3] for v in views]
+        print(json.dumps({"profile_only": True, "steps": args.steps, "views": views_n, "copy_bytes": 2 * src.numel() * 4,
+                          "instances_tested": sum(int(m.instances_tested) for m in met),
+                          "cluster_tests": tested, "clusters_drawn": selected}))
         return
 
     # ---------------- batched: one launch sequence for all views (graph replay) ----------------

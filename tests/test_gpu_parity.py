@@ -47,13 +47,13 @@ def _run(loaded, views, hzbs=None, fp64=False):
     return results
 
 
-def test_config0_icosphere_single_view(gpu_available):
-    spec, camera, height = scenes.config0()
+def test_icosphere_single_view(gpu_available):
+    spec, camera, height = scenes.icosphere_view()
     loaded = load_gpu(spec, max_meshlets=1 << 20, max_triangles=10_000_000, max_shadow=4_000_000)
     view = make_view(camera, height)
     gpu, = _run(loaded, [view])
     ref = oracle.cut(loaded.pool, view, fp64=True)
-    compare_cut(gpu, ref, 1 << 20, "config0")
+    compare_cut(gpu, ref, 1 << 20, "icosphere")
     assert ref.stats.mismatch_hard == 0 and ref.stats.near_threshold == 0
     n = int(ref.counters[2])
     assert n > 500 and tiling_ok(gpu[2][0], n, gpu[0], 0, 64, loaded.slot_of_instance)

@@ -57,6 +57,9 @@ struct vg_scene {
   std::vector<float2> m_err;
   std::vector<uint32_t> m_tris;
   std::vector<uint32_t> m_info;     // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
+  std::vector<uint32_t> m_run, r_first;  // LOD runs (see kMaxRun)
+  std::vector<float4> r_lod, r_parent;
+  std::vector<float2> r_err;
   uint64_t index_used = 0;          // indices of the meshes added so far
   std::vector<float4> i_col[4];
   std::vector<float> i_slot;        // mesh slot of each instance
@@ -64,6 +67,7 @@ struct vg_scene {
   bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
   // device scene
   DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
+  DevBuf d_run, d_rfirst, d_rlod, d_rparent, d_rerr;
   // per view slot
   DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs;
   std::vector<DevBuf> d_draws, d_hzbs;
@@ -264,7 +268,9 @@ int upload_scene(vg_scene* s) {
   int rc = 0;
   if (s->dirty_meshlets) {
     if ((rc = upload(s, s->d_cull, s->m_cull)) || (rc = upload(s, s->d_lod, s->m_lod)) || (rc = upload(s, s->d_parent, s->m_parent)) ||
-        (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)))
+        (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)) || (rc = upload(s, s->d_run, s->m_run)) ||
+        (rc = upload(s, s->d_rfirst, s->r_first)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
+        (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)))
       return rc;
     s->dirty_meshlets = false;
   }
@@ -314,6 +320,11 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.m_parent = static_cast<const float4*>(s->d_parent.ptr);
   p.m_err = static_cast<const float2*>(s->d_err.ptr);
   p.m_tris = static_cast<const uint32_t*>(s->d_tris.ptr);
+  p.m_run = static_cast<const uint32_t*>(s->d_run.ptr);
+  p.r_first = static_cast<const uint32_t*>(s->d_rfirst.ptr);
+  p.r_lod = static_cast<const float4*>(s->d_rlod.ptr);
+  p.r_parent = static_cast<const float4*>(s->d_rparent.ptr);
+  p.r_err = static_cast<const float2*>(s->d_rerr.ptr);
   p.i_col0 = static_cast<const float4*>(s->d_col[0].ptr);
   p.i_col1 = static_cast<const float4*>(s->d_col[1].ptr);
   p.i_col2 = static_cast<const float4*>(s->d_col[2].ptr);
@@ -623,7 +634,7 @@ void vg_scene_destroy(vg_scene* s) {
   if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
-  DevBuf* bufs[] = {&s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
+  DevBuf* bufs[] = {&s->d_run, &s->d_rfirst, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
                     &s->d_work_start, &s->d_work_scale, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
@@ -680,6 +691,41 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
     s->m_info.push_back(static_cast<uint32_t>(info[1] + s->index_used));
     s->m_info.push_back(info[2]);
     s->m_info.push_back(info[3]);
+  }
+  // LOD runs: consecutive meshlets of one level with the same LOD-test inputs (bit-equal spheres and
+  // errors; any sphere with a zero error or an error >= 5e29, which project to 0 and 1e30).
+  {
+    auto bits = [](float f) {
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      return u;
+    };
+    const float inf_half = static_cast<float>(1e30 * 0.5);
+    auto key_equal = [&](size_t a, size_t b, int which) {
+      const float ea = which ? s->m_err[a].y : s->m_err[a].x, eb = which ? s->m_err[b].y : s->m_err[b].x;
+      if (bits(ea) != bits(eb)) return false;
+      if (ea == 0.0f || ea >= inf_half) return true;
+      const float4 sa = which ? s->m_parent[a] : s->m_lod[a], sb = which ? s->m_parent[b] : s->m_lod[b];
+      return bits(sa.x) == bits(sb.x) && bits(sa.y) == bits(sb.y) && bits(sa.z) == bits(sb.z) && bits(sa.w) == bits(sb.w);
+    };
+    std::vector<uint8_t> level_start(n, 0);
+    for (uint32_t l = 0; l < L; l++) {
+      const uint32_t start = mesh->level_ranges[l * 2], end = start + mesh->level_ranges[l * 2 + 1];
+      if (start < end) level_start[start] = 1;
+      if (start < end && end < n) level_start[end] = 1;
+    }
+    for (uint32_t i = 0; i < n;) {
+      const size_t a = m.meshlet_base + i;
+      uint32_t len = 1;
+      while (i + len < n && len < kMaxRun && !level_start[i + len] && key_equal(a, a + len, 0) && key_equal(a, a + len, 1)) len++;
+      const uint32_t run = static_cast<uint32_t>(s->r_first.size());
+      s->r_first.push_back(static_cast<uint32_t>(a) | (len << kRunCountShift));
+      s->r_lod.push_back(s->m_lod[a]);
+      s->r_parent.push_back(s->m_parent[a]);
+      s->r_err.push_back(s->m_err[a]);
+      for (uint32_t k = 0; k < len; k++) s->m_run.push_back(run);
+      i += len;
+    }
   }
   uint64_t mesh_indices = 0;
   for (uint32_t i = 0; i < n; i++) mesh_indices += static_cast<uint64_t>(mesh->meshlet_info[static_cast<size_t>(i) * 4]) * 3;

@@ -33,6 +33,13 @@ enum : uint32_t {
   M_BUDGET_TESTS = 7, M_THRESHOLD = 8, M_BUDGET_BIN = 9
 };
 constexpr uint32_t kMaxBudgetBins = 1024;
+// A LOD run: consecutive meshlets of one level whose LOD test has the same inputs (same LOD and parent
+// spheres and errors; a zero error projects to 0 and an error >= 5e29 to 1e30 whatever the sphere). The
+// meshlet pass tests each run once. Runs hold at most kMaxRun meshlets.
+constexpr uint32_t kMaxRun = 4;
+constexpr uint32_t kRunCountShift = 24;
+// Per-wave LDS ring of meshlets in the cut awaiting their visibility test (power of two, >= 63 + 64 * kMaxRun).
+constexpr uint32_t kQueue = 512;
 
 // Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
 #define VG_INF_HALF ((float)(1e30 * 0.5))
@@ -68,6 +75,11 @@ struct Params {
   const float4* __restrict__ m_parent;    // parent LOD sphere
   const float2* __restrict__ m_err;       // (error, parentError)
   const uint32_t* __restrict__ m_tris;    // triangle count
+  const uint32_t* __restrict__ m_run;     // LOD run of each meshlet
+  const uint32_t* __restrict__ r_first;   // per run: first meshlet | count << 24
+  const float4* __restrict__ r_lod;       // per run: LOD sphere
+  const float4* __restrict__ r_parent;    // parent LOD sphere
+  const float2* __restrict__ r_err;       // (error, parentError)
   const float4* __restrict__ i_col0;      // instance matrix columns, SoA
   const float4* __restrict__ i_col1;
   const float4* __restrict__ i_col2;
@@ -171,6 +183,15 @@ __host__ __device__ inline float vg_logf(float x) {
 // ------------------------------------------------------------------ wave64 helpers
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Rank of the calling wave with consecutive ranks on one XCD: workgroups are dispatched to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8. Giving each XCD a contiguous eighth of the work keeps
+// the data a wave touches (one mesh's clusters, neighbouring instances) in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_wave_rank() {
+  uint32_t b = blockIdx.x;
+  if ((gridDim.x & 7u) == 0) b = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  return b * (blockDim.x / 64) + threadIdx.x / 64;
+}
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
 template <typename T>

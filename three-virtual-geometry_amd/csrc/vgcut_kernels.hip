@@ -234,8 +234,9 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
   if (retry_round && !p.flags[retry_round - 1]) return;
   const uint32_t lane = lane_id();
   const uint32_t queue_count = *p.cell_queue_count;
-  const uint32_t waves = gridDim.x * (blockDim.x / 64);
-  for (uint32_t q = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < queue_count; q += waves) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), rank = xcd_wave_rank();
+  const uint32_t q_end = static_cast<uint32_t>(queue_count * (rank + 1) / waves);
+  for (uint32_t q = static_cast<uint32_t>(queue_count * rank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
     const uint32_t vi = uniform(entry.x);
     const uint32_t cell = uniform(entry.y);
@@ -339,7 +340,10 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
         if (p.fp_meshlets && survived) mark_range(p.fp_meshlets, lr.first, lr.last);
         const bool work = survived && !fast;
         if (__ballot(work)) {
-          const uint32_t len = work ? lr.last - lr.first : 0u;
+          // Tasks are the LOD runs covering [first, last) (level ranges hold whole runs).
+          const uint32_t run_first = work ? p.m_run[lr.first] : 0u;
+          const uint32_t run_last = work ? p.m_run[lr.last - 1] + 1 : 0u;
+          const uint32_t len = run_last - run_first;
           const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
           const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
           const uint32_t task_incl = wave_scan(len);
@@ -354,13 +358,13 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
           if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
           if (work && item_base + rank < p.work_capacity) {
             const uint32_t tag = slot | (cut << 16) | ((cut == 0 && occlusion_on ? 1u : 0u) << 17) | (vi << 18);
-            p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
+            p.work[item_base + rank] = make_uint4(instance, run_first, run_last, tag);
             p.work_start[item_base + rank] = task_base + task_incl - len;
             p.work_scale[item_base + rank] = scale;
           }
           wave_atomic_add_uniform(counter_at(p, v.slot, cut, kWork), 1u, work);
           wave_count_add(metric_at(p, v.slot, M_WORK0 + cut), 1, work);
-          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), len, work);
+          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), lr.last - lr.first, work);
         }
       }
     }
@@ -380,7 +384,7 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
   if (total == 0 || items == 0 || p.flags[3]) return;
   const uint32_t lane = lane_id();
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
-  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+  const uint64_t wave = xcd_wave_rank();
   const uint64_t seg_begin = total * wave / waves;
   const uint64_t seg_end = total * (wave + 1) / waves;
   if (seg_begin >= seg_end) return;
@@ -443,19 +447,19 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
   }
 }
 
-// The meshlet pass in two phases per wave. Phase 1 (every cluster test): the LOD decision, own <= high,
-// parent > low and the blend range. Clusters in the cut are queued in LDS; phase 2 runs on full waves of
-// queued clusters: frustum and occlusion tests (drawn = visible and in the cut, so the order does not
-// change any decision), then compaction into the draw list. The frustum and occlusion work is thus paid
-// per cluster in the cut, not per cluster tested, and no lane idles in phase 2.
+// The meshlet pass in two phases per wave. Phase 1 runs once per LOD run (consecutive meshlets whose
+// LOD test has identical inputs): own <= high, parent > low and the blend range. The meshlets of runs in
+// the cut go to a per-wave LDS ring; phase 2 runs on full waves of them: frustum and occlusion tests
+// (drawn = visible and in the cut, so the order changes no decision), then compaction into the draw
+// list. The per-cluster-test cost is one LOD evaluation per run; visibility is paid per meshlet in the cut.
 __global__ __launch_bounds__(256) void k_meshlets(Params p) {
-  __shared__ uint4 queue[4][128];
+  __shared__ uint4 ring[4][kQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
-  uint32_t queued = 0;  // wave-uniform
+  uint32_t head = 0, tail = 0;  // wave-uniform ring positions
 
   auto drain = [&](uint32_t n) {
     const bool active = lane < n;
-    const uint4 c = active ? queue[wib][lane] : make_uint4(0, 0, 0, 0);
+    const uint4 c = active ? ring[wib][(head + lane) & (kQueue - 1)] : make_uint4(0, 0, 0, 0);
     const uint4 w = active ? p.work[c.x] : make_uint4(0, 0, 0, 0);
     const uint32_t meshlet = c.y, tag = w.w;
     const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
@@ -474,6 +478,7 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
       drawn = visible;
       if (drawn) tris = p.m_tris[meshlet];
     }
+    head += n;
     const uint32_t packed_tris = tris | (c.z << 8);
     const uint32_t vslot = v.slot;
     wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
@@ -489,17 +494,17 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t meshlet) {
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t run) {
     const uint32_t cut = (w.w >> 16) & 1u, vi = w.w >> 18;
     const ViewDev& v = p.views[vi];
     bool lod = false;
-    uint32_t fades = 0;
+    uint32_t fades = 0, first = 0, count = 0;
     if (valid) {
       const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
       const float scale = p.work_scale[item];
-      const float2 err = p.m_err[meshlet];
-      const float own = projected_error(v, c0, c1, c2, c3, scale, p.m_lod[meshlet], err.x);
-      const float parent = projected_error(v, c0, c1, c2, c3, scale, p.m_parent[meshlet], err.y);
+      const float2 err = p.r_err[run];
+      const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
+      const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
       const float low = v.low[cut], high = v.high[cut], fade = v.fade_scale;
       uint32_t fade_low = 0, fade_high = kFadeSteps;
       if (fade > 0.0f) {
@@ -508,22 +513,29 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
       }
       lod = own <= high && parent > low && fade_high > fade_low;
       fades = fade_low | (fade_high << 8);
+      if (lod) {
+        const uint32_t fc = p.r_first[run];
+        first = fc & ((1u << kRunCountShift) - 1u);
+        count = fc >> kRunCountShift;
+      }
     }
-    const uint64_t mask = __ballot(lod);
-    if (lod) queue[wib][queued + static_cast<uint32_t>(__popcll(mask & lanes_below()))] = make_uint4(item, meshlet, fades, 0u);
-    queued += static_cast<uint32_t>(__popcll(mask));
-    if (queued >= 64) {
-      wave_sync();
-      drain(64);
-      const uint4 rest = lane + 64 < queued ? queue[wib][lane + 64] : make_uint4(0, 0, 0, 0);
-      wave_sync();
-      if (lane + 64 < queued) queue[wib][lane] = rest;
-      queued -= 64;
+    // Append the runs' meshlets to the ring (before the append fewer than 64 are pending).
+    const uint32_t incl = wave_scan(count);
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total) {
+      const uint32_t at = tail + incl - count;
+      for (uint32_t k = 0; k < count; k++) ring[wib][(at + k) & (kQueue - 1)] = make_uint4(item, first + k, fades, 0u);
+      tail += total;
+      while (tail - head >= 64) {
+        wave_sync();
+        drain(64);
+        wave_sync();
+      }
     }
   });
-  if (queued) {
+  if (tail != head) {
     wave_sync();
-    drain(queued);
+    drain(tail - head);
   }
 }
 
@@ -548,8 +560,9 @@ __device__ __forceinline__ uint32_t bin_of(const float* __restrict__ t, uint32_t
 __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   const uint32_t lane = lane_id();
   const uint32_t queue_count = *p.cell_queue_count;
-  const uint32_t waves = gridDim.x * (blockDim.x / 64);
-  for (uint32_t q = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < queue_count; q += waves) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), wrank = xcd_wave_rank();
+  const uint32_t q_end = static_cast<uint32_t>(queue_count * (wrank + 1) / waves);
+  for (uint32_t q = static_cast<uint32_t>(queue_count * wrank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
     const uint32_t vi = uniform(entry.x);
     const uint32_t cell = uniform(entry.y);

@@ -1,6 +1,6 @@
 """Seeded synthetic scenes of the cut's benchmark configurations (inputs, not part of the cut).
 
-config0  one icosphere of 1,003,520 triangles, one identity instance, one 1080p camera at 3R
+icosphere_view  one icosphere of 1,003,520 triangles, one identity instance, one 1080p camera at 3R
 ruins    procedural props (boxes, columns, arches, spheres) scattered uniformly over a square
 world    props placed by Bridson Poisson-disk sampling over a noise heightfield, with per-mesh
          draw distances
@@ -80,7 +80,7 @@ def build_props(rng: np.random.Generator, triangles, seed_base: int, threads: in
         return list(pool.map(lambda p: scenegen.build_mesh(**p), params))
 
 
-def config0() -> Tuple[SceneSpec, cam.Camera, float]:
+def icosphere_view() -> Tuple[SceneSpec, cam.Camera, float]:
     """Plain single view: icosphere R = 1 of 1,003,520 triangles, camera at (0, 0, 3R), 1920 x 1080, 60 degrees."""
     data = scenegen.icosphere(triangles=1_003_520, radius=1.0, seed=1, amplitude=0.05)
     spec = SceneSpec([data], [np.eye(4, dtype=np.float32).reshape(1, 16)], [1e30], [0.0], [False])
