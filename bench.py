@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="a few batched launches for rocprofv3")
     ap.add_argument("--check", action="store_true", help="compare the batched cut of every view with the oracle")
     ap.add_argument("--max-meshlets", type=int, default=1 << 22, help="draw-list entries per cut per view")
+    ap.add_argument("--schedule", type=str, default=None,
+                    help="fixed launch schedule 'instance_blocks,meshlet_blocks,meshlet_waves' (per CU) instead of tuning")
     return ap.parse_args()
 
 
@@ -157,6 +159,17 @@ def main():
         for v in views:
             scene.set_state(v.slot, initial)
 
+    # Launch schedules: tuned on the batched views and, for the plain path, on the heaviest single view.
+    if args.schedule:
+        fixed = tuple(int(x) for x in args.schedule.split(","))
+        scene.set_schedule(*fixed)
+        batched_schedule = plain_schedule = fixed
+    else:
+        t0 = time.time()
+        plain_schedule = scene.tune([views[0]], repeats=5)[0].as_tuple()
+        batched_schedule = scene.tune(views, repeats=5)[0].as_tuple()
+        log(f"tuned schedules: batched {batched_schedule}, plain {plain_schedule} ({time.time() - t0:.1f} s)")
+
     # Warm-up (uploads, graph capture), then the work counts of one batched step.
     for _ in range(max(1, args.warmup)):
         scene.cut(views, graph=True)
@@ -186,6 +199,7 @@ def main():
         scene.sync()
         met = [scene.read(v.slot, max_entries=1)[3] for v in views]
         print(json.dumps({"profile_only": True, "steps": args.steps, "views": views_n, "copy_bytes": 2 * src.numel() * 4,
+                          "schedule": batched_schedule,
                           "instances_tested": sum(int(m.instances_tested) for m in met),
                           "cluster_tests": tested, "clusters_drawn": selected}))
         return
@@ -202,6 +216,7 @@ def main():
     # ---------------- plain: one view per launch sequence ----------------
     plain_s = None
     if not args.no_plain:
+        scene.set_schedule(*plain_schedule)
         for v in views[:4]:
             scene.cut([v])
         scene.sync()
@@ -216,18 +231,21 @@ def main():
 
     # ---------------- latency: launch to completion ----------------
     lat_batched, lat_plain = [], []
+    scene.set_schedule(*batched_schedule)
     for i in range(args.latency_samples):
         t = time.perf_counter()
         scene.cut(views, graph=True)
         scene.sync()
         lat_batched.append((time.perf_counter() - t) * 1e3)
     if not args.no_plain:
+        scene.set_schedule(*plain_schedule)
         for i in range(args.latency_samples):
             v = views[i % views_n]
             t = time.perf_counter()
             scene.cut([v])
             scene.sync()
             lat_plain.append((time.perf_counter() - t) * 1e3)
+    scene.set_schedule(*batched_schedule)
 
     # ---------------- per-kernel times (HIP events on the library's stream) ----------------
     scene.profile(True)
@@ -327,9 +345,12 @@ def main():
         "plain": None if plain_s is None else {
             "views_clusters_per_s": tested / plain_s, "ms_per_step": plain_s * 1e3, "launches_per_step": views_n,
             "batched_speedup": plain_s / batched_s,
-            "latency_ms_p50": percentile(lat_plain, 50), "latency_ms_p99": percentile(lat_plain, 99)},
+            "latency_ms_p50": percentile(lat_plain, 50), "latency_ms_p99": percentile(lat_plain, 99),
+            "schedule": dict(zip(("instance_blocks_per_cu", "meshlet_blocks_per_cu", "meshlet_waves_per_simd"), plain_schedule))},
         "batched": {"latency_ms_p50": percentile(lat_batched, 50), "latency_ms_p99": percentile(lat_batched, 99),
-                    "visible_clusters_per_s": selected / batched_s, "views_per_s": views_n / batched_s},
+                    "visible_clusters_per_s": selected / batched_s, "views_per_s": views_n / batched_s,
+                    "schedule": dict(zip(("instance_blocks_per_cu", "meshlet_blocks_per_cu", "meshlet_waves_per_simd"),
+                                         batched_schedule))},
         "work_per_step": {"cluster_tests": tested, "clusters_drawn": selected, "work_items": W, "occlusion_tests": O,
                           "unique_instances": inst_unique, "unique_meshlets": meshlet_unique, "cells": cells},
         "kernel_ms": kernel_ms,

@@ -181,6 +181,24 @@ int vg_profile_read(vg_scene* scene, double* kernel_ms, uint32_t max_kernels, ui
 int vg_footprint(vg_scene* scene, const vg_view* views, uint32_t count, uint64_t* instances, uint64_t* meshlets);
 
 /*
+ * Launch schedule of the persistent passes. The batched form has no counterpart in the reference
+ * (which dispatches one workgroup per work item, GeometryPool.ts:926-968); results do not depend on the
+ * schedule, only the time does. instance_blocks_per_cu: instance-pass workgroups (256 threads) per CU,
+ * 1-16. meshlet_blocks_per_cu: meshlet-pass workgroups per CU, 1-32. meshlet_waves_per_simd: the
+ * occupancy the meshlet pass is compiled for (VG_MESHLET_VARIANTS: 4, 5 or 6; higher = fewer registers).
+ * vg_tune times candidate schedules on `count` views (`repeats` eager cuts each, the median kept, the
+ * controller state restored after each), sets the fastest on the scene and returns it with its time.
+ */
+typedef struct vg_schedule {
+  uint32_t instance_blocks_per_cu;
+  uint32_t meshlet_blocks_per_cu;
+  uint32_t meshlet_waves_per_simd;
+} vg_schedule;
+int vg_schedule_get(const vg_scene* scene, vg_schedule* schedule);
+int vg_schedule_set(vg_scene* scene, const vg_schedule* schedule);
+int vg_tune(vg_scene* scene, const vg_view* views, uint32_t count, uint32_t repeats, vg_schedule* best, float* best_ms);
+
+/*
  * The triangle budget's threshold table: bins values lo * (hi / lo)^(b / (bins - 1)) rounded to f32 (the
  * last one exactly hi). A view's histogram bins clusters by these values (times the controller scale).
  */

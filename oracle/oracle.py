@@ -62,6 +62,7 @@ class OrStats(ctypes.Structure):
         ("clusters_tested", ctypes.c_uint64 * 2),
         ("occlusion_tests", ctypes.c_uint64),
         ("retries", ctypes.c_uint64 * 2),
+        ("draw_overflow", ctypes.c_uint64 * 2),
         ("lod_checked", ctypes.c_uint64),
         ("near_threshold", ctypes.c_uint64),
         ("mismatch_near", ctypes.c_uint64),

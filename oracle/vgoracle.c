@@ -555,6 +555,7 @@ static void prefix_pass(ctx_t* c, int cut) {
 static float fill_of(ctx_t* c, int cut) {
   float requested = (float)*counter(c, cut, C_REQUESTED);
   float selected = (float)*counter(c, cut, C_SELECTED);
+  if (*counter(c, cut, C_SELECTED) > c->sc->max_meshlets) c->st->draw_overflow[cut]++;
   return fmaxf(requested / (float)c->sc->cut_capacity[cut], selected / (float)c->sc->max_meshlets);
 }
 

@@ -57,6 +57,10 @@ typedef struct or_stats {
   uint64_t clusters_tested[2];   /* meshlet-pass range lengths + fast-path meshlets */
   uint64_t occlusion_tests;
   uint64_t retries[2];
+  /* rounds (checks and the steer) whose cut overflowed the draw list: the index counts then depend on the
+     order of the appends (GeometryPool.ts:957-962 counts only appends that got a slot), and with them the
+     fill and the controller state */
+  uint64_t draw_overflow[2];
   /* fp64 check of the LOD decision (cut.ts:26-48 form) on every tested cluster */
   uint64_t lod_checked;
   uint64_t near_threshold;       /* within 1e-6 relative of a threshold in fp64 */

@@ -35,8 +35,13 @@ def draw_set(draws_cut: np.ndarray, n: int):
 
 
 def compare_cut(gpu, ref: oracle.CutResult, max_meshlets: int, label: str = ""):
+    """Bit-exact comparison with the oracle. The controller words are skipped when a round of the cut
+    overflowed the draw list: its index counts then depend on the append order (in the reference too)."""
     counters, args, draws, metrics = gpu
-    assert np.array_equal(counters, ref.counters), f"{label}: counters differ\n{np.nonzero(counters != ref.counters)}"
+    words = len(counters) - 8 if state_order_dependent(ref) else len(counters)
+    diff = np.nonzero(counters[:words] != ref.counters[:words])[0]
+    assert len(diff) == 0, (f"{label}: counters differ at {diff[:16].tolist()}: gpu {counters[diff[:16]].tolist()} "
+                            f"oracle {ref.counters[diff[:16]].tolist()}")
     assert np.array_equal(args, ref.args), f"{label}: draw args differ"
     for cut in (0, 1):
         selected = int(ref.counters[2 + cut * 5])
@@ -50,6 +55,10 @@ def compare_cut(gpu, ref: oracle.CutResult, max_meshlets: int, label: str = ""):
     assert list(metrics.work_items) == list(st.work_items), f"{label}: work items"
     assert list(metrics.clusters_tested) == list(st.clusters_tested), f"{label}: clusters tested"
     assert metrics.occlusion_tests == st.occlusion_tests, f"{label}: occlusion tests"
+
+
+def state_order_dependent(ref: oracle.CutResult) -> bool:
+    return int(ref.stats.draw_overflow[0]) + int(ref.stats.draw_overflow[1]) > 0
 
 
 def tiling_ok(draws_cut: np.ndarray, n: int, counters: np.ndarray, cut: int, slots: int, slot_of_instance) -> bool:

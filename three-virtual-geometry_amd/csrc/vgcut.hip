@@ -57,7 +57,8 @@ struct vg_scene {
   std::vector<float2> m_err;
   std::vector<uint32_t> m_tris;
   std::vector<uint32_t> m_info;     // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
-  std::vector<uint32_t> m_run, r_first;  // LOD runs (see kMaxRun)
+  std::vector<uint32_t> m_run;  // LOD runs (see kMaxRun)
+  std::vector<uint2> r_span;
   std::vector<float4> r_lod, r_parent;
   std::vector<float2> r_err;
   uint64_t index_used = 0;          // indices of the meshes added so far
@@ -67,7 +68,7 @@ struct vg_scene {
   bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
   // device scene
   DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
-  DevBuf d_run, d_rfirst, d_rlod, d_rparent, d_rerr;
+  DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
   // per view slot
   DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs;
   std::vector<DevBuf> d_draws, d_hzbs;
@@ -103,6 +104,9 @@ struct vg_scene {
   // footprint bitmaps (measurement only)
   DevBuf d_fp_instances, d_fp_meshlets, d_fp_counts;
   bool footprint = false;
+  // launch schedule of the persistent passes
+  vg_schedule schedule{6, 10, 5};
+  uint64_t graph_schedule = 0;
   // captured launch sequence
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -269,7 +273,7 @@ int upload_scene(vg_scene* s) {
   if (s->dirty_meshlets) {
     if ((rc = upload(s, s->d_cull, s->m_cull)) || (rc = upload(s, s->d_lod, s->m_lod)) || (rc = upload(s, s->d_parent, s->m_parent)) ||
         (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)) || (rc = upload(s, s->d_run, s->m_run)) ||
-        (rc = upload(s, s->d_rfirst, s->r_first)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
+        (rc = upload(s, s->d_rspan, s->r_span)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
         (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)))
       return rc;
     s->dirty_meshlets = false;
@@ -321,7 +325,7 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.m_err = static_cast<const float2*>(s->d_err.ptr);
   p.m_tris = static_cast<const uint32_t*>(s->d_tris.ptr);
   p.m_run = static_cast<const uint32_t*>(s->d_run.ptr);
-  p.r_first = static_cast<const uint32_t*>(s->d_rfirst.ptr);
+  p.r_span = static_cast<const uint2*>(s->d_rspan.ptr);
   p.r_lod = static_cast<const float4*>(s->d_rlod.ptr);
   p.r_parent = static_cast<const float4*>(s->d_rparent.ptr);
   p.r_err = static_cast<const float2*>(s->d_rerr.ptr);
@@ -382,12 +386,32 @@ hipEvent_t take_event(vg_scene* s) {
 
 // The launch sequence of one cut (GeometryPool.ts:1081-1092): begin, cells, instances, meshlets,
 // prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer).
+using MeshletKernel = void (*)(Params);
+MeshletKernel meshlet_kernel(uint32_t waves_per_simd) {
+  switch (waves_per_simd) {
+    case 4: return k_meshlets<4>;
+    case 5: return k_meshlets<5>;
+    case 6: return k_meshlets<6>;
+    default: return nullptr;
+  }
+}
+
+bool schedule_valid(const vg_schedule& sc) {
+  return sc.instance_blocks_per_cu >= 1 && sc.instance_blocks_per_cu <= 16 && sc.meshlet_blocks_per_cu >= 1 &&
+         sc.meshlet_blocks_per_cu <= 32 && meshlet_kernel(sc.meshlet_waves_per_simd) != nullptr;
+}
+
+uint64_t schedule_key(const vg_schedule& sc) {
+  return (static_cast<uint64_t>(sc.instance_blocks_per_cu) << 32) | (sc.meshlet_blocks_per_cu << 8) | sc.meshlet_waves_per_simd;
+}
+
 int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   vg_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   Params p = make_params(s, view_count);
-  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * 4;
-  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * 8;
+  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
+  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
+  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
   std::vector<hipEvent_t> ev;
   const bool timed = s->profile && !capturing;
   if (timed)
@@ -434,7 +458,7 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   vg_ctx* ctx = s->ctx;
   if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_cut: host-only context (device < 0)");
   if (!views || count == 0) return fail(ctx, VG_E_INVALID, "vg_cut: no views");
-  if (count > ctx->cfg.max_views || count >= (1u << 14)) return fail(ctx, VG_E_INVALID, "vg_cut: more views than max_views");
+  if (count > ctx->cfg.max_views || count > kMaxLaunchViews) return fail(ctx, VG_E_INVALID, "vg_cut: more views than max_views");
   std::vector<uint8_t> seen(ctx->cfg.max_views, 0);
   for (uint32_t i = 0; i < count; i++) {
     if (views[i].slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_cut: view slot out of range");
@@ -545,7 +569,7 @@ int vg_create(const vg_config* config, vg_ctx** out) {
   if (!c.mesh_slots) c.mesh_slots = 64;
   if (!c.hzb_max_size) c.hzb_max_size = 1024;
   if (!c.budget_bins) c.budget_bins = 256;
-  if (c.budget_bins > kMaxBudgetBins || c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views >= (1u << 14) || c.hzb_max_size > 4096 ||
+  if (c.budget_bins > kMaxBudgetBins || c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views > kMaxLaunchViews || c.hzb_max_size > 4096 ||
       c.max_triangles > 0x55555555u || c.max_shadow_triangles > 0x55555555u)
     return VG_E_INVALID;
   auto* ctx = new (std::nothrow) vg_ctx();
@@ -634,7 +658,7 @@ void vg_scene_destroy(vg_scene* s) {
   if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
-  DevBuf* bufs[] = {&s->d_run, &s->d_rfirst, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
+  DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
                     &s->d_work_start, &s->d_work_scale, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
@@ -718,8 +742,8 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
       const size_t a = m.meshlet_base + i;
       uint32_t len = 1;
       while (i + len < n && len < kMaxRun && !level_start[i + len] && key_equal(a, a + len, 0) && key_equal(a, a + len, 1)) len++;
-      const uint32_t run = static_cast<uint32_t>(s->r_first.size());
-      s->r_first.push_back(static_cast<uint32_t>(a) | (len << kRunCountShift));
+      const uint32_t run = static_cast<uint32_t>(s->r_span.size());
+      s->r_span.push_back(make_uint2(static_cast<uint32_t>(a), len));
       s->r_lod.push_back(s->m_lod[a]);
       s->r_parent.push_back(s->m_parent[a]);
       s->r_err.push_back(s->m_err[a]);
@@ -876,7 +900,7 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
   vg_ctx* ctx = s->ctx;
   if (int rc = prepare_cut(s, views, count)) return rc;
   if (!s->graph_exec || s->graph_views != count || s->graph_version != s->layout_version ||
-      s->graph_budget != static_cast<uint64_t>(s->launch_budget)) {
+      s->graph_budget != static_cast<uint64_t>(s->launch_budget) || s->graph_schedule != schedule_key(s->schedule)) {
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->graph_exec = nullptr;
@@ -892,6 +916,7 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
     s->graph_views = count;
     s->graph_version = s->layout_version;
     s->graph_budget = s->launch_budget;
+    s->graph_schedule = schedule_key(s->schedule);
   }
   VG_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
   return VG_OK;
@@ -1116,6 +1141,107 @@ int vg_mesh_instance_base(const vg_scene* s, uint32_t slot, uint32_t* base, uint
   return VG_OK;
 }
 
+int vg_schedule_get(const vg_scene* s, vg_schedule* schedule) {
+  if (!s || !schedule) return VG_E_INVALID;
+  *schedule = s->schedule;
+  return VG_OK;
+}
+
+int vg_schedule_set(vg_scene* s, const vg_schedule* schedule) {
+  if (!s || !schedule) return VG_E_INVALID;
+  if (!schedule_valid(*schedule))
+    return fail(s->ctx, VG_E_INVALID, "vg_schedule_set: blocks per CU out of range or no meshlet-pass variant for that occupancy");
+  s->schedule = *schedule;
+  return VG_OK;
+}
+
+// Times candidate schedules by coordinate descent: the meshlet-pass variant and grid first (instance
+// grid fixed), then the instance grid. Each candidate runs one untimed cut, then `repeats` cuts timed
+// one by one with events on the context's stream; the median counts. The slots' counters (controller
+// state included) are restored before every candidate and at the end.
+int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats, vg_schedule* best, float* best_ms) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (repeats == 0 || repeats > 64) return fail(ctx, VG_E_INVALID, "vg_tune: repeats must be 1-64");
+  if (int rc = prepare_cut(s, views, count)) return rc;
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint8_t> saved(s->d_counters.bytes);
+  VG_HIP(ctx, hipMemcpy(saved.data(), s->d_counters.ptr, saved.size(), hipMemcpyDeviceToHost));
+  auto restore = [&]() -> int {
+    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    VG_HIP(ctx, hipMemcpy(s->d_counters.ptr, saved.data(), saved.size(), hipMemcpyHostToDevice));
+    return VG_OK;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VG_HIP(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(ctx, VG_E_DEVICE, "vg_tune: hipEventCreate");
+  }
+  const bool profile = s->profile;
+  s->profile = false;
+  const vg_schedule start = s->schedule;
+  int rc = VG_OK;
+  auto time_it = [&](const vg_schedule& sc, float* ms) -> int {
+    if (int r = restore()) return r;
+    s->schedule = sc;
+    std::vector<float> t;
+    for (uint32_t i = 0; i <= repeats; i++) {
+      if (int r = prepare_cut(s, views, count)) return r;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail(ctx, VG_E_DEVICE, "vg_tune: hipEventRecord");
+      if (int r = enqueue_cut(s, count)) return r;
+      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        return fail(ctx, VG_E_DEVICE, "vg_tune: event");
+      float e = 0;
+      if (hipEventElapsedTime(&e, e0, e1) != hipSuccess) return fail(ctx, VG_E_DEVICE, "vg_tune: hipEventElapsedTime");
+      if (i) t.push_back(e);  // the first cut warms up
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return VG_OK;
+  };
+  vg_schedule chosen = start;
+  float chosen_ms = 0;
+  rc = time_it(chosen, &chosen_ms);
+  static const uint32_t kMeshlet[][2] = {{4, 4}, {4, 8}, {5, 5}, {5, 10}, {6, 6}, {6, 12}};
+  for (const auto& m : kMeshlet) {
+    if (rc) break;
+    vg_schedule c = chosen;
+    c.meshlet_waves_per_simd = m[0];
+    c.meshlet_blocks_per_cu = m[1];
+    if (schedule_key(c) == schedule_key(chosen)) continue;
+    float ms = 0;
+    rc = time_it(c, &ms);
+    if (!rc && ms < chosen_ms) {
+      chosen = c;
+      chosen_ms = ms;
+    }
+  }
+  static const uint32_t kInstance[] = {4, 6, 8, 12};
+  for (uint32_t b : kInstance) {
+    if (rc) break;
+    vg_schedule c = chosen;
+    c.instance_blocks_per_cu = b;
+    if (schedule_key(c) == schedule_key(chosen)) continue;
+    float ms = 0;
+    rc = time_it(c, &ms);
+    if (!rc && ms < chosen_ms) {
+      chosen = c;
+      chosen_ms = ms;
+    }
+  }
+  const int restored = restore();
+  s->profile = profile;
+  s->schedule = rc ? start : chosen;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (restored) return restored;
+  if (best) *best = chosen;
+  if (best_ms) *best_ms = chosen_ms;
+  return VG_OK;
+}
+
 // Test hook: evaluates the arithmetic the decisions depend on for n input pairs (see k_arith_probe).
 int vg_arith_probe(const float* a, const float* b, float* out, uint32_t n) {
   if (!a || !b || !out || n == 0) return VG_E_INVALID;
@@ -1141,6 +1267,27 @@ int vg_arith_probe(const float* a, const float* b, float* out, uint32_t n) {
   }
   (void)hipFree(da);
   (void)hipFree(db);
+  (void)hipFree(dout);
+  return rc;
+}
+
+// Test hook: wave-scan probe over `groups` groups of 64 values (see k_scan_probe); out holds 128 per group.
+int vg_scan_probe(const uint32_t* in, uint32_t* out, uint32_t groups) {
+  if (!in || !out || groups == 0) return VG_E_INVALID;
+  uint32_t *din = nullptr, *dout = nullptr;
+  if (hipMalloc(&din, sizeof(uint32_t) * 64 * groups) != hipSuccess) return VG_E_NOMEM;
+  if (hipMalloc(&dout, sizeof(uint32_t) * 128 * groups) != hipSuccess) {
+    (void)hipFree(din);
+    return VG_E_NOMEM;
+  }
+  int rc = VG_OK;
+  if (hipMemcpy(din, in, sizeof(uint32_t) * 64 * groups, hipMemcpyHostToDevice) != hipSuccess) rc = VG_E_DEVICE;
+  if (rc == VG_OK) {
+    hipLaunchKernelGGL(k_scan_probe, dim3(groups), dim3(64), 0, 0, din, dout, groups);
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(out, dout, sizeof(uint32_t) * 128 * groups, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = VG_E_DEVICE;
+  }
+  (void)hipFree(din);
   (void)hipFree(dout);
   return rc;
 }

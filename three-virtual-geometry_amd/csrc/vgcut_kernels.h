@@ -37,9 +37,16 @@ constexpr uint32_t kMaxBudgetBins = 1024;
 // spheres and errors; a zero error projects to 0 and an error >= 5e29 to 1e30 whatever the sphere). The
 // meshlet pass tests each run once. Runs hold at most kMaxRun meshlets.
 constexpr uint32_t kMaxRun = 4;
-constexpr uint32_t kRunCountShift = 24;
-// Per-wave LDS ring of meshlets in the cut awaiting their visibility test (power of two, >= 63 + 64 * kMaxRun).
-constexpr uint32_t kQueue = 512;
+// Work-item tag: mesh slot in bits 0-15, then flags, then the launched view's index.
+constexpr uint32_t kTagSlotMask = 0xffffu;
+constexpr uint32_t kTagShadow = 1u << 16;     // the item belongs to the shadow cut
+constexpr uint32_t kTagOcclusion = 1u << 17;  // the mesh opted into occlusion (camera cut only)
+constexpr uint32_t kTagFast = 1u << 18;       // fast path: tasks are meshlets drawn without cluster tests
+constexpr int kTagViewShift = 19;
+constexpr uint32_t kMaxLaunchViews = 1u << (32 - kTagViewShift);
+// Per-wave LDS ring of meshlets in the cut awaiting their visibility test: at most 63 wait before a
+// wave appends up to 64 * kMaxRun, so 64 * (kMaxRun + 1) entries suffice.
+constexpr uint32_t kQueue = 64 * (kMaxRun + 1);
 
 // Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
 #define VG_INF_HALF ((float)(1e30 * 0.5))
@@ -76,7 +83,7 @@ struct Params {
   const float2* __restrict__ m_err;       // (error, parentError)
   const uint32_t* __restrict__ m_tris;    // triangle count
   const uint32_t* __restrict__ m_run;     // LOD run of each meshlet
-  const uint32_t* __restrict__ r_first;   // per run: first meshlet | count << 24
+  const uint2* __restrict__ r_span;       // per run: (first meshlet, meshlet count)
   const float4* __restrict__ r_lod;       // per run: LOD sphere
   const float4* __restrict__ r_parent;    // parent LOD sphere
   const float2* __restrict__ r_err;       // (error, parentError)
@@ -133,7 +140,8 @@ __device__ __forceinline__ float4 mat_mul(const float4& c0, const float4& c1, co
   return r;
 }
 
-__device__ __forceinline__ float4 mat_mul16(const float* m, float x, float y, float z, float w) {
+template <typename P>
+__device__ __forceinline__ float4 mat_mul16(P m, float x, float y, float z, float w) {
   float4 r;
   r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
   r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
@@ -145,7 +153,8 @@ __device__ __forceinline__ float4 mat_mul16(const float* m, float x, float y, fl
 // sqrt and / are correctly rounded: HIP's default -fhip-fp32-correctly-rounded-divide-sqrt lowers
 // llvm.sqrt.f32 and fdiv to IEEE-exact sequences on gfx950 (checked against the host by a GPU test).
 __device__ __forceinline__ float len3(float x, float y, float z) { return __builtin_sqrtf((x * x + y * y) + z * z); }
-__device__ __forceinline__ float dot3(const float* n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
+template <typename P>
+__device__ __forceinline__ float dot3(P n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
 // fdlibm e_logf.c (Sun Microsystems, 1993): x = 2^k (1 + f), log(1 + f) = f - s (f - R(z)),
@@ -180,6 +189,27 @@ __host__ __device__ inline float vg_logf(float x) {
   return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
 }
 
+// ------------------------------------------------------------------ scalar (wave-uniform) reads
+
+// Read-only data at wave-uniform addresses goes through the scalar cache: a pointer in the constant
+// address space makes such loads s_load into SGPRs instead of per-lane vector loads. The data may be
+// written by an earlier kernel of the launch sequence (each dispatch starts with that cache invalidated),
+// never by the kernel that reads it this way.
+#define VG_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const VG_CONST T* scalar(const T* p) {
+  return (const VG_CONST T*)(p);
+}
+// Vector-type records (HIP's vector types cannot be copied out of the constant address space whole).
+__device__ __forceinline__ float4 scalar_f4(const float4* p, size_t i) {
+  const VG_CONST float* f = (const VG_CONST float*)(p + i);
+  return make_float4(f[0], f[1], f[2], f[3]);
+}
+__device__ __forceinline__ uint2 scalar_u2(const uint2* p, size_t i) {
+  const VG_CONST uint32_t* u = (const VG_CONST uint32_t*)(p + i);
+  return make_uint2(u[0], u[1]);
+}
+
 // ------------------------------------------------------------------ wave64 helpers
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -199,14 +229,25 @@ __device__ __forceinline__ T uniform(T v) {
   return static_cast<T>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
 
-// Inclusive prefix sum over the 64 lanes.
+// Inclusive prefix sum over the 64 lanes with DPP moves: Hillis-Steele within each row of 16 lanes
+// (row_shr 1, 2, 4, 8; lanes shifted in from outside the row read 0), then row_bcast:15 adds row 0's
+// total to row 1 and row 2's to row 3, and row_bcast:31 adds the total of rows 0-1 to rows 2-3.
+template <int kCtrl, int kRowMask, bool kBoundZero>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, kRowMask, 0xf, kBoundZero));
+}
 __device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t t = __shfl_up(v, d, 64);
-    if (static_cast<int>(lane_id()) >= d) v += t;
-  }
+  v += dpp<0x111, 0xf, true>(v);   // row_shr:1
+  v += dpp<0x112, 0xf, true>(v);   // row_shr:2
+  v += dpp<0x114, 0xf, true>(v);   // row_shr:4
+  v += dpp<0x118, 0xf, true>(v);   // row_shr:8
+  v += dpp<0x142, 0xa, false>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp<0x143, 0xc, false>(v);  // row_bcast:31 into rows 2 and 3
   return v;
+}
+// The sum over the 64 lanes (wave-uniform).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_scan(v)), 63));
 }
 
 __device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
@@ -249,10 +290,10 @@ __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t val
 __device__ __forceinline__ uint32_t wave_atomic_add_uniform(uint32_t* addr, uint32_t value, bool active) {
   uint32_t v = active ? value : 0u;
   uint32_t incl = wave_scan(v);
-  uint32_t total = __shfl(incl, 63, 64);
+  uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
   uint32_t base = 0;
   if (lane_id() == 0 && total) base = atomicAdd(addr, total);
-  base = __shfl(base, 0, 64);
+  base = uniform(base);
   return base + incl - v;
 }
 
@@ -275,10 +316,9 @@ __device__ __forceinline__ void wave_count_add_keyed(unsigned long long* addr, u
   }
 }
 
-__device__ __forceinline__ void wave_count_add(unsigned long long* addr, uint64_t value, bool active) {
-  uint64_t v = active ? value : 0ull;
-  uint64_t incl = wave_scan64(v);
-  uint64_t total = __shfl(incl, 63, 64);
+// 64-bit counter add of a 32-bit per-lane value: one atomic per wave.
+__device__ __forceinline__ void wave_count_add(unsigned long long* addr, uint32_t value, bool active) {
+  const uint32_t total = wave_sum(active ? value : 0u);
   if (lane_id() == 0 && total) atomicAdd(addr, static_cast<unsigned long long>(total));
 }
 

@@ -61,11 +61,12 @@ __device__ void load_bands(ViewDev& v, const uint32_t* state) {
 }
 
 // frustumDistance, GeometryPool.ts:690-698. mode 0: plain, 1: swept (x 1), 2: swept x select(flag, 1, 0).
-__device__ __forceinline__ float frustum_distance(const ViewDev& v, float cx, float cy, float cz, int mode, bool flag) {
+template <typename V>
+__device__ __forceinline__ float frustum_distance(const V& v, float cx, float cy, float cz, int mode, bool flag) {
   float best = 1e30f;
 #pragma unroll
   for (int p = 0; p < 6; p++) {
-    const float* n = &v.planes[p * 4];
+    const auto n = &v.planes[p * 4];
     float d = dot3(n, cx, cy, cz) + n[3];
     if (mode == 1) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * 1.0f;
     if (mode == 2) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * (flag ? 1.0f : 0.0f);
@@ -75,7 +76,8 @@ __device__ __forceinline__ float frustum_distance(const ViewDev& v, float cx, fl
 }
 
 // projectedError, GeometryPool.ts:678-683.
-__device__ __forceinline__ float projected_error(const ViewDev& v, const float4& c0, const float4& c1, const float4& c2,
+template <typename V>
+__device__ __forceinline__ float projected_error(const V& v, const float4& c0, const float4& c1, const float4& c2,
                                                  const float4& c3, float scale, const float4& s, float error) {
   float4 q = mat_mul(c0, c1, c2, c3, s.x, s.y, s.z, 1.0f);
   float4 vc = mat_mul16(v.view, q.x, q.y, q.z, q.w);
@@ -87,12 +89,15 @@ __device__ __forceinline__ float projected_error(const ViewDev& v, const float4&
 __device__ __forceinline__ uint32_t ceil_log2(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
 // occluded, OcclusionCulling.ts:119-170. `hzb` is only read when the sphere is in front of the near plane.
-__device__ bool occluded(const ViewDev& v, const float* __restrict__ hzb, float vx, float vy, float vz, float radius) {
+// The HZB header (sizes, level table) is read as wave-uniform data, the texels per lane.
+template <typename V>
+__device__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, float vy, float vz, float radius) {
+  const auto header = scalar(hzb);
   float depth = -vz;
   float near_depth = depth - radius;
   float far_depth = depth + radius;
   if (near_depth <= v.near_plane) return false;
-  const float* P = v.proj;
+  const auto P = v.proj;
   float x_lo = vx - radius, x_hi = vx + radius, y_lo = vy - radius, y_hi = vy + radius;
   float sx = P[0], sy = P[5];
   float min_x = fminf(x_lo / near_depth, x_lo / far_depth) * sx - P[8];
@@ -100,17 +105,17 @@ __device__ bool occluded(const ViewDev& v, const float* __restrict__ hzb, float 
   float min_y = fminf(y_lo / near_depth, y_lo / far_depth) * sy - P[9];
   float max_y = fmaxf(y_hi / near_depth, y_hi / far_depth) * sy - P[9];
   float min_z = P[14] / near_depth - P[10];
-  float W = hzb[1], H = hzb[2];
+  float W = header[1], H = header[2];
   float x0 = clampf(floorf((min_x * 0.5f + 0.5f) * W), 0.0f, W - 1.0f);
   float x1 = clampf(floorf((max_x * 0.5f + 0.5f) * W), 0.0f, W - 1.0f);
   float y0 = clampf(floorf((0.5f - max_y * 0.5f) * H), 0.0f, H - 1.0f);
   float y1 = clampf(floorf((0.5f - min_y * 0.5f) * H), 0.0f, H - 1.0f);
   uint32_t extent = static_cast<uint32_t>(fmaxf(x1 - x0, y1 - y0)) + 1u;
   uint32_t level = ceil_log2(extent);
-  uint32_t top = static_cast<uint32_t>(hzb[kHzbLevelCountSlot]) - 1u;
+  uint32_t top = static_cast<uint32_t>(header[kHzbLevelCountSlot]) - 1u;
   level = level > top ? top : level;
   float scale = static_cast<float>(1u << level);
-  float offset = hzb[level * 4], lw = hzb[level * 4 + 1], lh = hzb[level * 4 + 2];
+  float offset = hzb[level * 4], lw = hzb[level * 4 + 1], lh = hzb[level * 4 + 2];  // level differs per lane
   auto sample = [&](float fx, float fy) {
     float tx = clampf(floorf(fx / scale), 0.0f, lw - 1.0f);
     float ty = clampf(floorf(fy / scale), 0.0f, lh - 1.0f);
@@ -121,12 +126,13 @@ __device__ bool occluded(const ViewDev& v, const float* __restrict__ hzb, float 
 }
 
 // occludedSphere, GeometryPool.ts:701-705.
-__device__ __forceinline__ bool occluded_sphere(const Params& p, const ViewDev& v, bool on, float cx, float cy, float cz,
+template <typename V>
+__device__ __forceinline__ bool occluded_sphere(const Params& p, const V& v, bool on, float cx, float cy, float cz,
                                                 float radius) {
   if (!(on && v.hzb_valid)) return false;
   float4 vc = mat_mul16(v.view, cx, cy, cz, 1.0f);
   float margin = (fabsf(vc.z) * v.high[0]) / v.proj_scale;
-  return occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, radius + margin);
+  return occluded(v, scalar(p.hzbs)[v.slot], vc.x, vc.y, vc.z, radius + margin);
 }
 
 // ------------------------------------------------------------------ 0. begin (resetNode, GeometryPool.ts:714-725)
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
 
 __global__ __launch_bounds__(256) void k_cells(Params p) {
   const uint32_t vi = blockIdx.y;
-  const ViewDev& v = p.views[vi];
+  const auto& v = scalar(p.views)[vi];
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   bool keep = false;
   if (cell < p.cell_count) {
@@ -200,16 +206,17 @@ struct LevelResult {
   uint32_t first, last, possible, guaranteed;
 };
 
-__device__ __forceinline__ LevelResult select_levels(const Params& p, uint32_t level_base, uint32_t level_count, float low,
-                                                     float high, float pmin, float pmax) {
+// selectLevels (GeometryPool.ts:771-800) for one band [low, high].
+__device__ __forceinline__ LevelResult select_band(const Params& p, uint32_t level_base, uint32_t level_count, float low, float high,
+                                                   float pmin, float pmax) {
   LevelResult r{0xffffffffu, 0u, 0u, 0u};
   for (uint32_t i = 0; i < level_count; i++) {
-    float4 l0 = p.mesh_table[level_base + i * 2];
-    float4 l1 = p.mesh_table[level_base + i * 2 + 1];
-    uint32_t level_first = static_cast<uint32_t>(l0.x);
-    uint32_t size = static_cast<uint32_t>(l0.y);
-    bool possible = (l0.z * pmax <= high) && (l0.w >= VG_INF_HALF || l0.w * pmin > low);
-    bool sure = (l1.x * pmin <= low * VG_MARGIN_LO) && (l1.y >= VG_INF_HALF || l1.y * pmax > high * VG_MARGIN_HI);
+    const float4 l0 = p.mesh_table[level_base + i * 2];
+    const float4 l1 = p.mesh_table[level_base + i * 2 + 1];
+    const uint32_t level_first = static_cast<uint32_t>(l0.x);
+    const uint32_t size = static_cast<uint32_t>(l0.y);
+    const bool possible = (l0.z * pmax <= high) && (l0.w >= VG_INF_HALF || l0.w * pmin > low);
+    const bool sure = (l1.x * pmin <= low * VG_MARGIN_LO) && (l1.y >= VG_INF_HALF || l1.y * pmax > high * VG_MARGIN_HI);
     if (possible) {
       r.first = min(r.first, level_first);
       r.last = max(r.last, level_first + size);
@@ -218,6 +225,44 @@ __device__ __forceinline__ LevelResult select_levels(const Params& p, uint32_t l
     }
   }
   return r;
+}
+
+// selectLevels for the camera and the shadow cut in one pass over the level table (the level records
+// are wave-uniform loads): a level is possible if its error can project under `high` and its parent
+// error over `low` somewhere in [pmin, pmax], guaranteed if that holds with margin at both ends.
+template <bool kBoth, typename V>
+__device__ __forceinline__ void select_levels(const Params& p, const V& v, uint32_t level_base, uint32_t level_count,
+                                              float pmin, float pmax, LevelResult (&r)[2]) {
+  const float low0 = v.low[0], high0 = v.high[0], low1 = v.low[1], high1 = v.high[1];
+  r[0] = r[1] = LevelResult{0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t i = 0; i < level_count; i++) {
+    const float4 l0 = scalar_f4(p.mesh_table, level_base + i * 2);
+    const float4 l1 = scalar_f4(p.mesh_table, level_base + i * 2 + 1);
+    const uint32_t level_first = static_cast<uint32_t>(l0.x);
+    const uint32_t size = static_cast<uint32_t>(l0.y);
+    const bool open_parent = l0.w >= VG_INF_HALF, open_guard = l1.y >= VG_INF_HALF;
+    const float err_max = l0.z * pmax, parent_min = l0.w * pmin, guard_min = l1.x * pmin, guard_max = l1.y * pmax;
+    {
+      const bool possible = err_max <= high0 && (open_parent || parent_min > low0);
+      const bool sure = guard_min <= low0 * VG_MARGIN_LO && (open_guard || guard_max > high0 * VG_MARGIN_HI);
+      if (possible) {
+        r[0].first = min(r[0].first, level_first);
+        r[0].last = max(r[0].last, level_first + size);
+        r[0].possible += size;
+        if (sure) r[0].guaranteed += size;
+      }
+    }
+    if (kBoth) {
+      const bool possible = err_max <= high1 && (open_parent || parent_min > low1);
+      const bool sure = guard_min <= low1 * VG_MARGIN_LO && (open_guard || guard_max > high1 * VG_MARGIN_HI);
+      if (possible) {
+        r[1].first = min(r[1].first, level_first);
+        r[1].last = max(r[1].last, level_first + size);
+        r[1].possible += size;
+        if (sure) r[1].guaranteed += size;
+      }
+    }
+  }
 }
 
 // Sets bits [first, last) of a bitmap (measurement only).
@@ -230,145 +275,207 @@ __device__ void mark_range(uint32_t* bits, uint32_t first, uint32_t last) {
   }
 }
 
+// One wave per queued (view, cell), the cell's 128 instances in two rounds of 64 lanes. Each
+// (instance, cut) that survives the culls becomes at most one work item: a range of LOD runs for the
+// meshlet pass, or, on the fast path, the instance's <= kFastPathMax guaranteed meshlets flagged to be
+// drawn without cluster tests. The cell's items are appended with one atomic, lane-major, so item order
+// and first-task order agree. Counters are summed in registers over consecutive cells of one view.
 __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
   if (retry_round && !p.flags[retry_round - 1]) return;
   const uint32_t lane = lane_id();
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), rank = xcd_wave_rank();
+  const uint32_t q_begin = static_cast<uint32_t>(queue_count * rank / waves);
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (rank + 1) / waves);
-  for (uint32_t q = static_cast<uint32_t>(queue_count * rank / waves); q < q_end; q++) {
-    uint2 entry = p.cell_queue[q];
+  if (q_begin >= q_end) return;
+
+  // Counters of the current view (wave-uniform), flushed when the view changes.
+  uint32_t acc_slot = 0xffffffffu;
+  uint32_t acc_instances = 0, acc_occ_tests = 0, acc_occluded = 0, acc_survived[2] = {0, 0}, acc_work[2] = {0, 0};
+  uint64_t acc_clusters[2] = {0, 0};
+  auto flush = [&]() {
+    if (acc_slot == 0xffffffffu || lane != 0) return;
+    if (acc_instances) atomicAdd(metric_at(p, acc_slot, M_INSTANCES), static_cast<unsigned long long>(acc_instances));
+    if (acc_occ_tests) atomicAdd(metric_at(p, acc_slot, M_OCCLUSION), static_cast<unsigned long long>(acc_occ_tests));
+    if (acc_occluded) atomicAdd(view_counters(p, acc_slot) + 1, acc_occluded);
+    for (uint32_t c = 0; c < 2; c++) {
+      if (acc_survived[c]) atomicAdd(counter_at(p, acc_slot, c, kSurvived), acc_survived[c]);
+      if (acc_work[c]) {
+        atomicAdd(counter_at(p, acc_slot, c, kWork), acc_work[c]);
+        atomicAdd(metric_at(p, acc_slot, M_WORK0 + c), static_cast<unsigned long long>(acc_work[c]));
+      }
+      if (acc_clusters[c]) atomicAdd(metric_at(p, acc_slot, M_CLUSTERS0 + c), static_cast<unsigned long long>(acc_clusters[c]));
+    }
+  };
+
+  uint2 entry = scalar_u2(p.cell_queue, q_begin);
+  for (uint32_t q = q_begin; q < q_end; q++) {
     const uint32_t vi = uniform(entry.x);
     const uint32_t cell = uniform(entry.y);
-    const ViewDev& v = p.views[vi];
+    if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next cell, loaded while this one runs
+    const auto& v = scalar(p.views)[vi];
     bool redo_camera = true, redo_shadow = true;
     if (retry_round) {
       redo_camera = v.redo[0] != 0;
       redo_shadow = v.redo[1] != 0;
       if (!redo_camera && !redo_shadow) continue;
     }
-    const float4 record = p.c_rec[cell];
+    const uint32_t vslot = uniform(v.slot);
+    if (vslot != acc_slot) {
+      flush();
+      acc_slot = vslot;
+      acc_instances = acc_occ_tests = acc_occluded = 0;
+      acc_survived[0] = acc_survived[1] = acc_work[0] = acc_work[1] = 0;
+      acc_clusters[0] = acc_clusters[1] = 0;
+    }
+    const float4 record = scalar_f4(p.c_rec, cell);
     const uint32_t slot = uniform(static_cast<uint32_t>(record.y));
     const uint32_t first_instance = uniform(static_cast<uint32_t>(record.z));
-    const uint32_t count = uniform(static_cast<uint32_t>(record.w));
-    const float4 levers = p.mesh_table[slot * kMeshStride + 3];
+    const uint32_t count = min(uniform(static_cast<uint32_t>(record.w)), kCellSize);
+    const float4* mesh = p.mesh_table + slot * kMeshStride;
+    const float4 levers = scalar_f4(mesh, 3);
     const uint32_t level_count = min(static_cast<uint32_t>(levers.z), kMaxLevels);
     const uint32_t level_base = static_cast<uint32_t>(levers.w);
-    const float4 object_sphere = p.mesh_table[slot * kMeshStride + 0];
-    const float4 lod_bounds = p.mesh_table[slot * kMeshStride + 1];
-    const float4 cull_bounds = p.mesh_table[slot * kMeshStride + 2];
-    const float4 flags = p.mesh_table[slot * kMeshStride + 5];
+    const float4 object_sphere = scalar_f4(mesh, 0);
+    const float4 lod_bounds = scalar_f4(mesh, 1);
+    const float4 cull_bounds = scalar_f4(mesh, 2);
+    const float4 flags = scalar_f4(mesh, 5);
     const bool shadow_on = flags.y > 0.5f, occlusion_on = flags.z > 0.5f;
     const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
     const float max_draw_distance = levers.x;
-    uint32_t* draws_base = reinterpret_cast<uint32_t*>(p.draws[v.slot]);
+    const bool occlusion_test = occlusion_on && v.hzb_valid != 0;
+    acc_instances += count;
 
-    for (uint32_t r = 0; r < kCellSize; r += 64) {
-      const bool valid = r + lane < count;
-      const uint32_t instance = first_instance + r + lane;
-      float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = make_float4(0, 0, 0, 1);
-      if (valid) {
-        c0 = p.i_col0[instance];
-        c1 = p.i_col1[instance];
-        c2 = p.i_col2[instance];
-        c3 = p.i_col3[instance];
-      }
+    // Both rounds' matrices are loaded up front.
+    float4 m[2][4];
+#pragma unroll
+    for (uint32_t r = 0; r < 2; r++) {
+      const uint32_t instance = first_instance + r * 64 + lane;
+      const bool valid = r * 64 + lane < count;
+      m[r][0] = valid ? p.i_col0[instance] : make_float4(0, 0, 0, 0);
+      m[r][1] = valid ? p.i_col1[instance] : make_float4(0, 0, 0, 0);
+      m[r][2] = valid ? p.i_col2[instance] : make_float4(0, 0, 0, 0);
+      m[r][3] = valid ? p.i_col3[instance] : make_float4(0, 0, 0, 1);
+    }
+
+    // The lane's items: (round, cut) -> task range [lo, hi) (empty when lo == hi), fast flags.
+    uint32_t item_lo[2][2], item_hi[2][2], fast_bits = 0;
+    float item_scale[2];
+    uint32_t lane_clusters[2] = {0, 0};
+#pragma unroll
+    for (uint32_t r = 0; r < 2; r++) {
+      item_lo[r][0] = item_hi[r][0] = item_lo[r][1] = item_hi[r][1] = 0;
+      item_scale[r] = 0.0f;
+      if (r * 64 >= count) continue;  // wave-uniform
+      const bool valid = r * 64 + lane < count;
+      const uint32_t instance = first_instance + r * 64 + lane;
+      const float4 c0 = m[r][0], c1 = m[r][1], c2 = m[r][2], c3 = m[r][3];
       const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      item_scale[r] = scale;
       const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
       const float radius = object_sphere.w * scale;
       const bool no_frustum = v.frustum == 0;
-      bool in_camera = valid && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
-      bool in_shadow = valid && shadow_on && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, false) >= -radius);
-      in_camera = in_camera && redo_camera;
-      in_shadow = in_shadow && redo_shadow;
+      bool in_camera = valid && redo_camera && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
+      const bool in_shadow = valid && redo_shadow && shadow_on &&
+                             (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, false) >= -radius);
       const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
       const float view_distance = len3(vc.x, vc.y, vc.z);
       const bool too_small = view_distance > radius && (radius * v.proj_scale) / view_distance < min_pixels;
       const bool too_far = (view_distance - radius) > max_draw_distance;
       const bool keep = (in_camera || in_shadow) && !too_small && !too_far;
-      wave_count_add(metric_at(p, v.slot, M_INSTANCES), 1, valid);
       if (p.fp_instances && valid) atomicOr(&p.fp_instances[instance >> 5], 1u << (instance & 31));
+      const bool test = keep && in_camera && occlusion_test;
       bool occ = false;
-      if (keep && in_camera && occlusion_on && v.hzb_valid) occ = occluded_sphere(p, v, true, center.x, center.y, center.z, radius);
-      wave_count_add(metric_at(p, v.slot, M_OCCLUSION), 1, keep && in_camera && occlusion_on && v.hzb_valid);
-      if (occ) in_camera = false;
-      wave_atomic_add_uniform(view_counters(p, v.slot) + 1, 1u, occ);
+      if (test) occ = occluded_sphere(p, v, true, center.x, center.y, center.z, radius);
+      acc_occ_tests += static_cast<uint32_t>(__popcll(__ballot(test)));
+      acc_occluded += static_cast<uint32_t>(__popcll(__ballot(occ)));
+      in_camera = in_camera && !occ;
 
+      const bool in_cut[2] = {keep && in_camera, keep && in_shadow};
+      const bool any0 = __ballot(in_cut[0]) != 0, any1 = __ballot(in_cut[1]) != 0;
+      if (!any0 && !any1) continue;
       const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
       const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
       const float d = len3(lv.x, lv.y, lv.z);
       const float rr = lod_bounds.w * scale;
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
       const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
+      LevelResult lr[2];
+      if (any1) select_levels<true>(p, v, level_base, level_count, pmin, pmax, lr);
+      else select_levels<false>(p, v, level_base, level_count, pmin, pmax, lr);
       const float4 cc = mat_mul(c0, c1, c2, c3, cull_bounds.x, cull_bounds.y, cull_bounds.z, 1.0f);
       const float cull_radius = cull_bounds.w * scale;
-
+      bool work[2];
 #pragma unroll
       for (uint32_t cut = 0; cut < 2; cut++) {
-        const bool in_cut = keep && (cut == 0 ? in_camera : in_shadow);
-        LevelResult lr{0xffffffffu, 0u, 0u, 0u};
-        if (__ballot(in_cut)) lr = select_levels(p, level_base, level_count, v.low[cut], v.high[cut], pmin, pmax);
-        const bool survived = in_cut && lr.last > lr.first;
-        const bool fast = survived && lr.possible == lr.last - lr.first && lr.guaranteed == lr.possible &&
-                          lr.possible <= kFastPathMax &&
-                          (v.frustum == 0 || frustum_distance(v, cc.x, cc.y, cc.z, cut == 1 ? 1 : 0, false) >= cull_radius);
-        if (!__ballot(survived)) continue;
-        wave_atomic_add_uniform(counter_at(p, v.slot, cut, kSurvived), 1u, survived);
-        // Fast path: the lane's `possible` meshlets go straight to the draw list.
-        if (__ballot(fast)) {
-          const uint32_t base = wave_atomic_add_uniform(counter_at(p, v.slot, cut, kSelected), lr.possible, fast);
-          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), lr.possible, fast);
-          uint32_t tris[kFastPathMax];
-          uint32_t emit_sum = 0, emit_n = 0;
-#pragma unroll
-          for (uint32_t k = 0; k < kFastPathMax; k++) {
-            tris[k] = 0;
-            if (fast && k < lr.possible && base + k < p.max_meshlets) {
-              tris[k] = p.m_tris[lr.first + k];
-              emit_sum += tris[k] * 3;
-              emit_n = k + 1;
-            }
-          }
-          uint32_t index = wave_atomic_add_uniform(mesh_count_at(p, v.slot, cut, slot, 0), emit_sum, fast && emit_n > 0);
-          for (uint32_t k = 0; k < emit_n; k++) {
-            uint4 e = make_uint4(instance, lr.first + k, index, tris[k] | kFullFadeBits);
-            reinterpret_cast<uint4*>(draws_base)[base + k + cut * p.max_meshlets] = e;
-            index += tris[k] * 3;
-          }
+        const bool survived = in_cut[cut] && lr[cut].last > lr[cut].first;
+        const bool fast = survived && lr[cut].possible == lr[cut].last - lr[cut].first && lr[cut].guaranteed == lr[cut].possible &&
+                          lr[cut].possible <= kFastPathMax &&
+                          (no_frustum || frustum_distance(v, cc.x, cc.y, cc.z, cut == 1 ? 1 : 0, false) >= cull_radius);
+        work[cut] = survived && !fast;
+        acc_survived[cut] += static_cast<uint32_t>(__popcll(__ballot(survived)));
+        acc_work[cut] += static_cast<uint32_t>(__popcll(__ballot(work[cut])));
+        if (survived) lane_clusters[cut] += lr[cut].last - lr[cut].first;
+        if (p.fp_meshlets && survived) mark_range(p.fp_meshlets, lr[cut].first, lr[cut].last);
+        if (fast) {
+          item_lo[r][cut] = lr[cut].first;
+          item_hi[r][cut] = lr[cut].last;
+          fast_bits |= 1u << (r * 2 + cut);
         }
-        // Work item: (instance, first, last, tag) plus the item's first cluster test.
-        if (p.fp_meshlets && survived) mark_range(p.fp_meshlets, lr.first, lr.last);
-        const bool work = survived && !fast;
-        if (__ballot(work)) {
-          // Tasks are the LOD runs covering [first, last) (level ranges hold whole runs).
-          const uint32_t run_first = work ? p.m_run[lr.first] : 0u;
-          const uint32_t run_last = work ? p.m_run[lr.last - 1] + 1 : 0u;
-          const uint32_t len = run_last - run_first;
-          const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
-          const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
-          const uint32_t task_incl = wave_scan(len);
-          const uint32_t tasks = __shfl(task_incl, 63, 64);
-          unsigned long long old = 0;
-          if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
-          const uint32_t old_lo = __shfl(static_cast<uint32_t>(old), 0, 64);
-          const uint32_t old_hi = __shfl(static_cast<uint32_t>(old >> 32), 0, 64);
-          const uint64_t o = (static_cast<uint64_t>(old_hi) << 32) | old_lo;
-          const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
-          const uint64_t task_base = o & kTaskMask;
-          if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
-          if (work && item_base + rank < p.work_capacity) {
-            const uint32_t tag = slot | (cut << 16) | ((cut == 0 && occlusion_on ? 1u : 0u) << 17) | (vi << 18);
-            p.work[item_base + rank] = make_uint4(instance, run_first, run_last, tag);
-            p.work_start[item_base + rank] = task_base + task_incl - len;
-            p.work_scale[item_base + rank] = scale;
-          }
-          wave_atomic_add_uniform(counter_at(p, v.slot, cut, kWork), 1u, work);
-          wave_count_add(metric_at(p, v.slot, M_WORK0 + cut), 1, work);
-          wave_count_add(metric_at(p, v.slot, M_CLUSTERS0 + cut), lr.last - lr.first, work);
+      }
+      // Work items cover the LOD runs of [first, last) (level ranges hold whole runs).
+#pragma unroll
+      for (uint32_t cut = 0; cut < 2; cut++) {
+        if (work[cut]) {
+          item_lo[r][cut] = p.m_run[lr[cut].first];
+          item_hi[r][cut] = p.m_run[lr[cut].last - 1] + 1;
         }
       }
     }
+    acc_clusters[0] += wave_sum(lane_clusters[0]);
+    acc_clusters[1] += wave_sum(lane_clusters[1]);
+
+    // Append the cell's items: lane-major, (round, cut) order within a lane.
+    uint32_t n_items = 0, n_tasks = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint32_t len = item_hi[k >> 1][k & 1] - item_lo[k >> 1][k & 1];
+      n_items += len ? 1u : 0u;
+      n_tasks += len;
+    }
+    const uint32_t items_incl = wave_scan(n_items);
+    const uint32_t items = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(items_incl), 63));
+    if (items == 0) continue;
+    const uint32_t tasks_incl = wave_scan(n_tasks);
+    const uint32_t tasks = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tasks_incl), 63));
+    unsigned long long old = 0;
+    if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
+    const uint64_t o = (static_cast<uint64_t>(uniform(static_cast<uint32_t>(old >> 32))) << 32) | uniform(static_cast<uint32_t>(old));
+    const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
+    const uint64_t task_base = o & kTaskMask;
+    if (lane == 0 && (task_base + tasks >= kTaskLimit || static_cast<uint64_t>(item_base) + items > p.work_capacity))
+      atomicOr(&p.flags[3], 1u);
+    uint32_t at = item_base + items_incl - n_items;
+    uint64_t task_at = task_base + tasks_incl - n_tasks;
+    const uint32_t tag_base = slot | (vi << kTagViewShift);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint32_t r = k >> 1, cut = k & 1;
+      const uint32_t lo = item_lo[r][cut], hi = item_hi[r][cut];
+      if (hi > lo) {
+        if (at < p.work_capacity) {
+          const uint32_t tag = tag_base | (cut ? kTagShadow : 0u) | (cut == 0 && occlusion_on ? kTagOcclusion : 0u) |
+                               ((fast_bits >> k) & 1u ? kTagFast : 0u);
+          p.work[at] = make_uint4(first_instance + r * 64 + lane, lo, hi, tag);
+          p.work_start[at] = task_at;
+          p.work_scale[at] = item_scale[r];
+        }
+        at++;
+        task_at += hi - lo;
+      }
+    }
   }
+  flush();
 }
 
 // ------------------------------------------------------------------ 3. meshlets (meshletPass, GeometryPool.ts:926-968)
@@ -452,35 +559,48 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
 // the cut go to a per-wave LDS ring; phase 2 runs on full waves of them: frustum and occlusion tests
 // (drawn = visible and in the cut, so the order changes no decision), then compaction into the draw
 // list. The per-cluster-test cost is one LOD evaluation per run; visibility is paid per meshlet in the cut.
-__global__ __launch_bounds__(256) void k_meshlets(Params p) {
+// Compiled for several occupancy targets (waves per SIMD); the launch schedule picks one.
+template <int kWavesPerSimd>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_meshlets(Params p) {
   __shared__ uint4 ring[4][kQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   uint32_t head = 0, tail = 0;  // wave-uniform ring positions
 
   auto drain = [&](uint32_t n) {
     const bool active = lane < n;
-    const uint4 c = active ? ring[wib][(head + lane) & (kQueue - 1)] : make_uint4(0, 0, 0, 0);
+    const uint4 c = active ? ring[wib][(head + lane) % kQueue] : make_uint4(0, 0, 0, 0);
     const uint4 w = active ? p.work[c.x] : make_uint4(0, 0, 0, 0);
     const uint32_t meshlet = c.y, tag = w.w;
-    const uint32_t mslot = tag & 0xffffu, cut = (tag >> 16) & 1u, vi = tag >> 18;
-    const bool occ_on = ((tag >> 17) & 1u) != 0;
-    const ViewDev& v = p.views[vi];
-    bool drawn = false, occ_test = false;
-    uint32_t tris = 0;
-    if (active) {
-      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-      const float scale = p.work_scale[c.x];
-      const float4 cs = p.m_cull[meshlet];
-      const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
-      bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
-      occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
-      if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
-      drawn = visible;
-      if (drawn) tris = p.m_tris[meshlet];
+    const uint32_t mslot = tag & kTagSlotMask, cut = (tag & kTagShadow) ? 1u : 0u, vi = tag >> kTagViewShift;
+    const bool occ_on = (tag & kTagOcclusion) != 0, fast = (tag & kTagFast) != 0;
+    bool drawn = active && fast;  // fast path: guaranteed meshlets of a culling sphere inside the frustum
+    bool occ_test = false;
+    uint32_t vslot = 0;
+    // Visibility, one launched view at a time (its constants are wave-uniform there).
+    for (uint64_t pending = __ballot(active); pending;) {
+      const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+      const bool mine = active && vi == vu;
+      if (mine) {
+        // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
+        const auto& v = scalar(p.views)[uniform(vu)];
+        vslot = v.slot;
+        if (!fast) {
+          const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+          const float scale = p.work_scale[c.x];
+          const float4 cs = p.m_cull[meshlet];
+          const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+          bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
+          occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
+          if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
+          drawn = visible;
+        }
+      }
+      pending &= ~__ballot(mine);
     }
+    uint32_t tris = 0;
+    if (drawn) tris = p.m_tris[meshlet];
     head += n;
     const uint32_t packed_tris = tris | (c.z << 8);
-    const uint32_t vslot = v.slot;
     wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
     const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
     const bool write = drawn && at < p.max_meshlets;
@@ -494,42 +614,60 @@ __global__ __launch_bounds__(256) void k_meshlets(Params p) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t run) {
-    const uint32_t cut = (w.w >> 16) & 1u, vi = w.w >> 18;
-    const ViewDev& v = p.views[vi];
-    bool lod = false;
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t task) {
+    const uint32_t cut = (w.w & kTagShadow) ? 1u : 0u, vi = w.w >> kTagViewShift;
+    const bool fast = (w.w & kTagFast) != 0;
     uint32_t fades = 0, first = 0, count = 0;
-    if (valid) {
-      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-      const float scale = p.work_scale[item];
-      const float2 err = p.r_err[run];
-      const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
-      const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
-      const float low = v.low[cut], high = v.high[cut], fade = v.fade_scale;
-      uint32_t fade_low = 0, fade_high = kFadeSteps;
-      if (fade > 0.0f) {
-        fade_low = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(own, 1e-30f) / low) * fade), 0.0f, 255.0f));
-        fade_high = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(parent, 1e-30f) / low) * fade), 0.0f, 255.0f));
+    if (valid && fast) {
+      // Fast-path item: the task is a meshlet, drawn at full fade.
+      fades = kFadeSteps << 8;
+      first = task;
+      count = 1;
+    }
+    // The LOD test of a run, one launched view at a time (its constants are wave-uniform there).
+    for (uint64_t pending = __ballot(valid && !fast); pending;) {
+      const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+      const bool mine = valid && !fast && vi == vu;
+      if (mine) {
+        // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
+        const auto& v = scalar(p.views)[uniform(vu)];
+        const uint32_t run = task;
+        const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+        const float scale = p.work_scale[item];
+        const float2 err = p.r_err[run];
+        const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
+        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
+        const float low = cut ? v.low[1] : v.low[0], high = cut ? v.high[1] : v.high[0], fade = v.fade_scale;
+        uint32_t fade_low = 0, fade_high = kFadeSteps;
+        if (fade > 0.0f) {
+          fade_low = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(own, 1e-30f) / low) * fade), 0.0f, 255.0f));
+          fade_high = static_cast<uint32_t>(clampf(rintf(vg_logf(fmaxf(parent, 1e-30f) / low) * fade), 0.0f, 255.0f));
+        }
+        const bool lod = own <= high && parent > low && fade_high > fade_low;
+        fades = fade_low | (fade_high << 8);
+        if (lod) {
+          const uint2 span = p.r_span[run];
+          first = span.x;
+          count = span.y;
+        }
       }
-      lod = own <= high && parent > low && fade_high > fade_low;
-      fades = fade_low | (fade_high << 8);
-      if (lod) {
-        const uint32_t fc = p.r_first[run];
-        first = fc & ((1u << kRunCountShift) - 1u);
-        count = fc >> kRunCountShift;
-      }
+      pending &= ~__ballot(mine);
     }
     // Append the runs' meshlets to the ring (before the append fewer than 64 are pending).
     const uint32_t incl = wave_scan(count);
-    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
     if (total) {
       const uint32_t at = tail + incl - count;
-      for (uint32_t k = 0; k < count; k++) ring[wib][(at + k) & (kQueue - 1)] = make_uint4(item, first + k, fades, 0u);
+      for (uint32_t k = 0; k < count; k++) ring[wib][(at + k) % kQueue] = make_uint4(item, first + k, fades, 0u);
       tail += total;
       while (tail - head >= 64) {
         wave_sync();
         drain(64);
         wave_sync();
+      }
+      if (head >= kQueue) {  // keep the ring positions small (their difference is what counts)
+        head -= kQueue;
+        tail -= kQueue;
       }
     }
   });
@@ -609,7 +747,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
       const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
       LevelResult lr{0xffffffffu, 0u, 0u, 0u};
-      if (__ballot(keep)) lr = select_levels(p, level_base, level_count, v.budget_low, v.budget_high, pmin, pmax);
+      if (__ballot(keep)) lr = select_band(p, level_base, level_count, v.budget_low, v.budget_high, pmin, pmax);
       const bool work = keep && lr.last > lr.first;
       if (!__ballot(work)) continue;
       const uint32_t len = work ? lr.last - lr.first : 0u;
@@ -626,7 +764,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       const uint64_t task_base = o & kTaskMask;
       if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
       if (work && item_base + rank < p.work_capacity) {
-        const uint32_t tag = slot | ((occlusion_on ? 1u : 0u) << 17) | (vi << 18);
+        const uint32_t tag = slot | (occlusion_on ? kTagOcclusion : 0u) | (vi << kTagViewShift);
         p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
         p.work_start[item_base + rank] = task_base + task_incl - len;
         p.work_scale[item_base + rank] = scale;
@@ -655,8 +793,8 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
     }
   };
   for_each_task(p, [&](bool valid, uint32_t, const uint4& w, uint32_t meshlet) {
-    const uint32_t vi = w.w >> 18;
-    const bool occ_on = ((w.w >> 17) & 1u) != 0;
+    const uint32_t vi = w.w >> kTagViewShift;
+    const bool occ_on = (w.w & kTagOcclusion) != 0;
     const ViewDev& v = p.views[vi];
     uint32_t a = 0, e = 0, tris = 0;
     if (valid) {
@@ -886,6 +1024,15 @@ __global__ void k_arith_probe(const float* a, const float* b, float* out, uint32
   out[i * 5 + 2] = vg_logf(fabsf(x));
   out[i * 5 + 3] = rintf(x);
   out[i * 5 + 4] = static_cast<float>(__float_as_uint(y) >> 3);
+}
+
+// Wave-scan probe: for each group of 64 values, the inclusive prefix sum and the group's sum.
+__global__ __launch_bounds__(64) void k_scan_probe(const uint32_t* in, uint32_t* out, uint32_t groups) {
+  const uint32_t g = blockIdx.x;
+  if (g >= groups) return;
+  const uint32_t x = in[g * 64 + threadIdx.x];
+  out[g * 128 + threadIdx.x] = wave_scan(x);
+  out[g * 128 + 64 + threadIdx.x] = wave_sum(x);
 }
 
 }  // namespace vg

@@ -33,6 +33,7 @@ EXPORTS = (
     "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
+    "vg_schedule_get", "vg_schedule_set", "vg_tune",
 )
 
 
@@ -92,6 +93,19 @@ class View(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    """Launch schedule of the persistent passes (vg_schedule): grids per CU and the meshlet-pass variant."""
+
+    _fields_ = [
+        ("instance_blocks_per_cu", ctypes.c_uint32),
+        ("meshlet_blocks_per_cu", ctypes.c_uint32),
+        ("meshlet_waves_per_simd", ctypes.c_uint32),
+    ]
+
+    def as_tuple(self):
+        return (self.instance_blocks_per_cu, self.meshlet_blocks_per_cu, self.meshlet_waves_per_simd)
+
+
 class Metrics(ctypes.Structure):
     _fields_ = [
         ("cells_tested", ctypes.c_uint64),
@@ -142,10 +156,14 @@ def load():
         "vg_counter_words": ([P], U32),
         "vg_draw_args_words": ([P], U32),
         "vg_arith_probe": ([P, P, P, U32], ctypes.c_int),
+        "vg_scan_probe": ([P, P, U32], ctypes.c_int),
         "vg_profile": ([P, U32], ctypes.c_int),
         "vg_budget_thresholds": ([ctypes.c_float, ctypes.c_float, U32, P], ctypes.c_int),
         "vg_profile_read": ([P, P, U32, ctypes.POINTER(U64), P], ctypes.c_int),
         "vg_footprint": ([P, P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64)], ctypes.c_int),
+        "vg_schedule_get": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
+        "vg_schedule_set": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
+        "vg_tune": ([P, P, U32, U32, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -318,6 +336,24 @@ class Scene:
         return int(a.value), int(b.value)
 
     @property
+    def schedule(self) -> Schedule:
+        sc = Schedule()
+        self.ctx.check(self.lib.vg_schedule_get(self.handle, ctypes.byref(sc)), "vg_schedule_get")
+        return sc
+
+    def set_schedule(self, instance_blocks_per_cu: int, meshlet_blocks_per_cu: int, meshlet_waves_per_simd: int):
+        sc = Schedule(instance_blocks_per_cu, meshlet_blocks_per_cu, meshlet_waves_per_simd)
+        self.ctx.check(self.lib.vg_schedule_set(self.handle, ctypes.byref(sc)), "vg_schedule_set")
+
+    def tune(self, views, repeats: int = 5):
+        """Times candidate launch schedules on `views`, keeps the fastest; returns (schedule, median ms)."""
+        arr = (View * len(views))(*views)
+        sc, ms = Schedule(), ctypes.c_float()
+        self.ctx.check(self.lib.vg_tune(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views), repeats, ctypes.byref(sc),
+                                        ctypes.byref(ms)), "vg_tune")
+        return sc, float(ms.value)
+
+    @property
     def mesh_slots(self) -> int:
         return int(self.lib.vg_scene_mesh_slots(self.handle))
 
@@ -348,6 +384,17 @@ def budget_thresholds(lo: float, hi: float, bins: int) -> np.ndarray:
     rc = load().vg_budget_thresholds(lo, hi, bins, out.ctypes.data)
     if rc != VG_OK:
         raise VgError(rc, "vg_budget_thresholds: need bins > 0 and 0 < lo <= hi")
+    return out
+
+
+def scan_probe(values: np.ndarray) -> np.ndarray:
+    """Runs the device wave-scan probe on groups of 64 u32: per group (inclusive scan, group sum)."""
+    lib = load()
+    v = np.ascontiguousarray(values, dtype=np.uint32).reshape(-1, 64)
+    out = np.empty((len(v), 2, 64), dtype=np.uint32)
+    rc = lib.vg_scan_probe(v.ctypes.data, out.ctypes.data, len(v))
+    if rc != VG_OK:
+        raise VgError(rc, "vg_scan_probe failed")
     return out
 
 
