@@ -424,7 +424,7 @@ int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   mark();
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   mark();
-  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, view_count), dim3(256), 0, st, p);
+  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
   if (s->launch_budget) {
     hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
     hipLaunchKernelGGL(k_budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
@@ -1217,7 +1217,7 @@ int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats,
       chosen_ms = ms;
     }
   }
-  static const uint32_t kInstance[] = {4, 6, 8, 12};
+  static const uint32_t kInstance[] = {4, 6, 7, 8, 12};
   for (uint32_t b : kInstance) {
     if (rc) break;
     vg_schedule c = chosen;

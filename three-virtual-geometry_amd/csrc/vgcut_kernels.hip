@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
     v.redo[0] = v.redo[1] = 0;
     v.budget_bin = 0;
     *reinterpret_cast<float*>(metric_at(p, v.slot, M_THRESHOLD)) = v.error_threshold;
+    *metric_at(p, v.slot, M_CELLS) = p.cell_count;  // the cell pass tests every cell in every view
     if (v.budget) {
       const float* t = p.budget_thresholds + static_cast<size_t>(blockIdx.x) * p.budget_bins;
       v.budget_scale = get_state(state, kStateScale);
@@ -168,36 +169,57 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
 
 // ------------------------------------------------------------------ 1. cells (cellNode, GeometryPool.ts:728-760)
 
+// One thread per cell, looping over a group of up to 64 launched views (blockIdx.y): the view records
+// are wave-uniform (scalar loads), the cell's data is read once for all of them. The (view, cell) pairs
+// that pass are appended cell-major with one atomic per wave, so the instance pass meets the views of
+// one cell one after another and reuses the cell's instance data.
 __global__ __launch_bounds__(256) void k_cells(Params p) {
-  const uint32_t vi = blockIdx.y;
-  const auto& v = scalar(p.views)[vi];
+  __shared__ uint32_t visible[64];
+  const uint32_t lane = lane_id();
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
-  bool keep = false;
-  if (cell < p.cell_count) {
-    float4 sphere = p.c_sphere[cell];
-    float4 record = p.c_rec[cell];
-    uint32_t slot = static_cast<uint32_t>(record.y);
-    float4 flags = p.mesh_table[slot * kMeshStride + 5];
-    if (flags.x != 0.0f) {
-      float radius = sphere.w;
-      bool out = v.frustum == 1 && frustum_distance(v, sphere.x, sphere.y, sphere.z, 2, flags.y > 0.5f) < -radius;
-      if (!out) {
-        float4 levers = p.mesh_table[slot * kMeshStride + 3];
-        float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
-        float max_radius = record.x;
-        float4 vc = mat_mul16(v.view, sphere.x, sphere.y, sphere.z, 1.0f);
-        float view_distance = len3(vc.x, vc.y, vc.z);
-        float gap = view_distance - radius;
-        bool small = gap > max_radius && (max_radius * v.proj_scale) / fmaxf(gap, 1e-6f) < min_pixels;
-        keep = !small && !(gap > levers.x);
-      }
+  const uint32_t v_first = blockIdx.y * 64, v_end = min(v_first + 64, p.view_count);
+  if (threadIdx.x < 64) visible[threadIdx.x] = 0;
+  __syncthreads();
+  const bool valid = cell < p.cell_count;
+  float4 sphere = make_float4(0, 0, 0, 0), record = sphere, flags = sphere, levers = sphere;
+  if (valid) {
+    sphere = p.c_sphere[cell];
+    record = p.c_rec[cell];
+    const uint32_t slot = static_cast<uint32_t>(record.y);
+    flags = p.mesh_table[slot * kMeshStride + 5];
+    levers = p.mesh_table[slot * kMeshStride + 3];
+  }
+  const bool enabled = valid && flags.x != 0.0f;
+  const bool sweep = flags.y > 0.5f;
+  const float radius = sphere.w, max_radius = record.x;
+  uint64_t mask = 0;
+  if (__ballot(enabled)) {
+    for (uint32_t vi = v_first; vi < v_end; vi++) {
+      const auto& v = scalar(p.views)[vi];
+      bool keep = enabled && !(v.frustum == 1 && frustum_distance(v, sphere.x, sphere.y, sphere.z, 2, sweep) < -radius);
+      const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
+      const float4 vc = mat_mul16(v.view, sphere.x, sphere.y, sphere.z, 1.0f);
+      const float view_distance = len3(vc.x, vc.y, vc.z);
+      const float gap = view_distance - radius;
+      const bool small = gap > max_radius && (max_radius * v.proj_scale) / fmaxf(gap, 1e-6f) < min_pixels;
+      keep = keep && !small && !(gap > levers.x);
+      if (keep) mask |= 1ull << (vi - v_first);
+      const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(keep)));
+      if (lane == 0 && n) atomicAdd(&visible[vi - v_first], n);
     }
   }
-  uint32_t tested = cell < p.cell_count ? 1u : 0u;
-  wave_count_add(metric_at(p, v.slot, M_CELLS), tested, tested != 0);
-  uint32_t rank = wave_atomic_add_uniform(p.cell_queue_count, 1u, keep);
-  wave_atomic_add_uniform(view_counters(p, v.slot), 1u, keep);  // counters[0]: visible cells
-  if (keep) p.cell_queue[rank] = make_uint2(vi, cell);
+  const uint32_t n = static_cast<uint32_t>(__popcll(mask));
+  const uint32_t incl = wave_scan(n);
+  const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(p.cell_queue_count, total);
+  uint32_t at = uniform(base) + incl - n;
+  for (uint64_t m = mask; m; m &= m - 1) p.cell_queue[at++] = make_uint2(v_first + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1), cell);
+  __syncthreads();
+  if (threadIdx.x < v_end - v_first && visible[threadIdx.x]) {
+    const uint32_t slot = scalar(p.views)[v_first + threadIdx.x].slot;
+    atomicAdd(view_counters(p, slot), visible[threadIdx.x]);  // counters[0]: visible cells
+  }
 }
 
 // ------------------------------------------------------------------ 2. instances (instancePass + selectLevels, GeometryPool.ts:771-913)
@@ -308,11 +330,26 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     }
   };
 
+  // The current cell: its record, mesh and, per round of 64 instances, the view-independent values
+  // (scale, bounding-sphere centre, LOD-sphere centre, culling-sphere centre), kept while consecutive
+  // queue entries name the same cell.
+  uint32_t cur_cell = 0xffffffffu, slot = 0, first_instance = 0, count = 0, level_count = 0, level_base = 0;
+  float4 object_sphere = make_float4(0, 0, 0, 0), lod_bounds = object_sphere, cull_bounds = object_sphere;
+  float4 flags = object_sphere, levers = object_sphere;
+  float d_scale[2], d_center[2][3], d_lq[2][4], d_cc[2][3];
+#pragma unroll
+  for (uint32_t r = 0; r < 2; r++) {
+    d_scale[r] = 0.0f;
+    d_center[r][0] = d_center[r][1] = d_center[r][2] = 0.0f;
+    d_lq[r][0] = d_lq[r][1] = d_lq[r][2] = d_lq[r][3] = 0.0f;
+    d_cc[r][0] = d_cc[r][1] = d_cc[r][2] = 0.0f;
+  }
+
   uint2 entry = scalar_u2(p.cell_queue, q_begin);
   for (uint32_t q = q_begin; q < q_end; q++) {
     const uint32_t vi = uniform(entry.x);
     const uint32_t cell = uniform(entry.y);
-    if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next cell, loaded while this one runs
+    if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next entry, loaded while this one runs
     const auto& v = scalar(p.views)[vi];
     bool redo_camera = true, redo_shadow = true;
     if (retry_round) {
@@ -328,35 +365,54 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       acc_survived[0] = acc_survived[1] = acc_work[0] = acc_work[1] = 0;
       acc_clusters[0] = acc_clusters[1] = 0;
     }
-    const float4 record = scalar_f4(p.c_rec, cell);
-    const uint32_t slot = uniform(static_cast<uint32_t>(record.y));
-    const uint32_t first_instance = uniform(static_cast<uint32_t>(record.z));
-    const uint32_t count = min(uniform(static_cast<uint32_t>(record.w)), kCellSize);
-    const float4* mesh = p.mesh_table + slot * kMeshStride;
-    const float4 levers = scalar_f4(mesh, 3);
-    const uint32_t level_count = min(static_cast<uint32_t>(levers.z), kMaxLevels);
-    const uint32_t level_base = static_cast<uint32_t>(levers.w);
-    const float4 object_sphere = scalar_f4(mesh, 0);
-    const float4 lod_bounds = scalar_f4(mesh, 1);
-    const float4 cull_bounds = scalar_f4(mesh, 2);
-    const float4 flags = scalar_f4(mesh, 5);
+    if (cell != cur_cell) {
+      cur_cell = cell;
+      const float4 record = scalar_f4(p.c_rec, cell);
+      slot = uniform(static_cast<uint32_t>(record.y));
+      first_instance = uniform(static_cast<uint32_t>(record.z));
+      count = min(uniform(static_cast<uint32_t>(record.w)), kCellSize);
+      const float4* mesh = p.mesh_table + slot * kMeshStride;
+      levers = scalar_f4(mesh, 3);
+      level_count = min(static_cast<uint32_t>(levers.z), kMaxLevels);
+      level_base = static_cast<uint32_t>(levers.w);
+      object_sphere = scalar_f4(mesh, 0);
+      lod_bounds = scalar_f4(mesh, 1);
+      cull_bounds = scalar_f4(mesh, 2);
+      flags = scalar_f4(mesh, 5);
+      float4 m[2][4];
+#pragma unroll
+      for (uint32_t r = 0; r < 2; r++) {
+        const uint32_t instance = first_instance + r * 64 + lane;
+        const bool valid = r * 64 + lane < count;
+        m[r][0] = valid ? p.i_col0[instance] : make_float4(0, 0, 0, 0);
+        m[r][1] = valid ? p.i_col1[instance] : make_float4(0, 0, 0, 0);
+        m[r][2] = valid ? p.i_col2[instance] : make_float4(0, 0, 0, 0);
+        m[r][3] = valid ? p.i_col3[instance] : make_float4(0, 0, 0, 1);
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < 2; r++) {
+        const float4 c0 = m[r][0], c1 = m[r][1], c2 = m[r][2], c3 = m[r][3];
+        d_scale[r] = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+        const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+        d_center[r][0] = center.x;
+        d_center[r][1] = center.y;
+        d_center[r][2] = center.z;
+        const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
+        d_lq[r][0] = lq.x;
+        d_lq[r][1] = lq.y;
+        d_lq[r][2] = lq.z;
+        d_lq[r][3] = lq.w;
+        const float4 cc = mat_mul(c0, c1, c2, c3, cull_bounds.x, cull_bounds.y, cull_bounds.z, 1.0f);
+        d_cc[r][0] = cc.x;
+        d_cc[r][1] = cc.y;
+        d_cc[r][2] = cc.z;
+      }
+    }
     const bool shadow_on = flags.y > 0.5f, occlusion_on = flags.z > 0.5f;
     const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
     const float max_draw_distance = levers.x;
     const bool occlusion_test = occlusion_on && v.hzb_valid != 0;
     acc_instances += count;
-
-    // Both rounds' matrices are loaded up front.
-    float4 m[2][4];
-#pragma unroll
-    for (uint32_t r = 0; r < 2; r++) {
-      const uint32_t instance = first_instance + r * 64 + lane;
-      const bool valid = r * 64 + lane < count;
-      m[r][0] = valid ? p.i_col0[instance] : make_float4(0, 0, 0, 0);
-      m[r][1] = valid ? p.i_col1[instance] : make_float4(0, 0, 0, 0);
-      m[r][2] = valid ? p.i_col2[instance] : make_float4(0, 0, 0, 0);
-      m[r][3] = valid ? p.i_col3[instance] : make_float4(0, 0, 0, 1);
-    }
 
     // The lane's items: (round, cut) -> task range [lo, hi) (empty when lo == hi), fast flags.
     uint32_t item_lo[2][2], item_hi[2][2], fast_bits = 0;
@@ -369,10 +425,9 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       if (r * 64 >= count) continue;  // wave-uniform
       const bool valid = r * 64 + lane < count;
       const uint32_t instance = first_instance + r * 64 + lane;
-      const float4 c0 = m[r][0], c1 = m[r][1], c2 = m[r][2], c3 = m[r][3];
-      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+      const float scale = d_scale[r];
       item_scale[r] = scale;
-      const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+      const float3 center = make_float3(d_center[r][0], d_center[r][1], d_center[r][2]);
       const float radius = object_sphere.w * scale;
       const bool no_frustum = v.frustum == 0;
       bool in_camera = valid && redo_camera && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
@@ -394,8 +449,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       const bool in_cut[2] = {keep && in_camera, keep && in_shadow};
       const bool any0 = __ballot(in_cut[0]) != 0, any1 = __ballot(in_cut[1]) != 0;
       if (!any0 && !any1) continue;
-      const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
-      const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
+      const float4 lv = mat_mul16(v.view, d_lq[r][0], d_lq[r][1], d_lq[r][2], d_lq[r][3]);
       const float d = len3(lv.x, lv.y, lv.z);
       const float rr = lod_bounds.w * scale;
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
@@ -403,7 +457,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       LevelResult lr[2];
       if (any1) select_levels<true>(p, v, level_base, level_count, pmin, pmax, lr);
       else select_levels<false>(p, v, level_base, level_count, pmin, pmax, lr);
-      const float4 cc = mat_mul(c0, c1, c2, c3, cull_bounds.x, cull_bounds.y, cull_bounds.z, 1.0f);
+      const float3 cc = make_float3(d_cc[r][0], d_cc[r][1], d_cc[r][2]);
       const float cull_radius = cull_bounds.w * scale;
       bool work[2];
 #pragma unroll
