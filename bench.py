@@ -35,6 +35,23 @@ MAX_TRIANGLES = 300_000_000
 MAX_SHADOW_TRIANGLES = 100_000_000
 
 
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world64_rocprof.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary of this workload (two --pmc
+    runs, FETCH_SIZE and WRITE_SIZE, calibrated on a device copy), or (None, reason)."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None, "no rocprofv3 summary"
+    with open(TRAFFIC_SUMMARY) as f:
+        summary = json.load(f)
+    for name, k in summary.get("kernels", {}).items():
+        if name.split("<")[0] == kernel and "hbm_bytes" in k:
+            return k["hbm_bytes"], (f"profiles/world64_rocprof.json: {name}, FETCH_SIZE x {summary['fetch_factor']:.2f} + "
+                                    f"WRITE_SIZE x {summary['write_factor']:.2f} per dispatch, avg dispatch {k['avg_us']:.0f} us")
+    return None, f"{kernel} not in the rocprofv3 summary"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -277,6 +294,7 @@ def main():
     dom_bytes = per_kernel_bytes.get(dominant, launch_bytes)
     dom_ms = kernel_ms[dominant]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_source = measured_traffic(dominant)
 
     # ---------------- CPU baseline: the oracle, single thread, on a bounded sample of the views ----------------
     cpu = None
@@ -355,7 +373,8 @@ def main():
                           "unique_instances": inst_unique, "unique_meshlets": meshlet_unique, "cells": cells},
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
+                     "algorithmic_bytes": dom_bytes,
                      "launch_algorithmic_bytes": launch_bytes, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
     }

@@ -114,11 +114,13 @@ __device__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, fl
   uint32_t level = ceil_log2(extent);
   uint32_t top = static_cast<uint32_t>(header[kHzbLevelCountSlot]) - 1u;
   level = level > top ? top : level;
-  float scale = static_cast<float>(1u << level);
+  // fx / 2^level == fx * 2^-level exactly (fx is a texel coordinate in [0, 4096): no underflow), so the
+  // division by the mip scale is a multiplication by its exact reciprocal.
+  const float inv_scale = __builtin_ldexpf(1.0f, -static_cast<int>(level));
   float offset = hzb[level * 4], lw = hzb[level * 4 + 1], lh = hzb[level * 4 + 2];  // level differs per lane
   auto sample = [&](float fx, float fy) {
-    float tx = clampf(floorf(fx / scale), 0.0f, lw - 1.0f);
-    float ty = clampf(floorf(fy / scale), 0.0f, lh - 1.0f);
+    float tx = clampf(floorf(fx * inv_scale), 0.0f, lw - 1.0f);
+    float ty = clampf(floorf(fy * inv_scale), 0.0f, lh - 1.0f);
     return hzb[static_cast<uint32_t>(offset + ty * lw + tx)];
   };
   float m = fmaxf(fmaxf(sample(x0, y0), sample(x1, y0)), fmaxf(sample(x0, y1), sample(x1, y1)));

@@ -90,6 +90,19 @@ def summarize(trace_db: str, fetch_db: str, write_db: str, copy_bytes: float) ->
     return {"fetch_factor": f_fetch, "write_factor": f_write, "copy_bytes": copy_bytes, "kernels": kernels}
 
 
+def add_counters(summary: dict, db: str) -> None:
+    """Per kernel, the busy-dispatch average of every counter in a further --pmc database (instruction mix)."""
+    con = sqlite3.connect(db)
+    names = [r[0] for r in con.execute("select distinct counter_name from counters_collection").fetchall()]
+    for counter in sorted(names):
+        for name, values in counter_values(db, counter).items():
+            k = summary["kernels"].get(short_name(name))
+            if k is None or not values:
+                continue
+            b = busy(values) or values
+            k.setdefault("counters", {})[counter] = sum(b) / len(b)
+
+
 def markdown(summary: dict, title: str, commands: List[str]) -> str:
     lines = [f"# {title}", "", "Commands (from the repository root):", ""]
     lines += [f"    {c}" for c in commands]
@@ -103,6 +116,13 @@ def markdown(summary: dict, title: str, commands: List[str]) -> str:
                          f"{k['write_kib']:.0f} | {k['hbm_bytes'] / 1e6:.1f} MB |")
         else:
             lines.append(f"| {name} | {k['dispatches']} ({k['busy_dispatches']}) | {k['avg_us']:.1f} | - | - | - |")
+    counters = sorted({c for k in summary["kernels"].values() for c in k.get("counters", {})})
+    if counters:
+        lines += ["", "Instruction mix per busy dispatch (SQ counters, summed over the device):", "",
+                  "| kernel | " + " | ".join(counters) + " |", "| --- |" + " --- |" * len(counters)]
+        for name, k in sorted(summary["kernels"].items(), key=lambda kv: -kv[1]["total_ms"]):
+            if "counters" in k:
+                lines.append(f"| {name} | " + " | ".join(f"{k['counters'].get(c, 0):.3g}" for c in counters) + " |")
     return "\n".join(lines) + "\n"
 
 
@@ -112,12 +132,15 @@ def main():
     ap.add_argument("--fetch", required=True, help="results db of the --pmc FETCH_SIZE run")
     ap.add_argument("--write", required=True, help="results db of the --pmc WRITE_SIZE run")
     ap.add_argument("--copy-bytes", type=float, required=True, help="bytes read (= written) by the calibration copy")
+    ap.add_argument("--counters", action="append", default=[], help="further --pmc databases (instruction mix)")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--command", action="append", default=[], help="the commands that produced the databases")
     ap.add_argument("--markdown", required=True)
     ap.add_argument("--json", required=True)
     args = ap.parse_args()
     s = summarize(args.trace, args.fetch, args.write, args.copy_bytes)
+    for db in args.counters:
+        add_counters(s, db)
     s["commands"] = args.command
     with open(args.markdown, "w") as f:
         f.write(markdown(s, args.title, args.command))
