@@ -1,0 +1,43 @@
+// Test driver: exercises the vgcut N-API addon from Node the way the TypeScript host would. Reads a JSON
+// description (context config, meshes with instances, optional packed vg_view records to cut on the GPU)
+// and writes what the addon returned as JSON (raw buffers as hex, so the comparison is bit-exact).
+'use strict';
+const fs = require('fs');
+const [, , addonPath, inPath, outPath] = process.argv;
+const vg = require(addonPath);
+const spec = JSON.parse(fs.readFileSync(inPath, 'utf8'));
+const hex = (ab) => Buffer.from(ab).toString('hex');
+const out = { slots: [], orders: [] };
+const ctx = vg.createContext(spec.config);
+const scene = vg.createScene(ctx);
+for (const m of spec.meshes) {
+  const data = {
+    meshletBounds: new Float32Array(m.meshletBounds), meshletInfo: new Uint32Array(m.meshletInfo),
+    levelRanges: new Uint32Array(m.levelRanges), levelErrors: new Float32Array(m.levelErrors),
+    levelGuardErrors: new Float32Array(m.levelGuardErrors), boundingSphere: new Float32Array(m.boundingSphere),
+    lodBoundsSphere: new Float32Array(m.lodBoundsSphere), cullBoundsSphere: new Float32Array(m.cullBoundsSphere),
+  };
+  const slot = vg.addMesh(scene, data);
+  const order = vg.setInstances(scene, slot, new Float32Array(m.matrices));
+  if (m.flags) vg.setFlags(scene, slot, m.flags);
+  out.slots.push(slot);
+  out.orders.push(Array.from(order));
+}
+out.pool = [0, 1, 2, 3, 4].map((b) => hex(vg.poolRead(scene, b)));
+const finish = () => fs.writeFileSync(outPath, JSON.stringify(out));
+if (!spec.views) {
+  finish();
+} else {
+  const buf = Buffer.from(spec.views, 'hex');
+  const views = buf.buffer.slice(buf.byteOffset, buf.byteOffset + buf.length);
+  vg.cut(scene, views, spec.viewCount).then(() => {
+    out.results = spec.viewSlots.map((slot) => {
+      const counters = new Uint32Array(vg.counterWords(ctx));
+      const args = new Uint32Array(vg.drawArgsWords(ctx));
+      const list = new Uint32Array(8 * spec.maxEntries);
+      vg.read(scene, slot, counters, args, list);
+      return { counters: hex(counters.buffer), args: hex(args.buffer), list: hex(list.buffer) };
+    });
+    finish();
+  }).catch((e) => { out.error = e.message; finish(); process.exitCode = 3; });
+}

@@ -1,0 +1,303 @@
+// vgcut_napi.cc: the N-API addon the TypeScript host loads in place of the TSL compute nodes (build.sh or
+// node-gyp with binding.gyp; links lib/libvgcut.so). JS values in, C ABI calls out.
+#include <node_api.h>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "vgcut.h"
+
+#define CHECK_NAPI(call)                                              \
+  do {                                                                \
+    if ((call) != napi_ok) {                                          \
+      napi_throw_error(env, nullptr, "vgcut: N-API call failed");     \
+      return nullptr;                                                 \
+    }                                                                 \
+  } while (0)
+
+static napi_value throw_vg(napi_env env, int rc, vg_ctx* ctx) {
+  char code[16];
+  snprintf(code, sizeof(code), "%d", rc);
+  napi_throw_error(env, code, ctx ? vg_last_error(ctx) : "vgcut error");
+  return nullptr;
+}
+
+template <typename T>
+static T* typed(napi_env env, napi_value v, size_t* n) {
+  napi_typedarray_type type;
+  void* data = nullptr;
+  napi_value buffer;
+  size_t offset = 0;
+  if (napi_get_typedarray_info(env, v, &type, n, &data, &buffer, &offset) != napi_ok) return nullptr;
+  return static_cast<T*>(data);
+}
+
+static napi_value prop(napi_env env, napi_value obj, const char* name) {
+  napi_value v = nullptr;
+  napi_get_named_property(env, obj, name, &v);
+  return v;
+}
+
+// createContext({maxTriangles, maxShadowTriangles, maxMeshlets, maxViews, meshSlots, device}) -> external
+static napi_value CreateContext(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vg_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  auto u32 = [&](const char* name, uint32_t* out) {
+    uint32_t x;
+    if (napi_get_value_uint32(env, prop(env, argv[0], name), &x) == napi_ok) *out = x;
+  };
+  u32("maxTriangles", &cfg.max_triangles);
+  u32("maxShadowTriangles", &cfg.max_shadow_triangles);
+  u32("maxMeshlets", &cfg.max_meshlets);
+  u32("maxViews", &cfg.max_views);
+  u32("meshSlots", &cfg.mesh_slots);
+  int32_t device = 0;
+  napi_get_value_int32(env, prop(env, argv[0], "device"), &device);
+  cfg.device = device;
+  vg_ctx* ctx = nullptr;
+  if (int rc = vg_create(&cfg, &ctx)) return throw_vg(env, rc, nullptr);
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, ctx, [](napi_env, void* p, void*) { vg_destroy(static_cast<vg_ctx*>(p)); }, nullptr, &out));
+  return out;
+}
+
+struct Scene {
+  vg_ctx* ctx;
+  vg_scene* scene;
+};
+
+// createScene(ctx) -> external (one GeometryPool)
+static napi_value CreateScene(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* ctx = nullptr;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
+  auto* s = new Scene{static_cast<vg_ctx*>(ctx), nullptr};
+  if (int rc = vg_scene_create(s->ctx, &s->scene)) {
+    delete s;
+    return throw_vg(env, rc, static_cast<vg_ctx*>(ctx));
+  }
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, s, [](napi_env, void* p, void*) {
+    auto* sc = static_cast<Scene*>(p);
+    vg_scene_destroy(sc->scene);
+    delete sc;
+  }, nullptr, &out));
+  return out;
+}
+
+// addMesh(scene, data: VirtualMeshData) -> slot (GeometryPool.add, geometry part)
+static napi_value AddMesh(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  auto* s = static_cast<Scene*>(p);
+  size_t nb = 0, ni = 0, nr = 0, ne = 0, ng = 0, n4 = 0;
+  vg_mesh_desc d;
+  memset(&d, 0, sizeof(d));
+  d.meshlet_bounds = typed<float>(env, prop(env, argv[1], "meshletBounds"), &nb);
+  d.meshlet_info = typed<uint32_t>(env, prop(env, argv[1], "meshletInfo"), &ni);
+  d.level_ranges = typed<uint32_t>(env, prop(env, argv[1], "levelRanges"), &nr);
+  d.level_errors = typed<float>(env, prop(env, argv[1], "levelErrors"), &ne);
+  d.level_guard_errors = typed<float>(env, prop(env, argv[1], "levelGuardErrors"), &ng);
+  d.meshlet_count = static_cast<uint32_t>(ni / 4);
+  d.level_count = static_cast<uint32_t>(nr / 2);
+  const char* spheres[3] = {"boundingSphere", "lodBoundsSphere", "cullBoundsSphere"};
+  float* dst[3] = {d.bounding_sphere, d.lod_bounds_sphere, d.cull_bounds_sphere};
+  for (int k = 0; k < 3; k++) {
+    const float* src = typed<float>(env, prop(env, argv[1], spheres[k]), &n4);
+    if (src && n4 == 4) memcpy(dst[k], src, sizeof(float) * 4);
+  }
+  uint32_t slot = 0;
+  if (int rc = vg_mesh_add(s->scene, &d, &slot)) return throw_vg(env, rc, s->ctx);
+  napi_value out;
+  CHECK_NAPI(napi_create_uint32(env, slot, &out));
+  return out;
+}
+
+// setInstances(scene, slot, matrices: Float32Array, colors?: Float32Array) -> Uint32Array gpuToInput
+static napi_value SetInstances(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  auto* s = static_cast<Scene*>(p);
+  size_t nm = 0, nc = 0;
+  const float* m = typed<float>(env, argv[2], &nm);
+  const float* c = argc > 3 ? typed<float>(env, argv[3], &nc) : nullptr;
+  const uint32_t count = static_cast<uint32_t>(nm / 16);
+  napi_value ab, out;
+  void* order = nullptr;
+  CHECK_NAPI(napi_create_arraybuffer(env, sizeof(uint32_t) * count, &order, &ab));
+  if (int rc = vg_instances_set(s->scene, slot, m, c, count, static_cast<uint32_t*>(order))) return throw_vg(env, rc, s->ctx);
+  CHECK_NAPI(napi_create_typedarray(env, napi_uint32_array, count, ab, 0, &out));
+  return out;
+}
+
+// setFlags(scene, slot, {maxDrawDistance, minPixelRadius, enabled, shadow, occlusion}) (GeometryPool.writeFlags)
+static napi_value SetFlags(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  auto* s = static_cast<Scene*>(p);
+  double max_draw = 1e30, min_px = 0.0;
+  bool enabled = true, shadow = true, occlusion = false;
+  napi_get_value_double(env, prop(env, argv[2], "maxDrawDistance"), &max_draw);
+  napi_get_value_double(env, prop(env, argv[2], "minPixelRadius"), &min_px);
+  napi_get_value_bool(env, prop(env, argv[2], "enabled"), &enabled);
+  napi_get_value_bool(env, prop(env, argv[2], "shadow"), &shadow);
+  napi_get_value_bool(env, prop(env, argv[2], "occlusion"), &occlusion);
+  if (int rc = vg_mesh_flags(s->scene, slot, static_cast<float>(max_draw), static_cast<float>(min_px), enabled, shadow, occlusion))
+    return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
+// cut(scene, views: ArrayBuffer of packed vg_view records, count) -> Promise<void>; the wait runs off
+// the JS thread (napi_async_work), so the render loop keeps going while the GPU cuts.
+struct CutWork {
+  Scene* s;
+  std::vector<vg_view> views;
+  int rc = 0;
+  napi_deferred deferred;
+  napi_async_work work;
+};
+
+static napi_value Cut(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  void* data = nullptr;
+  size_t bytes = 0;
+  uint32_t count = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_arraybuffer_info(env, argv[1], &data, &bytes));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[2], &count));
+  if (bytes < sizeof(vg_view) * count) {
+    napi_throw_range_error(env, nullptr, "vgcut: view buffer smaller than count records");
+    return nullptr;
+  }
+  auto* w = new CutWork{static_cast<Scene*>(p), std::vector<vg_view>(count)};
+  memcpy(w->views.data(), data, sizeof(vg_view) * count);
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
+  CHECK_NAPI(napi_create_string_utf8(env, "vgcut.cut", NAPI_AUTO_LENGTH, &name));
+  CHECK_NAPI(napi_create_async_work(
+      env, nullptr, name,
+      [](napi_env, void* d) {
+        auto* w = static_cast<CutWork*>(d);
+        w->rc = vg_cut_graph(w->s->scene, w->views.data(), static_cast<uint32_t>(w->views.size()));
+        if (!w->rc) w->rc = vg_sync(w->s->scene);
+      },
+      [](napi_env env, napi_status, void* d) {
+        auto* w = static_cast<CutWork*>(d);
+        napi_value result;
+        if (w->rc) {
+          napi_value msg;
+          napi_create_string_utf8(env, vg_last_error(w->s->ctx), NAPI_AUTO_LENGTH, &msg);
+          napi_create_error(env, nullptr, msg, &result);
+          napi_reject_deferred(env, w->deferred, result);
+        } else {
+          napi_get_undefined(env, &result);
+          napi_resolve_deferred(env, w->deferred, result);
+        }
+        napi_delete_async_work(env, w->work);
+        delete w;
+      },
+      w, &w->work));
+  CHECK_NAPI(napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// read(scene, viewSlot, counters: Uint32Array, drawArgs: Uint32Array, drawList: Uint32Array) (readCounters)
+static napi_value Read(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  auto* s = static_cast<Scene*>(p);
+  size_t nc = 0, na = 0, nl = 0;
+  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc);
+  uint32_t* args = typed<uint32_t>(env, argv[3], &na);
+  uint32_t* list = typed<uint32_t>(env, argv[4], &nl);
+  if (nc < vg_counter_words(s->ctx) || na < vg_draw_args_words(s->ctx)) {
+    napi_throw_range_error(env, nullptr, "vgcut: counters or drawArgs array too small");
+    return nullptr;
+  }
+  if (int rc = vg_read(s->scene, slot, counters, args, list, static_cast<uint32_t>(nl / 8), nullptr)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
+// poolRead(scene, buffer) -> ArrayBuffer copy of a pool buffer in the reference layout (VG_POOL_*)
+static napi_value PoolRead(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t which = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &which));
+  auto* s = static_cast<Scene*>(p);
+  uint64_t bytes = 0;
+  if (int rc = vg_scene_pool_read(s->scene, which, nullptr, 0, &bytes)) return throw_vg(env, rc, s->ctx);
+  napi_value ab;
+  void* data = nullptr;
+  CHECK_NAPI(napi_create_arraybuffer(env, bytes, &data, &ab));
+  if (int rc = vg_scene_pool_read(s->scene, which, data, bytes, &bytes)) return throw_vg(env, rc, s->ctx);
+  return ab;
+}
+
+// counterWords(ctx), drawArgsWords(ctx): array sizes for read()
+static napi_value CounterWords(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  void* ctx = nullptr;
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
+  CHECK_NAPI(napi_create_uint32(env, vg_counter_words(static_cast<vg_ctx*>(ctx)), &out));
+  return out;
+}
+
+static napi_value DrawArgsWords(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  void* ctx = nullptr;
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
+  CHECK_NAPI(napi_create_uint32(env, vg_draw_args_words(static_cast<vg_ctx*>(ctx)), &out));
+  return out;
+}
+
+static napi_value Init(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+      {"createContext", nullptr, CreateContext, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"createScene", nullptr, CreateScene, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"addMesh", nullptr, AddMesh, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setInstances", nullptr, SetInstances, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setFlags", nullptr, SetFlags, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cut", nullptr, Cut, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"read", nullptr, Read, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"poolRead", nullptr, PoolRead, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"counterWords", nullptr, CounterWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"drawArgsWords", nullptr, DrawArgsWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+  };
+  napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)
