@@ -199,6 +199,64 @@ int vg_schedule_set(vg_scene* scene, const vg_schedule* schedule);
 int vg_tune(vg_scene* scene, const vg_view* views, uint32_t count, uint32_t repeats, vg_schedule* best, float* best_ms);
 
 /*
+ * Request server: cut requests against resident scenes, batched continuously (no counterpart in the
+ * reference, which cuts its views inside VirtualGeometry.update, VirtualGeometry.ts:392-507; this is
+ * the serving form of that path). A worker thread keeps a FIFO of requests; whenever a lane (a HIP
+ * stream) is free it launches one batch: queued requests of one scene, oldest first, that fit the
+ * batch's view limit, the scene's free view slots and the HBM budget. A batch's transient buffers
+ * (cell queue, work list) come from the device's stream-ordered memory pool and are returned when it
+ * ends; requests that do not fit wait while others overtake them, up to max_bypass_us. Batches on
+ * different lanes overlap on the device. Each request's views start from a controller at rest (scale 1),
+ * so a result does not depend on the batch it ran in. Results stay in the request's view slots until
+ * vg_server_release. Scenes given to a server must not be changed or cut directly while it runs, and
+ * only the vg_server_* calls may be made on its context meanwhile.
+ */
+typedef struct vg_server_config {
+  uint64_t hbm_budget;          /* bytes for resident scenes + in-flight batches; 0: 90% of device memory */
+  uint32_t lanes;               /* batches in flight, 1-16 (default 4) */
+  uint32_t max_batch_views;     /* views per batch, 1 - max_views (default max_views) */
+  uint32_t max_batch_requests;  /* requests per batch; 0: no limit, 1: one request per launch */
+  uint32_t max_bypass_us;       /* how long later requests may overtake one that does not fit (default 2000) */
+} vg_server_config;
+
+typedef struct vg_server vg_server;
+
+typedef struct vg_request_done {
+  uint64_t ticket;
+  int32_t status;           /* VG_OK or a negative code */
+  uint32_t views;
+  uint32_t batch_requests;  /* requests in the batch it ran in */
+  uint32_t lane;
+  double queued_us;         /* submit -> launch */
+  double run_us;            /* launch -> completion seen by the worker */
+  double total_us;          /* submit -> completion */
+} vg_request_done;
+
+typedef struct vg_server_stats {
+  uint64_t hbm_budget, resident_bytes, inflight_bytes, peak_inflight_bytes;
+  uint64_t batches, requests_done, max_batch_requests_seen;
+  uint32_t queued, running;
+} vg_server_stats;
+
+int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** out);
+/* Stops the worker after the batches in flight end; queued requests are dropped. */
+void vg_server_destroy(vg_server* server);
+const char* vg_server_last_error(const vg_server* server);
+/* Makes a scene resident for serving: uploads it and allocates all its view slots; refused
+   (VG_E_NOMEM) when its bytes do not fit the budget next to the scenes already resident. */
+int vg_server_add_scene(vg_server* server, vg_scene* scene, uint32_t* scene_id);
+/* Queues a request of `count` views (their `slot` fields are ignored: the server assigns slots). */
+int vg_server_submit(vg_server* server, uint32_t scene_id, const vg_view* views, uint32_t count, uint64_t* ticket);
+/* Completed requests, oldest first; waits up to wait_us for one when none is ready. */
+int vg_server_poll(vg_server* server, vg_request_done* done, uint32_t max_done, uint32_t* count, uint32_t wait_us);
+/* Results of view `view_index` of a completed request (as vg_read). */
+int vg_server_read(vg_server* server, uint64_t ticket, uint32_t view_index, uint32_t* counters, uint32_t* draw_args,
+                   uint32_t* draw_list, uint32_t max_entries, vg_metrics* metrics);
+/* Frees a completed request's view slots. */
+int vg_server_release(vg_server* server, uint64_t ticket);
+int vg_server_stats_get(const vg_server* server, vg_server_stats* stats);
+
+/*
  * The triangle budget's threshold table: bins values lo * (hi / lo)^(b / (bins - 1)) rounded to f32 (the
  * last one exactly hi). A view's histogram bins clusters by these values (times the controller scale).
  */

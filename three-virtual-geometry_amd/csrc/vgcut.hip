@@ -13,111 +13,14 @@
 
 #include "../../include/vgcut.h"
 #include "vgcut_kernels.hip"
+#include "vgcut_host.h"
 
 using namespace vg;
-
-namespace {
-
-constexpr uint32_t kInstanceStride = 16;  // floats per caller matrix
-constexpr uint32_t kMaxMeshletsPerScene = 1u << 24;  // ids travel as f32 in the level tables and cells
-constexpr uint32_t kMaxInstancesPerScene = 1u << 24;
-
-struct DevBuf {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-};
-
-struct MeshHost {
-  bool used = false;
-  uint32_t meshlet_base = 0, meshlet_count = 0;
-  uint32_t level_base = 0, level_count = 0;  // level_base in float4 units of the mesh table
-  float bounding[4], lod_bounds[4], cull_bounds[4];
-  uint32_t instance_base = 0, instance_count = 0;
-  std::vector<float4> cell_sphere, cell_rec;
-};
-
-}  // namespace
-
-struct vg_ctx {
-  vg_config cfg;
-  hipStream_t stream = nullptr;
-  int cus = 256;
-  std::string error;
-};
-
-struct vg_scene {
-  vg_ctx* ctx = nullptr;
-  uint32_t slots = 64;
-  std::vector<MeshHost> meshes;
-  uint32_t mesh_slots = 0;  // highest used slot + 1
-  uint32_t level_used = 0;  // float4 entries of level tables
-  // host mirrors (device layout)
-  std::vector<float4> mesh_table;
-  std::vector<float4> m_cull, m_lod, m_parent;
-  std::vector<float2> m_err;
-  std::vector<uint32_t> m_tris;
-  std::vector<uint32_t> m_info;     // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
-  std::vector<uint32_t> m_run;  // LOD runs (see kMaxRun)
-  std::vector<uint2> r_span;
-  std::vector<float4> r_lod, r_parent;
-  std::vector<float2> r_err;
-  uint64_t index_used = 0;          // indices of the meshes added so far
-  std::vector<float4> i_col[4];
-  std::vector<float> i_slot;        // mesh slot of each instance
-  std::vector<float4> c_sphere, c_rec;
-  bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
-  // device scene
-  DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
-  DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
-  // per view slot
-  DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs;
-  std::vector<DevBuf> d_draws, d_hzbs;
-  std::vector<uint8_t> hzb_valid;
-  std::vector<uint4*> h_draw_ptrs;
-  std::vector<float*> h_hzb_ptrs;
-  uint32_t counter_stride = 0, args_stride = 0, hzb_floats = 0;
-  // launch scratch
-  DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_scale, d_work_counter, d_flags;
-  DevBuf d_hzb_stage;
-  uint32_t view_capacity = 0;
-  size_t cell_queue_capacity = 0;
-  uint32_t work_capacity = 0;
-  // Pinned staging ring for view records: a launch reuses a buffer only after its earlier copy ran,
-  // so back-to-back launches do not wait for each other on the host.
-  static constexpr int kRing = 16;
-  ViewDev* h_ring[kRing] = {nullptr};
-  float* h_ring_thresholds[kRing] = {nullptr};
-  hipEvent_t ring_copied[kRing] = {nullptr};
-  bool ring_pending[kRing] = {false};
-  uint32_t ring_next = 0;
-  uint64_t layout_version = 1;
-  // profiling: events around each kernel of eager launch sequences
-  bool profile = false;
-  std::vector<hipEvent_t> event_pool;
-  std::vector<std::vector<hipEvent_t>> pending;  // per sequence: kKernelSlots + 1 events
-  double kernel_ms[16] = {0};
-  uint64_t profiled_sequences = 0;
-  // triangle budget: per view slot of a launch, its threshold table and histogram
-  DevBuf d_budget_thresholds, d_budget_hist;
-  bool launch_budget = false;
-  uint64_t graph_budget = 0;
-  // footprint bitmaps (measurement only)
-  DevBuf d_fp_instances, d_fp_meshlets, d_fp_counts;
-  bool footprint = false;
-  // launch schedule of the persistent passes
-  vg_schedule schedule{6, 10, 5};
-  uint64_t graph_schedule = 0;
-  // captured launch sequence
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  uint32_t graph_views = 0;
-  uint64_t graph_version = 0;
-  uint64_t device_bytes = 0;
-};
+using namespace vgi;
 
 extern "C" int vg_budget_thresholds(float lo, float hi, uint32_t bins, float* out);
 
-namespace {
+namespace vgi {
 
 int fail(vg_ctx* ctx, int code, const std::string& message) {
   if (ctx) ctx->error = message;
@@ -150,6 +53,82 @@ int ensure(vg_scene* s, DevBuf& b, size_t bytes) {
   s->device_bytes += n;
   s->layout_version++;
   return 0;
+}
+
+// Lane scratch: the same, accounted to the lane.
+int ensure_lane(vg_ctx* ctx, Lane* lane, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return 0;
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    lane->device_bytes -= b.bytes;
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t n = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, n);
+  if (e != hipSuccess) return fail(ctx, VG_E_NOMEM, std::string("hipMalloc of ") + std::to_string(n) + " bytes failed");
+  b.bytes = n;
+  lane->device_bytes += n;
+  lane->layout_version++;
+  return 0;
+}
+
+void release_lane(Lane* lane, DevBuf& b) {
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    lane->device_bytes -= b.bytes;
+  }
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+int lane_init(vg_ctx* ctx, Lane* lane, hipStream_t stream) {
+  if (stream) {
+    lane->stream = stream;
+    lane->owns_stream = false;
+  } else {
+    VG_HIP(ctx, hipStreamCreateWithFlags(&lane->stream, hipStreamNonBlocking));
+    lane->owns_stream = true;
+  }
+  int rc = 0;
+  if ((rc = ensure_lane(ctx, lane, lane->d_cell_queue_count, 64)) || (rc = ensure_lane(ctx, lane, lane->d_work_counter, 64)) ||
+      (rc = ensure_lane(ctx, lane, lane->d_flags, 64)))
+    return rc;
+  VG_HIP(ctx, hipMemset(lane->d_flags.ptr, 0, lane->d_flags.bytes));
+  for (int r = 0; r < Lane::kRing; r++) VG_HIP(ctx, hipEventCreateWithFlags(&lane->ring_copied[r], hipEventDisableTiming));
+  return 0;
+}
+
+void lane_destroy(Lane* lane) {
+  if (lane->stream) (void)hipStreamSynchronize(lane->stream);
+  if (lane->graph_exec) (void)hipGraphExecDestroy(lane->graph_exec);
+  if (lane->graph) (void)hipGraphDestroy(lane->graph);
+  lane->graph_exec = nullptr;
+  lane->graph = nullptr;
+  DevBuf* bufs[] = {&lane->d_views, &lane->d_cell_queue_count, &lane->d_work_counter, &lane->d_flags, &lane->d_budget_thresholds,
+                    &lane->d_budget_hist};
+  for (DevBuf* b : bufs) release_lane(lane, *b);
+  if (!lane->borrowed) {
+    DevBuf* t[] = {&lane->d_cell_queue, &lane->d_work, &lane->d_work_start, &lane->d_work_scale};
+    for (DevBuf* b : t) release_lane(lane, *b);
+  }
+  for (int r = 0; r < Lane::kRing; r++) {
+    if (lane->h_ring[r]) (void)hipHostFree(lane->h_ring[r]);
+    if (lane->h_ring_thresholds[r]) (void)hipHostFree(lane->h_ring_thresholds[r]);
+    if (lane->ring_copied[r]) (void)hipEventDestroy(lane->ring_copied[r]);
+    lane->h_ring[r] = nullptr;
+    lane->h_ring_thresholds[r] = nullptr;
+    lane->ring_copied[r] = nullptr;
+  }
+  if (lane->owns_stream && lane->stream) (void)hipStreamDestroy(lane->stream);
+  lane->stream = nullptr;
+}
+
+void transient_sizes(const vg_scene* s, uint32_t count, size_t* queue_bytes, size_t* work_items) {
+  // Every (view, cell) can be queued once; every (view, instance, cut) can be a work item.
+  const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
+  *queue_bytes = sizeof(uint2) * cells * count;
+  *work_items = std::min<uint64_t>(static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64, kMaxWorkItems);
 }
 
 void release(vg_scene* s, DevBuf& b) {
@@ -314,7 +293,7 @@ int ensure_slot(vg_scene* s, uint32_t slot) {
   return 0;
 }
 
-Params make_params(vg_scene* s, uint32_t view_count) {
+Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   vg_ctx* ctx = s->ctx;
   Params p;
   std::memset(&p, 0, sizeof(p));
@@ -348,18 +327,18 @@ Params make_params(vg_scene* s, uint32_t view_count) {
   p.draws = static_cast<uint4* const*>(s->d_draw_ptrs.ptr);
   p.hzbs = static_cast<float* const*>(s->d_hzb_ptrs.ptr);
   p.metrics = static_cast<unsigned long long*>(s->d_metrics.ptr);
-  p.views = static_cast<ViewDev*>(s->d_views.ptr);
+  p.views = static_cast<ViewDev*>(lane->d_views.ptr);
   p.view_count = view_count;
-  p.cell_queue = static_cast<uint2*>(s->d_cell_queue.ptr);
-  p.cell_queue_count = static_cast<uint32_t*>(s->d_cell_queue_count.ptr);
-  p.work = static_cast<uint4*>(s->d_work.ptr);
-  p.work_start = static_cast<unsigned long long*>(s->d_work_start.ptr);
-  p.work_scale = static_cast<float*>(s->d_work_scale.ptr);
-  p.work_counter = static_cast<unsigned long long*>(s->d_work_counter.ptr);
-  p.work_capacity = s->work_capacity;
-  p.flags = static_cast<uint32_t*>(s->d_flags.ptr);
-  p.budget_thresholds = static_cast<const float*>(s->d_budget_thresholds.ptr);
-  p.budget_hist = static_cast<unsigned long long*>(s->d_budget_hist.ptr);
+  p.cell_queue = static_cast<uint2*>(lane->d_cell_queue.ptr);
+  p.cell_queue_count = static_cast<uint32_t*>(lane->d_cell_queue_count.ptr);
+  p.work = static_cast<uint4*>(lane->d_work.ptr);
+  p.work_start = static_cast<unsigned long long*>(lane->d_work_start.ptr);
+  p.work_scale = static_cast<float*>(lane->d_work_scale.ptr);
+  p.work_counter = static_cast<unsigned long long*>(lane->d_work_counter.ptr);
+  p.work_capacity = lane->work_capacity;
+  p.flags = static_cast<uint32_t*>(lane->d_flags.ptr);
+  p.budget_thresholds = static_cast<const float*>(lane->d_budget_thresholds.ptr);
+  p.budget_hist = static_cast<unsigned long long*>(lane->d_budget_hist.ptr);
   p.budget_bins = ctx->cfg.budget_bins;
   if (s->footprint) {
     p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
@@ -405,15 +384,15 @@ uint64_t schedule_key(const vg_schedule& sc) {
   return (static_cast<uint64_t>(sc.instance_blocks_per_cu) << 32) | (sc.meshlet_blocks_per_cu << 8) | sc.meshlet_waves_per_simd;
 }
 
-int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
+int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   vg_ctx* ctx = s->ctx;
-  hipStream_t st = ctx->stream;
-  Params p = make_params(s, view_count);
+  hipStream_t st = lane->stream;
+  Params p = make_params(s, lane, view_count);
   const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
   const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
   const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
   std::vector<hipEvent_t> ev;
-  const bool timed = s->profile && !capturing;
+  const bool timed = s->profile && !capturing && lane == &s->lane;
   if (timed)
     for (int i = 0; i <= kKernelSlots; i++) ev.push_back(take_event(s));
   int k = 0;
@@ -425,7 +404,7 @@ int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   mark();
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
-  if (s->launch_budget) {
+  if (lane->launch_budget) {
     hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
     hipLaunchKernelGGL(k_budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
     hipLaunchKernelGGL(k_budget_pick, dim3(view_count), dim3(64), 0, st, p);
@@ -454,7 +433,27 @@ int enqueue_cut(vg_scene* s, uint32_t view_count, bool capturing = false) {
   return 0;
 }
 
-int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
+int lane_reserve_views(vg_ctx* ctx, Lane* lane) {
+  VG_HIP(ctx, hipStreamSynchronize(lane->stream));
+  if (int rc = ensure_lane(ctx, lane, lane->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
+  for (int r = 0; r < Lane::kRing; r++) {
+    if (lane->h_ring[r]) (void)hipHostFree(lane->h_ring[r]);
+    if (lane->h_ring_thresholds[r]) (void)hipHostFree(lane->h_ring_thresholds[r]);
+    lane->h_ring[r] = nullptr;
+    lane->h_ring_thresholds[r] = nullptr;
+    lane->ring_pending[r] = false;
+    VG_HIP(ctx, hipHostMalloc(&lane->h_ring[r], sizeof(ViewDev) * ctx->cfg.max_views, hipHostMallocDefault));
+    VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&lane->h_ring_thresholds[r]),
+                              sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins, hipHostMallocDefault));
+  }
+  if (int rc = ensure_lane(ctx, lane, lane->d_budget_thresholds, sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins)) return rc;
+  if (int rc = ensure_lane(ctx, lane, lane->d_budget_hist, sizeof(unsigned long long) * ctx->cfg.max_views * (ctx->cfg.budget_bins + 1)))
+    return rc;
+  lane->view_capacity = ctx->cfg.max_views;
+  return 0;
+}
+
+int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, bool fresh) {
   vg_ctx* ctx = s->ctx;
   if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_cut: host-only context (device < 0)");
   if (!views || count == 0) return fail(ctx, VG_E_INVALID, "vg_cut: no views");
@@ -471,42 +470,31 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   if (int rc = upload_scene(s)) return rc;
   for (uint32_t i = 0; i < count; i++)
     if (int rc = ensure_slot(s, views[i].slot)) return rc;
-  // Scratch: every (view, cell) can be queued once; every (view, instance, cut) can be a work item.
-  const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
-  if (s->cell_queue_capacity < cells * count) {
-    if (int rc = ensure(s, s->d_cell_queue, sizeof(uint2) * cells * count)) return rc;
-    s->cell_queue_capacity = cells * count;
-  }
-  const uint64_t want = std::min<uint64_t>(static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64, kMaxWorkItems);
-  if (s->work_capacity < want) {
-    if (int rc = ensure(s, s->d_work, sizeof(uint4) * want)) return rc;
-    if (int rc = ensure(s, s->d_work_start, sizeof(unsigned long long) * want)) return rc;
-    if (int rc = ensure(s, s->d_work_scale, sizeof(float) * want)) return rc;
-    s->work_capacity = static_cast<uint32_t>(want);
-  }
-  if (s->view_capacity < count) {
-    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (int rc = ensure(s, s->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
-    for (int r = 0; r < vg_scene::kRing; r++) {
-      if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
-      if (s->h_ring_thresholds[r]) (void)hipHostFree(s->h_ring_thresholds[r]);
-      s->h_ring[r] = nullptr;
-      s->h_ring_thresholds[r] = nullptr;
-      s->ring_pending[r] = false;
-      VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_ring[r]), sizeof(ViewDev) * ctx->cfg.max_views, hipHostMallocDefault));
-      VG_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&s->h_ring_thresholds[r]),
-                                sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins, hipHostMallocDefault));
+  size_t queue_bytes = 0, want = 0;
+  transient_sizes(s, count, &queue_bytes, &want);
+  if (lane->borrowed) {
+    if (lane->d_cell_queue.bytes < queue_bytes || lane->work_capacity < want)
+      return fail(ctx, VG_E_INVALID, "vg_cut: lane buffers smaller than the launch needs");
+  } else {
+    if (lane->cell_queue_capacity < queue_bytes / sizeof(uint2)) {
+      if (int rc = ensure_lane(ctx, lane, lane->d_cell_queue, queue_bytes)) return rc;
+      lane->cell_queue_capacity = queue_bytes / sizeof(uint2);
     }
-    if (int rc = ensure(s, s->d_budget_thresholds, sizeof(float) * ctx->cfg.max_views * ctx->cfg.budget_bins)) return rc;
-    if (int rc = ensure(s, s->d_budget_hist, sizeof(unsigned long long) * ctx->cfg.max_views * (ctx->cfg.budget_bins + 1))) return rc;
-    s->view_capacity = ctx->cfg.max_views;
+    if (lane->work_capacity < want) {
+      if (int rc = ensure_lane(ctx, lane, lane->d_work, sizeof(uint4) * want)) return rc;
+      if (int rc = ensure_lane(ctx, lane, lane->d_work_start, sizeof(unsigned long long) * want)) return rc;
+      if (int rc = ensure_lane(ctx, lane, lane->d_work_scale, sizeof(float) * want)) return rc;
+      lane->work_capacity = static_cast<uint32_t>(want);
+    }
   }
-  const int ring = static_cast<int>(s->ring_next++ % vg_scene::kRing);
-  if (s->ring_pending[ring]) {
-    VG_HIP(ctx, hipEventSynchronize(s->ring_copied[ring]));
-    s->ring_pending[ring] = false;
+  if (lane->view_capacity < count)
+    if (int rc = lane_reserve_views(ctx, lane)) return rc;
+  const int ring = static_cast<int>(lane->ring_next++ % Lane::kRing);
+  if (lane->ring_pending[ring]) {
+    VG_HIP(ctx, hipEventSynchronize(lane->ring_copied[ring]));
+    lane->ring_pending[ring] = false;
   }
-  ViewDev* staging = s->h_ring[ring];
+  ViewDev* staging = static_cast<ViewDev*>(lane->h_ring[ring]);
   for (uint32_t i = 0; i < count; i++) {
     const vg_view& in = views[i];
     ViewDev& v = staging[i];
@@ -530,24 +518,92 @@ int prepare_cut(vg_scene* s, const vg_view* views, uint32_t count) {
     v.range_lo = in.threshold_range[0];
     v.range_hi = in.threshold_range[1];
     v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
+    v.fresh = fresh ? 1u : 0u;
   }
-  s->launch_budget = false;
+  lane->launch_budget = false;
   for (uint32_t i = 0; i < count; i++)
     if (views[i].triangle_budget) {
-      s->launch_budget = true;
+      lane->launch_budget = true;
       vg_budget_thresholds(views[i].threshold_range[0], views[i].threshold_range[1], ctx->cfg.budget_bins,
-                           s->h_ring_thresholds[ring] + static_cast<size_t>(i) * ctx->cfg.budget_bins);
+                           lane->h_ring_thresholds[ring] + static_cast<size_t>(i) * ctx->cfg.budget_bins);
     }
-  if (s->launch_budget)
-    VG_HIP(ctx, hipMemcpyAsync(s->d_budget_thresholds.ptr, s->h_ring_thresholds[ring], sizeof(float) * count * ctx->cfg.budget_bins,
-                               hipMemcpyHostToDevice, ctx->stream));
-  VG_HIP(ctx, hipMemcpyAsync(s->d_views.ptr, staging, sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
-  VG_HIP(ctx, hipEventRecord(s->ring_copied[ring], ctx->stream));
-  s->ring_pending[ring] = true;
+  if (lane->launch_budget)
+    VG_HIP(ctx, hipMemcpyAsync(lane->d_budget_thresholds.ptr, lane->h_ring_thresholds[ring], sizeof(float) * count * ctx->cfg.budget_bins,
+                               hipMemcpyHostToDevice, lane->stream));
+  VG_HIP(ctx, hipMemcpyAsync(lane->d_views.ptr, staging, sizeof(ViewDev) * count, hipMemcpyHostToDevice, lane->stream));
+  VG_HIP(ctx, hipEventRecord(lane->ring_copied[ring], lane->stream));
+  lane->ring_pending[ring] = true;
   return 0;
 }
 
-}  // namespace
+int launch_graph(vg_scene* s, Lane* lane, uint32_t count) {
+  vg_ctx* ctx = s->ctx;
+  const uint64_t key[4] = {s->layout_version, lane->layout_version, static_cast<uint64_t>(lane->launch_budget), schedule_key(s->schedule)};
+  if (!lane->graph_exec || lane->graph_views != count || std::memcmp(key, lane->graph_key, sizeof(key)) != 0) {
+    if (lane->graph_exec) (void)hipGraphExecDestroy(lane->graph_exec);
+    if (lane->graph) (void)hipGraphDestroy(lane->graph);
+    lane->graph_exec = nullptr;
+    lane->graph = nullptr;
+    VG_HIP(ctx, hipStreamBeginCapture(lane->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_cut(s, lane, count, true);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(lane->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    lane->graph = graph;
+    VG_HIP(ctx, hipGraphInstantiate(&lane->graph_exec, graph, nullptr, nullptr, 0));
+    lane->graph_views = count;
+    std::memcpy(lane->graph_key, key, sizeof(key));
+  }
+  VG_HIP(ctx, hipGraphLaunch(lane->graph_exec, lane->stream));
+  return VG_OK;
+}
+
+int read_slot(vg_scene* s, hipStream_t stream, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list,
+              uint32_t max_entries, vg_metrics* metrics) {
+  vg_ctx* ctx = s->ctx;
+  if (counters)
+    VG_HIP(ctx, hipMemcpyAsync(counters, static_cast<uint32_t*>(s->d_counters.ptr) + static_cast<size_t>(view_slot) * s->counter_stride,
+                               sizeof(uint32_t) * s->counter_stride, hipMemcpyDeviceToHost, stream));
+  if (draw_args)
+    VG_HIP(ctx, hipMemcpyAsync(draw_args, static_cast<uint32_t*>(s->d_args.ptr) + static_cast<size_t>(view_slot) * s->args_stride,
+                               sizeof(uint32_t) * s->args_stride, hipMemcpyDeviceToHost, stream));
+  if (draw_list && max_entries) {
+    const uint32_t n = std::min(max_entries, ctx->cfg.max_meshlets);
+    if (!s->d_draws[view_slot].ptr) {
+      std::memset(draw_list, 0, sizeof(uint4) * 2 * static_cast<size_t>(max_entries));
+    } else {
+      const uint4* d = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
+      VG_HIP(ctx, hipMemcpyAsync(draw_list, d, sizeof(uint4) * n, hipMemcpyDeviceToHost, stream));
+      VG_HIP(ctx, hipMemcpyAsync(draw_list + static_cast<size_t>(max_entries) * 4, d + ctx->cfg.max_meshlets, sizeof(uint4) * n,
+                                 hipMemcpyDeviceToHost, stream));
+    }
+  }
+  unsigned long long m[kMetricWords];
+  if (metrics)
+    VG_HIP(ctx, hipMemcpyAsync(m, static_cast<unsigned long long*>(s->d_metrics.ptr) + static_cast<size_t>(view_slot) * kMetricWords,
+                               sizeof(m), hipMemcpyDeviceToHost, stream));
+  VG_HIP(ctx, hipStreamSynchronize(stream));
+  if (metrics) {
+    std::memset(metrics, 0, sizeof(*metrics));
+    metrics->cells_tested = m[M_CELLS];
+    metrics->instances_tested = m[M_INSTANCES];
+    metrics->work_items[0] = m[M_WORK0];
+    metrics->work_items[1] = m[M_WORK1];
+    metrics->clusters_tested[0] = m[M_CLUSTERS0];
+    metrics->clusters_tested[1] = m[M_CLUSTERS1];
+    metrics->occlusion_tests = m[M_OCCLUSION];
+    float t;
+    const uint32_t bits = static_cast<uint32_t>(m[M_THRESHOLD]);
+    std::memcpy(&t, &bits, 4);
+    metrics->error_threshold = t;
+    metrics->budget_bin = static_cast<uint32_t>(m[M_BUDGET_BIN]);
+    metrics->budget_cluster_tests = m[M_BUDGET_TESTS];
+  }
+  return VG_OK;
+}
+
+}  // namespace vgi
 
 // ------------------------------------------------------------------ C ABI
 
@@ -628,7 +684,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
         (rc = ensure(s, s->d_args, sizeof(uint32_t) * s->args_stride * views)) ||
         (rc = ensure(s, s->d_metrics, sizeof(unsigned long long) * kMetricWords * views)) ||
         (rc = ensure(s, s->d_draw_ptrs, sizeof(uint4*) * views)) || (rc = ensure(s, s->d_hzb_ptrs, sizeof(float*) * views)) ||
-        (rc = ensure(s, s->d_cell_queue_count, 64)) || (rc = ensure(s, s->d_work_counter, 64)) || (rc = ensure(s, s->d_flags, 64)))
+        (rc = lane_init(ctx, &s->lane, ctx->stream)))
       return rc;
     // Controller state: scale 1 for both cuts (GeometryPool.ts:602-603), everything else 0.
     std::vector<uint32_t> counters(static_cast<size_t>(s->counter_stride) * views, 0u);
@@ -641,8 +697,6 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
     VG_HIP(ctx, hipMemset(s->d_metrics.ptr, 0, s->d_metrics.bytes));
     VG_HIP(ctx, hipMemset(s->d_draw_ptrs.ptr, 0, s->d_draw_ptrs.bytes));
     VG_HIP(ctx, hipMemset(s->d_hzb_ptrs.ptr, 0, s->d_hzb_ptrs.bytes));
-    VG_HIP(ctx, hipMemset(s->d_flags.ptr, 0, s->d_flags.bytes));
-    for (int r = 0; r < vg_scene::kRing; r++) VG_HIP(ctx, hipEventCreateWithFlags(&s->ring_copied[r], hipEventDisableTiming));
     return 0;
   };
   if (device && (rc = init())) {
@@ -656,21 +710,13 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
 void vg_scene_destroy(vg_scene* s) {
   if (!s) return;
   if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
+  lane_destroy(&s->lane);
   DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
-                    &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_views, &s->d_cell_queue, &s->d_cell_queue_count, &s->d_work,
-                    &s->d_work_start, &s->d_work_scale, &s->d_work_counter, &s->d_flags, &s->d_hzb_stage, &s->d_fp_instances,
-                    &s->d_fp_meshlets, &s->d_fp_counts, &s->d_budget_thresholds, &s->d_budget_hist};
+                    &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_hzbs) release(s, b);
-  for (int r = 0; r < vg_scene::kRing; r++) {
-    if (s->h_ring[r]) (void)hipHostFree(s->h_ring[r]);
-    if (s->h_ring_thresholds[r]) (void)hipHostFree(s->h_ring_thresholds[r]);
-    if (s->ring_copied[r]) (void)hipEventDestroy(s->ring_copied[r]);
-  }
   for (auto& seq : s->pending)
     for (hipEvent_t e : seq)
       if (e) (void)hipEventDestroy(e);
@@ -678,7 +724,7 @@ void vg_scene_destroy(vg_scene* s) {
   delete s;
 }
 
-uint64_t vg_scene_bytes(const vg_scene* s) { return s ? s->device_bytes : 0; }
+uint64_t vg_scene_bytes(const vg_scene* s) { return s ? s->device_bytes + s->lane.device_bytes : 0; }
 
 int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
   if (!s || !mesh || !slot_out) return VG_E_INVALID;
@@ -891,35 +937,14 @@ int vg_hzb_invalidate(vg_scene* s, uint32_t view_slot) {
 
 int vg_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   if (!s) return VG_E_INVALID;
-  if (int rc = prepare_cut(s, views, count)) return rc;
-  return enqueue_cut(s, count);
+  if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
+  return enqueue_cut(s, &s->lane, count, false);
 }
 
 int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
   if (!s) return VG_E_INVALID;
-  vg_ctx* ctx = s->ctx;
-  if (int rc = prepare_cut(s, views, count)) return rc;
-  if (!s->graph_exec || s->graph_views != count || s->graph_version != s->layout_version ||
-      s->graph_budget != static_cast<uint64_t>(s->launch_budget) || s->graph_schedule != schedule_key(s->schedule)) {
-    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    s->graph_exec = nullptr;
-    s->graph = nullptr;
-    VG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_cut(s, count, true);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc) return rc;
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
-    s->graph = graph;
-    VG_HIP(ctx, hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
-    s->graph_views = count;
-    s->graph_version = s->layout_version;
-    s->graph_budget = s->launch_budget;
-    s->graph_schedule = schedule_key(s->schedule);
-  }
-  VG_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
-  return VG_OK;
+  if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
+  return launch_graph(s, &s->lane, count);
 }
 
 int vg_sync(vg_scene* s) {
@@ -927,9 +952,9 @@ int vg_sync(vg_scene* s) {
   vg_ctx* ctx = s->ctx;
   if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
   VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (bool& p : s->ring_pending) p = false;
+  for (bool& p : s->lane.ring_pending) p = false;
   uint32_t flags[4] = {0, 0, 0, 0};
-  VG_HIP(ctx, hipMemcpy(flags, s->d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
+  VG_HIP(ctx, hipMemcpy(flags, s->lane.d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[3]) return fail(ctx, VG_E_CAPACITY, "vg_cut: work list capacity exceeded; results are incomplete");
   if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut: meshlet pass lost its work item (internal error)");
   return VG_OK;
@@ -940,44 +965,7 @@ int vg_read(vg_scene* s, uint32_t view_slot, uint32_t* counters, uint32_t* draw_
   if (!s || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
   if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
-  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (counters)
-    VG_HIP(ctx, hipMemcpy(counters, static_cast<uint32_t*>(s->d_counters.ptr) + static_cast<size_t>(view_slot) * s->counter_stride,
-                          sizeof(uint32_t) * s->counter_stride, hipMemcpyDeviceToHost));
-  if (draw_args)
-    VG_HIP(ctx, hipMemcpy(draw_args, static_cast<uint32_t*>(s->d_args.ptr) + static_cast<size_t>(view_slot) * s->args_stride,
-                          sizeof(uint32_t) * s->args_stride, hipMemcpyDeviceToHost));
-  if (draw_list && max_entries) {
-    const uint32_t n = std::min(max_entries, ctx->cfg.max_meshlets);
-    if (!s->d_draws[view_slot].ptr) {
-      std::memset(draw_list, 0, sizeof(uint4) * 2 * static_cast<size_t>(max_entries));
-    } else {
-      const uint4* d = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
-      VG_HIP(ctx, hipMemcpy(draw_list, d, sizeof(uint4) * n, hipMemcpyDeviceToHost));
-      VG_HIP(ctx, hipMemcpy(draw_list + static_cast<size_t>(max_entries) * 4, d + ctx->cfg.max_meshlets, sizeof(uint4) * n,
-                            hipMemcpyDeviceToHost));
-    }
-  }
-  if (metrics) {
-    unsigned long long m[kMetricWords];
-    VG_HIP(ctx, hipMemcpy(m, static_cast<unsigned long long*>(s->d_metrics.ptr) + static_cast<size_t>(view_slot) * kMetricWords,
-                          sizeof(m), hipMemcpyDeviceToHost));
-    std::memset(metrics, 0, sizeof(*metrics));
-    metrics->cells_tested = m[M_CELLS];
-    metrics->instances_tested = m[M_INSTANCES];
-    metrics->work_items[0] = m[M_WORK0];
-    metrics->work_items[1] = m[M_WORK1];
-    metrics->clusters_tested[0] = m[M_CLUSTERS0];
-    metrics->clusters_tested[1] = m[M_CLUSTERS1];
-    metrics->occlusion_tests = m[M_OCCLUSION];
-    float t;
-    const uint32_t bits = static_cast<uint32_t>(m[M_THRESHOLD]);
-    std::memcpy(&t, &bits, 4);
-    metrics->error_threshold = t;
-    metrics->budget_bin = static_cast<uint32_t>(m[M_BUDGET_BIN]);
-    metrics->budget_cluster_tests = m[M_BUDGET_TESTS];
-  }
-  return VG_OK;
+  return read_slot(s, ctx->stream, view_slot, counters, draw_args, draw_list, max_entries, metrics);
 }
 
 int vg_state_get(vg_scene* s, uint32_t view_slot, uint32_t* state) {
@@ -1108,7 +1096,7 @@ int vg_profile_read(vg_scene* s, double* kernel_ms, uint32_t max_kernels, uint64
 int vg_footprint(vg_scene* s, const vg_view* views, uint32_t count, uint64_t* instances, uint64_t* meshlets) {
   if (!s) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
-  if (int rc = prepare_cut(s, views, count)) return rc;
+  if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
   const size_t iw = (s->i_col[0].size() + 31) / 32 + 1, mw = (s->m_cull.size() + 31) / 32 + 1;
   int rc = 0;
   if ((rc = ensure(s, s->d_fp_instances, iw * 4)) || (rc = ensure(s, s->d_fp_meshlets, mw * 4)) || (rc = ensure(s, s->d_fp_counts, 16)))
@@ -1117,7 +1105,7 @@ int vg_footprint(vg_scene* s, const vg_view* views, uint32_t count, uint64_t* in
   VG_HIP(ctx, hipMemsetAsync(s->d_fp_meshlets.ptr, 0, mw * 4, ctx->stream));
   VG_HIP(ctx, hipMemsetAsync(s->d_fp_counts.ptr, 0, 16, ctx->stream));
   s->footprint = true;
-  rc = enqueue_cut(s, count);
+  rc = enqueue_cut(s, &s->lane, count, false);
   s->footprint = false;
   if (rc) return rc;
   auto* counts = static_cast<unsigned long long*>(s->d_fp_counts.ptr);
@@ -1163,7 +1151,7 @@ int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats,
   if (!s) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
   if (repeats == 0 || repeats > 64) return fail(ctx, VG_E_INVALID, "vg_tune: repeats must be 1-64");
-  if (int rc = prepare_cut(s, views, count)) return rc;
+  if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
   VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint8_t> saved(s->d_counters.bytes);
   VG_HIP(ctx, hipMemcpy(saved.data(), s->d_counters.ptr, saved.size(), hipMemcpyDeviceToHost));
@@ -1187,9 +1175,9 @@ int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats,
     s->schedule = sc;
     std::vector<float> t;
     for (uint32_t i = 0; i <= repeats; i++) {
-      if (int r = prepare_cut(s, views, count)) return r;
+      if (int r = prepare_cut(s, &s->lane, views, count, false)) return r;
       if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail(ctx, VG_E_DEVICE, "vg_tune: hipEventRecord");
-      if (int r = enqueue_cut(s, count)) return r;
+      if (int r = enqueue_cut(s, &s->lane, count, false)) return r;
       if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
         return fail(ctx, VG_E_DEVICE, "vg_tune: event");
       float e = 0;

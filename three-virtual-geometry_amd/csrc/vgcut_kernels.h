@@ -65,7 +65,8 @@ struct ViewDev {
   float lod_blend_max, lod_scale_limit, min_pixel_radius, max_error_threshold;
   uint32_t frustum, occlusion, slot, budget;
   float range_lo, range_hi;
-  uint32_t hzb_valid, pad0;
+  uint32_t hzb_valid;
+  uint32_t fresh;  // 1: the slot's controller state starts at rest this launch (scale 1, as on a new slot)
   // derived
   float low[2], high[2];
   float fade_scale;

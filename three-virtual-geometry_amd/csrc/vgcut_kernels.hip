@@ -146,7 +146,9 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   if (threadIdx.x < kMetricWords) *metric_at(p, v.slot, threadIdx.x) = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t* state = state_at(p, v.slot);
+    uint32_t* state = state_at(p, v.slot);
+    if (v.fresh)  // controller at rest, as GeometryPool.ts:602-603 creates it
+      for (uint32_t i = 0; i < kStateWords; i++) state[i] = i == kStateScale || i == kStateScale + 1 ? 10000u : 0u;
     load_bands(v, state);
     v.redo[0] = v.redo[1] = 0;
     v.budget_bin = 0;

@@ -1,0 +1,479 @@
+// Request server behind vg_server_* (include/vgcut.h): continuous batching of cut requests over
+// resident scenes. One worker thread owns the launch side: it completes batches whose event has
+// fired, then launches new batches on free lanes, each a group of queued requests of one scene that
+// fits the view limit, the scene's free view slots and the HBM budget. Client threads only touch the
+// queue and the completed list under the mutex; reads of finished requests use a stream of their own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "vgcut_host.h"
+
+using namespace vgi;
+
+namespace {
+
+double now_us() {
+  return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t scene = 0;
+  std::vector<vg_view> views;
+  std::vector<uint32_t> slots;  // assigned view slots, one per view
+  double t_submit = 0, t_start = 0, t_done = 0;
+  int status = VG_OK;
+  uint32_t batch_requests = 0, lane = 0;
+  bool done = false;
+};
+
+struct SceneEntry {
+  vg_scene* scene = nullptr;
+  uint64_t resident = 0;
+  std::vector<uint32_t> free_slots;
+};
+
+struct Batch {
+  uint32_t scene = 0;
+  uint32_t lane = 0;
+  uint32_t views = 0;
+  std::vector<Request*> requests;
+  uint64_t bytes = 0;
+};
+
+// Transient bytes of a launch of `views` views of a scene: the (view, cell) queue and the work list
+// (uint4 item + u64 first task + f32 scale per entry).
+uint64_t transient_bytes(const vg_scene* s, uint32_t views) {
+  size_t queue_bytes = 0, work_items = 0;
+  transient_sizes(s, views, &queue_bytes, &work_items);
+  return queue_bytes + work_items * (sizeof(uint4) + sizeof(unsigned long long) + sizeof(float));
+}
+
+}  // namespace
+
+struct vg_server {
+  vg_ctx* ctx = nullptr;
+  vg_server_config cfg{};
+  std::vector<SceneEntry> scenes;
+  std::vector<std::unique_ptr<Lane>> lanes;
+  std::vector<hipEvent_t> lane_done;
+  std::vector<uint32_t*> lane_flags;  // pinned: the lane's 4 flag words after its batch
+  std::vector<int> lane_batch;        // index into `running` or -1
+  hipMemPool_t pool = nullptr;
+  hipStream_t read_stream = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+  std::deque<uint64_t> completed;
+  std::vector<Batch> running;
+  uint64_t resident = 0, inflight = 0, peak_inflight = 0, batches = 0, requests_done = 0, max_batch_seen = 0;
+  uint64_t next_ticket = 1;
+  uint64_t generation = 0;  // bumped by submit and release: the worker waits for it when it cannot progress
+  bool stop = false;
+  std::thread worker;
+  std::string error;
+
+  void set_error(const std::string& e) { error = e; }
+
+  // Marks the batches whose event fired as done. Called with the mutex held.
+  void complete() {
+    for (size_t i = 0; i < running.size();) {
+      Batch& b = running[i];
+      const hipError_t q = hipEventQuery(lane_done[b.lane]);
+      if (q == hipErrorNotReady) {
+        i++;
+        continue;
+      }
+      int status = VG_OK;
+      if (q != hipSuccess) {
+        status = VG_E_DEVICE;
+        set_error(std::string("batch failed on the device: ") + hipGetErrorString(q));
+      } else if (lane_flags[b.lane][3]) {
+        status = VG_E_CAPACITY;
+      } else if (lane_flags[b.lane][2]) {
+        status = VG_E_DEVICE;
+      }
+      const double t = now_us();
+      for (Request* r : b.requests) {
+        r->t_done = t;
+        r->status = status;
+        r->done = true;
+        completed.push_back(r->ticket);
+        requests_done++;
+      }
+      inflight -= b.bytes;
+      lane_batch[b.lane] = -1;
+      running.erase(running.begin() + static_cast<std::ptrdiff_t>(i));
+      for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
+      cv_done.notify_all();
+    }
+  }
+
+  int free_lane() const {
+    for (size_t l = 0; l < lane_batch.size(); l++)
+      if (lane_batch[l] < 0) return static_cast<int>(l);
+    return -1;
+  }
+
+  // Picks the next batch from the queue (FIFO, one scene, fits views, slots and memory). With the mutex held.
+  bool pick(Batch& b) {
+    const uint64_t budget_left = cfg.hbm_budget > resident + inflight ? cfg.hbm_budget - resident - inflight : 0;
+    const double t = now_us();
+    std::vector<uint32_t> slots_left(scenes.size());
+    for (size_t k = 0; k < scenes.size(); k++) slots_left[k] = static_cast<uint32_t>(scenes[k].free_slots.size());
+    bool have = false;
+    for (size_t i = 0; i < queue.size(); i++) {
+      Request* r = queue[i];
+      const SceneEntry& se = scenes[r->scene];
+      const uint32_t n = static_cast<uint32_t>(r->views.size());
+      if (!have) {
+        if (n <= slots_left[r->scene] && n <= cfg.max_batch_views && transient_bytes(se.scene, n) <= budget_left) {
+          b.scene = r->scene;
+          b.views = n;
+          b.requests.push_back(r);
+          have = true;
+        } else if (i == 0 && t - r->t_submit > cfg.max_bypass_us) {
+          return false;  // the oldest request has waited long enough: nothing overtakes it
+        }
+        continue;
+      }
+      if (r->scene != b.scene) continue;
+      if (cfg.max_batch_requests && b.requests.size() >= cfg.max_batch_requests) break;
+      if (b.views + n > cfg.max_batch_views || b.views + n > slots_left[b.scene]) continue;
+      if (transient_bytes(se.scene, b.views + n) > budget_left) continue;
+      b.views += n;
+      b.requests.push_back(r);
+    }
+    return have;
+  }
+
+  // Launches a picked batch on a lane. With the mutex held.
+  int launch(Batch& b, int lane_index) {
+    SceneEntry& se = scenes[b.scene];
+    vg_scene* s = se.scene;
+    Lane* lane = lanes[lane_index].get();
+    std::vector<vg_view> views;
+    views.reserve(b.views);
+    const double t = now_us();
+    for (Request* r : b.requests) {
+      r->slots.clear();
+      for (const vg_view& v : r->views) {
+        const uint32_t slot = se.free_slots.back();
+        se.free_slots.pop_back();
+        r->slots.push_back(slot);
+        views.push_back(v);
+        views.back().slot = slot;
+      }
+      r->t_start = t;
+      r->batch_requests = static_cast<uint32_t>(b.requests.size());
+      r->lane = static_cast<uint32_t>(lane_index);
+      queue.erase(std::find(queue.begin(), queue.end(), r));
+    }
+    size_t queue_bytes = 0, work_items = 0;
+    transient_sizes(s, b.views, &queue_bytes, &work_items);
+    b.bytes = transient_bytes(s, b.views);
+    b.lane = static_cast<uint32_t>(lane_index);
+    void* ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
+    const size_t sizes[4] = {queue_bytes, sizeof(uint4) * work_items, sizeof(unsigned long long) * work_items, sizeof(float) * work_items};
+    int rc = VG_OK;
+    for (int k = 0; k < 4 && rc == VG_OK; k++)
+      if (hipMallocFromPoolAsync(&ptrs[k], sizes[k], pool, lane->stream) != hipSuccess) rc = VG_E_NOMEM;
+    if (rc == VG_OK) {
+      lane->borrowed = true;
+      lane->d_cell_queue = DevBuf{ptrs[0], sizes[0]};
+      lane->d_work = DevBuf{ptrs[1], sizes[1]};
+      lane->d_work_start = DevBuf{ptrs[2], sizes[2]};
+      lane->d_work_scale = DevBuf{ptrs[3], sizes[3]};
+      lane->cell_queue_capacity = queue_bytes / sizeof(uint2);
+      lane->work_capacity = static_cast<uint32_t>(work_items);
+      rc = prepare_cut(s, lane, views.data(), b.views, true);
+      if (rc == VG_OK) rc = enqueue_cut(s, lane, b.views, false);
+      if (rc == VG_OK && hipMemcpyAsync(lane_flags[lane_index], lane->d_flags.ptr, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                        lane->stream) != hipSuccess)
+        rc = VG_E_DEVICE;
+    }
+    for (int k = 0; k < 4; k++)
+      if (ptrs[k]) (void)hipFreeAsync(ptrs[k], lane->stream);
+    lane->d_cell_queue = lane->d_work = lane->d_work_start = lane->d_work_scale = DevBuf{};
+    lane->work_capacity = 0;
+    lane->cell_queue_capacity = 0;
+    if (rc == VG_OK && hipEventRecord(lane_done[lane_index], lane->stream) != hipSuccess) rc = VG_E_DEVICE;
+    if (rc != VG_OK) {
+      set_error("batch launch failed: " + ctx->error);
+      (void)hipStreamSynchronize(lane->stream);
+      const double t_fail = now_us();
+      for (Request* r : b.requests) {
+        r->status = rc;
+        r->done = true;
+        r->t_done = t_fail;
+        completed.push_back(r->ticket);
+        requests_done++;
+      }
+      cv_done.notify_all();
+      return rc;
+    }
+    inflight += b.bytes;
+    peak_inflight = std::max(peak_inflight, inflight);
+    batches++;
+    max_batch_seen = std::max<uint64_t>(max_batch_seen, b.requests.size());
+    running.push_back(std::move(b));
+    lane_batch[lane_index] = static_cast<int>(running.size() - 1);
+    return VG_OK;
+  }
+
+  void run() {
+    if (hipSetDevice(ctx->cfg.device) != hipSuccess) {
+      std::lock_guard<std::mutex> lk(mu);
+      set_error("hipSetDevice failed in the server worker");
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    uint64_t seen = generation;
+    while (!stop) {
+      complete();
+      bool launched = false;
+      for (int lane = free_lane(); lane >= 0 && !queue.empty(); lane = free_lane()) {
+        Batch b;
+        if (!pick(b)) break;
+        launch(b, lane);
+        launched = true;
+      }
+      if (running.empty() && !launched) {
+        // Nothing on the device and nothing launchable (empty queue, or every queued request waits for
+        // view slots its client has not released): sleep until a submit or a release changes that.
+        cv_work.wait(lk, [&] { return stop || generation != seen; });
+        seen = generation;
+        continue;
+      }
+      // Batches in flight: poll their events, letting clients in between.
+      lk.unlock();
+      std::this_thread::yield();
+      lk.lock();
+    }
+  }
+};
+
+extern "C" {
+
+int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** out) {
+  if (!ctx || !out) return VG_E_INVALID;
+  *out = nullptr;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_server_create: host-only context (device < 0)");
+  vg_server_config c{};
+  if (config) c = *config;
+  if (!c.lanes) c.lanes = 4;
+  if (!c.max_batch_views) c.max_batch_views = ctx->cfg.max_views;
+  if (!c.max_bypass_us) c.max_bypass_us = 2000;
+  if (c.lanes > 16 || c.max_batch_views > ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_server_create: lanes 1-16, max_batch_views <= max_views");
+  if (!c.hbm_budget) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(ctx, VG_E_DEVICE, "vg_server_create: hipMemGetInfo");
+    c.hbm_budget = static_cast<uint64_t>(0.9 * static_cast<double>(total_b));
+  }
+  auto* sv = new (std::nothrow) vg_server();
+  if (!sv) return fail(ctx, VG_E_NOMEM, "out of host memory");
+  sv->ctx = ctx;
+  sv->cfg = c;
+  int rc = VG_OK;
+  auto init = [&]() -> int {
+    if (hipDeviceGetDefaultMemPool(&sv->pool, ctx->cfg.device) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipDeviceGetDefaultMemPool");
+    uint64_t keep = UINT64_MAX;  // keep freed transient memory reserved in the pool between batches
+    if (hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+      return fail(ctx, VG_E_DEVICE, "hipMemPoolSetAttribute");
+    // Memory a batch frees is reused by later batches of the same lane (stream order makes that safe);
+    // no reuse across lanes, where it would rest on the allocator's own cross-stream tracking.
+    int no = 0;
+    if (hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowOpportunistic, &no) != hipSuccess ||
+        hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowInternalDependencies, &no) != hipSuccess)
+      return fail(ctx, VG_E_DEVICE, "hipMemPoolSetAttribute (reuse)");
+    if (hipStreamCreateWithFlags(&sv->read_stream, hipStreamNonBlocking) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipStreamCreate");
+    for (uint32_t l = 0; l < c.lanes; l++) {
+      sv->lanes.emplace_back(new Lane());
+      if (int r = lane_init(ctx, sv->lanes.back().get(), nullptr)) return r;
+      if (int r = lane_reserve_views(ctx, sv->lanes.back().get())) return r;
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipEventCreate");
+      sv->lane_done.push_back(e);
+      uint32_t* flags = nullptr;
+      if (hipHostMalloc(reinterpret_cast<void**>(&flags), 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        return fail(ctx, VG_E_NOMEM, "hipHostMalloc");
+      std::memset(flags, 0, 4 * sizeof(uint32_t));
+      sv->lane_flags.push_back(flags);
+      sv->lane_batch.push_back(-1);
+    }
+    return VG_OK;
+  };
+  if ((rc = init())) {
+    vg_server_destroy(sv);
+    return rc;
+  }
+  sv->worker = std::thread([sv] { sv->run(); });
+  *out = sv;
+  return VG_OK;
+}
+
+void vg_server_destroy(vg_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+    sv->generation++;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto& lane : sv->lanes) {
+    if (lane->stream) (void)hipStreamSynchronize(lane->stream);
+    lane->borrowed = true;  // its transient buffers went back to the pool with each batch
+    lane_destroy(lane.get());
+  }
+  for (hipEvent_t e : sv->lane_done) (void)hipEventDestroy(e);
+  for (uint32_t* f : sv->lane_flags) (void)hipHostFree(f);
+  if (sv->read_stream) (void)hipStreamDestroy(sv->read_stream);
+  delete sv;
+}
+
+const char* vg_server_last_error(const vg_server* sv) { return sv ? sv->error.c_str() : "no server"; }
+
+int vg_server_add_scene(vg_server* sv, vg_scene* s, uint32_t* scene_id) {
+  if (!sv || !s || !scene_id || s->ctx != sv->ctx) return VG_E_INVALID;
+  vg_ctx* ctx = sv->ctx;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (int rc = upload_scene(s)) return rc;
+  for (uint32_t slot = 0; slot < ctx->cfg.max_views; slot++)
+    if (int rc = ensure_slot(s, slot)) return rc;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, VG_E_DEVICE, "vg_server_add_scene: upload");
+  const uint64_t bytes = vg_scene_bytes(s);
+  if (sv->resident + bytes > sv->cfg.hbm_budget) {
+    sv->set_error("vg_server_add_scene: the scene does not fit the HBM budget");
+    return VG_E_NOMEM;
+  }
+  SceneEntry e;
+  e.scene = s;
+  e.resident = bytes;
+  for (uint32_t slot = ctx->cfg.max_views; slot-- > 0;) e.free_slots.push_back(slot);
+  sv->resident += bytes;
+  sv->scenes.push_back(std::move(e));
+  *scene_id = static_cast<uint32_t>(sv->scenes.size() - 1);
+  return VG_OK;
+}
+
+int vg_server_submit(vg_server* sv, uint32_t scene_id, const vg_view* views, uint32_t count, uint64_t* ticket) {
+  if (!sv || !views || !count || !ticket) return VG_E_INVALID;
+  std::unique_ptr<Request> r(new (std::nothrow) Request());
+  if (!r) return VG_E_NOMEM;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (scene_id >= sv->scenes.size()) return VG_E_INVALID;
+    const SceneEntry& se = sv->scenes[scene_id];
+    if (count > sv->cfg.max_batch_views || count > sv->ctx->cfg.max_views) {
+      sv->set_error("vg_server_submit: more views than a batch holds");
+      return VG_E_INVALID;
+    }
+    if (sv->resident + transient_bytes(se.scene, count) > sv->cfg.hbm_budget) {
+      sv->set_error("vg_server_submit: the request alone does not fit the HBM budget");
+      return VG_E_NOMEM;
+    }
+    for (uint32_t i = 0; i < count; i++)
+      if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0) ||
+          (views[i].triangle_budget && !(views[i].threshold_range[0] > 0 && views[i].threshold_range[1] >= views[i].threshold_range[0]))) {
+        sv->set_error("vg_server_submit: invalid view");
+        return VG_E_INVALID;
+      }
+    r->ticket = sv->next_ticket++;
+    r->scene = scene_id;
+    r->views.assign(views, views + count);
+    r->t_submit = now_us();
+    *ticket = r->ticket;
+    sv->queue.push_back(r.get());
+    sv->requests.emplace(r->ticket, std::move(r));
+    sv->generation++;
+  }
+  sv->cv_work.notify_one();
+  return VG_OK;
+}
+
+int vg_server_poll(vg_server* sv, vg_request_done* done, uint32_t max_done, uint32_t* count, uint32_t wait_us) {
+  if (!sv || !count || (max_done && !done)) return VG_E_INVALID;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  if (sv->completed.empty() && wait_us)
+    sv->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !sv->completed.empty(); });
+  uint32_t n = 0;
+  while (n < max_done && !sv->completed.empty()) {
+    const uint64_t t = sv->completed.front();
+    sv->completed.pop_front();
+    auto it = sv->requests.find(t);
+    if (it == sv->requests.end()) continue;
+    const Request& r = *it->second;
+    vg_request_done& d = done[n++];
+    d.ticket = r.ticket;
+    d.status = r.status;
+    d.views = static_cast<uint32_t>(r.views.size());
+    d.batch_requests = r.batch_requests;
+    d.lane = r.lane;
+    d.queued_us = r.t_start - r.t_submit;
+    d.run_us = r.t_done - r.t_start;
+    d.total_us = r.t_done - r.t_submit;
+  }
+  *count = n;
+  return VG_OK;
+}
+
+int vg_server_read(vg_server* sv, uint64_t ticket, uint32_t view_index, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list,
+                   uint32_t max_entries, vg_metrics* metrics) {
+  if (!sv) return VG_E_INVALID;
+  vg_scene* s = nullptr;
+  uint32_t slot = 0;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->requests.find(ticket);
+    if (it == sv->requests.end() || !it->second->done || it->second->status != VG_OK || view_index >= it->second->slots.size())
+      return VG_E_INVALID;
+    s = sv->scenes[it->second->scene].scene;
+    slot = it->second->slots[view_index];
+  }
+  return read_slot(s, sv->read_stream, slot, counters, draw_args, draw_list, max_entries, metrics);
+}
+
+int vg_server_release(vg_server* sv, uint64_t ticket) {
+  if (!sv) return VG_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->requests.find(ticket);
+  if (it == sv->requests.end() || !it->second->done) return VG_E_INVALID;
+  SceneEntry& se = sv->scenes[it->second->scene];
+  for (uint32_t slot : it->second->slots) se.free_slots.push_back(slot);
+  sv->requests.erase(it);
+  sv->generation++;
+  sv->cv_work.notify_one();  // freed slots may let a queued request in
+  return VG_OK;
+}
+
+int vg_server_stats_get(const vg_server* sv, vg_server_stats* st) {
+  if (!sv || !st) return VG_E_INVALID;
+  auto* m = const_cast<vg_server*>(sv);
+  std::lock_guard<std::mutex> lk(m->mu);
+  st->hbm_budget = sv->cfg.hbm_budget;
+  st->resident_bytes = sv->resident;
+  st->inflight_bytes = sv->inflight;
+  st->peak_inflight_bytes = sv->peak_inflight;
+  st->batches = sv->batches;
+  st->requests_done = sv->requests_done;
+  st->max_batch_requests_seen = sv->max_batch_seen;
+  st->queued = static_cast<uint32_t>(sv->queue.size());
+  st->running = 0;
+  for (const Batch& b : sv->running) st->running += static_cast<uint32_t>(b.requests.size());
+  return VG_OK;
+}
+
+}  // extern "C"

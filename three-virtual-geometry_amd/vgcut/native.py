@@ -34,6 +34,8 @@ EXPORTS = (
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune",
+    "vg_server_create", "vg_server_destroy", "vg_server_last_error", "vg_server_add_scene", "vg_server_submit", "vg_server_poll",
+    "vg_server_read", "vg_server_release", "vg_server_stats_get",
 )
 
 
@@ -106,6 +108,43 @@ class Schedule(ctypes.Structure):
         return (self.instance_blocks_per_cu, self.meshlet_blocks_per_cu, self.meshlet_waves_per_simd)
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64),
+        ("lanes", ctypes.c_uint32),
+        ("max_batch_views", ctypes.c_uint32),
+        ("max_batch_requests", ctypes.c_uint32),
+        ("max_bypass_us", ctypes.c_uint32),
+    ]
+
+
+class RequestDone(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("views", ctypes.c_uint32),
+        ("batch_requests", ctypes.c_uint32),
+        ("lane", ctypes.c_uint32),
+        ("queued_us", ctypes.c_double),
+        ("run_us", ctypes.c_double),
+        ("total_us", ctypes.c_double),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64),
+        ("inflight_bytes", ctypes.c_uint64),
+        ("peak_inflight_bytes", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("requests_done", ctypes.c_uint64),
+        ("max_batch_requests_seen", ctypes.c_uint64),
+        ("queued", ctypes.c_uint32),
+        ("running", ctypes.c_uint32),
+    ]
+
+
 class Metrics(ctypes.Structure):
     _fields_ = [
         ("cells_tested", ctypes.c_uint64),
@@ -161,6 +200,15 @@ def load():
         "vg_budget_thresholds": ([ctypes.c_float, ctypes.c_float, U32, P], ctypes.c_int),
         "vg_profile_read": ([P, P, U32, ctypes.POINTER(U64), P], ctypes.c_int),
         "vg_footprint": ([P, P, U32, ctypes.POINTER(U64), ctypes.POINTER(U64)], ctypes.c_int),
+        "vg_server_create": ([P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)], ctypes.c_int),
+        "vg_server_destroy": ([P], None),
+        "vg_server_last_error": ([P], ctypes.c_char_p),
+        "vg_server_add_scene": ([P, P, ctypes.POINTER(U32)], ctypes.c_int),
+        "vg_server_submit": ([P, U32, P, U32, ctypes.POINTER(U64)], ctypes.c_int),
+        "vg_server_poll": ([P, ctypes.POINTER(RequestDone), U32, ctypes.POINTER(U32), U32], ctypes.c_int),
+        "vg_server_read": ([P, U64, U32, P, P, P, U32, ctypes.POINTER(Metrics)], ctypes.c_int),
+        "vg_server_release": ([P, U64], ctypes.c_int),
+        "vg_server_stats_get": ([P, ctypes.POINTER(ServerStats)], ctypes.c_int),
         "vg_schedule_get": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
         "vg_schedule_set": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
         "vg_tune": ([P, P, U32, U32, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
@@ -369,6 +417,75 @@ class Scene:
     def close(self):
         if self.handle:
             self.lib.vg_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server over resident scenes (vg_server_*): continuous batching on `lanes` HIP streams."""
+
+    def __init__(self, ctx: "Context", hbm_budget: int = 0, lanes: int = 4, max_batch_views: int = 0, max_batch_requests: int = 0,
+                 max_bypass_us: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.scenes = []  # keeps the Scene objects alive while the server uses them
+        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us)
+        handle = ctypes.c_void_p()
+        ctx.check(self.lib.vg_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(handle)), "vg_server_create")
+        self.handle = handle
+
+    def check(self, rc: int, what: str):
+        if rc != VG_OK:
+            msg = self.lib.vg_server_last_error(self.handle)
+            raise VgError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def add_scene(self, scene: "Scene") -> int:
+        sid = ctypes.c_uint32()
+        self.check(self.lib.vg_server_add_scene(self.handle, scene.handle, ctypes.byref(sid)), "vg_server_add_scene")
+        self.scenes.append(scene)
+        return int(sid.value)
+
+    def submit(self, scene_id: int, views) -> int:
+        arr = (View * len(views))(*views)
+        t = ctypes.c_uint64()
+        self.check(self.lib.vg_server_submit(self.handle, scene_id, ctypes.cast(arr, ctypes.c_void_p), len(views), ctypes.byref(t)),
+                   "vg_server_submit")
+        return int(t.value)
+
+    def poll(self, max_done: int = 256, wait_us: int = 0):
+        done = (RequestDone * max_done)()
+        n = ctypes.c_uint32()
+        self.check(self.lib.vg_server_poll(self.handle, done, max_done, ctypes.byref(n), wait_us), "vg_server_poll")
+        return [done[i] for i in range(n.value)]
+
+    def read(self, ticket: int, view_index: int, max_entries: Optional[int] = None):
+        """Counters, draw args, draw lists (camera, shadow) and metrics of one view of a completed request."""
+        if max_entries is None:
+            max_entries = self.ctx.config.max_meshlets
+        counters = np.empty(self.ctx.counter_words, dtype=np.uint32)
+        args = np.empty(self.ctx.draw_args_words, dtype=np.uint32)
+        draws = np.empty((2, max_entries, 4), dtype=np.uint32)
+        met = Metrics()
+        self.check(self.lib.vg_server_read(self.handle, ticket, view_index, counters.ctypes.data, args.ctypes.data, draws.ctypes.data,
+                                           max_entries, ctypes.byref(met)), "vg_server_read")
+        return counters, args, draws, met
+
+    def release(self, ticket: int):
+        self.check(self.lib.vg_server_release(self.handle, ticket), "vg_server_release")
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        self.check(self.lib.vg_server_stats_get(self.handle, ctypes.byref(st)), "vg_server_stats_get")
+        return st
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
