@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="a few batched launches for rocprofv3")
     ap.add_argument("--check", action="store_true", help="compare the batched cut of every view with the oracle")
     ap.add_argument("--max-meshlets", type=int, default=1 << 22, help="draw-list entries per cut per view")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the served-requests measurement")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="requests per rate = rate x this (at least 1000)")
     ap.add_argument("--schedule", type=str, default=None,
                     help="fixed launch schedule 'instance_blocks,meshlet_blocks,meshlet_waves' (per CU) instead of tuning")
     return ap.parse_args()
@@ -133,6 +136,144 @@ def mixed_views(spec, count, vg_make_view, cam, scenegen, size=6144.0):
     return out
 
 
+SERVE_SIZES = (12_000, 50_000, 113_000, 152_000, 500_000, 1_000_000, 2_000_000, 3_000_000)
+SERVE_RATES = (2_000, 5_000, 10_000, 20_000)
+
+
+def serve_requests(resident, count, make_view, cam, scenegen, seed=11):
+    """A pool of cut requests: (scene index, ctypes View array, view count), seeded. Each request is a
+    scene, 1-16 ground views at 720p or 1080p, an error threshold in [0.5, 2] px and, for 30% of them, a
+    triangle budget in [2M, 10M] (threshold range 0.5-3 px)."""
+    from vgcut import native
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(count):
+        k = int(rng.integers(0, len(resident)))
+        spec, size = resident[k][1], resident[k][2]
+        hf = spec.heightfield
+        n = int(rng.integers(1, 17))
+        t = float(rng.uniform(0.5, 2.0))
+        budget = int(rng.integers(2_000_000, 10_000_001)) if rng.random() < 0.3 else 0
+        xz = rng.uniform(-0.45 * size, 0.45 * size, (n, 2)).astype(np.float32)
+        g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
+        arr = (native.View * n)()
+        for i in range(n):
+            yaw = rng.uniform(0, 2 * math.pi)
+            eye = np.array([xz[i, 0], g[i] + rng.uniform(2, 60), xz[i, 1]])
+            target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
+            height = int(rng.choice([720, 1080]))
+            camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
+            arr[i] = make_view(camera, height, slot=i, error_threshold=t, triangle_budget=budget, threshold_range=(0.5, 3.0))
+        pool.append((k, arr, n))
+    return pool
+
+
+def serve_stream(server, ids, pool, rate, count, seed):
+    """Poisson arrivals at `rate` req/s of `count` requests drawn in order from the pool; completed
+    requests are released as they are seen. Returns (offered req/s, elapsed s, done records)."""
+    import ctypes
+
+    from vgcut import native
+
+    lib, handle = server.lib, server.handle
+    done_buf = (native.RequestDone * 512)()
+    n_done = ctypes.c_uint32()
+    ticket = ctypes.c_uint64()
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
+    records = []
+    which = {}
+
+    def drain(wait_us):
+        lib.vg_server_poll(handle, done_buf, 512, ctypes.byref(n_done), wait_us)
+        for j in range(n_done.value):
+            d = done_buf[j]
+            records.append((which.pop(d.ticket), d.status, d.batch_requests, d.total_us, d.queued_us))
+            lib.vg_server_release(handle, d.ticket)
+
+    t0 = time.perf_counter()
+    t_next = t0
+    for i in range(count):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            drain(0)
+        k, arr, n = pool[i % len(pool)]
+        rc = lib.vg_server_submit(handle, ids[k], ctypes.cast(arr, ctypes.c_void_p), n, ctypes.byref(ticket))
+        if rc != 0:
+            raise RuntimeError(f"vg_server_submit failed ({rc})")
+        which[ticket.value] = i % len(pool)
+    t_sub = time.perf_counter()
+    while len(records) < count:
+        drain(1000)
+    return count / (t_sub - t0), time.perf_counter() - t0, records
+
+
+def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate):
+    """Served cut requests over 8 resident scenes (12k-3M instances): batched server vs one request per
+    launch, Poisson arrivals at each rate."""
+    from vgcut import native, scenes
+
+    t0 = time.time()
+    ctx = native.Context(max_triangles=100_000_000, max_shadow_triangles=10_000_000, max_meshlets=1 << 20, max_views=48,
+                         mesh_slots=64, device=0)
+    resident = []
+    for k, n in enumerate(SERVE_SIZES):
+        size = 6144.0 * math.sqrt(n / 3_000_000)
+        spec = scenes.world(instances=n, size=size, seed=100 + k, meshes=spec_meshes)
+        scene = native.Scene(ctx)
+        scenes.load_into(scene, spec)
+        resident.append((scene, spec, size))
+    pool = serve_requests(resident, 256, make_view, cam, scenegen)
+    log(f"serving: {len(SERVE_SIZES)} scenes, {sum(s.instance_count for _, s, _ in resident)} instances ({time.time() - t0:.1f} s)")
+
+    def server_for(batched):
+        sv = native.Server(ctx, lanes=4 if batched else 1, max_batch_views=48, max_batch_requests=0 if batched else 1)
+        return sv, [sv.add_scene(sc) for sc, _, _ in resident]
+
+    # Work of each pool request (cluster tests) and the plain service time, from one pass alone.
+    plain, plain_ids = server_for(False)
+    tests, service = [], []
+    for k, arr, n in pool:
+        t = plain.submit_array(plain_ids[k], arr, n)
+        d = []
+        while not d:
+            d = plain.poll(4, wait_us=100_000)
+        service.append(d[0].run_us)
+        mets = [plain.read(t, i, 1)[3] for i in range(n)]
+        tests.append(sum(int(m.clusters_tested[0] + m.clusters_tested[1]) for m in mets))
+        plain.release(t)
+    capacity = 1e6 / float(np.mean(service))  # plain requests per second, alone and back to back
+    out = {"scenes_instances": list(SERVE_SIZES), "request_pool": len(pool), "views_per_request_mean": float(np.mean([p[2] for p in pool])),
+           "cluster_tests_per_request_mean": float(np.mean(tests)), "plain_service_ms_mean": float(np.mean(service)) / 1e3,
+           "plain_capacity_req_per_s": capacity, "hbm_resident_bytes": int(plain.stats().resident_bytes), "rates": []}
+    batched, batched_ids = server_for(True)
+    loads = sorted({int(capacity * f) for f in (0.25, 0.5, 0.75, 0.9)} | set(rates))
+    for rate in loads:
+        # about `requests_per_rate` seconds of arrivals, and at most about 2 s of work at plain capacity
+        count = int(max(100, min(rate * requests_per_rate, 2.0 * capacity)))
+        row = {"offered_req_per_s": rate, "requests": count}
+        for name, sv, ids in (("batched", batched, batched_ids), ("plain", plain, plain_ids)):
+            offered, elapsed, rec = serve_stream(sv, ids, pool, rate, count, seed=rate)
+            lat = [r[3] / 1e3 for r in rec]
+            ok = sum(1 for r in rec if r[1] == 0)
+            row[name] = {"submitted_req_per_s": offered, "completed_req_per_s": count / elapsed,
+                         "views_clusters_per_s": sum(tests[r[0]] for r in rec) / elapsed,
+                         "latency_ms_p50": percentile(lat, 50), "latency_ms_p99": percentile(lat, 99),
+                         "mean_batch_requests": float(np.mean([r[2] for r in rec])), "ok": ok}
+        for name, sv in (("batched", batched), ("plain", plain)):
+            st = sv.stats()
+            row[name]["peak_inflight_bytes"] = int(st.peak_inflight_bytes)
+            row[name]["reruns_so_far"] = int(st.reruns)
+        out["rates"].append(row)
+        log(f"serving {rate} req/s: batched p50 {row['batched']['latency_ms_p50']:.2f} p99 {row['batched']['latency_ms_p99']:.2f} ms "
+            f"({row['batched']['completed_req_per_s']:.0f} req/s, batch {row['batched']['mean_batch_requests']:.1f}); "
+            f"plain p50 {row['plain']['latency_ms_p50']:.2f} p99 {row['plain']['latency_ms_p99']:.2f} ms "
+            f"({row['plain']['completed_req_per_s']:.0f} req/s)")
+    batched.close()
+    plain.close()
+    return out
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
@@ -155,6 +296,10 @@ def main():
     log(f"world: {spec.instance_count} instances, {len(spec.meshes)} meshes, "
         f"{sum(d.meshletCount for d in spec.meshes)} meshlets, {spec.full_detail_triangles / 1e9:.1f}B full-detail "
         f"triangles ({time.time() - t0:.1f} s)")
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
+        return
 
     views_n = args.views
     ctx = native.Context(max_triangles=MAX_TRIANGLES, max_shadow_triangles=MAX_SHADOW_TRIANGLES, max_meshlets=args.max_meshlets,
@@ -377,6 +522,8 @@ def main():
                      "algorithmic_bytes": dom_bytes,
                      "launch_algorithmic_bytes": launch_bytes, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
+        "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES,
+                                                                               args.serve_seconds),
     }
     print(json.dumps(out))
 

@@ -205,7 +205,9 @@ int vg_tune(vg_scene* scene, const vg_view* views, uint32_t count, uint32_t repe
  * stream) is free it launches one batch: queued requests of one scene, oldest first, that fit the
  * batch's view limit, the scene's free view slots and the HBM budget. A batch's transient buffers
  * (cell queue, work list) come from the device's stream-ordered memory pool and are returned when it
- * ends; requests that do not fit wait while others overtake them, up to max_bypass_us. Batches on
+ * ends; the work list starts at first_work_items and a batch that overflows it runs again at its worst-case
+ * size (every (view, instance, cut)), outside the budget check. Requests that do not fit wait while
+ * others overtake them, up to max_bypass_us. Batches on
  * different lanes overlap on the device. Each request's views start from a controller at rest (scale 1),
  * so a result does not depend on the batch it ran in. Results stay in the request's view slots until
  * vg_server_release. Scenes given to a server must not be changed or cut directly while it runs, and
@@ -217,6 +219,7 @@ typedef struct vg_server_config {
   uint32_t max_batch_views;     /* views per batch, 1 - max_views (default max_views) */
   uint32_t max_batch_requests;  /* requests per batch; 0: no limit, 1: one request per launch */
   uint32_t max_bypass_us;       /* how long later requests may overtake one that does not fit (default 2000) */
+  uint32_t first_work_items;    /* work-list items of a batch's first launch (default 1 << 22) */
 } vg_server_config;
 
 typedef struct vg_server vg_server;
@@ -234,7 +237,7 @@ typedef struct vg_request_done {
 
 typedef struct vg_server_stats {
   uint64_t hbm_budget, resident_bytes, inflight_bytes, peak_inflight_bytes;
-  uint64_t batches, requests_done, max_batch_requests_seen;
+  uint64_t batches, requests_done, max_batch_requests_seen, reruns;
   uint32_t queued, running;
 } vg_server_stats;
 

@@ -473,7 +473,9 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
   size_t queue_bytes = 0, want = 0;
   transient_sizes(s, count, &queue_bytes, &want);
   if (lane->borrowed) {
-    if (lane->d_cell_queue.bytes < queue_bytes || lane->work_capacity < want)
+    // The caller's work list may be smaller than the worst case: an overflow sets the capacity flag
+    // (the launch then reports VG_E_CAPACITY). The cell queue must hold every (view, cell).
+    if (lane->d_cell_queue.bytes < queue_bytes || lane->work_capacity == 0)
       return fail(ctx, VG_E_INVALID, "vg_cut: lane buffers smaller than the launch needs");
   } else {
     if (lane->cell_queue_capacity < queue_bytes / sizeof(uint2)) {

@@ -117,7 +117,8 @@ struct Params {
   float* work_scale;     // instance scale of each item (max column length, computed once)
   unsigned long long* work_counter;
   uint32_t work_capacity;
-  uint32_t* flags;       // [0] any retry this round, [1] overflow
+  uint32_t* flags;       // [0], [1] a view needs retry round 1, 2; [2] internal error; [3] work-list overflow;
+                         // [4] most work items any pass appended (sizes later work lists)
   // footprint (measurement only): bit per instance tested / per meshlet in a candidate range
   uint32_t* fp_instances;
   uint32_t* fp_meshlets;

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *p.cell_queue_count = 0;
     *p.work_counter = 0;
-    p.flags[0] = p.flags[1] = p.flags[2] = p.flags[3] = 0;
+    p.flags[0] = p.flags[1] = p.flags[2] = p.flags[3] = p.flags[4] = 0;
   }
 }
 
@@ -513,6 +513,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     const uint64_t task_base = o & kTaskMask;
     if (lane == 0 && (task_base + tasks >= kTaskLimit || static_cast<uint64_t>(item_base) + items > p.work_capacity))
       atomicOr(&p.flags[3], 1u);
+    if (lane == 0) atomicMax(&p.flags[4], item_base + items);
     uint32_t at = item_base + items_incl - n_items;
     uint64_t task_at = task_base + tasks_incl - n_tasks;
     const uint32_t tag_base = slot | (vi << kTagViewShift);
@@ -821,6 +822,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
       const uint64_t task_base = o & kTaskMask;
       if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
+      if (lane == 0) atomicMax(&p.flags[4], item_base + items);
       if (work && item_base + rank < p.work_capacity) {
         const uint32_t tag = slot | (occlusion_on ? kTagOcclusion : 0u) | (vi << kTagViewShift);
         p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);

@@ -43,6 +43,7 @@ struct SceneEntry {
   vg_scene* scene = nullptr;
   uint64_t resident = 0;
   std::vector<uint32_t> free_slots;
+  uint32_t items_per_view = 0;  // most work items per view any batch of this scene used (0: none yet)
 };
 
 struct Batch {
@@ -51,14 +52,30 @@ struct Batch {
   uint32_t views = 0;
   std::vector<Request*> requests;
   uint64_t bytes = 0;
+  bool full = false;  // work list at its worst-case size (after an overflow of the optimistic one)
 };
 
-// Transient bytes of a launch of `views` views of a scene: the (view, cell) queue and the work list
-// (uint4 item + u64 first task + f32 scale per entry).
-uint64_t transient_bytes(const vg_scene* s, uint32_t views) {
+// A batch's work list starts at this many items (uint4 item + u64 first task + f32 scale: 28 B each)
+// when the worst case (every (view, instance, cut)) is larger. Real cuts use a small fraction of the
+// worst case; a batch that overflows it is run again at the worst-case size, which gives the same
+// results because every request starts from a fresh controller state.
+constexpr uint32_t kFirstWorkItems = 1u << 22;
+constexpr size_t kWorkItemBytes = sizeof(uint4) + sizeof(unsigned long long) + sizeof(float);
+
+// The first launch of a batch sizes its work list from what batches of the scene used so far (1.25x the
+// most items per view, times the views), or `first_items` before any batch ran; never above the worst case.
+void batch_sizes(const SceneEntry& se, uint32_t views, bool full, uint32_t first_items, size_t* queue_bytes, size_t* work_items) {
+  transient_sizes(se.scene, views, queue_bytes, work_items);
+  if (full) return;
+  const size_t guess = se.items_per_view ? static_cast<size_t>(se.items_per_view) * views * 5 / 4 + 4096 : first_items;
+  *work_items = std::min<size_t>(*work_items, guess);
+}
+
+// Transient bytes of a launch of `views` views of a scene: the (view, cell) queue and the work list.
+uint64_t transient_bytes(const SceneEntry& se, uint32_t views, bool full, uint32_t first_items) {
   size_t queue_bytes = 0, work_items = 0;
-  transient_sizes(s, views, &queue_bytes, &work_items);
-  return queue_bytes + work_items * (sizeof(uint4) + sizeof(unsigned long long) + sizeof(float));
+  batch_sizes(se, views, full, first_items, &queue_bytes, &work_items);
+  return queue_bytes + work_items * kWorkItemBytes;
 }
 
 }  // namespace
@@ -69,7 +86,7 @@ struct vg_server {
   std::vector<SceneEntry> scenes;
   std::vector<std::unique_ptr<Lane>> lanes;
   std::vector<hipEvent_t> lane_done;
-  std::vector<uint32_t*> lane_flags;  // pinned: the lane's 4 flag words after its batch
+  std::vector<uint32_t*> lane_flags;  // pinned: the lane's first 8 flag words after its batch
   std::vector<int> lane_batch;        // index into `running` or -1
   hipMemPool_t pool = nullptr;
   hipStream_t read_stream = nullptr;
@@ -79,7 +96,7 @@ struct vg_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
   std::deque<uint64_t> completed;
   std::vector<Batch> running;
-  uint64_t resident = 0, inflight = 0, peak_inflight = 0, batches = 0, requests_done = 0, max_batch_seen = 0;
+  uint64_t resident = 0, inflight = 0, peak_inflight = 0, batches = 0, requests_done = 0, max_batch_seen = 0, reruns = 0;
   uint64_t next_ticket = 1;
   uint64_t generation = 0;  // bumped by submit and release: the worker waits for it when it cannot progress
   bool stop = false;
@@ -105,6 +122,27 @@ struct vg_server {
         status = VG_E_CAPACITY;
       } else if (lane_flags[b.lane][2]) {
         status = VG_E_DEVICE;
+      }
+      {
+        SceneEntry& se = scenes[b.scene];
+        const uint32_t per_view = (lane_flags[b.lane][4] + b.views - 1) / std::max(1u, b.views);
+        se.items_per_view = std::max(se.items_per_view, per_view);
+      }
+      if (status == VG_E_CAPACITY && !b.full) {
+        // The optimistic work list overflowed: run the same batch (same slots) again at full size.
+        Batch again;
+        again.scene = b.scene;
+        again.views = b.views;
+        again.requests = b.requests;
+        again.full = true;
+        const uint32_t lane = b.lane;
+        inflight -= b.bytes;
+        running.erase(running.begin() + static_cast<std::ptrdiff_t>(i));
+        for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
+        lane_batch[lane] = -1;
+        reruns++;
+        launch(again, static_cast<int>(lane));
+        continue;
       }
       const double t = now_us();
       for (Request* r : b.requests) {
@@ -140,7 +178,7 @@ struct vg_server {
       const SceneEntry& se = scenes[r->scene];
       const uint32_t n = static_cast<uint32_t>(r->views.size());
       if (!have) {
-        if (n <= slots_left[r->scene] && n <= cfg.max_batch_views && transient_bytes(se.scene, n) <= budget_left) {
+        if (n <= slots_left[r->scene] && n <= cfg.max_batch_views && transient_bytes(se, n, false, cfg.first_work_items) <= budget_left) {
           b.scene = r->scene;
           b.views = n;
           b.requests.push_back(r);
@@ -153,7 +191,7 @@ struct vg_server {
       if (r->scene != b.scene) continue;
       if (cfg.max_batch_requests && b.requests.size() >= cfg.max_batch_requests) break;
       if (b.views + n > cfg.max_batch_views || b.views + n > slots_left[b.scene]) continue;
-      if (transient_bytes(se.scene, b.views + n) > budget_left) continue;
+      if (transient_bytes(scenes[b.scene], b.views + n, false, cfg.first_work_items) > budget_left) continue;
       b.views += n;
       b.requests.push_back(r);
     }
@@ -169,22 +207,25 @@ struct vg_server {
     views.reserve(b.views);
     const double t = now_us();
     for (Request* r : b.requests) {
-      r->slots.clear();
-      for (const vg_view& v : r->views) {
-        const uint32_t slot = se.free_slots.back();
-        se.free_slots.pop_back();
-        r->slots.push_back(slot);
-        views.push_back(v);
-        views.back().slot = slot;
+      if (!b.full) {  // a first launch: assign view slots and leave the queue (a rerun keeps both)
+        r->slots.clear();
+        for (size_t i = 0; i < r->views.size(); i++) {
+          r->slots.push_back(se.free_slots.back());
+          se.free_slots.pop_back();
+        }
+        r->t_start = t;
+        queue.erase(std::find(queue.begin(), queue.end(), r));
       }
-      r->t_start = t;
+      for (size_t i = 0; i < r->views.size(); i++) {
+        views.push_back(r->views[i]);
+        views.back().slot = r->slots[i];
+      }
       r->batch_requests = static_cast<uint32_t>(b.requests.size());
       r->lane = static_cast<uint32_t>(lane_index);
-      queue.erase(std::find(queue.begin(), queue.end(), r));
     }
     size_t queue_bytes = 0, work_items = 0;
-    transient_sizes(s, b.views, &queue_bytes, &work_items);
-    b.bytes = transient_bytes(s, b.views);
+    batch_sizes(se, b.views, b.full, cfg.first_work_items, &queue_bytes, &work_items);
+    b.bytes = transient_bytes(se, b.views, b.full, cfg.first_work_items);
     b.lane = static_cast<uint32_t>(lane_index);
     void* ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
     const size_t sizes[4] = {queue_bytes, sizeof(uint4) * work_items, sizeof(unsigned long long) * work_items, sizeof(float) * work_items};
@@ -201,7 +242,7 @@ struct vg_server {
       lane->work_capacity = static_cast<uint32_t>(work_items);
       rc = prepare_cut(s, lane, views.data(), b.views, true);
       if (rc == VG_OK) rc = enqueue_cut(s, lane, b.views, false);
-      if (rc == VG_OK && hipMemcpyAsync(lane_flags[lane_index], lane->d_flags.ptr, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+      if (rc == VG_OK && hipMemcpyAsync(lane_flags[lane_index], lane->d_flags.ptr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                         lane->stream) != hipSuccess)
         rc = VG_E_DEVICE;
     }
@@ -276,6 +317,7 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
   if (!c.lanes) c.lanes = 4;
   if (!c.max_batch_views) c.max_batch_views = ctx->cfg.max_views;
   if (!c.max_bypass_us) c.max_bypass_us = 2000;
+  if (!c.first_work_items) c.first_work_items = kFirstWorkItems;
   if (c.lanes > 16 || c.max_batch_views > ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_server_create: lanes 1-16, max_batch_views <= max_views");
   if (!c.hbm_budget) {
     size_t free_b = 0, total_b = 0;
@@ -307,9 +349,9 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipEventCreate");
       sv->lane_done.push_back(e);
       uint32_t* flags = nullptr;
-      if (hipHostMalloc(reinterpret_cast<void**>(&flags), 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+      if (hipHostMalloc(reinterpret_cast<void**>(&flags), 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return fail(ctx, VG_E_NOMEM, "hipHostMalloc");
-      std::memset(flags, 0, 4 * sizeof(uint32_t));
+      std::memset(flags, 0, 8 * sizeof(uint32_t));
       sv->lane_flags.push_back(flags);
       sv->lane_batch.push_back(-1);
     }
@@ -381,7 +423,7 @@ int vg_server_submit(vg_server* sv, uint32_t scene_id, const vg_view* views, uin
       sv->set_error("vg_server_submit: more views than a batch holds");
       return VG_E_INVALID;
     }
-    if (sv->resident + transient_bytes(se.scene, count) > sv->cfg.hbm_budget) {
+    if (sv->resident + transient_bytes(se, count, false, sv->cfg.first_work_items) > sv->cfg.hbm_budget) {
       sv->set_error("vg_server_submit: the request alone does not fit the HBM budget");
       return VG_E_NOMEM;
     }
@@ -470,6 +512,7 @@ int vg_server_stats_get(const vg_server* sv, vg_server_stats* st) {
   st->batches = sv->batches;
   st->requests_done = sv->requests_done;
   st->max_batch_requests_seen = sv->max_batch_seen;
+  st->reruns = sv->reruns;
   st->queued = static_cast<uint32_t>(sv->queue.size());
   st->running = 0;
   for (const Batch& b : sv->running) st->running += static_cast<uint32_t>(b.requests.size());

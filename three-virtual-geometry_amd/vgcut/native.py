@@ -115,6 +115,7 @@ class ServerConfig(ctypes.Structure):
         ("max_batch_views", ctypes.c_uint32),
         ("max_batch_requests", ctypes.c_uint32),
         ("max_bypass_us", ctypes.c_uint32),
+        ("first_work_items", ctypes.c_uint32),
     ]
 
 
@@ -140,6 +141,7 @@ class ServerStats(ctypes.Structure):
         ("batches", ctypes.c_uint64),
         ("requests_done", ctypes.c_uint64),
         ("max_batch_requests_seen", ctypes.c_uint64),
+        ("reruns", ctypes.c_uint64),
         ("queued", ctypes.c_uint32),
         ("running", ctypes.c_uint32),
     ]
@@ -430,11 +432,11 @@ class Server:
     """Request server over resident scenes (vg_server_*): continuous batching on `lanes` HIP streams."""
 
     def __init__(self, ctx: "Context", hbm_budget: int = 0, lanes: int = 4, max_batch_views: int = 0, max_batch_requests: int = 0,
-                 max_bypass_us: int = 0):
+                 max_bypass_us: int = 0, first_work_items: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         self.scenes = []  # keeps the Scene objects alive while the server uses them
-        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us)
+        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us, first_work_items)
         handle = ctypes.c_void_p()
         ctx.check(self.lib.vg_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(handle)), "vg_server_create")
         self.handle = handle
@@ -454,6 +456,13 @@ class Server:
         arr = (View * len(views))(*views)
         t = ctypes.c_uint64()
         self.check(self.lib.vg_server_submit(self.handle, scene_id, ctypes.cast(arr, ctypes.c_void_p), len(views), ctypes.byref(t)),
+                   "vg_server_submit")
+        return int(t.value)
+
+    def submit_array(self, scene_id: int, views_array, count: int) -> int:
+        """submit() for a prebuilt ctypes array of View (no per-call copy of the records)."""
+        t = ctypes.c_uint64()
+        self.check(self.lib.vg_server_submit(self.handle, scene_id, ctypes.cast(views_array, ctypes.c_void_p), count, ctypes.byref(t)),
                    "vg_server_submit")
         return int(t.value)
 

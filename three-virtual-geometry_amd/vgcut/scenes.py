@@ -129,11 +129,14 @@ def ground_poses(spec: SceneSpec, count: int, half_extent: float, eye_height: fl
 
 
 def world(mesh_count=64, tri_range=(100_000, 2_000_000), instances=3_000_000, size=6144.0, amplitude=200.0,
-          draw_range=(200.0, 6000.0), seed=6) -> SceneSpec:
-    """Props by Bridson Poisson-disk sampling over a size x size noise heightfield (y = terrain height)."""
+          draw_range=(200.0, 6000.0), seed=6, meshes=None) -> SceneSpec:
+    """Props by Bridson Poisson-disk sampling over a size x size noise heightfield (y = terrain height).
+    `meshes`: a prop set to reuse (built from `seed` otherwise)."""
     rng = np.random.default_rng(seed)
     tris = log_uniform(rng, tri_range[0], tri_range[1], mesh_count).astype(np.int64)
-    meshes = build_props(rng, tris, seed * 1000)
+    if meshes is None:
+        meshes = build_props(rng, tris, seed * 1000)
+    mesh_count = len(meshes)
     # Disk radius for about 1.1x the wanted count (Bridson with 12 candidates packs about 0.595 points per
     # r^2), then an even random subsample to exactly `instances`.
     radius = math.sqrt(0.595 * size * size / (instances * 1.1))
