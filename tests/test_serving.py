@@ -166,14 +166,15 @@ def test_admission_by_hbm_budget_and_view_slots(gpu_available):
     specs = [small_ruins(seed=91, instances=1000, meshes=3), small_ruins(seed=92, instances=4000, meshes=4)]
     ctx = native.Context(max_triangles=2_000_000, max_shadow_triangles=1_000_000, max_meshlets=MAX_MESHLETS, max_views=4, device=0)
     small, large = make_scenes(ctx, specs)
-    # A budget that holds both scenes and two small 2-view requests (about 0.12 MB of cell queue and work
-    # list each) but not a 4-view request of the large scene (about 0.9 MB).
+    # A budget that holds both scenes and two small 2-view requests (about 0.46 MB of cell queue and work
+    # list each: 16 work-list partitions of up to 4 queue entries x 256 items) but not a 4-view request of
+    # the large scene (about 1.26 MB: partitions of up to 11 entries).
     probe = native.Server(ctx, lanes=1)
     probe.add_scene(small)
     probe.add_scene(large)
     resident = probe.stats().resident_bytes
     probe.close()
-    server = native.Server(ctx, hbm_budget=resident + 300_000, lanes=2, max_batch_views=4)
+    server = native.Server(ctx, hbm_budget=resident + 1_000_000, lanes=2, max_batch_views=4)
     s_id, l_id = server.add_scene(small), server.add_scene(large)
     rng = np.random.default_rng(3)
     with pytest.raises(native.VgError) as e:

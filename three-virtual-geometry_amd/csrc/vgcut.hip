@@ -91,7 +91,8 @@ int lane_init(vg_ctx* ctx, Lane* lane, hipStream_t stream) {
     lane->owns_stream = true;
   }
   int rc = 0;
-  if ((rc = ensure_lane(ctx, lane, lane->d_cell_queue_count, 64)) || (rc = ensure_lane(ctx, lane, lane->d_work_counter, 64)) ||
+  if ((rc = ensure_lane(ctx, lane, lane->d_cell_queue_count, 64)) ||
+      (rc = ensure_lane(ctx, lane, lane->d_work_counter, sizeof(unsigned long long) * kWorkParts * kWorkCounterSpacing)) ||
       (rc = ensure_lane(ctx, lane, lane->d_flags, 64)))
     return rc;
   VG_HIP(ctx, hipMemset(lane->d_flags.ptr, 0, lane->d_flags.bytes));
@@ -125,10 +126,15 @@ void lane_destroy(Lane* lane) {
 }
 
 void transient_sizes(const vg_scene* s, uint32_t count, size_t* queue_bytes, size_t* work_items) {
-  // Every (view, cell) can be queued once; every (view, instance, cut) can be a work item.
+  // Every (view, cell) can be queued once. A partition of the work list takes the items of at most
+  // ceil(entries / kWorkParts) + 2 queue entries, each at most 2 cuts x 128 instances, nor more than all
+  // (view, instance, cut) pairs.
   const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
   *queue_bytes = sizeof(uint2) * cells * count;
-  *work_items = std::min<uint64_t>(static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64, kMaxWorkItems);
+  const uint64_t entries = static_cast<uint64_t>(cells) * count;
+  const uint64_t per_part = std::min<uint64_t>(((entries + kWorkParts - 1) / kWorkParts + 2) * 2 * kCellSize,
+                                               static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64);
+  *work_items = std::min<uint64_t>(per_part, kMaxWorkItems / kWorkParts) * kWorkParts;
 }
 
 void release(vg_scene* s, DevBuf& b) {
@@ -335,7 +341,7 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.work_start = static_cast<unsigned long long*>(lane->d_work_start.ptr);
   p.work_scale = static_cast<float*>(lane->d_work_scale.ptr);
   p.work_counter = static_cast<unsigned long long*>(lane->d_work_counter.ptr);
-  p.work_capacity = lane->work_capacity;
+  p.work_capacity = lane->work_capacity / kWorkParts;  // per partition
   p.flags = static_cast<uint32_t*>(lane->d_flags.ptr);
   p.budget_thresholds = static_cast<const float*>(lane->d_budget_thresholds.ptr);
   p.budget_hist = static_cast<unsigned long long*>(lane->d_budget_hist.ptr);

@@ -26,6 +26,12 @@ constexpr int kTaskBits = 36;
 constexpr uint64_t kTaskMask = (1ull << kTaskBits) - 1;
 constexpr uint64_t kTaskLimit = 1ull << 35;
 constexpr uint32_t kMaxWorkItems = (1u << 28) - 1;
+// The work list is kWorkParts partitions, each with its own packed counter (kWorkCounterSpacing u64
+// apart, so on separate cache lines): waves of the instance pass append to the partition of their rank,
+// which spreads the appends over kWorkParts atomics instead of one. The meshlet pass walks the
+// partitions' task ranges one after another as one flat task space.
+constexpr uint32_t kWorkParts = 16;
+constexpr uint32_t kWorkCounterSpacing = 32;
 // Metrics words (u64) per view slot.
 constexpr uint32_t kMetricWords = 16;
 enum : uint32_t {
@@ -115,8 +121,8 @@ struct Params {
   uint4* work;           // (instance, first, last, tag)
   unsigned long long* work_start;  // first task of each item
   float* work_scale;     // instance scale of each item (max column length, computed once)
-  unsigned long long* work_counter;
-  uint32_t work_capacity;
+  unsigned long long* work_counter;  // kWorkParts packed (items << kTaskBits | tasks) counters
+  uint32_t work_capacity;            // items per partition; partition q holds items [q * work_capacity, ...)
   uint32_t* flags;       // [0], [1] a view needs retry round 1, 2; [2] internal error; [3] work-list overflow;
                          // [4] most work items any pass appended (sizes later work lists)
   // footprint (measurement only): bit per instance tested / per meshlet in a candidate range
@@ -247,6 +253,12 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
   v += dpp<0x143, 0xc, false>(v);  // row_bcast:31 into rows 2 and 3
   return v;
 }
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(lane)));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(lane)));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // The sum over the 64 lanes (wave-uniform).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_scan(v)), 63));

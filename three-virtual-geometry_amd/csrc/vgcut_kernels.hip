@@ -164,9 +164,9 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   if (v.budget)
     for (uint32_t i = threadIdx.x; i <= p.budget_bins; i += blockDim.x)
       p.budget_hist[static_cast<size_t>(blockIdx.x) * (p.budget_bins + 1) + i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kWorkParts) p.work_counter[threadIdx.x * kWorkCounterSpacing] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *p.cell_queue_count = 0;
-    *p.work_counter = 0;
     p.flags[0] = p.flags[1] = p.flags[2] = p.flags[3] = p.flags[4] = 0;
   }
 }
@@ -256,14 +256,17 @@ __device__ __forceinline__ LevelResult select_band(const Params& p, uint32_t lev
 // selectLevels for the camera and the shadow cut in one pass over the level table (the level records
 // are wave-uniform loads): a level is possible if its error can project under `high` and its parent
 // error over `low` somewhere in [pmin, pmax], guaranteed if that holds with margin at both ends.
+// `levels`: the mesh's level table (2 float4 per level) staged in the wave's LDS; every lane reads the
+// same entry (a broadcast read).
 template <bool kBoth, typename V>
-__device__ __forceinline__ void select_levels(const Params& p, const V& v, uint32_t level_base, uint32_t level_count,
-                                              float pmin, float pmax, LevelResult (&r)[2]) {
+__device__ __forceinline__ void select_levels(const float4* levels, const V& v, uint32_t level_count, float pmin, float pmax,
+                                              LevelResult (&r)[2]) {
   const float low0 = v.low[0], high0 = v.high[0], low1 = v.low[1], high1 = v.high[1];
   r[0] = r[1] = LevelResult{0xffffffffu, 0u, 0u, 0u};
+#pragma unroll 2
   for (uint32_t i = 0; i < level_count; i++) {
-    const float4 l0 = scalar_f4(p.mesh_table, level_base + i * 2);
-    const float4 l1 = scalar_f4(p.mesh_table, level_base + i * 2 + 1);
+    const float4 l0 = levels[i * 2];
+    const float4 l1 = levels[i * 2 + 1];
     const uint32_t level_first = static_cast<uint32_t>(l0.x);
     const uint32_t size = static_cast<uint32_t>(l0.y);
     const bool open_parent = l0.w >= VG_INF_HALF, open_guard = l1.y >= VG_INF_HALF;
@@ -307,13 +310,18 @@ __device__ void mark_range(uint32_t* bits, uint32_t first, uint32_t last) {
 // drawn without cluster tests. The cell's items are appended with one atomic, lane-major, so item order
 // and first-task order agree. Counters are summed in registers over consecutive cells of one view.
 __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
+  __shared__ float4 level_table[4][2 * kMaxLevels];  // per wave: the current mesh's level table
   if (retry_round && !p.flags[retry_round - 1]) return;
-  const uint32_t lane = lane_id();
+  const uint32_t lane = lane_id(), wib = threadIdx.x / 64;
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), rank = xcd_wave_rank();
   const uint32_t q_begin = static_cast<uint32_t>(queue_count * rank / waves);
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (rank + 1) / waves);
   if (q_begin >= q_end) return;
+  // This wave's work-list partition: ranks are split evenly, so each partition takes the items of an
+  // even share of the queue.
+  const uint32_t part = static_cast<uint32_t>(rank * kWorkParts / waves);
+  const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
 
   // Counters of the current view (wave-uniform), flushed when the view changes.
   uint32_t acc_slot = 0xffffffffu;
@@ -337,7 +345,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
   // The current cell: its record, mesh and, per round of 64 instances, the view-independent values
   // (scale, bounding-sphere centre, LOD-sphere centre, culling-sphere centre), kept while consecutive
   // queue entries name the same cell.
-  uint32_t cur_cell = 0xffffffffu, slot = 0, first_instance = 0, count = 0, level_count = 0, level_base = 0;
+  uint32_t cur_cell = 0xffffffffu, cur_mesh = 0xffffffffu, slot = 0, first_instance = 0, count = 0, level_count = 0, level_base = 0;
   float4 object_sphere = make_float4(0, 0, 0, 0), lod_bounds = object_sphere, cull_bounds = object_sphere;
   float4 flags = object_sphere, levers = object_sphere;
   float d_scale[2], d_center[2][3], d_lq[2][4], d_cc[2][3];
@@ -349,6 +357,8 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     d_cc[r][0] = d_cc[r][1] = d_cc[r][2] = 0.0f;
   }
 
+  // The most items this wave's partition held after one of its appends (one atomicMax at the end).
+  uint32_t high_water = 0;
   uint2 entry = scalar_u2(p.cell_queue, q_begin);
   for (uint32_t q = q_begin; q < q_end; q++) {
     const uint32_t vi = uniform(entry.x);
@@ -383,6 +393,15 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       lod_bounds = scalar_f4(mesh, 1);
       cull_bounds = scalar_f4(mesh, 2);
       flags = scalar_f4(mesh, 5);
+      if (slot != cur_mesh) {  // stage the mesh's level table in LDS (one coalesced load)
+        cur_mesh = slot;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < 2 * level_count; i += 64) level_table[wib][i] = p.mesh_table[level_base + i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
       float4 m[2][4];
 #pragma unroll
       for (uint32_t r = 0; r < 2; r++) {
@@ -459,8 +478,8 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
       const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
       LevelResult lr[2];
-      if (any1) select_levels<true>(p, v, level_base, level_count, pmin, pmax, lr);
-      else select_levels<false>(p, v, level_base, level_count, pmin, pmax, lr);
+      if (any1) select_levels<true>(level_table[wib], v, level_count, pmin, pmax, lr);
+      else select_levels<false>(level_table[wib], v, level_count, pmin, pmax, lr);
       const float3 cc = make_float3(d_cc[r][0], d_cc[r][1], d_cc[r][2]);
       const float cull_radius = cull_bounds.w * scale;
       bool work[2];
@@ -493,7 +512,8 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     acc_clusters[0] += wave_sum(lane_clusters[0]);
     acc_clusters[1] += wave_sum(lane_clusters[1]);
 
-    // Append the cell's items: lane-major, (round, cut) order within a lane.
+    // Append the cell's items: lane-major, (round, cut) order within a lane; one atomic on the wave's
+    // partition counter places them and gives their first task.
     uint32_t n_items = 0, n_tasks = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 4; k++) {
@@ -507,13 +527,13 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     const uint32_t tasks_incl = wave_scan(n_tasks);
     const uint32_t tasks = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tasks_incl), 63));
     unsigned long long old = 0;
-    if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
+    if (lane == 0) old = atomicAdd(p.work_counter + part * kWorkCounterSpacing, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
     const uint64_t o = (static_cast<uint64_t>(uniform(static_cast<uint32_t>(old >> 32))) << 32) | uniform(static_cast<uint32_t>(old));
     const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
     const uint64_t task_base = o & kTaskMask;
     if (lane == 0 && (task_base + tasks >= kTaskLimit || static_cast<uint64_t>(item_base) + items > p.work_capacity))
       atomicOr(&p.flags[3], 1u);
-    if (lane == 0) atomicMax(&p.flags[4], item_base + items);
+    high_water = max(high_water, item_base + items);
     uint32_t at = item_base + items_incl - n_items;
     uint64_t task_at = task_base + tasks_incl - n_tasks;
     const uint32_t tag_base = slot | (vi << kTagViewShift);
@@ -525,9 +545,10 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
         if (at < p.work_capacity) {
           const uint32_t tag = tag_base | (cut ? kTagShadow : 0u) | (cut == 0 && occlusion_on ? kTagOcclusion : 0u) |
                                ((fast_bits >> k) & 1u ? kTagFast : 0u);
-          p.work[at] = make_uint4(first_instance + r * 64 + lane, lo, hi, tag);
-          p.work_start[at] = task_at;
-          p.work_scale[at] = item_scale[r];
+          const size_t g = part_base + at;
+          p.work[g] = make_uint4(first_instance + r * 64 + lane, lo, hi, tag);
+          p.work_start[g] = task_at;
+          p.work_scale[g] = item_scale[r];
         }
         at++;
         task_at += hi - lo;
@@ -535,45 +556,38 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     }
   }
   flush();
+  if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
 }
 
 // ------------------------------------------------------------------ 3. meshlets (meshletPass, GeometryPool.ts:926-968)
 
-// Calls body(valid, item, w, meshlet) for 64 consecutive tasks at a time over this wave's equal share
-// of the flat task space (all lanes together). The wave finds its first item by a 64-ary search over
-// the items' first tasks, then walks forward with shuffles.
+// Walks the local tasks [lb, le) of one work-list partition (its items at [base, base + items)), 64
+// consecutive tasks at a time: the wave finds the item holding lb by a 64-ary search over the items'
+// first tasks, then maps lanes to items with a 6-step shuffle search over the next 64 item starts, whose
+// load for the next chunk is issued before the current chunk's body runs.
 template <typename Body>
-__device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
-  const unsigned long long packed = *p.work_counter;
-  const uint32_t items = min(static_cast<uint32_t>(packed >> kTaskBits), p.work_capacity);
-  const uint64_t total = packed & kTaskMask;
-  if (total == 0 || items == 0 || p.flags[3]) return;
+__device__ __forceinline__ void walk_partition(const Params& p, size_t base, uint32_t items, uint64_t lb, uint64_t le, Body&& body) {
   const uint32_t lane = lane_id();
-  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
-  const uint64_t wave = xcd_wave_rank();
-  const uint64_t seg_begin = total * wave / waves;
-  const uint64_t seg_end = total * (wave + 1) / waves;
-  if (seg_begin >= seg_end) return;
-
-  // The item holding seg_begin: the last item whose first task is <= seg_begin (64-ary search).
+  const unsigned long long* __restrict__ starts = p.work_start + base;
+  const uint4* __restrict__ work = p.work + base;
+  // The item holding lb: the last item whose first task is <= lb (64-ary search).
   uint32_t lo = 0, hi = items;
   while (hi - lo > 1) {
     const uint32_t step = (hi - lo + 63) / 64;
     const uint32_t probe = lo + lane * step;
-    const bool le = probe < hi && p.work_start[probe] <= seg_begin;
-    uint32_t n = static_cast<uint32_t>(__popcll(__ballot(le)));  // >= 1: work_start[lo] <= seg_begin
+    const bool le_ = probe < hi && starts[probe] <= lb;
+    uint32_t n = static_cast<uint32_t>(__popcll(__ballot(le_)));  // >= 1: starts[lo] <= lb
     n = n < 1 ? 1u : n;
     const uint32_t nlo = lo + (n - 1) * step;
     hi = min(hi, nlo + step);
     lo = nlo;
   }
   uint32_t cur = uniform(lo);
-  uint64_t cur_start = p.work_start[cur];
+  uint64_t cur_start = starts[cur];
+  uint64_t ns = cur + 1 + lane < items ? starts[cur + 1 + lane] : ~0ull;
 
-  for (uint64_t k0 = seg_begin; k0 < seg_end; k0 += 64) {
+  for (uint64_t k0 = lb; k0 < le; k0 += 64) {
     // Distances from k0 to the first tasks of the next 64 items (at least 1, clamped to 65).
-    const uint32_t next = cur + 1 + lane;
-    const uint64_t ns = next < items ? p.work_start[next] : ~0ull;
     const uint32_t dist = ns - k0 > 65 ? 65u : static_cast<uint32_t>(ns - k0);
     // pos = number of items after `cur` that start at or before task k0 + lane.
     uint32_t pos = 0;
@@ -586,30 +600,66 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
     const uint64_t k = k0 + lane;
     const uint32_t item = cur + pos;
     const uint64_t item_start = pos == 0 ? cur_start : k0 + prev;
-    bool valid = k < seg_end && item < items && k >= item_start;
+    bool valid = k < le && item < items && k >= item_start;
+
+    // The item holding task k0 + 64, and the first tasks of its successors, loaded while this chunk runs.
+    const uint32_t last_pos = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), 63));
+    const uint32_t last_prev = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(prev), 63));
+    const uint32_t after = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dist), static_cast<int>(last_pos)));
+    uint32_t next_cur = cur;
+    uint64_t next_start = cur_start;
+    if (last_pos > 0) {
+      next_start = k0 + last_prev;
+      next_cur += last_pos;
+    }
+    if (after == 64) {
+      next_cur += 1;
+      next_start = k0 + 64;
+    }
+    next_cur = uniform(next_cur);
+    if (k0 + 64 < le) ns = next_cur + 1 + lane < items ? starts[next_cur + 1 + lane] : ~0ull;
+
     uint4 w = make_uint4(0, 0, 0, 0);
-    if (valid) w = p.work[item];
-    const uint32_t meshlet = w.y + static_cast<uint32_t>(k - item_start);
-    if (valid && meshlet >= w.z) {  // the walk lost its item: report instead of reading past the range
+    if (valid) w = work[item];
+    const uint32_t task = w.y + static_cast<uint32_t>(k - item_start);
+    if (valid && task >= w.z) {  // the walk lost its item: report instead of reading past the range
       atomicOr(&p.flags[2], 1u);
       valid = false;
       w = make_uint4(0, 0, 0, 0);
     }
-    body(valid, item, w, meshlet);
+    body(valid, static_cast<uint32_t>(base + item), w, task);
+    cur = next_cur;
+    cur_start = next_start;
+  }
+}
 
-    // The item holding task k0 + 64.
-    const uint32_t last_pos = __shfl(pos, 63, 64);
-    const uint32_t last_prev = __shfl(prev, 63, 64);
-    const uint32_t after = __shfl(dist, static_cast<int>(last_pos), 64);
-    if (last_pos > 0) {
-      cur_start = k0 + last_prev;
-      cur += last_pos;
-    }
-    if (after == 64) {
-      cur += 1;
-      cur_start = k0 + 64;
-    }
-    cur = uniform(cur);
+// Calls body(valid, item, w, task) for 64 consecutive tasks at a time over this wave's equal share of
+// the flat task space: the partitions' task ranges, one after another (all lanes together).
+template <typename Body>
+__device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
+  if (p.flags[3]) return;
+  const uint32_t lane = lane_id();
+  const unsigned long long packed = lane < kWorkParts ? p.work_counter[lane * kWorkCounterSpacing] : 0ull;
+  const uint32_t part_items = min(static_cast<uint32_t>(packed >> kTaskBits), p.work_capacity);
+  const uint64_t part_tasks = part_items ? (packed & kTaskMask) : 0ull;
+  const uint64_t part_end = wave_scan64(part_tasks);  // end of each partition's tasks in the flat space
+  const uint64_t total = readlane64(part_end, 63);
+  if (total == 0) return;
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  const uint64_t wave = xcd_wave_rank();
+  const uint64_t seg_begin = total * wave / waves;
+  const uint64_t seg_end = total * (wave + 1) / waves;
+  for (uint64_t k0 = seg_begin; k0 < seg_end;) {
+    // The partition holding task k0: the first whose end is past it.
+    const uint32_t q = static_cast<uint32_t>(__popcll(__ballot(lane < kWorkParts && part_end <= k0)));
+    if (q >= kWorkParts) break;  // cannot happen: k0 < total
+    const uint64_t q_end = readlane64(part_end, q);
+    const uint64_t q_start = q_end - readlane64(part_tasks, q);
+    const uint64_t stop = min(seg_end, q_end);
+    walk_partition(p, static_cast<size_t>(q) * p.work_capacity,
+                   static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(part_items), static_cast<int>(q))), k0 - q_start,
+                   stop - q_start, body);
+    k0 = stop;
   }
 }
 
@@ -621,17 +671,22 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
 // Compiled for several occupancy targets (waves per SIMD); the launch schedule picks one.
 template <int kWavesPerSimd>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_meshlets(Params p) {
+  // Per wave: meshlets in the cut awaiting their visibility test, as (instance, meshlet, tag, fades) and the
+  // instance scale.
   __shared__ uint4 ring[4][kQueue];
+  __shared__ float ring_scale[4][kQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   uint32_t head = 0, tail = 0;  // wave-uniform ring positions
 
   auto drain = [&](uint32_t n) {
     const bool active = lane < n;
-    const uint4 c = active ? ring[wib][(head + lane) % kQueue] : make_uint4(0, 0, 0, 0);
-    const uint4 w = active ? p.work[c.x] : make_uint4(0, 0, 0, 0);
-    const uint32_t meshlet = c.y, tag = w.w;
+    const uint32_t at_ring = (head + lane) % kQueue;
+    const uint4 c = active ? ring[wib][at_ring] : make_uint4(0, 0, 0, 0);
+    const float scale = active ? ring_scale[wib][at_ring] : 0.0f;
+    const uint32_t instance = c.x, meshlet = c.y, tag = c.z;
     const uint32_t mslot = tag & kTagSlotMask, cut = (tag & kTagShadow) ? 1u : 0u, vi = tag >> kTagViewShift;
     const bool occ_on = (tag & kTagOcclusion) != 0, fast = (tag & kTagFast) != 0;
+    const uint32_t meshlet_tris = active ? p.m_tris[meshlet] : 0u;
     bool drawn = active && fast;  // fast path: guaranteed meshlets of a culling sphere inside the frustum
     bool occ_test = false;
     uint32_t vslot = 0;
@@ -644,8 +699,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
         const auto& v = scalar(p.views)[uniform(vu)];
         vslot = v.slot;
         if (!fast) {
-          const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-          const float scale = p.work_scale[c.x];
+          const float4 c0 = p.i_col0[instance], c1 = p.i_col1[instance], c2 = p.i_col2[instance], c3 = p.i_col3[instance];
           const float4 cs = p.m_cull[meshlet];
           const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
           bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
@@ -656,16 +710,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       }
       pending &= ~__ballot(mine);
     }
-    uint32_t tris = 0;
-    if (drawn) tris = p.m_tris[meshlet];
+    const uint32_t tris = drawn ? meshlet_tris : 0u;
     head += n;
-    const uint32_t packed_tris = tris | (c.z << 8);
+    const uint32_t packed_tris = tris | (c.w << 8);
     wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
     const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
     const bool write = drawn && at < p.max_meshlets;
     const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
     const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
-    if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(w.x, meshlet, first_index, packed_tris);
+    if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(instance, meshlet, first_index, packed_tris);
   };
   auto wave_sync = [&]() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -677,6 +730,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const uint32_t cut = (w.w & kTagShadow) ? 1u : 0u, vi = w.w >> kTagViewShift;
     const bool fast = (w.w & kTagFast) != 0;
     uint32_t fades = 0, first = 0, count = 0;
+    float scale = 0.0f;
     if (valid && fast) {
       // Fast-path item: the task is a meshlet, drawn at full fade.
       fades = kFadeSteps << 8;
@@ -692,7 +746,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
         const auto& v = scalar(p.views)[uniform(vu)];
         const uint32_t run = task;
         const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-        const float scale = p.work_scale[item];
+        scale = p.work_scale[item];
         const float2 err = p.r_err[run];
         const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
         const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
@@ -717,7 +771,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
     if (total) {
       const uint32_t at = tail + incl - count;
-      for (uint32_t k = 0; k < count; k++) ring[wib][(at + k) % kQueue] = make_uint4(item, first + k, fades, 0u);
+      for (uint32_t k = 0; k < count; k++) {
+        const uint32_t r = (at + k) % kQueue;
+        ring[wib][r] = make_uint4(w.x, first + k, w.w, fades);
+        ring_scale[wib][r] = scale;
+      }
       tail += total;
       while (tail - head >= 64) {
         wave_sync();
@@ -759,6 +817,9 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), wrank = xcd_wave_rank();
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (wrank + 1) / waves);
+  const uint32_t part = static_cast<uint32_t>(wrank * kWorkParts / waves);  // this wave's work-list partition
+  const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
+  uint32_t high_water = 0;  // the most items the partition held after one of this wave's appends
   for (uint32_t q = static_cast<uint32_t>(queue_count * wrank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
     const uint32_t vi = uniform(entry.x);
@@ -815,23 +876,25 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       const uint32_t task_incl = wave_scan(len);
       const uint32_t tasks = __shfl(task_incl, 63, 64);
       unsigned long long old = 0;
-      if (lane == 0) old = atomicAdd(p.work_counter, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
+      if (lane == 0) old = atomicAdd(p.work_counter + part * kWorkCounterSpacing, (static_cast<unsigned long long>(items) << kTaskBits) | tasks);
       const uint32_t old_lo = __shfl(static_cast<uint32_t>(old), 0, 64);
       const uint32_t old_hi = __shfl(static_cast<uint32_t>(old >> 32), 0, 64);
       const uint64_t o = (static_cast<uint64_t>(old_hi) << 32) | old_lo;
       const uint32_t item_base = static_cast<uint32_t>(o >> kTaskBits);
       const uint64_t task_base = o & kTaskMask;
       if (lane == 0 && (task_base + tasks >= kTaskLimit || item_base + items > p.work_capacity)) atomicOr(&p.flags[3], 1u);
-      if (lane == 0) atomicMax(&p.flags[4], item_base + items);
+      high_water = max(high_water, item_base + items);
       if (work && item_base + rank < p.work_capacity) {
         const uint32_t tag = slot | (occlusion_on ? kTagOcclusion : 0u) | (vi << kTagViewShift);
-        p.work[item_base + rank] = make_uint4(instance, lr.first, lr.last, tag);
-        p.work_start[item_base + rank] = task_base + task_incl - len;
-        p.work_scale[item_base + rank] = scale;
+        const size_t g = part_base + item_base + rank;
+        p.work[g] = make_uint4(instance, lr.first, lr.last, tag);
+        p.work_start[g] = task_base + task_incl - len;
+        p.work_scale[g] = scale;
       }
       wave_count_add(metric_at(p, v.slot, M_BUDGET_TESTS), len, work);
     }
   }
+  if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
 }
 
 __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
@@ -906,7 +969,8 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
 // One block per view with a budget: prefix over the bins, pick the threshold, redo the bands.
 __global__ __launch_bounds__(64) void k_budget_pick(Params p) {
   ViewDev& v = p.views[blockIdx.x];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *p.work_counter = 0;  // the regular instance pass refills the work list
+  if (blockIdx.x == 0 && threadIdx.x < kWorkParts)  // the regular instance pass refills the work list
+    p.work_counter[threadIdx.x * kWorkCounterSpacing] = 0;
   if (!v.budget) return;
   const uint32_t bins = p.budget_bins;
   const unsigned long long* h = p.budget_hist + static_cast<size_t>(blockIdx.x) * (bins + 1);
@@ -987,10 +1051,11 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
   const uint32_t slot = v.slot;
   prefix_cut(p, slot, threadIdx.x / 64);
   __syncthreads();
+  if (mode == 0 && blockIdx.x == 0 && threadIdx.x < kWorkParts)  // the retry round's instance pass refills the work list
+    p.work_counter[threadIdx.x * kWorkCounterSpacing] = 0;
   if (threadIdx.x != 0) return;
   uint32_t* state = state_at(p, slot);
   if (mode == 0) {
-    if (blockIdx.x == 0) *p.work_counter = 0;  // the retry round's instance pass refills the work list
     bool any = false;
     for (uint32_t cut = 0; cut < 2; cut++) {
       const float fill = fill_of(p, slot, cut);

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "vgcut_host.h"
+#include "vgcut_kernels.h"
 
 using namespace vgi;
 
@@ -68,7 +69,8 @@ void batch_sizes(const SceneEntry& se, uint32_t views, bool full, uint32_t first
   transient_sizes(se.scene, views, queue_bytes, work_items);
   if (full) return;
   const size_t guess = se.items_per_view ? static_cast<size_t>(se.items_per_view) * views * 5 / 4 + 4096 : first_items;
-  *work_items = std::min<size_t>(*work_items, guess);
+  // whole partitions (the device splits the list into kWorkParts equal parts)
+  *work_items = std::min<size_t>(*work_items, (guess + vg::kWorkParts - 1) / vg::kWorkParts * vg::kWorkParts);
 }
 
 // Transient bytes of a launch of `views` views of a scene: the (view, cell) queue and the work list.
@@ -125,7 +127,10 @@ struct vg_server {
       }
       {
         SceneEntry& se = scenes[b.scene];
-        const uint32_t per_view = (lane_flags[b.lane][4] + b.views - 1) / std::max(1u, b.views);
+        // flags[4]: the most items any partition took; the list is split evenly, so the batch needed
+        // kWorkParts times that
+        const uint64_t need = static_cast<uint64_t>(lane_flags[b.lane][4]) * vg::kWorkParts;
+        const uint32_t per_view = static_cast<uint32_t>((need + b.views - 1) / std::max(1u, b.views));
         se.items_per_view = std::max(se.items_per_view, per_view);
       }
       if (status == VG_E_CAPACITY && !b.full) {
