@@ -587,32 +587,26 @@ __device__ __forceinline__ void walk_partition(const Params& p, size_t base, uin
   uint64_t ns = cur + 1 + lane < items ? starts[cur + 1 + lane] : ~0ull;
 
   for (uint64_t k0 = lb; k0 < le; k0 += 64) {
-    // Distances from k0 to the first tasks of the next 64 items (at least 1, clamped to 65).
+    // Distances from k0 to the first tasks of the next 64 items (at least 1, clamped to 65). Item starts
+    // strictly increase, so the items starting inside this chunk are the set bits of `starts_here`
+    // (bit t: an item starts at task k0 + t): a lane's item is `cur` plus the number of starts at or
+    // before it, and the item's first task is the highest such start.
     const uint32_t dist = ns - k0 > 65 ? 65u : static_cast<uint32_t>(ns - k0);
-    // pos = number of items after `cur` that start at or before task k0 + lane.
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t s = 32; s >= 1; s >>= 1) {
-      const uint32_t d = __shfl(dist, static_cast<int>(pos + s - 1), 64);
-      if (d <= lane) pos += s;
-    }
-    const uint32_t prev = __shfl(dist, static_cast<int>(pos > 0 ? pos - 1 : 0), 64);
+    const uint64_t bit = dist < 64 ? 1ull << dist : 0ull;
+    const uint64_t starts_here = (static_cast<uint64_t>(wave_or(static_cast<uint32_t>(bit >> 32))) << 32) |
+                                 wave_or(static_cast<uint32_t>(bit));
+    const uint64_t upto = starts_here & ((2ull << lane) - 1ull);  // starts at tasks k0 + 1 .. k0 + lane
+    const uint32_t pos = static_cast<uint32_t>(__popcll(upto));
     const uint64_t k = k0 + lane;
     const uint32_t item = cur + pos;
-    const uint64_t item_start = pos == 0 ? cur_start : k0 + prev;
+    const uint64_t item_start = pos == 0 ? cur_start : k0 + (63 - static_cast<uint32_t>(__clzll(upto)));
     bool valid = k < le && item < items && k >= item_start;
 
     // The item holding task k0 + 64, and the first tasks of its successors, loaded while this chunk runs.
-    const uint32_t last_pos = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), 63));
-    const uint32_t last_prev = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(prev), 63));
-    const uint32_t after = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dist), static_cast<int>(last_pos)));
-    uint32_t next_cur = cur;
-    uint64_t next_start = cur_start;
-    if (last_pos > 0) {
-      next_start = k0 + last_prev;
-      next_cur += last_pos;
-    }
-    if (after == 64) {
+    const uint32_t last_pos = static_cast<uint32_t>(__popcll(starts_here));
+    uint32_t next_cur = cur + last_pos;
+    uint64_t next_start = last_pos ? k0 + (63 - static_cast<uint32_t>(__clzll(starts_here))) : cur_start;
+    if (__ballot(dist == 64)) {  // an item starts exactly at the next chunk
       next_cur += 1;
       next_start = k0 + 64;
     }
@@ -677,6 +671,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
   __shared__ float ring_scale[4][kQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   uint32_t head = 0, tail = 0;  // wave-uniform ring positions
+  // Occlusion tests of the current view slot, added to its metric when the slot changes (and at the end).
+  uint32_t occ_slot = 0xffffffffu, occ_count = 0;
+  auto occ_flush = [&]() {
+    if (occ_count && lane == 0) atomicAdd(metric_at(p, occ_slot, M_OCCLUSION), static_cast<unsigned long long>(occ_count));
+    occ_count = 0;
+  };
 
   auto drain = [&](uint32_t n) {
     const bool active = lane < n;
@@ -713,7 +713,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const uint32_t tris = drawn ? meshlet_tris : 0u;
     head += n;
     const uint32_t packed_tris = tris | (c.w << 8);
-    wave_count_add_keyed(metric_at(p, vslot, M_OCCLUSION), 1u, occ_test);
+    for (uint64_t tests = __ballot(occ_test); tests;) {
+      const uint32_t key = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vslot), __ffsll(static_cast<unsigned long long>(tests)) - 1));
+      const uint64_t same = __ballot(occ_test && vslot == key);
+      if (key != occ_slot) {
+        occ_flush();
+        occ_slot = key;
+      }
+      occ_count += static_cast<uint32_t>(__popcll(same));
+      tests &= ~same;
+    }
     const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
     const bool write = drawn && at < p.max_meshlets;
     const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
@@ -792,6 +801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     wave_sync();
     drain(tail - head);
   }
+  occ_flush();
 }
 
 // ------------------------------------------------------------------ triangle budget (projected-error histogram)
