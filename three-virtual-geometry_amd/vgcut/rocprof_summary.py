@@ -12,7 +12,7 @@ HBM traffic: FETCH_SIZE and WRITE_SIZE are reported in KiB per dispatch. Their s
 plain device copy of `--copy-bytes` bytes read and the same written (the largest `copyBuffer` dispatch
 of the run): factor = true bytes / counted bytes, per counter. A kernel's traffic per dispatch is
 FETCH_SIZE * 1024 * fetch_factor + WRITE_SIZE * 1024 * write_factor, averaged over its busy dispatches
-(those above 1% of its largest one, which leaves out the empty retry rounds).
+(those above 10% of its largest one, which leaves out the retry rounds that find nothing to redo).
 """
 from __future__ import annotations
 
@@ -57,7 +57,7 @@ def counter_values(db: str, counter: str) -> Dict[str, List[float]]:
 
 def busy(values: List[float]) -> List[float]:
     top = max(values) if values else 0.0
-    return [v for v in values if v > 0.01 * top]
+    return [v for v in values if v > 0.1 * top]
 
 
 def calibration(values: Dict[str, List[float]], true_bytes: float) -> float:
