@@ -42,6 +42,12 @@ typedef struct vg_config {
   uint32_t hzb_max_size;         /* largest HZB side (OcclusionCulling maxSize, default 1024) */
   uint32_t budget_bins;          /* thresholds of the triangle-budget histogram (default 256) */
   int32_t device;                /* HIP device ordinal; negative: host-only context (pool layout, no cuts) */
+  /*
+   * 1: every cut ends with the expand pass (GeometryPool.ts:997-1021): each view slot holds an index
+   * buffer per cut of max_triangles * 3 (camera) and max_shadow_triangles * 3 (shadow) u32, and every
+   * mesh must come with its meshlet triangles. 0: cuts end at the indirect args (no index buffers).
+   */
+  uint32_t expand;
 } vg_config;
 
 /* The VirtualMeshData fields the cut reads (buildVirtualMesh.ts:75-138). */
@@ -56,6 +62,13 @@ typedef struct vg_mesh_desc {
   float bounding_sphere[4];
   float lod_bounds_sphere[4];
   float cull_bounds_sphere[4];
+  /*
+   * VirtualMeshData.meshletTriangles (buildVirtualMesh.ts:75-138, GeometryPool.ts:430): 3 local vertex
+   * ids per triangle packed a | b << 8 | c << 16; meshlet i's triangles start at meshlet_info[4i + 1] / 3.
+   * Needed by the expand pass; may be NULL (triangle_count 0) on a context without it.
+   */
+  const uint32_t* meshlet_triangles;
+  uint64_t triangle_count;
 } vg_mesh_desc;
 
 /*
@@ -148,6 +161,15 @@ int vg_sync(vg_scene* scene);
  */
 int vg_read(vg_scene* scene, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list,
             uint32_t max_entries, vg_metrics* metrics);
+/*
+ * The index buffer of a view slot's cut (0 camera, 1 shadow) that the expand pass wrote
+ * (GeometryPool.ts:997-1021; cameraIndex / shadowIndex, read by the indirect draws, VirtualMesh.ts:266-284):
+ * `count` u32 from index `first`, after vg_sync. An index is (draw-list position << 7) | local vertex,
+ * where the position counts shadow entries from max_meshlets. Contexts created with expand only.
+ */
+int vg_index_read(vg_scene* scene, uint32_t view_slot, uint32_t cut, uint64_t first, uint64_t count, uint32_t* dst);
+/* Device address and capacity (u32 indices) of that index buffer, for consumers on the same device. */
+int vg_index_buffer(vg_scene* scene, uint32_t view_slot, uint32_t cut, void** device_ptr, uint64_t* capacity);
 /* Capacity-controller state words of a view slot (8 u32, x10000). */
 int vg_state_get(vg_scene* scene, uint32_t view_slot, uint32_t* state);
 int vg_state_set(vg_scene* scene, uint32_t view_slot, const uint32_t* state);
@@ -163,6 +185,7 @@ int vg_state_set(vg_scene* scene, uint32_t view_slot, const uint32_t* state);
 #define VG_POOL_CELLS 2          /* f32: 8 per cell */
 #define VG_POOL_MESHLET_BOUNDS 3 /* f32: 16 per meshlet (pool-global ids) */
 #define VG_POOL_MESHLET_INFO 4   /* u32: 4 per meshlet, firstIndex rebased to the pool */
+#define VG_POOL_MESHLET_TRIANGLES 5 /* u32: 1 per triangle (packed local ids), at firstIndex / 3 */
 int vg_scene_pool_read(const vg_scene* scene, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes);
 /* Highest used mesh slot + 1 (GeometryPool.meshSlots). */
 uint32_t vg_scene_mesh_slots(const vg_scene* scene);
@@ -171,7 +194,7 @@ int vg_mesh_instance_base(const vg_scene* scene, uint32_t slot, uint32_t* base, 
 
 /*
  * Measurement. vg_profile(scene, 1) times every kernel of eager launch sequences with HIP events on the
- * context's stream; vg_profile_read returns the summed milliseconds per kernel position (11 positions,
+ * context's stream; vg_profile_read returns the summed milliseconds per kernel position (12 positions,
  * `names` receives their names), the number of sequences, and resets them. vg_footprint runs one cut
  * (like vg_cut, then waits) and counts the distinct instances tested and distinct meshlet records in
  * candidate ranges over all views (the unique inputs one launch must read).

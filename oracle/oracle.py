@@ -100,6 +100,9 @@ def load():
         lib.or_budget_histogram.argtypes = [ctypes.POINTER(OrScene), ctypes.POINTER(OrView), ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_uint32, ctypes.c_void_p]
         lib.or_budget_histogram.restype = ctypes.c_int
+        lib.or_expand.argtypes = [ctypes.POINTER(OrScene), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        lib.or_expand.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -117,6 +120,7 @@ class Pool:
     mesh_slots: int
     max_meshlets: int
     cut_capacity: tuple
+    triangles: Optional[np.ndarray] = None  # u32 meshletTriangles of the pool
 
     def struct(self) -> OrScene:
         s = OrScene()
@@ -154,6 +158,7 @@ def pool_from_scene(scene, max_meshlets: int, max_triangles: int, max_shadow_tri
         mesh_slots=scene.mesh_slots,
         max_meshlets=max_meshlets,
         cut_capacity=(max_triangles * 3, max_shadow_triangles * 3),
+        triangles=f(native.VG_POOL_MESHLET_TRIANGLES, np.uint32),
     )
 
 
@@ -233,6 +238,26 @@ def occluded(proj, near: float, hzb: np.ndarray, view_center, radius: float) -> 
 
 def logf(x: float) -> float:
     return float(load().or_logf(x))
+
+
+def expand(pool: Pool, cut: int, counters: np.ndarray, draws: np.ndarray, indices: Optional[np.ndarray] = None):
+    """The expand pass (GeometryPool.ts:997-1021) over a cut result: `counters` of the view slot and its
+    draw list `draws` (2 * max_meshlets uvec4, camera then shadow). Returns (indices, written): the index
+    buffer of `cut` (a copy of `indices`, or zeros, with the written positions replaced) and the count."""
+    lib = load()
+    cap = int(pool.cut_capacity[cut])
+    out = np.zeros(cap, dtype=np.uint32) if indices is None else np.array(indices, dtype=np.uint32, copy=True)
+    assert len(out) == cap
+    d = np.ascontiguousarray(draws, dtype=np.uint32).reshape(-1)
+    assert len(d) == 2 * pool.max_meshlets * 4
+    c = np.ascontiguousarray(counters, dtype=np.uint32)
+    t = np.ascontiguousarray(pool.triangles, dtype=np.uint32)
+    written = ctypes.c_uint64()
+    sc = pool.struct()
+    rc = lib.or_expand(ctypes.byref(sc), t.ctypes.data, len(t), cut, c.ctypes.data, d.ctypes.data, out.ctypes.data, ctypes.byref(written))
+    if rc != 0:
+        raise RuntimeError(f"or_expand failed ({rc})")
+    return out, int(written.value)
 
 
 def budget_histogram(pool: Pool, view, thresholds: np.ndarray, state: Optional[np.ndarray] = None,

@@ -751,3 +751,41 @@ int or_budget_histogram(const or_scene* s, const or_view* v, const uint32_t* sta
   free(c.cell_list);
   return 0;
 }
+
+/*
+ * expandPass(cut), GeometryPool.ts:997-1021: for draw-list positions at < C_DRAWN of `cut`, the entry's
+ * mesh slot is its instance record's .w of column 4, its part comes from the blend bits, and triangle i
+ * writes 3 indices at regionStart + firstIndex + 3i, each (position << 7) | local id (GeometryPool.ts:92,
+ * 1009), when index + 2 < capacity. Sums are u32 as in WGSL; an index whose + 2 wraps past 2^32 is
+ * skipped (the reference's storage access would be out of bounds there).
+ */
+int or_expand(const or_scene* s, const uint32_t* triangles, uint64_t triangle_words, int cut, const uint32_t* counters,
+              const uint32_t* draw_list, uint32_t* indices, uint64_t* written) {
+  if (!s || !triangles || !counters || !draw_list || !indices || cut < 0 || cut > 1) return -1;
+  const uint32_t capacity = s->cut_capacity[cut];
+  const uint32_t drawn = counters[2 + cut * CUT_COUNTERS + C_DRAWN];
+  uint64_t n = 0;
+  for (uint32_t at = 0; at < drawn; at++) {
+    const uint32_t position = at + (uint32_t)cut * s->max_meshlets;
+    const uint32_t* e = &draw_list[(size_t)position * 4];
+    if (e[0] >= s->instance_count || e[1] >= s->meshlet_count) return -2;
+    const uint32_t slot = (uint32_t)s->instances[(size_t)e[0] * INSTANCE_FLOATS + 19];
+    const uint32_t part = (e[3] & 0xffff00u) == FULL_FADE_BITS ? 0u : 1u;
+    const uint32_t region = counters[PER_MESH_COUNTERS + (uint32_t)cut * s->slots * 4 + 1 + slot * 4 + part * 2];
+    const uint32_t first_triangle = s->info[(size_t)e[1] * 4 + 1] / 3u;
+    const uint32_t base = position << 7;
+    const uint32_t count = e[3] & 255u;
+    for (uint32_t i = 0; i < count; i++) {
+      const uint32_t index = region + e[2] + i * 3u;
+      if (index > 0xfffffffdu || index + 2u >= capacity) continue;
+      if ((uint64_t)first_triangle + i >= triangle_words) return -3;
+      const uint32_t packed = triangles[first_triangle + i];
+      indices[index] = base | (packed & 255u);
+      indices[index + 1] = base | ((packed >> 8) & 255u);
+      indices[index + 2] = base | ((packed >> 16) & 255u);
+      n += 3;
+    }
+  }
+  if (written) *written = n;
+  return 0;
+}

@@ -99,6 +99,15 @@ float or_logf(float x);
 int or_budget_histogram(const or_scene* scene, const or_view* view, const uint32_t* state, const float* thresholds,
                         uint32_t bins, uint64_t* triangles);
 
+/*
+ * The expand pass (GeometryPool.ts:997-1021) over a given cut result: `counters` and `draw_list` as
+ * or_cut or the GPU leave them (2 * max_meshlets entries), `triangles` the pool's meshletTriangles.
+ * Writes into `indices` (capacity cut_capacity[cut]) and leaves other positions as they are; `written`
+ * receives the number of indices written. Returns 0, or < 0 on an entry outside the pool.
+ */
+int or_expand(const or_scene* scene, const uint32_t* triangles, uint64_t triangle_words, int cut, const uint32_t* counters,
+              const uint32_t* draw_list, uint32_t* indices, uint64_t* written);
+
 #ifdef __cplusplus
 }
 #endif

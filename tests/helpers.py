@@ -94,9 +94,10 @@ class Loaded:
     slot_of_instance: np.ndarray
 
 
-def load_gpu(spec, max_meshlets=1 << 16, max_triangles=2_000_000, max_shadow=1_000_000, max_views=1, occlusion=False, mesh_slots=64):
+def load_gpu(spec, max_meshlets=1 << 16, max_triangles=2_000_000, max_shadow=1_000_000, max_views=1, occlusion=False, mesh_slots=64,
+             expand=False):
     ctx = native.Context(max_triangles=max_triangles, max_shadow_triangles=max_shadow, max_meshlets=max_meshlets,
-                         max_views=max_views, mesh_slots=mesh_slots, device=0)
+                         max_views=max_views, mesh_slots=mesh_slots, device=0, expand=expand)
     scene = native.Scene(ctx)
     scenes.load_into(scene, spec, occlusion=occlusion)
     pool = oracle.pool_from_scene(scene, max_meshlets, max_triangles, max_shadow)

@@ -107,7 +107,7 @@ void lane_destroy(Lane* lane) {
   lane->graph_exec = nullptr;
   lane->graph = nullptr;
   DevBuf* bufs[] = {&lane->d_views, &lane->d_cell_queue_count, &lane->d_work_counter, &lane->d_flags, &lane->d_budget_thresholds,
-                    &lane->d_budget_hist};
+                    &lane->d_budget_hist, &lane->d_expand_plan};
   for (DevBuf* b : bufs) release_lane(lane, *b);
   if (!lane->borrowed) {
     DevBuf* t[] = {&lane->d_cell_queue, &lane->d_work, &lane->d_work_start, &lane->d_work_scale};
@@ -259,7 +259,8 @@ int upload_scene(vg_scene* s) {
     if ((rc = upload(s, s->d_cull, s->m_cull)) || (rc = upload(s, s->d_lod, s->m_lod)) || (rc = upload(s, s->d_parent, s->m_parent)) ||
         (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)) || (rc = upload(s, s->d_run, s->m_run)) ||
         (rc = upload(s, s->d_rspan, s->r_span)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
-        (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)))
+        (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)) ||
+        (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)))
       return rc;
     s->dirty_meshlets = false;
   }
@@ -270,6 +271,10 @@ int upload_scene(vg_scene* s) {
   if (s->dirty_instances) {
     for (int k = 0; k < 4; k++)
       if ((rc = upload(s, s->d_col[k], s->i_col[k]))) return rc;
+    std::vector<uint32_t> islot(s->i_slot.size());
+    for (size_t i = 0; i < islot.size(); i++) islot[i] = static_cast<uint32_t>(s->i_slot[i]);
+    if ((rc = upload(s, s->d_islot, islot))) return rc;
+    VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));  // islot is a temporary
     s->dirty_instances = false;
   }
   if (s->dirty_cells) {
@@ -294,6 +299,20 @@ int ensure_slot(vg_scene* s, uint32_t slot) {
     s->h_draw_ptrs[slot] = static_cast<uint4*>(s->d_draws[slot].ptr);
     VG_HIP(ctx, hipMemcpyAsync(static_cast<uint4**>(s->d_draw_ptrs.ptr) + slot, &s->h_draw_ptrs[slot], sizeof(uint4*),
                                hipMemcpyHostToDevice, ctx->stream));
+    VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (ctx->cfg.expand && !s->d_indices[2 * slot].ptr) {
+    // cameraIndex / shadowIndex (GeometryPool.ts:642-644): triangles * 3 indices per cut.
+    const size_t caps[2] = {static_cast<size_t>(ctx->cfg.max_triangles) * 3, static_cast<size_t>(ctx->cfg.max_shadow_triangles) * 3};
+    uint32_t* ptrs[2];
+    for (int c = 0; c < 2; c++) {
+      DevBuf& b = s->d_indices[2 * slot + c];
+      if (int rc = ensure(s, b, sizeof(uint32_t) * caps[c])) return rc;
+      VG_HIP(ctx, hipMemsetAsync(b.ptr, 0, b.bytes, ctx->stream));
+      ptrs[c] = static_cast<uint32_t*>(b.ptr);
+    }
+    VG_HIP(ctx, hipMemcpyAsync(static_cast<uint32_t**>(s->d_index_ptrs.ptr) + 2 * slot, ptrs, sizeof(ptrs), hipMemcpyHostToDevice,
+                               ctx->stream));
     VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return 0;
@@ -346,6 +365,11 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.budget_thresholds = static_cast<const float*>(lane->d_budget_thresholds.ptr);
   p.budget_hist = static_cast<unsigned long long*>(lane->d_budget_hist.ptr);
   p.budget_bins = ctx->cfg.budget_bins;
+  p.i_slot = static_cast<const uint32_t*>(s->d_islot.ptr);
+  p.m_first = static_cast<const uint32_t*>(s->d_first.ptr);
+  p.m_tri = static_cast<const uint32_t*>(s->d_tri.ptr);
+  p.indices = static_cast<uint32_t* const*>(s->d_index_ptrs.ptr);
+  p.expand_plan = static_cast<unsigned long long*>(lane->d_expand_plan.ptr);
   if (s->footprint) {
     p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
     p.fp_meshlets = static_cast<uint32_t*>(s->d_fp_meshlets.ptr);
@@ -353,10 +377,11 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   return p;
 }
 
-constexpr int kKernelSlots = 11;
+constexpr int kKernelSlots = 12;
 const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells (+ budget passes)", "k_instances", "k_meshlets", "k_prefix(check 1)",
                                                 "k_instances(retry 1)", "k_meshlets(retry 1)", "k_prefix(check 2)",
-                                                "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)"};
+                                                "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)",
+                                                "k_expand (+ plan)"};
 
 hipEvent_t take_event(vg_scene* s) {
   if (!s->event_pool.empty()) {
@@ -370,7 +395,8 @@ hipEvent_t take_event(vg_scene* s) {
 }
 
 // The launch sequence of one cut (GeometryPool.ts:1081-1092): begin, cells, instances, meshlets,
-// prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer).
+// prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer), then the expand pass on
+// contexts that have it.
 using MeshletKernel = void (*)(Params);
 MeshletKernel meshlet_kernel(uint32_t waves_per_simd) {
   switch (waves_per_simd) {
@@ -434,6 +460,11 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   mark();
   hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 1u, 0u);
   mark();
+  if (ctx->cfg.expand) {
+    hipLaunchKernelGGL(k_expand_plan, dim3(1), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_expand, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
+  }
+  mark();
   if (timed) s->pending.push_back(ev);
   VG_HIP(ctx, hipGetLastError());
   return 0;
@@ -442,6 +473,9 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
 int lane_reserve_views(vg_ctx* ctx, Lane* lane) {
   VG_HIP(ctx, hipStreamSynchronize(lane->stream));
   if (int rc = ensure_lane(ctx, lane, lane->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
+  if (ctx->cfg.expand)
+    if (int rc = ensure_lane(ctx, lane, lane->d_expand_plan, sizeof(unsigned long long) * (2 * static_cast<size_t>(ctx->cfg.max_views) + 1)))
+      return rc;
   for (int r = 0; r < Lane::kRing; r++) {
     if (lane->h_ring[r]) (void)hipHostFree(lane->h_ring[r]);
     if (lane->h_ring_thresholds[r]) (void)hipHostFree(lane->h_ring_thresholds[r]);
@@ -634,7 +668,7 @@ int vg_create(const vg_config* config, vg_ctx** out) {
   if (!c.hzb_max_size) c.hzb_max_size = 1024;
   if (!c.budget_bins) c.budget_bins = 256;
   if (c.budget_bins > kMaxBudgetBins || c.max_meshlets > (1u << 24) || c.mesh_slots > 65535 || c.max_views > kMaxLaunchViews || c.hzb_max_size > 4096 ||
-      c.max_triangles > 0x55555555u || c.max_shadow_triangles > 0x55555555u)
+      c.max_triangles > 0x55555555u || c.max_shadow_triangles > 0x55555555u || c.expand > 1)
     return VG_E_INVALID;
   auto* ctx = new (std::nothrow) vg_ctx();
   if (!ctx) return VG_E_NOMEM;
@@ -683,6 +717,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
   s->args_stride = s->slots * 20;
   s->d_draws.resize(views);
   s->d_hzbs.resize(views);
+  s->d_indices.resize(2 * static_cast<size_t>(views));
   s->hzb_valid.assign(views, 0);
   s->h_draw_ptrs.assign(views, nullptr);
   s->h_hzb_ptrs.assign(views, nullptr);
@@ -692,7 +727,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
         (rc = ensure(s, s->d_args, sizeof(uint32_t) * s->args_stride * views)) ||
         (rc = ensure(s, s->d_metrics, sizeof(unsigned long long) * kMetricWords * views)) ||
         (rc = ensure(s, s->d_draw_ptrs, sizeof(uint4*) * views)) || (rc = ensure(s, s->d_hzb_ptrs, sizeof(float*) * views)) ||
-        (rc = lane_init(ctx, &s->lane, ctx->stream)))
+        (rc = ensure(s, s->d_index_ptrs, sizeof(uint32_t*) * 2 * views)) || (rc = lane_init(ctx, &s->lane, ctx->stream)))
       return rc;
     // Controller state: scale 1 for both cuts (GeometryPool.ts:602-603), everything else 0.
     std::vector<uint32_t> counters(static_cast<size_t>(s->counter_stride) * views, 0u);
@@ -705,6 +740,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
     VG_HIP(ctx, hipMemset(s->d_metrics.ptr, 0, s->d_metrics.bytes));
     VG_HIP(ctx, hipMemset(s->d_draw_ptrs.ptr, 0, s->d_draw_ptrs.bytes));
     VG_HIP(ctx, hipMemset(s->d_hzb_ptrs.ptr, 0, s->d_hzb_ptrs.bytes));
+    VG_HIP(ctx, hipMemset(s->d_index_ptrs.ptr, 0, s->d_index_ptrs.bytes));
     return 0;
   };
   if (device && (rc = init())) {
@@ -721,9 +757,11 @@ void vg_scene_destroy(vg_scene* s) {
   lane_destroy(&s->lane);
   DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
-                    &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts};
+                    &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
+                    &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
+  for (DevBuf& b : s->d_indices) release(s, b);
   for (DevBuf& b : s->d_hzbs) release(s, b);
   for (auto& seq : s->pending)
     for (hipEvent_t e : seq)
@@ -742,6 +780,20 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
       !mesh->level_guard_errors)
     return fail(ctx, VG_E_INVALID, "vg_mesh_add: empty or incomplete mesh");
   if (s->m_cull.size() + n > kMaxMeshletsPerScene) return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: more than 2^24 meshlets in a scene");
+  for (uint32_t i = 0; i < n; i++)
+    if (mesh->meshlet_info[static_cast<size_t>(i) * 4] > 255)  // the draw list packs it into 8 bits (GeometryPool.ts:957-962)
+      return fail(ctx, VG_E_INVALID, "vg_mesh_add: a meshlet has more than 255 triangles");
+  if (ctx->cfg.expand && !mesh->meshlet_triangles)
+    return fail(ctx, VG_E_INVALID, "vg_mesh_add: the expand pass needs the meshlet triangles");
+  if (mesh->meshlet_triangles) {
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t* info = mesh->meshlet_info + static_cast<size_t>(i) * 4;
+      if (info[1] % 3 != 0 || static_cast<uint64_t>(info[1] / 3) + info[0] > mesh->triangle_count)
+        return fail(ctx, VG_E_INVALID, "vg_mesh_add: a meshlet's triangles lie outside meshlet_triangles");
+    }
+    if (s->index_used + mesh->triangle_count * 3 > 0xffffffffull)
+      return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: more than 2^32 indices in a scene");
+  }
   uint32_t slot = 0;
   while (slot < s->slots && s->meshes[slot].used) slot++;
   if (slot >= s->slots) return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: no free mesh slot");
@@ -769,6 +821,7 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
     s->m_info.push_back(static_cast<uint32_t>(info[1] + s->index_used));
     s->m_info.push_back(info[2]);
     s->m_info.push_back(info[3]);
+    s->m_first.push_back(static_cast<uint32_t>(info[1] + s->index_used));
   }
   // LOD runs: consecutive meshlets of one level with the same LOD-test inputs (bit-equal spheres and
   // errors; any sphere with a zero error or an error >= 5e29, which project to 0 and 1e30).
@@ -805,8 +858,15 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
       i += len;
     }
   }
+  // The pool's triangle array grows by the mesh's triangles (GeometryPool.ts:326, 430); firstIndex is
+  // rebased by 3 x its base. A mesh given without triangles takes the space its meshlets' counts need.
   uint64_t mesh_indices = 0;
   for (uint32_t i = 0; i < n; i++) mesh_indices += static_cast<uint64_t>(mesh->meshlet_info[static_cast<size_t>(i) * 4]) * 3;
+  if (mesh->meshlet_triangles) {
+    s->m_tri.resize(s->index_used / 3, 0u);
+    s->m_tri.insert(s->m_tri.end(), mesh->meshlet_triangles, mesh->meshlet_triangles + mesh->triangle_count);
+    mesh_indices = mesh->triangle_count * 3;
+  }
   s->index_used += mesh_indices;
   // Level table after the mesh records: (first meshlet, count, min own, max parent), (max own, min parent, 0, 0)
   // (GeometryPool.ts:432-445).
@@ -976,6 +1036,31 @@ int vg_read(vg_scene* s, uint32_t view_slot, uint32_t* counters, uint32_t* draw_
   return read_slot(s, ctx->stream, view_slot, counters, draw_args, draw_list, max_entries, metrics);
 }
 
+int vg_index_buffer(vg_scene* s, uint32_t view_slot, uint32_t cut, void** device_ptr, uint64_t* capacity) {
+  if (!s || cut > 1) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  if (!ctx->cfg.expand) return fail(ctx, VG_E_INVALID, "vg_index_buffer: the context was created without the expand pass");
+  if (view_slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_index_buffer: view slot out of range");
+  if (int rc = ensure_slot(s, view_slot)) return rc;
+  const DevBuf& b = s->d_indices[2 * view_slot + cut];
+  if (device_ptr) *device_ptr = b.ptr;
+  if (capacity) *capacity = static_cast<uint64_t>(cut ? ctx->cfg.max_shadow_triangles : ctx->cfg.max_triangles) * 3;
+  return VG_OK;
+}
+
+int vg_index_read(vg_scene* s, uint32_t view_slot, uint32_t cut, uint64_t first, uint64_t count, uint32_t* dst) {
+  void* ptr = nullptr;
+  uint64_t capacity = 0;
+  if (int rc = vg_index_buffer(s, view_slot, cut, &ptr, &capacity)) return rc;
+  if (first > capacity || count > capacity - first) return fail(s->ctx, VG_E_INVALID, "vg_index_read: range outside the index buffer");
+  if (count == 0) return VG_OK;
+  if (!dst) return VG_E_INVALID;
+  VG_HIP(s->ctx, hipMemcpyAsync(dst, static_cast<uint32_t*>(ptr) + first, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s->ctx->stream));
+  VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return VG_OK;
+}
+
 int vg_state_get(vg_scene* s, uint32_t view_slot, uint32_t* state) {
   if (!s || !state || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
   if (!s->ctx->stream) return fail(s->ctx, VG_E_DEVICE, "host-only context");
@@ -997,7 +1082,7 @@ int vg_state_set(vg_scene* s, uint32_t view_slot, const uint32_t* state) {
 }
 
 int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes) {
-  if (!s || buffer > VG_POOL_MESHLET_INFO) return VG_E_INVALID;
+  if (!s || buffer > VG_POOL_MESHLET_TRIANGLES) return VG_E_INVALID;
   std::vector<float> f;
   const void* src = nullptr;
   uint64_t bytes = 0;
@@ -1047,9 +1132,13 @@ int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t c
       bytes = sizeof(float) * f.size();
       break;
     }
-    default:
+    case VG_POOL_MESHLET_INFO:
       src = s->m_info.data();
       bytes = sizeof(uint32_t) * s->m_info.size();
+      break;
+    default:
+      src = s->m_tri.data();
+      bytes = sizeof(uint32_t) * s->m_tri.size();
       break;
   }
   if (size_bytes) *size_bytes = bytes;

@@ -42,6 +42,7 @@ struct Lane {
   bool owns_stream = false;
   DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_scale, d_work_counter, d_flags;
   DevBuf d_budget_thresholds, d_budget_hist;
+  DevBuf d_expand_plan;  // per (view, cut): first draw-list chunk of the expand pass; then the total
   uint32_t view_capacity = 0;
   size_t cell_queue_capacity = 0;
   uint32_t work_capacity = 0;
@@ -87,6 +88,8 @@ struct vg_scene {
   std::vector<float2> m_err;
   std::vector<uint32_t> m_tris;
   std::vector<uint32_t> m_info;  // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
+  std::vector<uint32_t> m_first; // meshletInfo.y per meshlet (rebased), for the expand pass
+  std::vector<uint32_t> m_tri;   // meshletTriangles of the pool (GeometryPool.ts:430): at firstIndex / 3
   std::vector<uint32_t> m_run;   // LOD runs (see kMaxRun)
   std::vector<uint2> r_span;
   std::vector<float4> r_lod, r_parent;
@@ -99,9 +102,11 @@ struct vg_scene {
   // device scene
   vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
+  vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs
   // per view slot
-  vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs;
+  vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;
   std::vector<vgi::DevBuf> d_draws, d_hzbs;
+  std::vector<vgi::DevBuf> d_indices;  // 2 per slot (camera, shadow) on contexts with the expand pass
   std::vector<uint8_t> hzb_valid;
   std::vector<uint4*> h_draw_ptrs;
   std::vector<float*> h_hzb_ptrs;

@@ -53,6 +53,9 @@ constexpr uint32_t kMaxLaunchViews = 1u << (32 - kTagViewShift);
 // Per-wave LDS ring of meshlets in the cut awaiting their visibility test: at most 63 wait before a
 // wave appends up to 64 * kMaxRun, so 64 * (kMaxRun + 1) entries suffice.
 constexpr uint32_t kQueue = 64 * (kMaxRun + 1);
+// An index is (draw-list position << kLocalVertexBits) | local vertex (GeometryPool.ts:92, 1009-1018).
+constexpr uint32_t kLocalVertexBits = 7;  // ceil(log2(MAX_MESHLET_VERTICES = 128))
+constexpr uint32_t kExpandBlocksPerCu = 8;  // persistent 256-thread blocks of the expand pass
 
 // Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
 #define VG_INF_HALF ((float)(1e30 * 0.5))
@@ -132,6 +135,12 @@ struct Params {
   const float* budget_thresholds;  // bins, ascending
   unsigned long long* budget_hist; // per view: bins + 1
   uint32_t budget_bins;
+  // expand pass (GeometryPool.ts:997-1021)
+  const uint32_t* __restrict__ i_slot;   // mesh slot of each instance
+  const uint32_t* __restrict__ m_first;  // meshletInfo.y: first index of each meshlet's triangles (pool)
+  const uint32_t* __restrict__ m_tri;    // meshletTriangles: local ids a | b << 8 | c << 16
+  uint32_t* const* indices;              // per slot: camera index buffer, shadow index buffer
+  unsigned long long* expand_plan;       // per (view, cut): first 64-entry chunk; then the chunk total
 };
 
 __host__ __device__ inline uint32_t state_base(uint32_t slots) { return kPerMeshCounters + 2u * slots * 4u; }

@@ -1099,6 +1099,105 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
   }
 }
 
+// ------------------------------------------------------------------ 5. expand (expandPass, GeometryPool.ts:997-1021)
+//
+// Each drawn entry of a cut writes 3 indices per triangle, (draw-list position << 7) | local vertex id,
+// at regionStart(cut, mesh slot, part) + firstIndex + 3i, skipping a triangle whose last index does not
+// fit the cut's index capacity. The reference runs a workgroup per drawn meshlet. Here the draw lists of
+// all launched views' cuts are numbered as one sequence of 64-entry chunks and persistent waves take
+// equal contiguous shares of it. A wave reads a chunk's 64 entries and their region starts with one
+// coalesced pass (a lane per entry), then writes the entries one after another, a lane per triangle,
+// with the next entry's triangle words loaded before the current entry's indices are stored.
+
+// expand_plan[2 * view + cut] = first chunk of that cut's draw list; expand_plan[2 * views] = the total.
+__global__ __launch_bounds__(64) void k_expand_plan(Params p) {
+  const uint32_t lane = lane_id();
+  const uint32_t segments = 2 * p.view_count;
+  unsigned long long carry = 0;
+  for (uint32_t base = 0; base < segments; base += 64) {
+    const uint32_t seg = base + lane;
+    uint32_t chunks = 0;
+    if (seg < segments) chunks = (*counter_at(p, p.views[seg >> 1].slot, seg & 1u, kDrawn) + 63u) / 64u;
+    const uint32_t incl = wave_scan(chunks);  // at most 64 x 2^18 (drawn <= 2^24 per cut)
+    if (seg < segments) p.expand_plan[seg] = carry + (incl - chunks);
+    carry += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+  }
+  if (lane == 0) p.expand_plan[segments] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_expand(Params p) {
+  const uint32_t lane = lane_id();
+  const uint32_t segments = 2 * p.view_count;
+  const unsigned long long* __restrict__ plan = p.expand_plan;
+  const uint64_t total = plan[segments];
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  const uint64_t wave = xcd_wave_rank();
+  const uint64_t c_begin = total * wave / waves, c_end = total * (wave + 1) / waves;
+  if (c_begin >= c_end) return;
+  // The segment holding c_begin: the last one whose first chunk is <= c_begin.
+  uint32_t lo = 0, hi = segments;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (plan[mid] <= c_begin) lo = mid;
+    else hi = mid;
+  }
+  uint32_t seg = lo;
+  const uint32_t* __restrict__ tri = p.m_tri;
+  for (uint64_t c = c_begin; c < c_end; c++) {
+    while (plan[seg + 1] <= c) seg++;  // past empty segments
+    const uint32_t cut = seg & 1u;
+    const uint32_t slot = uniform(p.views[seg >> 1].slot);
+    const uint32_t drawn = *counter_at(p, slot, cut, kDrawn);
+    const uint32_t first = static_cast<uint32_t>(c - plan[seg]) * 64u;
+    const uint32_t n = min(64u, drawn - first);
+    const uint32_t position0 = first + cut * p.max_meshlets;  // draw-list position of the chunk's first entry
+    // A lane per entry: where its indices go, its first triangle word and its triangle count.
+    uint32_t dst = 0, ftri = 0, tris = 0;
+    if (lane < n) {
+      const uint4 entry = p.draws[slot][position0 + lane];
+      const uint32_t part = (entry.w & 0xffff00u) == kFullFadeBits ? 0u : 1u;
+      const uint32_t region = *(mesh_count_at(p, slot, cut, p.i_slot[entry.x], part) + 1);
+      dst = region + entry.z;  // u32 sums, as in the reference
+      ftri = p.m_first[entry.y] / 3u;
+      tris = entry.w & 255u;
+    }
+    uint32_t* __restrict__ out = p.indices[2 * slot + cut];
+    // index + 2 < capacity, written so that an index near 2^32 cannot wrap past the test
+    const uint32_t limit = p.cut_capacity[cut] - 2u;
+    auto words = [&](uint32_t k, uint32_t& w0, uint32_t& w1) {
+      const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ftri), static_cast<int>(k)));
+      const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tris), static_cast<int>(k)));
+      w0 = lane < t ? tri[f + lane] : 0u;
+      w1 = lane + 64u < t ? tri[f + lane + 64u] : 0u;
+    };
+    uint32_t w0 = 0, w1 = 0;
+    words(0, w0, w1);
+    for (uint32_t k = 0; k < n; k++) {
+      uint32_t n0 = 0, n1 = 0;
+      if (k + 1 < n) words(k + 1, n0, n1);
+      const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dst), static_cast<int>(k)));
+      const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tris), static_cast<int>(k)));
+      const uint32_t base = (position0 + k) << kLocalVertexBits;
+      auto put = [&](uint32_t i, uint32_t w) {
+        const uint32_t index = d + i * 3u;
+        if (index < limit) {
+          out[index] = base | (w & 255u);
+          out[index + 1] = base | ((w >> 8) & 255u);
+          out[index + 2] = base | ((w >> 16) & 255u);
+        }
+      };
+      if (lane < t) put(lane, w0);
+      if (lane + 64u < t) put(lane + 64u, w1);
+      if (t > 128u) {  // meshlets of more than 128 triangles (up to 255)
+        const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ftri), static_cast<int>(k)));
+        for (uint32_t i = lane + 128u; i < t; i += 64u) put(i, tri[f + i]);
+      }
+      w0 = n0;
+      w1 = n1;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ HZB (OcclusionCulling.ts:302-350)
 
 // Reduces levels base+1 .. base+5 (those that exist) over 32 x 32 texel tiles of level `base`.

@@ -383,6 +383,9 @@ struct WipMeshlet {
   float error, parent_error;
   uint32_t tris, verts;
   int group;  // group this meshlet is a child of (-1: root)
+  // Triangles as 3 local vertex ids (< verts) packed a | b << 8 | c << 16 (buildVirtualMesh.ts
+  // meshletTriangles): two per grid quad, the piece's vertices numbered row by row.
+  std::vector<uint32_t> tri;
 };
 
 struct Builder {
@@ -399,14 +402,24 @@ struct Builder {
 
   int add_meshlet(const std::vector<Piece>& pieces, int level, const Sphere* lod, float error) {
     std::vector<V3> pts;
+    std::vector<uint32_t> tri;
     uint32_t tris = 0, verts = 0;
     for (const Piece& pc : pieces) {
       for (int j = pc.j0; j <= pc.j1; j++)
         for (int i = pc.i0; i <= pc.i1; i++) pts.push_back(pc.rep->at(i, j));
+      const uint32_t row = static_cast<uint32_t>(pc.i1 - pc.i0 + 1);
+      for (int j = pc.j0; j < pc.j1; j++)
+        for (int i = pc.i0; i < pc.i1; i++) {
+          const uint32_t v00 = verts + static_cast<uint32_t>(j - pc.j0) * row + static_cast<uint32_t>(i - pc.i0);
+          const uint32_t v10 = v00 + 1, v01 = v00 + row, v11 = v01 + 1;
+          tri.push_back(v00 | (v10 << 8) | (v11 << 16));
+          tri.push_back(v00 | (v11 << 8) | (v01 << 16));
+        }
       tris += pc.tris();
       verts += pc.verts();
     }
     WipMeshlet m;
+    m.tri = std::move(tri);
     m.level = level;
     m.cull = bounding_sphere(pts);
     m.lod = lod ? *lod : m.cull;
@@ -498,6 +511,7 @@ struct vgs_mesh {
   std::vector<float> level_errors, level_guard;
   std::vector<int32_t> rgroup;
   std::vector<uint32_t> rstart, rindices;
+  std::vector<uint32_t> triangles;
   float bounding[4], lodb[4], cullb[4];
   uint64_t leaf_tris = 0;
   uint32_t leaf_meshlets = 0, root_tris = 0, root_meshlets = 0;
@@ -638,6 +652,7 @@ extern "C" int vgs_mesh_build(const vgs_mesh_params* params, vgs_mesh** out) {
     o[3] = vertex_offset;
     first_index += m.tris * 3;
     vertex_offset += m.verts;
+    mesh->triangles.insert(mesh->triangles.end(), m.tri.begin(), m.tri.end());
     mesh->rgroup[i] = m.group;
     int l = m.level;
     if (mesh->level_ranges[l * 2 + 1] == 0) mesh->level_ranges[l * 2] = static_cast<uint32_t>(i);
@@ -698,6 +713,8 @@ extern "C" void vgs_mesh_get(const vgs_mesh* m, vgs_mesh_view* v) {
   v->leaf_meshlets = m->leaf_meshlets;
   v->root_triangles = m->root_tris;
   v->root_meshlets = m->root_meshlets;
+  v->meshlet_triangles = m->triangles.data();
+  v->triangle_count = m->triangles.size();
 }
 
 extern "C" void vgs_mesh_free(vgs_mesh* m) { delete m; }

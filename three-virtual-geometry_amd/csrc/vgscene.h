@@ -49,6 +49,8 @@ typedef struct vgs_mesh_view {
   uint32_t leaf_meshlets;
   uint32_t root_triangles;
   uint32_t root_meshlets;
+  const uint32_t* meshlet_triangles; /* 3 local vertex ids per triangle, packed a | b << 8 | c << 16 */
+  uint64_t triangle_count;
 } vgs_mesh_view;
 
 typedef struct vgs_mesh vgs_mesh;

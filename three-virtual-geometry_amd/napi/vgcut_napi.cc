@@ -37,7 +37,7 @@ static napi_value prop(napi_env env, napi_value obj, const char* name) {
   return v;
 }
 
-// createContext({maxTriangles, maxShadowTriangles, maxMeshlets, maxViews, meshSlots, device}) -> external
+// createContext({maxTriangles, maxShadowTriangles, maxMeshlets, maxViews, meshSlots, device, expand}) -> external
 static napi_value CreateContext(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -53,6 +53,8 @@ static napi_value CreateContext(napi_env env, napi_callback_info info) {
   u32("maxMeshlets", &cfg.max_meshlets);
   u32("maxViews", &cfg.max_views);
   u32("meshSlots", &cfg.mesh_slots);
+  bool expand = false;
+  if (napi_get_value_bool(env, prop(env, argv[0], "expand"), &expand) == napi_ok) cfg.expand = expand ? 1u : 0u;
   int32_t device = 0;
   napi_get_value_int32(env, prop(env, argv[0], "device"), &device);
   cfg.device = device;
@@ -105,6 +107,9 @@ static napi_value AddMesh(napi_env env, napi_callback_info info) {
   d.level_ranges = typed<uint32_t>(env, prop(env, argv[1], "levelRanges"), &nr);
   d.level_errors = typed<float>(env, prop(env, argv[1], "levelErrors"), &ne);
   d.level_guard_errors = typed<float>(env, prop(env, argv[1], "levelGuardErrors"), &ng);
+  size_t nt = 0;
+  d.meshlet_triangles = typed<uint32_t>(env, prop(env, argv[1], "meshletTriangles"), &nt);
+  d.triangle_count = d.meshlet_triangles ? nt : 0;
   d.meshlet_count = static_cast<uint32_t>(ni / 4);
   d.level_count = static_cast<uint32_t>(nr / 2);
   const char* spheres[3] = {"boundingSphere", "lodBoundsSphere", "cullBoundsSphere"};
@@ -243,6 +248,30 @@ static napi_value Read(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// readIndices(scene, slot, cut, first, out: Uint32Array): the expand pass's index buffer of a cut
+// (cameraIndex / shadowIndex), out.length indices from `first`
+static napi_value ReadIndices(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0, cut = 0;
+  double first = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[2], &cut));
+  CHECK_NAPI(napi_get_value_double(env, argv[3], &first));
+  auto* s = static_cast<Scene*>(p);
+  size_t n = 0;
+  uint32_t* out = typed<uint32_t>(env, argv[4], &n);
+  if (!out || first < 0) {
+    napi_throw_type_error(env, nullptr, "vgcut: readIndices needs a Uint32Array and first >= 0");
+    return nullptr;
+  }
+  if (int rc = vg_index_read(s->scene, slot, cut, static_cast<uint64_t>(first), n, out)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
 // poolRead(scene, buffer) -> ArrayBuffer copy of a pool buffer in the reference layout (VG_POOL_*)
 static napi_value PoolRead(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -293,6 +322,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"cut", nullptr, Cut, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"read", nullptr, Read, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"poolRead", nullptr, PoolRead, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readIndices", nullptr, ReadIndices, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"counterWords", nullptr, CounterWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"drawArgsWords", nullptr, DrawArgsWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };

@@ -24,6 +24,7 @@ VG_POOL_INSTANCES = 1
 VG_POOL_CELLS = 2
 VG_POOL_MESHLET_BOUNDS = 3
 VG_POOL_MESHLET_INFO = 4
+VG_POOL_MESHLET_TRIANGLES = 5
 
 LIBRARY = _native.library_path("libvgcut.so")
 
@@ -33,7 +34,7 @@ EXPORTS = (
     "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
-    "vg_schedule_get", "vg_schedule_set", "vg_tune",
+    "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
     "vg_server_create", "vg_server_destroy", "vg_server_last_error", "vg_server_add_scene", "vg_server_submit", "vg_server_poll",
     "vg_server_read", "vg_server_release", "vg_server_stats_get",
 )
@@ -55,6 +56,7 @@ class Config(ctypes.Structure):
         ("hzb_max_size", ctypes.c_uint32),
         ("budget_bins", ctypes.c_uint32),
         ("device", ctypes.c_int32),
+        ("expand", ctypes.c_uint32),
     ]
 
 
@@ -70,6 +72,8 @@ class MeshDesc(ctypes.Structure):
         ("bounding_sphere", ctypes.c_float * 4),
         ("lod_bounds_sphere", ctypes.c_float * 4),
         ("cull_bounds_sphere", ctypes.c_float * 4),
+        ("meshlet_triangles", ctypes.c_void_p),
+        ("triangle_count", ctypes.c_uint64),
     ]
 
 
@@ -214,6 +218,8 @@ def load():
         "vg_schedule_get": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
         "vg_schedule_set": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
         "vg_tune": ([P, P, U32, U32, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "vg_index_read": ([P, U32, U32, U64, U64, P], ctypes.c_int),
+        "vg_index_buffer": ([P, U32, U32, ctypes.POINTER(P), ctypes.POINTER(U64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -232,9 +238,10 @@ class Context:
     """A vg_ctx: capacities and the HIP stream (VirtualGeometry constructor options)."""
 
     def __init__(self, max_triangles=10_000_000, max_shadow_triangles=4_000_000, max_meshlets=1 << 20, max_views=1,
-                 mesh_slots=64, hzb_max_size=1024, budget_bins=256, device=0):
+                 mesh_slots=64, hzb_max_size=1024, budget_bins=256, device=0, expand=False):
         self.lib = load()
-        cfg = Config(max_triangles, max_shadow_triangles, max_meshlets, max_views, mesh_slots, hzb_max_size, budget_bins, device)
+        cfg = Config(max_triangles, max_shadow_triangles, max_meshlets, max_views, mesh_slots, hzb_max_size, budget_bins, device,
+                     1 if expand else 0)
         self.config = cfg
         h = ctypes.c_void_p()
         rc = self.lib.vg_create(ctypes.byref(cfg), ctypes.byref(h))
@@ -295,6 +302,11 @@ class Scene:
             d.bounding_sphere[i] = float(data.boundingSphere[i])
             d.lod_bounds_sphere[i] = float(data.lodBoundsSphere[i])
             d.cull_bounds_sphere[i] = float(data.cullBoundsSphere[i])
+        tri = getattr(data, "meshletTriangles", None)
+        if tri is not None:
+            tri = np.ascontiguousarray(tri, dtype=np.uint32)
+            d.meshlet_triangles = tri.ctypes.data
+            d.triangle_count = len(tri)
         slot = ctypes.c_uint32()
         self.ctx.check(self.lib.vg_mesh_add(self.handle, ctypes.byref(d), ctypes.byref(slot)), "vg_mesh_add")
         return int(slot.value)
@@ -347,6 +359,20 @@ class Scene:
                                         ctypes.byref(met)), "vg_read")
         return counters, args, draws, met
 
+    def index_capacity(self, cut: int) -> int:
+        """Capacity (u32 indices) of a cut's index buffer: triangles x 3 (contexts with the expand pass)."""
+        cap = ctypes.c_uint64()
+        self.ctx.check(self.lib.vg_index_buffer(self.handle, 0, cut, None, ctypes.byref(cap)), "vg_index_buffer")
+        return int(cap.value)
+
+    def read_indices(self, view_slot: int, cut: int, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        """`count` indices from `first` of a view slot's index buffer for `cut` (all of it by default)."""
+        if count is None:
+            count = self.index_capacity(cut) - first
+        out = np.empty(count, dtype=np.uint32)
+        self.ctx.check(self.lib.vg_index_read(self.handle, view_slot, cut, first, count, out.ctypes.data), "vg_index_read")
+        return out
+
     def state(self, view_slot: int) -> np.ndarray:
         s = np.empty(8, dtype=np.uint32)
         self.ctx.check(self.lib.vg_state_get(self.handle, view_slot, s.ctypes.data), "vg_state_get")
@@ -359,7 +385,7 @@ class Scene:
     def pool(self, buffer: int) -> np.ndarray:
         size = ctypes.c_uint64()
         self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, None, 0, ctypes.byref(size)), "vg_scene_pool_read")
-        dtype = np.uint32 if buffer == VG_POOL_MESHLET_INFO else np.float32
+        dtype = np.uint32 if buffer in (VG_POOL_MESHLET_INFO, VG_POOL_MESHLET_TRIANGLES) else np.float32
         out = np.empty(size.value // 4, dtype=dtype)
         self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, out.ctypes.data, size.value, None), "vg_scene_pool_read")
         return out

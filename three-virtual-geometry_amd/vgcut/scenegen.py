@@ -50,6 +50,8 @@ class _View(ctypes.Structure):
         ("leaf_meshlets", ctypes.c_uint32),
         ("root_triangles", ctypes.c_uint32),
         ("root_meshlets", ctypes.c_uint32),
+        ("meshlet_triangles", ctypes.POINTER(ctypes.c_uint32)),
+        ("triangle_count", ctypes.c_uint64),
     ]
 
 
@@ -81,8 +83,9 @@ def _load():
 class VirtualMeshData:
     """The fields of the reference's ``VirtualMeshData`` the cut reads, plus the DAG links.
 
-    Mirrors src/core/preprocess/buildVirtualMesh.ts:75-138 (positions, normals and index buffers are
-    not produced: nothing on the cut's path reads them).
+    Mirrors src/core/preprocess/buildVirtualMesh.ts:75-138. ``meshletTriangles`` (the local vertex ids
+    the expand pass writes as indices) is produced; positions, normals and meshlet vertex lists are not:
+    nothing on the cut's path or the expand pass reads them.
     """
 
     meshletCount: int
@@ -101,6 +104,7 @@ class VirtualMeshData:
     leafMeshlets: int
     rootTriangles: int
     rootMeshlets: int
+    meshletTriangles: Optional[np.ndarray] = None  # uint32 per triangle: local ids a | b << 8 | c << 16
 
     @property
     def levelCount(self) -> int:
@@ -152,6 +156,7 @@ def build_mesh(kind: int, seed: int = 1, size=(1.0, 1.0, 1.0), noise_amplitude: 
             leafMeshlets=int(v.leaf_meshlets),
             rootTriangles=int(v.root_triangles),
             rootMeshlets=int(v.root_meshlets),
+            meshletTriangles=_arr(v.meshlet_triangles, int(v.triangle_count), np.uint32),
         )
     finally:
         lib.vgs_mesh_free(handle)
