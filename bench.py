@@ -10,6 +10,9 @@ Perspective views with at least 720 lines are occlusion culled against the raste
 A step cuts all 64 views: the plain form launches one view per launch sequence (64 launches), the
 batched form cuts them in one launch sequence replayed from a HIP graph. `value` is the batched
 form's views x clusters tested per second with the scene resident in HBM.
+
+Beside it: the expand pass (index writing) on the same views at the reference's default capacities,
+and served request streams over 8 resident scenes (batched server vs one request per launch).
 """
 from __future__ import annotations
 
@@ -33,21 +36,28 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # (no capacity retries at 1 px; a 300M-triangle index buffer is 3.6 GB).
 MAX_TRIANGLES = 300_000_000
 MAX_SHADOW_TRIANGLES = 100_000_000
+# The reference's default pool capacities (VirtualGeometry.ts:191-198, GeometryPool.ts:642-644): the expand
+# measurement runs at these, so the 64 views' index buffers (triangles x 3 u32 per cut) take 10.75 GB.
+REF_TRIANGLES = 10_000_000
+REF_SHADOW_TRIANGLES = 4_000_000
+REF_MESHLETS = 1 << 20
 
 
 TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world64_rocprof.json")
+EXPAND_TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world64_expand_rocprof.json")
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, path: str = TRAFFIC_SUMMARY):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary of this workload (two --pmc
     runs, FETCH_SIZE and WRITE_SIZE, calibrated on a device copy), or (None, reason)."""
-    if not os.path.exists(TRAFFIC_SUMMARY):
-        return None, "no rocprofv3 summary"
-    with open(TRAFFIC_SUMMARY) as f:
+    rel = os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
+    if not os.path.exists(path):
+        return None, f"no rocprofv3 summary ({rel})"
+    with open(path) as f:
         summary = json.load(f)
     for name, k in summary.get("kernels", {}).items():
         if name.split("<")[0] == kernel and "hbm_bytes" in k:
-            return k["hbm_bytes"], (f"profiles/world64_rocprof.json: {name}, FETCH_SIZE x {summary['fetch_factor']:.2f} + "
+            return k["hbm_bytes"], (f"{rel}: {name}, FETCH_SIZE x {summary['fetch_factor']:.2f} + "
                                     f"WRITE_SIZE x {summary['write_factor']:.2f} per dispatch, avg dispatch {k['avg_us']:.0f} us")
     return None, f"{kernel} not in the rocprofv3 summary"
 
@@ -74,6 +84,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-requests measurement")
     ap.add_argument("--serve-only", action="store_true", help="only the served-requests measurement")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="requests per rate = rate x this (at least 1000)")
+    ap.add_argument("--no-expand", action="store_true", help="skip the expand-pass measurement")
+    ap.add_argument("--expand-only", action="store_true", help="only the expand-pass measurement (with --profile-only: for rocprofv3)")
     ap.add_argument("--schedule", type=str, default=None,
                     help="fixed launch schedule 'instance_blocks,meshlet_blocks,meshlet_waves' (per CU) instead of tuning")
     return ap.parse_args()
@@ -274,6 +286,97 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate)
     return out
 
 
+def expand_bench(spec, vd, steps, warmup, schedule, profile_only=False):
+    """The cut followed by the expand pass (GeometryPool.ts:997-1021) for the 64 views at the reference's
+    default pool capacities (10M camera / 4M shadow triangles, 1M meshlets per view): the main views'
+    demand exceeds them, so the capacity controller retries those cuts coarser, as in the reference. The
+    expand pass then writes 3 indices per drawn triangle into each view's index buffers."""
+    import torch
+
+    from vgcut import native, scenes
+
+    views = [v for v, _ in vd]
+    ctx = native.Context(max_triangles=REF_TRIANGLES, max_shadow_triangles=REF_SHADOW_TRIANGLES, max_meshlets=REF_MESHLETS,
+                         max_views=len(views), mesh_slots=64, device=0, expand=True)
+    scene = native.Scene(ctx)
+    t0 = time.time()
+    scenes.load_into(scene, spec, occlusion=True)
+    for v, d in vd:
+        if d is not None:
+            scene.set_hzb(v.slot, d)
+    scene.set_schedule(*schedule)
+    for _ in range(max(1, warmup)):
+        scene.cut(views, graph=True)
+    scene.sync()
+    log(f"expand: scene loaded at the reference capacities, warm ({time.time() - t0:.1f} s)")
+    if profile_only:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        for _ in range(steps):
+            scene.cut(views, graph=False)
+        scene.sync()
+        return {"profile_only": True, "steps": steps, "views": len(views), "copy_bytes": 2 * src.numel() * 4}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scene.cut(views, graph=True)
+    scene.sync()
+    step_s = (time.perf_counter() - t0) / steps
+    scene.profile(True)
+    for _ in range(max(3, steps // 2)):
+        scene.cut(views, graph=False)
+    scene.sync()
+    kernels, seqs = scene.profile_read()
+    scene.profile(False)
+    kernel_ms = {name: ms / max(1, seqs) for name, ms in kernels}
+    expand_ms = kernel_ms["k_expand (+ plan, resolve)"]
+
+    # The last cut's work: drawn entries, indices written (exact, also where a region passes the capacity)
+    # and the triangles of the distinct meshlets drawn (their triangle words are read at least once).
+    slots = ctx.config.mesh_slots
+    info = scene.pool(native.VG_POOL_MESHLET_INFO).reshape(-1, 4)
+    slot_of = scene.pool(native.VG_POOL_INSTANCES).reshape(-1, 20)[:, 19].astype(np.int64)
+    drawn = written = 0
+    drawn_meshlets = []
+    for v in views:
+        counters = scene.read(v.slot, max_entries=1)[0]
+        n = (int(counters[3]), int(counters[8]))
+        draws = scene.read(v.slot, max_entries=max(1, max(n)))[2]
+        for cut in (0, 1):
+            d = draws[cut, :n[cut]].astype(np.int64)
+            drawn += n[cut]
+            drawn_meshlets.append(d[:, 1])
+            requested = int(counters[2 + cut * 5 + 4])
+            cap = (REF_TRIANGLES if cut == 0 else REF_SHADOW_TRIANGLES) * 3
+            if requested <= cap:
+                written += requested
+            else:
+                part = np.where((d[:, 3] & 0xFFFF00) == (255 << 16), 0, 1)
+                region = counters[12 + cut * slots * 4 + 1 + slot_of[d[:, 0]] * 4 + part * 2].astype(np.int64)
+                first = region + d[:, 2]
+                fits = np.clip((cap - 2 - first + 2) // 3, 0, d[:, 3] & 255)  # triangles i with first + 3i + 2 < cap
+                written += 3 * int(fits.sum())
+    unique = np.unique(np.concatenate(drawn_meshlets)) if drawn_meshlets else np.zeros(0, dtype=np.int64)
+    unique_triangles = int(info[unique, 0].astype(np.int64).sum())
+    algorithmic = 24 * drawn + 4 * written + 4 * unique_triangles
+    achieved = algorithmic / (expand_ms * 1e-3) / 1e9
+    traffic, source = measured_traffic("k_expand", EXPAND_TRAFFIC_SUMMARY)
+    out = {"workload": f"world {spec.instance_count} instances, {len(views)} mixed views, capacities 10M/4M triangles, 1M meshlets",
+           "ms_per_step_cut_and_expand": step_s * 1e3, "kernel_ms": kernel_ms, "expand_ms": expand_ms,
+           "drawn_meshlets": drawn, "indices_written": written, "distinct_meshlets": int(len(unique)),
+           "indices_per_s": written / (expand_ms * 1e-3),
+           "roofline": {"bound": "hbm", "kernel": "k_expand", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": algorithmic, "traffic": traffic,
+                        "traffic_source": source}}
+    log(f"expand: {drawn} drawn meshlets, {written} indices in {expand_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), "
+        f"cut + expand {step_s * 1e3:.2f} ms per step")
+    scene.close()
+    ctx.close()
+    return out
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
@@ -299,6 +402,11 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
+        return
+    if args.expand_only:
+        vd = mixed_views(spec, args.views, make_view, cam, scenegen)
+        schedule = tuple(int(x) for x in args.schedule.split(",")) if args.schedule else (12, 10, 5)
+        print(json.dumps({"expand": expand_bench(spec, vd, args.steps, args.warmup, schedule, args.profile_only)}))
         return
 
     views_n = args.views
@@ -489,6 +597,8 @@ def main():
         log(f"check: {bad} of {views_n} views differ from the oracle; {overflowing} overflow the draw list "
             f"(their draw order, hence firstIndex and per-mesh counts, is not determined)")
 
+    expand = None if args.no_expand else expand_bench(spec, vd, args.steps, args.warmup, batched_schedule)
+
     value = tested / batched_s
     out = {
         "metric": "views*clusters tested/s (batched, 64 mixed views, 3M-instance world)",
@@ -522,6 +632,7 @@ def main():
                      "algorithmic_bytes": dom_bytes,
                      "launch_algorithmic_bytes": launch_bytes, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
+        "expand": expand,
         "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES,
                                                                                args.serve_seconds),
     }

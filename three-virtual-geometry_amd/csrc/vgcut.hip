@@ -107,7 +107,7 @@ void lane_destroy(Lane* lane) {
   lane->graph_exec = nullptr;
   lane->graph = nullptr;
   DevBuf* bufs[] = {&lane->d_views, &lane->d_cell_queue_count, &lane->d_work_counter, &lane->d_flags, &lane->d_budget_thresholds,
-                    &lane->d_budget_hist, &lane->d_expand_plan};
+                    &lane->d_budget_hist, &lane->d_expand_plan, &lane->d_expand_records};
   for (DevBuf* b : bufs) release_lane(lane, *b);
   if (!lane->borrowed) {
     DevBuf* t[] = {&lane->d_cell_queue, &lane->d_work, &lane->d_work_start, &lane->d_work_scale};
@@ -370,6 +370,7 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.m_tri = static_cast<const uint32_t*>(s->d_tri.ptr);
   p.indices = static_cast<uint32_t* const*>(s->d_index_ptrs.ptr);
   p.expand_plan = static_cast<unsigned long long*>(lane->d_expand_plan.ptr);
+  p.expand_records = static_cast<uint4*>(lane->d_expand_records.ptr);
   if (s->footprint) {
     p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
     p.fp_meshlets = static_cast<uint32_t*>(s->d_fp_meshlets.ptr);
@@ -381,7 +382,7 @@ constexpr int kKernelSlots = 12;
 const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells (+ budget passes)", "k_instances", "k_meshlets", "k_prefix(check 1)",
                                                 "k_instances(retry 1)", "k_meshlets(retry 1)", "k_prefix(check 2)",
                                                 "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)",
-                                                "k_expand (+ plan)"};
+                                                "k_expand (+ plan, resolve)"};
 
 hipEvent_t take_event(vg_scene* s) {
   if (!s->event_pool.empty()) {
@@ -462,6 +463,7 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   mark();
   if (ctx->cfg.expand) {
     hipLaunchKernelGGL(k_expand_plan, dim3(1), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_expand_resolve, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
     hipLaunchKernelGGL(k_expand, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
   }
   mark();
@@ -473,9 +475,12 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
 int lane_reserve_views(vg_ctx* ctx, Lane* lane) {
   VG_HIP(ctx, hipStreamSynchronize(lane->stream));
   if (int rc = ensure_lane(ctx, lane, lane->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
-  if (ctx->cfg.expand)
+  if (ctx->cfg.expand) {
+    const size_t chunks = 2 * static_cast<size_t>(ctx->cfg.max_views) * ((ctx->cfg.max_meshlets + 63) / 64);
     if (int rc = ensure_lane(ctx, lane, lane->d_expand_plan, sizeof(unsigned long long) * (2 * static_cast<size_t>(ctx->cfg.max_views) + 1)))
       return rc;
+    if (int rc = ensure_lane(ctx, lane, lane->d_expand_records, sizeof(uint4) * 64 * chunks)) return rc;
+  }
   for (int r = 0; r < Lane::kRing; r++) {
     if (lane->h_ring[r]) (void)hipHostFree(lane->h_ring[r]);
     if (lane->h_ring_thresholds[r]) (void)hipHostFree(lane->h_ring_thresholds[r]);

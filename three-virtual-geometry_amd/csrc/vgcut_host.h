@@ -43,6 +43,7 @@ struct Lane {
   DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_scale, d_work_counter, d_flags;
   DevBuf d_budget_thresholds, d_budget_hist;
   DevBuf d_expand_plan;  // per (view, cut): first draw-list chunk of the expand pass; then the total
+  DevBuf d_expand_records;  // per draw-list entry of every view slot's cuts: resolved destination (16 B)
   uint32_t view_capacity = 0;
   size_t cell_queue_capacity = 0;
   uint32_t work_capacity = 0;

@@ -141,6 +141,7 @@ struct Params {
   const uint32_t* __restrict__ m_tri;    // meshletTriangles: local ids a | b << 8 | c << 16
   uint32_t* const* indices;              // per slot: camera index buffer, shadow index buffer
   unsigned long long* expand_plan;       // per (view, cut): first 64-entry chunk; then the chunk total
+  uint4* expand_records;                 // per drawn entry (64 per chunk): first index, first triangle, triangles, position
 };
 
 __host__ __device__ inline uint32_t state_base(uint32_t slots) { return kPerMeshCounters + 2u * slots * 4u; }
@@ -213,6 +214,7 @@ __host__ __device__ inline float vg_logf(float x) {
 // written by an earlier kernel of the launch sequence (each dispatch starts with that cache invalidated),
 // never by the kernel that reads it this way.
 #define VG_CONST __attribute__((address_space(4)))
+#define VG_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ __forceinline__ const VG_CONST T* scalar(const T* p) {
   return (const VG_CONST T*)(p);

@@ -1104,10 +1104,11 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
 // Each drawn entry of a cut writes 3 indices per triangle, (draw-list position << 7) | local vertex id,
 // at regionStart(cut, mesh slot, part) + firstIndex + 3i, skipping a triangle whose last index does not
 // fit the cut's index capacity. The reference runs a workgroup per drawn meshlet. Here the draw lists of
-// all launched views' cuts are numbered as one sequence of 64-entry chunks and persistent waves take
-// equal contiguous shares of it. A wave reads a chunk's 64 entries and their region starts with one
-// coalesced pass (a lane per entry), then writes the entries one after another, a lane per triangle,
-// with the next entry's triangle words loaded before the current entry's indices are stored.
+// all launched views' cuts are numbered as one sequence of 64-entry chunks dealt round-robin to
+// persistent waves (the draw lists of views clamped by their capacity have little to write, so
+// contiguous shares would leave some waves most of the work). A first pass resolves each entry's
+// destination; the writing pass then takes a chunk's resolved records with one coalesced load and
+// writes the entries that have anything to write, a lane per triangle, several entries at a time.
 
 // expand_plan[2 * view + cut] = first chunk of that cut's draw list; expand_plan[2 * views] = the total.
 __global__ __launch_bounds__(64) void k_expand_plan(Params p) {
@@ -1125,75 +1126,110 @@ __global__ __launch_bounds__(64) void k_expand_plan(Params p) {
   if (lane == 0) p.expand_plan[segments] = carry;
 }
 
+// The segment (view, cut) of chunk c, from a segment at or before it (chunks only grow along a wave).
+__device__ __forceinline__ uint32_t segment_of(const unsigned long long* __restrict__ plan, uint32_t seg, uint64_t c) {
+  while (plan[seg + 1] <= c) seg++;  // past empty segments
+  return seg;
+}
+
+// Resolves every drawn entry, a lane per entry, into (first index, first triangle word, triangles to
+// write, draw-list position): the gathers of its mesh slot and region start are independent across
+// entries here, so they overlap instead of forming a chain per chunk in the writing kernel. The count
+// is 0 for an entry wholly past the capacity.
+__global__ __launch_bounds__(256) void k_expand_resolve(Params p) {
+  const uint32_t lane = lane_id();
+  const uint32_t segments = 2 * p.view_count;
+  const unsigned long long* __restrict__ plan = p.expand_plan;
+  const uint64_t total = plan[segments];
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  uint32_t seg = 0;
+  for (uint64_t c = xcd_wave_rank(); c < total; c += waves) {
+    seg = segment_of(plan, seg, c);
+    const uint32_t cut = seg & 1u;
+    const uint32_t slot = uniform(p.views[seg >> 1].slot);
+    const uint32_t drawn = *counter_at(p, slot, cut, kDrawn);
+    const uint32_t first = static_cast<uint32_t>(c - plan[seg]) * 64u;
+    const uint32_t position = first + lane + cut * p.max_meshlets;
+    const uint32_t limit = p.cut_capacity[cut] - 2u;  // index + 2 < capacity  <=>  index < limit
+    uint4 r = make_uint4(0, 0, 0, position);
+    if (first + lane < drawn) {
+      const uint4 entry = p.draws[slot][position];
+      const uint32_t part = (entry.w & 0xffff00u) == kFullFadeBits ? 0u : 1u;
+      const uint32_t region = *(mesh_count_at(p, slot, cut, p.i_slot[entry.x], part) + 1);
+      const uint32_t dst = region + entry.z;  // u32 sums, as in the reference
+      const uint32_t tris = entry.w & 255u;
+      const uint32_t last = dst + 3u * (tris - 1u);
+      // Something to write: the first triangle fits (dst < limit), or a later one's index wraps.
+      const bool any = tris > 0 && (dst < limit || last < dst);
+      r = make_uint4(dst, p.m_first[entry.y] / 3u, any ? tris : 0u, position);
+    }
+    p.expand_records[c * 64 + lane] = r;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_expand(Params p) {
   const uint32_t lane = lane_id();
   const uint32_t segments = 2 * p.view_count;
   const unsigned long long* __restrict__ plan = p.expand_plan;
   const uint64_t total = plan[segments];
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
-  const uint64_t wave = xcd_wave_rank();
-  const uint64_t c_begin = total * wave / waves, c_end = total * (wave + 1) / waves;
-  if (c_begin >= c_end) return;
-  // The segment holding c_begin: the last one whose first chunk is <= c_begin.
-  uint32_t lo = 0, hi = segments;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (plan[mid] <= c_begin) lo = mid;
-    else hi = mid;
-  }
-  uint32_t seg = lo;
   const uint32_t* __restrict__ tri = p.m_tri;
-  for (uint64_t c = c_begin; c < c_end; c++) {
-    while (plan[seg + 1] <= c) seg++;  // past empty segments
+  // Global (not generic) stores; a triangle's 3 indices are one 12-byte store (4-byte aligned).
+  struct Triple {
+    uint32_t a, b, c;
+  };
+  uint32_t seg = 0;
+  uint64_t c = xcd_wave_rank();
+  uint4 next = c < total ? p.expand_records[c * 64 + lane] : make_uint4(0, 0, 0, 0);
+  for (; c < total; c += waves) {
+    seg = segment_of(plan, seg, c);
+    const uint4 r = next;  // this chunk's records; the next chunk's load overlaps its writes
+    if (c + waves < total) next = p.expand_records[(c + waves) * 64 + lane];
+    uint64_t todo = __ballot(r.z != 0);
+    if (!todo) continue;
     const uint32_t cut = seg & 1u;
     const uint32_t slot = uniform(p.views[seg >> 1].slot);
-    const uint32_t drawn = *counter_at(p, slot, cut, kDrawn);
-    const uint32_t first = static_cast<uint32_t>(c - plan[seg]) * 64u;
-    const uint32_t n = min(64u, drawn - first);
-    const uint32_t position0 = first + cut * p.max_meshlets;  // draw-list position of the chunk's first entry
-    // A lane per entry: where its indices go, its first triangle word and its triangle count.
-    uint32_t dst = 0, ftri = 0, tris = 0;
-    if (lane < n) {
-      const uint4 entry = p.draws[slot][position0 + lane];
-      const uint32_t part = (entry.w & 0xffff00u) == kFullFadeBits ? 0u : 1u;
-      const uint32_t region = *(mesh_count_at(p, slot, cut, p.i_slot[entry.x], part) + 1);
-      dst = region + entry.z;  // u32 sums, as in the reference
-      ftri = p.m_first[entry.y] / 3u;
-      tris = entry.w & 255u;
-    }
-    uint32_t* __restrict__ out = p.indices[2 * slot + cut];
-    // index + 2 < capacity, written so that an index near 2^32 cannot wrap past the test
     const uint32_t limit = p.cut_capacity[cut] - 2u;
-    auto words = [&](uint32_t k, uint32_t& w0, uint32_t& w1) {
-      const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ftri), static_cast<int>(k)));
-      const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tris), static_cast<int>(k)));
-      w0 = lane < t ? tri[f + lane] : 0u;
-      w1 = lane + 64u < t ? tri[f + lane + 64u] : 0u;
-    };
-    uint32_t w0 = 0, w1 = 0;
-    words(0, w0, w1);
-    for (uint32_t k = 0; k < n; k++) {
-      uint32_t n0 = 0, n1 = 0;
-      if (k + 1 < n) words(k + 1, n0, n1);
-      const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dst), static_cast<int>(k)));
-      const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tris), static_cast<int>(k)));
-      const uint32_t base = (position0 + k) << kLocalVertexBits;
-      auto put = [&](uint32_t i, uint32_t w) {
-        const uint32_t index = d + i * 3u;
-        if (index < limit) {
-          out[index] = base | (w & 255u);
-          out[index + 1] = base | ((w >> 8) & 255u);
-          out[index + 2] = base | ((w >> 16) & 255u);
-        }
-      };
-      if (lane < t) put(lane, w0);
-      if (lane + 64u < t) put(lane + 64u, w1);
-      if (t > 128u) {  // meshlets of more than 128 triangles (up to 255)
-        const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ftri), static_cast<int>(k)));
-        for (uint32_t i = lane + 128u; i < t; i += 64u) put(i, tri[f + i]);
+    VG_GLOBAL uint32_t* out = (VG_GLOBAL uint32_t*)(p.indices[2 * slot + cut]);
+    // Up to kBatch entries at a time: their triangle words are loaded together, then all their indices
+    // stored. A store counts in the same counter as a load here, so the wait before the stores also
+    // waits for the previous batch's stores; batching amortises that round trip over kBatch entries.
+    constexpr int kBatch = 16;
+    while (todo) {
+      uint32_t kk[kBatch], w0[kBatch], w1[kBatch];
+#pragma unroll
+      for (int g = 0; g < kBatch; g++) {
+        kk[g] = todo ? static_cast<uint32_t>(__builtin_ctzll(todo)) : 64u;
+        todo &= todo - 1;
+        const uint32_t k = kk[g] & 63u;
+        const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.y), static_cast<int>(k)));
+        const uint32_t t = kk[g] < 64u ? static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.z), static_cast<int>(k))) : 0u;
+        w0[g] = lane < t ? tri[f + lane] : 0u;
+        w1[g] = lane + 64u < t ? tri[f + lane + 64u] : 0u;
       }
-      w0 = n0;
-      w1 = n1;
+#pragma unroll
+      for (int g = 0; g < kBatch; g++) {
+        if (kk[g] >= 64u) break;
+        const uint32_t k = kk[g];
+        const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.x), static_cast<int>(k)));
+        const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.z), static_cast<int>(k)));
+        const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.w), static_cast<int>(k))) << kLocalVertexBits;
+        auto put = [&](uint32_t i, uint32_t w) {
+          const uint32_t index = d + 3u * i;  // the triangle's first index
+          if (index < limit) {
+            VG_GLOBAL Triple* q = (VG_GLOBAL Triple*)(out + index);
+            q->a = base | (w & 255u);
+            q->b = base | ((w >> 8) & 255u);
+            q->c = base | ((w >> 16) & 255u);
+          }
+        };
+        if (lane < t) put(lane, w0[g]);
+        if (lane + 64u < t) put(lane + 64u, w1[g]);
+        if (t > 128u) {  // meshlets of more than 128 triangles (up to 255)
+          const uint32_t f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.y), static_cast<int>(k)));
+          for (uint32_t i = lane + 128u; i < t; i += 64u) put(i, tri[f + i]);
+        }
+      }
     }
   }
 }
