@@ -88,7 +88,8 @@ typedef struct vg_view {
   float lod_scale_limit;        /* max(1, maxErrorThreshold / errorThreshold) */
   float min_pixel_radius;       /* default 0.7 */
   uint32_t frustum_culling;     /* 1: on */
-  uint32_t occlusion;           /* 1: occlusion culling wanted this frame (needs a valid HZB in the slot) */
+  uint32_t occlusion;           /* 1: occlusion culling wanted this frame (needs a valid HZB in the slot);
+                                   2: the same, in two phases (see vg_cut_phase2) */
   uint32_t slot;                /* view slot, < max_views */
   uint32_t triangle_budget;     /* 0: off; else the error threshold is chosen on the GPU (see vg_cut) */
   float threshold_range[2];     /* budget: smallest and largest threshold (VirtualGeometry.thresholdRange) */
@@ -150,6 +151,23 @@ int vg_hzb_read(vg_scene* scene, uint32_t view_slot, float* dst, uint64_t capaci
 int vg_cut(vg_scene* scene, const vg_view* views, uint32_t count);
 /* Same as vg_cut, replayed from a captured HIP graph when the view count matches the last capture. */
 int vg_cut_graph(vg_scene* scene, const vg_view* views, uint32_t count);
+/*
+ * Two-phase occlusion (the reference tests occluders in one phase, OcclusionCulling.ts:4-21; the second
+ * phase is this library's): a vg_cut of views with occlusion = 2 records its camera cut's occlusion
+ * rejects (instances and meshlets of the final round). After the caller has drawn that cut and set each
+ * slot's new HZB (vg_hzb_set, from the depth it drew), vg_cut_phase2 with the same views re-tests the
+ * rejects against it: rejected instances go through the instance and meshlet passes again, rejected
+ * meshlets through the occlusion test, at the cut's final band. What is visible now is appended to the
+ * camera draw list after the cut's entries, with its own regions laid out after the cut's requested
+ * indices and its own draw args (vg_read_phase2); with the expand pass, its indices are written there.
+ * It must directly follow the views' vg_cut on the scene (no instance changes in between).
+ */
+int vg_cut_phase2(vg_scene* scene, const vg_view* views, uint32_t count);
+/* The second phase's counters (reference layout; C_SELECTED, C_DRAWN and C_REQUESTED count both phases,
+   the per-mesh (count, start) pairs and args are the second phase's) and draw args, after vg_sync. */
+int vg_read_phase2(vg_scene* scene, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args);
+/* Meshlet rejects the slot's last cut recorded, and whether more were dropped (more than max_meshlets). */
+int vg_phase_rejects(vg_scene* scene, uint32_t view_slot, uint32_t* meshlets, uint32_t* dropped);
 /* Waits for the context's stream. */
 int vg_sync(vg_scene* scene);
 

@@ -101,8 +101,12 @@ def load():
                                             ctypes.c_uint32, ctypes.c_void_p]
         lib.or_budget_histogram.restype = ctypes.c_int
         lib.or_expand.argtypes = [ctypes.POINTER(OrScene), ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+                                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
         lib.or_expand.restype = ctypes.c_int
+        lib.or_cut_two_phase.argtypes = [ctypes.POINTER(OrScene), ctypes.POINTER(OrView), ctypes.POINTER(OrView), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint32, ctypes.POINTER(OrStats), ctypes.POINTER(OrStats), ctypes.c_void_p]
+        lib.or_cut_two_phase.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -218,6 +222,43 @@ def cut(pool: Pool, view, state: Optional[np.ndarray] = None, hzb: Optional[np.n
     return CutResult(np.concatenate([counters, st]), args, draws, st, stats)
 
 
+@dataclass
+class TwoPhaseResult:
+    first: CutResult        # the cut (its draws array is `draws` below, with the second phase's entries after)
+    counters2: np.ndarray   # second phase, reference layout (the 8 state words of the cut appended)
+    args2: np.ndarray
+    draws: np.ndarray       # (2, max_meshlets, 4)
+    stats2: OrStats
+    rejects: tuple          # (instances, meshlets, meshlet records dropped)
+
+
+def cut_two_phase(pool: Pool, view, hzb1: Optional[np.ndarray], hzb2: Optional[np.ndarray], state: Optional[np.ndarray] = None,
+                  reject_capacity: Optional[int] = None) -> TwoPhaseResult:
+    """Two-phase occlusion: the cut against hzb1 recording its camera occlusion rejects, then the second
+    phase against hzb2 (see or_cut_two_phase)."""
+    lib = load()
+    st = initial_state() if state is None else np.ascontiguousarray(state, dtype=np.uint32).copy()
+    words = lib.or_counter_words(pool.slots)
+    c1, c2 = np.zeros(words, dtype=np.uint32), np.zeros(words, dtype=np.uint32)
+    a1, a2 = np.zeros(pool.slots * 20, dtype=np.uint32), np.zeros(pool.slots * 20, dtype=np.uint32)
+    draws = np.zeros((2, pool.max_meshlets, 4), dtype=np.uint32)
+    s1, s2 = OrStats(), OrStats()
+    rejects = np.zeros(3, dtype=np.uint32)
+    sc = pool.struct()
+    h1 = None if hzb1 is None else np.ascontiguousarray(hzb1, dtype=np.float32)
+    h2 = None if hzb2 is None else np.ascontiguousarray(hzb2, dtype=np.float32)
+    v1, v2 = view_struct(view, h1), view_struct(view, h2)
+    cap = pool.max_meshlets if reject_capacity is None else reject_capacity
+    rc = lib.or_cut_two_phase(ctypes.byref(sc), ctypes.byref(v1), ctypes.byref(v2), st.ctypes.data, c1.ctypes.data, a1.ctypes.data,
+                              c2.ctypes.data, a2.ctypes.data, draws.ctypes.data, cap, ctypes.byref(s1), ctypes.byref(s2),
+                              rejects.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"or_cut_two_phase failed ({rc})")
+    first = CutResult(np.concatenate([c1, st]), a1, draws, st, s1)
+    return TwoPhaseResult(first, np.concatenate([c2, np.zeros(8, dtype=np.uint32)]), a2, draws, s2,
+                          (int(rejects[0]), int(rejects[1]), int(rejects[2])))
+
+
 def hzb_build(depth: np.ndarray, max_size: int = 1024) -> np.ndarray:
     lib = load()
     d = np.ascontiguousarray(depth, dtype=np.float32)
@@ -240,7 +281,7 @@ def logf(x: float) -> float:
     return float(load().or_logf(x))
 
 
-def expand(pool: Pool, cut: int, counters: np.ndarray, draws: np.ndarray, indices: Optional[np.ndarray] = None):
+def expand(pool: Pool, cut: int, counters: np.ndarray, draws: np.ndarray, indices: Optional[np.ndarray] = None, first: int = 0):
     """The expand pass (GeometryPool.ts:997-1021) over a cut result: `counters` of the view slot and its
     draw list `draws` (2 * max_meshlets uvec4, camera then shadow). Returns (indices, written): the index
     buffer of `cut` (a copy of `indices`, or zeros, with the written positions replaced) and the count."""
@@ -254,7 +295,8 @@ def expand(pool: Pool, cut: int, counters: np.ndarray, draws: np.ndarray, indice
     t = np.ascontiguousarray(pool.triangles, dtype=np.uint32)
     written = ctypes.c_uint64()
     sc = pool.struct()
-    rc = lib.or_expand(ctypes.byref(sc), t.ctypes.data, len(t), cut, c.ctypes.data, d.ctypes.data, out.ctypes.data, ctypes.byref(written))
+    rc = lib.or_expand(ctypes.byref(sc), t.ctypes.data, len(t), cut, c.ctypes.data, d.ctypes.data, first, out.ctypes.data,
+                       ctypes.byref(written))
     if rc != 0:
         raise RuntimeError(f"or_expand failed ({rc})")
     return out, int(written.value)

@@ -112,7 +112,27 @@ typedef struct {
   float low[2], high[2], fade_scale;
   int fp64;
   or_stats* st;
+  /* two-phase occlusion (the library's design, not the reference's): the cut records its camera
+     occlusion rejects of the final round; the second phase (phase 2) re-tests them */
+  int phase;
+  int record;
+  uint8_t* reject_inst;   /* per instance */
+  uint32_t* reject_mesh;  /* 5 words per record: instance, meshlet, fades, mesh slot, scale bits */
+  uint32_t reject_n, reject_cap, reject_overflow;
 } ctx_t;
+
+static int cell_has_reject(const ctx_t* c, uint32_t first, uint32_t count) {
+  for (uint32_t i = first; i < first + count; i++)
+    if (c->reject_inst[i]) return 1;
+  return 0;
+}
+
+static void clear_rejects(ctx_t* c) {
+  if (!c->record) return;
+  memset(c->reject_inst, 0, c->sc->instance_count);
+  c->reject_n = 0;
+  c->reject_overflow = 0;
+}
 
 static inline uint32_t* counter(ctx_t* c, int cut, int k) { return &c->counters[2 + cut * CUT_COUNTERS + k]; }
 static inline uint32_t* mesh_count(ctx_t* c, int cut, uint32_t slot, uint32_t part) {
@@ -353,6 +373,7 @@ static void cells_pass(ctx_t* c) {
     float gap = view_distance - radius;
     if (gap > max_radius && (max_radius * v->proj_scale) / fmaxf(gap, 1e-6f) < min_pixels) continue;
     if (gap > levers[0]) continue;
+    if (c->phase == 2 && !cell_has_reject(c, (uint32_t)record[2], (uint32_t)record[3])) continue;
     uint32_t at = c->counters[0]++;
     uint32_t* e = &c->cell_list[at * 4];
     e[0] = slot;
@@ -445,13 +466,14 @@ static void instance_pass(ctx_t* c, int retry) {
     float min_pixels = fmaxf(v->min_pixel_radius, levers[1]);
     float max_draw_distance = levers[0];
     for (uint32_t i = cell[1]; i < cell[1] + cell[2]; i++) {
+      if (c->phase == 2 && !c->reject_inst[i]) continue;
       const float* m = &s->instances[(size_t)i * INSTANCE_FLOATS];
       float scale = model_scale(m);
       f4 center = mat_mul(m, point(object_sphere));
       float radius = object_sphere[3] * scale;
       int no_frustum = v->frustum_culling == 0;
       int in_camera = no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, 0) >= -radius;
-      int in_shadow = shadow_on && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, 0) >= -radius);
+      int in_shadow = c->phase != 2 && shadow_on && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, 0) >= -radius);
       if (retry) {
         in_camera = in_camera && redo_camera;
         in_shadow = in_shadow && redo_shadow;
@@ -465,6 +487,7 @@ static void instance_pass(ctx_t* c, int retry) {
       if (in_camera && occluded_sphere(c, occlusion_on, center.x, center.y, center.z, radius)) {
         in_camera = 0;
         c->counters[1]++;
+        if (c->record) c->reject_inst[i] = 1;
       }
       f4 lod_center = mat_mul(v->view, mat_mul(m, point(lod_bounds)));
       float d = len3(lod_center.x, lod_center.y, lod_center.z);
@@ -509,7 +532,22 @@ static void meshlet_pass(ctx_t* c, int cut) {
       uint32_t fade_high = fade > 0.0f ? fade_step(parent, low, fade) : LOD_FADE_STEPS;
       int lod = own <= high && parent > low && fade_high > fade_low;
       /* drawn = visible && lod; the occlusion test (and its count) only matters for clusters in the cut */
-      if (cut == 0 && lod && visible) visible = !occluded_sphere(c, occlusion_on, center.x, center.y, center.z, cull[3] * scale);
+      if (cut == 0 && lod && visible) {
+        visible = !occluded_sphere(c, occlusion_on, center.x, center.y, center.z, cull[3] * scale);
+        if (!visible && c->record) {
+          if (c->reject_n < c->reject_cap) {
+            uint32_t* r = &c->reject_mesh[(size_t)c->reject_n * 5];
+            r[0] = instance;
+            r[1] = meshlet;
+            r[2] = fade_low | (fade_high << 8);
+            r[3] = slot;
+            memcpy(&r[4], &scale, 4);
+          } else {
+            c->reject_overflow = 1;
+          }
+          c->reject_n++;
+        }
+      }
       if (c->fp64) check64(c, cut, m, meshlet, lod);
       if (!(visible && lod)) continue;
       uint32_t at = (*counter(c, cut, C_SELECTED))++;
@@ -530,9 +568,9 @@ static void meshlet_pass(ctx_t* c, int cut) {
 }
 
 /* prefixPass, GeometryPool.ts:973-995 */
-static void prefix_pass(ctx_t* c, int cut) {
+static void prefix_pass_from(ctx_t* c, int cut, uint32_t offset) {
   const or_scene* s = c->sc;
-  uint32_t start = 0, capacity = s->cut_capacity[cut];
+  uint32_t start = offset, capacity = s->cut_capacity[cut];
   for (uint32_t i = 0; i < s->mesh_slots; i++)
     for (uint32_t part = 0; part < 2; part++) {
       uint32_t requested = *mesh_count(c, cut, i, part);
@@ -550,6 +588,8 @@ static void prefix_pass(ctx_t* c, int cut) {
   uint32_t selected = *counter(c, cut, C_SELECTED);
   *counter(c, cut, C_DRAWN) = selected < s->max_meshlets ? selected : s->max_meshlets;
 }
+
+static void prefix_pass(ctx_t* c, int cut) { prefix_pass_from(c, cut, 0); }
 
 /* fillOf, GeometryPool.ts:1027-1031 */
 static float fill_of(ctx_t* c, int cut) {
@@ -573,6 +613,7 @@ static int check_pass(ctx_t* c) {
       c->st->retries[cut]++;
       for (int k = 0; k < CUT_COUNTERS; k++) *counter(c, cut, k) = 0;
       if (cut == 0) c->counters[1] = 0;
+      if (cut == 0) clear_rejects(c);
       for (uint32_t i = 0; i < c->sc->mesh_slots; i++) {
         *mesh_count(c, cut, i, 0) = 0;
         *mesh_count(c, cut, i, 1) = 0;
@@ -594,54 +635,141 @@ static void steer_pass(ctx_t* c) {
   }
 }
 
+static int ctx_open(ctx_t* c, const or_scene* scene, const or_view* view, uint32_t* state, uint32_t* counters, uint32_t* draw_args,
+                    uint32_t* draw_list, int fp64_check, or_stats* stats) {
+  memset(c, 0, sizeof(*c));
+  memset(stats, 0, sizeof(*stats));
+  c->sc = scene;
+  c->v = view;
+  c->state = state;
+  c->counters = counters;
+  c->args = draw_args;
+  c->draw = draw_list;
+  c->fp64 = fp64_check;
+  c->st = stats;
+  c->cell_list = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (scene->cell_count + 1));
+  c->work_cap = scene->instance_count + 1;
+  c->work[0] = (uint32_t*)malloc(sizeof(uint32_t) * 4 * c->work_cap);
+  c->work[1] = (uint32_t*)malloc(sizeof(uint32_t) * 4 * c->work_cap);
+  if (!c->cell_list || !c->work[0] || !c->work[1]) return -1;
+  memset(draw_args, 0, sizeof(uint32_t) * 2 * scene->slots * 2 * 5);
+  return 0;
+}
+
+static void ctx_close(ctx_t* c) {
+  free(c->cell_list);
+  free(c->work[0]);
+  free(c->work[1]);
+}
+
+/* GeometryPool.ts:1081-1092: reset, cells, instance, meshlets x2, prefix x2, 2 retry rounds, steer */
+static void frame(ctx_t* c) {
+  reset(c);
+  load_bands(c);
+  cells_pass(c);
+  instance_pass(c, 0);
+  meshlet_pass(c, 0);
+  meshlet_pass(c, 1);
+  prefix_pass(c, 0);
+  prefix_pass(c, 1);
+  for (int round = 0; round < RETRY_ROUNDS; round++) {
+    int any = check_pass(c);
+    load_bands(c);
+    if (any) {
+      instance_pass(c, 1);
+      for (int cut = 0; cut < 2; cut++)
+        if (get_state(c, S_RETRY + cut) > 0.5f) meshlet_pass(c, cut);
+    }
+    prefix_pass(c, 0);
+    prefix_pass(c, 1);
+  }
+  steer_pass(c);
+}
+
 int or_cut(const or_scene* scene, const or_view* view, uint32_t* state, uint32_t* counters, uint32_t* draw_args,
            uint32_t* draw_list, int fp64_check, or_stats* stats) {
   ctx_t c;
-  memset(&c, 0, sizeof(c));
-  memset(stats, 0, sizeof(*stats));
-  c.sc = scene;
-  c.v = view;
-  c.state = state;
-  c.counters = counters;
-  c.args = draw_args;
-  c.draw = draw_list;
-  c.fp64 = fp64_check;
-  c.st = stats;
-  c.cell_list = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (scene->cell_count + 1));
-  c.work_cap = scene->instance_count + 1;
-  c.work[0] = (uint32_t*)malloc(sizeof(uint32_t) * 4 * c.work_cap);
-  c.work[1] = (uint32_t*)malloc(sizeof(uint32_t) * 4 * c.work_cap);
-  if (!c.cell_list || !c.work[0] || !c.work[1]) {
-    free(c.cell_list);
-    free(c.work[0]);
-    free(c.work[1]);
+  int rc = ctx_open(&c, scene, view, state, counters, draw_args, draw_list, fp64_check, stats);
+  if (rc == 0) frame(&c);
+  ctx_close(&c);
+  return rc;
+}
+
+/*
+ * Two-phase occlusion: the cut of `view1` (its HZB) recording the camera cut's occlusion rejects of the
+ * final round, then the second phase against `view2`'s HZB (the same view otherwise): cells holding a
+ * rejected instance, the instance pass on those instances (camera cut only), the meshlet pass on their
+ * items, and the occlusion re-test of the rejected meshlets, at the cut's final band; appended to the
+ * camera draw list after the cut's entries, regions after its requested indices.
+ */
+int or_cut_two_phase(const or_scene* scene, const or_view* view1, const or_view* view2, uint32_t* state, uint32_t* counters1,
+                     uint32_t* args1, uint32_t* counters2, uint32_t* args2, uint32_t* draw_list, uint32_t reject_capacity,
+                     or_stats* stats1, or_stats* stats2, uint32_t* rejects) {
+  ctx_t c1, c2;
+  int rc = ctx_open(&c1, scene, view1, state, counters1, args1, draw_list, 0, stats1);
+  int rc2 = ctx_open(&c2, scene, view2, state, counters2, args2, draw_list, 0, stats2);
+  c1.reject_inst = (uint8_t*)calloc(scene->instance_count + 1, 1);
+  c1.reject_mesh = (uint32_t*)malloc(sizeof(uint32_t) * 5 * ((size_t)reject_capacity + 1));
+  if (rc || rc2 || !c1.reject_inst || !c1.reject_mesh) {
+    ctx_close(&c1);
+    ctx_close(&c2);
+    free(c1.reject_inst);
+    free(c1.reject_mesh);
     return -1;
   }
-  memset(draw_args, 0, sizeof(uint32_t) * 2 * scene->slots * 2 * 5);
-  /* GeometryPool.ts:1081-1092: reset, cells, instance, meshlets x2, prefix x2, 2 retry rounds, steer */
-  reset(&c);
-  load_bands(&c);
-  cells_pass(&c);
-  instance_pass(&c, 0);
-  meshlet_pass(&c, 0);
-  meshlet_pass(&c, 1);
-  prefix_pass(&c, 0);
-  prefix_pass(&c, 1);
-  for (int round = 0; round < RETRY_ROUNDS; round++) {
-    int any = check_pass(&c);
-    load_bands(&c);
-    if (any) {
-      instance_pass(&c, 1);
-      for (int cut = 0; cut < 2; cut++)
-        if (get_state(&c, S_RETRY + cut) > 0.5f) meshlet_pass(&c, cut);
-    }
-    prefix_pass(&c, 0);
-    prefix_pass(&c, 1);
+  c1.record = 1;
+  c1.reject_cap = reject_capacity;
+  frame(&c1);
+  /* the second phase */
+  c2.phase = 2;
+  c2.reject_inst = c1.reject_inst;
+  reset(&c2);
+  *counter(&c2, 0, C_SELECTED) = counters1[2 + C_SELECTED];
+  c2.low[0] = c1.low[0];
+  c2.high[0] = c1.high[0];
+  c2.fade_scale = c1.fade_scale;
+  cells_pass(&c2);
+  instance_pass(&c2, 0);
+  meshlet_pass(&c2, 0);
+  const or_view* v = c2.v;
+  uint32_t n = c1.reject_n < reject_capacity ? c1.reject_n : reject_capacity;
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t* r = &c1.reject_mesh[(size_t)k * 5];
+    uint32_t instance = r[0], meshlet = r[1], fades = r[2], slot = r[3];
+    float scale;
+    memcpy(&scale, &r[4], 4);
+    const float* m = &scene->instances[(size_t)instance * INSTANCE_FLOATS];
+    const float* cull = &scene->bounds[(size_t)meshlet * 16 + 12];
+    f4 center = mat_mul(m, point(cull));
+    (void)v;
+    if (occluded_sphere(&c2, 1, center.x, center.y, center.z, cull[3] * scale)) continue;
+    uint32_t at = (*counter(&c2, 0, C_SELECTED))++;
+    if (at >= scene->max_meshlets) continue;
+    uint32_t tris = scene->info[meshlet * 4];
+    uint32_t packed = tris | (fades << 8);
+    uint32_t part = (packed & 0xffff00u) == FULL_FADE_BITS ? 0u : 1u;
+    uint32_t* mc = mesh_count(&c2, 0, slot, part);
+    uint32_t first_index = *mc;
+    *mc += tris * 3;
+    uint32_t* d = &draw_list[(size_t)at * 4];
+    d[0] = instance;
+    d[1] = meshlet;
+    d[2] = first_index;
+    d[3] = packed;
   }
-  steer_pass(&c);
-  free(c.cell_list);
-  free(c.work[0]);
-  free(c.work[1]);
+  prefix_pass_from(&c2, 0, counters1[2 + C_REQUESTED]);
+  prefix_pass(&c2, 1);
+  if (rejects) {
+    uint32_t ni = 0;
+    for (uint32_t i = 0; i < scene->instance_count; i++) ni += c1.reject_inst[i];
+    rejects[0] = ni;
+    rejects[1] = c1.reject_n;
+    rejects[2] = c1.reject_overflow;
+  }
+  ctx_close(&c1);
+  ctx_close(&c2);
+  free(c1.reject_inst);
+  free(c1.reject_mesh);
   return 0;
 }
 
@@ -760,12 +888,12 @@ int or_budget_histogram(const or_scene* s, const or_view* v, const uint32_t* sta
  * skipped (the reference's storage access would be out of bounds there).
  */
 int or_expand(const or_scene* s, const uint32_t* triangles, uint64_t triangle_words, int cut, const uint32_t* counters,
-              const uint32_t* draw_list, uint32_t* indices, uint64_t* written) {
+              const uint32_t* draw_list, uint32_t first, uint32_t* indices, uint64_t* written) {
   if (!s || !triangles || !counters || !draw_list || !indices || cut < 0 || cut > 1) return -1;
   const uint32_t capacity = s->cut_capacity[cut];
   const uint32_t drawn = counters[2 + cut * CUT_COUNTERS + C_DRAWN];
   uint64_t n = 0;
-  for (uint32_t at = 0; at < drawn; at++) {
+  for (uint32_t at = first; at < drawn; at++) {
     const uint32_t position = at + (uint32_t)cut * s->max_meshlets;
     const uint32_t* e = &draw_list[(size_t)position * 4];
     if (e[0] >= s->instance_count || e[1] >= s->meshlet_count) return -2;

@@ -81,6 +81,18 @@ uint32_t or_counter_words(uint32_t slots);
 int or_cut(const or_scene* scene, const or_view* view, uint32_t* state, uint32_t* counters, uint32_t* draw_args,
            uint32_t* draw_list, int fp64_check, or_stats* stats);
 
+/*
+ * Two-phase occlusion (the library's design; the reference has one phase): or_cut of view1 recording the
+ * camera cut's occlusion rejects of its final round (instances; meshlets up to reject_capacity), then the
+ * second phase against view2's HZB, appended to draw_list after the cut's entries, into counters2/args2
+ * (C_SELECTED/C_DRAWN/C_REQUESTED over both phases, per-mesh pairs and args of the second phase).
+ * rejects (may be NULL): [0] instances rejected, [1] meshlets rejected, [2] whether meshlet records were
+ * dropped.
+ */
+int or_cut_two_phase(const or_scene* scene, const or_view* view1, const or_view* view2, uint32_t* state, uint32_t* counters1,
+                     uint32_t* args1, uint32_t* counters2, uint32_t* args2, uint32_t* draw_list, uint32_t reject_capacity,
+                     or_stats* stats1, or_stats* stats2, uint32_t* rejects);
+
 /* HZB build (OcclusionCulling.ts:275-352): header + mip 0 = depth + max pyramid. Returns floats written. */
 uint32_t or_hzb_size(uint32_t max_size);
 int or_hzb_build(const float* depth, uint32_t width, uint32_t height, uint32_t max_size, float* hzb);
@@ -102,11 +114,12 @@ int or_budget_histogram(const or_scene* scene, const or_view* view, const uint32
 /*
  * The expand pass (GeometryPool.ts:997-1021) over a given cut result: `counters` and `draw_list` as
  * or_cut or the GPU leave them (2 * max_meshlets entries), `triangles` the pool's meshletTriangles.
- * Writes into `indices` (capacity cut_capacity[cut]) and leaves other positions as they are; `written`
- * receives the number of indices written. Returns 0, or < 0 on an entry outside the pool.
+ * Expands the entries [first, C_DRAWN) (first > 0: the second phase of two-phase occlusion, with its
+ * counters). Writes into `indices` (capacity cut_capacity[cut]) and leaves other positions as they are;
+ * `written` receives the number of indices written. Returns 0, or < 0 on an entry outside the pool.
  */
 int or_expand(const or_scene* scene, const uint32_t* triangles, uint64_t triangle_words, int cut, const uint32_t* counters,
-              const uint32_t* draw_list, uint32_t* indices, uint64_t* written);
+              const uint32_t* draw_list, uint32_t first, uint32_t* indices, uint64_t* written);
 
 #ifdef __cplusplus
 }

@@ -318,6 +318,30 @@ int ensure_slot(vg_scene* s, uint32_t slot) {
   return 0;
 }
 
+// Buffers of two-phase occlusion, sized for every view slot; the instance bitmap grows with the scene.
+int ensure_two_phase(vg_scene* s) {
+  vg_ctx* ctx = s->ctx;
+  const size_t views = ctx->cfg.max_views;
+  if (!s->d_phase.ptr) {
+    if (int rc = ensure(s, s->d_counters2, sizeof(uint32_t) * s->counter_stride * views)) return rc;
+    if (int rc = ensure(s, s->d_args2, sizeof(uint32_t) * s->args_stride * views)) return rc;
+    if (int rc = ensure(s, s->d_reject_meshlets, sizeof(uint4) * ctx->cfg.max_meshlets * views)) return rc;
+    if (int rc = ensure(s, s->d_phase, sizeof(uint32_t) * kPhaseWords * views)) return rc;
+    VG_HIP(ctx, hipMemsetAsync(s->d_counters2.ptr, 0, s->d_counters2.bytes, ctx->stream));
+    VG_HIP(ctx, hipMemsetAsync(s->d_args2.ptr, 0, s->d_args2.bytes, ctx->stream));
+    VG_HIP(ctx, hipMemsetAsync(s->d_phase.ptr, 0, s->d_phase.bytes, ctx->stream));
+  }
+  const uint32_t words = static_cast<uint32_t>((s->i_col[0].size() + 31) / 32 + 1);
+  if (s->reject_words < words) {
+    if (int rc = ensure(s, s->d_reject_bits, sizeof(uint32_t) * words * views)) return rc;
+    VG_HIP(ctx, hipMemsetAsync(s->d_reject_bits.ptr, 0, s->d_reject_bits.bytes, ctx->stream));
+    s->reject_words = words;
+    std::fill(s->phase1_ready.begin(), s->phase1_ready.end(), 0);
+  }
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   vg_ctx* ctx = s->ctx;
   Params p;
@@ -371,6 +395,13 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.indices = static_cast<uint32_t* const*>(s->d_index_ptrs.ptr);
   p.expand_plan = static_cast<unsigned long long*>(lane->d_expand_plan.ptr);
   p.expand_records = static_cast<uint4*>(lane->d_expand_records.ptr);
+  p.phase = 0;
+  p.reject_bits = static_cast<uint32_t*>(s->d_reject_bits.ptr);
+  p.reject_words = s->reject_words;
+  p.reject_meshlets = static_cast<uint4*>(s->d_reject_meshlets.ptr);
+  p.reject_capacity = ctx->cfg.max_meshlets;
+  p.phase_words = static_cast<uint32_t*>(s->d_phase.ptr);
+  p.counters1 = nullptr;
   if (s->footprint) {
     p.fp_instances = static_cast<uint32_t*>(s->d_fp_instances.ptr);
     p.fp_meshlets = static_cast<uint32_t*>(s->d_fp_meshlets.ptr);
@@ -472,13 +503,41 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   return 0;
 }
 
+int enqueue_phase2(vg_scene* s, Lane* lane, uint32_t view_count) {
+  vg_ctx* ctx = s->ctx;
+  hipStream_t st = lane->stream;
+  Params p = make_params(s, lane, view_count);
+  p.phase = 2;
+  p.counters1 = static_cast<const uint32_t*>(s->d_counters.ptr);
+  p.counters = static_cast<uint32_t*>(s->d_counters2.ptr);
+  p.args = static_cast<uint32_t*>(s->d_args2.ptr);
+  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
+  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
+  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
+  hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
+  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
+  hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_retest_plan, dim3(1), dim3(64), 0, st, p);
+  hipLaunchKernelGGL(k_retest, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_prefix, dim3(view_count), dim3(128), 0, st, p, 2u, 0u);
+  if (ctx->cfg.expand) {
+    hipLaunchKernelGGL(k_expand_plan, dim3(1), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_expand_resolve, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(k_expand, dim3(static_cast<uint32_t>(ctx->cus) * kExpandBlocksPerCu), dim3(256), 0, st, p);
+  }
+  VG_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
 int lane_reserve_views(vg_ctx* ctx, Lane* lane) {
   VG_HIP(ctx, hipStreamSynchronize(lane->stream));
   if (int rc = ensure_lane(ctx, lane, lane->d_views, sizeof(ViewDev) * ctx->cfg.max_views)) return rc;
+  // chunk plans of the expand pass and of the second phase's re-test
+  if (int rc = ensure_lane(ctx, lane, lane->d_expand_plan, sizeof(unsigned long long) * (2 * static_cast<size_t>(ctx->cfg.max_views) + 1)))
+    return rc;
   if (ctx->cfg.expand) {
     const size_t chunks = 2 * static_cast<size_t>(ctx->cfg.max_views) * ((ctx->cfg.max_meshlets + 63) / 64);
-    if (int rc = ensure_lane(ctx, lane, lane->d_expand_plan, sizeof(unsigned long long) * (2 * static_cast<size_t>(ctx->cfg.max_views) + 1)))
-      return rc;
     if (int rc = ensure_lane(ctx, lane, lane->d_expand_records, sizeof(uint4) * 64 * chunks)) return rc;
   }
   for (int r = 0; r < Lane::kRing; r++) {
@@ -511,8 +570,14 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
       return fail(ctx, VG_E_INVALID, "vg_cut: a triangle budget needs 0 < threshold_range[0] <= threshold_range[1]");
     if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0))
       return fail(ctx, VG_E_INVALID, "vg_cut: proj_scale and near_plane must be positive");
+    if (views[i].occlusion > 2) return fail(ctx, VG_E_INVALID, "vg_cut: occlusion must be 0, 1 or 2");
   }
   if (int rc = upload_scene(s)) return rc;
+  for (uint32_t i = 0; i < count; i++)
+    if (views[i].occlusion == 2) {
+      if (int rc = ensure_two_phase(s)) return rc;
+      break;
+    }
   for (uint32_t i = 0; i < count; i++)
     if (int rc = ensure_slot(s, views[i].slot)) return rc;
   size_t queue_bytes = 0, want = 0;
@@ -566,6 +631,7 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
     v.range_hi = in.threshold_range[1];
     v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
     v.fresh = fresh ? 1u : 0u;
+    v.two_phase = in.occlusion == 2 ? 1u : 0u;
   }
   lane->launch_budget = false;
   for (uint32_t i = 0; i < count; i++)
@@ -723,6 +789,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
   s->d_draws.resize(views);
   s->d_hzbs.resize(views);
   s->d_indices.resize(2 * static_cast<size_t>(views));
+  s->phase1_ready.assign(views, 0);
   s->hzb_valid.assign(views, 0);
   s->h_draw_ptrs.assign(views, nullptr);
   s->h_hzb_ptrs.assign(views, nullptr);
@@ -763,7 +830,8 @@ void vg_scene_destroy(vg_scene* s) {
   DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
-                    &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs};
+                    &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
+                    &s->d_reject_meshlets, &s->d_phase};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_indices) release(s, b);
@@ -899,6 +967,7 @@ int vg_instances_set(vg_scene* s, uint32_t slot, const float* matrices, const fl
                      uint32_t* gpu_to_input) {
   (void)colors;  // tints are raster-side data (GeometryPool.ts:490-491); the cut does not read them
   if (!s || slot >= s->slots || !s->meshes[slot].used || (!matrices && count)) return VG_E_INVALID;
+  std::fill(s->phase1_ready.begin(), s->phase1_ready.end(), 0);  // instance slots move: recorded rejects no longer apply
   vg_ctx* ctx = s->ctx;
   MeshHost& m = s->meshes[slot];
   const size_t total = s->i_col[0].size();
@@ -1008,16 +1077,68 @@ int vg_hzb_invalidate(vg_scene* s, uint32_t view_slot) {
   return VG_OK;
 }
 
+// After a cut: a view that asked for two-phase occlusion may take its second phase next.
+static void mark_phase1(vg_scene* s, const vg_view* views, uint32_t count) {
+  for (uint32_t i = 0; i < count; i++) s->phase1_ready[views[i].slot] = views[i].occlusion == 2 ? 1 : 0;
+}
+
 int vg_cut(vg_scene* s, const vg_view* views, uint32_t count) {
   if (!s) return VG_E_INVALID;
   if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
-  return enqueue_cut(s, &s->lane, count, false);
+  if (int rc = enqueue_cut(s, &s->lane, count, false)) return rc;
+  mark_phase1(s, views, count);
+  return VG_OK;
 }
 
 int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
   if (!s) return VG_E_INVALID;
   if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
-  return launch_graph(s, &s->lane, count);
+  if (int rc = launch_graph(s, &s->lane, count)) return rc;
+  mark_phase1(s, views, count);
+  return VG_OK;
+}
+
+int vg_cut_phase2(vg_scene* s, const vg_view* views, uint32_t count) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!views || count == 0) return fail(ctx, VG_E_INVALID, "vg_cut_phase2: no views");
+  for (uint32_t i = 0; i < count; i++)
+    if (views[i].slot >= ctx->cfg.max_views || views[i].occlusion != 2 || !s->phase1_ready[views[i].slot])
+      return fail(ctx, VG_E_INVALID, "vg_cut_phase2: every view needs occlusion = 2 and its slot's cut (vg_cut) just before");
+  if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
+  if (int rc = enqueue_phase2(s, &s->lane, count)) return rc;
+  for (uint32_t i = 0; i < count; i++) s->phase1_ready[views[i].slot] = 0;
+  return VG_OK;
+}
+
+int vg_read_phase2(vg_scene* s, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  if (view_slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_read_phase2: view slot out of range");
+  if (!s->d_phase.ptr) return fail(ctx, VG_E_INVALID, "vg_read_phase2: no view of this scene asked for two-phase occlusion");
+  if (counters)
+    VG_HIP(ctx, hipMemcpyAsync(counters, static_cast<uint32_t*>(s->d_counters2.ptr) + static_cast<size_t>(view_slot) * s->counter_stride,
+                               sizeof(uint32_t) * s->counter_stride, hipMemcpyDeviceToHost, ctx->stream));
+  if (draw_args)
+    VG_HIP(ctx, hipMemcpyAsync(draw_args, static_cast<uint32_t*>(s->d_args2.ptr) + static_cast<size_t>(view_slot) * s->args_stride,
+                               sizeof(uint32_t) * s->args_stride, hipMemcpyDeviceToHost, ctx->stream));
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return VG_OK;
+}
+
+int vg_phase_rejects(vg_scene* s, uint32_t view_slot, uint32_t* meshlets, uint32_t* dropped) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "host-only context");
+  if (view_slot >= ctx->cfg.max_views || !s->d_phase.ptr) return fail(ctx, VG_E_INVALID, "vg_phase_rejects: no two-phase data for this slot");
+  uint32_t w[kPhaseWords];
+  VG_HIP(ctx, hipMemcpyAsync(w, static_cast<uint32_t*>(s->d_phase.ptr) + static_cast<size_t>(view_slot) * kPhaseWords, sizeof(w),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (meshlets) *meshlets = w[P_REJECTS];
+  if (dropped) *dropped = w[P_OVERFLOW];
+  return VG_OK;
 }
 
 int vg_sync(vg_scene* s) {

@@ -108,6 +108,11 @@ struct vg_scene {
   vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;
   std::vector<vgi::DevBuf> d_draws, d_hzbs;
   std::vector<vgi::DevBuf> d_indices;  // 2 per slot (camera, shadow) on contexts with the expand pass
+  // two-phase occlusion (allocated by the first view that asks for it): the second phase's counters and
+  // args per slot, the recorded rejects (instance bitmap, meshlet records) and the per-slot phase words
+  vgi::DevBuf d_counters2, d_args2, d_reject_bits, d_reject_meshlets, d_phase;
+  uint32_t reject_words = 0;
+  std::vector<uint8_t> phase1_ready;  // per slot: its last cut recorded rejects and no second phase ran since
   std::vector<uint8_t> hzb_valid;
   std::vector<uint4*> h_draw_ptrs;
   std::vector<float*> h_hzb_ptrs;
@@ -152,6 +157,8 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
 int enqueue_cut(vg_scene* s, Lane* lane, uint32_t count, bool capturing);
 // Replays (capturing first when needed) the launch sequence from the lane's graph.
 int launch_graph(vg_scene* s, Lane* lane, uint32_t count);
+// Enqueues the second phase of two-phase occlusion for the views staged by prepare_cut.
+int enqueue_phase2(vg_scene* s, Lane* lane, uint32_t count);
 // Copies a view slot's results out on `stream` and waits for it.
 int read_slot(vg_scene* s, hipStream_t stream, uint32_t view_slot, uint32_t* counters, uint32_t* draw_args, uint32_t* draw_list,
               uint32_t max_entries, vg_metrics* metrics);

@@ -56,6 +56,10 @@ constexpr uint32_t kQueue = 64 * (kMaxRun + 1);
 // An index is (draw-list position << kLocalVertexBits) | local vertex (GeometryPool.ts:92, 1009-1018).
 constexpr uint32_t kLocalVertexBits = 7;  // ceil(log2(MAX_MESHLET_VERTICES = 128))
 constexpr uint32_t kExpandBlocksPerCu = 8;  // persistent 256-thread blocks of the expand pass
+// Per view slot words of two-phase occlusion: meshlet rejects recorded, whether more were dropped, and
+// the camera band of the cut's final round (binary32 bits), which the second phase tests with.
+constexpr uint32_t kPhaseWords = 8;
+enum : uint32_t { P_REJECTS = 0, P_OVERFLOW = 1, P_LOW = 2, P_HIGH = 3, P_FADE = 4 };
 
 // Literals as the reference's WGSL sees them: JS doubles rounded to binary32.
 #define VG_INF_HALF ((float)(1e30 * 0.5))
@@ -76,6 +80,7 @@ struct ViewDev {
   float range_lo, range_hi;
   uint32_t hzb_valid;
   uint32_t fresh;  // 1: the slot's controller state starts at rest this launch (scale 1, as on a new slot)
+  uint32_t two_phase;  // 1: occlusion in two phases: the cut records its occlusion rejects for the second
   // derived
   float low[2], high[2];
   float fade_scale;
@@ -142,6 +147,15 @@ struct Params {
   uint32_t* const* indices;              // per slot: camera index buffer, shadow index buffer
   unsigned long long* expand_plan;       // per (view, cut): first 64-entry chunk; then the chunk total
   uint4* expand_records;                 // per drawn entry (64 per chunk): first index, first triangle, triangles, position
+  // Two-phase occlusion: a cut records, for views that ask for it, the camera cut's occlusion rejects of
+  // its final round; the second phase re-tests them against the slot's new HZB.
+  uint32_t phase;                        // 0: a cut (recording for two-phase views); 2: the second phase
+  uint32_t* reject_bits;                 // per view slot: reject_words words; bit i: instance i occluded
+  uint32_t reject_words;
+  uint4* reject_meshlets;                // per view slot: (instance, meshlet, fades | mesh slot << 16, scale bits)
+  uint32_t reject_capacity;              // meshlet rejects per view slot
+  uint32_t* phase_words;                 // per view slot: kPhaseWords (see P_*)
+  const uint32_t* counters1;             // second phase: the cut's counters (p.counters holds the second phase's)
 };
 
 __host__ __device__ inline uint32_t state_base(uint32_t slots) { return kPerMeshCounters + 2u * slots * 4u; }

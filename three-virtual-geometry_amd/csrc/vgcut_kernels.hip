@@ -35,6 +35,40 @@ __device__ __forceinline__ unsigned long long* metric_at(const Params& p, uint32
   return p.metrics + static_cast<size_t>(slot) * kMetricWords + k;
 }
 
+__device__ __forceinline__ uint32_t* phase_at(const Params& p, uint32_t slot) { return p.phase_words + static_cast<size_t>(slot) * kPhaseWords; }
+
+// Whether any of instances [first, first + count) is a recorded occlusion reject of the slot.
+__device__ __forceinline__ bool any_rejected(const Params& p, uint32_t slot, uint32_t first, uint32_t count) {
+  if (count == 0) return false;
+  const uint32_t* bits = p.reject_bits + static_cast<size_t>(slot) * p.reject_words;
+  const uint32_t last = first + count - 1;
+  for (uint32_t w = first >> 5; w <= last >> 5; w++) {
+    const uint32_t lo = w == (first >> 5) ? (first & 31u) : 0u;
+    const uint32_t hi = w == (last >> 5) ? (last & 31u) : 31u;
+    const uint32_t mask = (hi == 31u ? 0xffffffffu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+    if (bits[w] & mask) return true;
+  }
+  return false;
+}
+
+// Records the occlusion rejects of instances [first, first + n) (n <= 64; bit j of `occluded`: instance
+// first + j) in the slot's bitmap, replacing what an earlier round recorded for them. Lane 0 only.
+__device__ __forceinline__ void record_instance_rejects(const Params& p, uint32_t slot, uint32_t first, uint32_t n, uint64_t occluded) {
+  uint32_t* bits = p.reject_bits + static_cast<size_t>(slot) * p.reject_words;
+  const uint32_t last = first + n - 1;
+  for (uint32_t w = first >> 5; w <= last >> 5; w++) {
+    const uint32_t lo = w == (first >> 5) ? (first & 31u) : 0u;
+    const uint32_t hi = w == (last >> 5) ? (last & 31u) : 31u;
+    const uint32_t range = (hi == 31u ? 0xffffffffu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+    // bits of `occluded` for this word: instance w * 32 + b is bit (w * 32 + b - first) of the mask
+    const int shift = static_cast<int>(w * 32u) - static_cast<int>(first);
+    const uint64_t m = shift >= 0 ? (occluded >> shift) : (occluded << (-shift));
+    const uint32_t set = static_cast<uint32_t>(m) & range;
+    atomicAnd(&bits[w], ~range);
+    if (set) atomicOr(&bits[w], set);
+  }
+}
+
 __device__ __forceinline__ float get_state(const uint32_t* state, uint32_t i) {
   return static_cast<float>(state[i]) / 10000.0f;
 }
@@ -143,7 +177,29 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   ViewDev& v = p.views[blockIdx.x];
   uint32_t* c = view_counters(p, v.slot);
   for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
+  if (p.phase == 2) {
+    // Second phase: the camera cut continues the cut's draw-list numbering, with its final band; there
+    // is no shadow cut, retry or budget, and the slot's metrics add up over both phases.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t* pw = phase_at(p, v.slot);
+      *counter_at(p, v.slot, 0, kSelected) = p.counters1[static_cast<size_t>(v.slot) * p.counter_stride + 2 + kSelected];
+      v.low[0] = __uint_as_float(pw[P_LOW]);
+      v.high[0] = __uint_as_float(pw[P_HIGH]);
+      v.fade_scale = __uint_as_float(pw[P_FADE]);
+      v.low[1] = v.high[1] = 0.0f;
+      v.redo[0] = v.redo[1] = 0;
+      v.budget = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kWorkParts) p.work_counter[threadIdx.x * kWorkCounterSpacing] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *p.cell_queue_count = 0;
+      p.flags[0] = p.flags[1] = p.flags[2] = p.flags[3] = p.flags[4] = 0;
+    }
+    return;
+  }
   if (threadIdx.x < kMetricWords) *metric_at(p, v.slot, threadIdx.x) = 0;
+  if (v.two_phase && threadIdx.x < 2) phase_at(p, v.slot)[threadIdx.x] = 0;  // P_REJECTS, P_OVERFLOW
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t* state = state_at(p, v.slot);
@@ -207,6 +263,8 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
       const float gap = view_distance - radius;
       const bool small = gap > max_radius && (max_radius * v.proj_scale) / fmaxf(gap, 1e-6f) < min_pixels;
       keep = keep && !small && !(gap > levers.x);
+      // second phase: only cells holding an instance the cut rejected by occlusion
+      if (p.phase == 2 && keep) keep = any_rejected(p, v.slot, static_cast<uint32_t>(record.z), min(static_cast<uint32_t>(record.w), kCellSize));
       if (keep) mask |= 1ull << (vi - v_first);
       const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(keep)));
       if (lane == 0 && n) atomicAdd(&visible[vi - v_first], n);
@@ -365,12 +423,13 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     const uint32_t cell = uniform(entry.y);
     if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next entry, loaded while this one runs
     const auto& v = scalar(p.views)[vi];
-    bool redo_camera = true, redo_shadow = true;
+    bool redo_camera = true, redo_shadow = p.phase != 2;  // the second phase has no shadow cut
     if (retry_round) {
       redo_camera = v.redo[0] != 0;
       redo_shadow = v.redo[1] != 0;
       if (!redo_camera && !redo_shadow) continue;
     }
+    const bool record = p.phase == 0 && v.two_phase != 0 && redo_camera;
     const uint32_t vslot = uniform(v.slot);
     if (vslot != acc_slot) {
       flush();
@@ -446,8 +505,10 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       item_lo[r][0] = item_hi[r][0] = item_lo[r][1] = item_hi[r][1] = 0;
       item_scale[r] = 0.0f;
       if (r * 64 >= count) continue;  // wave-uniform
-      const bool valid = r * 64 + lane < count;
       const uint32_t instance = first_instance + r * 64 + lane;
+      bool valid = r * 64 + lane < count;
+      if (p.phase == 2 && valid)  // the second phase re-tests the instances the cut rejected by occlusion
+        valid = (p.reject_bits[static_cast<size_t>(vslot) * p.reject_words + (instance >> 5)] >> (instance & 31u)) & 1u;
       const float scale = d_scale[r];
       item_scale[r] = scale;
       const float3 center = make_float3(d_center[r][0], d_center[r][1], d_center[r][2]);
@@ -468,6 +529,10 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       acc_occ_tests += static_cast<uint32_t>(__popcll(__ballot(test)));
       acc_occluded += static_cast<uint32_t>(__popcll(__ballot(occ)));
       in_camera = in_camera && !occ;
+      if (record) {
+        const uint64_t occluded = __ballot(occ);
+        if (lane == 0) record_instance_rejects(p, vslot, first_instance + r * 64, min(64u, count - r * 64), occluded);
+      }
 
       const bool in_cut[2] = {keep && in_camera, keep && in_shadow};
       const bool any0 = __ballot(in_cut[0]) != 0, any1 = __ballot(in_cut[1]) != 0;
@@ -688,7 +753,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const bool occ_on = (tag & kTagOcclusion) != 0, fast = (tag & kTagFast) != 0;
     const uint32_t meshlet_tris = active ? p.m_tris[meshlet] : 0u;
     bool drawn = active && fast;  // fast path: guaranteed meshlets of a culling sphere inside the frustum
-    bool occ_test = false;
+    bool occ_test = false, rejected = false;
     uint32_t vslot = 0;
     // Visibility, one launched view at a time (its constants are wave-uniform there).
     for (uint64_t pending = __ballot(active); pending;) {
@@ -705,6 +770,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
           bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
           occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
           if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
+          rejected = occ_test && !visible && v.two_phase && p.phase == 0;
           drawn = visible;
         }
       }
@@ -722,6 +788,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       }
       occ_count += static_cast<uint32_t>(__popcll(same));
       tests &= ~same;
+    }
+    if (__ballot(rejected)) {  // two-phase views: keep the occluded meshlets for the second phase
+      const uint32_t rat = wave_atomic_add(phase_at(p, vslot) + P_REJECTS, 1u, rejected);
+      if (rejected && rat < p.reject_capacity)
+        p.reject_meshlets[static_cast<size_t>(vslot) * p.reject_capacity + rat] =
+            make_uint4(instance, meshlet, c.w | (mslot << 16), __float_as_uint(scale));
+      else if (rejected)
+        atomicOr(phase_at(p, vslot) + P_OVERFLOW, 1u);
     }
     const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
     const bool write = drawn && at < p.max_meshlets;
@@ -1012,11 +1086,12 @@ __global__ __launch_bounds__(64) void k_budget_pick(Params p) {
 
 // ------------------------------------------------------------------ 4. prefix + capacity (GeometryPool.ts:973-995, 1023-1069)
 
-__device__ void prefix_cut(const Params& p, uint32_t slot, uint32_t cut) {
+// `offset`: the first region start (the second phase lays its regions out after the cut's).
+__device__ void prefix_cut(const Params& p, uint32_t slot, uint32_t cut, uint32_t offset = 0) {
   const uint32_t lane = lane_id();
   const uint32_t capacity = p.cut_capacity[cut];
   uint32_t* args = p.args + static_cast<size_t>(slot) * p.args_stride;
-  uint32_t carry = 0;
+  uint32_t carry = offset;
   for (uint32_t base = 0; base < p.mesh_slots; base += 64) {
     const uint32_t mesh = base + lane;
     const bool valid = mesh < p.mesh_slots;
@@ -1059,7 +1134,12 @@ __device__ float fill_of(const Params& p, uint32_t slot, uint32_t cut) {
 __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_t round) {
   ViewDev& v = p.views[blockIdx.x];
   const uint32_t slot = v.slot;
-  prefix_cut(p, slot, threadIdx.x / 64);
+  const uint32_t cut = threadIdx.x / 64;
+  if (mode == 2) {  // second phase: regions after the cut's requested indices; no check or steer
+    prefix_cut(p, slot, cut, cut == 0 ? p.counters1[static_cast<size_t>(slot) * p.counter_stride + 2 + kRequested] : 0u);
+    return;
+  }
+  prefix_cut(p, slot, cut);
   __syncthreads();
   if (mode == 0 && blockIdx.x == 0 && threadIdx.x < kWorkParts)  // the retry round's instance pass refills the work list
     p.work_counter[threadIdx.x * kWorkCounterSpacing] = 0;
@@ -1079,6 +1159,7 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
         any = true;
         for (uint32_t k = 0; k < kCutCounters; k++) *counter_at(p, slot, cut, k) = 0;
         if (cut == 0) view_counters(p, slot)[1] = 0;
+        if (cut == 0 && v.two_phase) phase_at(p, slot)[P_REJECTS] = phase_at(p, slot)[P_OVERFLOW] = 0;  // recorded again
         for (uint32_t i = 0; i < p.mesh_slots; i++) {
           *mesh_count_at(p, slot, cut, i, 0) = 0;
           *mesh_count_at(p, slot, cut, i, 1) = 0;
@@ -1088,6 +1169,12 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
     load_bands(v, state);
     if (any) atomicOr(&p.flags[round - 1], 1u);
   } else {
+    if (v.two_phase) {  // the band the final round tested with, for the second phase
+      uint32_t* pw = phase_at(p, slot);
+      pw[P_LOW] = __float_as_uint(v.low[0]);
+      pw[P_HIGH] = __float_as_uint(v.high[0]);
+      pw[P_FADE] = __float_as_uint(v.fade_scale);
+    }
     for (uint32_t cut = 0; cut < 2; cut++) {
       const float fill = fill_of(p, slot, cut);
       float scale = get_state(state, kStateScale + cut);
@@ -1096,6 +1183,66 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
       set_state(state, kStateScale + cut, fmaxf(fminf(scale, v.lod_scale_limit), 1.0f));
       set_state(state, kStateFill + cut, fill);
     }
+  }
+}
+
+// ------------------------------------------------------------------ second phase of two-phase occlusion
+//
+// The cut of a two-phase view records its camera cut's occlusion rejects (final round): instances
+// rejected by the instance test in a bitmap, meshlets rejected in the meshlet pass as (instance, meshlet,
+// blend range, mesh slot, scale). Once the caller has drawn the cut and set the slot's new HZB, the
+// second phase runs the cells, instance and meshlet passes again on the rejected instances only (camera
+// cut, the final round's band, occlusion against the new HZB) and re-tests the rejected meshlets' culling
+// spheres against the new HZB (their frustum and LOD tests passed). What is visible now is appended to the
+// camera draw list after the cut's entries and laid out in regions after the cut's indices.
+
+// expand_plan[view] = first 64-record chunk of the view's meshlet rejects; expand_plan[views] = the total.
+__global__ __launch_bounds__(64) void k_retest_plan(Params p) {
+  const uint32_t lane = lane_id();
+  unsigned long long carry = 0;
+  for (uint32_t base = 0; base < p.view_count; base += 64) {
+    const uint32_t vi = base + lane;
+    uint32_t chunks = 0;
+    if (vi < p.view_count) chunks = (min(phase_at(p, p.views[vi].slot)[P_REJECTS], p.reject_capacity) + 63u) / 64u;
+    const uint32_t incl = wave_scan(chunks);
+    if (vi < p.view_count) p.expand_plan[vi] = carry + (incl - chunks);
+    carry += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+  }
+  if (lane == 0) p.expand_plan[p.view_count] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_retest(Params p) {
+  const uint32_t lane = lane_id();
+  const unsigned long long* __restrict__ plan = p.expand_plan;
+  const uint64_t total = plan[p.view_count];
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  uint32_t vi = 0;
+  for (uint64_t c = xcd_wave_rank(); c < total; c += waves) {
+    while (plan[vi + 1] <= c) vi++;
+    const auto& v = scalar(p.views)[uniform(vi)];
+    const uint32_t slot = uniform(v.slot);
+    const uint32_t n = min(phase_at(p, slot)[P_REJECTS], p.reject_capacity);
+    const uint32_t e = static_cast<uint32_t>(c - plan[vi]) * 64u + lane;
+    const bool active = e < n;
+    const uint4 r = active ? p.reject_meshlets[static_cast<size_t>(slot) * p.reject_capacity + e] : make_uint4(0, 0, 0, 0);
+    const uint32_t instance = r.x, meshlet = r.y, fades = r.z & 0xffffu, mslot = r.z >> 16;
+    bool visible = active;
+    const bool occ_test = active && v.hzb_valid;  // recorded meshlets are of meshes with occlusion on
+    if (occ_test) {
+      const float4 c0 = p.i_col0[instance], c1 = p.i_col1[instance], c2 = p.i_col2[instance], c3 = p.i_col3[instance];
+      const float4 cs = p.m_cull[meshlet];
+      const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+      visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * __uint_as_float(r.w));
+    }
+    const uint32_t tests = static_cast<uint32_t>(__popcll(__ballot(occ_test)));
+    if (lane == 0 && tests) atomicAdd(metric_at(p, slot, M_OCCLUSION), static_cast<unsigned long long>(tests));
+    const uint32_t tris = visible ? p.m_tris[meshlet] : 0u;
+    const uint32_t packed_tris = tris | (fades << 8);
+    const uint32_t at = wave_atomic_add_uniform(counter_at(p, slot, 0, kSelected), 1u, visible);
+    const bool write = visible && at < p.max_meshlets;
+    const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
+    const uint32_t first_index = wave_atomic_add(mesh_count_at(p, slot, 0, mslot, part), tris * 3, write);
+    if (write) p.draws[slot][at] = make_uint4(instance, meshlet, first_index, packed_tris);
   }
 }
 
@@ -1110,6 +1257,11 @@ __global__ __launch_bounds__(128) void k_prefix(Params p, uint32_t mode, uint32_
 // destination; the writing pass then takes a chunk's resolved records with one coalesced load and
 // writes the entries that have anything to write, a lane per triangle, several entries at a time.
 
+// The first draw-list entry the expand pass writes: the second phase writes only the entries it appended.
+__device__ __forceinline__ uint32_t expand_first(const Params& p, uint32_t slot, uint32_t cut) {
+  return p.phase == 2 && cut == 0 ? p.counters1[static_cast<size_t>(slot) * p.counter_stride + 2 + kDrawn] : 0u;
+}
+
 // expand_plan[2 * view + cut] = first chunk of that cut's draw list; expand_plan[2 * views] = the total.
 __global__ __launch_bounds__(64) void k_expand_plan(Params p) {
   const uint32_t lane = lane_id();
@@ -1118,7 +1270,10 @@ __global__ __launch_bounds__(64) void k_expand_plan(Params p) {
   for (uint32_t base = 0; base < segments; base += 64) {
     const uint32_t seg = base + lane;
     uint32_t chunks = 0;
-    if (seg < segments) chunks = (*counter_at(p, p.views[seg >> 1].slot, seg & 1u, kDrawn) + 63u) / 64u;
+    if (seg < segments) {
+      const uint32_t slot = p.views[seg >> 1].slot;
+      chunks = (*counter_at(p, slot, seg & 1u, kDrawn) - expand_first(p, slot, seg & 1u) + 63u) / 64u;
+    }
     const uint32_t incl = wave_scan(chunks);  // at most 64 x 2^18 (drawn <= 2^24 per cut)
     if (seg < segments) p.expand_plan[seg] = carry + (incl - chunks);
     carry += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
@@ -1148,7 +1303,7 @@ __global__ __launch_bounds__(256) void k_expand_resolve(Params p) {
     const uint32_t cut = seg & 1u;
     const uint32_t slot = uniform(p.views[seg >> 1].slot);
     const uint32_t drawn = *counter_at(p, slot, cut, kDrawn);
-    const uint32_t first = static_cast<uint32_t>(c - plan[seg]) * 64u;
+    const uint32_t first = expand_first(p, slot, cut) + static_cast<uint32_t>(c - plan[seg]) * 64u;
     const uint32_t position = first + lane + cut * p.max_meshlets;
     const uint32_t limit = p.cut_capacity[cut] - 2u;  // index + 2 < capacity  <=>  index < limit
     uint4 r = make_uint4(0, 0, 0, position);

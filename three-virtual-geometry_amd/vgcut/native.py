@@ -35,6 +35,7 @@ EXPORTS = (
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
+    "vg_cut_phase2", "vg_read_phase2", "vg_phase_rejects",
     "vg_server_create", "vg_server_destroy", "vg_server_last_error", "vg_server_add_scene", "vg_server_submit", "vg_server_poll",
     "vg_server_read", "vg_server_release", "vg_server_stats_get",
 )
@@ -219,6 +220,9 @@ def load():
         "vg_schedule_set": ([P, ctypes.POINTER(Schedule)], ctypes.c_int),
         "vg_tune": ([P, P, U32, U32, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "vg_index_read": ([P, U32, U32, U64, U64, P], ctypes.c_int),
+        "vg_cut_phase2": ([P, P, U32], ctypes.c_int),
+        "vg_read_phase2": ([P, U32, P, P], ctypes.c_int),
+        "vg_phase_rejects": ([P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32)], ctypes.c_int),
         "vg_index_buffer": ([P, U32, U32, ctypes.POINTER(P), ctypes.POINTER(U64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -346,6 +350,26 @@ class Scene:
 
     def sync(self):
         self.ctx.check(self.lib.vg_sync(self.handle), "vg_sync")
+
+    def cut_phase2(self, views):
+        """Second phase of two-phase occlusion for the views of the last cut (occlusion = 2), against the
+        slots' current HZBs."""
+        arr = (View * len(views))(*views)
+        self.ctx.check(self.lib.vg_cut_phase2(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views)), "vg_cut_phase2")
+        self._keep = arr
+
+    def read_phase2(self, view_slot: int):
+        """The second phase's counters and draw args of a view slot."""
+        counters = np.empty(self.ctx.counter_words, dtype=np.uint32)
+        args = np.empty(self.ctx.draw_args_words, dtype=np.uint32)
+        self.ctx.check(self.lib.vg_read_phase2(self.handle, view_slot, counters.ctypes.data, args.ctypes.data), "vg_read_phase2")
+        return counters, args
+
+    def phase_rejects(self, view_slot: int):
+        """(meshlet rejects recorded by the slot's last cut, whether more were dropped)."""
+        n, dropped = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx.check(self.lib.vg_phase_rejects(self.handle, view_slot, ctypes.byref(n), ctypes.byref(dropped)), "vg_phase_rejects")
+        return int(n.value), bool(dropped.value)
 
     def read(self, view_slot: int, max_entries: Optional[int] = None):
         """Counters, draw args, draw lists (camera, shadow) and metrics of a view slot."""

@@ -56,7 +56,8 @@ def make_view(camera: cam.Camera, viewport_height: float, *, slot: int = 0, erro
     v.min_pixel_radius = min_pixel_radius
     v.frustum_culling = 1 if frustum_culling else 0
     # The occlusion test assumes a perspective camera (OcclusionCulling.ts:232-237).
-    v.occlusion = 1 if (occlusion and camera.isPerspectiveCamera) else 0
+    # occlusion=2: two-phase (the cut records its occlusion rejects for vg_cut_phase2).
+    v.occlusion = (2 if occlusion == 2 else 1) if (occlusion and camera.isPerspectiveCamera) else 0
     v.slot = slot
     v.triangle_budget = int(triangle_budget)
     v.threshold_range[0] = threshold_range[0]
