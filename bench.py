@@ -385,6 +385,64 @@ def expand_bench(spec, vd, steps, warmup, schedule, profile_only=False):
     return out
 
 
+def two_phase_bench(scene, vd, steps):
+    """Two-phase occlusion on the batched views: the views with an HZB ask for it (occlusion = 2), the cut
+    records its camera occlusion rejects, then the second phase re-tests them against each slot's HZB set
+    again from the same terrain depth (as when the depth drawn in between shows the same occluders).
+    Reports the cut's time with recording, the second phase's time and what it re-tested and added."""
+    import ctypes
+
+    import torch
+
+    from vgcut import native
+
+    views = []
+    for v, d in vd:
+        w = native.View()
+        ctypes.memmove(ctypes.byref(w), ctypes.byref(v), ctypes.sizeof(native.View))
+        if d is not None and v.occlusion:
+            w.occlusion = 2
+        views.append(w)
+    two = [v for v in views if v.occlusion == 2]
+    initial = np.zeros(8, dtype=np.uint32)
+    initial[0] = initial[1] = 10000
+    scene.cut(views, graph=True)
+    scene.cut_phase2(two)
+    scene.sync()
+    t_cut = t_phase2 = 0.0
+    for _ in range(steps):
+        for v in views:
+            scene.set_state(v.slot, initial)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.cut(views, graph=True)
+        scene.sync()
+        t_cut += time.perf_counter() - t0
+        for v, d in vd:  # the new depth of each two-phase slot (the same terrain: no new occluders)
+            if d is not None and v.occlusion:
+                scene.set_hzb(v.slot, d)
+        scene.sync()
+        t0 = time.perf_counter()
+        scene.cut_phase2(two)
+        scene.sync()
+        t_phase2 += time.perf_counter() - t0
+    rejected_meshlets = sum(scene.phase_rejects(v.slot)[0] for v in two)
+    rejected_instances = 0
+    added = 0
+    for v in two:
+        c1 = scene.read(v.slot, max_entries=1)[0]
+        c2, _ = scene.read_phase2(v.slot)
+        rejected_instances += int(c1[1])
+        added += int(c2[2]) - int(c1[2])
+    out = {"views_two_phase": len(two), "cut_ms_with_recording": t_cut / steps * 1e3, "phase2_ms": t_phase2 / steps * 1e3,
+           "instances_rejected": rejected_instances, "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added}
+    log(f"two-phase: cut {out['cut_ms_with_recording']:.2f} ms with recording, second phase {out['phase2_ms']:.2f} ms "
+        f"({rejected_instances} instances and {rejected_meshlets} meshlets re-tested, {added} clusters added)")
+    for v in views:
+        scene.set_state(v.slot, initial)
+    return out
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
@@ -605,6 +663,7 @@ def main():
         log(f"check: {bad} of {views_n} views differ from the oracle; {overflowing} overflow the draw list "
             f"(their draw order, hence firstIndex and per-mesh counts, is not determined)")
 
+    two_phase = two_phase_bench(scene, vd, args.steps)
     expand = None if args.no_expand else expand_bench(spec, vd, args.steps, args.warmup, batched_schedule)
 
     value = tested / batched_s
@@ -640,6 +699,7 @@ def main():
                      "algorithmic_bytes": dom_bytes,
                      "launch_algorithmic_bytes": launch_bytes, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
+        "two_phase_occlusion": two_phase,
         "expand": expand,
         "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES,
                                                                                args.serve_seconds),
