@@ -502,8 +502,8 @@ def main():
         batched_schedule = plain_schedule = fixed
     else:
         t0 = time.time()
-        plain_schedule = scene.tune([views[0]], repeats=5)[0].as_tuple()
-        batched_schedule = scene.tune(views, repeats=5)[0].as_tuple()
+        plain_schedule = scene.tune([views[0]], repeats=9)[0].as_tuple()
+        batched_schedule = scene.tune(views, repeats=9)[0].as_tuple()
         log(f"tuned schedules: batched {batched_schedule}, plain {plain_schedule} ({time.time() - t0:.1f} s)")
 
     # Warm-up (uploads, graph capture), then the work counts of one batched step.

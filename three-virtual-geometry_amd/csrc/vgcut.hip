@@ -1414,31 +1414,36 @@ int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats,
   vg_schedule chosen = start;
   float chosen_ms = 0;
   rc = time_it(chosen, &chosen_ms);
+  // Coordinate descent, two sweeps: the meshlet pass's occupancy variant and grid, then the instance
+  // pass's grid. The incumbent is timed again in each sweep, so one lucky timing does not stick.
   static const uint32_t kMeshlet[][2] = {{4, 4}, {4, 8}, {5, 5}, {5, 10}, {6, 6}, {6, 12}};
-  for (const auto& m : kMeshlet) {
-    if (rc) break;
-    vg_schedule c = chosen;
-    c.meshlet_waves_per_simd = m[0];
-    c.meshlet_blocks_per_cu = m[1];
-    if (schedule_key(c) == schedule_key(chosen)) continue;
-    float ms = 0;
-    rc = time_it(c, &ms);
-    if (!rc && ms < chosen_ms) {
-      chosen = c;
-      chosen_ms = ms;
-    }
-  }
   static const uint32_t kInstance[] = {4, 6, 7, 8, 12};
-  for (uint32_t b : kInstance) {
-    if (rc) break;
-    vg_schedule c = chosen;
-    c.instance_blocks_per_cu = b;
-    if (schedule_key(c) == schedule_key(chosen)) continue;
-    float ms = 0;
-    rc = time_it(c, &ms);
-    if (!rc && ms < chosen_ms) {
-      chosen = c;
-      chosen_ms = ms;
+  for (int sweep = 0; sweep < 2 && !rc; sweep++) {
+    if (sweep) rc = time_it(chosen, &chosen_ms);
+    for (const auto& m : kMeshlet) {
+      if (rc) break;
+      vg_schedule c = chosen;
+      c.meshlet_waves_per_simd = m[0];
+      c.meshlet_blocks_per_cu = m[1];
+      if (schedule_key(c) == schedule_key(chosen)) continue;
+      float ms = 0;
+      rc = time_it(c, &ms);
+      if (!rc && ms < chosen_ms) {
+        chosen = c;
+        chosen_ms = ms;
+      }
+    }
+    for (uint32_t b : kInstance) {
+      if (rc) break;
+      vg_schedule c = chosen;
+      c.instance_blocks_per_cu = b;
+      if (schedule_key(c) == schedule_key(chosen)) continue;
+      float ms = 0;
+      rc = time_it(c, &ms);
+      if (!rc && ms < chosen_ms) {
+        chosen = c;
+        chosen_ms = ms;
+      }
     }
   }
   const int restored = restore();
