@@ -87,6 +87,7 @@ struct ViewDev {
   uint32_t redo[2];  // retry flags (S_RETRY > 0.5) of the current round
   uint32_t budget_bin;
   float budget_scale, budget_low, budget_high;  // camera scale and band of the budget histogram
+  float budget_inv;  // (bins - 1) / ln(T[B-1] / T[0]): estimates a value's bin in the log-spaced table
 };
 
 struct Params {

@@ -215,6 +215,8 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
       v.budget_scale = get_state(state, kStateScale);
       v.budget_low = t[0] * v.budget_scale;
       v.budget_high = t[p.budget_bins - 1] * v.budget_scale;
+      const float ratio = t[p.budget_bins - 1] / t[0];
+      v.budget_inv = p.budget_bins > 1 && ratio > 1.0f ? static_cast<float>(p.budget_bins - 1) / logf(ratio) : 0.0f;
     }
   }
   if (v.budget)
@@ -886,14 +888,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
 // bins [lower_bound(own), lower_bound(parent)) of the ascending table T. The smallest T[b] whose total
 // fits the budget becomes the view's error threshold (the largest when none fits).
 
-__device__ __forceinline__ uint32_t bin_of(const float* __restrict__ t, uint32_t bins, float scale, float x) {
-  uint32_t lo = 0, hi = bins;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (t[mid] * scale >= x) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
+// lower_bound over the scaled table: the first b with t[b] * scale >= x (bins when none, and for NaN).
+// The table is log-spaced, so the log of x estimates b; stepping against the table itself from there
+// gives exactly the binary search's answer (t[b] * scale does not decrease with b).
+__device__ __forceinline__ uint32_t bin_of(const float* __restrict__ t, uint32_t bins, float scale, float inv, float x) {
+  if (x != x || x > t[bins - 1] * scale) return bins;
+  if (!(x > t[0] * scale)) return 0;
+  const float u = __logf(x / (t[0] * scale)) * inv;
+  int b = u > 0.0f ? static_cast<int>(ceilf(fminf(u, static_cast<float>(bins - 1)))) : 1;
+  b = b < 1 ? 1 : b;
+  while (b > 0 && t[b - 1] * scale >= x) b--;
+  while (b < static_cast<int>(bins) && t[b] * scale < x) b++;
+  return static_cast<uint32_t>(b);
 }
 
 __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
@@ -954,7 +960,9 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       if (__ballot(keep)) lr = select_band(p, level_base, level_count, v.budget_low, v.budget_high, pmin, pmax);
       const bool work = keep && lr.last > lr.first;
       if (!__ballot(work)) continue;
-      const uint32_t len = work ? lr.last - lr.first : 0u;
+      // The item covers the LOD runs of [first, last) (level ranges hold whole runs).
+      const uint32_t run_lo = work ? p.m_run[lr.first] : 0u, run_hi = work ? p.m_run[lr.last - 1] + 1 : 0u;
+      const uint32_t len = run_hi - run_lo;
       const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
       const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
       const uint32_t task_incl = wave_scan(len);
@@ -971,11 +979,11 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       if (work && item_base + rank < p.work_capacity) {
         const uint32_t tag = slot | (occlusion_on ? kTagOcclusion : 0u) | (vi << kTagViewShift);
         const size_t g = part_base + item_base + rank;
-        p.work[g] = make_uint4(instance, lr.first, lr.last, tag);
+        p.work[g] = make_uint4(instance, run_lo, run_hi, tag);
         p.work_start[g] = task_base + task_incl - len;
         p.work_scale[g] = scale;
       }
-      wave_count_add(metric_at(p, v.slot, M_BUDGET_TESTS), len, work);
+      wave_count_add(metric_at(p, v.slot, M_BUDGET_TESTS), work ? lr.last - lr.first : 0u, work);
     }
   }
   if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
@@ -999,32 +1007,42 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
       }
     }
   };
-  for_each_task(p, [&](bool valid, uint32_t, const uint4& w, uint32_t meshlet) {
+  // Per LOD run: its own and parent errors give the bins [a, e) of the thresholds where its meshlets are
+  // in the plain cut; each meshlet of a run with a < e that is visible adds its triangles there.
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t run) {
     const uint32_t vi = w.w >> kTagViewShift;
     const bool occ_on = (w.w & kTagOcclusion) != 0;
-    const ViewDev& v = p.views[vi];
     uint32_t a = 0, e = 0, tris = 0;
-    if (valid) {
-      const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
-      const float4 cs = p.m_cull[meshlet];
-      const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
-      bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -(cs.w * scale);
-      if (visible && occ_on && v.hzb_valid) {
-        const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
-        const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
-        visible = !occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, cs.w * scale + margin);
+    for (uint64_t pending = __ballot(valid); pending;) {  // one launched view at a time (uniform constants)
+      const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+      const bool mine = valid && vi == vu;
+      if (mine) {
+        const auto& v = scalar(p.views)[uniform(vu)];
+        const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
+        const float scale = p.work_scale[item];
+        const float2 err = p.r_err[run];
+        const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
+        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
+        const float* t = p.budget_thresholds + static_cast<size_t>(uniform(vu)) * bins;
+        a = bin_of(t, bins, v.budget_scale, v.budget_inv, own);
+        e = bin_of(t, bins, v.budget_scale, v.budget_inv, parent);
+        if (a < e) {
+          const uint2 span = p.r_span[run];
+          for (uint32_t k = 0; k < span.y; k++) {
+            const uint32_t meshlet = span.x + k;
+            const float4 cs = p.m_cull[meshlet];
+            const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+            bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -(cs.w * scale);
+            if (visible && occ_on && v.hzb_valid) {
+              const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
+              const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
+              visible = !occluded(v, scalar(p.hzbs)[v.slot], vc.x, vc.y, vc.z, cs.w * scale + margin);
+            }
+            if (visible) tris += p.m_tris[meshlet];
+          }
+        }
       }
-      if (visible) {
-        const float2 err = p.m_err[meshlet];
-        const float own = projected_error(v, c0, c1, c2, c3, scale, p.m_lod[meshlet], err.x);
-        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.m_parent[meshlet], err.y);
-        const float* t = p.budget_thresholds + static_cast<size_t>(vi) * bins;
-        const float camera_scale = v.budget_scale;
-        a = bin_of(t, bins, camera_scale, own);
-        e = bin_of(t, bins, camera_scale, parent);
-        if (a < e) tris = p.m_tris[meshlet];
-      }
+      pending &= ~__ballot(mine);
     }
     const bool add = tris != 0;
     // Switch the wave's LDS histogram to the chunk's first contributing view when needed.
