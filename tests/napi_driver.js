@@ -25,19 +25,39 @@ for (const m of spec.meshes) {
 }
 out.pool = [0, 1, 2, 3, 4].map((b) => hex(vg.poolRead(scene, b)));
 const finish = () => fs.writeFileSync(outPath, JSON.stringify(out));
+// Depth images as {slot, width, height, depth: hex float32}: the HZB of each slot for a cut.
+const setHzbs = (list) => {
+  for (const h of list || []) {
+    const b = Buffer.from(h.depth, 'hex');
+    vg.setHzb(scene, h.slot, new Float32Array(b.buffer, b.byteOffset, b.length / 4), h.width, h.height);
+  }
+};
+const readSlots = () => spec.viewSlots.map((slot) => {
+  const counters = new Uint32Array(vg.counterWords(ctx));
+  const args = new Uint32Array(vg.drawArgsWords(ctx));
+  const list = new Uint32Array(8 * spec.maxEntries);
+  vg.read(scene, slot, counters, args, list);
+  return { counters: hex(counters.buffer), args: hex(args.buffer), list: hex(list.buffer) };
+});
 if (!spec.views) {
   finish();
 } else {
   const buf = Buffer.from(spec.views, 'hex');
   const views = buf.buffer.slice(buf.byteOffset, buf.byteOffset + buf.length);
+  setHzbs(spec.hzb1);
   vg.cut(scene, views, spec.viewCount).then(() => {
-    out.results = spec.viewSlots.map((slot) => {
-      const counters = new Uint32Array(vg.counterWords(ctx));
-      const args = new Uint32Array(vg.drawArgsWords(ctx));
-      const list = new Uint32Array(8 * spec.maxEntries);
-      vg.read(scene, slot, counters, args, list);
-      return { counters: hex(counters.buffer), args: hex(args.buffer), list: hex(list.buffer) };
+    out.results = readSlots();
+    if (!spec.hzb2) return null;
+    // two-phase occlusion: the new depth, then the second phase of the same views
+    setHzbs(spec.hzb2);
+    return vg.cutPhase2(scene, views, spec.viewCount).then(() => {
+      out.phase2 = spec.viewSlots.map((slot) => {
+        const counters = new Uint32Array(vg.counterWords(ctx));
+        const args = new Uint32Array(vg.drawArgsWords(ctx));
+        vg.readPhase2(scene, slot, counters, args);
+        return { counters: hex(counters.buffer), args: hex(args.buffer) };
+      });
+      out.after = readSlots();
     });
-    finish();
-  }).catch((e) => { out.error = e.message; finish(); process.exitCode = 3; });
+  }).then(() => finish()).catch((e) => { out.error = e.message; finish(); process.exitCode = 3; });
 }

@@ -28,20 +28,20 @@ needs_addon = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
 FLAGS = {"maxDrawDistance": 1e30, "minPixelRadius": 0.0, "enabled": True, "shadow": False, "occlusion": False}
 
 
-def mesh_json(data, matrices):
+def mesh_json(data, matrices, occlusion=False):
     return {k: np.asarray(getattr(data, k)).ravel().tolist() for k in
             ["meshletBounds", "meshletInfo", "levelRanges", "levelErrors", "levelGuardErrors", "boundingSphere",
              "lodBoundsSphere", "cullBoundsSphere"]} | {"matrices": np.asarray(matrices, dtype=np.float32).ravel().tolist(),
-                                                        "flags": FLAGS}
+                                                        "flags": FLAGS | {"occlusion": occlusion}}
 
 
-def load_python(scene, spec):
+def load_python(scene, spec, occlusion=False):
     for d, m in zip(spec.meshes, spec.matrices):
         if len(m):
             slot = scene.add_mesh(d)
             scene.set_instances(slot, m)
             scene.set_flags(slot, FLAGS["maxDrawDistance"], FLAGS["minPixelRadius"], FLAGS["enabled"], FLAGS["shadow"],
-                            FLAGS["occlusion"])
+                            occlusion)
 
 
 def run_node(tmp_path, spec):
@@ -105,3 +105,57 @@ def test_node_host_cut_equals_python_host(tmp_path, gpu_available):
         for cut in (0, 1):
             n = min(int(counters[2 + 5 * cut]), max_meshlets)
             assert np.array_equal(draw_set(node_draws[cut], n), draw_set(draws[cut], n)), f"slot {v.slot} cut {cut}"
+
+
+@pytest.mark.gpu
+@needs_addon
+def test_node_host_two_phase_equals_python_host(tmp_path, gpu_available):
+    """Two-phase occlusion driven from Node (setHzb, cut, setHzb, cutPhase2, readPhase2) equals the same
+    frame driven from Python."""
+    from vgcut import camera as cam
+    from vgcut import scenegen
+
+    spec = small_ruins(seed=72, instances=2000, meshes=3)
+    max_meshlets = 1 << 15
+    cams = [ground_camera((0.0, 2.0, -70.0), (0.0, 0.0, 60.0)), ground_camera((50.0, 2.0, 10.0), (-60.0, 2.0, 0.0))]
+    views = [make_view(c, 540, slot=i, occlusion=2) for i, c in enumerate(cams)]
+    depths = {}
+    for phase, seed in ((1, 80), (2, 90)):
+        for i, c in enumerate(cams):
+            vp = cam.mat_mul(c.projectionMatrix, c.matrixWorldInverse)
+            depths[(phase, i)] = scenegen.heightfield_depth(vp, 480, 270, (0.0, 0.0), 120.0, 128, 6.0, 1 / 25.0, seed + i)
+    hzb = lambda phase: [{"slot": i, "width": 480, "height": 270, "depth": depths[(phase, i)].astype(np.float32).tobytes().hex()}
+                         for i in range(len(cams))]
+    packed = b"".join(ctypes.string_at(ctypes.addressof(v), ctypes.sizeof(v)) for v in views)
+    config = {"maxViews": 2, "meshSlots": 8, "maxMeshlets": max_meshlets, "maxTriangles": 20_000_000,
+              "maxShadowTriangles": 10_000_000, "device": 0}
+    got = run_node(tmp_path, {"config": config,
+                              "meshes": [mesh_json(d, m, occlusion=True) for d, m in zip(spec.meshes, spec.matrices) if len(m)],
+                              "views": packed.hex(), "viewCount": len(views), "viewSlots": [0, 1], "maxEntries": max_meshlets,
+                              "hzb1": hzb(1), "hzb2": hzb(2)})
+    assert "error" not in got, got.get("error")
+    ctx = native.Context(max_triangles=20_000_000, max_shadow_triangles=10_000_000, max_meshlets=max_meshlets, max_views=2,
+                         mesh_slots=8, device=0)
+    scene = native.Scene(ctx)
+    load_python(scene, spec, occlusion=True)
+    for i in range(len(cams)):
+        scene.set_hzb(i, depths[(1, i)])
+    scene.cut(views)
+    scene.sync()
+    first = [scene.read(v.slot) for v in views]
+    for i in range(len(cams)):
+        scene.set_hzb(i, depths[(2, i)])
+    scene.cut_phase2(views)
+    scene.sync()
+    added = 0
+    for v, f, res, p2, after in zip(views, first, got["results"], got["phase2"], got["after"]):
+        assert bytes.fromhex(res["counters"]) == f[0].tobytes(), f"slot {v.slot} first-phase counters"
+        c2, a2 = scene.read_phase2(v.slot)
+        assert bytes.fromhex(p2["counters"]) == c2.tobytes(), f"slot {v.slot} second-phase counters"
+        assert bytes.fromhex(p2["args"]) == a2.tobytes(), f"slot {v.slot} second-phase args"
+        draws = scene.read(v.slot)[2]
+        node_draws = np.frombuffer(bytes.fromhex(after["list"]), dtype=np.uint32).reshape(2, max_meshlets, 4)
+        n1, n2 = int(f[0][2]), min(int(c2[2]), max_meshlets)
+        assert np.array_equal(draw_set(node_draws[0][n1:], n2 - n1), draw_set(draws[0][n1:], n2 - n1)), f"slot {v.slot} second phase"
+        added += n2 - n1
+    assert added > 0

@@ -169,17 +169,41 @@ static napi_value SetFlags(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// cut(scene, views: ArrayBuffer of packed vg_view records, count) -> Promise<void>; the wait runs off
-// the JS thread (napi_async_work), so the render loop keeps going while the GPU cuts.
+// setHzb(scene, viewSlot, depth: Float32Array, width, height): the slot's HZB from a depth image (row 0 at
+// the top), as OcclusionCulling's copy and reduce nodes build it
+static napi_value SetHzb(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0, width = 0, height = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[3], &width));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[4], &height));
+  auto* s = static_cast<Scene*>(p);
+  size_t n = 0;
+  const float* depth = typed<float>(env, argv[2], &n);
+  if (!depth || n < static_cast<size_t>(width) * height) {
+    napi_throw_range_error(env, nullptr, "vgcut: depth array smaller than width x height");
+    return nullptr;
+  }
+  if (int rc = vg_hzb_set(s->scene, slot, depth, width, height)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
+// The cut (and the second phase) run off the JS thread as napi_async_work and resolve a promise, so the
+// render loop keeps going while the GPU cuts.
 struct CutWork {
   Scene* s;
   std::vector<vg_view> views;
+  bool phase2 = false;  // vg_cut_phase2 instead of the cut
   int rc = 0;
   napi_deferred deferred;
   napi_async_work work;
 };
 
-static napi_value Cut(napi_env env, napi_callback_info info) {
+static napi_value QueueCut(napi_env env, napi_callback_info info, bool phase2) {
   size_t argc = 3;
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
@@ -195,6 +219,7 @@ static napi_value Cut(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   auto* w = new CutWork{static_cast<Scene*>(p), std::vector<vg_view>(count)};
+  w->phase2 = phase2;
   memcpy(w->views.data(), data, sizeof(vg_view) * count);
   napi_value promise, name;
   CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
@@ -203,7 +228,8 @@ static napi_value Cut(napi_env env, napi_callback_info info) {
       env, nullptr, name,
       [](napi_env, void* d) {
         auto* w = static_cast<CutWork*>(d);
-        w->rc = vg_cut_graph(w->s->scene, w->views.data(), static_cast<uint32_t>(w->views.size()));
+        const uint32_t n = static_cast<uint32_t>(w->views.size());
+        w->rc = w->phase2 ? vg_cut_phase2(w->s->scene, w->views.data(), n) : vg_cut_graph(w->s->scene, w->views.data(), n);
         if (!w->rc) w->rc = vg_sync(w->s->scene);
       },
       [](napi_env env, napi_status, void* d) {
@@ -224,6 +250,34 @@ static napi_value Cut(napi_env env, napi_callback_info info) {
       w, &w->work));
   CHECK_NAPI(napi_queue_async_work(env, w->work));
   return promise;
+}
+
+// cut(scene, views: ArrayBuffer of packed vg_view records, count) -> Promise (VirtualGeometry.update's compute)
+static napi_value Cut(napi_env env, napi_callback_info info) { return QueueCut(env, info, false); }
+
+// cutPhase2(scene, views, count) -> Promise: the second phase of two-phase occlusion (views with occlusion 2,
+// after setting each slot's new HZB)
+static napi_value CutPhase2(napi_env env, napi_callback_info info) { return QueueCut(env, info, true); }
+
+// readPhase2(scene, viewSlot, counters: Uint32Array, drawArgs: Uint32Array): the second phase's counters and args
+static napi_value ReadPhase2(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  uint32_t slot = 0;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  auto* s = static_cast<Scene*>(p);
+  size_t nc = 0, na = 0;
+  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc);
+  uint32_t* args = typed<uint32_t>(env, argv[3], &na);
+  if (!counters || !args || nc < vg_counter_words(s->ctx) || na < vg_draw_args_words(s->ctx)) {
+    napi_throw_range_error(env, nullptr, "vgcut: counters or drawArgs array too small");
+    return nullptr;
+  }
+  if (int rc = vg_read_phase2(s->scene, slot, counters, args)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
 }
 
 // read(scene, viewSlot, counters: Uint32Array, drawArgs: Uint32Array, drawList: Uint32Array) (readCounters)
@@ -323,6 +377,9 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"read", nullptr, Read, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"poolRead", nullptr, PoolRead, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"readIndices", nullptr, ReadIndices, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cutPhase2", nullptr, CutPhase2, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setHzb", nullptr, SetHzb, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readPhase2", nullptr, ReadPhase2, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"counterWords", nullptr, CounterWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"drawArgsWords", nullptr, DrawArgsWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
