@@ -98,7 +98,10 @@ def test_many_instances_match_fp64(gpu_free_ruins):
         for t in (0.5, 1.0, 2.0, 3.0):
             r = oracle.cut(pool, make_view(camera, 1080, error_threshold=t, shadow_sweep=(10.0, -30.0, 5.0)), fp64=True)
             assert r.stats.mismatch_hard == 0
-            assert r.stats.near_threshold <= 1e-3 * max(1, r.stats.lod_checked)
+            # The flag covers the fade-step boundaries too: each drawn meshlet's two step values fall within
+            # 1e-6 * fadeScale of a half-step with probability 2 * 2e-6 * 255 / ln(2) ~ 1.5e-3, and siblings
+            # that share their LOD inputs are flagged together.
+            assert r.stats.near_threshold <= 4e-3 * max(1, r.stats.lod_checked)
             checked += r.stats.lod_checked
     assert checked > 100_000
 
@@ -303,10 +306,19 @@ def test_planted_mistakes_are_caught(terrain_pool, gpu_free_ruins):
     sel = np.zeros(d.meshletCount, dtype=np.uint8)
     sel[good.draws[0, :n, 1]] = 1
     assert not cut64.verify_cut_coverage(d, sel)
-    # verifyCutCoverage works per group (any output of a replacing group covers), so drop a whole group:
-    # every selected meshlet sharing the first one's LOD sphere and error.
-    B = d.meshletBounds.reshape(-1, 16)
-    first = int(good.draws[0, 0, 1])
-    same = np.all(B[:, [0, 1, 2, 3, 8]] == B[first, [0, 1, 2, 3, 8]], axis=1)
-    sel[same] = 0
-    assert cut64.verify_cut_coverage(d, sel)
+    # verifyCutCoverage works per group (any output of a replacing group covers) and a group's outputs may
+    # merge into different coarser groups, so one dropped meshlet can leave every leaf covered through a
+    # sibling. Drop every selected meshlet on leaf 0's paths to the coarsest level: leaf 0 must be reported.
+    group, start, lst = d.replacementGroup, d.replacementStart, d.replacementIndices
+    reach, todo = set(), [0]
+    while todo:
+        i = todo.pop()
+        if i in reach:
+            continue
+        reach.add(i)
+        if group[i] >= 0:
+            todo.extend(int(k) for k in lst[start[group[i]]:start[group[i] + 1]])
+    dropped = [i for i in reach if sel[i]]
+    assert dropped
+    sel[dropped] = 0
+    assert 0 in cut64.verify_cut_coverage(d, sel)

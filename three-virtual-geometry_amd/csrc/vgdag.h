@@ -1,0 +1,92 @@
+/*
+ * Meshlet LOD DAG builder for arbitrary indexed triangle meshes: the reference's `buildVirtualMesh`
+ * (src/core/preprocess/buildVirtualMesh.ts:185-601) with its partitioner (partition.ts:16-136),
+ * written without meshoptimizer. Host-only C++ (vgdag.cpp).
+ *
+ * Per level: partition the current meshlets into groups of ~group_size that share boundary edges,
+ * merge each group, simplify it to simplify_ratio of its triangles with the group border locked
+ * (quadric half-edge collapses onto existing vertices, so every level indexes the input vertex
+ * buffer), split the result back into meshlets of <= 128 triangles and <= 128 vertices. Groups that
+ * keep more than max_group_keep_ratio of their triangles stay roots. Once the level is small
+ * (<= 1024 triangles) or stalls, the tail merges every root and keeps halving with borders free.
+ *
+ * Not built: the reference's voxel proxies and piece-by-piece aggregate simplification
+ * (voxelProxy.ts, aggregate.ts; options voxelLods, aggregateLods), which serve foliage made of
+ * separate small pieces. The builder behaves as the reference does with both switched off.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* buildVirtualMesh.ts:36-73 VirtualMeshBuildOptions and its DEFAULTS (buildVirtualMesh.ts:161-173). */
+typedef struct vgd_options {
+  uint32_t group_size;          /* meshlets per group before simplification (12) */
+  uint32_t max_lod_levels;      /* (24) */
+  double max_group_keep_ratio;  /* a group keeping more than this fraction of its triangles stays roots (0.9) */
+  double simplify_ratio;        /* fraction of a group's triangles its simplification keeps (0.5) */
+  double min_level_reduction;   /* stop when a level removed less than this fraction (0.03) */
+  int32_t prune;                /* remove small disconnected components while simplifying (0) */
+  int32_t tail_simplify;        /* keep halving below one meshlet once the roots cover the mesh (1) */
+  uint32_t min_root_triangles;  /* the tail stops at this many triangles (4) */
+  uint32_t threads;             /* worker threads for the groups of a level; 0: hardware concurrency */
+} vgd_options;
+
+void vgd_default_options(vgd_options* options);
+
+/* The built mesh: buildVirtualMesh.ts:75-138 VirtualMeshData (normals, colors, uvs are the caller's). */
+typedef struct vgd_mesh_view {
+  uint32_t vertex_count;
+  const float* positions;            /* 3 per vertex: the input positions */
+  uint64_t index_count;
+  const uint32_t* indices;           /* concatenated meshlet triangle lists, indexing positions */
+  uint32_t meshlet_count;
+  const float* meshlet_bounds;       /* 16 per meshlet: lod sphere, parent sphere, (error, parent error, 0, 0), cull sphere */
+  const uint32_t* meshlet_info;      /* 4 per meshlet: triangle count, first index, lod level, vertex offset */
+  uint64_t meshlet_vertex_count;
+  const uint32_t* meshlet_vertices;  /* global vertex ids per meshlet, in meshlet-local order */
+  const uint32_t* meshlet_triangles; /* per triangle: local ids a | b << 8 | c << 16 */
+  uint32_t level_count;
+  const uint32_t* level_ranges;      /* 2 per level: first meshlet, count */
+  const float* level_errors;         /* 2 per level: smallest own error, largest parent error */
+  const float* level_guard_errors;   /* 2 per level: largest own error, smallest parent error */
+  const int32_t* replacement_group;  /* per meshlet: its group, -1 for roots */
+  uint32_t group_count;
+  const uint32_t* replacement_start; /* group_count + 1 */
+  const uint32_t* replacement_indices;
+  float bounding_sphere[4];
+  float lod_bounds_sphere[4];
+  float cull_bounds_sphere[4];
+  uint64_t leaf_triangles;
+  uint32_t leaf_meshlets;
+  uint32_t root_triangles;
+  uint32_t root_meshlets;
+  uint32_t lod_levels;               /* stats.lodLevels */
+} vgd_mesh_view;
+
+typedef struct vgd_mesh vgd_mesh;
+
+/*
+ * Builds the DAG of an indexed triangle mesh. Returns 0, or -1 for invalid input (index_count not a
+ * multiple of 3, an index >= vertex_count, vertex_count >= 2^26, no triangles), -2 for an internal
+ * failure. `options` may be NULL (defaults).
+ */
+int vgd_build(const float* positions, uint32_t vertex_count, const uint32_t* indices, uint64_t index_count,
+              const vgd_options* options, vgd_mesh** out);
+void vgd_mesh_get(const vgd_mesh* mesh, vgd_mesh_view* view);
+void vgd_mesh_free(vgd_mesh* mesh);
+
+/*
+ * partitionMeshlets (partition.ts:16-97) on its own: meshlet i has edge_count[i] boundary edge keys at
+ * edges + edge_offset[i] and a center at centers + 3 i. Writes each meshlet's group to group_of and
+ * returns the number of groups; groups are numbered in the order the reference returns them, and a
+ * group lists its meshlets in insertion order at members (ordered by group, then insertion).
+ */
+uint32_t vgd_partition(uint32_t count, const uint64_t* edges, const uint32_t* edge_offset, const uint32_t* edge_count,
+                       const double* centers, uint32_t group_size, uint32_t* group_of, uint32_t* members);
+
+#ifdef __cplusplus
+}
+#endif

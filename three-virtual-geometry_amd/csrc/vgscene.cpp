@@ -22,10 +22,13 @@
 #include "vgscene.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <cstdlib>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -718,6 +721,217 @@ extern "C" void vgs_mesh_get(const vgs_mesh* m, vgs_mesh_view* v) {
 }
 
 extern "C" void vgs_mesh_free(vgs_mesh* m) { delete m; }
+
+// ------------------------------------------------------------------ indexed triangle meshes
+
+namespace {
+
+// Characteristic size the noise amplitude is a fraction of (as the patch displacements use it).
+double kind_scale(const vgs_mesh_params& m) {
+  switch (m.kind) {
+    case VGS_SPHERE: return m.size[0];
+    case VGS_BOX: return std::max(m.size[0], std::max(m.size[1], m.size[2]));
+    case VGS_COLUMN: return std::max(m.size[0], m.size[1] * 0.5);
+    case VGS_ARCH: return std::max(m.size[0], m.size[1]);
+    default: return 1.0;
+  }
+}
+
+// Merges the points flagged in `candidate` that lie closer than `eps` to an earlier flagged point (grid
+// hash over cells of side eps, 27-cell search); other points keep their own ids. Returns the new id of
+// every input point and compacts `pts`.
+std::vector<uint32_t> weld_points(std::vector<V3>& pts, const std::vector<uint8_t>& candidate, double eps) {
+  struct CellHash {
+    size_t operator()(const std::array<int64_t, 3>& c) const {
+      return static_cast<size_t>((c[0] * 73856093) ^ (c[1] * 19349663) ^ (c[2] * 83492791));
+    }
+  };
+  std::unordered_map<std::array<int64_t, 3>, std::vector<uint32_t>, CellHash> cells;
+  std::vector<uint32_t> id(pts.size());
+  std::vector<V3> out;
+  out.reserve(pts.size());
+  for (size_t i = 0; i < pts.size(); i++) {
+    const V3 p = pts[i];
+    if (!candidate[i]) {
+      id[i] = static_cast<uint32_t>(out.size());
+      out.push_back(p);
+      continue;
+    }
+    const std::array<int64_t, 3> c{static_cast<int64_t>(std::floor(p.x / eps)), static_cast<int64_t>(std::floor(p.y / eps)),
+                                   static_cast<int64_t>(std::floor(p.z / eps))};
+    int64_t found = -1;
+    for (int dx = -1; dx <= 1 && found < 0; dx++)
+      for (int dy = -1; dy <= 1 && found < 0; dy++)
+        for (int dz = -1; dz <= 1 && found < 0; dz++) {
+          auto it = cells.find({c[0] + dx, c[1] + dy, c[2] + dz});
+          if (it == cells.end()) continue;
+          for (uint32_t q : it->second)
+            if (len(out[q] - p) <= eps) {
+              found = q;
+              break;
+            }
+        }
+    if (found < 0) {
+      found = static_cast<int64_t>(out.size());
+      out.push_back(p);
+      cells[c].push_back(static_cast<uint32_t>(found));
+    }
+    id[i] = static_cast<uint32_t>(found);
+  }
+  pts.swap(out);
+  return id;
+}
+
+// Makes the winding consistent across edge-adjacent triangles, then turns every connected piece so
+// that its signed volume is positive (outward normals on closed pieces).
+void orient(std::vector<uint32_t>& idx, const std::vector<V3>& pts) {
+  const size_t nt = idx.size() / 3;
+  struct E {
+    uint64_t key;
+    uint32_t tri;
+    uint8_t dir;
+  };
+  std::vector<E> edges;
+  edges.reserve(idx.size());
+  for (size_t t = 0; t < nt; t++)
+    for (int k = 0; k < 3; k++) {
+      const uint32_t a = idx[t * 3 + k], b = idx[t * 3 + (k + 1) % 3];
+      edges.push_back({a < b ? (static_cast<uint64_t>(a) << 32) | b : (static_cast<uint64_t>(b) << 32) | a,
+                       static_cast<uint32_t>(t), static_cast<uint8_t>(a < b)});
+    }
+  std::sort(edges.begin(), edges.end(), [](const E& x, const E& y) { return x.key < y.key || (x.key == y.key && x.tri < y.tri); });
+  std::vector<std::vector<std::pair<uint32_t, uint8_t>>> nb(nt);  // (neighbour, my dir ^ its dir)
+  for (size_t i = 0; i + 1 < edges.size();) {
+    size_t j = i;
+    while (j < edges.size() && edges[j].key == edges[i].key) j++;
+    if (j - i == 2) {
+      nb[edges[i].tri].push_back({edges[i + 1].tri, static_cast<uint8_t>(edges[i].dir ^ edges[i + 1].dir)});
+      nb[edges[i + 1].tri].push_back({edges[i].tri, static_cast<uint8_t>(edges[i].dir ^ edges[i + 1].dir)});
+    }
+    i = j;
+  }
+  std::vector<int8_t> flip(nt, -1);
+  std::vector<uint32_t> queue, piece;
+  for (size_t s = 0; s < nt; s++) {
+    if (flip[s] >= 0) continue;
+    flip[s] = 0;
+    queue.assign(1, static_cast<uint32_t>(s));
+    piece.clear();
+    while (!queue.empty()) {
+      const uint32_t t = queue.back();
+      queue.pop_back();
+      piece.push_back(t);
+      for (auto [u, differ] : nb[t]) {
+        // Shared edge directions must differ once flips apply: f_u = f_t ^ (d_t ^ d_u) ^ 1.
+        const int8_t want = static_cast<int8_t>(flip[t] ^ differ ^ 1);
+        if (flip[u] < 0) {
+          flip[u] = want;
+          queue.push_back(u);
+        }
+      }
+    }
+    double volume = 0;
+    for (uint32_t t : piece) {
+      V3 a = pts[idx[t * 3]], b = pts[idx[t * 3 + 1]], c = pts[idx[t * 3 + 2]];
+      if (flip[t]) std::swap(b, c);
+      volume += dot(a, V3{b.y * c.z - b.z * c.y, b.z * c.x - b.x * c.z, b.x * c.y - b.y * c.x});
+    }
+    if (volume < 0)
+      for (uint32_t t : piece) flip[t] ^= 1;
+  }
+  for (size_t t = 0; t < nt; t++)
+    if (flip[t]) std::swap(idx[t * 3 + 1], idx[t * 3 + 2]);
+}
+
+}  // namespace
+
+extern "C" int vgs_mesh_triangles(const vgs_mesh_params* params, vgs_triangles* out) {
+  std::memset(out, 0, sizeof(*out));
+  vgs_mesh_params flat = *params;
+  flat.noise_amplitude = 0.0;
+  std::vector<Patch> patches = make_patches(flat);
+  if (patches.empty()) return -1;
+  const bool column = params->kind == VGS_COLUMN, arch = params->kind == VGS_ARCH;
+  if (arch) {
+    // The feet close the tube's ends: their grids match the tube's side resolution.
+    const int n = std::max(1, patches[0].nv / 4);
+    for (size_t p = 1; p < patches.size(); p++) patches[p].nu = patches[p].nv = n;
+  }
+  std::vector<V3> pts;
+  std::vector<uint8_t> rim;  // on a patch's boundary: may coincide with another patch's point
+  std::vector<uint32_t> idx;
+  for (size_t p = 0; p < patches.size(); p++) {
+    if (column && p > 0) continue;  // the caps are fans over the side's rim (below)
+    const Patch& pa = patches[p];
+    const uint32_t base = static_cast<uint32_t>(pts.size()), row = static_cast<uint32_t>(pa.nu + 1);
+    for (int j = 0; j <= pa.nv; j++)
+      for (int i = 0; i <= pa.nu; i++) {
+        pts.push_back(pa.eval(static_cast<double>(i) / pa.nu, static_cast<double>(j) / pa.nv));
+        rim.push_back(i == 0 || j == 0 || i == pa.nu || j == pa.nv);
+      }
+    for (int j = 0; j < pa.nv; j++)
+      for (int i = 0; i < pa.nu; i++) {
+        const uint32_t v00 = base + static_cast<uint32_t>(j) * row + static_cast<uint32_t>(i);
+        const uint32_t v10 = v00 + 1, v01 = v00 + row, v11 = v01 + 1;
+        idx.insert(idx.end(), {v00, v10, v11, v00, v11, v01});
+      }
+  }
+  if (column) {
+    const Patch& side = patches[0];
+    const uint32_t row = static_cast<uint32_t>(side.nu + 1);
+    for (int top = 0; top < 2; top++) {
+      const uint32_t center = static_cast<uint32_t>(pts.size());
+      pts.push_back(V3{0, top ? params->size[1] : 0.0, 0});
+      rim.push_back(0);
+      const uint32_t ring = top ? static_cast<uint32_t>(side.nv) * row : 0;
+      for (int i = 0; i < side.nu; i++) idx.insert(idx.end(), {center, ring + i, ring + i + 1});
+    }
+  }
+  const double scale = kind_scale(*params);
+  const std::vector<uint32_t> id = weld_points(pts, rim, 1e-6 * std::max(scale, 1e-9));
+  std::vector<uint32_t> tri;
+  tri.reserve(idx.size());
+  for (size_t t = 0; t < idx.size(); t += 3) {
+    const uint32_t a = id[idx[t]], b = id[idx[t + 1]], c = id[idx[t + 2]];
+    if (a == b || b == c || a == c) continue;
+    tri.insert(tri.end(), {a, b, c});
+  }
+  orient(tri, pts);
+  if (params->kind != VGS_TERRAIN && params->noise_amplitude != 0.0) {
+    // Displacement along the area-weighted vertex normal: continuous across patch seams and edges.
+    std::vector<V3> n(pts.size(), V3{0, 0, 0});
+    for (size_t t = 0; t < tri.size(); t += 3) {
+      const V3 a = pts[tri[t]], b = pts[tri[t + 1]], c = pts[tri[t + 2]];
+      const V3 e1 = b - a, e2 = c - a;
+      const V3 f{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+      for (int k = 0; k < 3; k++) n[tri[t + k]] = n[tri[t + k]] + f;
+    }
+    const double amp = params->noise_amplitude * scale, freq = params->noise_frequency;
+    for (size_t v = 0; v < pts.size(); v++) pts[v] = pts[v] + normalize(n[v]) * (amp * fbm3(pts[v] * freq, params->seed));
+  }
+  out->vertex_count = static_cast<uint32_t>(pts.size());
+  out->index_count = tri.size();
+  out->positions = static_cast<float*>(std::malloc(pts.size() * 3 * sizeof(float)));
+  out->indices = static_cast<uint32_t*>(std::malloc(std::max<size_t>(1, tri.size()) * sizeof(uint32_t)));
+  if (!out->positions || !out->indices) {
+    vgs_triangles_free(out);
+    return -2;
+  }
+  for (size_t v = 0; v < pts.size(); v++) {
+    out->positions[v * 3] = static_cast<float>(pts[v].x);
+    out->positions[v * 3 + 1] = static_cast<float>(pts[v].y);
+    out->positions[v * 3 + 2] = static_cast<float>(pts[v].z);
+  }
+  std::memcpy(out->indices, tri.data(), tri.size() * sizeof(uint32_t));
+  return 0;
+}
+
+extern "C" void vgs_triangles_free(vgs_triangles* t) {
+  std::free(t->positions);
+  std::free(t->indices);
+  t->positions = nullptr;
+  t->indices = nullptr;
+}
 
 // ------------------------------------------------------------------ placement
 

@@ -59,6 +59,23 @@ int vgs_mesh_build(const vgs_mesh_params* params, vgs_mesh** out);
 void vgs_mesh_get(const vgs_mesh* mesh, vgs_mesh_view* view);
 void vgs_mesh_free(vgs_mesh* mesh);
 
+/*
+ * The same procedural mesh as one watertight indexed triangle mesh, the input of the DAG builder
+ * (vgdag.h): each patch is tessellated at its leaf resolution without displacement, coincident points
+ * are welded, windings are made consistent and outward, and every vertex is displaced along its
+ * area-weighted normal by amplitude * size * fbm(position * frequency). Column caps are fans over the
+ * side's rims; arch feet take the tube's side resolution, so no seam leaves a crack.
+ */
+typedef struct vgs_triangles {
+  uint32_t vertex_count;
+  float* positions;  /* 3 per vertex */
+  uint64_t index_count;
+  uint32_t* indices; /* 3 per triangle */
+} vgs_triangles;
+
+int vgs_mesh_triangles(const vgs_mesh_params* params, vgs_triangles* out);
+void vgs_triangles_free(vgs_triangles* triangles);
+
 /* Bridson Poisson-disk samples over [x0, x0+w] x [z0, z0+h]; returns the count written (<= max_points). */
 uint32_t vgs_poisson_disk(double x0, double z0, double w, double h, double radius, uint32_t seed, uint32_t candidates,
                           uint32_t max_points, float* out_xz);

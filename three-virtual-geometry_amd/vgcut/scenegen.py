@@ -30,6 +30,15 @@ class _Params(ctypes.Structure):
     ]
 
 
+class _Triangles(ctypes.Structure):
+    _fields_ = [
+        ("vertex_count", ctypes.c_uint32),
+        ("positions", ctypes.POINTER(ctypes.c_float)),
+        ("index_count", ctypes.c_uint64),
+        ("indices", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
 class _View(ctypes.Structure):
     _fields_ = [
         ("meshlet_count", ctypes.c_uint32),
@@ -68,6 +77,10 @@ def _load():
         lib.vgs_mesh_get.restype = None
         lib.vgs_mesh_free.argtypes = [ctypes.c_void_p]
         lib.vgs_mesh_free.restype = None
+        lib.vgs_mesh_triangles.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Triangles)]
+        lib.vgs_mesh_triangles.restype = ctypes.c_int
+        lib.vgs_triangles_free.argtypes = [ctypes.POINTER(_Triangles)]
+        lib.vgs_triangles_free.restype = None
         lib.vgs_poisson_disk.argtypes = [ctypes.c_double] * 5 + [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
         lib.vgs_poisson_disk.restype = ctypes.c_uint32
         lib.vgs_heightfield.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p]
@@ -84,8 +97,8 @@ class VirtualMeshData:
     """The fields of the reference's ``VirtualMeshData`` the cut reads, plus the DAG links.
 
     Mirrors src/core/preprocess/buildVirtualMesh.ts:75-138. ``meshletTriangles`` (the local vertex ids
-    the expand pass writes as indices) is produced; positions, normals and meshlet vertex lists are not:
-    nothing on the cut's path or the expand pass reads them.
+    the expand pass writes as indices) is always produced; positions, the flat index list and meshlet
+    vertex lists come from the DAG builder (``vgcut.dag``) only. Normals, colors and uvs are the caller's.
     """
 
     meshletCount: int
@@ -105,6 +118,12 @@ class VirtualMeshData:
     rootTriangles: int
     rootMeshlets: int
     meshletTriangles: Optional[np.ndarray] = None  # uint32 per triangle: local ids a | b << 8 | c << 16
+    # Produced by the DAG builder (vgcut.dag) only: the vertex buffer, the flat meshlet index lists and the
+    # per-meshlet vertex lists that meshletTriangles' local ids index (meshletInfo[3] is the offset).
+    positions: Optional[np.ndarray] = None  # float32 [vertices * 3]
+    indices: Optional[np.ndarray] = None  # uint32, meshletInfo[1] is each meshlet's first
+    meshletVertices: Optional[np.ndarray] = None  # uint32
+    lodLevels: int = 0
 
     @property
     def levelCount(self) -> int:
@@ -117,10 +136,7 @@ def _arr(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
 
 
-def build_mesh(kind: int, seed: int = 1, size=(1.0, 1.0, 1.0), noise_amplitude: float = 0.05,
-               noise_frequency: float = 4.0, resolution: int = 16, target_triangles: int = 0) -> VirtualMeshData:
-    """Builds one procedural mesh and its DAG (see csrc/vgscene.cpp for the construction)."""
-    lib = _load()
+def _params(kind, seed, size, noise_amplitude, noise_frequency, resolution, target_triangles) -> _Params:
     p = _Params()
     p.kind = kind
     p.seed = seed
@@ -130,6 +146,44 @@ def build_mesh(kind: int, seed: int = 1, size=(1.0, 1.0, 1.0), noise_amplitude: 
     p.noise_frequency = noise_frequency
     p.resolution = resolution
     p.target_triangles = target_triangles
+    return p
+
+
+def triangle_mesh(kind: int, seed: int = 1, size=(1.0, 1.0, 1.0), noise_amplitude: float = 0.05, noise_frequency: float = 4.0,
+                  resolution: int = 16, target_triangles: int = 0):
+    """The procedural mesh as one watertight indexed triangle mesh: (float32 [v, 3] positions, uint32 indices)."""
+    lib = _load()
+    p = _params(kind, seed, size, noise_amplitude, noise_frequency, resolution, target_triangles)
+    t = _Triangles()
+    rc = lib.vgs_mesh_triangles(ctypes.byref(p), ctypes.byref(t))
+    if rc != 0:
+        raise ValueError(f"vgs_mesh_triangles failed ({rc}) for kind {kind}")
+    try:
+        pos = _arr(t.positions, t.vertex_count * 3, np.float32).reshape(-1, 3)
+        idx = _arr(t.indices, int(t.index_count), np.uint32)
+    finally:
+        lib.vgs_triangles_free(ctypes.byref(t))
+    return pos, idx
+
+
+def build_mesh(kind: int, seed: int = 1, size=(1.0, 1.0, 1.0), noise_amplitude: float = 0.05,
+               noise_frequency: float = 4.0, resolution: int = 16, target_triangles: int = 0, builder: str = "dag",
+               threads: int = 0) -> VirtualMeshData:
+    """Builds one procedural mesh and its DAG.
+
+    ``builder="dag"`` tessellates the mesh (``triangle_mesh``) and runs the DAG builder (vgcut.dag, the
+    reference's buildVirtualMesh at its defaults) with ``threads`` workers; ``builder="patch"`` uses the
+    grid-patch construction of csrc/vgscene.cpp, which builds in a fraction of the time.
+    """
+    if builder == "dag":
+        from . import dag
+
+        pos, idx = triangle_mesh(kind, seed, size, noise_amplitude, noise_frequency, resolution, target_triangles)
+        return dag.build_virtual_mesh(pos, idx, threads=threads)
+    if builder != "patch":
+        raise ValueError(f"unknown builder {builder}")
+    lib = _load()
+    p = _params(kind, seed, size, noise_amplitude, noise_frequency, resolution, target_triangles)
     handle = ctypes.c_void_p()
     rc = lib.vgs_mesh_build(ctypes.byref(p), ctypes.byref(handle))
     if rc != 0:
