@@ -2,7 +2,10 @@
 
 - the ruins scene (113,106 instances of 48 props of 0.2M-5M triangles): 8 seeded poses at 2 m eye
   height, each at 1080p and 2160p, occlusion against the rasterized 30 m heightfield, the reference's
-  default capacities, all 16 views in one launch;
+  default triangle capacities (so the capacity retries run) with a draw list of 8M entries, all 16 views
+  in one launch. No round may overflow the draw list: an overflowed list keeps the entries appended
+  first, so the requested-triangle counts and the retry's scale depend on the append order (in the
+  reference too);
 - the 3M-instance world: every other frame of the 16-frame seeded fly-through (8 frames; the oracle's
   budget histogram takes seconds per frame) at 1080p with an 8M-triangle budget (histogram pick; this
   world's cut exceeds 8M triangles even at 3 px, so the pick is clamped to the last threshold of
@@ -16,7 +19,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import compare_cut
+from helpers import compare_cut, state_order_dependent
 from oracle import oracle
 from vgcut import camera as cam
 from vgcut import native, scenegen, scenes
@@ -39,7 +42,7 @@ def terrain_depth(spec, camera, w, h, extent):
 def test_ruins_full_size_16_views(gpu_available):
     spec = scenes.ruins()
     assert spec.instance_count == 113_106 and len(spec.meshes) == 48
-    ctx = native.Context(max_views=16, device=0)  # 10M / 4M triangles, 1M meshlets: the reference's defaults
+    ctx = native.Context(max_meshlets=8 << 20, max_views=16, device=0)  # 10M / 4M triangles: the reference's defaults
     scene = native.Scene(ctx)
     scenes.load_into(scene, spec, occlusion=True)
     pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, ctx.config.max_triangles, ctx.config.max_shadow_triangles)
@@ -63,6 +66,7 @@ def test_ruins_full_size_16_views(gpu_available):
     for i, (v, hzb) in enumerate(zip(views, hzbs)):
         ref = oracle.cut(pool, v, hzb=hzb, fp64=i < 2)
         assert ref.stats.mismatch_hard == 0
+        assert not state_order_dependent(ref), f"ruins view {v.slot}: the draw list overflowed"
         compare_cut(scene.read(v.slot), ref, ctx.config.max_meshlets, f"ruins view {v.slot}")
         occluded += int(ref.counters[1])
     assert occluded > 0
@@ -71,7 +75,7 @@ def test_ruins_full_size_16_views(gpu_available):
 def test_world_flythrough_budget_two_phase(gpu_available):
     spec = scenes.world()
     assert spec.instance_count == 3_000_000
-    ctx = native.Context(max_triangles=20_000_000, max_shadow_triangles=10_000_000, max_meshlets=1 << 20, max_views=1, device=0)
+    ctx = native.Context(max_triangles=20_000_000, max_shadow_triangles=10_000_000, max_meshlets=4 << 20, max_views=1, device=0)
     scene = native.Scene(ctx)
     scenes.load_into(scene, spec, occlusion=True)
     pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, ctx.config.max_triangles, ctx.config.max_shadow_triangles)
@@ -96,6 +100,7 @@ def test_world_flythrough_budget_two_phase(gpu_available):
         scene.sync()
         gpu = scene.read(0)
         assert gpu[3].budget_bin == b, (f, gpu[3].budget_bin, b)
+        assert not state_order_dependent(ref.first), f"world frame {f}: the draw list overflowed"
         compare_cut(gpu, ref.first, ctx.config.max_meshlets, f"world frame {f} first phase")
         scene.set_hzb(0, depth)
         scene.cut_phase2([view])
