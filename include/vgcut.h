@@ -69,6 +69,16 @@ typedef struct vg_mesh_desc {
    */
   const uint32_t* meshlet_triangles;
   uint64_t triangle_count;
+  /*
+   * VirtualMeshData.positions (xyz per vertex) and meshletVertices (the mesh's vertex ids per meshlet, at
+   * meshlet_info[4i + 3]; buildVirtualMesh.ts:75-138), stored as the pool's vertexData / meshletVertices
+   * (GeometryPool.ts:221-226, 386-429). Needed by the occluder raster (vg_hzb_from_cut) only, together with
+   * meshlet_triangles; may be NULL (counts 0): the mesh then draws nothing into occluder depth.
+   */
+  const float* positions;
+  uint32_t vertex_count;
+  const uint32_t* meshlet_vertices;
+  uint64_t meshlet_vertex_count;
 } vg_mesh_desc;
 
 /*
@@ -138,6 +148,20 @@ int vg_mesh_flags(vg_scene* scene, uint32_t slot, float max_draw_distance, float
 /* OcclusionCulling HZB copy + reduce (OcclusionCulling.ts:275-352) from a host depth image (row 0 at top). */
 int vg_hzb_set(vg_scene* scene, uint32_t view_slot, const float* depth, uint32_t width, uint32_t height);
 int vg_hzb_invalidate(vg_scene* scene, uint32_t view_slot);
+/*
+ * The occluder pass (OcclusionCulling.ts:4-21, 244-273) as a software raster on the device: the camera
+ * entries of the view slot's last cut (draw-list positions [0, drawn); with `with_phase2`, also what the
+ * slot's second phase appended, when it ran after that cut) are drawn depth-only with `view_proj`
+ * (column-major clip = view_proj * world, WebGPU depth in [0, 1]) into a width x height depth image
+ * (cleared to 1, row 0 at the top), and the slot's HZB is built from it, in stream order with the cut
+ * (no host round trip; vg_hzb_set builds it from a host image instead). Meshes added without positions
+ * draw nothing. Triangles are skipped when back-facing, degenerate, with a vertex in front of the near
+ * plane (clip z < 0 or w <= 0) or more than 2^20 pixels off screen: fewer occluders only make the depth
+ * farther, never hide what is visible. Coverage: pixel centres on or inside all three edges of the
+ * triangle snapped to 1/256 pixel; depth: the barycentric blend of the vertices' clip z / w, minimum kept.
+ */
+int vg_hzb_from_cut(vg_scene* scene, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height,
+                    uint32_t with_phase2);
 /* The slot's HZB buffer (OcclusionCulling hzbAttribute): `size_floats` receives its length; dst may be NULL. */
 int vg_hzb_read(vg_scene* scene, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats);
 
@@ -204,6 +228,8 @@ int vg_state_set(vg_scene* scene, uint32_t view_slot, const uint32_t* state);
 #define VG_POOL_MESHLET_BOUNDS 3 /* f32: 16 per meshlet (pool-global ids) */
 #define VG_POOL_MESHLET_INFO 4   /* u32: 4 per meshlet, firstIndex rebased to the pool */
 #define VG_POOL_MESHLET_TRIANGLES 5 /* u32: 1 per triangle (packed local ids), at firstIndex / 3 */
+#define VG_POOL_VERTICES 6       /* f32: 4 per vertex (x, y, z, 0), pool-global ids */
+#define VG_POOL_MESHLET_VERTICES 7 /* u32: pool-global vertex ids; meshletInfo[3] is each meshlet's offset */
 int vg_scene_pool_read(const vg_scene* scene, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes);
 /* Highest used mesh slot + 1 (GeometryPool.meshSlots). */
 uint32_t vg_scene_mesh_slots(const vg_scene* scene);

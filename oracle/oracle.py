@@ -107,6 +107,10 @@ def load():
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint32, ctypes.POINTER(OrStats), ctypes.POINTER(OrStats), ctypes.c_void_p]
         lib.or_cut_two_phase.restype = ctypes.c_int
+        lib.or_raster.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                                  ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32,
+                                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        lib.or_raster.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -125,6 +129,8 @@ class Pool:
     max_meshlets: int
     cut_capacity: tuple
     triangles: Optional[np.ndarray] = None  # u32 meshletTriangles of the pool
+    vertices: Optional[np.ndarray] = None  # f32, 4 per vertex (x, y, z, 0)
+    meshlet_vertices: Optional[np.ndarray] = None  # u32 pool-global vertex ids
 
     def struct(self) -> OrScene:
         s = OrScene()
@@ -163,6 +169,8 @@ def pool_from_scene(scene, max_meshlets: int, max_triangles: int, max_shadow_tri
         max_meshlets=max_meshlets,
         cut_capacity=(max_triangles * 3, max_shadow_triangles * 3),
         triangles=f(native.VG_POOL_MESHLET_TRIANGLES, np.uint32),
+        vertices=f(native.VG_POOL_VERTICES, np.float32),
+        meshlet_vertices=f(native.VG_POOL_MESHLET_VERTICES, np.uint32),
     )
 
 
@@ -267,6 +275,25 @@ def hzb_build(depth: np.ndarray, max_size: int = 1024) -> np.ndarray:
     rc = lib.or_hzb_build(d.ctypes.data, w, h, max_size, out.ctypes.data)
     if rc != 0:
         raise ValueError("or_hzb_build: depth larger than max_size")
+    return out
+
+
+def raster(pool: Pool, draws: np.ndarray, count: int, view_proj, width: int, height: int) -> np.ndarray:
+    """The occluder raster of vg_hzb_from_cut: the first `count` entries of `draws` (camera entries, uvec4)
+    drawn depth-only with `view_proj` (column-major) into a (height, width) image cleared to 1."""
+    lib = load()
+    d = np.ascontiguousarray(draws, dtype=np.uint32).reshape(-1, 4)
+    assert count <= len(d)
+    vp = np.ascontiguousarray(np.asarray(view_proj, dtype=np.float32).reshape(16))
+    out = np.ones((height, width), dtype=np.float32)
+    t = np.ascontiguousarray(pool.triangles if pool.triangles is not None else np.zeros(1, np.uint32), dtype=np.uint32)
+    mv = np.ascontiguousarray(pool.meshlet_vertices if pool.meshlet_vertices is not None else np.zeros(1, np.uint32), dtype=np.uint32)
+    vx = np.ascontiguousarray(pool.vertices if pool.vertices is not None else np.zeros(4, np.float32), dtype=np.float32)
+    rc = lib.or_raster(pool.instances.ctypes.data, len(pool.instances) // 20, pool.info.ctypes.data, len(pool.info) // 4,
+                       t.ctypes.data, len(t), mv.ctypes.data, len(mv), vx.ctypes.data, len(vx) // 4, d.ctypes.data, count,
+                       vp.ctypes.data, width, height, out.ctypes.data)
+    if rc != 0:
+        raise ValueError("or_raster: an entry lies outside the pool")
     return out
 
 

@@ -917,3 +917,103 @@ int or_expand(const or_scene* s, const uint32_t* triangles, uint64_t triangle_wo
   if (written) *written = n;
   return 0;
 }
+
+/* ------------------------------------------------------------------ occluder raster (vg_hzb_from_cut) */
+
+typedef struct {
+  int32_t x, y; /* 1/256 pixel, y down */
+  float z;
+  int ok;
+} rvert_t;
+
+static rvert_t raster_vertex(const float* m, const float* vp, const float* p, uint32_t W, uint32_t H) {
+  rvert_t v = {0, 0, 0.0f, 0};
+  const float wx = ((m[0] * p[0] + m[4] * p[1]) + m[8] * p[2]) + m[12];
+  const float wy = ((m[1] * p[0] + m[5] * p[1]) + m[9] * p[2]) + m[13];
+  const float wz = ((m[2] * p[0] + m[6] * p[1]) + m[10] * p[2]) + m[14];
+  const float cx = ((vp[0] * wx + vp[4] * wy) + vp[8] * wz) + vp[12];
+  const float cy = ((vp[1] * wx + vp[5] * wy) + vp[9] * wz) + vp[13];
+  const float cz = ((vp[2] * wx + vp[6] * wy) + vp[10] * wz) + vp[14];
+  const float cw = ((vp[3] * wx + vp[7] * wy) + vp[11] * wz) + vp[15];
+  if (!(cw > 0.0f) || !(cz >= 0.0f)) return v;
+  const float sx = ((cx / cw) * 0.5f + 0.5f) * (float)W;
+  const float sy = (0.5f - (cy / cw) * 0.5f) * (float)H;
+  if (!(fabsf(sx) <= 1048576.0f) || !(fabsf(sy) <= 1048576.0f)) return v;
+  v.x = (int32_t)rintf(sx * 256.0f);
+  v.y = (int32_t)rintf(sy * 256.0f);
+  v.z = cz / cw;
+  v.ok = 1;
+  return v;
+}
+
+static int32_t floor_shift8(int64_t v) { return (int32_t)(v >= 0 ? v / 256 : -((-v + 255) / 256)); }
+
+static void raster_triangle(rvert_t a, rvert_t b, rvert_t c, uint32_t W, uint32_t H, float* depth) {
+  if (!a.ok || !b.ok || !c.ok) return;
+  const int64_t signed_area = (int64_t)(b.x - a.x) * (c.y - a.y) - (int64_t)(b.y - a.y) * (c.x - a.x);
+  if (signed_area >= 0) return; /* back-facing (screen y is down) or degenerate */
+  const rvert_t t = b;           /* edge order: positive area */
+  b = c;
+  c = t;
+  const int64_t area = -signed_area;
+  int32_t minx = a.x, maxx = a.x, miny = a.y, maxy = a.y;
+  if (b.x < minx) minx = b.x;
+  if (c.x < minx) minx = c.x;
+  if (b.x > maxx) maxx = b.x;
+  if (c.x > maxx) maxx = c.x;
+  if (b.y < miny) miny = b.y;
+  if (c.y < miny) miny = c.y;
+  if (b.y > maxy) maxy = b.y;
+  if (c.y > maxy) maxy = c.y;
+  int32_t x0 = floor_shift8((int64_t)minx - 128 + 255), x1 = floor_shift8((int64_t)maxx - 128);
+  int32_t y0 = floor_shift8((int64_t)miny - 128 + 255), y1 = floor_shift8((int64_t)maxy - 128);
+  if (x0 < 0) x0 = 0;
+  if (y0 < 0) y0 = 0;
+  if (x1 > (int32_t)W - 1) x1 = (int32_t)W - 1;
+  if (y1 > (int32_t)H - 1) y1 = (int32_t)H - 1;
+  for (int32_t y = y0; y <= y1; y++)
+    for (int32_t x = x0; x <= x1; x++) {
+      const int64_t px = (int64_t)x * 256 + 128, py = (int64_t)y * 256 + 128;
+      const int64_t e0 = (int64_t)(c.x - b.x) * (py - b.y) - (int64_t)(c.y - b.y) * (px - b.x);
+      const int64_t e1 = (int64_t)(a.x - c.x) * (py - c.y) - (int64_t)(a.y - c.y) * (px - c.x);
+      const int64_t e2 = (int64_t)(b.x - a.x) * (py - a.y) - (int64_t)(b.y - a.y) * (px - a.x);
+      if (e0 < 0 || e1 < 0 || e2 < 0) continue;
+      float z = (((float)e0 * a.z + (float)e1 * b.z) + (float)e2 * c.z) / (float)area;
+      z = fminf(fmaxf(z, 0.0f), 1.0f);
+      float* d = &depth[(size_t)y * W + (size_t)x];
+      if (z < *d) *d = z;
+    }
+}
+
+int or_raster(const float* instances, uint32_t instance_count, const uint32_t* info, uint32_t meshlet_count,
+              const uint32_t* triangles, uint64_t triangle_words, const uint32_t* meshlet_vertices,
+              uint64_t meshlet_vertex_words, const float* vertices, uint64_t vertex_count, const uint32_t* draw_list,
+              uint32_t count, const float* view_proj, uint32_t width, uint32_t height, float* depth) {
+  rvert_t v[128];
+  for (uint32_t e = 0; e < count; e++) {
+    const uint32_t instance = draw_list[e * 4], meshlet = draw_list[e * 4 + 1];
+    if (instance >= instance_count || meshlet >= meshlet_count) return -1;
+    const uint32_t* mi = &info[(size_t)meshlet * 4];
+    if (mi[3] == 0xffffffffu) continue; /* the mesh was added without positions */
+    const uint64_t first = mi[1] / 3;
+    if (first + mi[0] > triangle_words) return -1;
+    uint32_t nv = 0;
+    for (uint32_t t = 0; t < mi[0]; t++)
+      for (int k = 0; k < 3; k++) {
+        const uint32_t l = (triangles[first + t] >> (8 * k)) & 255u;
+        if (l + 1 > nv) nv = l + 1;
+      }
+    if (nv > 128 || (uint64_t)mi[3] + nv > meshlet_vertex_words) return -1;
+    const float* m = &instances[(size_t)instance * INSTANCE_FLOATS];
+    for (uint32_t k = 0; k < nv; k++) {
+      const uint32_t id = meshlet_vertices[mi[3] + k];
+      if (id >= vertex_count) return -1;
+      v[k] = raster_vertex(m, view_proj, &vertices[(size_t)id * 4], width, height);
+    }
+    for (uint32_t t = 0; t < mi[0]; t++) {
+      const uint32_t packed = triangles[first + t];
+      raster_triangle(v[packed & 255u], v[(packed >> 8) & 255u], v[(packed >> 16) & 255u], width, height, depth);
+    }
+  }
+  return 0;
+}

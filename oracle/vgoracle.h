@@ -121,6 +121,18 @@ int or_budget_histogram(const or_scene* scene, const or_view* view, const uint32
 int or_expand(const or_scene* scene, const uint32_t* triangles, uint64_t triangle_words, int cut, const uint32_t* counters,
               const uint32_t* draw_list, uint32_t first, uint32_t* indices, uint64_t* written);
 
+/*
+ * The occluder pass (OcclusionCulling.ts:4-21, 244-273) as vg_hzb_from_cut rasterizes it: the camera entries
+ * draw_list[0 .. count) (uvec4) of a cut, over the pool's instances (20 f32 each), meshlet info (4 u32 each,
+ * [3] = offset into meshlet_vertices or ~0 for none), meshletTriangles, meshlet_vertices (pool-global ids)
+ * and vertices (4 f32 each), are drawn depth-only with `view_proj` into `depth` (width x height, row 0 at
+ * the top), which must hold 1.0 on entry. Returns 0, or -1 on an entry outside the pool.
+ */
+int or_raster(const float* instances, uint32_t instance_count, const uint32_t* info, uint32_t meshlet_count,
+              const uint32_t* triangles, uint64_t triangle_words, const uint32_t* meshlet_vertices,
+              uint64_t meshlet_vertex_words, const float* vertices, uint64_t vertex_count, const uint32_t* draw_list,
+              uint32_t count, const float* view_proj, uint32_t width, uint32_t height, float* depth);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,20 +10,40 @@ const hex = (ab) => Buffer.from(ab).toString('hex');
 const out = { slots: [], orders: [] };
 const ctx = vg.createContext(spec.config);
 const scene = vg.createScene(ctx);
+const fromHex = (h, T) => {
+  const b = Buffer.from(h, 'hex');
+  return new T(b.buffer.slice(b.byteOffset, b.byteOffset + b.length));
+};
+out.built = [];
 for (const m of spec.meshes) {
-  const data = {
-    meshletBounds: new Float32Array(m.meshletBounds), meshletInfo: new Uint32Array(m.meshletInfo),
-    levelRanges: new Uint32Array(m.levelRanges), levelErrors: new Float32Array(m.levelErrors),
-    levelGuardErrors: new Float32Array(m.levelGuardErrors), boundingSphere: new Float32Array(m.boundingSphere),
-    lodBoundsSphere: new Float32Array(m.lodBoundsSphere), cullBoundsSphere: new Float32Array(m.cullBoundsSphere),
-  };
+  let data;
+  if (m.source) {
+    // A raw triangle mesh: the addon's DAG builder makes the VirtualMeshData, as the host's preprocess would.
+    data = vg.buildVirtualMesh(fromHex(m.source.positions, Float32Array), fromHex(m.source.indices, Uint32Array), m.source.options || {});
+    const fields = ['meshletBounds', 'meshletInfo', 'meshletVertices', 'meshletTriangles', 'levelRanges', 'levelErrors',
+      'levelGuardErrors', 'replacementGroup', 'replacementStart', 'replacementIndices', 'indices', 'positions', 'boundingSphere',
+      'lodBoundsSphere', 'cullBoundsSphere'];
+    const b = { meshletCount: data.meshletCount, stats: data.stats };
+    for (const f of fields) b[f] = hex(data[f].buffer);
+    out.built.push(b);
+  } else {
+    data = {
+      meshletBounds: new Float32Array(m.meshletBounds), meshletInfo: new Uint32Array(m.meshletInfo),
+      levelRanges: new Uint32Array(m.levelRanges), levelErrors: new Float32Array(m.levelErrors),
+      levelGuardErrors: new Float32Array(m.levelGuardErrors), boundingSphere: new Float32Array(m.boundingSphere),
+      lodBoundsSphere: new Float32Array(m.lodBoundsSphere), cullBoundsSphere: new Float32Array(m.cullBoundsSphere),
+    };
+    if (m.meshletTriangles) data.meshletTriangles = fromHex(m.meshletTriangles, Uint32Array);
+    if (m.positions) data.positions = fromHex(m.positions, Float32Array);
+    if (m.meshletVertices) data.meshletVertices = fromHex(m.meshletVertices, Uint32Array);
+  }
   const slot = vg.addMesh(scene, data);
   const order = vg.setInstances(scene, slot, new Float32Array(m.matrices));
   if (m.flags) vg.setFlags(scene, slot, m.flags);
   out.slots.push(slot);
   out.orders.push(Array.from(order));
 }
-out.pool = [0, 1, 2, 3, 4].map((b) => hex(vg.poolRead(scene, b)));
+out.pool = [0, 1, 2, 3, 4, 5, 6, 7].map((b) => hex(vg.poolRead(scene, b)));
 const finish = () => fs.writeFileSync(outPath, JSON.stringify(out));
 // Depth images as {slot, width, height, depth: hex float32}: the HZB of each slot for a cut.
 const setHzbs = (list) => {

@@ -1,7 +1,8 @@
 """The N-API addon (three-virtual-geometry_amd/napi): the binding the TypeScript host adds, driven from Node.
 
 CPU: the pool a Node host builds through the addon is byte-identical to the one the Python host builds
-through ctypes (same C ABI). GPU: a cut launched from Node equals the same cut launched from Python.
+through ctypes (same C ABI), and so is a DAG the addon's buildVirtualMesh makes from raw triangles.
+GPU: a cut launched from Node equals the same cut launched from Python.
 """
 import ctypes
 import json
@@ -13,7 +14,7 @@ import numpy as np
 import pytest
 
 from helpers import draw_set, ground_camera, small_ruins
-from vgcut import native
+from vgcut import dag, native, scenegen, scenes
 from vgcut.virtual_geometry import make_view
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -29,10 +30,14 @@ FLAGS = {"maxDrawDistance": 1e30, "minPixelRadius": 0.0, "enabled": True, "shado
 
 
 def mesh_json(data, matrices, occlusion=False):
-    return {k: np.asarray(getattr(data, k)).ravel().tolist() for k in
-            ["meshletBounds", "meshletInfo", "levelRanges", "levelErrors", "levelGuardErrors", "boundingSphere",
-             "lodBoundsSphere", "cullBoundsSphere"]} | {"matrices": np.asarray(matrices, dtype=np.float32).ravel().tolist(),
-                                                        "flags": FLAGS | {"occlusion": occlusion}}
+    out = {k: np.asarray(getattr(data, k)).ravel().tolist() for k in
+           ["meshletBounds", "meshletInfo", "levelRanges", "levelErrors", "levelGuardErrors", "boundingSphere",
+            "lodBoundsSphere", "cullBoundsSphere"]} | {"matrices": np.asarray(matrices, dtype=np.float32).ravel().tolist(),
+                                                       "flags": FLAGS | {"occlusion": occlusion}}
+    for k, dtype in (("meshletTriangles", np.uint32), ("positions", np.float32), ("meshletVertices", np.uint32)):
+        if getattr(data, k, None) is not None:
+            out[k] = np.ascontiguousarray(getattr(data, k), dtype=dtype).tobytes().hex()
+    return out
 
 
 def load_python(scene, spec, occlusion=False):
@@ -74,7 +79,40 @@ def test_node_host_builds_the_same_pool(tmp_path):
     assert got["slots"] == list(range(len(orders)))
     for a, b in zip(got["orders"], orders):
         assert a == list(np.asarray(b).tolist())
-    for buf in range(5):
+    for buf in range(8):
+        assert bytes.fromhex(got["pool"][buf]) == scene.pool(buf).tobytes(), f"pool buffer {buf}"
+
+
+@needs_addon
+def test_node_host_builds_the_dag(tmp_path):
+    """buildVirtualMesh through the addon (buildVirtualMesh.ts:185) equals the Python binding's build bit for bit,
+    and the pool a Node host fills from it equals the Python host's."""
+    pos, idx = scenegen.triangle_mesh(scenegen.COLUMN, seed=5, size=(0.5, 3.0, 0), noise_amplitude=0.05,
+                                      noise_frequency=1.5, target_triangles=30_000)
+    rng = np.random.default_rng(3)
+    n = 40
+    mats = scenes.yaw_scale_translate(rng.uniform(0, 6.28, n), rng.uniform(0.5, 2.0, n),
+                                      np.stack([rng.uniform(-30, 30, n), np.zeros(n), rng.uniform(-30, 30, n)], 1))
+    options = {"groupSize": 10, "minRootTriangles": 8}
+    mesh = {"source": {"positions": pos.tobytes().hex(), "indices": idx.tobytes().hex(), "options": options},
+            "matrices": mats.ravel().tolist(), "flags": FLAGS}
+    got = run_node(tmp_path, {"config": {"maxViews": 2, "meshSlots": 8, "device": -1}, "meshes": [mesh]})
+    d = dag.build_virtual_mesh(pos, idx, **options)
+    b = got["built"][0]
+    assert b["meshletCount"] == d.meshletCount
+    assert b["stats"]["lodLevels"] == d.lodLevels and b["stats"]["rootMeshlets"] == d.rootMeshlets
+    assert b["stats"]["leafTriangles"] == d.leafTriangles == len(idx) // 3
+    for f in ("meshletBounds", "meshletInfo", "meshletVertices", "meshletTriangles", "levelRanges", "levelErrors",
+              "levelGuardErrors", "replacementGroup", "replacementStart", "replacementIndices", "indices", "positions",
+              "boundingSphere", "lodBoundsSphere", "cullBoundsSphere"):
+        assert bytes.fromhex(b[f]) == np.ascontiguousarray(getattr(d, f)).tobytes(), f
+    ctx = native.Context(max_views=2, mesh_slots=8, device=-1)
+    scene = native.Scene(ctx)
+    slot = scene.add_mesh(d)
+    scene.set_instances(slot, mats)
+    scene.set_flags(slot, FLAGS["maxDrawDistance"], FLAGS["minPixelRadius"], FLAGS["enabled"], FLAGS["shadow"],
+                    FLAGS["occlusion"])
+    for buf in range(8):
         assert bytes.fromhex(got["pool"][buf]) == scene.pool(buf).tobytes(), f"pool buffer {buf}"
 
 

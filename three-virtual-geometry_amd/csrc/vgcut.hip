@@ -13,6 +13,7 @@
 
 #include "../../include/vgcut.h"
 #include "vgcut_kernels.hip"
+#include "vgraster_kernels.hip"
 #include "vgcut_host.h"
 
 using namespace vg;
@@ -260,7 +261,9 @@ int upload_scene(vg_scene* s) {
         (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)) || (rc = upload(s, s->d_run, s->m_run)) ||
         (rc = upload(s, s->d_rspan, s->r_span)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
         (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)) ||
-        (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)))
+        (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)) ||
+        (rc = upload(s, s->d_vlist, s->m_vlist)) || (rc = upload(s, s->d_verts, s->m_verts)) ||
+        (rc = upload(s, s->d_mverts, s->m_mverts)))
       return rc;
     s->dirty_meshlets = false;
   }
@@ -790,6 +793,7 @@ int vg_scene_create(vg_ctx* ctx, vg_scene** out) {
   s->d_hzbs.resize(views);
   s->d_indices.resize(2 * static_cast<size_t>(views));
   s->phase1_ready.assign(views, 0);
+  s->phase2_ran.assign(views, 0);
   s->hzb_valid.assign(views, 0);
   s->h_draw_ptrs.assign(views, nullptr);
   s->h_hzb_ptrs.assign(views, nullptr);
@@ -867,6 +871,28 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
     if (s->index_used + mesh->triangle_count * 3 > 0xffffffffull)
       return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: more than 2^32 indices in a scene");
   }
+  // Occluder raster inputs: each meshlet's vertex list holds its largest local id (GeometryPool.ts:1102).
+  const bool has_vertices = mesh->positions && mesh->meshlet_vertices && mesh->vertex_count && mesh->meshlet_vertex_count;
+  std::vector<uint32_t> vcount;
+  if (has_vertices) {
+    if (!mesh->meshlet_triangles) return fail(ctx, VG_E_INVALID, "vg_mesh_add: positions need the meshlet triangles too");
+    if (s->m_verts.size() + mesh->vertex_count > 0xffffffffull || s->m_mverts.size() + mesh->meshlet_vertex_count > 0xfffffffeull)
+      return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: more than 2^32 vertices in a scene");
+    for (uint64_t i = 0; i < mesh->meshlet_vertex_count; i++)
+      if (mesh->meshlet_vertices[i] >= mesh->vertex_count) return fail(ctx, VG_E_INVALID, "vg_mesh_add: a meshlet vertex id >= vertex_count");
+    vcount.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t* info = mesh->meshlet_info + static_cast<size_t>(i) * 4;
+      uint32_t top = 0;
+      for (uint32_t t = 0; t < info[0]; t++) {
+        const uint32_t packed = mesh->meshlet_triangles[info[1] / 3 + t];
+        top = std::max(top, std::max(packed & 255u, std::max((packed >> 8) & 255u, (packed >> 16) & 255u)) + 1);
+      }
+      if (top > 128 || static_cast<uint64_t>(info[3]) + top > mesh->meshlet_vertex_count)
+        return fail(ctx, VG_E_INVALID, "vg_mesh_add: a meshlet's local ids lie outside its vertex list (or above 127)");
+      vcount[i] = top;
+    }
+  }
   uint32_t slot = 0;
   while (slot < s->slots && s->meshes[slot].used) slot++;
   if (slot >= s->slots) return fail(ctx, VG_E_CAPACITY, "vg_mesh_add: no free mesh slot");
@@ -893,7 +919,10 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
     s->m_info.push_back(info[0]);
     s->m_info.push_back(static_cast<uint32_t>(info[1] + s->index_used));
     s->m_info.push_back(info[2]);
-    s->m_info.push_back(info[3]);
+    // vertexOffset rebased to the pool's meshlet vertex list (GeometryPool.ts:225, 428); no list: ~0
+    const uint32_t voff = has_vertices ? static_cast<uint32_t>(info[3] + s->m_mverts.size()) : 0xffffffffu;
+    s->m_info.push_back(voff);
+    s->m_vlist.push_back(make_uint2(has_vertices ? voff : 0u, has_vertices ? vcount[i] : 0u));
     s->m_first.push_back(static_cast<uint32_t>(info[1] + s->index_used));
   }
   // LOD runs: consecutive meshlets of one level with the same LOD-test inputs (bit-equal spheres and
@@ -941,6 +970,12 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
     mesh_indices = mesh->triangle_count * 3;
   }
   s->index_used += mesh_indices;
+  if (has_vertices) {
+    const uint32_t vbase = static_cast<uint32_t>(s->m_verts.size());
+    for (uint32_t v = 0; v < mesh->vertex_count; v++)
+      s->m_verts.push_back(make_float4(mesh->positions[v * 3ull], mesh->positions[v * 3ull + 1], mesh->positions[v * 3ull + 2], 0.0f));
+    for (uint64_t i = 0; i < mesh->meshlet_vertex_count; i++) s->m_mverts.push_back(mesh->meshlet_vertices[i] + vbase);
+  }
   // Level table after the mesh records: (first meshlet, count, min own, max parent), (max own, min parent, 0, 0)
   // (GeometryPool.ts:432-445).
   m.level_count = L;
@@ -1057,6 +1092,70 @@ int vg_hzb_set(vg_scene* s, uint32_t view_slot, const float* depth, uint32_t wid
   return VG_OK;
 }
 
+int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height, uint32_t with_phase2) {
+  if (!s || !view_proj || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_hzb_from_cut: host-only context");
+  const uint32_t max_size = ctx->cfg.hzb_max_size;
+  if (width == 0 || height == 0 || width > max_size || height > max_size)
+    return fail(ctx, VG_E_INVALID, "vg_hzb_from_cut: depth size must be within [1, hzb_max_size]");
+  if (!s->d_draws[view_slot].ptr) return fail(ctx, VG_E_INVALID, "vg_hzb_from_cut: the slot has no cut yet");
+  if (int rc = upload_scene(s)) return rc;
+  uint32_t total = kHzbHeader;
+  for (uint32_t dim = max_size;; dim = (dim + 1) / 2) {
+    total += dim * dim;
+    if (dim == 1) break;
+  }
+  if (!s->d_hzbs[view_slot].ptr) {
+    if (int rc = ensure(s, s->d_hzbs[view_slot], sizeof(float) * total)) return rc;
+    s->h_hzb_ptrs[view_slot] = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+    VG_HIP(ctx, hipMemcpyAsync(static_cast<float**>(s->d_hzb_ptrs.ptr) + view_slot, &s->h_hzb_ptrs[view_slot], sizeof(float*),
+                               hipMemcpyHostToDevice, ctx->stream));
+  }
+  float* hzb = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+  RasterParams r;
+  for (int k = 0; k < 4; k++) r.col[k] = static_cast<const float4*>(s->d_col[k].ptr);
+  r.first = static_cast<const uint32_t*>(s->d_first.ptr);
+  r.tris = static_cast<const uint32_t*>(s->d_tris.ptr);
+  r.vlist = static_cast<const uint2*>(s->d_vlist.ptr);
+  r.tri = static_cast<const uint32_t*>(s->d_tri.ptr);
+  r.mverts = static_cast<const uint32_t*>(s->d_mverts.ptr);
+  r.verts = static_cast<const float4*>(s->d_verts.ptr);
+  r.draws = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
+  // Camera entries: [0, drawn) of the cut, or [0, min(selected, max)) when its second phase appended more.
+  const bool both = with_phase2 && s->phase2_ran[view_slot] && s->d_counters2.ptr;
+  const uint32_t* counters = static_cast<const uint32_t*>(both ? s->d_counters2.ptr : s->d_counters.ptr) +
+                             static_cast<size_t>(view_slot) * s->counter_stride;
+  r.count = counters + 2 + (both ? kSelected : kDrawn);
+  r.clamp = ctx->cfg.max_meshlets;
+  r.width = width;
+  r.height = height;
+  r.depth = reinterpret_cast<uint32_t*>(hzb + kHzbHeader);
+  std::memcpy(r.vp, view_proj, sizeof(r.vp));
+  hipLaunchKernelGGL(k_hzb_header, dim3(1), dim3(64), 0, ctx->stream, hzb, max_size, width, height);
+  const uint32_t pixels = width * height;
+  hipLaunchKernelGGL(k_raster_clear, dim3(std::min<uint32_t>((pixels + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth, pixels);
+  if (!s->m_verts.empty())
+    hipLaunchKernelGGL(k_raster, dim3(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu), dim3(256), 0, ctx->stream, r);
+  uint32_t lw = width, lh = height, levels = 0;
+  std::vector<uint32_t> ws, hs;
+  for (uint32_t dim = max_size;; dim = (dim + 1) / 2) {
+    ws.push_back(lw);
+    hs.push_back(lh);
+    levels++;
+    if ((lw == 1 && lh == 1) || dim == 1) break;
+    lw = lw > 1 ? (lw + 1) / 2 : 1;
+    lh = lh > 1 ? (lh + 1) / 2 : 1;
+  }
+  for (uint32_t base = 0; base + 1 < levels; base += 5) {
+    dim3 grid((ws[base] + 31) / 32, (hs[base] + 31) / 32);
+    hipLaunchKernelGGL(k_hzb_pass, grid, dim3(256), 0, ctx->stream, hzb, base);
+  }
+  VG_HIP(ctx, hipGetLastError());
+  s->hzb_valid[view_slot] = 1;
+  return VG_OK;
+}
+
 int vg_hzb_read(vg_scene* s, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats) {
   if (!s || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
@@ -1079,7 +1178,10 @@ int vg_hzb_invalidate(vg_scene* s, uint32_t view_slot) {
 
 // After a cut: a view that asked for two-phase occlusion may take its second phase next.
 static void mark_phase1(vg_scene* s, const vg_view* views, uint32_t count) {
-  for (uint32_t i = 0; i < count; i++) s->phase1_ready[views[i].slot] = views[i].occlusion == 2 ? 1 : 0;
+  for (uint32_t i = 0; i < count; i++) {
+    s->phase1_ready[views[i].slot] = views[i].occlusion == 2 ? 1 : 0;
+    s->phase2_ran[views[i].slot] = 0;
+  }
 }
 
 int vg_cut(vg_scene* s, const vg_view* views, uint32_t count) {
@@ -1107,7 +1209,10 @@ int vg_cut_phase2(vg_scene* s, const vg_view* views, uint32_t count) {
       return fail(ctx, VG_E_INVALID, "vg_cut_phase2: every view needs occlusion = 2 and its slot's cut (vg_cut) just before");
   if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
   if (int rc = enqueue_phase2(s, &s->lane, count)) return rc;
-  for (uint32_t i = 0; i < count; i++) s->phase1_ready[views[i].slot] = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    s->phase1_ready[views[i].slot] = 0;
+    s->phase2_ran[views[i].slot] = 1;
+  }
   return VG_OK;
 }
 
@@ -1208,7 +1313,7 @@ int vg_state_set(vg_scene* s, uint32_t view_slot, const uint32_t* state) {
 }
 
 int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t capacity_bytes, uint64_t* size_bytes) {
-  if (!s || buffer > VG_POOL_MESHLET_TRIANGLES) return VG_E_INVALID;
+  if (!s || buffer > VG_POOL_MESHLET_VERTICES) return VG_E_INVALID;
   std::vector<float> f;
   const void* src = nullptr;
   uint64_t bytes = 0;
@@ -1261,6 +1366,14 @@ int vg_scene_pool_read(const vg_scene* s, uint32_t buffer, void* dst, uint64_t c
     case VG_POOL_MESHLET_INFO:
       src = s->m_info.data();
       bytes = sizeof(uint32_t) * s->m_info.size();
+      break;
+    case VG_POOL_VERTICES:
+      src = s->m_verts.data();
+      bytes = sizeof(float4) * s->m_verts.size();
+      break;
+    case VG_POOL_MESHLET_VERTICES:
+      src = s->m_mverts.data();
+      bytes = sizeof(uint32_t) * s->m_mverts.size();
       break;
     default:
       src = s->m_tri.data();

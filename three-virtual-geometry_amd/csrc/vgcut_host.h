@@ -91,6 +91,11 @@ struct vg_scene {
   std::vector<uint32_t> m_info;  // host mirror of meshletInfo, firstIndex rebased (GeometryPool.ts:417-425)
   std::vector<uint32_t> m_first; // meshletInfo.y per meshlet (rebased), for the expand pass
   std::vector<uint32_t> m_tri;   // meshletTriangles of the pool (GeometryPool.ts:430): at firstIndex / 3
+  // occluder raster inputs: the pool's vertex positions and meshlet vertex lists (GeometryPool.ts:221-226,
+  // 386-429), and per meshlet its list's (offset, vertex count), count 0 for meshes added without positions
+  std::vector<float4> m_verts;
+  std::vector<uint32_t> m_mverts;
+  std::vector<uint2> m_vlist;
   std::vector<uint32_t> m_run;   // LOD runs (see kMaxRun)
   std::vector<uint2> r_span;
   std::vector<float4> r_lod, r_parent;
@@ -104,6 +109,7 @@ struct vg_scene {
   vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
   vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs
+  vgi::DevBuf d_verts, d_mverts, d_vlist;  // occluder raster inputs
   // per view slot
   vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;
   std::vector<vgi::DevBuf> d_draws, d_hzbs;
@@ -113,6 +119,7 @@ struct vg_scene {
   vgi::DevBuf d_counters2, d_args2, d_reject_bits, d_reject_meshlets, d_phase;
   uint32_t reject_words = 0;
   std::vector<uint8_t> phase1_ready;  // per slot: its last cut recorded rejects and no second phase ran since
+  std::vector<uint8_t> phase2_ran;    // per slot: a second phase appended to its last cut
   std::vector<uint8_t> hzb_valid;
   std::vector<uint4*> h_draw_ptrs;
   std::vector<float*> h_hzb_ptrs;

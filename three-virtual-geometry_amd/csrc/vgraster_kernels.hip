@@ -1,0 +1,193 @@
+// The occluder pass (src/core/runtime/OcclusionCulling.ts:4-21, 244-273) as a software raster: the camera
+// entries of a view slot's draw list are drawn depth-only into level 0 of the slot's HZB buffer, which
+// k_hzb_pass then reduces. The reference draws the occluders' previous cuts with the hardware raster
+// (three.js, depth-only material); here the same draw list is read where the cut left it on the device.
+//
+//   k_hzb_header  writes the HZB header (level offsets and sizes) for a width x height level 0
+//   k_raster_clear  level 0 <- 1.0 (WebGPU far depth)
+//   k_raster      one wave per drawn meshlet: the meshlet's vertices are transformed once into LDS
+//                 (instance matrix, then view_proj; ((c0 x + c1 y) + c2 z) + c3 in binary32 without
+//                 contraction), snapped to 1/256 pixel; each lane sets up its triangles (at most 2 of a
+//                 meshlet's 128); triangles whose pixel box holds at most kRasterLanePixels pixel centres
+//                 are filled by their lane, larger ones by the whole wave, one at a time. Depth is kept
+//                 with atomicMin on the float bits (all values in [0, 1], so integer order is float order),
+//                 which makes the result independent of the order of the writes.
+#include "vgcut_kernels.h"
+
+namespace vg {
+
+constexpr uint32_t kRasterLanePixels = 64;
+constexpr int32_t kRasterNoVertex = INT32_MIN;
+constexpr uint32_t kRasterBlocksPerCu = 8;
+
+struct RasterParams {
+  const float4* col[4];   // instance matrix columns
+  const uint32_t* first;  // per meshlet: firstIndex (pool-rebased); its triangles are at first / 3
+  const uint32_t* tris;   // per meshlet: triangle count
+  const uint2* vlist;     // per meshlet: (offset into meshlet_vertices, vertex count; 0 = no positions)
+  const uint32_t* tri;    // packed local vertex ids a | b << 8 | c << 16
+  const uint32_t* mverts; // pool-global vertex ids
+  const float4* verts;    // positions (x, y, z, 0)
+  const uint4* draws;     // the slot's draw list; camera entries from position 0
+  const uint32_t* count;  // entries to draw: min(*count, clamp)
+  uint32_t clamp;
+  uint32_t width, height;
+  uint32_t* depth;        // level 0 of the HZB, float bits, row 0 at the top
+  float vp[16];           // column-major clip = vp * world
+};
+
+struct RasterVertex {
+  int32_t x, y;  // screen position in 1/256 pixel (y down); x == kRasterNoVertex: not drawable
+  float z;       // clip z / w
+};
+
+// Transforms an object-space position to a snapped screen vertex; false when it cannot be drawn.
+__device__ __forceinline__ RasterVertex raster_vertex(const RasterParams& r, float4 c0, float4 c1, float4 c2, float4 c3, float4 p) {
+  const float wx = ((c0.x * p.x + c1.x * p.y) + c2.x * p.z) + c3.x;
+  const float wy = ((c0.y * p.x + c1.y * p.y) + c2.y * p.z) + c3.y;
+  const float wz = ((c0.z * p.x + c1.z * p.y) + c2.z * p.z) + c3.z;
+  const float* m = r.vp;
+  const float cx = ((m[0] * wx + m[4] * wy) + m[8] * wz) + m[12];
+  const float cy = ((m[1] * wx + m[5] * wy) + m[9] * wz) + m[13];
+  const float cz = ((m[2] * wx + m[6] * wy) + m[10] * wz) + m[14];
+  const float cw = ((m[3] * wx + m[7] * wy) + m[11] * wz) + m[15];
+  RasterVertex v{kRasterNoVertex, 0, 0.0f};
+  if (!(cw > 0.0f) || !(cz >= 0.0f)) return v;
+  const float sx = ((cx / cw) * 0.5f + 0.5f) * static_cast<float>(r.width);
+  const float sy = (0.5f - (cy / cw) * 0.5f) * static_cast<float>(r.height);
+  if (!(fabsf(sx) <= 1048576.0f) || !(fabsf(sy) <= 1048576.0f)) return v;
+  v.x = static_cast<int32_t>(rintf(sx * 256.0f));
+  v.y = static_cast<int32_t>(rintf(sy * 256.0f));
+  v.z = cz / cw;
+  return v;
+}
+
+// A front-facing triangle in edge order (positive area) and the pixel box of its centres.
+struct RasterTriangle {
+  RasterVertex a, b, c;
+  int64_t area;
+  int32_t x0, x1, y0, y1;  // inclusive pixel box, clamped to the image; empty when x0 > x1 or y0 > y1
+};
+
+// Screen y points down, so front faces (counter-clockwise in clip space) have negative signed area here:
+// they are swapped into positive order; back faces and degenerate triangles give an empty box.
+__device__ __forceinline__ RasterTriangle raster_setup(RasterVertex a, RasterVertex b, RasterVertex c, uint32_t W, uint32_t H) {
+  RasterTriangle t;
+  t.x0 = 1;
+  t.x1 = 0;
+  t.y0 = 1;
+  t.y1 = 0;
+  t.area = 0;
+  if (a.x == kRasterNoVertex || b.x == kRasterNoVertex || c.x == kRasterNoVertex) return t;
+  const int64_t area = static_cast<int64_t>(b.x - a.x) * (c.y - a.y) - static_cast<int64_t>(b.y - a.y) * (c.x - a.x);
+  if (area >= 0) return t;
+  t.a = a;
+  t.b = c;
+  t.c = b;
+  t.area = -area;
+  const int32_t minx = min(a.x, min(b.x, c.x)), maxx = max(a.x, max(b.x, c.x));
+  const int32_t miny = min(a.y, min(b.y, c.y)), maxy = max(a.y, max(b.y, c.y));
+  // Pixel centres sit at (i + 1/2) * 256: the first centre >= min and the last <= max (arithmetic shifts floor).
+  t.x0 = max((minx - 128 + 255) >> 8, 0);
+  t.x1 = min((maxx - 128) >> 8, static_cast<int32_t>(W) - 1);
+  t.y0 = max((miny - 128 + 255) >> 8, 0);
+  t.y1 = min((maxy - 128) >> 8, static_cast<int32_t>(H) - 1);
+  return t;
+}
+
+__device__ __forceinline__ void raster_pixel(uint32_t* depth, uint32_t W, const RasterTriangle& t, int32_t x, int32_t y) {
+  const int64_t px = static_cast<int64_t>(x) * 256 + 128, py = static_cast<int64_t>(y) * 256 + 128;
+  const RasterVertex& a = t.a;
+  const RasterVertex& b = t.b;
+  const RasterVertex& c = t.c;
+  const int64_t e0 = static_cast<int64_t>(c.x - b.x) * (py - b.y) - static_cast<int64_t>(c.y - b.y) * (px - b.x);
+  const int64_t e1 = static_cast<int64_t>(a.x - c.x) * (py - c.y) - static_cast<int64_t>(a.y - c.y) * (px - c.x);
+  const int64_t e2 = static_cast<int64_t>(b.x - a.x) * (py - a.y) - static_cast<int64_t>(b.y - a.y) * (px - a.x);
+  if ((e0 | e1 | e2) < 0) return;
+  float z = ((static_cast<float>(e0) * a.z + static_cast<float>(e1) * b.z) + static_cast<float>(e2) * c.z) / static_cast<float>(t.area);
+  z = fminf(fmaxf(z, 0.0f), 1.0f);
+  atomicMin(depth + static_cast<size_t>(y) * W + x, __float_as_uint(z));
+}
+
+__device__ __forceinline__ uint32_t raster_box_pixels(const RasterTriangle& t) {
+  if (t.x0 > t.x1 || t.y0 > t.y1) return 0;
+  return static_cast<uint32_t>(t.x1 - t.x0 + 1) * static_cast<uint32_t>(t.y1 - t.y0 + 1);
+}
+
+__global__ __launch_bounds__(64) void k_hzb_header(float* hzb, uint32_t max_size, uint32_t width, uint32_t height) {
+  for (uint32_t i = threadIdx.x; i < kHzbHeader; i += blockDim.x) hzb[i] = 0.0f;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  uint32_t offset = kHzbHeader, dim = max_size, lw = width, lh = height, levels = 0;
+  for (uint32_t i = 0; i < kHzbMaxLevels; i++) {
+    hzb[i * 4] = static_cast<float>(offset);
+    hzb[i * 4 + 1] = static_cast<float>(lw);
+    hzb[i * 4 + 2] = static_cast<float>(lh);
+    hzb[i * 4 + 3] = 0.0f;
+    levels++;
+    if ((lw == 1 && lh == 1) || dim == 1) break;
+    offset += dim * dim;
+    dim = (dim + 1) / 2;
+    lw = lw > 1 ? (lw + 1) / 2 : 1;
+    lh = lh > 1 ? (lh + 1) / 2 : 1;
+  }
+  hzb[kHzbLevelCountSlot] = static_cast<float>(levels);
+}
+
+__global__ __launch_bounds__(256) void k_raster_clear(uint32_t* depth, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) depth[i] = 0x3f800000u;
+}
+
+__global__ __launch_bounds__(256) void k_raster(RasterParams r) {
+  __shared__ RasterVertex lds[4][128];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  RasterVertex* V = lds[wave];
+  const uint32_t n = min(*r.count, r.clamp);
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  auto wave_sync = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  for (uint32_t e = xcd_wave_rank(); e < n; e += waves) {
+    const uint4 entry = r.draws[e];
+    const uint32_t instance = uniform(entry.x), meshlet = uniform(entry.y);
+    const uint2 vl = r.vlist[meshlet];
+    const uint32_t nv = uniform(vl.y);
+    if (nv == 0) continue;
+    const uint32_t tris = uniform(r.tris[meshlet]), first = uniform(r.first[meshlet]) / 3;
+    const float4 c0 = r.col[0][instance], c1 = r.col[1][instance], c2 = r.col[2][instance], c3 = r.col[3][instance];
+    for (uint32_t k = lane; k < nv; k += 64) V[k] = raster_vertex(r, c0, c1, c2, c3, r.verts[r.mverts[vl.x + k]]);
+    wave_sync();
+    bool big[2] = {false, false};
+    for (uint32_t s = 0; s < 2; s++) {
+      const uint32_t t = lane + s * 64;
+      if (t >= tris) continue;
+      const uint32_t packed = r.tri[first + t];
+      const RasterTriangle tr = raster_setup(V[packed & 255u], V[(packed >> 8) & 255u], V[(packed >> 16) & 255u], r.width, r.height);
+      const uint32_t pixels = raster_box_pixels(tr);
+      if (pixels > kRasterLanePixels) {
+        big[s] = true;
+        continue;
+      }
+      for (int32_t y = tr.y0; y <= tr.y1; y++)
+        for (int32_t x = tr.x0; x <= tr.x1; x++) raster_pixel(r.depth, r.width, tr, x, y);
+    }
+    // Large triangles: the whole wave strides over each one's pixel box.
+    for (uint32_t s = 0; s < 2; s++) {
+      uint64_t mask = __ballot(big[s]);
+      while (mask) {
+        const uint32_t t = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1) + s * 64;
+        mask &= mask - 1;
+        const uint32_t packed = uniform(r.tri[first + t]);
+        const RasterTriangle tr = raster_setup(V[packed & 255u], V[(packed >> 8) & 255u], V[(packed >> 16) & 255u], r.width, r.height);
+        const uint32_t bw = static_cast<uint32_t>(tr.x1 - tr.x0 + 1), pixels = raster_box_pixels(tr);
+        for (uint32_t i = lane; i < pixels; i += 64)
+          raster_pixel(r.depth, r.width, tr, tr.x0 + static_cast<int32_t>(i % bw), tr.y0 + static_cast<int32_t>(i / bw));
+      }
+    }
+    wave_sync();
+  }
+}
+
+}  // namespace vg

@@ -1,10 +1,11 @@
 // vgcut_napi.cc: the N-API addon the TypeScript host loads in place of the TSL compute nodes (build.sh or
-// node-gyp with binding.gyp; links lib/libvgcut.so). JS values in, C ABI calls out.
+// node-gyp with binding.gyp; links lib/libvgcut.so and lib/libvgdag.so). JS values in, C ABI calls out.
 #include <node_api.h>
 #include <cstdio>
 #include <cstring>
 #include <vector>
 #include "vgcut.h"
+#include "vgdag.h"
 
 #define CHECK_NAPI(call)                                              \
   do {                                                                \
@@ -91,7 +92,8 @@ static napi_value CreateScene(napi_env env, napi_callback_info info) {
   return out;
 }
 
-// addMesh(scene, data: VirtualMeshData) -> slot (GeometryPool.add, geometry part)
+// addMesh(scene, data: VirtualMeshData) -> slot (GeometryPool.add, geometry part; positions and
+// meshletVertices, when present, feed the occluder raster)
 static napi_value AddMesh(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -110,6 +112,11 @@ static napi_value AddMesh(napi_env env, napi_callback_info info) {
   size_t nt = 0;
   d.meshlet_triangles = typed<uint32_t>(env, prop(env, argv[1], "meshletTriangles"), &nt);
   d.triangle_count = d.meshlet_triangles ? nt : 0;
+  size_t npos = 0, nmv = 0;
+  d.positions = typed<float>(env, prop(env, argv[1], "positions"), &npos);
+  d.meshlet_vertices = typed<uint32_t>(env, prop(env, argv[1], "meshletVertices"), &nmv);
+  d.vertex_count = d.positions ? static_cast<uint32_t>(npos / 3) : 0;
+  d.meshlet_vertex_count = d.meshlet_vertices ? nmv : 0;
   d.meshlet_count = static_cast<uint32_t>(ni / 4);
   d.level_count = static_cast<uint32_t>(nr / 2);
   const char* spheres[3] = {"boundingSphere", "lodBoundsSphere", "cullBoundsSphere"};
@@ -366,6 +373,135 @@ static napi_value DrawArgsWords(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// hzbFromCut(scene, slot, viewProj: Float32Array(16), width, height, withPhase2?) -- the occluder pass of
+// OcclusionCulling.beginFrame (OcclusionCulling.ts:244-273) on the device, queued on the scene's stream
+static napi_value HzbFromCut(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  auto* s = static_cast<Scene*>(p);
+  uint32_t slot = 0, width = 0, height = 0;
+  bool with_phase2 = false;
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  size_t n = 0;
+  const float* vp = typed<float>(env, argv[2], &n);
+  if (!vp || n != 16) {
+    napi_throw_type_error(env, nullptr, "hzbFromCut: viewProj must be a Float32Array of 16");
+    return nullptr;
+  }
+  CHECK_NAPI(napi_get_value_uint32(env, argv[3], &width));
+  CHECK_NAPI(napi_get_value_uint32(env, argv[4], &height));
+  if (argc > 5) napi_get_value_bool(env, argv[5], &with_phase2);
+  if (int rc = vg_hzb_from_cut(s->scene, slot, vp, width, height, with_phase2 ? 1u : 0u)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
+// A new typed array holding a copy of `n` elements at `src`.
+static napi_value copy_typed(napi_env env, napi_typedarray_type type, const void* src, size_t n, size_t elem) {
+  void* data = nullptr;
+  napi_value ab, out;
+  if (napi_create_arraybuffer(env, n * elem, &data, &ab) != napi_ok) return nullptr;
+  if (n) memcpy(data, src, n * elem);
+  if (napi_create_typedarray(env, type, n, ab, 0, &out) != napi_ok) return nullptr;
+  return out;
+}
+
+// buildVirtualMesh(positions: Float32Array, indices: Uint32Array, options?) -> VirtualMeshData
+// (buildVirtualMesh.ts:185; options with the reference's names, voxelLods / aggregateLods only false)
+static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  size_t np = 0, ni = 0;
+  const float* positions = typed<float>(env, argv[0], &np);
+  const uint32_t* indices = typed<uint32_t>(env, argv[1], &ni);
+  if (!positions || !indices) {
+    napi_throw_type_error(env, nullptr, "buildVirtualMesh: positions must be a Float32Array, indices a Uint32Array");
+    return nullptr;
+  }
+  vgd_options o;
+  vgd_default_options(&o);
+  if (argc > 2) {
+    napi_valuetype t;
+    CHECK_NAPI(napi_typeof(env, argv[2], &t));
+    if (t == napi_object) {
+      auto u32 = [&](const char* name, uint32_t* out) {
+        uint32_t x;
+        if (napi_get_value_uint32(env, prop(env, argv[2], name), &x) == napi_ok) *out = x;
+      };
+      auto f64 = [&](const char* name, double* out) {
+        double x;
+        if (napi_get_value_double(env, prop(env, argv[2], name), &x) == napi_ok) *out = x;
+      };
+      auto flag = [&](const char* name, int32_t* out) {
+        bool x;
+        if (napi_get_value_bool(env, prop(env, argv[2], name), &x) == napi_ok) *out = x ? 1 : 0;
+      };
+      u32("groupSize", &o.group_size);
+      u32("maxLodLevels", &o.max_lod_levels);
+      f64("maxGroupKeepRatio", &o.max_group_keep_ratio);
+      f64("simplifyRatio", &o.simplify_ratio);
+      f64("minLevelReduction", &o.min_level_reduction);
+      flag("prune", &o.prune);
+      flag("tailSimplify", &o.tail_simplify);
+      u32("minRootTriangles", &o.min_root_triangles);
+      u32("threads", &o.threads);
+      for (const char* unbuilt : {"voxelLods", "aggregateLods"}) {
+        bool on = false;
+        if (napi_get_value_bool(env, prop(env, argv[2], unbuilt), &on) == napi_ok && on) {
+          napi_throw_error(env, nullptr, "buildVirtualMesh: voxel and aggregate levels are not supported");
+          return nullptr;
+        }
+      }
+    }
+  }
+  vgd_mesh* mesh = nullptr;
+  if (int rc = vgd_build(positions, static_cast<uint32_t>(np / 3), indices, ni, &o, &mesh)) {
+    napi_throw_error(env, nullptr, rc == -1 ? "buildVirtualMesh: invalid mesh" : "buildVirtualMesh: build failed");
+    return nullptr;
+  }
+  vgd_mesh_view v;
+  vgd_mesh_get(mesh, &v);
+  napi_value out, stats, x;
+  CHECK_NAPI(napi_create_object(env, &out));
+  auto put = [&](napi_value obj, const char* name, napi_value val) { napi_set_named_property(env, obj, name, val); };
+  const size_t n = v.meshlet_count;
+  put(out, "positions", copy_typed(env, napi_float32_array, v.positions, static_cast<size_t>(v.vertex_count) * 3, 4));
+  put(out, "indices", copy_typed(env, napi_uint32_array, v.indices, v.index_count, 4));
+  napi_create_uint32(env, v.meshlet_count, &x);
+  put(out, "meshletCount", x);
+  put(out, "meshletBounds", copy_typed(env, napi_float32_array, v.meshlet_bounds, n * 16, 4));
+  put(out, "meshletInfo", copy_typed(env, napi_uint32_array, v.meshlet_info, n * 4, 4));
+  put(out, "meshletVertices", copy_typed(env, napi_uint32_array, v.meshlet_vertices, v.meshlet_vertex_count, 4));
+  put(out, "meshletTriangles", copy_typed(env, napi_uint32_array, v.meshlet_triangles, v.index_count / 3, 4));
+  put(out, "levelRanges", copy_typed(env, napi_uint32_array, v.level_ranges, v.level_count * 2, 4));
+  put(out, "levelErrors", copy_typed(env, napi_float32_array, v.level_errors, v.level_count * 2, 4));
+  put(out, "levelGuardErrors", copy_typed(env, napi_float32_array, v.level_guard_errors, v.level_count * 2, 4));
+  put(out, "replacementGroup", copy_typed(env, napi_int32_array, v.replacement_group, n, 4));
+  put(out, "replacementStart", copy_typed(env, napi_uint32_array, v.replacement_start, v.group_count + 1, 4));
+  put(out, "replacementIndices",
+      copy_typed(env, napi_uint32_array, v.replacement_indices, v.replacement_start[v.group_count], 4));
+  put(out, "boundingSphere", copy_typed(env, napi_float32_array, v.bounding_sphere, 4, 4));
+  put(out, "lodBoundsSphere", copy_typed(env, napi_float32_array, v.lod_bounds_sphere, 4, 4));
+  put(out, "cullBoundsSphere", copy_typed(env, napi_float32_array, v.cull_bounds_sphere, 4, 4));
+  CHECK_NAPI(napi_create_object(env, &stats));
+  napi_create_double(env, static_cast<double>(v.leaf_triangles), &x);
+  put(stats, "leafTriangles", x);
+  napi_create_uint32(env, v.leaf_meshlets, &x);
+  put(stats, "leafMeshlets", x);
+  napi_create_uint32(env, v.root_triangles, &x);
+  put(stats, "rootTriangles", x);
+  napi_create_uint32(env, v.root_meshlets, &x);
+  put(stats, "rootMeshlets", x);
+  napi_create_uint32(env, v.lod_levels, &x);
+  put(stats, "lodLevels", x);
+  put(out, "stats", stats);
+  vgd_mesh_free(mesh);
+  return out;
+}
+
 static napi_value Init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"createContext", nullptr, CreateContext, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
@@ -382,6 +518,8 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"readPhase2", nullptr, ReadPhase2, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"counterWords", nullptr, CounterWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"drawArgsWords", nullptr, DrawArgsWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"buildVirtualMesh", nullptr, BuildVirtualMesh, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"hzbFromCut", nullptr, HzbFromCut, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
   return exports;

@@ -25,13 +25,15 @@ VG_POOL_CELLS = 2
 VG_POOL_MESHLET_BOUNDS = 3
 VG_POOL_MESHLET_INFO = 4
 VG_POOL_MESHLET_TRIANGLES = 5
+VG_POOL_VERTICES = 6
+VG_POOL_MESHLET_VERTICES = 7
 
 LIBRARY = _native.library_path("libvgcut.so")
 
 # Every entry point include/vgcut.h declares (the ABI test checks the library exports them).
 EXPORTS = (
     "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
-    "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
+    "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_from_cut", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
@@ -75,6 +77,10 @@ class MeshDesc(ctypes.Structure):
         ("cull_bounds_sphere", ctypes.c_float * 4),
         ("meshlet_triangles", ctypes.c_void_p),
         ("triangle_count", ctypes.c_uint64),
+        ("positions", ctypes.c_void_p),
+        ("vertex_count", ctypes.c_uint32),
+        ("meshlet_vertices", ctypes.c_void_p),
+        ("meshlet_vertex_count", ctypes.c_uint64),
     ]
 
 
@@ -188,6 +194,7 @@ def load():
         "vg_instances_set": ([P, U32, P, P, U32, P], ctypes.c_int),
         "vg_mesh_flags": ([P, U32, ctypes.c_float, ctypes.c_float, U32, U32, U32], ctypes.c_int),
         "vg_hzb_set": ([P, U32, P, U32, U32], ctypes.c_int),
+        "vg_hzb_from_cut": ([P, U32, P, U32, U32, U32], ctypes.c_int),
         "vg_hzb_invalidate": ([P, U32], ctypes.c_int),
         "vg_hzb_read": ([P, U32, P, U64, ctypes.POINTER(U64)], ctypes.c_int),
         "vg_cut": ([P, P, U32], ctypes.c_int),
@@ -311,6 +318,15 @@ class Scene:
             tri = np.ascontiguousarray(tri, dtype=np.uint32)
             d.meshlet_triangles = tri.ctypes.data
             d.triangle_count = len(tri)
+        pos = getattr(data, "positions", None)
+        mverts = getattr(data, "meshletVertices", None)
+        if pos is not None and mverts is not None and tri is not None:
+            pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+            mverts = np.ascontiguousarray(mverts, dtype=np.uint32)
+            d.positions = pos.ctypes.data
+            d.vertex_count = len(pos) // 3
+            d.meshlet_vertices = mverts.ctypes.data
+            d.meshlet_vertex_count = len(mverts)
         slot = ctypes.c_uint32()
         self.ctx.check(self.lib.vg_mesh_add(self.handle, ctypes.byref(d), ctypes.byref(slot)), "vg_mesh_add")
         return int(slot.value)
@@ -330,6 +346,13 @@ class Scene:
         d = np.ascontiguousarray(depth, dtype=np.float32)
         h, w = d.shape
         self.ctx.check(self.lib.vg_hzb_set(self.handle, view_slot, d.ctypes.data, w, h), "vg_hzb_set")
+
+    def hzb_from_cut(self, view_slot: int, view_proj, width: int, height: int, with_phase2: bool = False):
+        """The occluder pass on the device: the slot's last camera cut drawn depth-only with `view_proj`
+        (column-major) into a width x height depth image, and the slot's HZB built from it."""
+        vp = np.ascontiguousarray(np.asarray(view_proj, dtype=np.float32).reshape(16))
+        self.ctx.check(self.lib.vg_hzb_from_cut(self.handle, view_slot, vp.ctypes.data, width, height, int(bool(with_phase2))),
+                       "vg_hzb_from_cut")
 
     def hzb(self, view_slot: int) -> np.ndarray:
         n = ctypes.c_uint64()
@@ -409,7 +432,7 @@ class Scene:
     def pool(self, buffer: int) -> np.ndarray:
         size = ctypes.c_uint64()
         self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, None, 0, ctypes.byref(size)), "vg_scene_pool_read")
-        dtype = np.uint32 if buffer in (VG_POOL_MESHLET_INFO, VG_POOL_MESHLET_TRIANGLES) else np.float32
+        dtype = np.uint32 if buffer in (VG_POOL_MESHLET_INFO, VG_POOL_MESHLET_TRIANGLES, VG_POOL_MESHLET_VERTICES) else np.float32
         out = np.empty(size.value // 4, dtype=dtype)
         self.ctx.check(self.lib.vg_scene_pool_read(self.handle, buffer, out.ctypes.data, size.value, None), "vg_scene_pool_read")
         return out
