@@ -304,7 +304,7 @@ def expand_bench(spec, vd, steps, warmup, schedule, profile_only=False):
 
     views = [v for v, _ in vd]
     ctx = native.Context(max_triangles=REF_TRIANGLES, max_shadow_triangles=REF_SHADOW_TRIANGLES, max_meshlets=REF_MESHLETS,
-                         max_views=len(views), mesh_slots=64, device=0, expand=True)
+                         max_views=len(views), mesh_slots=72, device=0, expand=True)
     scene = native.Scene(ctx)
     t0 = time.time()
     scenes.load_into(scene, spec, occlusion=True)
@@ -393,58 +393,94 @@ def expand_bench(spec, vd, steps, warmup, schedule, profile_only=False):
 
 
 def two_phase_bench(scene, vd, steps):
-    """Two-phase occlusion on the batched views: the views with an HZB ask for it (occlusion = 2), the cut
-    records its camera occlusion rejects, then the second phase re-tests them against each slot's HZB set
-    again from the same terrain depth (as when the depth drawn in between shows the same occluders).
-    Reports the cut's time with recording, the second phase's time and what it re-tested and added."""
+    """Two-phase occlusion against the previous frame's visible set (config 2's occlusion, OcclusionCulling.ts:4-21):
+    on each view with an HZB (occlusion = 2), the occluder raster draws the slot's last cut, both phases, with
+    the camera into its HZB (vg_hzb_from_cut, on the device); the cut records its camera occlusion rejects;
+    the raster draws that cut's clusters; the second phase re-tests the rejects against them. Reports each
+    step's time, the rasters' share, what was re-tested and added, and the raster's algorithmic bytes."""
     import ctypes
 
     import torch
 
+    from vgcut import camera as cam
     from vgcut import native
 
-    views = []
+    views, rasters = [], []
     for v, d in vd:
         w = native.View()
         ctypes.memmove(ctypes.byref(w), ctypes.byref(v), ctypes.sizeof(native.View))
         if d is not None and v.occlusion:
             w.occlusion = 2
+            vp = np.asarray(cam.mat_mul(np.array(v.projection_matrix[:]), np.array(v.view_matrix[:])), dtype=np.float32)
+            rasters.append((w.slot, vp, d.shape[1], d.shape[0]))
         views.append(w)
     two = [v for v in views if v.occlusion == 2]
     initial = np.zeros(8, dtype=np.uint32)
     initial[0] = initial[1] = 10000
-    scene.cut(views, graph=True)
-    scene.cut_phase2(two)
+
+    def frame(timed):
+        t = [0.0, 0.0, 0.0, 0.0]
+        def mark(i, t0):
+            scene.sync()
+            t[i] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for slot, vp, w, h in rasters:
+            scene.hzb_from_cut(slot, vp, w, h, with_phase2=True)
+        mark(0, t0)
+        t0 = time.perf_counter()
+        scene.cut(views, graph=True)
+        mark(1, t0)
+        t0 = time.perf_counter()
+        for slot, vp, w, h in rasters:
+            scene.hzb_from_cut(slot, vp, w, h)
+        mark(2, t0)
+        t0 = time.perf_counter()
+        scene.cut_phase2(two)
+        mark(3, t0)
+        return t
+
+    scene.cut(views, graph=True)  # a first frame: the previous visible set for the rasters
     scene.sync()
-    t_cut = t_phase2 = 0.0
+    for _ in range(2):
+        frame(False)
+    total = [0.0, 0.0, 0.0, 0.0]
     for _ in range(steps):
         for v in views:
             scene.set_state(v.slot, initial)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        scene.cut(views, graph=True)
-        scene.sync()
-        t_cut += time.perf_counter() - t0
-        for v, d in vd:  # the new depth of each two-phase slot (the same terrain: no new occluders)
-            if d is not None and v.occlusion:
-                scene.set_hzb(v.slot, d)
-        scene.sync()
-        t0 = time.perf_counter()
-        scene.cut_phase2(two)
-        scene.sync()
-        t_phase2 += time.perf_counter() - t0
+        total = [a + b for a, b in zip(total, frame(True))]
+    ms = [x / steps * 1e3 for x in total]
     rejected_meshlets = sum(scene.phase_rejects(v.slot)[0] for v in two)
-    rejected_instances = 0
-    added = 0
+    rejected_instances = added = drawn = 0
+    raster_bytes = 0
+    pool_info = scene.pool(native.VG_POOL_MESHLET_INFO).reshape(-1, 4)
+    pool_tri = scene.pool(native.VG_POOL_MESHLET_TRIANGLES)
     for v in two:
-        c1 = scene.read(v.slot, max_entries=1)[0]
+        c1, _, draws, _ = scene.read(v.slot)
         c2, _ = scene.read_phase2(v.slot)
         rejected_instances += int(c1[1])
         added += int(c2[2]) - int(c1[2])
-    out = {"views_two_phase": len(two), "cut_ms_with_recording": t_cut / steps * 1e3, "phase2_ms": t_phase2 / steps * 1e3,
-           "instances_rejected": rejected_instances, "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added}
-    log(f"two-phase: cut {out['cut_ms_with_recording']:.2f} ms with recording, second phase {out['phase2_ms']:.2f} ms "
-        f"({rejected_instances} instances and {rejected_meshlets} meshlets re-tested, {added} clusters added)")
+        n = min(int(c1[2]), scene.ctx.config.max_meshlets)
+        drawn += n
+        # Per drawn camera entry: the entry, 4 matrix columns, meshlet records (first, count, vertex list),
+        # the vertex ids and positions, the packed triangles (vertex count = largest local id + 1).
+        m = draws[0, :n, 1].astype(np.int64)
+        info = pool_info[m]
+        has = info[:, 3] != 0xFFFFFFFF
+        tris = info[:, 0].astype(np.int64)
+        nv = np.zeros(n, dtype=np.int64)
+        for k in np.nonzero(has)[0]:
+            p = pool_tri[info[k, 1] // 3: info[k, 1] // 3 + tris[k]]
+            nv[k] = int(max((p & 255).max(), ((p >> 8) & 255).max(), ((p >> 16) & 255).max())) + 1
+        raster_bytes += int(n * (16 + 64 + 16) + (nv * 20).sum() + (tris[has] * 4).sum())
+    out = {"views_two_phase": len(two), "ms_per_frame": sum(ms), "raster_previous_ms": ms[0], "cut_ms_with_recording": ms[1],
+           "raster_first_phase_ms": ms[2], "phase2_ms": ms[3], "instances_rejected": rejected_instances,
+           "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added, "raster_entries_first_phase": drawn,
+           "raster_algorithmic_bytes_first_phase": raster_bytes,
+           "raster_first_phase_gbs": raster_bytes / (ms[2] * 1e-3) / 1e9 if ms[2] > 0 else None}
+    log(f"two-phase: {out['ms_per_frame']:.2f} ms per frame: raster of the last frame {ms[0]:.2f}, cut {ms[1]:.2f}, raster of "
+        f"its clusters {ms[2]:.2f} ({drawn} entries, {raster_bytes / 1e9:.2f} GB), second phase {ms[3]:.2f}; "
+        f"{rejected_instances} instances and {rejected_meshlets} meshlets re-tested, {added} clusters added")
     for v in views:
         scene.set_state(v.slot, initial)
     return out
@@ -468,13 +504,13 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda:0")
     t0 = time.time()
-    spec = scenes.world(mesh_count=args.meshes, instances=args.instances)
+    spec = scenes.world(mesh_count=args.meshes, instances=args.instances, terrain_grid=512)
     log(f"world: {spec.instance_count} instances, {len(spec.meshes)} meshes, "
         f"{sum(d.meshletCount for d in spec.meshes)} meshlets, {spec.full_detail_triangles / 1e9:.1f}B full-detail "
         f"triangles ({time.time() - t0:.1f} s)")
 
     if args.serve_only:
-        print(json.dumps({"serving": serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
+        print(json.dumps({"serving": serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
         return
     if args.expand_only:
         vd = mixed_views(spec, args.views, make_view, cam, scenegen)
@@ -484,7 +520,7 @@ def main():
 
     views_n = args.views
     ctx = native.Context(max_triangles=MAX_TRIANGLES, max_shadow_triangles=MAX_SHADOW_TRIANGLES, max_meshlets=args.max_meshlets,
-                         max_views=views_n, mesh_slots=64, device=0)
+                         max_views=views_n, mesh_slots=72, device=0)
     scene = native.Scene(ctx)
     t0 = time.time()
     scenes.load_into(scene, spec, occlusion=True)
@@ -708,7 +744,7 @@ def main():
         "cpu_baseline": cpu,
         "two_phase_occlusion": two_phase,
         "expand": expand,
-        "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes, make_view, cam, scenegen, SERVE_RATES,
+        "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES,
                                                                                args.serve_seconds),
     }
     print(json.dumps(out))

@@ -835,7 +835,8 @@ void vg_scene_destroy(vg_scene* s) {
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
                     &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
-                    &s->d_reject_meshlets, &s->d_phase};
+                    &s->d_reject_meshlets, &s->d_phase, &s->d_verts, &s->d_mverts, &s->d_vlist, &s->d_raster_chunks,
+                    &s->d_raster_count};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_indices) release(s, b);
@@ -1132,11 +1133,23 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
   r.height = height;
   r.depth = reinterpret_cast<uint32_t*>(hzb + kHzbHeader);
   std::memcpy(r.vp, view_proj, sizeof(r.vp));
+  constexpr uint32_t kChunkCapacity = 1u << 20;  // 32 MB of queued rectangles; beyond it the waves fill them
+  if (!s->d_raster_chunks.ptr) {
+    if (int rc = ensure(s, s->d_raster_chunks, sizeof(RasterChunk) * kChunkCapacity)) return rc;
+    if (int rc = ensure(s, s->d_raster_count, sizeof(uint32_t))) return rc;
+  }
+  r.chunks = static_cast<RasterChunk*>(s->d_raster_chunks.ptr);
+  r.chunk_count = static_cast<uint32_t*>(s->d_raster_count.ptr);
+  r.chunk_capacity = kChunkCapacity;
   hipLaunchKernelGGL(k_hzb_header, dim3(1), dim3(64), 0, ctx->stream, hzb, max_size, width, height);
   const uint32_t pixels = width * height;
-  hipLaunchKernelGGL(k_raster_clear, dim3(std::min<uint32_t>((pixels + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth, pixels);
-  if (!s->m_verts.empty())
-    hipLaunchKernelGGL(k_raster, dim3(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu), dim3(256), 0, ctx->stream, r);
+  hipLaunchKernelGGL(k_raster_clear, dim3(std::min<uint32_t>((pixels + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth, pixels,
+                     r.chunk_count);
+  if (!s->m_verts.empty()) {
+    const dim3 grid(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu);
+    hipLaunchKernelGGL(k_raster<kRasterSub>, grid, dim3(256), 0, ctx->stream, r);
+    hipLaunchKernelGGL(k_raster_chunks, grid, dim3(256), 0, ctx->stream, r);
+  }
   uint32_t lw = width, lh = height, levels = 0;
   std::vector<uint32_t> ws, hs;
   for (uint32_t dim = max_size;; dim = (dim + 1) / 2) {

@@ -110,6 +110,7 @@ struct vg_scene {
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
   vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs
   vgi::DevBuf d_verts, d_mverts, d_vlist;  // occluder raster inputs
+  vgi::DevBuf d_raster_chunks, d_raster_count;  // the raster's queue of large-triangle rectangles
   // per view slot
   vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;
   std::vector<vgi::DevBuf> d_draws, d_hzbs;

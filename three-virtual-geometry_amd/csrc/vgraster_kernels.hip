@@ -5,11 +5,14 @@
 //
 //   k_hzb_header  writes the HZB header (level offsets and sizes) for a width x height level 0
 //   k_raster_clear  level 0 <- 1.0 (WebGPU far depth)
-//   k_raster      one wave per drawn meshlet: the meshlet's vertices are transformed once into LDS
-//                 (instance matrix, then view_proj; ((c0 x + c1 y) + c2 z) + c3 in binary32 without
-//                 contraction), snapped to 1/256 pixel; each lane sets up its triangles (at most 2 of a
-//                 meshlet's 128); triangles whose pixel box holds at most kRasterLanePixels pixel centres
-//                 are filled by their lane, larger ones by the whole wave, one at a time. Depth is kept
+//   k_raster      kSub drawn meshlets per wave, one per group of 64 / kSub lanes: a meshlet's vertices are
+//                 transformed once into LDS (instance matrix, then view_proj; ((c0 x + c1 y) + c2 z) + c3
+//                 in binary32 without contraction), snapped to 1/256 pixel; each lane sets up its share of
+//                 the meshlet's triangles; triangles whose pixel box holds at most kRasterLanePixels pixel
+//                 centres are filled by their lane; larger ones are cut into kChunkSide^2 rectangles and
+//                 queued (by the whole lane group when the queue is full)
+//   k_raster_chunks  one block per queued rectangle, one pixel per thread: a triangle near the camera
+//                 covering much of the image is spread over the whole GPU instead of one wave Depth is kept
 //                 with atomicMin on the float bits (all values in [0, 1], so integer order is float order),
 //                 which makes the result independent of the order of the writes.
 #include "vgcut_kernels.h"
@@ -19,6 +22,8 @@ namespace vg {
 constexpr uint32_t kRasterLanePixels = 64;
 constexpr int32_t kRasterNoVertex = INT32_MIN;
 constexpr uint32_t kRasterBlocksPerCu = 8;
+constexpr uint32_t kRasterSub = 4;  // meshlets per wave
+constexpr int32_t kChunkSide = 64;    // large triangles are filled in rectangles of this side, one block each
 
 struct RasterParams {
   const float4* col[4];   // instance matrix columns
@@ -33,6 +38,9 @@ struct RasterParams {
   uint32_t clamp;
   uint32_t width, height;
   uint32_t* depth;        // level 0 of the HZB, float bits, row 0 at the top
+  struct RasterChunk* chunks;  // large triangles, cut into pixel rectangles of at most kChunkSide^2
+  uint32_t* chunk_count;
+  uint32_t chunk_capacity;
   float vp[16];           // column-major clip = vp * world
 };
 
@@ -109,6 +117,13 @@ __device__ __forceinline__ void raster_pixel(uint32_t* depth, uint32_t W, const 
   atomicMin(depth + static_cast<size_t>(y) * W + x, __float_as_uint(z));
 }
 
+// A rectangle of a large triangle's pixel box: filled by one block of k_raster_chunks.
+struct RasterChunk {
+  RasterVertex a, b, c;
+  int32_t area_lo, area_hi;
+  int32_t x0, x1, y0, y1;
+};
+
 __device__ __forceinline__ uint32_t raster_box_pixels(const RasterTriangle& t) {
   if (t.x0 > t.x1 || t.y0 > t.y1) return 0;
   return static_cast<uint32_t>(t.x1 - t.x0 + 1) * static_cast<uint32_t>(t.y1 - t.y0 + 1);
@@ -134,59 +149,127 @@ __global__ __launch_bounds__(64) void k_hzb_header(float* hzb, uint32_t max_size
   hzb[kHzbLevelCountSlot] = static_cast<float>(levels);
 }
 
-__global__ __launch_bounds__(256) void k_raster_clear(uint32_t* depth, uint32_t n) {
+__global__ __launch_bounds__(256) void k_raster_clear(uint32_t* depth, uint32_t n, uint32_t* chunk_count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *chunk_count = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) depth[i] = 0x3f800000u;
 }
 
+// kSub meshlets per wave, each on 64 / kSub consecutive lanes (a lane group): every load of a meshlet (its
+// vertex ids, positions and packed triangles) is issued before any is used, and the groups' meshlets are in
+// flight together, so the dependent chain entry -> vertex list -> positions costs one wait per kSub meshlets.
+template <uint32_t kSub>
 __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
-  __shared__ RasterVertex lds[4][128];
-  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-  RasterVertex* V = lds[wave];
+  constexpr uint32_t kLanes = 64 / kSub, kPer = 128 / kLanes;
+  __shared__ RasterVertex lds[4][kSub][128];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, group = lane / kLanes, gl = lane % kLanes;
+  RasterVertex* V = lds[wave][group];
+  const uint64_t group_mask = (kSub == 1 ? ~0ull : ((1ull << kLanes) - 1ull)) << (group * kLanes);
   const uint32_t n = min(*r.count, r.clamp);
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t slots = gridDim.x * (blockDim.x >> 6) * kSub;
   auto wave_sync = [&]() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  for (uint32_t e = xcd_wave_rank(); e < n; e += waves) {
-    const uint4 entry = r.draws[e];
-    const uint32_t instance = uniform(entry.x), meshlet = uniform(entry.y);
-    const uint2 vl = r.vlist[meshlet];
-    const uint32_t nv = uniform(vl.y);
-    if (nv == 0) continue;
-    const uint32_t tris = uniform(r.tris[meshlet]), first = uniform(r.first[meshlet]) / 3;
-    const float4 c0 = r.col[0][instance], c1 = r.col[1][instance], c2 = r.col[2][instance], c3 = r.col[3][instance];
-    for (uint32_t k = lane; k < nv; k += 64) V[k] = raster_vertex(r, c0, c1, c2, c3, r.verts[r.mverts[vl.x + k]]);
+  for (uint32_t base = xcd_wave_rank() * kSub; base < n; base += slots) {
+    const uint32_t e = base + group;
+    uint32_t nv = 0, tris = 0, first = 0, voff = 0, instance = 0;
+    if (e < n) {
+      const uint4 entry = r.draws[e];
+      instance = entry.x;
+      const uint2 vl = r.vlist[entry.y];
+      voff = vl.x;
+      nv = vl.y;
+      tris = r.tris[entry.y];
+      first = r.first[entry.y] / 3;
+    }
+    if (nv == 0) tris = 0;
+    uint32_t ids[kPer], packed[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+      const uint32_t k = gl + j * kLanes;
+      ids[j] = k < nv ? r.mverts[voff + k] : 0u;
+      packed[j] = k < tris ? r.tri[first + k] : 0u;
+    }
+    float4 pos[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) pos[j] = gl + j * kLanes < nv ? r.verts[ids[j]] : make_float4(0, 0, 0, 0);
+    if (nv) {
+      const float4 c0 = r.col[0][instance], c1 = r.col[1][instance], c2 = r.col[2][instance], c3 = r.col[3][instance];
+#pragma unroll
+      for (uint32_t j = 0; j < kPer; j++)
+        if (gl + j * kLanes < nv) V[gl + j * kLanes] = raster_vertex(r, c0, c1, c2, c3, pos[j]);
+    }
     wave_sync();
-    bool big[2] = {false, false};
-    for (uint32_t s = 0; s < 2; s++) {
-      const uint32_t t = lane + s * 64;
-      if (t >= tris) continue;
-      const uint32_t packed = r.tri[first + t];
-      const RasterTriangle tr = raster_setup(V[packed & 255u], V[(packed >> 8) & 255u], V[(packed >> 16) & 255u], r.width, r.height);
+    bool big[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+      big[j] = false;
+      if (gl + j * kLanes >= tris) continue;
+      const uint32_t p = packed[j];
+      const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], r.width, r.height);
       const uint32_t pixels = raster_box_pixels(tr);
       if (pixels > kRasterLanePixels) {
-        big[s] = true;
+        // Large: its kChunkSide^2 rectangles go to the chunk queue (filled by every block of k_raster_chunks);
+        // when the queue is full, the lane group fills it below.
+        const uint32_t cw = static_cast<uint32_t>(tr.x1 - tr.x0) / kChunkSide + 1, ch = static_cast<uint32_t>(tr.y1 - tr.y0) / kChunkSide + 1;
+        const uint32_t at = atomicAdd(r.chunk_count, cw * ch);
+        if (at + cw * ch > r.chunk_capacity) {
+          big[j] = true;
+          continue;
+        }
+        for (uint32_t q = 0; q < cw * ch; q++) {
+          RasterChunk c;
+          c.a = tr.a;
+          c.b = tr.b;
+          c.c = tr.c;
+          c.area_lo = static_cast<int32_t>(static_cast<uint64_t>(tr.area));
+          c.area_hi = static_cast<int32_t>(static_cast<uint64_t>(tr.area) >> 32);
+          c.x0 = tr.x0 + static_cast<int32_t>(q % cw) * kChunkSide;
+          c.y0 = tr.y0 + static_cast<int32_t>(q / cw) * kChunkSide;
+          c.x1 = min(c.x0 + kChunkSide - 1, tr.x1);
+          c.y1 = min(c.y0 + kChunkSide - 1, tr.y1);
+          r.chunks[at + q] = c;
+        }
         continue;
       }
       for (int32_t y = tr.y0; y <= tr.y1; y++)
         for (int32_t x = tr.x0; x <= tr.x1; x++) raster_pixel(r.depth, r.width, tr, x, y);
     }
-    // Large triangles: the whole wave strides over each one's pixel box.
-    for (uint32_t s = 0; s < 2; s++) {
-      uint64_t mask = __ballot(big[s]);
+    // Large triangles the queue had no room for: each lane group strides over its meshlet's, one at a time.
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+      uint64_t mask = __ballot(big[j]) & group_mask;
       while (mask) {
-        const uint32_t t = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1) + s * 64;
+        const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
         mask &= mask - 1;
-        const uint32_t packed = uniform(r.tri[first + t]);
-        const RasterTriangle tr = raster_setup(V[packed & 255u], V[(packed >> 8) & 255u], V[(packed >> 16) & 255u], r.width, r.height);
+        const uint32_t p = static_cast<uint32_t>(__shfl(static_cast<int>(packed[j]), static_cast<int>(src)));
+        const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], r.width, r.height);
         const uint32_t bw = static_cast<uint32_t>(tr.x1 - tr.x0 + 1), pixels = raster_box_pixels(tr);
-        for (uint32_t i = lane; i < pixels; i += 64)
+        for (uint32_t i = gl; i < pixels; i += kLanes)
           raster_pixel(r.depth, r.width, tr, tr.x0 + static_cast<int32_t>(i % bw), tr.y0 + static_cast<int32_t>(i / bw));
       }
     }
     wave_sync();
+  }
+}
+
+// The queued rectangles of large triangles, one block each: 16 x 16 pixel tiles, one pixel per thread.
+__global__ __launch_bounds__(256) void k_raster_chunks(RasterParams r) {
+  const uint32_t n = min(*r.chunk_count, r.chunk_capacity);
+  const int32_t tx = static_cast<int32_t>(threadIdx.x & 15u), ty = static_cast<int32_t>(threadIdx.x >> 4);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const RasterChunk c = r.chunks[i];
+    RasterTriangle t;
+    t.a = c.a;
+    t.b = c.b;
+    t.c = c.c;
+    t.area = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(c.area_hi)) << 32) | static_cast<uint32_t>(c.area_lo));
+    for (int32_t y0 = c.y0; y0 <= c.y1; y0 += 16)
+      for (int32_t x0 = c.x0; x0 <= c.x1; x0 += 16) {
+        const int32_t x = x0 + tx, y = y0 + ty;
+        if (x <= c.x1 && y <= c.y1) raster_pixel(r.depth, r.width, t, x, y);
+      }
   }
 }
 

@@ -3,7 +3,8 @@
 icosphere_view  one icosphere of 1,003,520 triangles, one identity instance, one 1080p camera at 3R
 ruins    procedural props (boxes, columns, arches, spheres) scattered uniformly over a square
 world    props placed by Bridson Poisson-disk sampling over a noise heightfield, with per-mesh
-         draw distances
+         draw distances, and (terrain_grid > 0) the heightfield itself as one more mesh with one instance,
+         so that drawing the previous frame's visible set gives ground occluders too
 Every generator takes its sizes as arguments so tests can build small versions of the same scenes.
 """
 from __future__ import annotations
@@ -26,6 +27,7 @@ class SceneSpec:
     min_pixel_radius: List[float] = field(default_factory=list)
     cast_shadow: List[bool] = field(default_factory=list)
     heightfield: Optional[dict] = None  # amplitude, frequency, seed of the ground (placement and occluder depth)
+    props: Optional[int] = None  # meshes[:props] are the props (placed instances); later ones (the terrain) are not
 
     @property
     def instance_count(self) -> int:
@@ -128,10 +130,28 @@ def ground_poses(spec: SceneSpec, count: int, half_extent: float, eye_height: fl
     return poses
 
 
+def terrain_mesh(size: float, amplitude: float, frequency: float, seed: int, grid: int):
+    """The noise heightfield over [-size/2, size/2]^2 as a grid of grid x grid quads, DAG-built like the props
+    (two triangles per quad, facing +y)."""
+    from . import dag
+
+    xs = np.linspace(-size / 2, size / 2, grid + 1, dtype=np.float64)
+    gx, gz = np.meshgrid(xs, xs, indexing="ij")  # vertex (i, j) at x = xs[i], z = xs[j]
+    xz = np.stack([gx.ravel(), gz.ravel()], 1).astype(np.float32)
+    y = scenegen.heightfield(xz, amplitude, frequency, seed)
+    pos = np.stack([xz[:, 0], y, xz[:, 1]], 1).astype(np.float32)
+    i, j = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    v00 = (i * (grid + 1) + j).ravel()
+    v01, v10, v11 = v00 + 1, v00 + grid + 1, v00 + grid + 2
+    idx = np.stack([v00, v01, v11, v00, v11, v10], 1).astype(np.uint32).ravel()
+    return dag.build_virtual_mesh(pos, idx)
+
+
 def world(mesh_count=64, tri_range=(100_000, 2_000_000), instances=3_000_000, size=6144.0, amplitude=200.0,
-          draw_range=(200.0, 6000.0), seed=6, meshes=None) -> SceneSpec:
+          draw_range=(200.0, 6000.0), seed=6, meshes=None, terrain_grid=0) -> SceneSpec:
     """Props by Bridson Poisson-disk sampling over a size x size noise heightfield (y = terrain height).
-    `meshes`: a prop set to reuse (built from `seed` otherwise)."""
+    `meshes`: a prop set to reuse (built from `seed` otherwise). `terrain_grid` > 0 adds the heightfield
+    as one more mesh (terrain_grid^2 quads, one identity instance, no draw distance)."""
     rng = np.random.default_rng(seed)
     tris = log_uniform(rng, tri_range[0], tri_range[1], mesh_count).astype(np.int64)
     if meshes is None:
@@ -156,7 +176,14 @@ def world(mesh_count=64, tri_range=(100_000, 2_000_000), instances=3_000_000, si
     mats = [all_m[which == k] for k in range(mesh_count)]
     drng = np.random.default_rng(seed + 2000)
     draw = list(drng.uniform(draw_range[0], draw_range[1], mesh_count))
-    return SceneSpec(meshes, mats, draw, [0.0] * mesh_count, [False] * mesh_count, heightfield=hf)
+    spec = SceneSpec(list(meshes), mats, draw, [0.0] * mesh_count, [False] * mesh_count, heightfield=hf, props=mesh_count)
+    if terrain_grid > 0:
+        spec.meshes.append(terrain_mesh(size, amplitude, hf["frequency"], hf["seed"], terrain_grid))
+        spec.matrices.append(np.eye(4, dtype=np.float32).reshape(1, 16))
+        spec.max_draw_distance.append(1e30)
+        spec.min_pixel_radius.append(0.0)
+        spec.cast_shadow.append(False)
+    return spec
 
 
 def flythrough(spec: SceneSpec, frames: int, size: float, height: float, seed: int):
