@@ -45,6 +45,7 @@ REF_MESHLETS = 1 << 20
 
 TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world64_rocprof.json")
 EXPAND_TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world64_expand_rocprof.json")
+RASTER_TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "world24_raster_rocprof.json")
 
 
 def measured_traffic(kernel: str, path: str = TRAFFIC_SUMMARY):
@@ -86,6 +87,7 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="requests per rate = rate x this (at least 1000)")
     ap.add_argument("--no-expand", action="store_true", help="skip the expand-pass measurement")
     ap.add_argument("--expand-only", action="store_true", help="only the expand-pass measurement (with --profile-only: for rocprofv3)")
+    ap.add_argument("--two-phase-only", action="store_true", help="only the two-phase occlusion frames with their rasters (for rocprofv3)")
     ap.add_argument("--schedule", type=str, default=None,
                     help="fixed launch schedule 'instance_blocks,meshlet_blocks,meshlet_waves' (per CU) instead of tuning")
     return ap.parse_args()
@@ -397,7 +399,8 @@ def two_phase_bench(scene, vd, steps):
     on each view with an HZB (occlusion = 2), the occluder raster draws the slot's last cut, both phases, with
     the camera into its HZB (vg_hzb_from_cut, on the device); the cut records its camera occlusion rejects;
     the raster draws that cut's clusters; the second phase re-tests the rejects against them. Reports each
-    step's time, the rasters' share, what was re-tested and added, and the raster's algorithmic bytes."""
+    step's time, the rasters' share, what was re-tested and added, and the raster kernels' roofline (their
+    event-timed duration, algorithmic bytes, HBM traffic from the committed rocprof summary)."""
     import ctypes
 
     import torch
@@ -444,42 +447,65 @@ def two_phase_bench(scene, vd, steps):
     for _ in range(2):
         frame(False)
     total = [0.0, 0.0, 0.0, 0.0]
+    scene.profile_read()
+    scene.profile(True)
     for _ in range(steps):
         for v in views:
             scene.set_state(v.slot, initial)
         torch.cuda.synchronize()
         total = [a + b for a, b in zip(total, frame(True))]
+    kernels, _ = scene.profile_read()
+    scene.profile(False)
+    raster_kernel_ms = dict(kernels)["k_raster (+ chunks)"] / steps  # both rasters of a frame
     ms = [x / steps * 1e3 for x in total]
     rejected_meshlets = sum(scene.phase_rejects(v.slot)[0] for v in two)
     rejected_instances = added = drawn = 0
     raster_bytes = 0
     pool_info = scene.pool(native.VG_POOL_MESHLET_INFO).reshape(-1, 4)
     pool_tri = scene.pool(native.VG_POOL_MESHLET_TRIANGLES)
+
+    def entry_bytes(meshlets):
+        # Per drawn camera entry: the entry, 4 matrix columns, meshlet records (first, count, vertex list),
+        # the vertex ids and positions, the packed triangles (vertex count = largest local id + 1).
+        info = pool_info[meshlets]
+        has = info[:, 3] != 0xFFFFFFFF
+        tris = info[:, 0].astype(np.int64)
+        nv = np.zeros(len(meshlets), dtype=np.int64)
+        for k in np.nonzero(has)[0]:
+            p = pool_tri[info[k, 1] // 3: info[k, 1] // 3 + tris[k]]
+            nv[k] = int(max((p & 255).max(), ((p >> 8) & 255).max(), ((p >> 16) & 255).max())) + 1
+        return int(len(meshlets) * (16 + 64 + 16) + (nv * 20).sum() + (tris[has] * 4).sum())
+
     for v in two:
         c1, _, draws, _ = scene.read(v.slot)
         c2, _ = scene.read_phase2(v.slot)
         rejected_instances += int(c1[1])
         added += int(c2[2]) - int(c1[2])
-        n = min(int(c1[2]), scene.ctx.config.max_meshlets)
-        drawn += n
-        # Per drawn camera entry: the entry, 4 matrix columns, meshlet records (first, count, vertex list),
-        # the vertex ids and positions, the packed triangles (vertex count = largest local id + 1).
-        m = draws[0, :n, 1].astype(np.int64)
-        info = pool_info[m]
-        has = info[:, 3] != 0xFFFFFFFF
-        tris = info[:, 0].astype(np.int64)
-        nv = np.zeros(n, dtype=np.int64)
-        for k in np.nonzero(has)[0]:
-            p = pool_tri[info[k, 1] // 3: info[k, 1] // 3 + tris[k]]
-            nv[k] = int(max((p & 255).max(), ((p >> 8) & 255).max(), ((p >> 16) & 255).max())) + 1
-        raster_bytes += int(n * (16 + 64 + 16) + (nv * 20).sum() + (tris[has] * 4).sum())
+        mm = scene.ctx.config.max_meshlets
+        n1, n2 = min(int(c1[2]), mm), min(int(c2[2]), mm)
+        drawn += n1
+        # A frame's two rasters: the previous frame's cut (both phases; the views and the controller state
+        # repeat, so it equals this frame's) and this frame's first phase.
+        raster_bytes += entry_bytes(draws[0, :n2, 1].astype(np.int64)) + entry_bytes(draws[0, :n1, 1].astype(np.int64))
+    achieved = raster_bytes / (raster_kernel_ms * 1e-3) / 1e9 if raster_kernel_ms > 0 else None
+    traffic, source = 0.0, []
+    for name in ("k_raster", "k_raster_chunks"):
+        t, src = measured_traffic(name, RASTER_TRAFFIC_SUMMARY)
+        if t is None:
+            traffic, source = None, [src]
+            break
+        traffic += t * 2 * len(rasters)  # per dispatch; two rasters per view per frame
+        source.append(src)
     out = {"views_two_phase": len(two), "ms_per_frame": sum(ms), "raster_previous_ms": ms[0], "cut_ms_with_recording": ms[1],
            "raster_first_phase_ms": ms[2], "phase2_ms": ms[3], "instances_rejected": rejected_instances,
            "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added, "raster_entries_first_phase": drawn,
-           "raster_algorithmic_bytes_first_phase": raster_bytes,
-           "raster_first_phase_gbs": raster_bytes / (ms[2] * 1e-3) / 1e9 if ms[2] > 0 else None}
+           "raster": {"kernel_ms_per_frame": raster_kernel_ms, "rasters_per_frame": 2 * len(rasters),
+                      "roofline": {"bound": "hbm", "kernel": "k_raster + k_raster_chunks", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+                                   "algorithmic_bytes": raster_bytes, "traffic": traffic, "traffic_source": "; ".join(source)}}}
     log(f"two-phase: {out['ms_per_frame']:.2f} ms per frame: raster of the last frame {ms[0]:.2f}, cut {ms[1]:.2f}, raster of "
-        f"its clusters {ms[2]:.2f} ({drawn} entries, {raster_bytes / 1e9:.2f} GB), second phase {ms[3]:.2f}; "
+        f"its clusters {ms[2]:.2f} ({drawn} entries), second phase {ms[3]:.2f}; raster kernels {raster_kernel_ms:.2f} ms per frame, "
+        f"{raster_bytes / 1e9:.2f} GB algorithmic ({'-' if achieved is None else f'{achieved:.0f}'} GB/s); "
         f"{rejected_instances} instances and {rejected_meshlets} meshlets re-tested, {added} clusters added")
     for v in views:
         scene.set_state(v.slot, initial)
@@ -530,6 +556,15 @@ def main():
         if depth is not None:
             scene.set_hzb(v.slot, depth)
     log(f"scene loaded, {sum(d is not None for _, d in vd)} HZBs ({time.time() - t0:.1f} s)")
+    if args.two_phase_only:
+        # A plain 1 GiB device copy first: its FETCH_SIZE / WRITE_SIZE calibrate the counters' units.
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(json.dumps({"two_phase_occlusion": two_phase_bench(scene, vd, args.steps), "copy_bytes": 2 * (1 << 28) * 4}))
+        return
 
     initial = np.zeros(8, dtype=np.uint32)
     initial[0] = initial[1] = 10000

@@ -238,8 +238,9 @@ int vg_mesh_instance_base(const vg_scene* scene, uint32_t slot, uint32_t* base, 
 
 /*
  * Measurement. vg_profile(scene, 1) times every kernel of eager launch sequences with HIP events on the
- * context's stream; vg_profile_read returns the summed milliseconds per kernel position (12 positions,
- * `names` receives their names), the number of sequences, and resets them. vg_footprint runs one cut
+ * context's stream, and the raster kernels of vg_hzb_from_cut; vg_profile_read returns the summed
+ * milliseconds per kernel position (13 positions, `names` receives their names), the number of cut
+ * launch sequences, and resets them. vg_footprint runs one cut
  * (like vg_cut, then waits) and counts the distinct instances tested and distinct meshlet records in
  * candidate ranges over all views (the unique inputs one launch must read).
  */

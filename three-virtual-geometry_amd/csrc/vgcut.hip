@@ -412,11 +412,12 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   return p;
 }
 
-constexpr int kKernelSlots = 12;
+constexpr int kKernelSlots = 13;
+constexpr int kRasterSlot = 12;  // vg_hzb_from_cut's raster kernels, outside the cut's launch sequence
 const char* const kKernelNames[kKernelSlots] = {"k_begin", "k_cells (+ budget passes)", "k_instances", "k_meshlets", "k_prefix(check 1)",
                                                 "k_instances(retry 1)", "k_meshlets(retry 1)", "k_prefix(check 2)",
                                                 "k_instances(retry 2)", "k_meshlets(retry 2)", "k_prefix(steer)",
-                                                "k_expand (+ plan, resolve)"};
+                                                "k_expand (+ plan, resolve)", "k_raster (+ chunks)"};
 
 hipEvent_t take_event(vg_scene* s) {
   if (!s->event_pool.empty()) {
@@ -460,8 +461,10 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
   std::vector<hipEvent_t> ev;
   const bool timed = s->profile && !capturing && lane == &s->lane;
-  if (timed)
-    for (int i = 0; i <= kKernelSlots; i++) ev.push_back(take_event(s));
+  if (timed) {
+    for (int i = 0; i < kKernelSlots; i++) ev.push_back(take_event(s));
+    ev.push_back(nullptr);  // the raster's slot is timed by vg_hzb_from_cut
+  }
   int k = 0;
   auto mark = [&]() {
     if (timed && ev[k]) (void)hipEventRecord(ev[k], st);
@@ -1147,8 +1150,19 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
                      r.chunk_count);
   if (!s->m_verts.empty()) {
     const dim3 grid(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu);
+    std::vector<hipEvent_t> ev;
+    if (s->profile) {
+      ev.assign(kKernelSlots + 1, nullptr);
+      ev[kRasterSlot] = take_event(s);
+      ev[kRasterSlot + 1] = take_event(s);
+      if (ev[kRasterSlot]) (void)hipEventRecord(ev[kRasterSlot], ctx->stream);
+    }
     hipLaunchKernelGGL(k_raster<kRasterSub>, grid, dim3(256), 0, ctx->stream, r);
     hipLaunchKernelGGL(k_raster_chunks, grid, dim3(256), 0, ctx->stream, r);
+    if (s->profile) {
+      if (ev[kRasterSlot + 1]) (void)hipEventRecord(ev[kRasterSlot + 1], ctx->stream);
+      s->pending.push_back(ev);
+    }
   }
   uint32_t lw = width, lh = height, levels = 0;
   std::vector<uint32_t> ws, hs;
@@ -1429,7 +1443,7 @@ int vg_profile_read(vg_scene* s, double* kernel_ms, uint32_t max_kernels, uint64
     }
     for (hipEvent_t e : seq)
       if (e) s->event_pool.push_back(e);
-    s->profiled_sequences++;
+    if (seq[0]) s->profiled_sequences++;  // launch sequences of cuts (raster records time only its own slot)
   }
   s->pending.clear();
   for (uint32_t i = 0; i < max_kernels && i < static_cast<uint32_t>(kKernelSlots); i++) {
