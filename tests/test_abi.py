@@ -33,6 +33,18 @@ def test_library_exports_every_declared_symbol():
     assert declared_functions() <= exported
 
 
+def test_dag_library_exports_every_declared_symbol():
+    """include/vgdag.h (the DAG builder's C ABI) against lib/libvgdag.so and the Python binding."""
+    from vgcut import _native, dag
+
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "vgdag.h")).read(), flags=re.S)
+    declared = set(re.findall(r"\b(vgd_[a-z0-9_]+)\s*\(", text))
+    assert declared == set(dag.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", _native.library_path("libvgdag.so")], capture_output=True, text=True,
+                         check=True).stdout
+    assert declared <= set(re.findall(r" T (vgd_[a-z0-9_]+)", out))
+
+
 def test_host_only_context_and_defaults():
     ctx = native.Context(device=-1)
     assert ctx.counter_words == 12 + 8 * 64 + 8

@@ -18,7 +18,7 @@
 //    target, each cluster snapped to its vertex nearest the cluster mean; error = largest snap distance;
 //  - sphere bounds (meshopt computeSphereBounds): an axis-extremal pair grown over every sphere, then made
 //    exact by the reference's containment pass (buildVirtualMesh.ts:746-762).
-#include "vgdag.h"
+#include "../../include/vgdag.h"
 
 #include <algorithm>
 #include <array>

@@ -1,4 +1,4 @@
-"""Meshlet LOD DAG builder for indexed triangle meshes (csrc/vgdag.cpp).
+"""Meshlet LOD DAG builder for indexed triangle meshes (lib/libvgdag.so, include/vgdag.h).
 
 Python face of the reference's ``buildVirtualMesh(src, options)`` (src/core/preprocess/buildVirtualMesh.ts:185)
 and ``partitionMeshlets(meshlets, groupSize)`` (src/core/preprocess/partition.ts:16), with the reference's
@@ -62,12 +62,13 @@ class _View(ctypes.Structure):
 
 
 _lib = None
+EXPORTS = ("vgd_default_options", "vgd_build", "vgd_mesh_get", "vgd_mesh_free", "vgd_partition")
 
 
 def _load():
     global _lib
     if _lib is None:
-        lib = ctypes.CDLL(_native.library_path("libvgscene.so"))
+        lib = ctypes.CDLL(_native.library_path("libvgdag.so"))
         lib.vgd_default_options.argtypes = [ctypes.POINTER(_Options)]
         lib.vgd_default_options.restype = None
         lib.vgd_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Options),
