@@ -314,10 +314,12 @@ __device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
 // address per wave. Returns the lane's old value (the group base plus its exclusive prefix). All 64
 // lanes must call it together.
 __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t value, bool active) {
-  uint32_t result = 0;
   uint64_t pending = __ballot(active);
   const uint32_t lane = lane_id();
   const uint64_t a = reinterpret_cast<uint64_t>(addr);
+  // Every group's atomic is issued before any result is read: the groups' round trips overlap, and each
+  // lane then reads its group leader's result with one lane shuffle.
+  uint32_t leader_base = 0, my_leader = 0, my_offset = 0;
   while (pending) {
     int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
     uint32_t lo = __shfl(static_cast<uint32_t>(a), leader, 64);
@@ -328,13 +330,15 @@ __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t val
     uint32_t v = mine ? value : 0u;
     uint32_t incl = wave_scan(v);
     uint32_t total = __shfl(incl, 63, 64);
-    uint32_t base = 0;
-    if (lane == static_cast<uint32_t>(leader) && total) base = atomicAdd(reinterpret_cast<uint32_t*>(la), total);
-    base = __shfl(base, leader, 64);
-    if (mine) result = base + incl - v;
+    if (lane == static_cast<uint32_t>(leader) && total) leader_base = atomicAdd(reinterpret_cast<uint32_t*>(la), total);
+    if (mine) {
+      my_leader = static_cast<uint32_t>(leader);
+      my_offset = incl - v;
+    }
     pending &= ~group;
   }
-  return result;
+  const uint32_t base = __shfl(leader_base, static_cast<int>(my_leader), 64);
+  return active ? base + my_offset : 0u;
 }
 
 // One atomicAdd per wave on a wave-uniform address; returns each active lane's old value.
