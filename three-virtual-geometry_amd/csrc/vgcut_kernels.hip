@@ -122,10 +122,25 @@ __device__ __forceinline__ float projected_error(const V& v, const float4& c0, c
 
 __device__ __forceinline__ uint32_t ceil_log2(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
+// The HZB level table row (offset, width, height, unused) of a level, read from the header: rows are 16 bytes
+// and 16-byte aligned, so one load per lane.
+struct HzbLevels {
+  const float* __restrict__ hzb;
+  __device__ __forceinline__ float4 operator()(uint32_t level) const { return reinterpret_cast<const float4*>(hzb)[level]; }
+};
+
+// The same rows from a wave's own copy of the table in LDS (loaded alongside the wave's other inputs, so the
+// test needs one dependent global round trip, for the texels, instead of two).
+struct HzbLevelsLds {
+  const float4* table;
+  __device__ __forceinline__ float4 operator()(uint32_t level) const { return table[level]; }
+};
+
 // occluded, OcclusionCulling.ts:119-170. `hzb` is only read when the sphere is in front of the near plane.
-// The HZB header (sizes, level table) is read as wave-uniform data, the texels per lane.
-template <typename V>
-__device__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, float vy, float vz, float radius) {
+// The HZB header (sizes, level count) is read as wave-uniform data, the level rows and texels per lane.
+template <typename V, typename Levels>
+__device__ bool occluded_by(const V& v, const float* __restrict__ hzb, const Levels& levels, float vx, float vy, float vz,
+                            float radius) {
   const auto header = scalar(hzb);
   float depth = -vz;
   float near_depth = depth - radius;
@@ -151,7 +166,8 @@ __device__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, fl
   // fx / 2^level == fx * 2^-level exactly (fx is a texel coordinate in [0, 4096): no underflow), so the
   // division by the mip scale is a multiplication by its exact reciprocal.
   const float inv_scale = __builtin_ldexpf(1.0f, -static_cast<int>(level));
-  float offset = hzb[level * 4], lw = hzb[level * 4 + 1], lh = hzb[level * 4 + 2];  // level differs per lane
+  const float4 row = levels(level);  // level differs per lane
+  const float offset = row.x, lw = row.y, lh = row.z;
   auto sample = [&](float fx, float fy) {
     float tx = clampf(floorf(fx * inv_scale), 0.0f, lw - 1.0f);
     float ty = clampf(floorf(fy * inv_scale), 0.0f, lh - 1.0f);
@@ -161,14 +177,25 @@ __device__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, fl
   return min_z < 1.0f && min_z > m + 1e-6f;
 }
 
+template <typename V>
+__device__ __forceinline__ bool occluded(const V& v, const float* __restrict__ hzb, float vx, float vy, float vz, float radius) {
+  return occluded_by(v, hzb, HzbLevels{hzb}, vx, vy, vz, radius);
+}
+
 // occludedSphere, GeometryPool.ts:701-705.
+template <typename V, typename Levels>
+__device__ __forceinline__ bool occluded_sphere_by(const Params& p, const V& v, const Levels& levels, float cx, float cy,
+                                                   float cz, float radius) {
+  if (!v.hzb_valid) return false;
+  float4 vc = mat_mul16(v.view, cx, cy, cz, 1.0f);
+  float margin = (fabsf(vc.z) * v.high[0]) / v.proj_scale;
+  return occluded_by(v, scalar(p.hzbs)[v.slot], levels, vc.x, vc.y, vc.z, radius + margin);
+}
+
 template <typename V>
 __device__ __forceinline__ bool occluded_sphere(const Params& p, const V& v, bool on, float cx, float cy, float cz,
                                                 float radius) {
-  if (!(on && v.hzb_valid)) return false;
-  float4 vc = mat_mul16(v.view, cx, cy, cz, 1.0f);
-  float margin = (fabsf(vc.z) * v.high[0]) / v.proj_scale;
-  return occluded(v, scalar(p.hzbs)[v.slot], vc.x, vc.y, vc.z, radius + margin);
+  return on && occluded_sphere_by(p, v, HzbLevels{scalar(p.hzbs)[v.slot]}, cx, cy, cz, radius);
 }
 
 // ------------------------------------------------------------------ 0. begin (resetNode, GeometryPool.ts:714-725)
@@ -736,6 +763,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
   // instance scale.
   __shared__ uint4 ring[4][kQueue];
   __shared__ float ring_scale[4][kQueue];
+  // Per wave: the HZB level table of the view whose meshlets are being tested (one float per lane to copy).
+  static_assert(kHzbMaxLevels * 4 == 64, "the level table is copied one float per lane");
+  __shared__ float4 hzb_rows[4][kHzbMaxLevels];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   uint32_t head = 0, tail = 0;  // wave-uniform ring positions
   // Occlusion tests of the current view slot, added to its metric when the slot changes (and at the end).
@@ -743,6 +773,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
   auto occ_flush = [&]() {
     if (occ_count && lane == 0) atomicAdd(metric_at(p, occ_slot, M_OCCLUSION), static_cast<unsigned long long>(occ_count));
     occ_count = 0;
+  };
+
+  auto wave_sync = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
   auto drain = [&](uint32_t n) {
@@ -757,21 +793,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     bool drawn = active && fast;  // fast path: guaranteed meshlets of a culling sphere inside the frustum
     bool occ_test = false, rejected = false;
     uint32_t vslot = 0;
+    // The instance columns and cull sphere of every tested lane, loaded before the per-view loop so that
+    // their latencies overlap (and are paid once however many views the wave holds).
+    const bool tested = active && !fast;
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, cs = c0;
+    if (tested) {
+      c0 = p.i_col0[instance];
+      c1 = p.i_col1[instance];
+      c2 = p.i_col2[instance];
+      c3 = p.i_col3[instance];
+      cs = p.m_cull[meshlet];
+    }
     // Visibility, one launched view at a time (its constants are wave-uniform there).
     for (uint64_t pending = __ballot(active); pending;) {
       const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
       const bool mine = active && vi == vu;
+      // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
+      const auto& v = scalar(p.views)[uniform(vu)];
+      if (v.hzb_valid && __ballot(mine && tested && cut == 0 && occ_on)) {
+        // Copy the view's HZB level table to the wave's LDS rows (all lanes: the table is 64 floats).
+        reinterpret_cast<float*>(hzb_rows[wib])[lane] = scalar(p.hzbs)[v.slot][lane];
+        wave_sync();
+      }
       if (mine) {
-        // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
-        const auto& v = scalar(p.views)[uniform(vu)];
         vslot = v.slot;
         if (!fast) {
-          const float4 c0 = p.i_col0[instance], c1 = p.i_col1[instance], c2 = p.i_col2[instance], c3 = p.i_col3[instance];
-          const float4 cs = p.m_cull[meshlet];
           const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
           bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, cut == 1 ? 1 : 0, false) >= -(cs.w * scale);
           occ_test = cut == 0 && visible && occ_on && v.hzb_valid;
-          if (occ_test) visible = !occluded_sphere(p, v, true, center.x, center.y, center.z, cs.w * scale);
+          if (occ_test) visible = !occluded_sphere_by(p, v, HzbLevelsLds{hzb_rows[wib]}, center.x, center.y, center.z, cs.w * scale);
           rejected = occ_test && !visible && v.two_phase && p.phase == 0;
           drawn = visible;
         }
@@ -805,11 +855,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
     if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(instance, meshlet, first_index, packed_tris);
   };
-  auto wave_sync = [&]() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  };
 
   for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t task) {
     const uint32_t cut = (w.w & kTagShadow) ? 1u : 0u, vi = w.w >> kTagViewShift;
@@ -822,19 +867,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       first = task;
       count = 1;
     }
+    // The run's records and its instance's columns, loaded before the per-view loop so that their latencies
+    // overlap (the span too: loading it only for runs in the cut would add a dependent round trip).
+    const bool run_test = valid && !fast;
+    const uint32_t run = task;
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, lod_sphere = c0, parent_sphere = c0;
+    float2 err = make_float2(0.0f, 0.0f);
+    uint2 span = make_uint2(0, 0);
+    if (run_test) {
+      c0 = p.i_col0[w.x];
+      c1 = p.i_col1[w.x];
+      c2 = p.i_col2[w.x];
+      c3 = p.i_col3[w.x];
+      scale = p.work_scale[item];
+      err = p.r_err[run];
+      lod_sphere = p.r_lod[run];
+      parent_sphere = p.r_parent[run];
+      span = p.r_span[run];
+    }
     // The LOD test of a run, one launched view at a time (its constants are wave-uniform there).
-    for (uint64_t pending = __ballot(valid && !fast); pending;) {
+    for (uint64_t pending = __ballot(run_test); pending;) {
       const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
-      const bool mine = valid && !fast && vi == vu;
+      const bool mine = run_test && vi == vu;
       if (mine) {
         // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
         const auto& v = scalar(p.views)[uniform(vu)];
-        const uint32_t run = task;
-        const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-        scale = p.work_scale[item];
-        const float2 err = p.r_err[run];
-        const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
-        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
+        const float own = projected_error(v, c0, c1, c2, c3, scale, lod_sphere, err.x);
+        const float parent = projected_error(v, c0, c1, c2, c3, scale, parent_sphere, err.y);
         const float low = cut ? v.low[1] : v.low[0], high = cut ? v.high[1] : v.high[0], fade = v.fade_scale;
         uint32_t fade_low = 0, fade_high = kFadeSteps;
         if (fade > 0.0f) {
@@ -844,7 +903,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
         const bool lod = own <= high && parent > low && fade_high > fade_low;
         fades = fade_low | (fade_high << 8);
         if (lod) {
-          const uint2 span = p.r_span[run];
           first = span.x;
           count = span.y;
         }
