@@ -162,7 +162,8 @@ int vg_hzb_invalidate(vg_scene* scene, uint32_t view_slot);
  */
 int vg_hzb_from_cut(vg_scene* scene, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height,
                     uint32_t with_phase2);
-/* The slot's HZB buffer (OcclusionCulling hzbAttribute): `size_floats` receives its length; dst may be NULL. */
+/* The slot's HZB buffer (OcclusionCulling hzbAttribute): `size_floats` receives its length; dst may be NULL.
+ * Each level has a region sized for hzb_max_size; its texels outside the level's width x height are 0. */
 int vg_hzb_read(vg_scene* scene, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats);
 
 /*

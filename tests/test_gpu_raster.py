@@ -10,6 +10,7 @@ import pytest
 from helpers import compare_cut, draw_set, ground_camera, load_gpu, small_ruins
 from oracle import oracle
 from vgcut import camera as cam
+from vgcut import dag, native
 from vgcut.virtual_geometry import make_view
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +94,41 @@ def test_frame_loop_driven_by_the_raster(gpu_available):
         state = ref.first.state
         prev_draws, prev_n = ref.draws[0], n2
     assert hidden > 0, "the rasterized occluders must hide something"
+
+
+def test_chunk_queue_overflow_and_a_smaller_image(gpu_available):
+    """N copies of a quad filling the image: every triangle's pixel box is the whole image, so it is queued
+    as (W / 64) x (H / 64) rectangles and the 2^20-rectangle queue overflows. The first raster (1024 x 1024,
+    256 per triangle, a multiple of the capacity) fills every queue slot; the second (1024 x 576, 144 per
+    triangle) has a reservation running past the capacity, whose slots inside the queue must not replay the
+    first raster's rectangles. The second raster is also smaller than the first: the HZB buffer must hold
+    none of the first image. Both must equal the oracle, whole buffers compared (one quad's raster: the
+    copies coincide)."""
+    S, N = 200.0, 3700
+    pos = np.array([[-S, -S, 0.0], [S, -S, 0.0], [S, S, 0.0], [-S, S, 0.0]], np.float32)
+    data = dag.build_virtual_mesh(pos, np.array([0, 1, 2, 0, 2, 3], np.uint32))
+    assert data.meshletCount == 1
+    ctx = native.Context(max_views=1, mesh_slots=2, device=0)
+    scene = native.Scene(ctx)
+    slot = scene.add_mesh(data)
+    scene.set_instances(slot, np.tile(np.eye(4, dtype=np.float32).reshape(1, 16), (N, 1)))
+    pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, ctx.config.max_triangles, ctx.config.max_shadow_triangles)
+    chunks = []
+    for distance, w, h in ((5.0, 1024, 1024), (50.0, 1024, 576)):
+        camera = cam.perspective_camera(60, w / h, 0.1, 20000).look_at((0.0, 0.0, distance), (0.0, 0.0, 0.0))
+        vp = vp_of(camera)
+        scene.cut([make_view(camera, h)])
+        scene.sync()
+        counters, _, draws, _ = scene.read(0)
+        assert int(counters[2]) == N
+        chunks.append(2 * N * (w // 64) * (h // 64))
+        print(f"quad raster {w} x {h}: {chunks[-1]} rectangles", flush=True)
+        scene.hzb_from_cut(0, vp, w, h)
+        depth = oracle.raster(pool, draws[0], 1, vp, w, h)
+        assert np.all(depth < 1)
+        want = oracle.hzb_build(depth, ctx.config.hzb_max_size)
+        got = scene.hzb(0)
+        assert np.array_equal(got[:len(want)], want), f"{w} x {h}: HZB differs from the oracle"
+    capacity = 1 << 20
+    assert chunks[0] > capacity and chunks[0] % (2 * 16 * 16) == 0 and capacity % (16 * 16) == 0
+    assert chunks[1] > capacity and capacity % (16 * 9) != 0

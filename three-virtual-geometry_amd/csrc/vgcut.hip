@@ -1084,6 +1084,8 @@ int vg_hzb_set(vg_scene* s, uint32_t view_slot, const float* depth, uint32_t wid
                                hipMemcpyHostToDevice, ctx->stream));
   }
   float* hzb = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+  hipLaunchKernelGGL(k_hzb_clear, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, 4096u)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<uint32_t*>(hzb + kHzbHeader), 0u, total - kHzbHeader, nullptr);
   VG_HIP(ctx, hipMemcpyAsync(hzb, header.data(), sizeof(float) * kHzbHeader, hipMemcpyHostToDevice, ctx->stream));
   VG_HIP(ctx, hipMemcpyAsync(hzb + offsets[0], depth, sizeof(float) * width * height, hipMemcpyHostToDevice, ctx->stream));
   for (uint32_t base = 0; base + 1 < levels; base += 5) {
@@ -1145,9 +1147,8 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
   r.chunk_count = static_cast<uint32_t*>(s->d_raster_count.ptr);
   r.chunk_capacity = kChunkCapacity;
   hipLaunchKernelGGL(k_hzb_header, dim3(1), dim3(64), 0, ctx->stream, hzb, max_size, width, height);
-  const uint32_t pixels = width * height;
-  hipLaunchKernelGGL(k_raster_clear, dim3(std::min<uint32_t>((pixels + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth, pixels,
-                     r.chunk_count);
+  hipLaunchKernelGGL(k_hzb_clear, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth,
+                     width * height, total - kHzbHeader, r.chunk_count);
   if (!s->m_verts.empty()) {
     const dim3 grid(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu);
     std::vector<hipEvent_t> ev;

@@ -4,7 +4,7 @@
 // (three.js, depth-only material); here the same draw list is read where the cut left it on the device.
 //
 //   k_hzb_header  writes the HZB header (level offsets and sizes) for a width x height level 0
-//   k_raster_clear  level 0 <- 1.0 (WebGPU far depth)
+//   k_hzb_clear   level 0 <- 1.0 (WebGPU far depth), every other texel of the buffer <- 0
 //   k_raster      kSub drawn meshlets per wave, one per group of 64 / kSub lanes: a meshlet's vertices are
 //                 transformed once into LDS (instance matrix, then view_proj; ((c0 x + c1 y) + c2 z) + c3
 //                 in binary32 without contraction), snapped to 1/256 pixel; each lane sets up its share of
@@ -149,9 +149,12 @@ __global__ __launch_bounds__(64) void k_hzb_header(float* hzb, uint32_t max_size
   hzb[kHzbLevelCountSlot] = static_cast<float>(levels);
 }
 
-__global__ __launch_bounds__(256) void k_raster_clear(uint32_t* depth, uint32_t n, uint32_t* chunk_count) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *chunk_count = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) depth[i] = 0x3f800000u;
+// Clears an HZB's texel area: the first `far` texels (a depth image about to be rasterized) to 1.0 and the
+// rest of the `n` texels (every level's area, used or not) to 0, so the buffer holds no earlier contents
+// whatever the image size. Also resets the chunk queue when one is given.
+__global__ __launch_bounds__(256) void k_hzb_clear(uint32_t* texels, uint32_t far, uint32_t n, uint32_t* chunk_count) {
+  if (chunk_count && blockIdx.x == 0 && threadIdx.x == 0) *chunk_count = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) texels[i] = i < far ? 0x3f800000u : 0u;
 }
 
 // kSub meshlets per wave, each on 64 / kSub consecutive lanes (a lane group): every load of a meshlet (its
@@ -214,7 +217,15 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
         // when the queue is full, the lane group fills it below.
         const uint32_t cw = static_cast<uint32_t>(tr.x1 - tr.x0) / kChunkSide + 1, ch = static_cast<uint32_t>(tr.y1 - tr.y0) / kChunkSide + 1;
         const uint32_t at = atomicAdd(r.chunk_count, cw * ch);
-        if (at + cw * ch > r.chunk_capacity) {
+        if (at >= r.chunk_capacity || cw * ch > r.chunk_capacity - at) {
+          // The reservation runs past the queue: the slots it holds inside the queue (read by
+          // k_raster_chunks, which takes min(count, capacity)) get empty rectangles.
+          for (uint32_t q = at; q < r.chunk_capacity && q - at < cw * ch; q++) {
+            RasterChunk c{};
+            c.x0 = c.y0 = 1;
+            c.x1 = c.y1 = 0;
+            r.chunks[q] = c;
+          }
           big[j] = true;
           continue;
         }
