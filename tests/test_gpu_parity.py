@@ -103,6 +103,26 @@ def test_ruins_views(gpu_available, threshold, shadows, seed):
         assert int(ref.counters[2]) > 0
 
 
+def test_projective_instances(gpu_available):
+    """Instance matrices whose last row is not (0, 0, 0, 1): the meshlet pass then keeps the transformed w of
+    the LOD spheres (scenes of affine instances skip it, since it is exactly 1 there)."""
+    spec = small_ruins(seed=8)
+    rng = np.random.default_rng(8)
+    m = spec.matrices[0].copy()
+    m[::2, 3] = rng.uniform(-2e-3, 2e-3, len(m[::2])).astype(np.float32)  # c0.w: row 3 of column 0
+    m[1::3, 15] = np.float32(1.25)  # c3.w
+    spec.matrices[0] = m
+    loaded = load_gpu(spec)
+    state = oracle.initial_state()
+    for k, (eye, target) in enumerate((((-30.0, 6.0, -50.0), (10.0, 0.0, 40.0)), ((40.0, 20.0, 30.0), (-50.0, 0.0, -20.0)))):
+        view = make_view(ground_camera(eye, target), 1080, error_threshold=0.75)
+        gpu, = _run(loaded, [view])
+        ref = oracle.cut(loaded.pool, view, state=state, fp64=True)
+        state = ref.state
+        compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"projective view {k}")
+        assert ref.stats.mismatch_hard == 0 and int(ref.counters[2]) > 0
+
+
 def test_capacity_retry_and_steer_over_frames(gpu_available):
     """Tiny draw buffers: overflowing cuts are redone coarser in the same frame; the scale glides across frames."""
     spec = small_ruins(seed=21, shadows=True)

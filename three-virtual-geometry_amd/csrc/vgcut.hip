@@ -265,6 +265,8 @@ int upload_scene(vg_scene* s) {
         (rc = upload(s, s->d_vlist, s->m_vlist)) || (rc = upload(s, s->d_verts, s->m_verts)) ||
         (rc = upload(s, s->d_mverts, s->m_mverts)))
       return rc;
+    auto finite4 = [](const float4& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z) && std::isfinite(v.w); };
+    s->finite_runs = std::all_of(s->r_lod.begin(), s->r_lod.end(), finite4) && std::all_of(s->r_parent.begin(), s->r_parent.end(), finite4);
     s->dirty_meshlets = false;
   }
   if (s->dirty_table) {
@@ -278,6 +280,13 @@ int upload_scene(vg_scene* s) {
     for (size_t i = 0; i < islot.size(); i++) islot[i] = static_cast<uint32_t>(s->i_slot[i]);
     if ((rc = upload(s, s->d_islot, islot))) return rc;
     VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));  // islot is a temporary
+    bool affine = true;
+    for (size_t i = 0; affine && i < s->i_col[0].size(); i++) {
+      const float4 c[4] = {s->i_col[0][i], s->i_col[1][i], s->i_col[2][i], s->i_col[3][i]};
+      affine = c[0].w == 0.0f && c[1].w == 0.0f && c[2].w == 0.0f && c[3].w == 1.0f;
+      for (int k = 0; affine && k < 4; k++) affine = std::isfinite(c[k].x) && std::isfinite(c[k].y) && std::isfinite(c[k].z);
+    }
+    s->affine_instances = affine;
     s->dirty_instances = false;
   }
   if (s->dirty_cells) {
@@ -434,14 +443,16 @@ hipEvent_t take_event(vg_scene* s) {
 // prefix + check, 2 retry rounds (instances, meshlets, prefix + check / steer), then the expand pass on
 // contexts that have it.
 using MeshletKernel = void (*)(Params);
-MeshletKernel meshlet_kernel(uint32_t waves_per_simd) {
+MeshletKernel meshlet_kernel(uint32_t waves_per_simd, bool affine = false) {
   switch (waves_per_simd) {
-    case 4: return k_meshlets<4>;
-    case 5: return k_meshlets<5>;
-    case 6: return k_meshlets<6>;
+    case 4: return affine ? k_meshlets<4, true> : k_meshlets<4, false>;
+    case 5: return affine ? k_meshlets<5, true> : k_meshlets<5, false>;
+    case 6: return affine ? k_meshlets<6, true> : k_meshlets<6, false>;
     default: return nullptr;
   }
 }
+
+bool affine_scene(const vg_scene* s) { return s->affine_instances && s->finite_runs; }
 
 bool schedule_valid(const vg_schedule& sc) {
   return sc.instance_blocks_per_cu >= 1 && sc.instance_blocks_per_cu <= 16 && sc.meshlet_blocks_per_cu >= 1 &&
@@ -458,7 +469,7 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   Params p = make_params(s, lane, view_count);
   const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
   const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
-  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
+  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd, affine_scene(s));
   std::vector<hipEvent_t> ev;
   const bool timed = s->profile && !capturing && lane == &s->lane;
   if (timed) {
@@ -519,7 +530,7 @@ int enqueue_phase2(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.args = static_cast<uint32_t*>(s->d_args2.ptr);
   const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
   const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
-  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd);
+  const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd, affine_scene(s));
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
@@ -657,7 +668,8 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
 
 int launch_graph(vg_scene* s, Lane* lane, uint32_t count) {
   vg_ctx* ctx = s->ctx;
-  const uint64_t key[4] = {s->layout_version, lane->layout_version, static_cast<uint64_t>(lane->launch_budget), schedule_key(s->schedule)};
+  const uint64_t key[5] = {s->layout_version, lane->layout_version, static_cast<uint64_t>(lane->launch_budget), schedule_key(s->schedule),
+                           affine_scene(s) ? 1u : 0u};
   if (!lane->graph_exec || lane->graph_views != count || std::memcmp(key, lane->graph_key, sizeof(key)) != 0) {
     if (lane->graph_exec) (void)hipGraphExecDestroy(lane->graph_exec);
     if (lane->graph) (void)hipGraphDestroy(lane->graph);

@@ -65,7 +65,7 @@ struct Lane {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_views = 0;
-  uint64_t graph_key[4] = {0, 0, 0, 0};
+  uint64_t graph_key[5] = {0, 0, 0, 0, 0};
 };
 
 }  // namespace vgi
@@ -105,6 +105,9 @@ struct vg_scene {
   std::vector<float> i_slot;  // mesh slot of each instance
   std::vector<float4> c_sphere, c_rec;
   bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
+  // Every instance matrix affine (last row 0, 0, 0, 1) with finite entries / every run's LOD spheres finite:
+  // together they select the meshlet pass that takes the transformed w as 1.
+  bool affine_instances = false, finite_runs = false;
   // device scene
   vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;

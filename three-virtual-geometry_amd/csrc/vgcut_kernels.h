@@ -173,6 +173,14 @@ __device__ __forceinline__ float4 mat_mul(const float4& c0, const float4& c1, co
   return r;
 }
 
+// base[index] through a 32-bit byte offset from a wave-uniform base (the SGPR-base form of a global load, no
+// per-lane 64-bit address arithmetic). For arrays under 4 GB: a scene holds at most 2^24 meshlets, runs and
+// instances, and a work-list partition at most 2^24 items.
+template <typename T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t index) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + index * static_cast<uint32_t>(sizeof(T)));
+}
+
 template <typename P>
 __device__ __forceinline__ float4 mat_mul16(P m, float x, float y, float z, float w) {
   float4 r;

@@ -109,12 +109,13 @@ __device__ __forceinline__ float frustum_distance(const V& v, float cx, float cy
   return best;
 }
 
-// projectedError, GeometryPool.ts:678-683.
-template <typename V>
+// projectedError, GeometryPool.ts:678-683. kAffine: the instance matrix's last row is (0, 0, 0, 1) and the
+// sphere finite, so the transformed w is exactly ((0 x + 0 y) + 0 z) + 1 = 1 and is not computed.
+template <bool kAffine = false, typename V>
 __device__ __forceinline__ float projected_error(const V& v, const float4& c0, const float4& c1, const float4& c2,
                                                  const float4& c3, float scale, const float4& s, float error) {
   float4 q = mat_mul(c0, c1, c2, c3, s.x, s.y, s.z, 1.0f);
-  float4 vc = mat_mul16(v.view, q.x, q.y, q.z, q.w);
+  float4 vc = mat_mul16(v.view, q.x, q.y, q.z, kAffine ? 1.0f : q.w);
   float distance = fmaxf(len3(vc.x, vc.y, vc.z) - s.w * scale, v.near_plane);
   float pixels = ((error * scale) / distance) * v.proj_scale;
   return error >= VG_INF_HALF ? VG_INF : pixels;
@@ -705,10 +706,10 @@ __device__ __forceinline__ void walk_partition(const Params& p, size_t base, uin
       next_start = k0 + 64;
     }
     next_cur = uniform(next_cur);
-    if (k0 + 64 < le) ns = next_cur + 1 + lane < items ? starts[next_cur + 1 + lane] : ~0ull;
+    if (k0 + 64 < le) ns = next_cur + 1 + lane < items ? ld32(starts, next_cur + 1 + lane) : ~0ull;
 
     uint4 w = make_uint4(0, 0, 0, 0);
-    if (valid) w = work[item];
+    if (valid) w = ld32(work, item);
     const uint32_t task = w.y + static_cast<uint32_t>(k - item_start);
     if (valid && task >= w.z) {  // the walk lost its item: report instead of reading past the range
       atomicOr(&p.flags[2], 1u);
@@ -756,8 +757,9 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
 // the cut go to a per-wave LDS ring; phase 2 runs on full waves of them: frustum and occlusion tests
 // (drawn = visible and in the cut, so the order changes no decision), then compaction into the draw
 // list. The per-cluster-test cost is one LOD evaluation per run; visibility is paid per meshlet in the cut.
-// Compiled for several occupancy targets (waves per SIMD); the launch schedule picks one.
-template <int kWavesPerSimd>
+// Compiled for several occupancy targets (waves per SIMD); the launch schedule picks one. kAffine: every
+// instance matrix of the scene is affine and every run's LOD spheres finite (projected_error).
+template <int kWavesPerSimd, bool kAffine>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_meshlets(Params p) {
   // Per wave: meshlets in the cut awaiting their visibility test, as (instance, meshlet, tag, fades) and the
   // instance scale.
@@ -789,7 +791,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const uint32_t instance = c.x, meshlet = c.y, tag = c.z;
     const uint32_t mslot = tag & kTagSlotMask, cut = (tag & kTagShadow) ? 1u : 0u, vi = tag >> kTagViewShift;
     const bool occ_on = (tag & kTagOcclusion) != 0, fast = (tag & kTagFast) != 0;
-    const uint32_t meshlet_tris = active ? p.m_tris[meshlet] : 0u;
+    const uint32_t meshlet_tris = active ? ld32(p.m_tris, meshlet) : 0u;
     bool drawn = active && fast;  // fast path: guaranteed meshlets of a culling sphere inside the frustum
     bool occ_test = false, rejected = false;
     uint32_t vslot = 0;
@@ -798,11 +800,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     const bool tested = active && !fast;
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, cs = c0;
     if (tested) {
-      c0 = p.i_col0[instance];
-      c1 = p.i_col1[instance];
-      c2 = p.i_col2[instance];
-      c3 = p.i_col3[instance];
-      cs = p.m_cull[meshlet];
+      c0 = ld32(p.i_col0, instance);
+      c1 = ld32(p.i_col1, instance);
+      c2 = ld32(p.i_col2, instance);
+      c3 = ld32(p.i_col3, instance);
+      cs = ld32(p.m_cull, meshlet);
     }
     // Visibility, one launched view at a time (its constants are wave-uniform there).
     for (uint64_t pending = __ballot(active); pending;) {
@@ -875,15 +877,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     float2 err = make_float2(0.0f, 0.0f);
     uint2 span = make_uint2(0, 0);
     if (run_test) {
-      c0 = p.i_col0[w.x];
-      c1 = p.i_col1[w.x];
-      c2 = p.i_col2[w.x];
-      c3 = p.i_col3[w.x];
+      c0 = ld32(p.i_col0, w.x);
+      c1 = ld32(p.i_col1, w.x);
+      c2 = ld32(p.i_col2, w.x);
+      c3 = ld32(p.i_col3, w.x);
       scale = p.work_scale[item];
-      err = p.r_err[run];
-      lod_sphere = p.r_lod[run];
-      parent_sphere = p.r_parent[run];
-      span = p.r_span[run];
+      err = ld32(p.r_err, run);
+      lod_sphere = ld32(p.r_lod, run);
+      parent_sphere = ld32(p.r_parent, run);
+      span = ld32(p.r_span, run);
     }
     // The LOD test of a run, one launched view at a time (its constants are wave-uniform there).
     for (uint64_t pending = __ballot(run_test); pending;) {
@@ -892,8 +894,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       if (mine) {
         // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
         const auto& v = scalar(p.views)[uniform(vu)];
-        const float own = projected_error(v, c0, c1, c2, c3, scale, lod_sphere, err.x);
-        const float parent = projected_error(v, c0, c1, c2, c3, scale, parent_sphere, err.y);
+        const float own = projected_error<kAffine>(v, c0, c1, c2, c3, scale, lod_sphere, err.x);
+        const float parent = projected_error<kAffine>(v, c0, c1, c2, c3, scale, parent_sphere, err.y);
         const float low = cut ? v.low[1] : v.low[0], high = cut ? v.high[1] : v.high[0], fade = v.fade_scale;
         uint32_t fade_low = 0, fade_high = kFadeSteps;
         if (fade > 0.0f) {
