@@ -254,7 +254,7 @@ int vg_footprint(vg_scene* scene, const vg_view* views, uint32_t count, uint64_t
  * (which dispatches one workgroup per work item, GeometryPool.ts:926-968); results do not depend on the
  * schedule, only the time does. instance_blocks_per_cu: instance-pass workgroups (256 threads) per CU,
  * 1-16. meshlet_blocks_per_cu: meshlet-pass workgroups per CU, 1-32. meshlet_waves_per_simd: the
- * occupancy the meshlet pass is compiled for (VG_MESHLET_VARIANTS: 4, 5 or 6; higher = fewer registers).
+ * occupancy the meshlet pass is compiled for (VG_MESHLET_VARIANTS: 4, 5, 6 or 7; higher = fewer registers).
  * vg_tune times candidate schedules on `count` views (`repeats` eager cuts each, the median kept, the
  * controller state restored after each), sets the fastest on the scene and returns it with its time.
  */

@@ -448,11 +448,15 @@ MeshletKernel meshlet_kernel(uint32_t waves_per_simd, bool affine = false) {
     case 4: return affine ? k_meshlets<4, true> : k_meshlets<4, false>;
     case 5: return affine ? k_meshlets<5, true> : k_meshlets<5, false>;
     case 6: return affine ? k_meshlets<6, true> : k_meshlets<6, false>;
+    case 7: return affine ? k_meshlets<7, true> : k_meshlets<7, false>;
     default: return nullptr;
   }
 }
 
 bool affine_scene(const vg_scene* s) { return s->affine_instances && s->finite_runs; }
+
+static_assert(kMaxMeshletsPerScene <= (1u << 24) && kMaxInstancesPerScene <= (1u << 24),
+              "the meshlet pass's ring packs instance and meshlet ids in 24 bits");
 
 bool schedule_valid(const vg_schedule& sc) {
   return sc.instance_blocks_per_cu >= 1 && sc.instance_blocks_per_cu <= 16 && sc.meshlet_blocks_per_cu >= 1 &&
@@ -1569,7 +1573,7 @@ int vg_tune(vg_scene* s, const vg_view* views, uint32_t count, uint32_t repeats,
   rc = time_it(chosen, &chosen_ms);
   // Coordinate descent, two sweeps: the meshlet pass's occupancy variant and grid, then the instance
   // pass's grid. The incumbent is timed again in each sweep, so one lucky timing does not stick.
-  static const uint32_t kMeshlet[][2] = {{4, 4}, {4, 8}, {5, 5}, {5, 10}, {6, 6}, {6, 12}};
+  static const uint32_t kMeshlet[][2] = {{4, 4}, {4, 8}, {5, 5}, {5, 10}, {6, 6}, {6, 12}, {7, 7}, {7, 14}};
   static const uint32_t kInstance[] = {4, 6, 7, 8, 12};
   for (int sweep = 0; sweep < 2 && !rc; sweep++) {
     if (sweep) rc = time_it(chosen, &chosen_ms);

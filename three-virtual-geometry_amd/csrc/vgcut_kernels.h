@@ -88,6 +88,7 @@ struct ViewDev {
   uint32_t budget_bin;
   float budget_scale, budget_low, budget_high;  // camera scale and band of the budget histogram
   float budget_inv;  // (bins - 1) / ln(T[B-1] / T[0]): estimates a value's bin in the log-spaced table
+  float sweep_reach[6];  // per frustum plane: fmaxf(plane normal . sweep, 0), the swept frustum's extra reach
 };
 
 struct Params {

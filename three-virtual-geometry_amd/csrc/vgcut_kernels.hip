@@ -102,8 +102,9 @@ __device__ __forceinline__ float frustum_distance(const V& v, float cx, float cy
   for (int p = 0; p < 6; p++) {
     const auto n = &v.planes[p * 4];
     float d = dot3(n, cx, cy, cz) + n[3];
-    if (mode == 1) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * 1.0f;
-    if (mode == 2) d = d + fmaxf(dot3(n, v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f) * (flag ? 1.0f : 0.0f);
+    // sweep_reach[p] = fmaxf(n . sweep, 0), computed once per view by k_begin (x 1.0f is exact).
+    if (mode == 1) d = d + v.sweep_reach[p];
+    if (mode == 2) d = d + v.sweep_reach[p] * (flag ? 1.0f : 0.0f);
     best = fminf(best, d);
   }
   return best;
@@ -150,10 +151,21 @@ __device__ bool occluded_by(const V& v, const float* __restrict__ hzb, const Lev
   const auto P = v.proj;
   float x_lo = vx - radius, x_hi = vx + radius, y_lo = vy - radius, y_hi = vy + radius;
   float sx = P[0], sy = P[5];
-  float min_x = fminf(x_lo / near_depth, x_lo / far_depth) * sx - P[8];
-  float max_x = fmaxf(x_hi / near_depth, x_hi / far_depth) * sx - P[8];
-  float min_y = fminf(y_lo / near_depth, y_lo / far_depth) * sy - P[9];
-  float max_y = fmaxf(y_hi / near_depth, y_hi / far_depth) * sy - P[9];
+  float min_x, max_x, min_y, max_y;
+  if (radius >= 0.0f && near_depth > 0.0f) {
+    // 0 < near_depth <= far_depth: correctly rounded division is monotonic in the denominator,
+    // so fminf(x / near_depth, x / far_depth) is x / far_depth for x >= 0 (signed zeros and NaN included)
+    // and x / near_depth otherwise, and the reverse for fmaxf. One division per bound.
+    min_x = (x_lo >= 0.0f ? x_lo / far_depth : x_lo / near_depth) * sx - P[8];
+    max_x = (x_hi >= 0.0f ? x_hi / near_depth : x_hi / far_depth) * sx - P[8];
+    min_y = (y_lo >= 0.0f ? y_lo / far_depth : y_lo / near_depth) * sy - P[9];
+    max_y = (y_hi >= 0.0f ? y_hi / near_depth : y_hi / far_depth) * sy - P[9];
+  } else {
+    min_x = fminf(x_lo / near_depth, x_lo / far_depth) * sx - P[8];
+    max_x = fmaxf(x_hi / near_depth, x_hi / far_depth) * sx - P[8];
+    min_y = fminf(y_lo / near_depth, y_lo / far_depth) * sy - P[9];
+    max_y = fmaxf(y_hi / near_depth, y_hi / far_depth) * sy - P[9];
+  }
   float min_z = P[14] / near_depth - P[10];
   float W = header[1], H = header[2];
   float x0 = clampf(floorf((min_x * 0.5f + 0.5f) * W), 0.0f, W - 1.0f);
@@ -203,6 +215,7 @@ __device__ __forceinline__ bool occluded_sphere(const Params& p, const V& v, boo
 
 __global__ __launch_bounds__(256) void k_begin(Params p) {
   ViewDev& v = p.views[blockIdx.x];
+  if (threadIdx.x < 6) v.sweep_reach[threadIdx.x] = fmaxf(dot3(&v.planes[threadIdx.x * 4], v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f);
   uint32_t* c = view_counters(p, v.slot);
   for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
   if (p.phase == 2) {
@@ -761,10 +774,9 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
 // instance matrix of the scene is affine and every run's LOD spheres finite (projected_error).
 template <int kWavesPerSimd, bool kAffine>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_meshlets(Params p) {
-  // Per wave: meshlets in the cut awaiting their visibility test, as (instance, meshlet, tag, fades) and the
-  // instance scale.
+  // Per wave: meshlets in the cut awaiting their visibility test, 16 bytes each: (instance | fade_low << 24,
+  // meshlet | fade_high << 24, tag, instance scale bits). A scene holds fewer than 2^24 instances and meshlets.
   __shared__ uint4 ring[4][kQueue];
-  __shared__ float ring_scale[4][kQueue];
   // Per wave: the HZB level table of the view whose meshlets are being tested (one float per lane to copy).
   static_assert(kHzbMaxLevels * 4 == 64, "the level table is copied one float per lane");
   __shared__ float4 hzb_rows[4][kHzbMaxLevels];
@@ -785,10 +797,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
 
   auto drain = [&](uint32_t n) {
     const bool active = lane < n;
-    const uint32_t at_ring = (head + lane) % kQueue;
+    const uint32_t at_ring = head + lane >= kQueue ? head + lane - kQueue : head + lane;  // head < kQueue
     const uint4 c = active ? ring[wib][at_ring] : make_uint4(0, 0, 0, 0);
-    const float scale = active ? ring_scale[wib][at_ring] : 0.0f;
-    const uint32_t instance = c.x, meshlet = c.y, tag = c.z;
+    const float scale = __uint_as_float(c.w);
+    const uint32_t instance = c.x & 0xffffffu, meshlet = c.y & 0xffffffu, tag = c.z;
+    const uint32_t fades = (c.x >> 24) | ((c.y >> 24) << 8);
     const uint32_t mslot = tag & kTagSlotMask, cut = (tag & kTagShadow) ? 1u : 0u, vi = tag >> kTagViewShift;
     const bool occ_on = (tag & kTagOcclusion) != 0, fast = (tag & kTagFast) != 0;
     const uint32_t meshlet_tris = active ? ld32(p.m_tris, meshlet) : 0u;
@@ -832,7 +845,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     }
     const uint32_t tris = drawn ? meshlet_tris : 0u;
     head += n;
-    const uint32_t packed_tris = tris | (c.w << 8);
+    const uint32_t packed_tris = tris | (fades << 8);
     for (uint64_t tests = __ballot(occ_test); tests;) {
       const uint32_t key = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vslot), __ffsll(static_cast<unsigned long long>(tests)) - 1));
       const uint64_t same = __ballot(occ_test && vslot == key);
@@ -847,7 +860,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       const uint32_t rat = wave_atomic_add(phase_at(p, vslot) + P_REJECTS, 1u, rejected);
       if (rejected && rat < p.reject_capacity)
         p.reject_meshlets[static_cast<size_t>(vslot) * p.reject_capacity + rat] =
-            make_uint4(instance, meshlet, c.w | (mslot << 16), __float_as_uint(scale));
+            make_uint4(instance, meshlet, fades | (mslot << 16), __float_as_uint(scale));
       else if (rejected)
         atomicOr(phase_at(p, vslot) + P_OVERFLOW, 1u);
     }
@@ -917,9 +930,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     if (total) {
       const uint32_t at = tail + incl - count;
       for (uint32_t k = 0; k < count; k++) {
-        const uint32_t r = (at + k) % kQueue;
-        ring[wib][r] = make_uint4(w.x, first + k, w.w, fades);
-        ring_scale[wib][r] = scale;
+        const uint32_t r = at + k >= kQueue ? at + k - kQueue : at + k;  // at + k < 2 kQueue: head < kQueue, fewer than 64 + 4 * 64 pending
+        ring[wib][r] = make_uint4(w.x | (fades << 24), (first + k) | ((fades >> 8) << 24), w.w, __float_as_uint(scale));
       }
       tail += total;
       while (tail - head >= 64) {
