@@ -466,7 +466,8 @@ def two_phase_bench(scene, vd, steps):
 
     def entry_bytes(meshlets):
         # Per drawn camera entry: the entry, 4 matrix columns, meshlet records (first, count, vertex list),
-        # the vertex ids and positions, the packed triangles (vertex count = largest local id + 1).
+        # the positions (16 B, stored in vertex-list order), the packed triangles (vertex count = largest
+        # local id + 1).
         info = pool_info[meshlets]
         has = info[:, 3] != 0xFFFFFFFF
         tris = info[:, 0].astype(np.int64)
@@ -474,7 +475,7 @@ def two_phase_bench(scene, vd, steps):
         for k in np.nonzero(has)[0]:
             p = pool_tri[info[k, 1] // 3: info[k, 1] // 3 + tris[k]]
             nv[k] = int(max((p & 255).max(), ((p >> 8) & 255).max(), ((p >> 16) & 255).max())) + 1
-        return int(len(meshlets) * (16 + 64 + 16) + (nv * 20).sum() + (tris[has] * 4).sum())
+        return int(len(meshlets) * (16 + 64 + 16) + (nv * 16).sum() + (tris[has] * 4).sum())
 
     for v in two:
         c1, _, draws, _ = scene.read(v.slot)

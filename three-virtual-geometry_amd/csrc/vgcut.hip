@@ -262,9 +262,14 @@ int upload_scene(vg_scene* s) {
         (rc = upload(s, s->d_rspan, s->r_span)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
         (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)) ||
         (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)) ||
-        (rc = upload(s, s->d_vlist, s->m_vlist)) || (rc = upload(s, s->d_verts, s->m_verts)) ||
-        (rc = upload(s, s->d_mverts, s->m_mverts)))
+        (rc = upload(s, s->d_vlist, s->m_vlist)))
       return rc;
+    {
+      std::vector<float4> vpos(s->m_mverts.size());  // positions in meshlet-vertex order (ids checked by vg_mesh_add)
+      for (size_t i = 0; i < vpos.size(); i++) vpos[i] = s->m_verts[s->m_mverts[i]];
+      if ((rc = upload(s, s->d_vpos, vpos))) return rc;
+      VG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));  // vpos is a temporary
+    }
     auto finite4 = [](const float4& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z) && std::isfinite(v.w); };
     s->finite_runs = std::all_of(s->r_lod.begin(), s->r_lod.end(), finite4) && std::all_of(s->r_parent.begin(), s->r_parent.end(), finite4);
     s->dirty_meshlets = false;
@@ -854,7 +859,7 @@ void vg_scene_destroy(vg_scene* s) {
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
                     &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
-                    &s->d_reject_meshlets, &s->d_phase, &s->d_verts, &s->d_mverts, &s->d_vlist, &s->d_raster_chunks,
+                    &s->d_reject_meshlets, &s->d_phase, &s->d_vpos, &s->d_vlist, &s->d_raster_chunks,
                     &s->d_raster_count};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
@@ -1141,8 +1146,7 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
   r.tris = static_cast<const uint32_t*>(s->d_tris.ptr);
   r.vlist = static_cast<const uint2*>(s->d_vlist.ptr);
   r.tri = static_cast<const uint32_t*>(s->d_tri.ptr);
-  r.mverts = static_cast<const uint32_t*>(s->d_mverts.ptr);
-  r.verts = static_cast<const float4*>(s->d_verts.ptr);
+  r.vpos = static_cast<const float4*>(s->d_vpos.ptr);
   r.draws = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
   // Camera entries: [0, drawn) of the cut, or [0, min(selected, max)) when its second phase appended more.
   const bool both = with_phase2 && s->phase2_ran[view_slot] && s->d_counters2.ptr;

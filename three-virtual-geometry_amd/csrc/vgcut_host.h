@@ -112,7 +112,9 @@ struct vg_scene {
   vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
   vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs
-  vgi::DevBuf d_verts, d_mverts, d_vlist;  // occluder raster inputs
+  // Occluder raster inputs: each meshlet's vertex positions in its vertex-list order (x, y, z, 0), so a
+  // meshlet's are contiguous; the vertex lists' (offset, count).
+  vgi::DevBuf d_vpos, d_vlist;
   vgi::DevBuf d_raster_chunks, d_raster_count;  // the raster's queue of large-triangle rectangles
   // per view slot
   vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;

@@ -29,10 +29,9 @@ struct RasterParams {
   const float4* col[4];   // instance matrix columns
   const uint32_t* first;  // per meshlet: firstIndex (pool-rebased); its triangles are at first / 3
   const uint32_t* tris;   // per meshlet: triangle count
-  const uint2* vlist;     // per meshlet: (offset into meshlet_vertices, vertex count; 0 = no positions)
+  const uint2* vlist;     // per meshlet: (offset into vpos, vertex count; 0 = no positions)
   const uint32_t* tri;    // packed local vertex ids a | b << 8 | c << 16
-  const uint32_t* mverts; // pool-global vertex ids
-  const float4* verts;    // positions (x, y, z, 0)
+  const float4* vpos;     // per meshlet vertex, in vertex-list order: position (x, y, z, 0)
   const uint4* draws;     // the slot's draw list; camera entries from position 0
   const uint32_t* count;  // entries to draw: min(*count, clamp)
   uint32_t clamp;
@@ -158,8 +157,9 @@ __global__ __launch_bounds__(256) void k_hzb_clear(uint32_t* texels, uint32_t fa
 }
 
 // kSub meshlets per wave, each on 64 / kSub consecutive lanes (a lane group): every load of a meshlet (its
-// vertex ids, positions and packed triangles) is issued before any is used, and the groups' meshlets are in
-// flight together, so the dependent chain entry -> vertex list -> positions costs one wait per kSub meshlets.
+// positions, contiguous in vertex-list order, and packed triangles) is issued before any is used, and the
+// groups' meshlets are in flight together, so the dependent chain entry -> meshlet record -> positions costs
+// one wait per kSub meshlets.
 template <uint32_t kSub>
 __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
   constexpr uint32_t kLanes = 64 / kSub, kPer = 128 / kLanes;
@@ -187,16 +187,16 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
       first = r.first[entry.y] / 3;
     }
     if (nv == 0) tris = 0;
-    uint32_t ids[kPer], packed[kPer];
+    // The meshlet's positions are contiguous (vertex-list order): the lane group reads them coalesced, in the
+    // same round trip as the packed triangles.
+    uint32_t packed[kPer];
+    float4 pos[kPer];
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) {
       const uint32_t k = gl + j * kLanes;
-      ids[j] = k < nv ? r.mverts[voff + k] : 0u;
+      pos[j] = k < nv ? r.vpos[voff + k] : make_float4(0, 0, 0, 0);
       packed[j] = k < tris ? r.tri[first + k] : 0u;
     }
-    float4 pos[kPer];
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; j++) pos[j] = gl + j * kLanes < nv ? r.verts[ids[j]] : make_float4(0, 0, 0, 0);
     if (nv) {
       const float4 c0 = r.col[0][instance], c1 = r.col[1][instance], c2 = r.col[2][instance], c3 = r.col[3][instance];
 #pragma unroll
