@@ -102,6 +102,16 @@ __device__ __forceinline__ RasterTriangle raster_setup(RasterVertex a, RasterVer
   return t;
 }
 
+// The depth of a covered pixel from its edge functions: the barycentric blend of the vertices' depths,
+// kept with atomicMin on the float bits (depths in [0, 1]).
+__device__ __forceinline__ void raster_shade(uint32_t* depth, uint32_t W, const RasterTriangle& t, int64_t e0, int64_t e1,
+                                             int64_t e2, int32_t x, int32_t y) {
+  float z = ((static_cast<float>(e0) * t.a.z + static_cast<float>(e1) * t.b.z) + static_cast<float>(e2) * t.c.z) /
+            static_cast<float>(t.area);
+  z = fminf(fmaxf(z, 0.0f), 1.0f);
+  atomicMin(depth + static_cast<size_t>(y) * W + x, __float_as_uint(z));
+}
+
 __device__ __forceinline__ void raster_pixel(uint32_t* depth, uint32_t W, const RasterTriangle& t, int32_t x, int32_t y) {
   const int64_t px = static_cast<int64_t>(x) * 256 + 128, py = static_cast<int64_t>(y) * 256 + 128;
   const RasterVertex& a = t.a;
@@ -111,9 +121,29 @@ __device__ __forceinline__ void raster_pixel(uint32_t* depth, uint32_t W, const 
   const int64_t e1 = static_cast<int64_t>(a.x - c.x) * (py - c.y) - static_cast<int64_t>(a.y - c.y) * (px - c.x);
   const int64_t e2 = static_cast<int64_t>(b.x - a.x) * (py - a.y) - static_cast<int64_t>(b.y - a.y) * (px - a.x);
   if ((e0 | e1 | e2) < 0) return;
-  float z = ((static_cast<float>(e0) * a.z + static_cast<float>(e1) * b.z) + static_cast<float>(e2) * c.z) / static_cast<float>(t.area);
-  z = fminf(fmaxf(z, 0.0f), 1.0f);
-  atomicMin(depth + static_cast<size_t>(y) * W + x, __float_as_uint(z));
+  raster_shade(depth, W, t, e0, e1, e2, x, y);
+}
+
+// Fills the pixel box of t, stepping the edge functions instead of recomputing them: they are affine in the
+// pixel position, so e(x + 1, y) = e(x, y) + dx and e(x, y + 1) = e(x, y) + dy, exactly (64-bit integers;
+// snapped coordinates are below 2^29 in magnitude).
+__device__ __forceinline__ void raster_box(uint32_t* depth, uint32_t W, const RasterTriangle& t) {
+  if (t.x0 > t.x1 || t.y0 > t.y1) return;
+  const RasterVertex& a = t.a;
+  const RasterVertex& b = t.b;
+  const RasterVertex& c = t.c;
+  const int64_t px = static_cast<int64_t>(t.x0) * 256 + 128, py = static_cast<int64_t>(t.y0) * 256 + 128;
+  int64_t r0 = static_cast<int64_t>(c.x - b.x) * (py - b.y) - static_cast<int64_t>(c.y - b.y) * (px - b.x);
+  int64_t r1 = static_cast<int64_t>(a.x - c.x) * (py - c.y) - static_cast<int64_t>(a.y - c.y) * (px - c.x);
+  int64_t r2 = static_cast<int64_t>(b.x - a.x) * (py - a.y) - static_cast<int64_t>(b.y - a.y) * (px - a.x);
+  const int64_t dx0 = -static_cast<int64_t>(c.y - b.y) * 256, dy0 = static_cast<int64_t>(c.x - b.x) * 256;
+  const int64_t dx1 = -static_cast<int64_t>(a.y - c.y) * 256, dy1 = static_cast<int64_t>(a.x - c.x) * 256;
+  const int64_t dx2 = -static_cast<int64_t>(b.y - a.y) * 256, dy2 = static_cast<int64_t>(b.x - a.x) * 256;
+  for (int32_t y = t.y0; y <= t.y1; y++, r0 += dy0, r1 += dy1, r2 += dy2) {
+    int64_t e0 = r0, e1 = r1, e2 = r2;
+    for (int32_t x = t.x0; x <= t.x1; x++, e0 += dx0, e1 += dx1, e2 += dx2)
+      if ((e0 | e1 | e2) >= 0) raster_shade(depth, W, t, e0, e1, e2, x, y);
+  }
 }
 
 // A rectangle of a large triangle's pixel box: filled by one block of k_raster_chunks.
@@ -244,8 +274,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
         }
         continue;
       }
-      for (int32_t y = tr.y0; y <= tr.y1; y++)
-        for (int32_t x = tr.x0; x <= tr.x1; x++) raster_pixel(r.depth, r.width, tr, x, y);
+      raster_box(r.depth, r.width, tr);
     }
     // Large triangles the queue had no room for: each lane group strides over its meshlet's, one at a time.
 #pragma unroll
