@@ -113,14 +113,19 @@ def test_projective_instances(gpu_available):
     m[1::3, 15] = np.float32(1.25)  # c3.w
     spec.matrices[0] = m
     loaded = load_gpu(spec)
-    state = oracle.initial_state()
-    for k, (eye, target) in enumerate((((-30.0, 6.0, -50.0), (10.0, 0.0, 40.0)), ((40.0, 20.0, 30.0), (-50.0, 0.0, -20.0)))):
-        view = make_view(ground_camera(eye, target), 1080, error_threshold=0.75)
-        gpu, = _run(loaded, [view])
-        ref = oracle.cut(loaded.pool, view, state=state, fp64=True)
-        state = ref.state
-        compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"projective view {k}")
-        assert ref.stats.mismatch_hard == 0 and int(ref.counters[2]) > 0
+    cameras = (((-30.0, 6.0, -50.0), (10.0, 0.0, 40.0)), ((40.0, 20.0, 30.0), (-50.0, 0.0, -20.0)))
+    # Every occupancy variant of the general meshlet pass (4 to 7 waves per SIMD).
+    for sched in [(8, 8, 4), (6, 10, 5), (12, 12, 6), (12, 14, 7)]:
+        loaded.scene.set_schedule(*sched)
+        state = oracle.initial_state()
+        loaded.scene.set_state(0, state)
+        for k, (eye, target) in enumerate(cameras):
+            view = make_view(ground_camera(eye, target), 1080, error_threshold=0.75)
+            gpu, = _run(loaded, [view])
+            ref = oracle.cut(loaded.pool, view, state=state, fp64=True)
+            state = ref.state
+            compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"projective view {k}, schedule {sched}")
+            assert ref.stats.mismatch_hard == 0 and int(ref.counters[2]) > 0
 
 
 def test_capacity_retry_and_steer_over_frames(gpu_available):
