@@ -322,27 +322,37 @@ __device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
 // atomicAdd of `value` for every lane with `active`, aggregated per distinct address: one atomic per
 // address per wave. Returns the lane's old value (the group base plus its exclusive prefix). All 64
 // lanes must call it together.
+template <bool kOne = false>
 __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t value, bool active) {
   uint64_t pending = __ballot(active);
   const uint32_t lane = lane_id();
-  const uint64_t a = reinterpret_cast<uint64_t>(addr);
+  const uint32_t a_lo = static_cast<uint32_t>(reinterpret_cast<uint64_t>(addr));
+  const uint32_t a_hi = static_cast<uint32_t>(reinterpret_cast<uint64_t>(addr) >> 32);
   // Every group's atomic is issued before any result is read: the groups' round trips overlap, and each
-  // lane then reads its group leader's result with one lane shuffle.
+  // lane then reads its group leader's result with one lane shuffle. The leader is wave-uniform, so its
+  // address is read with readlane.
   uint32_t leader_base = 0, my_leader = 0, my_offset = 0;
   while (pending) {
-    int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-    uint32_t lo = __shfl(static_cast<uint32_t>(a), leader, 64);
-    uint32_t hi = __shfl(static_cast<uint32_t>(a >> 32), leader, 64);
-    uint64_t la = (static_cast<uint64_t>(hi) << 32) | lo;
-    bool mine = ((pending >> lane) & 1ull) && a == la;
-    uint64_t group = __ballot(mine);
-    uint32_t v = mine ? value : 0u;
-    uint32_t incl = wave_scan(v);
-    uint32_t total = __shfl(incl, 63, 64);
-    if (lane == static_cast<uint32_t>(leader) && total) leader_base = atomicAdd(reinterpret_cast<uint32_t*>(la), total);
+    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(a_lo), leader));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(a_hi), leader));
+    const bool mine = ((pending >> lane) & 1ull) && a_lo == lo && a_hi == hi;
+    const uint64_t group = __ballot(mine);
+    uint32_t prefix, total;
+    if (kOne) {  // every lane adds 1: the exclusive prefix counts the group's lanes below this one
+      prefix = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(group >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(group), 0u));
+      total = static_cast<uint32_t>(__popcll(group));
+    } else {
+      const uint32_t v = mine ? value : 0u;
+      const uint32_t incl = wave_scan(v);
+      prefix = incl - v;
+      total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    }
+    if (lane == static_cast<uint32_t>(leader) && total)
+      leader_base = atomicAdd(reinterpret_cast<uint32_t*>((static_cast<uint64_t>(hi) << 32) | lo), total);
     if (mine) {
       my_leader = static_cast<uint32_t>(leader);
-      my_offset = incl - v;
+      my_offset = prefix;
     }
     pending &= ~group;
   }
@@ -367,9 +377,9 @@ __device__ __forceinline__ void wave_count_add_keyed(unsigned long long* addr, u
   const uint32_t lane = lane_id();
   const uint64_t a = reinterpret_cast<uint64_t>(addr);
   while (pending) {
-    int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-    uint32_t lo = __shfl(static_cast<uint32_t>(a), leader, 64);
-    uint32_t hi = __shfl(static_cast<uint32_t>(a >> 32), leader, 64);
+    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(a)), leader));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(a >> 32)), leader));
     uint64_t la = (static_cast<uint64_t>(hi) << 32) | lo;
     bool mine = ((pending >> lane) & 1ull) && a == la;
     uint64_t group = __ballot(mine);

@@ -857,14 +857,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
       tests &= ~same;
     }
     if (__ballot(rejected)) {  // two-phase views: keep the occluded meshlets for the second phase
-      const uint32_t rat = wave_atomic_add(phase_at(p, vslot) + P_REJECTS, 1u, rejected);
+      const uint32_t rat = wave_atomic_add<true>(phase_at(p, vslot) + P_REJECTS, 1u, rejected);
       if (rejected && rat < p.reject_capacity)
         p.reject_meshlets[static_cast<size_t>(vslot) * p.reject_capacity + rat] =
             make_uint4(instance, meshlet, fades | (mslot << 16), __float_as_uint(scale));
       else if (rejected)
         atomicOr(phase_at(p, vslot) + P_OVERFLOW, 1u);
     }
-    const uint32_t at = wave_atomic_add(counter_at(p, vslot, cut, kSelected), 1u, drawn);
+    const uint32_t at = wave_atomic_add<true>(counter_at(p, vslot, cut, kSelected), 1u, drawn);
     const bool write = drawn && at < p.max_meshlets;
     const uint32_t part = (packed_tris & 0xffff00u) == kFullFadeBits ? 0u : 1u;
     const uint32_t first_index = wave_atomic_add(mesh_count_at(p, vslot, cut, mslot, part), tris * 3, write);
