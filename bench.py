@@ -93,94 +93,7 @@ def parse():
     return ap.parse_args()
 
 
-def mixed_views(spec, count, vg_make_view, cam, scenegen, size=6144.0):
-    """The 64 mixed views (main, shadow cascades, cube faces, viewers) and their occluder depth."""
-    from vgcut import scenes
-
-    hf = spec.heightfield
-    path = scenes.flythrough(spec, 16, size, 30.0, seed=9)
-    views, depths = [], []
-    rng = np.random.default_rng(10)
-    slot = 0
-    sun = np.array([-0.4, -1.0, -0.3])
-    sun /= np.linalg.norm(sun)
-
-    def terrain_depth(camera, w, h):
-        vp = cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse)
-        return scenegen.heightfield_depth(vp, w, h, (0.0, 0.0), size / 2, 256, hf["amplitude"], hf["frequency"], hf["seed"])
-
-    mains = []
-    for i in range(4):
-        eye, target = path[i * 4]
-        width, height = (1920, 1080) if i < 2 else (3840, 2160)
-        camera = cam.perspective_camera(60, width / height, 0.1, 20000).look_at(eye, target)
-        mains.append((camera, eye, target))
-        hz = (960, 540) if i < 2 else (1024, 576)
-        views.append((camera, height, True, terrain_depth(camera, *hz)))
-    for camera, eye, target in mains:  # 4 cascades per main camera, centred ahead of it
-        fwd = np.asarray(target, dtype=np.float64) - np.asarray(eye, dtype=np.float64)
-        fwd /= np.linalg.norm(fwd)
-        for half in (20.0, 60.0, 180.0, 540.0):
-            center = np.asarray(eye) + fwd * half
-            ortho = cam.orthographic_camera(-half, half, half, -half, 0.1, 2000).look_at(center - sun * (half + 300), center)
-            views.append((ortho, 2048, False, None))
-    for camera, eye, target in mains:  # a reflection probe near each main camera: 6 faces
-        probe = np.asarray(eye) + np.array([5.0, -10.0, 5.0])
-        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
-            up = (0, 0, 1) if abs(d[1]) == 1 else (0, 1, 0)
-            face = cam.perspective_camera(90, 1.0, 0.1, 60.0).look_at(probe, probe + np.array(d, dtype=np.float64), up)
-            views.append((face, 512, False, None))
-    xz = rng.uniform(-0.45 * size, 0.45 * size, (count, 2)).astype(np.float32)
-    g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
-    k = 0
-    while len(views) < count:
-        yaw = rng.uniform(0, 2 * math.pi)
-        eye = np.array([xz[k, 0], g[k] + rng.uniform(2, 60), xz[k, 1]])
-        target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
-        camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
-        views.append((camera, 720, True, terrain_depth(camera, 640, 360)))
-        k += 1
-    views = views[:count]
-    out = []
-    for slot, (camera, height, occl, depth) in enumerate(views):
-        # Orthographic views use the reference's perspective LOD formula (projScale = P[5] * h / 2);
-        # they keep every instance regardless of its size on screen.
-        min_px = 0.7 if camera.isPerspectiveCamera else 0.0
-        out.append((vg_make_view(camera, height, slot=slot, occlusion=occl, min_pixel_radius=min_px), depth))
-    return out
-
-
-SERVE_SIZES = (12_000, 50_000, 113_000, 152_000, 500_000, 1_000_000, 2_000_000, 3_000_000)
 SERVE_RATES = (2_000, 5_000, 10_000, 20_000)
-
-
-def serve_requests(resident, count, make_view, cam, scenegen, seed=11):
-    """A pool of cut requests: (scene index, ctypes View array, view count), seeded. Each request is a
-    scene, 1-16 ground views at 720p or 1080p, an error threshold in [0.5, 2] px and, for 30% of them, a
-    triangle budget in [2M, 10M] (threshold range 0.5-3 px)."""
-    from vgcut import native
-
-    rng = np.random.default_rng(seed)
-    pool = []
-    for _ in range(count):
-        k = int(rng.integers(0, len(resident)))
-        spec, size = resident[k][1], resident[k][2]
-        hf = spec.heightfield
-        n = int(rng.integers(1, 17))
-        t = float(rng.uniform(0.5, 2.0))
-        budget = int(rng.integers(2_000_000, 10_000_001)) if rng.random() < 0.3 else 0
-        xz = rng.uniform(-0.45 * size, 0.45 * size, (n, 2)).astype(np.float32)
-        g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
-        arr = (native.View * n)()
-        for i in range(n):
-            yaw = rng.uniform(0, 2 * math.pi)
-            eye = np.array([xz[i, 0], g[i] + rng.uniform(2, 60), xz[i, 1]])
-            target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
-            height = int(rng.choice([720, 1080]))
-            camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
-            arr[i] = make_view(camera, height, slot=i, error_threshold=t, triangle_budget=budget, threshold_range=(0.5, 3.0))
-        pool.append((k, arr, n))
-    return pool
 
 
 def serve_stream(server, ids, pool, rate, count, seed):
@@ -230,21 +143,20 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate)
     t0 = time.time()
     ctx = native.Context(max_triangles=100_000_000, max_shadow_triangles=10_000_000, max_meshlets=1 << 20, max_views=48,
                          mesh_slots=64, device=0)
+    specs = scenes.serving_specs(spec_meshes)
     resident = []
-    for k, n in enumerate(SERVE_SIZES):
-        size = 6144.0 * math.sqrt(n / 3_000_000)
-        spec = scenes.world(instances=n, size=size, seed=100 + k, meshes=spec_meshes)
+    for spec, side in specs:
         scene = native.Scene(ctx)
         scenes.load_into(scene, spec)
-        resident.append((scene, spec, size))
-    pool = serve_requests(resident, 256, make_view, cam, scenegen)
+        resident.append((scene, spec, side))
+    pool = scenes.serve_requests(specs, 256)
     # Each scene's launch schedule, tuned once on its largest request of the pool (both servers use it).
     for k, (scene, _, _) in enumerate(resident):
         mine = [(n, arr) for kk, arr, n in pool if kk == k]
         if mine:
             n, arr = max(mine, key=lambda r: r[0])
             scene.tune(list(arr)[:n], repeats=3)
-    log(f"serving: {len(SERVE_SIZES)} scenes, {sum(s.instance_count for _, s, _ in resident)} instances, "
+    log(f"serving: {len(scenes.SERVE_SIZES)} scenes, {sum(s.instance_count for _, s, _ in resident)} instances, "
         f"schedules tuned ({time.time() - t0:.1f} s)")
 
     def server_for(batched):
@@ -264,7 +176,7 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate)
         tests.append(sum(int(m.clusters_tested[0] + m.clusters_tested[1]) for m in mets))
         plain.release(t)
     capacity = 1e6 / float(np.mean(service))  # plain requests per second, alone and back to back
-    out = {"scenes_instances": list(SERVE_SIZES), "request_pool": len(pool), "views_per_request_mean": float(np.mean([p[2] for p in pool])),
+    out = {"scenes_instances": list(scenes.SERVE_SIZES), "request_pool": len(pool), "views_per_request_mean": float(np.mean([p[2] for p in pool])),
            "cluster_tests_per_request_mean": float(np.mean(tests)), "plain_service_ms_mean": float(np.mean(service)) / 1e3,
            "plain_capacity_req_per_s": capacity, "hbm_resident_bytes": int(plain.stats().resident_bytes), "rates": []}
     batched, batched_ids = server_for(True)
@@ -540,7 +452,7 @@ def main():
         print(json.dumps({"serving": serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
         return
     if args.expand_only:
-        vd = mixed_views(spec, args.views, make_view, cam, scenegen)
+        vd = scenes.mixed_views(spec, args.views)
         schedule = tuple(int(x) for x in args.schedule.split(",")) if args.schedule else (12, 10, 5)
         print(json.dumps({"expand": expand_bench(spec, vd, args.steps, args.warmup, schedule, args.profile_only)}))
         return
@@ -551,7 +463,7 @@ def main():
     scene = native.Scene(ctx)
     t0 = time.time()
     scenes.load_into(scene, spec, occlusion=True)
-    vd = mixed_views(spec, views_n, make_view, cam, scenegen)
+    vd = scenes.mixed_views(spec, views_n)
     views = [v for v, _ in vd]
     for v, depth in vd:
         if depth is not None:

@@ -284,3 +284,42 @@ def test_triangle_budget_pick_and_cut(gpu_available):
             else:
                 ref = oracle.cut(loaded.pool, v)
             compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"budget slot {v.slot} graph={graph}")
+
+
+def test_graph_replay_after_instance_changes(gpu_available):
+    """A captured launch sequence is not replayed after the scene's contents change without a reallocation:
+    a mesh set to no instances, then another mesh's instance count changed on a scene small enough that its
+    cell buffers stay within their first allocation. Each graph cut equals the eager cut and the oracle."""
+    from vgcut import scenes
+
+    spec = small_ruins(seed=57, instances=300, meshes=3)
+    loaded = load_gpu(spec)
+    scene = loaded.scene
+    view = make_view(ground_camera((-25.0, 6.0, -30.0), (10.0, 0.0, 25.0)), 1080)
+    rng = np.random.default_rng(57)
+
+    def check(label):
+        scene.set_state(0, oracle.initial_state())
+        scene.cut([view], graph=True)
+        scene.sync()
+        graph = scene.read(0)
+        scene.set_state(0, oracle.initial_state())
+        scene.cut([view])
+        scene.sync()
+        eager = scene.read(0)
+        pool = oracle.pool_from_scene(scene, loaded.ctx.config.max_meshlets, 2_000_000, 1_000_000)
+        ref = oracle.cut(pool, view)
+        compare_cut(graph, ref, loaded.ctx.config.max_meshlets, f"{label}: graph")
+        compare_cut(eager, ref, loaded.ctx.config.max_meshlets, f"{label}: eager")
+        return int(ref.counters[2])
+
+    drawn = [check("as loaded")]
+    scene.set_instances(0, np.zeros((0, 16), dtype=np.float32))
+    drawn.append(check("mesh 0 without instances"))
+    n = len(spec.matrices[1])
+    m = n + 140  # one more cell
+    more = scenes.yaw_scale_translate(rng.uniform(0, 6.28, m), rng.uniform(0.5, 2.0, m),
+                                      np.stack([rng.uniform(-30, 30, m), np.zeros(m), rng.uniform(-30, 30, m)], 1))
+    scene.set_instances(1, more)
+    drawn.append(check("mesh 1 with 140 more instances"))
+    assert drawn[0] != drawn[1] and drawn[1] != drawn[2], drawn

@@ -197,3 +197,64 @@ def test_node_host_two_phase_equals_python_host(tmp_path, gpu_available):
         assert np.array_equal(draw_set(node_draws[0][n1:], n2 - n1), draw_set(draws[0][n1:], n2 - n1)), f"slot {v.slot} second phase"
         added += n2 - n1
     assert added > 0
+
+
+GUARDS_JS = r"""
+'use strict';
+const vg = require(process.argv[2]);
+const out = {};
+const attempt = (name, fn) => {
+  try { fn(); out[name] = 'ok'; } catch (e) { out[name] = (e.constructor && e.constructor.name) + ':' + (e.code || '') + ':' + e.message; }
+};
+const ctx = vg.createContext({ maxViews: 1, meshSlots: 4, device: -1 });
+const scene = vg.createScene(ctx);
+const mesh = {
+  meshletBounds: new Float32Array(16 * 2 - 1), meshletInfo: new Uint32Array([1, 0, 3, 0, 1, 3, 3, 0]),
+  levelRanges: new Uint32Array([0, 2]), levelErrors: new Float32Array([0, 1]), levelGuardErrors: new Float32Array([0, 1]),
+  boundingSphere: new Float32Array([0, 0, 0, 1]), lodBoundsSphere: new Float32Array([0, 0, 0, 1]),
+  cullBoundsSphere: new Float32Array([0, 0, 0, 1]),
+};
+attempt('shortBounds', () => vg.addMesh(scene, mesh));
+attempt('shortErrors', () => vg.addMesh(scene, Object.assign({}, mesh, { meshletBounds: new Float32Array(32), levelErrors: new Float32Array(1) })));
+attempt('boundsAsFloat64', () => vg.addMesh(scene, Object.assign({}, mesh, { meshletBounds: new Float64Array(32) })));
+const slot = vg.addMesh(scene, Object.assign({}, mesh, { meshletBounds: new Float32Array(32) }));
+attempt('matricesAsUint8', () => vg.setInstances(scene, slot, new Uint8Array(64)));
+attempt('matricesRagged', () => vg.setInstances(scene, slot, new Float32Array(17)));
+vg.setInstances(scene, slot, new Float32Array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]));
+const words = vg.counterWords(ctx), argWords = vg.drawArgsWords(ctx);
+attempt('countersAsUint8', () => vg.read(scene, 0, new Uint8Array(words * 4), new Uint32Array(argWords), new Uint32Array(8)));
+attempt('argsAsInt16', () => vg.read(scene, 0, new Uint32Array(words), new Int16Array(argWords * 2), new Uint32Array(8)));
+attempt('viewProjAsFloat64', () => vg.hzbFromCut(scene, 0, new Float64Array(16), 8, 8));
+// A cut runs on a worker thread: until its promise settles every call on the context throws.
+const promise = vg.cut(scene, new ArrayBuffer(292), 1);
+attempt('readWhileCutting', () => vg.read(scene, 0, new Uint32Array(words), new Uint32Array(argWords), new Uint32Array(8)));
+attempt('setFlagsWhileCutting', () => vg.setFlags(scene, slot, { enabled: true }));
+attempt('secondCutWhileCutting', () => vg.cut(scene, new ArrayBuffer(292), 1));
+attempt('sceneWhileCutting', () => vg.createScene(ctx));
+promise.then(() => { out.cut = 'resolved'; }, (e) => { out.cut = 'rejected:' + e.message; }).then(() => {
+  attempt('setFlagsAfter', () => vg.setFlags(scene, slot, { enabled: true }));
+  console.log(JSON.stringify(out));
+});
+"""
+
+
+@needs_addon
+def test_addon_checks_arrays_and_pending_cuts(tmp_path):
+    """Typed arrays of another element type are refused with a TypeError (not read or written by element
+    counts of the wrong size), short arrays with a RangeError before the library reads them, and every call
+    on a context throws while a cut of it runs on a worker thread; calls work again once it settled. The
+    context is host-only, so the cut itself rejects (no device) after the guard was tested."""
+    script = tmp_path / "guards.js"
+    script.write_text(GUARDS_JS)
+    r = subprocess.run([NODE, str(script), ADDON], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["shortBounds"].startswith("RangeError"), out
+    assert out["shortErrors"].startswith("RangeError"), out
+    for k in ("boundsAsFloat64", "matricesAsUint8", "countersAsUint8", "argsAsInt16", "viewProjAsFloat64"):
+        assert out[k].startswith("TypeError"), (k, out[k])
+    assert out["matricesRagged"].startswith("RangeError"), out
+    for k in ("readWhileCutting", "setFlagsWhileCutting", "secondCutWhileCutting", "sceneWhileCutting"):
+        assert "VG_BUSY" in out[k], (k, out[k])
+    assert out["cut"].startswith("rejected:") and "host-only" in out["cut"], out
+    assert out["setFlagsAfter"] == "ok", out

@@ -583,21 +583,69 @@ int lane_reserve_views(vg_ctx* ctx, Lane* lane) {
   return 0;
 }
 
+// The device view records of `views` (vg_view -> ViewDev) and, for views with a triangle budget, their
+// threshold tables (budget_bins floats per view). Returns whether any view has a budget.
+bool stage_views(vg_scene* s, const vg_view* views, uint32_t count, bool fresh, ViewDev* staging, float* thresholds) {
+  vg_ctx* ctx = s->ctx;
+  bool budget = false;
+  for (uint32_t i = 0; i < count; i++) {
+    const vg_view& in = views[i];
+    ViewDev& v = staging[i];
+    std::memset(&v, 0, sizeof(v));
+    std::memcpy(v.view, in.view_matrix, sizeof(v.view));
+    std::memcpy(v.proj, in.projection_matrix, sizeof(v.proj));
+    std::memcpy(v.planes, in.frustum_planes, sizeof(v.planes));
+    std::memcpy(v.sweep, in.shadow_sweep, sizeof(float) * 3);
+    v.proj_scale = in.proj_scale;
+    v.near_plane = in.near_plane;
+    v.error_threshold = in.error_threshold;
+    v.shadow_error_scale = in.shadow_error_scale;
+    v.lod_blend_max = in.lod_blend_max;
+    v.lod_scale_limit = in.lod_scale_limit;
+    v.min_pixel_radius = in.min_pixel_radius;
+    v.max_error_threshold = in.max_error_threshold;
+    v.frustum = in.frustum_culling;
+    v.occlusion = in.occlusion;
+    v.slot = in.slot;
+    v.budget = in.triangle_budget;
+    v.range_lo = in.threshold_range[0];
+    v.range_hi = in.threshold_range[1];
+    v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
+    v.fresh = fresh ? 1u : 0u;
+    v.two_phase = in.occlusion == 2 ? 1u : 0u;
+    if (in.triangle_budget) {
+      budget = true;
+      vg_budget_thresholds(in.threshold_range[0], in.threshold_range[1], ctx->cfg.budget_bins,
+                           thresholds + static_cast<size_t>(i) * ctx->cfg.budget_bins);
+    }
+  }
+  return budget;
+}
+
+// The checks every cut entry point makes on its views.
+int check_views(vg_ctx* ctx, const vg_view* views, uint32_t count, const char* who) {
+  const std::string w(who);
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, w + ": host-only context (device < 0)");
+  if (!views || count == 0) return fail(ctx, VG_E_INVALID, w + ": no views");
+  if (count > ctx->cfg.max_views || count > kMaxLaunchViews) return fail(ctx, VG_E_INVALID, w + ": more views than max_views");
+  uint64_t seen[kMaxLaunchViews / 64] = {0};
+  for (uint32_t i = 0; i < count; i++) {
+    const uint32_t slot = views[i].slot;
+    if (slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, w + ": view slot out of range");
+    if (seen[slot / 64] >> (slot % 64) & 1ull) return fail(ctx, VG_E_INVALID, w + ": two views share a slot");
+    seen[slot / 64] |= 1ull << (slot % 64);
+    if (views[i].triangle_budget && !(views[i].threshold_range[0] > 0 && views[i].threshold_range[1] >= views[i].threshold_range[0]))
+      return fail(ctx, VG_E_INVALID, w + ": a triangle budget needs 0 < threshold_range[0] <= threshold_range[1]");
+    if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0))
+      return fail(ctx, VG_E_INVALID, w + ": proj_scale and near_plane must be positive");
+    if (views[i].occlusion > 2) return fail(ctx, VG_E_INVALID, w + ": occlusion must be 0, 1 or 2");
+  }
+  return 0;
+}
+
 int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, bool fresh) {
   vg_ctx* ctx = s->ctx;
-  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_cut: host-only context (device < 0)");
-  if (!views || count == 0) return fail(ctx, VG_E_INVALID, "vg_cut: no views");
-  if (count > ctx->cfg.max_views || count > kMaxLaunchViews) return fail(ctx, VG_E_INVALID, "vg_cut: more views than max_views");
-  std::vector<uint8_t> seen(ctx->cfg.max_views, 0);
-  for (uint32_t i = 0; i < count; i++) {
-    if (views[i].slot >= ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_cut: view slot out of range");
-    if (seen[views[i].slot]++) return fail(ctx, VG_E_INVALID, "vg_cut: two views share a slot");
-    if (views[i].triangle_budget && !(views[i].threshold_range[0] > 0 && views[i].threshold_range[1] >= views[i].threshold_range[0]))
-      return fail(ctx, VG_E_INVALID, "vg_cut: a triangle budget needs 0 < threshold_range[0] <= threshold_range[1]");
-    if (!(views[i].proj_scale > 0) || !(views[i].near_plane > 0))
-      return fail(ctx, VG_E_INVALID, "vg_cut: proj_scale and near_plane must be positive");
-    if (views[i].occlusion > 2) return fail(ctx, VG_E_INVALID, "vg_cut: occlusion must be 0, 1 or 2");
-  }
+  if (int rc = check_views(ctx, views, count, "vg_cut")) return rc;
   if (int rc = upload_scene(s)) return rc;
   for (uint32_t i = 0; i < count; i++)
     if (views[i].occlusion == 2) {
@@ -633,39 +681,7 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
     lane->ring_pending[ring] = false;
   }
   ViewDev* staging = static_cast<ViewDev*>(lane->h_ring[ring]);
-  for (uint32_t i = 0; i < count; i++) {
-    const vg_view& in = views[i];
-    ViewDev& v = staging[i];
-    std::memset(&v, 0, sizeof(v));
-    std::memcpy(v.view, in.view_matrix, sizeof(v.view));
-    std::memcpy(v.proj, in.projection_matrix, sizeof(v.proj));
-    std::memcpy(v.planes, in.frustum_planes, sizeof(v.planes));
-    std::memcpy(v.sweep, in.shadow_sweep, sizeof(float) * 3);
-    v.proj_scale = in.proj_scale;
-    v.near_plane = in.near_plane;
-    v.error_threshold = in.error_threshold;
-    v.shadow_error_scale = in.shadow_error_scale;
-    v.lod_blend_max = in.lod_blend_max;
-    v.lod_scale_limit = in.lod_scale_limit;
-    v.min_pixel_radius = in.min_pixel_radius;
-    v.max_error_threshold = in.max_error_threshold;
-    v.frustum = in.frustum_culling;
-    v.occlusion = in.occlusion;
-    v.slot = in.slot;
-    v.budget = in.triangle_budget;
-    v.range_lo = in.threshold_range[0];
-    v.range_hi = in.threshold_range[1];
-    v.hzb_valid = (in.occlusion && s->hzb_valid[in.slot]) ? 1u : 0u;
-    v.fresh = fresh ? 1u : 0u;
-    v.two_phase = in.occlusion == 2 ? 1u : 0u;
-  }
-  lane->launch_budget = false;
-  for (uint32_t i = 0; i < count; i++)
-    if (views[i].triangle_budget) {
-      lane->launch_budget = true;
-      vg_budget_thresholds(views[i].threshold_range[0], views[i].threshold_range[1], ctx->cfg.budget_bins,
-                           lane->h_ring_thresholds[ring] + static_cast<size_t>(i) * ctx->cfg.budget_bins);
-    }
+  lane->launch_budget = stage_views(s, views, count, fresh, staging, lane->h_ring_thresholds[ring]);
   if (lane->launch_budget)
     VG_HIP(ctx, hipMemcpyAsync(lane->d_budget_thresholds.ptr, lane->h_ring_thresholds[ring], sizeof(float) * count * ctx->cfg.budget_bins,
                                hipMemcpyHostToDevice, lane->stream));
@@ -677,8 +693,12 @@ int prepare_cut(vg_scene* s, Lane* lane, const vg_view* views, uint32_t count, b
 
 int launch_graph(vg_scene* s, Lane* lane, uint32_t count) {
   vg_ctx* ctx = s->ctx;
-  const uint64_t key[5] = {s->layout_version, lane->layout_version, static_cast<uint64_t>(lane->launch_budget), schedule_key(s->schedule),
-                           affine_scene(s) ? 1u : 0u};
+  // Everything enqueue_cut bakes into the captured launches: buffer addresses (the layout versions count
+  // reallocations), and the scalars and grids taken from the scene's contents (cell count, used mesh
+  // slots, the reject bitmap's words), which change without a reallocation when instances are set.
+  const uint64_t key[8] = {s->layout_version, lane->layout_version, static_cast<uint64_t>(lane->launch_budget), schedule_key(s->schedule),
+                           affine_scene(s) ? 1u : 0u, static_cast<uint64_t>(s->c_sphere.size()),
+                           (static_cast<uint64_t>(s->mesh_slots) << 32) | s->reject_words, static_cast<uint64_t>(s->footprint)};
   if (!lane->graph_exec || lane->graph_views != count || std::memcmp(key, lane->graph_key, sizeof(key)) != 0) {
     if (lane->graph_exec) (void)hipGraphExecDestroy(lane->graph_exec);
     if (lane->graph) (void)hipGraphDestroy(lane->graph);

@@ -65,7 +65,7 @@ struct Lane {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_views = 0;
-  uint64_t graph_key[5] = {0, 0, 0, 0, 0};
+  uint64_t graph_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace vgi

@@ -23,12 +23,42 @@ static napi_value throw_vg(napi_env env, int rc, vg_ctx* ctx) {
 }
 
 template <typename T>
-static T* typed(napi_env env, napi_value v, size_t* n) {
+struct ArrayType;
+template <>
+struct ArrayType<float> {
+  static constexpr napi_typedarray_type value = napi_float32_array;
+  static constexpr const char* name = "Float32Array";
+};
+template <>
+struct ArrayType<uint32_t> {
+  static constexpr napi_typedarray_type value = napi_uint32_array;
+  static constexpr const char* name = "Uint32Array";
+};
+
+// The data and element count of a typed array of T's own type (Float32Array for float, Uint32Array for
+// uint32_t). Absent (undefined or null): nullptr, *n = 0, nothing thrown. Any other value, a typed array
+// of another element type included: a TypeError is thrown and nullptr returned with *bad set, so a short
+// Uint8Array can never pass a check made in elements of T.
+template <typename T>
+static T* typed(napi_env env, napi_value v, size_t* n, bool* bad, const char* what) {
+  *n = 0;
+  *bad = false;
+  napi_valuetype vt;
+  if (!v || napi_typeof(env, v, &vt) != napi_ok || vt == napi_undefined || vt == napi_null) return nullptr;
   napi_typedarray_type type;
   void* data = nullptr;
   napi_value buffer;
-  size_t offset = 0;
-  if (napi_get_typedarray_info(env, v, &type, n, &data, &buffer, &offset) != napi_ok) return nullptr;
+  size_t offset = 0, count = 0;
+  bool is_typed = false;
+  if (napi_is_typedarray(env, v, &is_typed) != napi_ok || !is_typed ||
+      napi_get_typedarray_info(env, v, &type, &count, &data, &buffer, &offset) != napi_ok || type != ArrayType<T>::value) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "vgcut: %s must be a %s", what, ArrayType<T>::name);
+    napi_throw_type_error(env, nullptr, msg);
+    *bad = true;
+    return nullptr;
+  }
+  *n = count;
   return static_cast<T*>(data);
 }
 
@@ -37,6 +67,12 @@ static napi_value prop(napi_env env, napi_value obj, const char* name) {
   napi_get_named_property(env, obj, name, &v);
   return v;
 }
+
+// A context and whether a cut on it is running on a worker thread (set and cleared on the JS thread).
+struct Ctx {
+  vg_ctx* ctx;
+  bool in_flight;
+};
 
 // createContext({maxTriangles, maxShadowTriangles, maxMeshlets, maxViews, meshSlots, device, expand}) -> external
 static napi_value CreateContext(napi_env env, napi_callback_info info) {
@@ -61,32 +97,75 @@ static napi_value CreateContext(napi_env env, napi_callback_info info) {
   cfg.device = device;
   vg_ctx* ctx = nullptr;
   if (int rc = vg_create(&cfg, &ctx)) return throw_vg(env, rc, nullptr);
+  auto* c = new Ctx{ctx, false};
   napi_value out;
-  CHECK_NAPI(napi_create_external(env, ctx, [](napi_env, void* p, void*) { vg_destroy(static_cast<vg_ctx*>(p)); }, nullptr, &out));
+  CHECK_NAPI(napi_create_external(env, c, [](napi_env, void* p, void*) {
+    auto* c = static_cast<Ctx*>(p);
+    vg_destroy(c->ctx);
+    delete c;
+  }, nullptr, &out));
   return out;
 }
 
+// A scene keeps its context's external alive (ctx_ref) while it exists.
 struct Scene {
+  Ctx* c;
   vg_ctx* ctx;
   vg_scene* scene;
+  napi_ref ctx_ref;
 };
+
+// Calls on a context are made from the JS thread only and never while a cut of that context runs on a
+// worker thread (the library's calls on one context are not thread safe, and the cut may be capturing a
+// graph on the context's stream): every entry point checks the flag and throws instead.
+static bool busy(napi_env env, const Ctx* c) {
+  if (!c->in_flight) return false;
+  napi_throw_error(env, "VG_BUSY", "vgcut: a cut on this context is still running; await its promise first");
+  return true;
+}
+
+static Scene* scene_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "vgcut: expected a scene");
+    return nullptr;
+  }
+  auto* s = static_cast<Scene*>(p);
+  return busy(env, s->c) ? nullptr : s;
+}
+
+static Ctx* ctx_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "vgcut: expected a context");
+    return nullptr;
+  }
+  return static_cast<Ctx*>(p);
+}
 
 // createScene(ctx) -> external (one GeometryPool)
 static napi_value CreateScene(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* ctx = nullptr;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
-  auto* s = new Scene{static_cast<vg_ctx*>(ctx), nullptr};
+  Ctx* c = ctx_of(env, argv[0]);
+  if (!c || busy(env, c)) return nullptr;
+  auto* s = new Scene{c, c->ctx, nullptr, nullptr};
   if (int rc = vg_scene_create(s->ctx, &s->scene)) {
     delete s;
-    return throw_vg(env, rc, static_cast<vg_ctx*>(ctx));
+    return throw_vg(env, rc, c->ctx);
+  }
+  if (napi_create_reference(env, argv[0], 1, &s->ctx_ref) != napi_ok) {
+    vg_scene_destroy(s->scene);
+    delete s;
+    napi_throw_error(env, nullptr, "vgcut: N-API call failed");
+    return nullptr;
   }
   napi_value out;
-  CHECK_NAPI(napi_create_external(env, s, [](napi_env, void* p, void*) {
+  CHECK_NAPI(napi_create_external(env, s, [](napi_env env, void* p, void*) {
     auto* sc = static_cast<Scene*>(p);
     vg_scene_destroy(sc->scene);
+    napi_delete_reference(env, sc->ctx_ref);
     delete sc;
   }, nullptr, &out));
   return out;
@@ -98,33 +177,48 @@ static napi_value AddMesh(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
-  auto* s = static_cast<Scene*>(p);
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   size_t nb = 0, ni = 0, nr = 0, ne = 0, ng = 0, n4 = 0;
+  bool bad = false;
   vg_mesh_desc d;
   memset(&d, 0, sizeof(d));
-  d.meshlet_bounds = typed<float>(env, prop(env, argv[1], "meshletBounds"), &nb);
-  d.meshlet_info = typed<uint32_t>(env, prop(env, argv[1], "meshletInfo"), &ni);
-  d.level_ranges = typed<uint32_t>(env, prop(env, argv[1], "levelRanges"), &nr);
-  d.level_errors = typed<float>(env, prop(env, argv[1], "levelErrors"), &ne);
-  d.level_guard_errors = typed<float>(env, prop(env, argv[1], "levelGuardErrors"), &ng);
+#define VG_FIELD(T, dst, count, name)                                      \
+  do {                                                                     \
+    dst = typed<T>(env, prop(env, argv[1], name), &count, &bad, name);     \
+    if (bad) return nullptr;                                               \
+  } while (0)
+  VG_FIELD(float, d.meshlet_bounds, nb, "meshletBounds");
+  VG_FIELD(uint32_t, d.meshlet_info, ni, "meshletInfo");
+  VG_FIELD(uint32_t, d.level_ranges, nr, "levelRanges");
+  VG_FIELD(float, d.level_errors, ne, "levelErrors");
+  VG_FIELD(float, d.level_guard_errors, ng, "levelGuardErrors");
   size_t nt = 0;
-  d.meshlet_triangles = typed<uint32_t>(env, prop(env, argv[1], "meshletTriangles"), &nt);
+  VG_FIELD(uint32_t, d.meshlet_triangles, nt, "meshletTriangles");
   d.triangle_count = d.meshlet_triangles ? nt : 0;
   size_t npos = 0, nmv = 0;
-  d.positions = typed<float>(env, prop(env, argv[1], "positions"), &npos);
-  d.meshlet_vertices = typed<uint32_t>(env, prop(env, argv[1], "meshletVertices"), &nmv);
+  VG_FIELD(float, d.positions, npos, "positions");
+  VG_FIELD(uint32_t, d.meshlet_vertices, nmv, "meshletVertices");
   d.vertex_count = d.positions ? static_cast<uint32_t>(npos / 3) : 0;
   d.meshlet_vertex_count = d.meshlet_vertices ? nmv : 0;
   d.meshlet_count = static_cast<uint32_t>(ni / 4);
   d.level_count = static_cast<uint32_t>(nr / 2);
+  // Every array the library reads is as long as the counts taken from meshletInfo and levelRanges.
+  if (ni % 4 || nr % 2 || npos % 3 || nb < static_cast<size_t>(d.meshlet_count) * 16 || ne < static_cast<size_t>(d.level_count) * 2 ||
+      ng < static_cast<size_t>(d.level_count) * 2) {
+    napi_throw_range_error(env, nullptr,
+                           "vgcut: addMesh needs meshletBounds of 16, levelErrors and levelGuardErrors of 2 per level "
+                           "(meshletInfo of 4 per meshlet, levelRanges of 2 per level, positions of 3 per vertex)");
+    return nullptr;
+  }
   const char* spheres[3] = {"boundingSphere", "lodBoundsSphere", "cullBoundsSphere"};
   float* dst[3] = {d.bounding_sphere, d.lod_bounds_sphere, d.cull_bounds_sphere};
   for (int k = 0; k < 3; k++) {
-    const float* src = typed<float>(env, prop(env, argv[1], spheres[k]), &n4);
+    const float* src = nullptr;
+    VG_FIELD(float, src, n4, spheres[k]);
     if (src && n4 == 4) memcpy(dst[k], src, sizeof(float) * 4);
   }
+#undef VG_FIELD
   uint32_t slot = 0;
   if (int rc = vg_mesh_add(s->scene, &d, &slot)) return throw_vg(env, rc, s->ctx);
   napi_value out;
@@ -137,15 +231,21 @@ static napi_value SetInstances(napi_env env, napi_callback_info info) {
   size_t argc = 4;
   napi_value argv[4];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
-  auto* s = static_cast<Scene*>(p);
   size_t nm = 0, nc = 0;
-  const float* m = typed<float>(env, argv[2], &nm);
-  const float* c = argc > 3 ? typed<float>(env, argv[3], &nc) : nullptr;
+  bool bad = false;
+  const float* m = typed<float>(env, argv[2], &nm, &bad, "matrices");
+  if (bad) return nullptr;
+  const float* c = argc > 3 ? typed<float>(env, argv[3], &nc, &bad, "colors") : nullptr;
+  if (bad) return nullptr;
   const uint32_t count = static_cast<uint32_t>(nm / 16);
+  if (nm % 16 || (c && nc < static_cast<size_t>(count) * 4)) {
+    napi_throw_range_error(env, nullptr, "vgcut: setInstances needs 16 floats per matrix and 4 per color");
+    return nullptr;
+  }
   napi_value ab, out;
   void* order = nullptr;
   CHECK_NAPI(napi_create_arraybuffer(env, sizeof(uint32_t) * count, &order, &ab));
@@ -159,11 +259,10 @@ static napi_value SetFlags(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
-  auto* s = static_cast<Scene*>(p);
   double max_draw = 1e30, min_px = 0.0;
   bool enabled = true, shadow = true, occlusion = false;
   napi_get_value_double(env, prop(env, argv[2], "maxDrawDistance"), &max_draw);
@@ -182,15 +281,16 @@ static napi_value SetHzb(napi_env env, napi_callback_info info) {
   size_t argc = 5;
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0, width = 0, height = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
   CHECK_NAPI(napi_get_value_uint32(env, argv[3], &width));
   CHECK_NAPI(napi_get_value_uint32(env, argv[4], &height));
-  auto* s = static_cast<Scene*>(p);
   size_t n = 0;
-  const float* depth = typed<float>(env, argv[2], &n);
+  bool bad = false;
+  const float* depth = typed<float>(env, argv[2], &n, &bad, "depth");
+  if (bad) return nullptr;
   if (!depth || n < static_cast<size_t>(width) * height) {
     napi_throw_range_error(env, nullptr, "vgcut: depth array smaller than width x height");
     return nullptr;
@@ -201,11 +301,14 @@ static napi_value SetHzb(napi_env env, napi_callback_info info) {
 
 // The cut (and the second phase) run off the JS thread as napi_async_work and resolve a promise, so the
 // render loop keeps going while the GPU cuts.
+// The scene's external is referenced until the work completes (its finalizer cannot run mid-cut), and the
+// context's in-flight flag is set for as long.
 struct CutWork {
   Scene* s;
   std::vector<vg_view> views;
   bool phase2 = false;  // vg_cut_phase2 instead of the cut
   int rc = 0;
+  napi_ref scene_ref = nullptr;
   napi_deferred deferred;
   napi_async_work work;
 };
@@ -214,20 +317,25 @@ static napi_value QueueCut(napi_env env, napi_callback_info info, bool phase2) {
   size_t argc = 3;
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   void* data = nullptr;
   size_t bytes = 0;
   uint32_t count = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_arraybuffer_info(env, argv[1], &data, &bytes));
   CHECK_NAPI(napi_get_value_uint32(env, argv[2], &count));
   if (bytes < sizeof(vg_view) * count) {
     napi_throw_range_error(env, nullptr, "vgcut: view buffer smaller than count records");
     return nullptr;
   }
-  auto* w = new CutWork{static_cast<Scene*>(p), std::vector<vg_view>(count)};
+  auto* w = new CutWork{s, std::vector<vg_view>(count)};
   w->phase2 = phase2;
   memcpy(w->views.data(), data, sizeof(vg_view) * count);
+  if (napi_create_reference(env, argv[0], 1, &w->scene_ref) != napi_ok) {
+    delete w;
+    napi_throw_error(env, nullptr, "vgcut: N-API call failed");
+    return nullptr;
+  }
   napi_value promise, name;
   CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
   CHECK_NAPI(napi_create_string_utf8(env, "vgcut.cut", NAPI_AUTO_LENGTH, &name));
@@ -241,6 +349,8 @@ static napi_value QueueCut(napi_env env, napi_callback_info info, bool phase2) {
       },
       [](napi_env env, napi_status, void* d) {
         auto* w = static_cast<CutWork*>(d);
+        w->s->c->in_flight = false;
+        napi_delete_reference(env, w->scene_ref);
         napi_value result;
         if (w->rc) {
           napi_value msg;
@@ -256,6 +366,7 @@ static napi_value QueueCut(napi_env env, napi_callback_info info, bool phase2) {
       },
       w, &w->work));
   CHECK_NAPI(napi_queue_async_work(env, w->work));
+  s->c->in_flight = true;
   return promise;
 }
 
@@ -271,14 +382,16 @@ static napi_value ReadPhase2(napi_env env, napi_callback_info info) {
   size_t argc = 4;
   napi_value argv[4];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
-  auto* s = static_cast<Scene*>(p);
   size_t nc = 0, na = 0;
-  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc);
-  uint32_t* args = typed<uint32_t>(env, argv[3], &na);
+  bool bad = false;
+  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc, &bad, "counters");
+  if (bad) return nullptr;
+  uint32_t* args = typed<uint32_t>(env, argv[3], &na, &bad, "drawArgs");
+  if (bad) return nullptr;
   if (!counters || !args || nc < vg_counter_words(s->ctx) || na < vg_draw_args_words(s->ctx)) {
     napi_throw_range_error(env, nullptr, "vgcut: counters or drawArgs array too small");
     return nullptr;
@@ -292,16 +405,19 @@ static napi_value Read(napi_env env, napi_callback_info info) {
   size_t argc = 5;
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
-  auto* s = static_cast<Scene*>(p);
   size_t nc = 0, na = 0, nl = 0;
-  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc);
-  uint32_t* args = typed<uint32_t>(env, argv[3], &na);
-  uint32_t* list = typed<uint32_t>(env, argv[4], &nl);
-  if (nc < vg_counter_words(s->ctx) || na < vg_draw_args_words(s->ctx)) {
+  bool bad = false;
+  uint32_t* counters = typed<uint32_t>(env, argv[2], &nc, &bad, "counters");
+  if (bad) return nullptr;
+  uint32_t* args = typed<uint32_t>(env, argv[3], &na, &bad, "drawArgs");
+  if (bad) return nullptr;
+  uint32_t* list = argc > 4 ? typed<uint32_t>(env, argv[4], &nl, &bad, "drawList") : nullptr;
+  if (bad) return nullptr;
+  if (!counters || !args || nc < vg_counter_words(s->ctx) || na < vg_draw_args_words(s->ctx)) {
     napi_throw_range_error(env, nullptr, "vgcut: counters or drawArgs array too small");
     return nullptr;
   }
@@ -315,16 +431,17 @@ static napi_value ReadIndices(napi_env env, napi_callback_info info) {
   size_t argc = 5;
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t slot = 0, cut = 0;
   double first = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
   CHECK_NAPI(napi_get_value_uint32(env, argv[2], &cut));
   CHECK_NAPI(napi_get_value_double(env, argv[3], &first));
-  auto* s = static_cast<Scene*>(p);
   size_t n = 0;
-  uint32_t* out = typed<uint32_t>(env, argv[4], &n);
+  bool bad = false;
+  uint32_t* out = typed<uint32_t>(env, argv[4], &n, &bad, "out");
+  if (bad) return nullptr;
   if (!out || first < 0) {
     napi_throw_type_error(env, nullptr, "vgcut: readIndices needs a Uint32Array and first >= 0");
     return nullptr;
@@ -338,11 +455,10 @@ static napi_value PoolRead(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
   uint32_t which = 0;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &which));
-  auto* s = static_cast<Scene*>(p);
   uint64_t bytes = 0;
   if (int rc = vg_scene_pool_read(s->scene, which, nullptr, 0, &bytes)) return throw_vg(env, rc, s->ctx);
   napi_value ab;
@@ -356,20 +472,20 @@ static napi_value PoolRead(napi_env env, napi_callback_info info) {
 static napi_value CounterWords(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
-  void* ctx = nullptr;
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
-  CHECK_NAPI(napi_create_uint32(env, vg_counter_words(static_cast<vg_ctx*>(ctx)), &out));
+  Ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  CHECK_NAPI(napi_create_uint32(env, vg_counter_words(c->ctx), &out));
   return out;
 }
 
 static napi_value DrawArgsWords(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
-  void* ctx = nullptr;
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &ctx));
-  CHECK_NAPI(napi_create_uint32(env, vg_draw_args_words(static_cast<vg_ctx*>(ctx)), &out));
+  Ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  CHECK_NAPI(napi_create_uint32(env, vg_draw_args_words(c->ctx), &out));
   return out;
 }
 
@@ -379,14 +495,15 @@ static napi_value HzbFromCut(napi_env env, napi_callback_info info) {
   size_t argc = 6;
   napi_value argv[6];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  void* p = nullptr;
-  CHECK_NAPI(napi_get_value_external(env, argv[0], &p));
-  auto* s = static_cast<Scene*>(p);
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
   uint32_t slot = 0, width = 0, height = 0;
   bool with_phase2 = false;
   CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
   size_t n = 0;
-  const float* vp = typed<float>(env, argv[2], &n);
+  bool bad = false;
+  const float* vp = typed<float>(env, argv[2], &n, &bad, "viewProj");
+  if (bad) return nullptr;
   if (!vp || n != 16) {
     napi_throw_type_error(env, nullptr, "hzbFromCut: viewProj must be a Float32Array of 16");
     return nullptr;
@@ -415,10 +532,13 @@ static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   size_t np = 0, ni = 0;
-  const float* positions = typed<float>(env, argv[0], &np);
-  const uint32_t* indices = typed<uint32_t>(env, argv[1], &ni);
-  if (!positions || !indices) {
-    napi_throw_type_error(env, nullptr, "buildVirtualMesh: positions must be a Float32Array, indices a Uint32Array");
+  bool bad = false;
+  const float* positions = typed<float>(env, argv[0], &np, &bad, "positions");
+  if (bad) return nullptr;
+  const uint32_t* indices = typed<uint32_t>(env, argv[1], &ni, &bad, "indices");
+  if (bad) return nullptr;
+  if (!positions || !indices || np % 3 || ni % 3) {
+    napi_throw_type_error(env, nullptr, "buildVirtualMesh: positions must be a Float32Array of 3 per vertex, indices a Uint32Array of 3 per triangle");
     return nullptr;
   }
   vgd_options o;

@@ -221,3 +221,106 @@ def load_into(scene, spec: SceneSpec, occlusion: bool = False):
                         spec.cast_shadow[k] if spec.cast_shadow else False, occlusion)
         slots.append(slot)
     return slots
+
+
+def mixed_views(spec: SceneSpec, count: int = 64, size: float = 6144.0):
+    """The batched workload's mixed views of a world scene, as (vg_view, occluder depth or None), slots 0..count-1:
+    4 main cameras on the fly-through path (2 at 1920 x 1080, 2 at 3840 x 2160, device pixels, occlusion
+    against the terrain's depth at the reference's HZB sizes), 4 orthographic shadow cascades per main
+    camera at 2048^2 (half extents 20-540 m, toward a fixed sun), 6 cube-map faces of a probe near each
+    main camera at 512^2 (90 degrees, 60 m far) and the rest 720p viewers at seeded poses (occlusion
+    against the terrain at 640 x 360). Orthographic views keep the reference's perspective LOD formula
+    and every instance however small on screen (minPixelRadius 0)."""
+    from .virtual_geometry import make_view
+
+    hf = spec.heightfield
+    path = flythrough(spec, 16, size, 30.0, seed=9)
+    views = []
+    rng = np.random.default_rng(10)
+    sun = np.array([-0.4, -1.0, -0.3])
+    sun /= np.linalg.norm(sun)
+
+    def terrain_depth(camera, w, h):
+        vp = cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse)
+        return scenegen.heightfield_depth(vp, w, h, (0.0, 0.0), size / 2, 256, hf["amplitude"], hf["frequency"], hf["seed"])
+
+    mains = []
+    for i in range(4):
+        eye, target = path[i * 4]
+        width, height = (1920, 1080) if i < 2 else (3840, 2160)
+        camera = cam.perspective_camera(60, width / height, 0.1, 20000).look_at(eye, target)
+        mains.append((camera, eye, target))
+        hz = (960, 540) if i < 2 else (1024, 576)
+        views.append((camera, height, True, terrain_depth(camera, *hz)))
+    for camera, eye, target in mains:  # 4 cascades per main camera, centred ahead of it
+        fwd = np.asarray(target, dtype=np.float64) - np.asarray(eye, dtype=np.float64)
+        fwd /= np.linalg.norm(fwd)
+        for half in (20.0, 60.0, 180.0, 540.0):
+            center = np.asarray(eye) + fwd * half
+            ortho = cam.orthographic_camera(-half, half, half, -half, 0.1, 2000).look_at(center - sun * (half + 300), center)
+            views.append((ortho, 2048, False, None))
+    for camera, eye, target in mains:  # a reflection probe near each main camera: 6 faces
+        probe = np.asarray(eye) + np.array([5.0, -10.0, 5.0])
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            up = (0, 0, 1) if abs(d[1]) == 1 else (0, 1, 0)
+            face = cam.perspective_camera(90, 1.0, 0.1, 60.0).look_at(probe, probe + np.array(d, dtype=np.float64), up)
+            views.append((face, 512, False, None))
+    xz = rng.uniform(-0.45 * size, 0.45 * size, (count, 2)).astype(np.float32)
+    g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
+    k = 0
+    while len(views) < count:
+        yaw = rng.uniform(0, 2 * math.pi)
+        eye = np.array([xz[k, 0], g[k] + rng.uniform(2, 60), xz[k, 1]])
+        target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
+        camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
+        views.append((camera, 720, True, terrain_depth(camera, 640, 360)))
+        k += 1
+    out = []
+    for slot, (camera, height, occl, depth) in enumerate(views[:count]):
+        min_px = 0.7 if camera.isPerspectiveCamera else 0.0
+        out.append((make_view(camera, height, slot=slot, occlusion=occl, min_pixel_radius=min_px), depth))
+    return out
+
+
+# Instance counts of the served resident scenes.
+SERVE_SIZES = (12_000, 50_000, 113_000, 152_000, 500_000, 1_000_000, 2_000_000, 3_000_000)
+
+
+def serving_specs(props, sizes=SERVE_SIZES):
+    """The served resident scenes: world scenes of `sizes` instances over the prop set `props`, each on a
+    square sized for the world's instance density, seeded 100, 101, ...; returns (spec, side length)."""
+    out = []
+    for k, n in enumerate(sizes):
+        side = 6144.0 * math.sqrt(n / 3_000_000)
+        out.append((world(instances=n, size=side, seed=100 + k, meshes=props), side))
+    return out
+
+
+def serve_requests(resident, count: int, seed: int = 11):
+    """A seeded pool of cut requests over `resident` [(spec, side)]: (scene index, ctypes vg_view array, view
+    count). A request is a scene, 1-16 ground views at 720p or 1080p, an error threshold in [0.5, 2] px and,
+    for 30% of them, a triangle budget in [2M, 10M] (threshold range 0.5-3 px)."""
+    from . import native
+    from .virtual_geometry import make_view
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(count):
+        k = int(rng.integers(0, len(resident)))
+        spec, side = resident[k]
+        hf = spec.heightfield
+        n = int(rng.integers(1, 17))
+        t = float(rng.uniform(0.5, 2.0))
+        budget = int(rng.integers(2_000_000, 10_000_001)) if rng.random() < 0.3 else 0
+        xz = rng.uniform(-0.45 * side, 0.45 * side, (n, 2)).astype(np.float32)
+        g = scenegen.heightfield(xz, hf["amplitude"], hf["frequency"], hf["seed"])
+        arr = (native.View * n)()
+        for i in range(n):
+            yaw = rng.uniform(0, 2 * math.pi)
+            eye = np.array([xz[i, 0], g[i] + rng.uniform(2, 60), xz[i, 1]])
+            target = eye + np.array([math.sin(yaw), -0.1, math.cos(yaw)]) * 100
+            height = int(rng.choice([720, 1080]))
+            camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
+            arr[i] = make_view(camera, height, slot=i, error_threshold=t, triangle_budget=budget, threshold_range=(0.5, 3.0))
+        pool.append((k, arr, n))
+    return pool
