@@ -177,6 +177,22 @@ int vg_cut(vg_scene* scene, const vg_view* views, uint32_t count);
 /* Same as vg_cut, replayed from a captured HIP graph when the view count matches the last capture. */
 int vg_cut_graph(vg_scene* scene, const vg_view* views, uint32_t count);
 /*
+ * vg_cut without allocating: no counterpart in the reference. For callers that cannot allocate device
+ * memory while they run (code being captured into a HIP graph, tight serving loops, hosts that forbid
+ * hipMalloc in their frame loop). The cut's launch buffers (view records, the (view, cell) queue, the
+ * work list, its counters and flags, budget tables) live in a static array of 64 MiB that the library
+ * holds on each device it runs on until the process ends, instead of buffers the scene's lane grows.
+ * Limits: at most 64 views; the (view, cell) queue (8 bytes per view per cell) must fit, and the work list
+ * gets what is left (28 bytes per item): a launch whose work list overflows returns VG_E_CAPACITY;
+ * contexts with the expand pass are refused. The scene must already be on the device and each view slot
+ * must have outputs, which a vg_cut of those slots after the last scene change arranges (otherwise
+ * VG_E_INVALID, nothing launched). Calls no hipMalloc/hipFree; waits for the cut before returning.
+ * Every call on a device shares that device's array: calls that overlap on one device (from two threads,
+ * two contexts) race and must not be made.
+ */
+int vg_cut_static(vg_scene* scene, const vg_view* views, uint32_t count);
+
+/*
  * Two-phase occlusion (the reference tests occluders in one phase, OcclusionCulling.ts:4-21; the second
  * phase is this library's): a vg_cut of views with occlusion = 2 records its camera cut's occlusion
  * rejects (instances and meshlets of the final round). After the caller has drawn that cut and set each

@@ -21,6 +21,15 @@ using namespace vgi;
 
 extern "C" int vg_budget_thresholds(float lo, float hi, uint32_t bins, float* out);
 
+// Scratch of vg_cut_static: the launch buffers a cut needs beyond its inputs and outputs (view records,
+// the (view, cell) queue, the work list with its counters and flags, budget tables), in one array the
+// code object holds on every device it is loaded on, for the life of the process. Host code never uses
+// its name as an address: vg_create looks up its device address (hipGetSymbolAddress) with the context's
+// device current and keeps it in the context.
+constexpr size_t kStaticScratchBytes = 64ull << 20;
+constexpr uint32_t kStaticViews = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace vgi {
 
 int fail(vg_ctx* ctx, int code, const std::string& message) {
@@ -801,6 +810,16 @@ int vg_create(const vg_config* config, vg_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c.device) == hipSuccess && prop.multiProcessorCount > 0) ctx->cus = prop.multiProcessorCount;
+  // vg_cut_static's scratch: this device's copy of the array, and host staging sized for its largest call.
+  void* scratch_ptr = nullptr;
+  if (hipGetSymbolAddress(&scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess || !scratch_ptr) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VG_E_DEVICE;
+  }
+  ctx->static_scratch = scratch_ptr;
+  ctx->static_views.assign(sizeof(ViewDev) * kStaticViews, 0);
+  ctx->static_thresholds.assign(static_cast<size_t>(kStaticViews) * c.budget_bins, 0.0f);
   *out = ctx;
   return VG_OK;
 }
@@ -1265,6 +1284,86 @@ int vg_cut_graph(vg_scene* s, const vg_view* views, uint32_t count) {
   if (int rc = prepare_cut(s, &s->lane, views, count, false)) return rc;
   if (int rc = launch_graph(s, &s->lane, count)) return rc;
   mark_phase1(s, views, count);
+  return VG_OK;
+}
+
+int vg_cut_static(vg_scene* s, const vg_view* views, uint32_t count) {
+  if (!s) return VG_E_INVALID;
+  vg_ctx* ctx = s->ctx;
+  if (count > kStaticViews) return fail(ctx, VG_E_CAPACITY, "vg_cut_static: more than 64 views (the size its static scratch is laid out for)");
+  if (int rc = check_views(ctx, views, count, "vg_cut_static")) return rc;
+  if (!ctx->static_scratch) return fail(ctx, VG_E_DEVICE, "vg_cut_static: no static scratch on this context's device");
+  if (ctx->cfg.expand)
+    return fail(ctx, VG_E_INVALID, "vg_cut_static: contexts with the expand pass are not supported (its per-entry records do not fit the static scratch)");
+  if (s->dirty_table || s->dirty_meshlets || s->dirty_instances || s->dirty_cells)
+    return fail(ctx, VG_E_INVALID, "vg_cut_static: the scene has changes that are not on the device yet (a vg_cut uploads them)");
+  const uint32_t reject_words = static_cast<uint32_t>((s->i_col[0].size() + 31) / 32 + 1);
+  for (uint32_t i = 0; i < count; i++) {
+    if (!s->d_draws[views[i].slot].ptr)
+      return fail(ctx, VG_E_INVALID, "vg_cut_static: a view slot without outputs yet (a vg_cut of the slot allocates them)");
+    if (views[i].occlusion == 2 && (!s->d_phase.ptr || s->reject_words < reject_words))
+      return fail(ctx, VG_E_INVALID, "vg_cut_static: two-phase occlusion needs the scene's two-phase buffers (a vg_cut with occlusion 2 allocates them)");
+  }
+  // Layout of the scratch for this call, 256-byte aligned pieces; the work list takes what is left.
+  const uint32_t bins = ctx->cfg.budget_bins;
+  const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
+  size_t used = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = used;
+    used += (bytes + 255) & ~static_cast<size_t>(255);
+    return at;
+  };
+  const size_t at_views = take(sizeof(ViewDev) * count), at_queue_count = take(64);
+  const size_t at_work_counter = take(sizeof(unsigned long long) * kWorkParts * kWorkCounterSpacing), at_flags = take(64);
+  const size_t at_thresholds = take(sizeof(float) * count * bins), at_hist = take(sizeof(unsigned long long) * count * (bins + 1));
+  const size_t at_plan = take(sizeof(unsigned long long) * (2 * static_cast<size_t>(count) + 1));
+  const size_t at_queue = take(sizeof(uint2) * cells * count);
+  const size_t per_item = sizeof(uint4) + sizeof(unsigned long long) + sizeof(float);
+  const size_t left = used + 3 * 256 < kStaticScratchBytes ? kStaticScratchBytes - used - 3 * 256 : 0;
+  const size_t per_part = std::min<size_t>(left / per_item / kWorkParts, kMaxWorkItems / kWorkParts);
+  if (per_part == 0)
+    return fail(ctx, VG_E_CAPACITY, "vg_cut_static: " + std::to_string(count) + " views x " + std::to_string(cells) +
+                                        " cells do not fit its 64 MiB static scratch");
+  const size_t items = per_part * kWorkParts;
+  const size_t at_work = take(sizeof(uint4) * items), at_start = take(sizeof(unsigned long long) * items), at_scale = take(sizeof(float) * items);
+  if (used > kStaticScratchBytes) return fail(ctx, VG_E_CAPACITY, "vg_cut_static: the launch does not fit its static scratch");
+  char* base = static_cast<char*>(ctx->static_scratch);
+  Lane lane;
+  lane.stream = ctx->stream;
+  lane.borrowed = true;
+  auto piece = [&](DevBuf& b, size_t at, size_t bytes) {
+    b.ptr = base + at;
+    b.bytes = bytes;
+  };
+  piece(lane.d_views, at_views, sizeof(ViewDev) * count);
+  piece(lane.d_cell_queue_count, at_queue_count, 64);
+  piece(lane.d_work_counter, at_work_counter, sizeof(unsigned long long) * kWorkParts * kWorkCounterSpacing);
+  piece(lane.d_flags, at_flags, 64);
+  piece(lane.d_budget_thresholds, at_thresholds, sizeof(float) * count * bins);
+  piece(lane.d_budget_hist, at_hist, sizeof(unsigned long long) * count * (bins + 1));
+  piece(lane.d_expand_plan, at_plan, sizeof(unsigned long long) * (2 * static_cast<size_t>(count) + 1));
+  piece(lane.d_cell_queue, at_queue, sizeof(uint2) * cells * count);
+  piece(lane.d_work, at_work, sizeof(uint4) * items);
+  piece(lane.d_work_start, at_start, sizeof(unsigned long long) * items);
+  piece(lane.d_work_scale, at_scale, sizeof(float) * items);
+  lane.cell_queue_capacity = cells * count;
+  lane.work_capacity = static_cast<uint32_t>(items);
+  lane.view_capacity = count;
+  // Everything the kernels need zeroed (queue count, work counters, flags, histograms) k_begin clears.
+  lane.launch_budget = stage_views(s, views, count, false, reinterpret_cast<ViewDev*>(ctx->static_views.data()), ctx->static_thresholds.data());
+  if (lane.launch_budget)
+    VG_HIP(ctx, hipMemcpyAsync(lane.d_budget_thresholds.ptr, ctx->static_thresholds.data(), sizeof(float) * count * bins,
+                               hipMemcpyHostToDevice, ctx->stream));
+  VG_HIP(ctx, hipMemcpyAsync(lane.d_views.ptr, ctx->static_views.data(), sizeof(ViewDev) * count, hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = enqueue_cut(s, &lane, count, false)) return rc;
+  VG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  uint32_t flags[4] = {0, 0, 0, 0};
+  VG_HIP(ctx, hipMemcpy(flags, lane.d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
+  mark_phase1(s, views, count);
+  if (flags[3])
+    return fail(ctx, VG_E_CAPACITY, "vg_cut_static: the work list (" + std::to_string(items) +
+                                        " items in the static scratch) overflowed; results are incomplete");
+  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut_static: meshlet pass lost its work item (internal error)");
   return VG_OK;
 }
 

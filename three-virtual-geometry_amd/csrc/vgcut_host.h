@@ -75,6 +75,12 @@ struct vg_ctx {
   hipStream_t stream = nullptr;
   int cus = 256;
   std::string error;
+  // vg_cut_static: the device address of the library's static scratch array on this context's device
+  // (from hipGetSymbolAddress at vg_create), and host staging for its view records and budget tables,
+  // sized at vg_create so that the call itself allocates nothing.
+  void* static_scratch = nullptr;
+  std::vector<unsigned char> static_views;
+  std::vector<float> static_thresholds;
 };
 
 struct vg_scene {

@@ -37,7 +37,7 @@ EXPORTS = (
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
-    "vg_cut_phase2", "vg_read_phase2", "vg_phase_rejects",
+    "vg_cut_phase2", "vg_read_phase2", "vg_phase_rejects", "vg_cut_static",
     "vg_server_create", "vg_server_destroy", "vg_server_last_error", "vg_server_add_scene", "vg_server_submit", "vg_server_poll",
     "vg_server_read", "vg_server_release", "vg_server_stats_get",
 )
@@ -199,6 +199,7 @@ def load():
         "vg_hzb_read": ([P, U32, P, U64, ctypes.POINTER(U64)], ctypes.c_int),
         "vg_cut": ([P, P, U32], ctypes.c_int),
         "vg_cut_graph": ([P, P, U32], ctypes.c_int),
+        "vg_cut_static": ([P, P, U32], ctypes.c_int),
         "vg_sync": ([P], ctypes.c_int),
         "vg_read": ([P, U32, P, P, P, U32, ctypes.POINTER(Metrics)], ctypes.c_int),
         "vg_state_get": ([P, U32, P], ctypes.c_int),
@@ -369,6 +370,13 @@ class Scene:
         arr = (View * len(views))(*views)
         fn = self.lib.vg_cut_graph if graph else self.lib.vg_cut
         self.ctx.check(fn(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views)), "vg_cut")
+        self._keep = arr
+
+    def cut_static(self, views):
+        """vg_cut_static: the cut with its launch buffers in the library's static scratch (no allocation;
+        waits for the cut)."""
+        arr = (View * len(views))(*views)
+        self.ctx.check(self.lib.vg_cut_static(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(views)), "vg_cut_static")
         self._keep = arr
 
     def sync(self):
