@@ -83,7 +83,25 @@ def load():
     if _lib is None:
         if not os.path.exists(LIBRARY):
             build()
-        lib = ctypes.CDLL(LIBRARY)
+        _lib = _bind(ctypes.CDLL(LIBRARY))
+    return _lib
+
+
+def build_variant(out_path: str, mutant: int) -> str:
+    """Compiles the oracle with one planted mistake (-DOR_MUTANT=mutant, see vgoracle.c) into out_path, for
+    tests that show the oracle's checks catch such mistakes. Returns the path."""
+    subprocess.run(["gcc", "-O1", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-D_GNU_SOURCE",
+                    f"-DOR_MUTANT={int(mutant)}", "-shared", "-o", out_path, os.path.join(HERE, "vgoracle.c"), "-lm"], check=True)
+    return out_path
+
+
+def load_variant(path: str):
+    """A ctypes handle on another build of the oracle (build_variant), bound like the default one."""
+    return _bind(ctypes.CDLL(path))
+
+
+def _bind(lib):
+    if True:
         lib.or_counter_words.argtypes = [ctypes.c_uint32]
         lib.or_counter_words.restype = ctypes.c_uint32
         lib.or_cut.argtypes = [ctypes.POINTER(OrScene), ctypes.POINTER(OrView), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -111,8 +129,13 @@ def load():
                                   ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32,
                                   ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
         lib.or_raster.restype = ctypes.c_int
-        _lib = lib
-    return _lib
+        P = ctypes.c_void_p
+        lib.or_test_controller.argtypes = [ctypes.c_int, P, P, P, ctypes.c_uint32, ctypes.c_float, P, P]
+        lib.or_test_controller.restype = ctypes.c_int
+        lib.or_test_prefix.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, P, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, ctypes.c_uint32, P, P]
+        lib.or_test_prefix.restype = ctypes.c_int
+    return lib
 
 
 @dataclass
@@ -213,8 +236,9 @@ def initial_state() -> np.ndarray:
     return s
 
 
-def cut(pool: Pool, view, state: Optional[np.ndarray] = None, hzb: Optional[np.ndarray] = None, fp64: bool = False) -> CutResult:
-    lib = load()
+def cut(pool: Pool, view, state: Optional[np.ndarray] = None, hzb: Optional[np.ndarray] = None, fp64: bool = False,
+        lib=None) -> CutResult:
+    lib = lib or load()
     st = initial_state() if state is None else np.ascontiguousarray(state, dtype=np.uint32).copy()
     counters = np.zeros(lib.or_counter_words(pool.slots), dtype=np.uint32)
     args = np.zeros(pool.slots * 20, dtype=np.uint32)
@@ -361,3 +385,33 @@ def budget_pick(pool: Pool, view, budget: int, thresholds: np.ndarray, state: Op
     chosen.lod_scale_limit = float(max(np.float32(1.0), np.float32(view.max_error_threshold) / t))
     chosen.triangle_budget = 0
     return b, tris, chosen
+
+
+def controller(steer: bool, requested, selected, capacity, max_meshlets: int, lod_scale_limit: float,
+               state: Optional[np.ndarray] = None, lib=None):
+    """checkNode (steer False) or steerNode (steer True) of both cuts on given counts (GeometryPool.ts:1027-1069):
+    (the 8 state words after it, [redo camera, redo shadow])."""
+    lib = lib or load()
+    st = initial_state() if state is None else np.ascontiguousarray(state, dtype=np.uint32).copy()
+    req = np.ascontiguousarray(requested, dtype=np.uint32)
+    sel = np.ascontiguousarray(selected, dtype=np.uint32)
+    cap = np.ascontiguousarray(capacity, dtype=np.uint32)
+    redo = np.zeros(2, dtype=np.int32)
+    lib.or_test_controller(1 if steer else 0, req.ctypes.data, sel.ctypes.data, cap.ctypes.data, max_meshlets, lod_scale_limit,
+                           st.ctypes.data, redo.ctypes.data)
+    return st, redo.tolist()
+
+
+def prefix(slots: int, counts, capacity: int, cut: int = 0, selected: int = 0, max_meshlets: int = 1 << 20, offset: int = 0,
+           lib=None):
+    """prefixPass of one cut (GeometryPool.ts:973-995) on per-(slot, part) index counts [[part 0, part 1], ...]:
+    (counters, draw args of both cuts)."""
+    lib = lib or load()
+    c = np.ascontiguousarray(counts, dtype=np.uint32).reshape(-1)
+    counters = np.zeros(lib.or_counter_words(slots), dtype=np.uint32)
+    args = np.zeros(2 * slots * 2 * 5, dtype=np.uint32)
+    rc = lib.or_test_prefix(slots, len(c) // 2, cut, c.ctypes.data, capacity, selected, max_meshlets, offset, counters.ctypes.data,
+                            args.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("or_test_prefix failed")
+    return counters, args

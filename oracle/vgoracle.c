@@ -43,6 +43,23 @@
 #define K_MARGIN_HI ((float)(1 + 1e-3))
 #define K_STATE_FIXED 10000.0f
 
+/*
+ * OR_MUTANT (tests only; 0 in every build the tests check against): compiled with -DOR_MUTANT=k, the
+ * oracle carries one plausible mistake, k-th of the list below, so a test can show that its known answers
+ * catch it. 1-5: a capacity-controller constant (GeometryPool.ts:131-137, 1032-1069); 6-7: the prefix
+ * (GeometryPool.ts:973-995); 8-11: the distance and size culls (GeometryPool.ts:742-757, 877-893).
+ */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+#define FILL_TARGET (OR_MUTANT == 1 ? 0.9f : 0.82f)
+#define FILL_HIGH (OR_MUTANT == 2 ? 0.95f : 0.9f)
+#define RETRY_MAX (OR_MUTANT == 3 ? 1e30f : 4.0f)
+#define STEER_UP_MAX (OR_MUTANT == 4 ? 1.05f : 1.03f)
+#define STEER_DOWN (OR_MUTANT == 5 ? 1.02f : 1.01f)
+#define FILL_LOW 0.75f
+#define RETRY_MIN 1.1f
+
 typedef struct {
   float x, y, z, w;
 } f4;
@@ -371,8 +388,8 @@ static void cells_pass(ctx_t* c) {
     f4 vc = mat_mul(v->view, point(sphere));
     float view_distance = len3(vc.x, vc.y, vc.z);
     float gap = view_distance - radius;
-    if (gap > max_radius && (max_radius * v->proj_scale) / fmaxf(gap, 1e-6f) < min_pixels) continue;
-    if (gap > levers[0]) continue;
+    if ((gap > max_radius || OR_MUTANT == 8) && (max_radius * v->proj_scale) / fmaxf(gap, 1e-6f) < min_pixels) continue;
+    if ((OR_MUTANT == 9 ? view_distance : gap) > levers[0]) continue;
     if (c->phase == 2 && !cell_has_reject(c, (uint32_t)record[2], (uint32_t)record[3])) continue;
     uint32_t at = c->counters[0]++;
     uint32_t* e = &c->cell_list[at * 4];
@@ -463,7 +480,7 @@ static void instance_pass(ctx_t* c, int retry) {
     const float* cull_bounds = mesh_record(s, slot, 2);
     const float* flags = mesh_record(s, slot, 5);
     int shadow_on = flags[1] > 0.5f, occlusion_on = flags[2] > 0.5f;
-    float min_pixels = fmaxf(v->min_pixel_radius, levers[1]);
+    float min_pixels = OR_MUTANT == 11 ? v->min_pixel_radius : fmaxf(v->min_pixel_radius, levers[1]);
     float max_draw_distance = levers[0];
     for (uint32_t i = cell[1]; i < cell[1] + cell[2]; i++) {
       if (c->phase == 2 && !c->reject_inst[i]) continue;
@@ -482,7 +499,7 @@ static void instance_pass(ctx_t* c, int retry) {
       f4 vc = mat_mul(v->view, (f4){center.x, center.y, center.z, 1.0f});
       float view_distance = len3(vc.x, vc.y, vc.z);
       int too_small = view_distance > radius && (radius * v->proj_scale) / view_distance < min_pixels;
-      int too_far = (view_distance - radius) > max_draw_distance;
+      int too_far = (OR_MUTANT == 10 ? view_distance : view_distance - radius) > max_draw_distance;
       if (!((in_camera || in_shadow) && !too_small && !too_far)) continue;
       if (in_camera && occluded_sphere(c, occlusion_on, center.x, center.y, center.z, radius)) {
         in_camera = 0;
@@ -571,13 +588,14 @@ static void meshlet_pass(ctx_t* c, int cut) {
 static void prefix_pass_from(ctx_t* c, int cut, uint32_t offset) {
   const or_scene* s = c->sc;
   uint32_t start = offset, capacity = s->cut_capacity[cut];
-  for (uint32_t i = 0; i < s->mesh_slots; i++)
-    for (uint32_t part = 0; part < 2; part++) {
+  for (uint32_t k = 0; k < s->mesh_slots * 2; k++) {
+      /* slot-major: (slot 0, part 0), (slot 0, part 1), (slot 1, part 0), ... */
+      const uint32_t i = OR_MUTANT == 7 ? k % s->mesh_slots : k / 2, part = OR_MUTANT == 7 ? k / s->mesh_slots : k % 2;
       uint32_t requested = *mesh_count(c, cut, i, part);
       *mesh_start(c, cut, i, part) = start;
       uint32_t room = start < capacity ? capacity - start : 0u;
       uint32_t* a = &c->args[cut * s->slots * 10 + part * 5 + i * 10];
-      a[0] = requested < room ? requested : room;
+      a[0] = requested < room || OR_MUTANT == 6 ? requested : room;
       a[1] = 1;
       a[2] = start;
       a[3] = 0;
@@ -607,7 +625,7 @@ static int check_pass(ctx_t* c) {
     float scale = get_state(c, S_SCALE + cut);
     set_state(c, S_RETRY + cut, 0.0f);
     if (fill > 1.0f && scale < c->v->lod_scale_limit) {
-      set_state(c, S_SCALE + cut, fminf(scale * clampf(sqrtf(fill / 0.82f), 1.1f, 4.0f), c->v->lod_scale_limit));
+      set_state(c, S_SCALE + cut, fminf(scale * clampf(sqrtf(fill / FILL_TARGET), RETRY_MIN, RETRY_MAX), c->v->lod_scale_limit));
       set_state(c, S_RETRY + cut, 1.0f);
       any = 1;
       c->st->retries[cut]++;
@@ -628,8 +646,8 @@ static void steer_pass(ctx_t* c) {
   for (int cut = 0; cut < 2; cut++) {
     float fill = fill_of(c, cut);
     float scale = get_state(c, S_SCALE + cut);
-    if (fill > 0.9f) scale = scale * fminf(sqrtf(fill / 0.82f), 1.03f);
-    else if (fill < 0.75f) scale = scale / 1.01f;
+    if (fill > FILL_HIGH) scale = scale * fminf(sqrtf(fill / FILL_TARGET), STEER_UP_MAX);
+    else if (fill < FILL_LOW) scale = scale / STEER_DOWN;
     set_state(c, S_SCALE + cut, fmaxf(fminf(scale, c->v->lod_scale_limit), 1.0f));
     set_state(c, S_FILL + cut, fill);
   }
@@ -1015,5 +1033,73 @@ int or_raster(const float* instances, uint32_t instance_count, const uint32_t* i
       raster_triangle(v[packed & 255u], v[(packed >> 8) & 255u], v[(packed >> 16) & 255u], width, height, depth);
     }
   }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- test hooks */
+
+/* The capacity controller on given counts (no scene): checkNode (steer 0) or steerNode (steer 1) for both
+   cuts, GeometryPool.ts:1027-1069, updating the 8 state words; redo[cut] = whether checkNode redoes it. */
+int or_test_controller(int steer, const uint32_t* requested, const uint32_t* selected, const uint32_t* capacity,
+                       uint32_t max_meshlets, float lod_scale_limit, uint32_t* state, int* redo) {
+  or_scene sc;
+  or_view v;
+  or_stats st;
+  ctx_t c;
+  memset(&sc, 0, sizeof(sc));
+  memset(&v, 0, sizeof(v));
+  memset(&st, 0, sizeof(st));
+  memset(&c, 0, sizeof(c));
+  uint32_t counters[PER_MESH_COUNTERS + 8];
+  memset(counters, 0, sizeof(counters));
+  sc.slots = 1;
+  sc.mesh_slots = 1;
+  sc.max_meshlets = max_meshlets;
+  sc.cut_capacity[0] = capacity[0];
+  sc.cut_capacity[1] = capacity[1];
+  v.lod_scale_limit = lod_scale_limit;
+  c.sc = &sc;
+  c.v = &v;
+  c.st = &st;
+  c.state = state;
+  c.counters = counters;
+  for (int cut = 0; cut < 2; cut++) {
+    *counter(&c, cut, C_REQUESTED) = requested[cut];
+    *counter(&c, cut, C_SELECTED) = selected[cut];
+  }
+  if (steer) {
+    steer_pass(&c);
+  } else {
+    check_pass(&c);
+    for (int cut = 0; cut < 2; cut++) redo[cut] = (int)st.retries[cut];
+  }
+  return 0;
+}
+
+/* prefixPass (GeometryPool.ts:973-995) of one cut on given per-(slot, part) index counts (2 per slot,
+   slot-major), from region `offset`: fills `counters` (or_counter_words(slots) words; the pairs' starts,
+   C_REQUESTED, C_DRAWN from `selected`) and that cut's draw args in `args` (2 * slots * 2 * 5 words). */
+int or_test_prefix(uint32_t slots, uint32_t mesh_slots, int cut, const uint32_t* counts, uint32_t capacity, uint32_t selected,
+                   uint32_t max_meshlets, uint32_t offset, uint32_t* counters, uint32_t* args) {
+  if (mesh_slots > slots || cut < 0 || cut > 1) return -1;
+  or_scene sc;
+  or_stats st;
+  ctx_t c;
+  memset(&sc, 0, sizeof(sc));
+  memset(&st, 0, sizeof(st));
+  memset(&c, 0, sizeof(c));
+  sc.slots = slots;
+  sc.mesh_slots = mesh_slots;
+  sc.max_meshlets = max_meshlets;
+  sc.cut_capacity[cut] = capacity;
+  c.sc = &sc;
+  c.st = &st;
+  c.counters = counters;
+  c.args = args;
+  memset(counters, 0, sizeof(uint32_t) * or_counter_words(slots));
+  for (uint32_t i = 0; i < mesh_slots; i++)
+    for (uint32_t part = 0; part < 2; part++) *mesh_count(&c, cut, i, part) = counts[i * 2 + part];
+  *counter(&c, cut, C_SELECTED) = selected;
+  prefix_pass_from(&c, cut, offset);
   return 0;
 }

@@ -133,6 +133,18 @@ int or_raster(const float* instances, uint32_t instance_count, const uint32_t* i
               uint64_t meshlet_vertex_words, const float* vertices, uint64_t vertex_count, const uint32_t* draw_list,
               uint32_t count, const float* view_proj, uint32_t width, uint32_t height, float* depth);
 
+/*
+ * Test hooks. or_test_controller: checkNode (steer = 0) or steerNode (steer = 1) of both cuts on given
+ * requested indices, selected meshlets and capacities (GeometryPool.ts:1027-1069), updating `state` (8
+ * words); redo[cut] receives whether checkNode redoes that cut. or_test_prefix: prefixPass of one cut
+ * (GeometryPool.ts:973-995) on given index counts (2 per slot, slot-major) from region `offset`, into
+ * `counters` (or_counter_words(slots) words) and `args` (2 * slots * 2 * 5 words).
+ */
+int or_test_controller(int steer, const uint32_t* requested, const uint32_t* selected, const uint32_t* capacity,
+                       uint32_t max_meshlets, float lod_scale_limit, uint32_t* state, int* redo);
+int or_test_prefix(uint32_t slots, uint32_t mesh_slots, int cut, const uint32_t* counts, uint32_t capacity, uint32_t selected,
+                   uint32_t max_meshlets, uint32_t offset, uint32_t* counters, uint32_t* args);
+
 #ifdef __cplusplus
 }
 #endif
