@@ -320,27 +320,46 @@ def two_phase_bench(scene, vd, steps):
     from vgcut import camera as cam
     from vgcut import native
 
-    views, rasters = [], []
-    for v, d in vd:
-        w = native.View()
-        ctypes.memmove(ctypes.byref(w), ctypes.byref(v), ctypes.sizeof(native.View))
-        if d is not None and v.occlusion:
-            w.occlusion = 2
-            vp = np.asarray(cam.mat_mul(np.array(v.projection_matrix[:]), np.array(v.view_matrix[:])), dtype=np.float32)
-            rasters.append((w.slot, vp, d.shape[1], d.shape[0]))
-        views.append(w)
-    two = [v for v in views if v.occlusion == 2]
-    initial = np.zeros(8, dtype=np.uint32)
-    initial[0] = initial[1] = 10000
+    # The occlusion-culled views move: each frame every one of them steps `speed` metres along its own forward
+    # axis (the view matrix's z translation), so the previous frame's visible set, drawn from the new camera,
+    # leaves gaps the second phase fills. The other views stay where they are.
+    speed = 1.5
+    occluded = [bool(d is not None and v.occlusion) for v, d in vd]
+    size = {v.slot: (d.shape[1], d.shape[0]) for v, d in vd if d is not None}
 
-    def frame(timed):
+    def views_at(f):
+        views, rasters = [], []
+        for (v, _), occl in zip(vd, occluded):
+            w = native.View()
+            ctypes.memmove(ctypes.byref(w), ctypes.byref(v), ctypes.sizeof(native.View))
+            if occl:
+                V = np.array(v.view_matrix[:], dtype=np.float64)
+                V[14] += speed * f
+                P = np.array(v.projection_matrix[:], dtype=np.float64)
+                planes = cam.frustum_planes(cam.mat_mul(P, V)).reshape(24)
+                for i in range(16):
+                    w.view_matrix[i] = float(V[i])
+                for i in range(24):
+                    w.frustum_planes[i] = float(planes[i])
+                w.occlusion = 2
+                vp = np.asarray(cam.mat_mul(P, np.array(w.view_matrix[:], dtype=np.float64)), dtype=np.float32)
+                rasters.append((w.slot, vp) + size[w.slot])
+            views.append(w)
+        return views, [v for v in views if v.occlusion == 2], rasters
+
+    warm = 3
+    frames = [views_at(f) for f in range(warm + steps)]
+
+    def frame(f):
+        views, two, rasters = frames[f]
         t = [0.0, 0.0, 0.0, 0.0]
         def mark(i, t0):
             scene.sync()
             t[i] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        for slot, vp, w, h in rasters:
-            scene.hzb_from_cut(slot, vp, w, h, with_phase2=True)
+        if f > 0:
+            for slot, vp, w, h in rasters:
+                scene.hzb_from_cut(slot, vp, w, h, with_phase2=True)
         mark(0, t0)
         t0 = time.perf_counter()
         scene.cut(views, graph=True)
@@ -354,18 +373,18 @@ def two_phase_bench(scene, vd, steps):
         mark(3, t0)
         return t
 
-    scene.cut(views, graph=True)  # a first frame: the previous visible set for the rasters
-    scene.sync()
-    for _ in range(2):
-        frame(False)
+    for f in range(warm):
+        frame(f)
     total = [0.0, 0.0, 0.0, 0.0]
+    added_frames = []
     scene.profile_read()
     scene.profile(True)
-    for _ in range(steps):
-        for v in views:
-            scene.set_state(v.slot, initial)
+    for f in range(warm, warm + steps):
         torch.cuda.synchronize()
-        total = [a + b for a, b in zip(total, frame(True))]
+        total = [a + b for a, b in zip(total, frame(f))]
+        added_frames.append(sum(int(scene.read_phase2(v.slot)[0][2]) - int(scene.read(v.slot, max_entries=1)[0][2])
+                                for v in frames[f][1]))
+    views, two, rasters = frames[-1]
     kernels, _ = scene.profile_read()
     scene.profile(False)
     raster_kernel_ms = dict(kernels)["k_raster (+ chunks)"] / steps  # both rasters of a frame
@@ -397,8 +416,8 @@ def two_phase_bench(scene, vd, steps):
         mm = scene.ctx.config.max_meshlets
         n1, n2 = min(int(c1[2]), mm), min(int(c2[2]), mm)
         drawn += n1
-        # A frame's two rasters: the previous frame's cut (both phases; the views and the controller state
-        # repeat, so it equals this frame's) and this frame's first phase.
+        # A frame's two rasters: the previous frame's cut (both phases; counted as this frame's, which the
+        # camera's small step leaves about the same) and this frame's first phase.
         raster_bytes += entry_bytes(draws[0, :n2, 1].astype(np.int64)) + entry_bytes(draws[0, :n1, 1].astype(np.int64))
     achieved = raster_bytes / (raster_kernel_ms * 1e-3) / 1e9 if raster_kernel_ms > 0 else None
     traffic, source = 0.0, []
@@ -409,9 +428,10 @@ def two_phase_bench(scene, vd, steps):
             break
         traffic += t * 2 * len(rasters)  # per dispatch; two rasters per view per frame
         source.append(src)
-    out = {"views_two_phase": len(two), "ms_per_frame": sum(ms), "raster_previous_ms": ms[0], "cut_ms_with_recording": ms[1],
-           "raster_first_phase_ms": ms[2], "phase2_ms": ms[3], "instances_rejected": rejected_instances,
-           "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added, "raster_entries_first_phase": drawn,
+    out = {"views_two_phase": len(two), "camera_step_m_per_frame": speed, "ms_per_frame": sum(ms), "raster_previous_ms": ms[0],
+           "cut_ms_with_recording": ms[1], "raster_first_phase_ms": ms[2], "phase2_ms": ms[3], "instances_rejected": rejected_instances,
+           "meshlets_rejected": rejected_meshlets, "phase2_clusters_added": added, "phase2_clusters_added_per_frame": added_frames,
+           "raster_entries_first_phase": drawn,
            "raster": {"kernel_ms_per_frame": raster_kernel_ms, "rasters_per_frame": 2 * len(rasters),
                       "roofline": {"bound": "hbm", "kernel": "k_raster + k_raster_chunks", "achieved": achieved, "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
@@ -420,6 +440,8 @@ def two_phase_bench(scene, vd, steps):
         f"its clusters {ms[2]:.2f} ({drawn} entries), second phase {ms[3]:.2f}; raster kernels {raster_kernel_ms:.2f} ms per frame, "
         f"{raster_bytes / 1e9:.2f} GB algorithmic ({'-' if achieved is None else f'{achieved:.0f}'} GB/s); "
         f"{rejected_instances} instances and {rejected_meshlets} meshlets re-tested, {added} clusters added")
+    initial = np.zeros(8, dtype=np.uint32)
+    initial[0] = initial[1] = 10000
     for v in views:
         scene.set_state(v.slot, initial)
     return out
@@ -622,26 +644,41 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic, traffic_source = measured_traffic(dominant)
 
-    # ---------------- CPU baseline: the oracle, single thread, on a bounded sample of the views ----------------
+    # ---------------- CPU baseline: the reference's CPU cut restated in Node, and the C oracle beside it ----------------
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
 
         pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, MAX_TRIANGLES, MAX_SHADOW_TRIANGLES)
+        scales = [steady_states[v.slot][0] / 10000.0 for v in views]
+        threads = max(1, min(16, os.cpu_count() or 1))  # the box's CPU share
+        t0 = time.time()
+        node = oracle.node_cpu_baseline(pool, views, scales, seconds=args.cpu_seconds, threads=threads)
+        one, many = node["single_thread"], node["threads"]
+        log(f"CPU baseline (Node {node['node']}, {node['cpu_model']}, {node['logical_cpus']} logical CPUs): one thread "
+            f"{one['tests_per_s'] / 1e6:.2f}M cluster tests/s over {one['views']} views, {many['n']} threads "
+            f"{many['tests_per_s'] / 1e6:.1f}M/s ({time.time() - t0:.1f} s)")
+        # The C oracle (binary32, the GPU's expression order, occlusion included), one thread, about 6 s of views.
         hzbs = {v.slot: oracle.hzb_build(d, 1024) for v, d in vd if d is not None}
-        done_tests = 0
-        used = 0
+        done_tests = used = 0
         t0 = time.perf_counter()
         for v in views:
             r = oracle.cut(pool, v, state=steady_states[v.slot], hzb=hzbs.get(v.slot))
             done_tests += int(r.stats.clusters_tested[0] + r.stats.clusters_tested[1])
             used += 1
-            if time.perf_counter() - t0 > args.cpu_seconds:
+            if time.perf_counter() - t0 > args.cpu_seconds / 2:
                 break
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": done_tests / cpu_s, "unit": "views*clusters tested/s", "cores": 1, "kind": "port",
-               "sample": f"oracle cut of the first {used} of the {views_n} views (all culling, occlusion, retries), "
-                         f"{done_tests} cluster tests in {cpu_s:.1f} s"}
+        c_s = time.perf_counter() - t0
+        cpu = {"value": many["tests_per_s"], "unit": "views*clusters tested/s", "cores": many["n"], "kind": "port",
+               "cpu_model": node["cpu_model"], "logical_cpus": node["logical_cpus"], "runtime": f"node {node['node']}",
+               "sample": (f"oracle/cpu_cut.js: the reference's CPU cut (cut.ts, fp64, Math.hypot, view*model) with the GPU pass's cell "
+                          f"and instance culls, camera cut without occlusion, at each view's steady-state LOD scale; the first "
+                          f"{many['views']} of the {views_n} views, {many['n']} worker threads sharded by cells, median of 3 after a "
+                          f"warm-up: {many['tests']} cluster tests in {many['seconds']:.2f} s"),
+               "single_thread": {"value": one["tests_per_s"], "views": one["views"], "cluster_tests": one["tests"],
+                                 "seconds": one["seconds"]},
+               "oracle_c": {"value": done_tests / c_s, "cores": 1, "views": used, "cluster_tests": done_tests, "seconds": c_s,
+                            "what": "oracle/vgoracle.c: binary32 in the GPU kernels' order, all culling, occlusion, retries"}}
         del pool
 
     if args.check:
@@ -700,6 +737,9 @@ def main():
                                          batched_schedule))},
         "work_per_step": {"cluster_tests": tested, "clusters_drawn": selected, "work_items": W, "occlusion_tests": O,
                           "unique_instances": inst_unique, "unique_meshlets": meshlet_unique, "cells": cells},
+        # Each view's camera LOD scale in the capacity controller's steady state (its effective threshold is
+        # error_threshold_px times this), in slot order.
+        "steady_state_lod_scale": [round(int(steady_states[v.slot][0]) / 10000.0, 4) for v in views],
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
@@ -712,6 +752,23 @@ def main():
                                                                                args.serve_seconds),
     }
     print(json.dumps(out))
+    # The metric in one line at the end of the log (the JSON line above is long).
+    srv = out["serving"]
+    serve_txt = ""
+    if srv:
+        rows = srv["rates"]
+        serve_txt = "; served " + ", ".join(
+            f"{r['offered_req_per_s']} req/s p50/p99 {r['batched']['latency_ms_p50']:.2f}/{r['batched']['latency_ms_p99']:.2f} ms"
+            f" (plain {r['plain']['latency_ms_p50']:.2f}/{r['plain']['latency_ms_p99']:.2f})" for r in rows[:4])
+    plain = out["plain"]
+    log(f"summary: batched {out['ms_per_step']:.3f} ms per {views_n} views ({value / 1e9:.1f}G views*clusters/s), latency p50/p99 "
+        f"{out['batched']['latency_ms_p50']:.3f}/{out['batched']['latency_ms_p99']:.3f} ms"
+        + ("" if plain is None else f"; plain {plain['ms_per_step']:.3f} ms ({plain['views_clusters_per_s'] / 1e9:.1f}G/s), "
+                                    f"one-view latency p50/p99 {plain['latency_ms_p50']:.3f}/{plain['latency_ms_p99']:.3f} ms, "
+                                    f"batched speedup {plain['batched_speedup']:.2f}x")
+        + f"; {dominant} {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of HBM peak"
+        + ("" if cpu is None else f"; CPU baseline {cpu['value'] / 1e6:.1f}M/s on {cpu['cores']} threads")
+        + serve_txt)
 
 
 if __name__ == "__main__":

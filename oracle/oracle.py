@@ -415,3 +415,32 @@ def prefix(slots: int, counts, capacity: int, cut: int = 0, selected: int = 0, m
     if rc != 0:
         raise RuntimeError("or_test_prefix failed")
     return counters, args
+
+
+def node_cpu_baseline(pool: Pool, views, scales=None, seconds: float = 12.0, threads: Optional[int] = None, timeout: float = 900.0):
+    """Times the Node restatement of the reference's CPU cut (oracle/cpu_cut.js) on this pool and these views
+    (vg_view records; `scales`: each view's camera LOD scale, 1 by default): its JSON result (CPU model, one
+    thread over the views in order for about `seconds`, N worker threads over every view)."""
+    import json
+    import shutil
+    import tempfile
+
+    node = shutil.which("node")
+    if node is None:
+        raise RuntimeError("node is not installed")
+    with tempfile.TemporaryDirectory(prefix="vgcut_cpu_") as d:
+        for name, arr, dtype in (("mesh_table.f32", pool.mesh_table, np.float32), ("instances.f32", pool.instances, np.float32),
+                                 ("cells.f32", pool.cells, np.float32), ("bounds.f32", pool.bounds, np.float32)):
+            np.ascontiguousarray(arr, dtype=dtype).tofile(os.path.join(d, name))
+        with open(os.path.join(d, "views.bin"), "wb") as f:
+            for v in views:
+                f.write(ctypes.string_at(ctypes.addressof(v), ctypes.sizeof(v)))
+        s = np.ones(len(views), dtype=np.float32) if scales is None else np.asarray(scales, dtype=np.float32)
+        s.tofile(os.path.join(d, "scales.f32"))
+        cmd = [node, os.path.join(HERE, "cpu_cut.js"), d, "--seconds", str(seconds)]
+        if threads:
+            cmd += ["--threads", str(threads)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu_cut.js failed: {r.stderr[-2000:]}")
+        return json.loads(r.stdout.strip().splitlines()[-1])

@@ -1,10 +1,11 @@
 // Test driver: exercises the vgcut N-API addon from Node the way the TypeScript host would. Reads a JSON
-// description (context config, meshes with instances, optional packed vg_view records to cut on the GPU)
+// description (context config, meshes with instances, optional cameras whose views the JS host packs and cuts on the GPU)
 // and writes what the addon returned as JSON (raw buffers as hex, so the comparison is bit-exact).
 'use strict';
 const fs = require('fs');
 const [, , addonPath, inPath, outPath] = process.argv;
 const vg = require(addonPath);
+const host = require(require('path').join(__dirname, '..', 'three-virtual-geometry_amd', 'js', 'virtual_geometry.js'));
 const spec = JSON.parse(fs.readFileSync(inPath, 'utf8'));
 const hex = (ab) => Buffer.from(ab).toString('hex');
 const out = { slots: [], orders: [] };
@@ -59,18 +60,22 @@ const readSlots = () => spec.viewSlots.map((slot) => {
   vg.read(scene, slot, counters, args, list);
   return { counters: hex(counters.buffer), args: hex(args.buffer), list: hex(list.buffer) };
 });
-if (!spec.views) {
+// Cameras as {kind: 'perspective' | 'orthographic', fov, aspect | left, right, top, bottom, near, far, eye, target,
+// up?, height, and the view's levers}: the JS host makes their vg_view records.
+const cameraOf = (c) => (c.kind === 'orthographic' ? host.orthographicCamera(c.left, c.right, c.top, c.bottom, c.near, c.far)
+  : host.perspectiveCamera(c.fov, c.aspect, c.near, c.far)).lookAt(c.eye, c.target, c.up);
+if (!spec.cameras) {
   finish();
 } else {
-  const buf = Buffer.from(spec.views, 'hex');
-  const views = buf.buffer.slice(buf.byteOffset, buf.byteOffset + buf.length);
+  const views = host.packViews(spec.cameras.map((c) => host.viewRecord(cameraOf(c), c.height, c)));
+  out.views = hex(views);
   setHzbs(spec.hzb1);
-  vg.cut(scene, views, spec.viewCount).then(() => {
+  vg.cut(scene, views, spec.cameras.length).then(() => {
     out.results = readSlots();
     if (!spec.hzb2) return null;
     // two-phase occlusion: the new depth, then the second phase of the same views
     setHzbs(spec.hzb2);
-    return vg.cutPhase2(scene, views, spec.viewCount).then(() => {
+    return vg.cutPhase2(scene, views, spec.cameras.length).then(() => {
       out.phase2 = spec.viewSlots.map((slot) => {
         const counters = new Uint32Array(vg.counterWords(ctx));
         const args = new Uint32Array(vg.drawArgsWords(ctx));

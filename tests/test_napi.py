@@ -116,19 +116,49 @@ def test_node_host_builds_the_dag(tmp_path):
         assert bytes.fromhex(got["pool"][buf]) == scene.pool(buf).tobytes(), f"pool buffer {buf}"
 
 
+def camera_json(eye, target, height, slot, fov=60.0, aspect=16 / 9, near=0.1, far=2000.0, **levers):
+    return dict(kind="perspective", fov=fov, aspect=aspect, near=near, far=far, eye=list(eye), target=list(target), height=height,
+                slot=slot, **levers)
+
+
+def python_view(c):
+    """The vg_view the Python host makes for a camera_json description."""
+    from vgcut import camera as cam
+
+    if c["kind"] == "orthographic":
+        camera = cam.orthographic_camera(c["left"], c["right"], c["top"], c["bottom"], c["near"], c["far"])
+    else:
+        camera = cam.perspective_camera(c["fov"], c["aspect"], c["near"], c["far"])
+    camera.look_at(c["eye"], c["target"], c.get("up", (0.0, 1.0, 0.0)))
+    kw = {k: c[k] for k in ("errorThreshold", "occlusion", "minPixelRadius", "lodBlend", "maxErrorThreshold", "triangleBudget",
+                            "shadowErrorScale") if k in c}
+    names = dict(errorThreshold="error_threshold", minPixelRadius="min_pixel_radius", lodBlend="lod_blend",
+                 maxErrorThreshold="max_error_threshold", triangleBudget="triangle_budget", shadowErrorScale="shadow_error_scale")
+    kw = {names.get(k, k): v for k, v in kw.items()}
+    if "thresholdRange" in c:
+        kw["threshold_range"] = tuple(c["thresholdRange"])
+    if "shadowSweep" in c:
+        kw["shadow_sweep"] = tuple(c["shadowSweep"])
+    if c.get("frustumCulling") is False:
+        kw["frustum_culling"] = False
+    return make_view(camera, c["height"], slot=c["slot"], **kw)
+
+
 @pytest.mark.gpu
 @needs_addon
 def test_node_host_cut_equals_python_host(tmp_path, gpu_available):
     spec = small_spec()
     max_meshlets = 1 << 14
-    views = [make_view(ground_camera((-20.0, 3.0, -25.0), (10.0, 0.0, 20.0)), 1080, slot=0),
-             make_view(ground_camera((15.0, 12.0, 5.0), (-20.0, 0.0, -10.0)), 720, slot=1, error_threshold=0.5)]
-    packed = b"".join(ctypes.string_at(ctypes.addressof(v), ctypes.sizeof(v)) for v in views)
+    cams = [camera_json((-20.0, 3.0, -25.0), (10.0, 0.0, 20.0), 1080, 0),
+            camera_json((15.0, 12.0, 5.0), (-20.0, 0.0, -10.0), 720, 1, errorThreshold=0.5)]
+    views = [python_view(c) for c in cams]
     assert ctypes.sizeof(native.View) == 292
     config = {"maxViews": 2, "meshSlots": 8, "maxMeshlets": max_meshlets, "maxTriangles": 2_000_000,
               "maxShadowTriangles": 1_000_000, "device": 0}
     got = run_node(tmp_path, {"config": config, "meshes": [mesh_json(d, m) for d, m in zip(spec.meshes, spec.matrices) if len(m)],
-                              "views": packed.hex(), "viewCount": len(views), "viewSlots": [0, 1], "maxEntries": max_meshlets})
+                              "cameras": cams, "viewSlots": [0, 1], "maxEntries": max_meshlets})
+    packed = b"".join(ctypes.string_at(ctypes.addressof(v), ctypes.sizeof(v)) for v in views)
+    assert bytes.fromhex(got["views"]) == packed, "the JS host packs other view records than the Python host"
     ctx = native.Context(max_triangles=2_000_000, max_shadow_triangles=1_000_000, max_meshlets=max_meshlets, max_views=2,
                          mesh_slots=8, device=0)
     scene = native.Scene(ctx)
@@ -155,8 +185,10 @@ def test_node_host_two_phase_equals_python_host(tmp_path, gpu_available):
 
     spec = small_ruins(seed=72, instances=2000, meshes=3)
     max_meshlets = 1 << 15
-    cams = [ground_camera((0.0, 2.0, -70.0), (0.0, 0.0, 60.0)), ground_camera((50.0, 2.0, 10.0), (-60.0, 2.0, 0.0))]
-    views = [make_view(c, 540, slot=i, occlusion=2) for i, c in enumerate(cams)]
+    described = [camera_json((0.0, 2.0, -70.0), (0.0, 0.0, 60.0), 540, 0, occlusion=2),
+                 camera_json((50.0, 2.0, 10.0), (-60.0, 2.0, 0.0), 540, 1, occlusion=2)]
+    cams = [ground_camera(c["eye"], c["target"]) for c in described]
+    views = [python_view(c) for c in described]
     depths = {}
     for phase, seed in ((1, 80), (2, 90)):
         for i, c in enumerate(cams):
@@ -164,12 +196,11 @@ def test_node_host_two_phase_equals_python_host(tmp_path, gpu_available):
             depths[(phase, i)] = scenegen.heightfield_depth(vp, 480, 270, (0.0, 0.0), 120.0, 128, 6.0, 1 / 25.0, seed + i)
     hzb = lambda phase: [{"slot": i, "width": 480, "height": 270, "depth": depths[(phase, i)].astype(np.float32).tobytes().hex()}
                          for i in range(len(cams))]
-    packed = b"".join(ctypes.string_at(ctypes.addressof(v), ctypes.sizeof(v)) for v in views)
     config = {"maxViews": 2, "meshSlots": 8, "maxMeshlets": max_meshlets, "maxTriangles": 20_000_000,
               "maxShadowTriangles": 10_000_000, "device": 0}
     got = run_node(tmp_path, {"config": config,
                               "meshes": [mesh_json(d, m, occlusion=True) for d, m in zip(spec.meshes, spec.matrices) if len(m)],
-                              "views": packed.hex(), "viewCount": len(views), "viewSlots": [0, 1], "maxEntries": max_meshlets,
+                              "cameras": described, "viewSlots": [0, 1], "maxEntries": max_meshlets,
                               "hzb1": hzb(1), "hzb2": hzb(2)})
     assert "error" not in got, got.get("error")
     ctx = native.Context(max_triangles=20_000_000, max_shadow_triangles=10_000_000, max_meshlets=max_meshlets, max_views=2,
