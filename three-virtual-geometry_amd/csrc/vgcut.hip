@@ -14,6 +14,7 @@
 #include "../../include/vgcut.h"
 #include "vgcut_kernels.hip"
 #include "vgraster_kernels.hip"
+#include "vgcut_blocks.h"
 #include "vgcut_host.h"
 
 using namespace vg;
@@ -270,6 +271,8 @@ int upload_scene(vg_scene* s) {
         (rc = upload(s, s->d_err, s->m_err)) || (rc = upload(s, s->d_tris, s->m_tris)) || (rc = upload(s, s->d_run, s->m_run)) ||
         (rc = upload(s, s->d_rspan, s->r_span)) || (rc = upload(s, s->d_rlod, s->r_lod)) ||
         (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)) ||
+        (rc = upload(s, s->d_rblock, s->r_block)) || (rc = upload(s, s->d_bspan, s->b_span)) ||
+        (rc = upload(s, s->d_blod, s->b_lod)) || (rc = upload(s, s->d_bpar, s->b_par)) || (rc = upload(s, s->d_berr, s->b_err)) ||
         (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)) ||
         (rc = upload(s, s->d_vlist, s->m_vlist)))
       return rc;
@@ -380,6 +383,11 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.m_tris = static_cast<const uint32_t*>(s->d_tris.ptr);
   p.m_run = static_cast<const uint32_t*>(s->d_run.ptr);
   p.r_span = static_cast<const uint2*>(s->d_rspan.ptr);
+  p.r_block = static_cast<const uint32_t*>(s->d_rblock.ptr);
+  p.b_span = static_cast<const uint2*>(s->d_bspan.ptr);
+  p.b_lod = static_cast<const float4*>(s->d_blod.ptr);
+  p.b_par = static_cast<const float4*>(s->d_bpar.ptr);
+  p.b_err = static_cast<const float4*>(s->d_berr.ptr);
   p.r_lod = static_cast<const float4*>(s->d_rlod.ptr);
   p.r_parent = static_cast<const float4*>(s->d_rparent.ptr);
   p.r_err = static_cast<const float2*>(s->d_rerr.ptr);
@@ -894,7 +902,8 @@ void vg_scene_destroy(vg_scene* s) {
   if (!s) return;
   if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
   lane_destroy(&s->lane);
-  DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
+  DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_rblock, &s->d_bspan, &s->d_blod, &s->d_bpar,
+                    &s->d_berr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
                     &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
@@ -1021,7 +1030,19 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
       s->r_parent.push_back(s->m_parent[a]);
       s->r_err.push_back(s->m_err[a]);
       for (uint32_t k = 0; k < len; k++) s->m_run.push_back(run);
+      // A run opens a new block at a level boundary or when the current block is full.
+      if (i == 0 || level_start[i] || s->b_span.back().y == kBlockRuns) s->b_span.push_back(make_uint2(run, 0));
+      s->b_span.back().y++;
+      s->r_block.push_back(static_cast<uint32_t>(s->b_span.size() - 1));
       i += len;
+    }
+    // The new blocks' bounds.
+    for (size_t b = s->b_lod.size(); b < s->b_span.size(); b++) {
+      float4 lod, par, err;
+      block_bounds(s->r_lod.data(), s->r_parent.data(), s->r_err.data(), s->b_span[b].x, s->b_span[b].y, &lod, &par, &err);
+      s->b_lod.push_back(lod);
+      s->b_par.push_back(par);
+      s->b_err.push_back(err);
     }
   }
   // The pool's triangle array grows by the mesh's triangles (GeometryPool.ts:326, 430); firstIndex is

@@ -50,6 +50,15 @@ constexpr uint32_t kTagOcclusion = 1u << 17;  // the mesh opted into occlusion (
 constexpr uint32_t kTagFast = 1u << 18;       // fast path: tasks are meshlets drawn without cluster tests
 constexpr int kTagViewShift = 19;
 constexpr uint32_t kMaxLaunchViews = 1u << (32 - kTagViewShift);
+// LOD blocks: up to kBlockRuns consecutive LOD runs of one level. The meshlet pass first tests a block as
+// a whole against bounds of its runs (a sphere around their LOD-sphere centres and one around their parent
+// centres, the smallest own error and LOD radius, the largest parent error and parent radius) and skips it
+// when every run's own error certainly projects above the band or every parent's certainly below it; the
+// runs of the other blocks are tested one by one, exactly. Items of the meshlet pass are block ranges.
+constexpr uint32_t kBlockRuns = 16;
+// Per-wave LDS queue of blocks that passed, awaiting their runs' tests: at most 63 entries with untested
+// runs plus the last consumed one, plus one chunk's 64 survivors.
+constexpr uint32_t kBlockQueue = 128;
 // Per-wave LDS ring of meshlets in the cut awaiting their visibility test: at most 63 wait before a
 // wave appends up to 64 * kMaxRun, so 64 * (kMaxRun + 1) entries suffice.
 constexpr uint32_t kQueue = 64 * (kMaxRun + 1);
@@ -89,6 +98,9 @@ struct ViewDev {
   float budget_scale, budget_low, budget_high;  // camera scale and band of the budget histogram
   float budget_inv;  // (bins - 1) / ln(T[B-1] / T[0]): estimates a value's bin in the log-spaced table
   float sweep_reach[6];  // per frustum plane: fmaxf(plane normal . sweep, 0), the swept frustum's extra reach
+  // Bounds on the view matrix for the block test: its 3 x 3 part's spectral norm (Gershgorin on V^T V) and
+  // largest row sum of magnitudes, and its largest translation magnitude.
+  float norm_bound, row_bound, move_bound;
 };
 
 struct Params {
@@ -104,6 +116,11 @@ struct Params {
   const float4* __restrict__ r_lod;       // per run: LOD sphere
   const float4* __restrict__ r_parent;    // parent LOD sphere
   const float2* __restrict__ r_err;       // (error, parentError)
+  const uint32_t* __restrict__ r_block;   // block of each run
+  const uint2* __restrict__ b_span;       // per block: (first run, run count)
+  const float4* __restrict__ b_lod;       // per block: centre and radius of a sphere holding its runs' LOD-sphere centres
+  const float4* __restrict__ b_par;       // the same for their parent spheres' centres
+  const float4* __restrict__ b_err;       // (smallest own error, smallest LOD radius, largest parent error or +inf, largest parent radius)
   const float4* __restrict__ i_col0;      // instance matrix columns, SoA
   const float4* __restrict__ i_col1;
   const float4* __restrict__ i_col2;
@@ -164,7 +181,7 @@ __host__ __device__ inline uint32_t state_base(uint32_t slots) { return kPerMesh
 
 // ------------------------------------------------------------------ arithmetic
 
-__device__ __forceinline__ float4 mat_mul(const float4& c0, const float4& c1, const float4& c2, const float4& c3, float x,
+__host__ __device__ inline float4 mat_mul(const float4& c0, const float4& c1, const float4& c2, const float4& c3, float x,
                                           float y, float z, float w) {
   float4 r;
   r.x = ((c0.x * x + c1.x * y) + c2.x * z) + c3.x * w;
@@ -183,7 +200,7 @@ __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t index) {
 }
 
 template <typename P>
-__device__ __forceinline__ float4 mat_mul16(P m, float x, float y, float z, float w) {
+__host__ __device__ inline float4 mat_mul16(P m, float x, float y, float z, float w) {
   float4 r;
   r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
   r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
@@ -194,10 +211,10 @@ __device__ __forceinline__ float4 mat_mul16(P m, float x, float y, float z, floa
 
 // sqrt and / are correctly rounded: HIP's default -fhip-fp32-correctly-rounded-divide-sqrt lowers
 // llvm.sqrt.f32 and fdiv to IEEE-exact sequences on gfx950 (checked against the host by a GPU test).
-__device__ __forceinline__ float len3(float x, float y, float z) { return __builtin_sqrtf((x * x + y * y) + z * z); }
+__host__ __device__ inline float len3(float x, float y, float z) { return __builtin_sqrtf((x * x + y * y) + z * z); }
 template <typename P>
 __device__ __forceinline__ float dot3(P n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__host__ __device__ inline float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
 // fdlibm e_logf.c (Sun Microsystems, 1993): x = 2^k (1 + f), log(1 + f) = f - s (f - R(z)),
 // s = f / (2 + f). Basic operations only, so the host and device give the same bits.
@@ -229,6 +246,98 @@ __host__ __device__ inline float vg_logf(float x) {
   float R = t2 + t1;
   float hfsq = 0.5f * f * f;
   return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
+// projectedError, GeometryPool.ts:678-683. kAffine: the instance matrix's last row is (0, 0, 0, 1) and the
+// sphere finite, so the transformed w is exactly ((0 x + 0 y) + 0 z) + 1 = 1 and is not computed.
+template <bool kAffine = false, typename V>
+__host__ __device__ inline float projected_error(const V& v, const float4& c0, const float4& c1, const float4& c2,
+                                                 const float4& c3, float scale, const float4& s, float error) {
+  float4 q = mat_mul(c0, c1, c2, c3, s.x, s.y, s.z, 1.0f);
+  float4 vc = mat_mul16(v.view, q.x, q.y, q.z, kAffine ? 1.0f : q.w);
+  float distance = fmaxf(len3(vc.x, vc.y, vc.z) - s.w * scale, v.near_plane);
+  float pixels = ((error * scale) / distance) * v.proj_scale;
+  return error >= VG_INF_HALF ? VG_INF : pixels;
+}
+
+// Bounds of the block test (see kBlockRuns): whether every run of a block is certainly out of the band.
+// For a run i of the block with LOD centre p_i, |p_i - c| <= rho around the block centre c, so in view space
+// |V M p_i - V M c| <= |V|_2 |M|_2 rho (both norms bounded above by Gershgorin), and the exact binary32
+// distance d_i = max(|V (M p_i)| - r_i s, near) of the run's own test is at most
+//   dmax = max(|V M c| + |V|_2 |M|_2 rho - r_min s + slack, near),
+// where the slack, 1e-5 times the magnitude of every term the two evaluations add (the view-space
+// coordinates' sizes W, the distance, the reach, the radius), covers the rounding of both the exact
+// path's products (at most 2 sqrt(3) gamma_4 W, about 8.3e-7 W, with no FMA) and this bound's own. Then
+// own_i = e_i s P / d_i >= e_min s P / dmax, and e_min s P > high dmax (1 + 1e-5) leaves every own_i above
+// high. Likewise with the parent centres, a lower bound dpmin on every parent distance, and
+// pe_max s P (1 + 1e-5) <= low dpmin leaves every parent_i at most low (pe_max is +inf when a parent error
+// is >= 5e29, which projects to 1e30). Only for affine scenes (the transformed w is exactly 1).
+struct BlockBounds {
+  float scale, norm_m, row_m, move_m;  // item scale, |M|_2 bound, largest row sum of |M|, largest |translation|
+};
+
+__host__ __device__ inline BlockBounds instance_bounds(const float4& c0, const float4& c1, const float4& c2, const float4& c3, float scale) {
+  const float g00 = c0.x * c0.x + c0.y * c0.y + c0.z * c0.z, g11 = c1.x * c1.x + c1.y * c1.y + c1.z * c1.z;
+  const float g22 = c2.x * c2.x + c2.y * c2.y + c2.z * c2.z;
+  const float g01 = fabsf(c0.x * c1.x + c0.y * c1.y + c0.z * c1.z), g02 = fabsf(c0.x * c2.x + c0.y * c2.y + c0.z * c2.z);
+  const float g12 = fabsf(c1.x * c2.x + c1.y * c2.y + c1.z * c2.z);
+  BlockBounds b;
+  b.scale = scale;
+  b.norm_m = __builtin_sqrtf(fmaxf(g00 + g01 + g02, fmaxf(g01 + g11 + g12, g02 + g12 + g22)));
+  b.row_m = fmaxf(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fmaxf(fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y), fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z)));
+  b.move_m = fmaxf(fabsf(c3.x), fmaxf(fabsf(c3.y), fabsf(c3.z)));
+  return b;
+}
+
+// View-space distance from the camera to M c (FMA arithmetic: a bound, covered by the slack) and the
+// magnitude W of the view-space coordinates of any point within rho of c.
+template <typename V>
+__host__ __device__ inline float2 block_distance(const V& v, const float4& c0, const float4& c1, const float4& c2, const float4& c3,
+                                                 const BlockBounds& ib, const float4& sphere) {
+  const float qx = fmaf(c0.x, sphere.x, fmaf(c1.x, sphere.y, fmaf(c2.x, sphere.z, c3.x)));
+  const float qy = fmaf(c0.y, sphere.x, fmaf(c1.y, sphere.y, fmaf(c2.y, sphere.z, c3.y)));
+  const float qz = fmaf(c0.z, sphere.x, fmaf(c1.z, sphere.y, fmaf(c2.z, sphere.z, c3.z)));
+  const auto m = v.view;  // the record may be in the constant address space (scalar loads)
+  const float x = fmaf(m[0], qx, fmaf(m[4], qy, fmaf(m[8], qz, m[12])));
+  const float y = fmaf(m[1], qx, fmaf(m[5], qy, fmaf(m[9], qz, m[13])));
+  const float z = fmaf(m[2], qx, fmaf(m[6], qy, fmaf(m[10], qz, m[14])));
+  const float extent = fmaxf(fabsf(sphere.x), fmaxf(fabsf(sphere.y), fabsf(sphere.z))) + sphere.w;
+  const float w = fmaf(v.row_bound, fmaf(ib.row_m, extent, ib.move_m), v.move_bound);
+  return make_float2(__builtin_sqrtf(fmaf(x, x, fmaf(y, y, z * z))), w);
+}
+
+template <typename V>
+__host__ __device__ inline bool block_out_of_band(const V& v, const float4& c0, const float4& c1, const float4& c2, const float4& c3,
+                                                  const BlockBounds& ib, const float4& lod, const float4& par, const float4& err,
+                                                  float low, float high) {
+  const float kSlack = 1e-5f, kMargin = 1.00001f;
+  const float s = ib.scale, stretch = v.norm_bound * ib.norm_m;
+  const float2 own = block_distance(v, c0, c1, c2, c3, ib, lod);
+  const float reach = stretch * lod.w, rmin = err.y * s;
+  const float dmax = fmaxf(own.x + reach - rmin + kSlack * (own.y + own.x + reach + rmin), v.near_plane);
+  if (err.x * s * v.proj_scale > high * dmax * kMargin) return true;
+  const float2 pd = block_distance(v, c0, c1, c2, c3, ib, par);
+  const float preach = stretch * par.w, rpmax = err.w * s;
+  const float dpmin = fmaxf(pd.x - preach - rpmax - kSlack * (pd.y + pd.x + preach + rpmax), v.near_plane);
+  return err.z * s * v.proj_scale * kMargin <= low * dpmin;
+}
+
+// Upper bounds on the view matrix's stretch for the block test: sqrt of the largest Gershgorin row sum of
+// V^T V (>= the spectral norm of V's 3 x 3 part), its largest row sum of magnitudes, and the largest
+// translation magnitude. The products are rounded up by the block test's slack.
+__host__ __device__ inline void view_bounds(ViewDev& v) {
+  const float* m = v.view;
+  float g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g[i][j] = fabsf(m[i * 4] * m[j * 4] + m[i * 4 + 1] * m[j * 4 + 1] + m[i * 4 + 2] * m[j * 4 + 2]);
+  float gersh = 0.0f, rows = 0.0f;
+  for (int i = 0; i < 3; i++) {
+    gersh = fmaxf(gersh, g[i][0] + g[i][1] + g[i][2]);
+    rows = fmaxf(rows, fabsf(m[i]) + fabsf(m[4 + i]) + fabsf(m[8 + i]));
+  }
+  v.norm_bound = sqrtf(gersh);
+  v.row_bound = rows;
+  v.move_bound = fmaxf(fabsf(m[12]), fmaxf(fabsf(m[13]), fabsf(m[14])));
 }
 
 // ------------------------------------------------------------------ scalar (wave-uniform) reads

@@ -110,17 +110,6 @@ __device__ __forceinline__ float frustum_distance(const V& v, float cx, float cy
   return best;
 }
 
-// projectedError, GeometryPool.ts:678-683. kAffine: the instance matrix's last row is (0, 0, 0, 1) and the
-// sphere finite, so the transformed w is exactly ((0 x + 0 y) + 0 z) + 1 = 1 and is not computed.
-template <bool kAffine = false, typename V>
-__device__ __forceinline__ float projected_error(const V& v, const float4& c0, const float4& c1, const float4& c2,
-                                                 const float4& c3, float scale, const float4& s, float error) {
-  float4 q = mat_mul(c0, c1, c2, c3, s.x, s.y, s.z, 1.0f);
-  float4 vc = mat_mul16(v.view, q.x, q.y, q.z, kAffine ? 1.0f : q.w);
-  float distance = fmaxf(len3(vc.x, vc.y, vc.z) - s.w * scale, v.near_plane);
-  float pixels = ((error * scale) / distance) * v.proj_scale;
-  return error >= VG_INF_HALF ? VG_INF : pixels;
-}
 
 __device__ __forceinline__ uint32_t ceil_log2(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -216,6 +205,7 @@ __device__ __forceinline__ bool occluded_sphere(const Params& p, const V& v, boo
 __global__ __launch_bounds__(256) void k_begin(Params p) {
   ViewDev& v = p.views[blockIdx.x];
   if (threadIdx.x < 6) v.sweep_reach[threadIdx.x] = fmaxf(dot3(&v.planes[threadIdx.x * 4], v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f);
+  if (threadIdx.x == 6) view_bounds(v);
   uint32_t* c = view_counters(p, v.slot);
   for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
   if (p.phase == 2) {
@@ -608,12 +598,13 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
           fast_bits |= 1u << (r * 2 + cut);
         }
       }
-      // Work items cover the LOD runs of [first, last) (level ranges hold whole runs).
+      // Work items cover the LOD blocks of [first, last) (level ranges hold whole runs, runs and blocks
+      // never cross a level's bounds).
 #pragma unroll
       for (uint32_t cut = 0; cut < 2; cut++) {
         if (work[cut]) {
-          item_lo[r][cut] = p.m_run[lr[cut].first];
-          item_hi[r][cut] = p.m_run[lr[cut].last - 1] + 1;
+          item_lo[r][cut] = p.r_block[p.m_run[lr[cut].first]];
+          item_hi[r][cut] = p.r_block[p.m_run[lr[cut].last - 1]] + 1;
         }
       }
     }
@@ -765,13 +756,16 @@ __device__ __forceinline__ void for_each_task(const Params& p, Body&& body) {
   }
 }
 
-// The meshlet pass in two phases per wave. Phase 1 runs once per LOD run (consecutive meshlets whose
-// LOD test has identical inputs): own <= high, parent > low and the blend range. The meshlets of runs in
-// the cut go to a per-wave LDS ring; phase 2 runs on full waves of them: frustum and occlusion tests
-// (drawn = visible and in the cut, so the order changes no decision), then compaction into the draw
-// list. The per-cluster-test cost is one LOD evaluation per run; visibility is paid per meshlet in the cut.
+// The meshlet pass in three stages per wave. (1) Blocks: a lane per block of the work items' ranges; the
+// block test (above) drops blocks none of whose runs can be in the band; blocks that remain go to a
+// per-wave LDS queue. (2) Runs: whenever 64 runs are queued, a lane per run runs the exact LOD test (own
+// <= high, parent > low, the blend range), once per LOD run (consecutive meshlets whose LOD test has
+// identical inputs); the meshlets of runs in the cut go to a per-wave LDS ring. (3) Visibility: full waves
+// of those meshlets get the frustum and occlusion tests and are compacted (drawn = visible and in the cut,
+// so the order changes no decision). Fast-path items (guaranteed meshlets) go straight to the ring.
 // Compiled for several occupancy targets (waves per SIMD); the launch schedule picks one. kAffine: every
-// instance matrix of the scene is affine and every run's LOD spheres finite (projected_error).
+// instance matrix of the scene is affine and every run's LOD spheres finite (projected_error); only then
+// are blocks dropped (the bounds assume the transformed w is 1).
 template <int kWavesPerSimd, bool kAffine>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_meshlets(Params p) {
   // Per wave: meshlets in the cut awaiting their visibility test, 16 bytes each: (instance | fade_low << 24,
@@ -780,22 +774,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
   // Per wave: the HZB level table of the view whose meshlets are being tested (one float per lane to copy).
   static_assert(kHzbMaxLevels * 4 == 64, "the level table is copied one float per lane");
   __shared__ float4 hzb_rows[4][kHzbMaxLevels];
+  // Per wave: blocks that passed, (instance | run count << 24, first run, tag, instance scale bits), and the
+  // wave-relative position of each one's first run in the stream of queued runs.
+  static_assert((kBlockQueue & (kBlockQueue - 1)) == 0 && kBlockRuns < 256, "block queue indices wrap by masking");
+  __shared__ uint4 bq[4][kBlockQueue];
+  __shared__ uint32_t bq_start[4][kBlockQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   uint32_t head = 0, tail = 0;  // wave-uniform ring positions
+  // Block queue: entries [bq_head, bq_tail) (bq_head may be a consumed one), runs [rq_pos, rq_end) untested.
+  // It starts with an empty consumed entry at position 0, so later entries always follow the head.
+  uint32_t bq_head = 0, bq_tail = 1, rq_pos = 0, rq_end = 0;
+  if (lane == 0) {
+    bq[wib][0] = make_uint4(0, 0, 0, 0);
+    bq_start[wib][0] = 0;
+  }
   // Occlusion tests of the current view slot, added to its metric when the slot changes (and at the end).
   uint32_t occ_slot = 0xffffffffu, occ_count = 0;
-  auto occ_flush = [&]() {
+  auto occ_flush = [&]() __attribute__((always_inline)) {
     if (occ_count && lane == 0) atomicAdd(metric_at(p, occ_slot, M_OCCLUSION), static_cast<unsigned long long>(occ_count));
     occ_count = 0;
   };
 
-  auto wave_sync = [&]() {
+  auto wave_sync = [&]() __attribute__((always_inline)) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  auto drain = [&](uint32_t n) {
+  auto drain = [&](uint32_t n) __attribute__((always_inline)) {
     const bool active = lane < n;
     const uint32_t at_ring = head + lane >= kQueue ? head + lane - kQueue : head + lane;  // head < kQueue
     const uint4 c = active ? ring[wib][at_ring] : make_uint4(0, 0, 0, 0);
@@ -871,39 +877,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(instance, meshlet, first_index, packed_tris);
   };
 
-  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t task) {
-    const uint32_t cut = (w.w & kTagShadow) ? 1u : 0u, vi = w.w >> kTagViewShift;
-    const bool fast = (w.w & kTagFast) != 0;
-    uint32_t fades = 0, first = 0, count = 0;
-    float scale = 0.0f;
-    if (valid && fast) {
-      // Fast-path item: the task is a meshlet, drawn at full fade.
-      fades = kFadeSteps << 8;
-      first = task;
-      count = 1;
+  // Appends `count` (<= kMaxRun) meshlets from `first` per lane to the ring and drains full waves of it.
+  auto ring_append = [&](uint32_t count, uint32_t first, uint32_t fades, uint32_t instance, uint32_t tag, float scale) __attribute__((always_inline)) {
+    const uint32_t incl = wave_scan(count);
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    if (!total) return;
+    const uint32_t at = tail + incl - count;
+    for (uint32_t k = 0; k < count; k++) {
+      const uint32_t r = at + k >= kQueue ? at + k - kQueue : at + k;  // at + k < 2 kQueue: head < kQueue, fewer than 64 + 4 * 64 pending
+      ring[wib][r] = make_uint4(instance | (fades << 24), (first + k) | ((fades >> 8) << 24), tag, __float_as_uint(scale));
     }
+    tail += total;
+    while (tail - head >= 64) {
+      wave_sync();
+      drain(64);
+      wave_sync();
+    }
+    if (head >= kQueue) {  // keep the ring positions small (their difference is what counts)
+      head -= kQueue;
+      tail -= kQueue;
+    }
+  };
+
+  // Tests the next n (<= 64) queued runs, one per lane: the exact LOD test, then the ring.
+  auto run_stage = [&](uint32_t n) __attribute__((always_inline)) {
+    wave_sync();
+    // The entries after the head that start inside [rq_pos, rq_pos + 63]: bit d of `starts` = one starts at
+    // rq_pos + d. Each entry has at least one run, so no more than 64 of them start inside the window.
+    const uint32_t e_next = bq_head + 1 + lane;
+    const bool present = lane < bq_tail - bq_head - 1;
+    const uint32_t dist = present ? bq_start[wib][e_next & (kBlockQueue - 1)] - rq_pos : 0xffffffffu;
+    const uint64_t bit = dist < 64 ? 1ull << dist : 0ull;
+    const uint64_t starts = (static_cast<uint64_t>(wave_or(static_cast<uint32_t>(bit >> 32))) << 32) | wave_or(static_cast<uint32_t>(bit));
+    const bool valid = lane < n;
+    const uint32_t e = bq_head + static_cast<uint32_t>(__popcll(starts & ((2ull << lane) - 1ull)));
+    uint4 q = make_uint4(0, 0, 0, 0);
+    uint32_t first_pos = 0;
+    if (valid) {
+      q = bq[wib][e & (kBlockQueue - 1)];
+      first_pos = bq_start[wib][e & (kBlockQueue - 1)];
+    }
+    const uint32_t instance = q.x & 0xffffffu, tag = q.z, run = q.y + (rq_pos + lane - first_pos);
+    const float scale = __uint_as_float(q.w);
+    const uint32_t cut = (tag & kTagShadow) ? 1u : 0u, vi = tag >> kTagViewShift;
     // The run's records and its instance's columns, loaded before the per-view loop so that their latencies
     // overlap (the span too: loading it only for runs in the cut would add a dependent round trip).
-    const bool run_test = valid && !fast;
-    const uint32_t run = task;
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, lod_sphere = c0, parent_sphere = c0;
     float2 err = make_float2(0.0f, 0.0f);
     uint2 span = make_uint2(0, 0);
-    if (run_test) {
-      c0 = ld32(p.i_col0, w.x);
-      c1 = ld32(p.i_col1, w.x);
-      c2 = ld32(p.i_col2, w.x);
-      c3 = ld32(p.i_col3, w.x);
-      scale = p.work_scale[item];
+    if (valid) {
+      c0 = ld32(p.i_col0, instance);
+      c1 = ld32(p.i_col1, instance);
+      c2 = ld32(p.i_col2, instance);
+      c3 = ld32(p.i_col3, instance);
       err = ld32(p.r_err, run);
       lod_sphere = ld32(p.r_lod, run);
       parent_sphere = ld32(p.r_parent, run);
       span = ld32(p.r_span, run);
     }
+    uint32_t fades = 0, count = 0;
     // The LOD test of a run, one launched view at a time (its constants are wave-uniform there).
-    for (uint64_t pending = __ballot(run_test); pending;) {
+    for (uint64_t pending = __ballot(valid); pending;) {
       const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
-      const bool mine = run_test && vi == vu;
+      const bool mine = valid && vi == vu;
       if (mine) {
         // readfirstlane keeps the index uniform even where the compiler substitutes vi for vu.
         const auto& v = scalar(p.views)[uniform(vu)];
@@ -917,34 +953,73 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
         }
         const bool lod = own <= high && parent > low && fade_high > fade_low;
         fades = fade_low | (fade_high << 8);
-        if (lod) {
-          first = span.x;
-          count = span.y;
-        }
+        if (lod) count = span.y;
       }
       pending &= ~__ballot(mine);
     }
-    // Append the runs' meshlets to the ring (before the append fewer than 64 are pending).
-    const uint32_t incl = wave_scan(count);
-    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-    if (total) {
-      const uint32_t at = tail + incl - count;
-      for (uint32_t k = 0; k < count; k++) {
-        const uint32_t r = at + k >= kQueue ? at + k - kQueue : at + k;  // at + k < 2 kQueue: head < kQueue, fewer than 64 + 4 * 64 pending
-        ring[wib][r] = make_uint4(w.x | (fades << 24), (first + k) | ((fades >> 8) << 24), w.w, __float_as_uint(scale));
+    // Entries whose runs are all tested now: the new head is the last one starting at or before rq_pos + n.
+    const uint64_t upto_n = n >= 64 ? ~0ull : ((2ull << n) - 1ull);
+    const bool at_end = __ballot(dist == 64) != 0 && n >= 64;
+    bq_head += static_cast<uint32_t>(__popcll(starts & upto_n)) + (at_end ? 1u : 0u);
+    rq_pos += n;
+    ring_append(count, span.x, fades, instance, tag, scale);
+  };
+
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t task) __attribute__((always_inline)) {
+    const bool fast = valid && (w.w & kTagFast) != 0;
+    const bool test = valid && !fast;
+    // Fast-path items: the task is a meshlet, drawn at full fade.
+    if (__ballot(fast)) ring_append(fast ? 1u : 0u, task, kFadeSteps << 8, w.x, w.w, fast ? p.work_scale[item] : 0.0f);
+    if (!__ballot(test)) return;
+    // Blocks: the block's span and bounds and the instance's columns, loaded before the per-view loop.
+    uint2 bspan = make_uint2(0, 0);
+    float scale = 0.0f;
+    if (test) {
+      bspan = ld32(p.b_span, task);
+      scale = p.work_scale[item];
+    }
+    bool keep = test;
+    if (kAffine) {
+      float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, blod = c0, bpar = c0, berr = c0;
+      if (test) {
+        c0 = ld32(p.i_col0, w.x);
+        c1 = ld32(p.i_col1, w.x);
+        c2 = ld32(p.i_col2, w.x);
+        c3 = ld32(p.i_col3, w.x);
+        blod = ld32(p.b_lod, task);
+        bpar = ld32(p.b_par, task);
+        berr = ld32(p.b_err, task);
       }
-      tail += total;
-      while (tail - head >= 64) {
-        wave_sync();
-        drain(64);
-        wave_sync();
-      }
-      if (head >= kQueue) {  // keep the ring positions small (their difference is what counts)
-        head -= kQueue;
-        tail -= kQueue;
+      const BlockBounds ib = instance_bounds(c0, c1, c2, c3, scale);
+      const uint32_t cut = (w.w & kTagShadow) ? 1u : 0u, vi = w.w >> kTagViewShift;
+      for (uint64_t pending = __ballot(test); pending;) {
+        const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+        const bool mine = test && vi == vu;
+        if (mine) {
+          const auto& v = scalar(p.views)[uniform(vu)];
+          const float low = cut ? v.low[1] : v.low[0], high = cut ? v.high[1] : v.high[0];
+          keep = !block_out_of_band(v, c0, c1, c2, c3, ib, blod, bpar, berr, low, high);
+        }
+        pending &= ~__ballot(mine);
       }
     }
+    // Queue the blocks that remain (at most 64 entries and 63 + 64 * kBlockRuns runs pending after this).
+    const uint64_t kept = __ballot(keep);
+    if (kept) {
+      const uint32_t nruns = keep ? bspan.y : 0u;
+      const uint32_t runs_incl = wave_scan(nruns);
+      const uint32_t e = bq_tail + static_cast<uint32_t>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(kept >> 32),
+                                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(kept), 0u)));
+      if (keep) {
+        bq[wib][e & (kBlockQueue - 1)] = make_uint4(w.x | (nruns << 24), bspan.x, w.w, __float_as_uint(scale));
+        bq_start[wib][e & (kBlockQueue - 1)] = rq_end + runs_incl - nruns;
+      }
+      bq_tail += static_cast<uint32_t>(__popcll(kept));
+      rq_end += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(runs_incl), 63));
+      while (rq_end - rq_pos >= 64) run_stage(64);
+    }
   });
+  if (rq_end != rq_pos) run_stage(rq_end - rq_pos);
   if (tail != head) {
     wave_sync();
     drain(tail - head);
