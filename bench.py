@@ -135,13 +135,13 @@ def serve_stream(server, ids, pool, rate, count, seed):
     return count / (t_sub - t0), time.perf_counter() - t0, records
 
 
-def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate):
+def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate, max_meshlets):
     """Served cut requests over 8 resident scenes (12k-3M instances): batched server vs one request per
-    launch, Poisson arrivals at each rate."""
+    launch, Poisson arrivals at each rate. Each view slot has the draw capacity of the world's views."""
     from vgcut import native, scenes
 
     t0 = time.time()
-    ctx = native.Context(max_triangles=100_000_000, max_shadow_triangles=10_000_000, max_meshlets=1 << 20, max_views=48,
+    ctx = native.Context(max_triangles=MAX_TRIANGLES, max_shadow_triangles=MAX_SHADOW_TRIANGLES, max_meshlets=max_meshlets, max_views=48,
                          mesh_slots=64, device=0)
     specs = scenes.serving_specs(spec_meshes)
     resident = []
@@ -159,9 +159,17 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate)
     log(f"serving: {len(scenes.SERVE_SIZES)} scenes, {sum(s.instance_count for _, s, _ in resident)} instances, "
         f"schedules tuned ({time.time() - t0:.1f} s)")
 
-    def server_for(batched):
-        sv = native.Server(ctx, lanes=4 if batched else 1, max_batch_views=48, max_batch_requests=0 if batched else 1)
+    def server_for(batched, policy=native.VG_SERVE_SHORTEST):
+        sv = native.Server(ctx, lanes=4 if batched else 1, max_batch_views=48, max_batch_requests=0 if batched else 1,
+                           policy=policy if batched else native.VG_SERVE_FIFO)
         return sv, [sv.add_scene(sc) for sc, _, _ in resident]
+
+    # request classes: views (light 1-4, heavy 5-16) x scene (small <= 152k instances, large >= 500k)
+    def klass(k):
+        _, sid, n = k
+        return ("light" if n <= 4 else "heavy") + ("_small" if scenes.SERVE_SIZES[sid] <= 152_000 else "_large")
+
+    classes = [klass((None, k, n)) for k, _, n in pool]
 
     # Work of each pool request (cluster tests) and the plain service time, from one pass alone.
     plain, plain_ids = server_for(False)
@@ -179,30 +187,45 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate)
     out = {"scenes_instances": list(scenes.SERVE_SIZES), "request_pool": len(pool), "views_per_request_mean": float(np.mean([p[2] for p in pool])),
            "cluster_tests_per_request_mean": float(np.mean(tests)), "plain_service_ms_mean": float(np.mean(service)) / 1e3,
            "plain_capacity_req_per_s": capacity, "hbm_resident_bytes": int(plain.stats().resident_bytes), "rates": []}
+    out["class_counts"] = {c: classes.count(c) for c in sorted(set(classes))}
+    out["class_plain_service_ms_mean"] = {c: float(np.mean([service[i] for i in range(len(pool)) if classes[i] == c])) / 1e3
+                                          for c in sorted(set(classes))}
     batched, batched_ids = server_for(True)
-    loads = sorted({int(capacity * f) for f in (0.25, 0.5, 0.75, 0.9)} | set(rates))
+    fifo, fifo_ids = server_for(True, native.VG_SERVE_FIFO)
+    legs = (("batched", batched, batched_ids), ("batched_fifo", fifo, fifo_ids), ("plain", plain, plain_ids))
+    loads = sorted({int(capacity * f) for f in (0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0)} | set(rates))
     for rate in loads:
         # about `requests_per_rate` seconds of arrivals, and at most about 2 s of work at plain capacity
         count = int(max(100, min(rate * requests_per_rate, 2.0 * capacity)))
         row = {"offered_req_per_s": rate, "requests": count}
-        for name, sv, ids in (("batched", batched, batched_ids), ("plain", plain, plain_ids)):
+        for name, sv, ids in legs:
             offered, elapsed, rec = serve_stream(sv, ids, pool, rate, count, seed=rate)
             lat = [r[3] / 1e3 for r in rec]
             ok = sum(1 for r in rec if r[1] == 0)
             row[name] = {"submitted_req_per_s": offered, "completed_req_per_s": count / elapsed,
                          "views_clusters_per_s": sum(tests[r[0]] for r in rec) / elapsed,
                          "latency_ms_p50": percentile(lat, 50), "latency_ms_p99": percentile(lat, 99),
-                         "mean_batch_requests": float(np.mean([r[2] for r in rec])), "ok": ok}
-        for name, sv in (("batched", batched), ("plain", plain)):
+                         "mean_batch_requests": float(np.mean([r[2] for r in rec])), "ok": ok, "classes": {}}
+            for c in sorted(set(classes)):
+                cl = [r[3] / 1e3 for r in rec if classes[r[0]] == c]
+                if cl:
+                    row[name]["classes"][c] = {"n": len(cl), "p50_ms": percentile(cl, 50), "p99_ms": percentile(cl, 99)}
+            light = [r[3] / 1e3 for r in rec if classes[r[0]].startswith("light")]
+            row[name]["light_p99_ms"] = percentile(light, 99) if light else None
+        for name, sv, _ in legs:
             st = sv.stats()
             row[name]["peak_inflight_bytes"] = int(st.peak_inflight_bytes)
             row[name]["reruns_so_far"] = int(st.reruns)
         out["rates"].append(row)
-        log(f"serving {rate} req/s: batched p50 {row['batched']['latency_ms_p50']:.2f} p99 {row['batched']['latency_ms_p99']:.2f} ms "
-            f"({row['batched']['completed_req_per_s']:.0f} req/s, batch {row['batched']['mean_batch_requests']:.1f}); "
-            f"plain p50 {row['plain']['latency_ms_p50']:.2f} p99 {row['plain']['latency_ms_p99']:.2f} ms "
-            f"({row['plain']['completed_req_per_s']:.0f} req/s)")
+        log(f"serving {rate} req/s: " + "; ".join(
+            f"{name} p50 {row[name]['latency_ms_p50']:.2f} p99 {row[name]['latency_ms_p99']:.2f} ms light p99 {row[name]['light_p99_ms']:.2f} "
+            f"({row[name]['completed_req_per_s']:.0f} req/s, batch {row[name]['mean_batch_requests']:.1f})" for name, _, _ in legs))
+    for name, _, _ in legs:
+        ok_rates = [r["offered_req_per_s"] for r in out["rates"] if r[name]["light_p99_ms"] is not None and r[name]["light_p99_ms"] < 2.0]
+        out[f"{name}_light_p99_under_2ms_max_offered_req_per_s"] = max(ok_rates) if ok_rates else 0
+        out[f"{name}_capacity_req_per_s"] = max(r[name]["completed_req_per_s"] for r in out["rates"])
     batched.close()
+    fifo.close()
     plain.close()
     return out
 
@@ -471,7 +494,7 @@ def main():
         f"triangles ({time.time() - t0:.1f} s)")
 
     if args.serve_only:
-        print(json.dumps({"serving": serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES, args.serve_seconds)}))
+        print(json.dumps({"serving": serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES, args.serve_seconds, args.max_meshlets)}))
         return
     if args.expand_only:
         vd = scenes.mixed_views(spec, args.views)
@@ -749,7 +772,7 @@ def main():
         "two_phase_occlusion": two_phase,
         "expand": expand,
         "serving": None if args.no_serve or args.profile_only else serve_bench(spec.meshes[:spec.props], make_view, cam, scenegen, SERVE_RATES,
-                                                                               args.serve_seconds),
+                                                                               args.serve_seconds, args.max_meshlets),
     }
     print(json.dumps(out))
     # The metric in one line at the end of the log (the JSON line above is long).
@@ -757,9 +780,11 @@ def main():
     serve_txt = ""
     if srv:
         rows = srv["rates"]
-        serve_txt = "; served " + ", ".join(
+        serve_txt = (f"; served capacity {srv['batched_capacity_req_per_s']:.0f} req/s (plain {srv['plain_capacity_req_per_s']:.0f}), "
+                     f"light-class p99 < 2 ms up to {srv['batched_light_p99_under_2ms_max_offered_req_per_s']} req/s offered; " + ", ".join(
             f"{r['offered_req_per_s']} req/s p50/p99 {r['batched']['latency_ms_p50']:.2f}/{r['batched']['latency_ms_p99']:.2f} ms"
-            f" (plain {r['plain']['latency_ms_p50']:.2f}/{r['plain']['latency_ms_p99']:.2f})" for r in rows[:4])
+            f" light p99 {r['batched']['light_p99_ms']:.2f} (fifo {r['batched_fifo']['latency_ms_p99']:.2f}, plain "
+            f"{r['plain']['latency_ms_p50']:.2f}/{r['plain']['latency_ms_p99']:.2f})" for r in rows[:3]))
     plain = out["plain"]
     log(f"summary: batched {out['ms_per_step']:.3f} ms per {views_n} views ({value / 1e9:.1f}G views*clusters/s), latency p50/p99 "
         f"{out['batched']['latency_ms_p50']:.3f}/{out['batched']['latency_ms_p99']:.3f} ms"

@@ -286,13 +286,17 @@ int vg_tune(vg_scene* scene, const vg_view* views, uint32_t count, uint32_t repe
 /*
  * Request server: cut requests against resident scenes, batched continuously (no counterpart in the
  * reference, which cuts its views inside VirtualGeometry.update, VirtualGeometry.ts:392-507; this is
- * the serving form of that path). A worker thread keeps a FIFO of requests; whenever a lane (a HIP
- * stream) is free it launches one batch: queued requests of one scene, oldest first, that fit the
- * batch's view limit, the scene's free view slots and the HBM budget. A batch's transient buffers
+ * the serving form of that path). A worker thread keeps a queue of requests; whenever a lane (a HIP
+ * stream) is free it launches one batch: queued requests of one scene that fit the batch's view limit,
+ * the scene's free view slots and the HBM budget. Under VG_SERVE_FIFO the batch starts at the oldest
+ * request that fits and takes the scene's others oldest first; under VG_SERVE_SHORTEST (the default)
+ * it starts at the request of least estimated run time (views x the scene's measured time per view)
+ * and takes the scene's others cheapest first, except that a request queued longer than max_bypass_us
+ * starts the next batch, oldest first. A batch's transient buffers
  * (cell queue, work list) come from the device's stream-ordered memory pool and are returned when it
  * ends; the work list starts at first_work_items and a batch that overflows it runs again at its worst-case
  * size (every (view, instance, cut)), outside the budget check. Requests that do not fit wait while
- * others overtake them, up to max_bypass_us. Batches on
+ * others overtake them, up to max_bypass_us (default 2000 under FIFO, 20000 under SHORTEST). Batches on
  * different lanes overlap on the device. Each request's views start from a controller at rest (scale 1),
  * so a result does not depend on the batch it ran in. Results stay in the request's view slots until
  * vg_server_release. Scenes given to a server must not be changed or cut directly while it runs, and
@@ -305,7 +309,10 @@ typedef struct vg_server_config {
   uint32_t max_batch_requests;  /* requests per batch; 0: no limit, 1: one request per launch */
   uint32_t max_bypass_us;       /* how long later requests may overtake one that does not fit (default 2000) */
   uint32_t first_work_items;    /* work-list items of a batch's first launch (default 1 << 22) */
+  uint32_t policy;              /* VG_SERVE_SHORTEST (default) or VG_SERVE_FIFO */
 } vg_server_config;
+
+enum { VG_SERVE_SHORTEST = 0, VG_SERVE_FIFO = 1 };
 
 typedef struct vg_server vg_server;
 

@@ -137,6 +137,10 @@ def test_each_request_is_the_same_alone_and_in_the_mix(gpu_available):
     ids = [server.add_scene(s) for s in resident]
     run_mix(server, ids, reqs, alone)
     server.close()
+    server = native.Server(ctx, lanes=2, max_batch_views=16, policy=native.VG_SERVE_FIFO)
+    ids = [server.add_scene(s) for s in resident]
+    run_mix(server, ids, reqs, alone)
+    server.close()
     # a first work list far too small: every batch overflows it and runs again at full size
     server = native.Server(ctx, lanes=2, max_batch_views=16, first_work_items=64)
     ids = [server.add_scene(s) for s in resident]
@@ -192,3 +196,30 @@ def test_admission_by_hbm_budget_and_view_slots(gpu_available):
     server.release(t2)
     server.release(t3)
     server.close()
+
+
+def test_shortest_first_and_fifo_order(gpu_available):
+    """One lane: while a first request runs, a 16-view request of one scene and then a 1-view request of
+    another queue behind it. Shortest-first launches the 1-view request first; FIFO keeps arrival order."""
+    specs = [small_ruins(seed=95, instances=800, meshes=3), small_ruins(seed=96, instances=2500, meshes=3),
+             small_ruins(seed=97, instances=1500, meshes=3)]
+    ctx = native.Context(max_triangles=2_000_000, max_shadow_triangles=1_000_000, max_meshlets=MAX_MESHLETS, max_views=16, device=0)
+    resident = make_scenes(ctx, specs)
+    rng = np.random.default_rng(5)
+    first, heavy, light = request_views(rng, 4), request_views(rng, 16), request_views(rng, 1)
+    for policy, light_first in ((native.VG_SERVE_SHORTEST, True), (native.VG_SERVE_FIFO, False)):
+        server = native.Server(ctx, lanes=1, max_batch_views=16, policy=policy)
+        ids = [server.add_scene(s) for s in resident]
+        ta = server.submit(ids[2], first)
+        tb = server.submit(ids[1], heavy)
+        tc = server.submit(ids[0], light)
+        queued = {}
+        t0 = time.time()
+        while len(queued) < 3:
+            for d in server.poll(8, wait_us=2000):
+                assert d.status == native.VG_OK
+                queued[d.ticket] = d.queued_us
+                server.release(d.ticket)
+            assert time.time() - t0 < 30
+        assert (queued[tc] < queued[tb]) == light_first, (policy, queued[ta], queued[tb], queued[tc])
+        server.close()

@@ -45,6 +45,10 @@ struct SceneEntry {
   uint64_t resident = 0;
   std::vector<uint32_t> free_slots;
   uint32_t items_per_view = 0;  // most work items per view any batch of this scene used (0: none yet)
+  // Estimated run time of one view of this scene: a running mean of (batch run time / views) over the
+  // scene's finished batches; before the first one, a guess from the instance count.
+  double us_per_view = 0;
+  bool measured = false;
 };
 
 struct Batch {
@@ -54,6 +58,7 @@ struct Batch {
   std::vector<Request*> requests;
   uint64_t bytes = 0;
   bool full = false;  // work list at its worst-case size (after an overflow of the optimistic one)
+  double t_launch = 0;
 };
 
 // A batch's work list starts at this many items (uint4 item + u64 first task + f32 scale: 28 B each)
@@ -132,6 +137,11 @@ struct vg_server {
         const uint64_t need = static_cast<uint64_t>(lane_flags[b.lane][4]) * vg::kWorkParts;
         const uint32_t per_view = static_cast<uint32_t>((need + b.views - 1) / std::max(1u, b.views));
         se.items_per_view = std::max(se.items_per_view, per_view);
+        if (status == VG_OK && b.views) {
+          const double run = (now_us() - b.t_launch) / b.views;
+          se.us_per_view = se.measured ? 0.75 * se.us_per_view + 0.25 * run : run;
+          se.measured = true;
+        }
       }
       if (status == VG_E_CAPACITY && !b.full) {
         // The optimistic work list overflowed: run the same batch (same slots) again at full size.
@@ -171,8 +181,57 @@ struct vg_server {
     return -1;
   }
 
+  double estimate(const Request* r) const { return scenes[r->scene].us_per_view * static_cast<double>(r->views.size()); }
+
+  // Picks the next batch (one scene; fits the view limit, the scene's free slots and memory) under the
+  // shortest-first policy: the head is the oldest request that has waited longer than max_bypass_us,
+  // else the queued request of least estimated run time that fits; the batch is then filled with the
+  // head scene's other requests, least estimated run time first. An overdue head that does not fit
+  // stops the pick: nothing overtakes it. With the mutex held.
+  bool pick_shortest(Batch& b) {
+    const uint64_t budget_left = cfg.hbm_budget > resident + inflight ? cfg.hbm_budget - resident - inflight : 0;
+    const double t = now_us();
+    auto fits = [&](const Request* r, uint32_t views_before) {
+      const uint32_t n = views_before + static_cast<uint32_t>(r->views.size());
+      return n <= scenes[r->scene].free_slots.size() && n <= cfg.max_batch_views &&
+             transient_bytes(scenes[r->scene], n, false, cfg.first_work_items) <= budget_left;
+    };
+    size_t head = queue.size();
+    if (!queue.empty() && t - queue.front()->t_submit > cfg.max_bypass_us) {
+      if (!fits(queue.front(), 0)) return false;
+      head = 0;
+    } else {
+      double best = 0;
+      for (size_t i = 0; i < queue.size(); i++) {
+        const double e = estimate(queue[i]);
+        if ((head == queue.size() || e < best) && fits(queue[i], 0)) {
+          head = i;
+          best = e;
+        }
+      }
+      if (head == queue.size()) return false;
+    }
+    Request* h = queue[head];
+    b.scene = h->scene;
+    b.views = static_cast<uint32_t>(h->views.size());
+    b.requests.push_back(h);
+    std::vector<std::pair<double, Request*>> rest;
+    for (Request* r : queue)
+      if (r != h && r->scene == b.scene) rest.emplace_back(estimate(r), r);
+    std::stable_sort(rest.begin(), rest.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (const auto& e : rest) {
+      if (cfg.max_batch_requests && b.requests.size() >= cfg.max_batch_requests) break;
+      if (!fits(e.second, b.views)) continue;
+      b.views += static_cast<uint32_t>(e.second->views.size());
+      b.requests.push_back(e.second);
+    }
+    return true;
+  }
+
+  bool pick(Batch& b) { return cfg.policy == VG_SERVE_FIFO ? pick_fifo(b) : pick_shortest(b); }
+
   // Picks the next batch from the queue (FIFO, one scene, fits views, slots and memory). With the mutex held.
-  bool pick(Batch& b) {
+  bool pick_fifo(Batch& b) {
     const uint64_t budget_left = cfg.hbm_budget > resident + inflight ? cfg.hbm_budget - resident - inflight : 0;
     const double t = now_us();
     std::vector<uint32_t> slots_left(scenes.size());
@@ -232,6 +291,7 @@ struct vg_server {
     batch_sizes(se, b.views, b.full, cfg.first_work_items, &queue_bytes, &work_items);
     b.bytes = transient_bytes(se, b.views, b.full, cfg.first_work_items);
     b.lane = static_cast<uint32_t>(lane_index);
+    b.t_launch = t;
     void* ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
     const size_t sizes[4] = {queue_bytes, sizeof(uint4) * work_items, sizeof(unsigned long long) * work_items, sizeof(float) * work_items};
     int rc = VG_OK;
@@ -321,7 +381,8 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
   if (config) c = *config;
   if (!c.lanes) c.lanes = 4;
   if (!c.max_batch_views) c.max_batch_views = ctx->cfg.max_views;
-  if (!c.max_bypass_us) c.max_bypass_us = 2000;
+  if (!c.max_bypass_us) c.max_bypass_us = c.policy == VG_SERVE_FIFO ? 2000 : 20000;
+  if (c.policy > VG_SERVE_FIFO) return fail(ctx, VG_E_INVALID, "vg_server_create: unknown policy");
   if (!c.first_work_items) c.first_work_items = kFirstWorkItems;
   if (c.lanes > 16 || c.max_batch_views > ctx->cfg.max_views) return fail(ctx, VG_E_INVALID, "vg_server_create: lanes 1-16, max_batch_views <= max_views");
   if (!c.hbm_budget) {
@@ -409,6 +470,7 @@ int vg_server_add_scene(vg_server* sv, vg_scene* s, uint32_t* scene_id) {
   SceneEntry e;
   e.scene = s;
   e.resident = bytes;
+  e.us_per_view = 20.0 + 1e-4 * static_cast<double>(s->i_slot.size());  // replaced by the first measured batch
   for (uint32_t slot = ctx->cfg.max_views; slot-- > 0;) e.free_slots.push_back(slot);
   sv->resident += bytes;
   sv->scenes.push_back(std::move(e));

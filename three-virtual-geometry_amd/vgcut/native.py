@@ -18,6 +18,8 @@ VG_E_INVALID = -1
 VG_E_NOMEM = -2
 VG_E_CAPACITY = -3
 VG_E_DEVICE = -4
+VG_SERVE_SHORTEST = 0
+VG_SERVE_FIFO = 1
 
 VG_POOL_MESH_TABLE = 0
 VG_POOL_INSTANCES = 1
@@ -127,6 +129,7 @@ class ServerConfig(ctypes.Structure):
         ("max_batch_requests", ctypes.c_uint32),
         ("max_bypass_us", ctypes.c_uint32),
         ("first_work_items", ctypes.c_uint32),
+        ("policy", ctypes.c_uint32),
     ]
 
 
@@ -513,11 +516,11 @@ class Server:
     """Request server over resident scenes (vg_server_*): continuous batching on `lanes` HIP streams."""
 
     def __init__(self, ctx: "Context", hbm_budget: int = 0, lanes: int = 4, max_batch_views: int = 0, max_batch_requests: int = 0,
-                 max_bypass_us: int = 0, first_work_items: int = 0):
+                 max_bypass_us: int = 0, first_work_items: int = 0, policy: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         self.scenes = []  # keeps the Scene objects alive while the server uses them
-        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us, first_work_items)
+        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us, first_work_items, policy)
         handle = ctypes.c_void_p()
         ctx.check(self.lib.vg_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(handle)), "vg_server_create")
         self.handle = handle
