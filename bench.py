@@ -86,6 +86,7 @@ def parse():
     ap.add_argument("--serve-only", action="store_true", help="only the served-requests measurement")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="requests per rate = rate x this (at least 1000)")
     ap.add_argument("--no-expand", action="store_true", help="skip the expand-pass measurement")
+    ap.add_argument("--plain-configs", action="store_true", help="only the plain single-view configurations (icosphere, ruins)")
     ap.add_argument("--expand-only", action="store_true", help="only the expand-pass measurement (with --profile-only: for rocprofv3)")
     ap.add_argument("--two-phase-only", action="store_true", help="only the two-phase occlusion frames with their rasters (for rocprofv3)")
     ap.add_argument("--schedule", type=str, default=None,
@@ -470,6 +471,135 @@ def two_phase_bench(scene, vd, steps):
     return out
 
 
+def plain_configs_bench(cpu_seconds):
+    """The plain single-view configurations, each view cut by its own launch sequence in the controller's
+    steady state: the 1M-triangle icosphere (one 1080p view at 1 px), and the 113k-instance ruins (8 poses at
+    1080p and 2160p, 2 m eye height, the heightfield's depth as HZB), at the reference's default capacities
+    (10M / 4M triangles) with an 8M-entry draw list, graph replay; per view: launch-to-completion latency
+    p50/p99 over repeated launches, views*clusters tested/s and visible clusters/s; for the icosphere also the
+    Node CPU cut (one thread), the reference's CPU-runnable case. Then the world's fly-through (below)."""
+    import math
+
+    import torch
+
+    from oracle import oracle
+    from vgcut import camera as cam
+    from vgcut import native, scenegen, scenes
+    from vgcut.virtual_geometry import make_view
+
+    def run(scene, views, repeats):
+        lat, tests, drawn = [], 0, 0
+        for v in views:
+            for _ in range(6):  # uploads, graph capture, controller steady state
+                scene.cut([v], graph=True)
+            scene.sync()
+            counters, _, _, met = scene.read(v.slot, max_entries=1)
+            tests += int(met.clusters_tested[0] + met.clusters_tested[1])
+            drawn += int(counters[2]) + int(counters[7])
+            for _ in range(repeats):
+                t = time.perf_counter()
+                scene.cut([v], graph=True)
+                scene.sync()
+                lat.append((time.perf_counter() - t) * 1e3)
+        per_view_s = float(np.mean(lat)) / 1e3
+        return {"views": len(views), "cluster_tests_per_view": tests / len(views), "drawn_per_view": drawn / len(views),
+                "latency_ms_p50": percentile(lat, 50), "latency_ms_p99": percentile(lat, 99),
+                "views_clusters_per_s": tests / len(views) / per_view_s, "visible_clusters_per_s": drawn / len(views) / per_view_s}
+
+    out = {}
+    torch.zeros(1, device="cuda:0")
+    ctx = native.Context(max_meshlets=8 << 20, max_views=16, device=0)
+    t0 = time.time()
+    spec, camera, height = scenes.icosphere_view()
+    scene = native.Scene(ctx)
+    scenes.load_into(scene, spec)
+    view = make_view(camera, int(height), slot=0)
+    out["config0_icosphere"] = run(scene, [view], 200)
+    pool = oracle.pool_from_scene(scene, ctx.config.max_meshlets, ctx.config.max_triangles, ctx.config.max_shadow_triangles)
+    scale = scene.state(0)[0] / 10000.0
+    node = oracle.node_cpu_baseline(pool, [view], [scale], seconds=cpu_seconds, threads=1)
+    one = node["single_thread"]
+    out["config0_icosphere"]["cpu_baseline"] = {"value": one["tests_per_s"], "unit": "views*clusters tested/s", "cores": 1, "kind": "port",
+                                               "cpu_model": node["cpu_model"], "runtime": f"node {node['node']}",
+                                               "sample": f"oracle/cpu_cut.js, one thread: {one['tests']} cluster tests in {one['seconds']:.2f} s"}
+    log(f"config 0 ({time.time() - t0:.1f} s): {out['config0_icosphere']}")
+    scene.close()
+    del pool
+
+    t0 = time.time()
+    spec = scenes.ruins()
+    scene = native.Scene(ctx)
+    scenes.load_into(scene, spec, occlusion=True)
+    hf = spec.heightfield
+    rng = np.random.default_rng(4)
+    views = []
+    for pose in range(8):
+        eye = (float(rng.uniform(-400, 400)), 2.0, float(rng.uniform(-400, 400)))
+        yaw = float(rng.uniform(0, 2 * math.pi))
+        target = (eye[0] + 10 * math.cos(yaw), 2.0, eye[2] + 10 * math.sin(yaw))
+        for k, (h, hw, hh) in enumerate(((1080, 960, 540), (2160, 1024, 576))):
+            camera = cam.perspective_camera(60, 16 / 9, 0.1, 2000).look_at(eye, target)
+            vp = cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse)
+            scene.set_hzb(pose * 2 + k, scenegen.heightfield_depth(vp, hw, hh, (0.0, 0.0), 500.0, 256, hf["amplitude"], hf["frequency"], hf["seed"]))
+            views.append(make_view(camera, h, slot=pose * 2 + k, occlusion=True))
+    out["config1_ruins"] = {"1080p": run(scene, views[0::2], 40), "2160p": run(scene, views[1::2], 40)}
+    log(f"config 1 ({time.time() - t0:.1f} s): {out['config1_ruins']}")
+    scene.close()
+
+    # The world's 16-frame fly-through at 1080p with the 8M-triangle budget (GPU histogram pick) and two-phase
+    # occlusion: the first phase against the raster of the previous frame's visible set, the second against
+    # the raster of the first phase's clusters (960x540 depth).
+    t0 = time.time()
+    spec = scenes.world(terrain_grid=512)
+    wctx = native.Context(max_triangles=20_000_000, max_shadow_triangles=10_000_000, max_meshlets=4 << 20, max_views=1,
+                          mesh_slots=72, device=0)
+    scene = native.Scene(wctx)
+    scenes.load_into(scene, spec, occlusion=True)
+    path = scenes.flythrough(spec, 16, 6144.0, 30.0, seed=9)
+    w, h = 960, 540
+    frames = []
+    for lap in range(2):  # the first lap warms up (uploads, graph capture, the controller)
+        scene.set_hzb(0, np.ones((h, w), dtype=np.float32))
+        for f in range(16):
+            eye, target = path[f]
+            camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
+            vp = np.asarray(cam.mat_mul(camera.projectionMatrix, camera.matrixWorldInverse), dtype=np.float32)
+            view = make_view(camera, 1080, occlusion=2, triangle_budget=8_000_000, threshold_range=(0.5, 3.0))
+            torch.cuda.synchronize()
+            t = [time.perf_counter()]
+            if f:
+                scene.hzb_from_cut(0, vp, w, h, with_phase2=True)
+                scene.sync()
+            t.append(time.perf_counter())
+            scene.cut([view])
+            scene.sync()
+            t.append(time.perf_counter())
+            scene.hzb_from_cut(0, vp, w, h)
+            scene.sync()
+            t.append(time.perf_counter())
+            scene.cut_phase2([view])
+            scene.sync()
+            t.append(time.perf_counter())
+            counters, _, _, met = scene.read(0, max_entries=1)
+            c2, _ = scene.read_phase2(0)
+            if lap:
+                frames.append({"ms": (t[4] - t[0]) * 1e3, "raster_ms": (t[1] - t[0] + t[3] - t[2]) * 1e3,
+                               "cut_ms": (t[2] - t[1] + t[4] - t[3]) * 1e3, "tests": int(met.clusters_tested[0] + met.clusters_tested[1]),
+                               "drawn": int(c2[2]), "added": int(c2[2]) - int(counters[2])})
+    cut_s = sum(fr["cut_ms"] for fr in frames) / 1e3
+    out["config2_world_flythrough"] = {
+        "frames": len(frames), "frame_ms_p50": percentile([fr["ms"] for fr in frames], 50),
+        "frame_ms_p99": percentile([fr["ms"] for fr in frames], 99),
+        "cut_ms_p50": percentile([fr["cut_ms"] for fr in frames], 50), "cut_ms_p99": percentile([fr["cut_ms"] for fr in frames], 99),
+        "raster_ms_mean": float(np.mean([fr["raster_ms"] for fr in frames])),
+        "views_clusters_per_s": sum(fr["tests"] for fr in frames) / cut_s, "visible_clusters_per_s": sum(fr["drawn"] for fr in frames) / cut_s,
+        "cluster_tests_per_frame": float(np.mean([fr["tests"] for fr in frames])), "drawn_per_frame": float(np.mean([fr["drawn"] for fr in frames])),
+        "phase2_added_per_frame": float(np.mean([fr["added"] for fr in frames]))}
+    log(f"config 2 ({time.time() - t0:.1f} s): {out['config2_world_flythrough']}")
+    scene.close()
+    return out
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
@@ -487,6 +617,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda:0")
+    if args.plain_configs:
+        print(json.dumps({"plain_configs": plain_configs_bench(args.cpu_seconds)}))
+        return
     t0 = time.time()
     spec = scenes.world(mesh_count=args.meshes, instances=args.instances, terrain_grid=512)
     log(f"world: {spec.instance_count} instances, {len(spec.meshes)} meshes, "

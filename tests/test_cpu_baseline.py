@@ -31,7 +31,9 @@ def test_node_cpu_cut_counts_equal_the_oracle():
     views.append(make_view(ortho, 2048, slot=0, min_pixel_radius=0.0))
     scales.append(1.0)
     got = oracle.node_cpu_baseline(pool, views, scales, seconds=60.0, threads=2)
-    assert got["single_thread"]["views"] == len(views), "the one-thread run must cover every view here"
+    one = got["single_thread"]
+    passes = one["views"] // len(views)  # a short view list is cut again until a second has passed
+    assert passes >= 1 and one["views"] == passes * len(views), "the one-thread run must cover every view, whole passes here"
     tests = drawn = 0
     for v, s in zip(views, scales):
         state = oracle.initial_state()
@@ -41,7 +43,7 @@ def test_node_cpu_cut_counts_equal_the_oracle():
         tests += int(r.stats.clusters_tested[0])
         drawn += int(r.counters[2])
     assert drawn > 1000
-    assert got["single_thread"]["tests"] == tests and got["single_thread"]["drawn"] == drawn, (got["single_thread"], tests, drawn)
+    assert one["tests"] == passes * tests and one["drawn"] == passes * drawn, (one, tests, drawn)
     assert got["threads"]["tests"] == tests and got["threads"]["drawn"] == drawn
     assert got["threads"]["n"] == 2 and got["cpu_model"]
     scene.close()
