@@ -1384,7 +1384,7 @@ int vg_cut_static(vg_scene* s, const vg_view* views, uint32_t count) {
   if (flags[3])
     return fail(ctx, VG_E_CAPACITY, "vg_cut_static: the work list (" + std::to_string(items) +
                                         " items in the static scratch) overflowed; results are incomplete");
-  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut_static: meshlet pass lost its work item (internal error)");
+  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut_static: internal error in the meshlet pass");
   return VG_OK;
 }
 
@@ -1443,7 +1443,7 @@ int vg_sync(vg_scene* s) {
   uint32_t flags[4] = {0, 0, 0, 0};
   VG_HIP(ctx, hipMemcpy(flags, s->lane.d_flags.ptr, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[3]) return fail(ctx, VG_E_CAPACITY, "vg_cut: work list capacity exceeded; results are incomplete");
-  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut: meshlet pass lost its work item (internal error)");
+  if (flags[2]) return fail(ctx, VG_E_DEVICE, "vg_cut: internal error in the meshlet pass");
   return VG_OK;
 }
 

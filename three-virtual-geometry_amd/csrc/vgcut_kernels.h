@@ -59,9 +59,10 @@ constexpr uint32_t kBlockRuns = 16;
 // Per-wave LDS queue of blocks that passed, awaiting their runs' tests: at most 63 entries with untested
 // runs plus the last consumed one, plus one chunk's 64 survivors.
 constexpr uint32_t kBlockQueue = 128;
-// Per-wave LDS ring of meshlets in the cut awaiting their visibility test: at most 63 wait before a
-// wave appends up to 64 * kMaxRun, so 64 * (kMaxRun + 1) entries suffice.
-constexpr uint32_t kQueue = 64 * (kMaxRun + 1);
+// Per-wave LDS ring of meshlets in the cut awaiting their visibility test: a wave appends its lanes' k-th
+// meshlets one k at a time (at most 64 each) and drains full waves in between, so at most 63 + 64 wait.
+constexpr uint32_t kQueue = 128;
+static_assert((kQueue & (kQueue - 1)) == 0, "ring positions wrap by masking");
 // An index is (draw-list position << kLocalVertexBits) | local vertex (GeometryPool.ts:92, 1009-1018).
 constexpr uint32_t kLocalVertexBits = 7;  // ceil(log2(MAX_MESHLET_VERTICES = 128))
 constexpr uint32_t kExpandBlocksPerCu = 8;  // persistent 256-thread blocks of the expand pass
@@ -428,19 +429,17 @@ __device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
   return v;
 }
 
-// atomicAdd of `value` for every lane with `active`, aggregated per distinct address: one atomic per
-// address per wave. Returns the lane's old value (the group base plus its exclusive prefix). All 64
-// lanes must call it together.
-template <bool kOne = false>
-__device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t value, bool active) {
+// Groups the active lanes of a wave by address (`addr` per lane) for an aggregated add: each lane learns
+// its group's leader (the group's lowest lane) and its exclusive prefix in the group; each leader learns
+// the group's total. kOne: every active lane adds 1.
+template <bool kOne>
+__device__ __forceinline__ void wave_group(const uint32_t* addr, uint32_t value, bool active, uint32_t& leader_of, uint32_t& prefix_of,
+                                           uint32_t& leader_total) {
   uint64_t pending = __ballot(active);
   const uint32_t lane = lane_id();
   const uint32_t a_lo = static_cast<uint32_t>(reinterpret_cast<uint64_t>(addr));
   const uint32_t a_hi = static_cast<uint32_t>(reinterpret_cast<uint64_t>(addr) >> 32);
-  // Every group's atomic is issued before any result is read: the groups' round trips overlap, and each
-  // lane then reads its group leader's result with one lane shuffle. The leader is wave-uniform, so its
-  // address is read with readlane.
-  uint32_t leader_base = 0, my_leader = 0, my_offset = 0;
+  leader_of = prefix_of = leader_total = 0;
   while (pending) {
     const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
     const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(a_lo), leader));
@@ -448,7 +447,7 @@ __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t val
     const bool mine = ((pending >> lane) & 1ull) && a_lo == lo && a_hi == hi;
     const uint64_t group = __ballot(mine);
     uint32_t prefix, total;
-    if (kOne) {  // every lane adds 1: the exclusive prefix counts the group's lanes below this one
+    if (kOne) {  // the exclusive prefix counts the group's lanes below this one
       prefix = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(group >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(group), 0u));
       total = static_cast<uint32_t>(__popcll(group));
     } else {
@@ -457,16 +456,27 @@ __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t val
       prefix = incl - v;
       total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
     }
-    if (lane == static_cast<uint32_t>(leader) && total)
-      leader_base = atomicAdd(reinterpret_cast<uint32_t*>((static_cast<uint64_t>(hi) << 32) | lo), total);
+    if (lane == static_cast<uint32_t>(leader)) leader_total = total;
     if (mine) {
-      my_leader = static_cast<uint32_t>(leader);
-      my_offset = prefix;
+      leader_of = static_cast<uint32_t>(leader);
+      prefix_of = prefix;
     }
     pending &= ~group;
   }
-  const uint32_t base = __shfl(leader_base, static_cast<int>(my_leader), 64);
-  return active ? base + my_offset : 0u;
+}
+
+// atomicAdd of `value` for every lane with `active`, aggregated per distinct address: one atomic per
+// address per wave, all issued by their leaders in one instruction after the grouping (a loop of
+// returning atomics into one register would wait for each before the next). Returns the lane's old
+// value (the group base plus its exclusive prefix). All 64 lanes must call it together.
+template <bool kOne = false>
+__device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* addr, uint32_t value, bool active) {
+  uint32_t leader, prefix, total;
+  wave_group<kOne>(addr, value, active, leader, prefix, total);
+  uint32_t base = 0;
+  if (total) base = atomicAdd(addr, total);  // a leader's own address is its group's
+  const uint32_t b = __shfl(base, static_cast<int>(leader), 64);
+  return active ? b + prefix : 0u;
 }
 
 // One atomicAdd per wave on a wave-uniform address; returns each active lane's old value.

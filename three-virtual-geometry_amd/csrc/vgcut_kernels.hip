@@ -803,8 +803,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
 
   auto drain = [&](uint32_t n) __attribute__((always_inline)) {
     const bool active = lane < n;
-    const uint32_t at_ring = head + lane >= kQueue ? head + lane - kQueue : head + lane;  // head < kQueue
-    const uint4 c = active ? ring[wib][at_ring] : make_uint4(0, 0, 0, 0);
+    const uint4 c = active ? ring[wib][(head + lane) & (kQueue - 1)] : make_uint4(0, 0, 0, 0);
     const float scale = __uint_as_float(c.w);
     const uint32_t instance = c.x & 0xffffffu, meshlet = c.y & 0xffffffu, tag = c.z;
     const uint32_t fades = (c.x >> 24) | ((c.y >> 24) << 8);
@@ -877,25 +876,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerSi
     if (write) p.draws[vslot][at + cut * p.max_meshlets] = make_uint4(instance, meshlet, first_index, packed_tris);
   };
 
-  // Appends `count` (<= kMaxRun) meshlets from `first` per lane to the ring and drains full waves of it.
+  // Appends `count` (<= kMaxRun) meshlets from `first` per lane to the ring, the lanes' k-th meshlets
+  // together for k = 0, 1, ..., and drains a full wave whenever 64 wait (so at most 63 + 64 are queued).
+  // Ring positions run freely; their difference is what counts.
   auto ring_append = [&](uint32_t count, uint32_t first, uint32_t fades, uint32_t instance, uint32_t tag, float scale) __attribute__((always_inline)) {
-    const uint32_t incl = wave_scan(count);
-    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-    if (!total) return;
-    const uint32_t at = tail + incl - count;
-    for (uint32_t k = 0; k < count; k++) {
-      const uint32_t r = at + k >= kQueue ? at + k - kQueue : at + k;  // at + k < 2 kQueue: head < kQueue, fewer than 64 + 4 * 64 pending
-      ring[wib][r] = make_uint4(instance | (fades << 24), (first + k) | ((fades >> 8) << 24), tag, __float_as_uint(scale));
-    }
-    tail += total;
-    while (tail - head >= 64) {
-      wave_sync();
-      drain(64);
-      wave_sync();
-    }
-    if (head >= kQueue) {  // keep the ring positions small (their difference is what counts)
-      head -= kQueue;
-      tail -= kQueue;
+    for (uint32_t k = 0; k < kMaxRun; k++) {
+      const bool has = k < count;
+      const uint64_t round = __ballot(has);
+      if (!round) break;
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(round >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(round), 0u));
+      if (has)
+        ring[wib][(tail + below) & (kQueue - 1)] =
+            make_uint4(instance | (fades << 24), (first + k) | ((fades >> 8) << 24), tag, __float_as_uint(scale));
+      tail += static_cast<uint32_t>(__popcll(round));
+      if (tail - head >= 64) {
+        wave_sync();
+        drain(64);
+        wave_sync();
+      }
     }
   };
 
