@@ -513,7 +513,8 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
   if (lane->launch_budget) {
     hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
-    hipLaunchKernelGGL(k_budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+    void (*const budget_meshlets)(Params) = affine_scene(s) ? k_budget_meshlets<true> : k_budget_meshlets<false>;
+    hipLaunchKernelGGL(budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
     hipLaunchKernelGGL(k_budget_pick, dim3(view_count), dim3(64), 0, st, p);
   }
   mark();

@@ -1105,8 +1105,9 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       if (__ballot(keep)) lr = select_band(p, level_base, level_count, v.budget_low, v.budget_high, pmin, pmax);
       const bool work = keep && lr.last > lr.first;
       if (!__ballot(work)) continue;
-      // The item covers the LOD runs of [first, last) (level ranges hold whole runs).
-      const uint32_t run_lo = work ? p.m_run[lr.first] : 0u, run_hi = work ? p.m_run[lr.last - 1] + 1 : 0u;
+      // The item covers the LOD blocks of [first, last) (level ranges hold whole runs, runs and blocks never
+      // cross a level's bounds).
+      const uint32_t run_lo = work ? p.r_block[p.m_run[lr.first]] : 0u, run_hi = work ? p.r_block[p.m_run[lr.last - 1]] + 1 : 0u;
       const uint32_t len = run_hi - run_lo;
       const uint32_t rank = static_cast<uint32_t>(__popcll(__ballot(work) & lanes_below()));
       const uint32_t items = static_cast<uint32_t>(__popcll(__ballot(work)));
@@ -1134,15 +1135,27 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
 }
 
+// The budget histogram over LOD blocks, in stages per wave as the meshlet pass: a lane per block of the
+// budget items; on affine scenes the block test against the band [T[0], T[B-1]] times the camera scale
+// drops blocks none of whose runs can add to a bin (every run's own error above the band, so its first
+// bin is B, or every parent's at or below it, so its last is 0); the runs of the other blocks are queued
+// per wave in LDS and binned 64 at a time: a run's own and parent errors give the bins [a, e) of the
+// thresholds where its meshlets are in the plain cut, and each visible meshlet of a run with a < e adds
+// its triangles there.
+template <bool kAffine>
 __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
   // Per-wave LDS histogram of the wave's current view, flushed to the view's global histogram when
   // the view changes and at the end; lanes of another view add to global memory directly.
   __shared__ unsigned long long hist[4][kMaxBudgetBins + 1];
+  // Per wave: blocks that passed and the position of each one's first run in the stream of queued runs
+  // (as in the meshlet pass).
+  __shared__ uint4 bq[4][kBlockQueue];
+  __shared__ uint32_t bq_start[4][kBlockQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   const uint32_t bins = p.budget_bins;
   for (uint32_t i = lane; i <= bins; i += 64) hist[wib][i] = 0;
   uint32_t current = 0xffffffffu;
-  auto flush = [&]() {
+  auto flush = [&]() __attribute__((always_inline)) {
     if (current != 0xffffffffu) {
       unsigned long long* g = p.budget_hist + static_cast<size_t>(current) * (bins + 1);
       for (uint32_t i = lane; i <= bins; i += 64) {
@@ -1152,27 +1165,62 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
       }
     }
   };
-  // Per LOD run: its own and parent errors give the bins [a, e) of the thresholds where its meshlets are
-  // in the plain cut; each meshlet of a run with a < e that is visible adds its triangles there.
-  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t run) {
-    const uint32_t vi = w.w >> kTagViewShift;
-    const bool occ_on = (w.w & kTagOcclusion) != 0;
+  uint32_t bq_head = 0, bq_tail = 1, rq_pos = 0, rq_end = 0;
+  if (lane == 0) {
+    bq[wib][0] = make_uint4(0, 0, 0, 0);
+    bq_start[wib][0] = 0;
+  }
+  auto wave_sync = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  // Bins the next n (<= 64) queued runs, one per lane.
+  auto run_stage = [&](uint32_t n) __attribute__((always_inline)) {
+    wave_sync();
+    const uint32_t e_next = bq_head + 1 + lane;
+    const bool present = lane < bq_tail - bq_head - 1;
+    const uint32_t dist = present ? bq_start[wib][e_next & (kBlockQueue - 1)] - rq_pos : 0xffffffffu;
+    const uint64_t bit = dist < 64 ? 1ull << dist : 0ull;
+    const uint64_t starts = (static_cast<uint64_t>(wave_or(static_cast<uint32_t>(bit >> 32))) << 32) | wave_or(static_cast<uint32_t>(bit));
+    const bool valid = lane < n;
+    const uint32_t ent = bq_head + static_cast<uint32_t>(__popcll(starts & ((2ull << lane) - 1ull)));
+    uint4 q = make_uint4(0, 0, 0, 0);
+    uint32_t first_pos = 0;
+    if (valid) {
+      q = bq[wib][ent & (kBlockQueue - 1)];
+      first_pos = bq_start[wib][ent & (kBlockQueue - 1)];
+    }
+    const uint32_t instance = q.x & 0xffffffu, tag = q.z, run = q.y + (rq_pos + lane - first_pos);
+    const float scale = __uint_as_float(q.w);
+    const uint32_t vi = tag >> kTagViewShift;
+    const bool occ_on = (tag & kTagOcclusion) != 0;
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, lod_sphere = c0, parent_sphere = c0;
+    float2 err = make_float2(0.0f, 0.0f);
+    uint2 span = make_uint2(0, 0);
+    if (valid) {
+      c0 = ld32(p.i_col0, instance);
+      c1 = ld32(p.i_col1, instance);
+      c2 = ld32(p.i_col2, instance);
+      c3 = ld32(p.i_col3, instance);
+      err = ld32(p.r_err, run);
+      lod_sphere = ld32(p.r_lod, run);
+      parent_sphere = ld32(p.r_parent, run);
+      span = ld32(p.r_span, run);
+    }
     uint32_t a = 0, e = 0, tris = 0;
     for (uint64_t pending = __ballot(valid); pending;) {  // one launched view at a time (uniform constants)
       const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
       const bool mine = valid && vi == vu;
       if (mine) {
         const auto& v = scalar(p.views)[uniform(vu)];
-        const float4 c0 = p.i_col0[w.x], c1 = p.i_col1[w.x], c2 = p.i_col2[w.x], c3 = p.i_col3[w.x];
-        const float scale = p.work_scale[item];
-        const float2 err = p.r_err[run];
-        const float own = projected_error(v, c0, c1, c2, c3, scale, p.r_lod[run], err.x);
-        const float parent = projected_error(v, c0, c1, c2, c3, scale, p.r_parent[run], err.y);
+        const float own = projected_error<kAffine>(v, c0, c1, c2, c3, scale, lod_sphere, err.x);
+        const float parent = projected_error<kAffine>(v, c0, c1, c2, c3, scale, parent_sphere, err.y);
         const float* t = p.budget_thresholds + static_cast<size_t>(uniform(vu)) * bins;
         a = bin_of(t, bins, v.budget_scale, v.budget_inv, own);
         e = bin_of(t, bins, v.budget_scale, v.budget_inv, parent);
         if (a < e) {
-          const uint2 span = p.r_span[run];
           for (uint32_t k = 0; k < span.y; k++) {
             const uint32_t meshlet = span.x + k;
             const float4 cs = p.m_cull[meshlet];
@@ -1189,6 +1237,11 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
       }
       pending &= ~__ballot(mine);
     }
+    // Entries whose runs are all binned now: the new head is the last one starting at or before rq_pos + n.
+    const uint64_t upto_n = n >= 64 ? ~0ull : ((2ull << n) - 1ull);
+    const bool at_end = __ballot(dist == 64) != 0 && n >= 64;
+    bq_head += static_cast<uint32_t>(__popcll(starts & upto_n)) + (at_end ? 1u : 0u);
+    rq_pos += n;
     const bool add = tris != 0;
     // Switch the wave's LDS histogram to the chunk's first contributing view when needed.
     const uint64_t contrib = __ballot(add);
@@ -1209,7 +1262,57 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
         atomicAdd(&g[e], static_cast<unsigned long long>(0) - tris);
       }
     }
+  };
+
+  for_each_task(p, [&](bool valid, uint32_t item, const uint4& w, uint32_t task) __attribute__((always_inline)) {
+    if (!__ballot(valid)) return;
+    uint2 bspan = make_uint2(0, 0);
+    float scale = 0.0f;
+    if (valid) {
+      bspan = ld32(p.b_span, task);
+      scale = p.work_scale[item];
+    }
+    bool keep = valid;
+    if (kAffine) {
+      float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, blod = c0, bpar = c0, berr = c0;
+      if (valid) {
+        c0 = ld32(p.i_col0, w.x);
+        c1 = ld32(p.i_col1, w.x);
+        c2 = ld32(p.i_col2, w.x);
+        c3 = ld32(p.i_col3, w.x);
+        blod = ld32(p.b_lod, task);
+        bpar = ld32(p.b_par, task);
+        berr = ld32(p.b_err, task);
+      }
+      const BlockBounds ib = instance_bounds(c0, c1, c2, c3, scale);
+      const uint32_t vi = w.w >> kTagViewShift;
+      for (uint64_t pending = __ballot(valid); pending;) {
+        const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+        const bool mine = valid && vi == vu;
+        if (mine) {
+          const auto& v = scalar(p.views)[uniform(vu)];
+          keep = !block_out_of_band(v, c0, c1, c2, c3, ib, blod, bpar, berr, v.budget_low, v.budget_high);
+        }
+        pending &= ~__ballot(mine);
+      }
+    }
+    // Queue the blocks that remain (at most 64 entries and 63 + 64 * kBlockRuns runs pending after this).
+    const uint64_t kept = __ballot(keep);
+    if (kept) {
+      const uint32_t nruns = keep ? bspan.y : 0u;
+      const uint32_t runs_incl = wave_scan(nruns);
+      const uint32_t slot = bq_tail + static_cast<uint32_t>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(kept >> 32),
+                                                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(kept), 0u)));
+      if (keep) {
+        bq[wib][slot & (kBlockQueue - 1)] = make_uint4(w.x | (nruns << 24), bspan.x, w.w, __float_as_uint(scale));
+        bq_start[wib][slot & (kBlockQueue - 1)] = rq_end + runs_incl - nruns;
+      }
+      bq_tail += static_cast<uint32_t>(__popcll(kept));
+      rq_end += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(runs_incl), 63));
+      while (rq_end - rq_pos >= 64) run_stage(64);
+    }
   });
+  if (rq_end != rq_pos) run_stage(rq_end - rq_pos);
   flush();
 }
 
