@@ -118,7 +118,7 @@ def markdown(summary: dict, title: str, commands: List[str]) -> str:
             lines.append(f"| {name} | {k['dispatches']} ({k['busy_dispatches']}) | {k['avg_us']:.1f} | - | - | - |")
     counters = sorted({c for k in summary["kernels"].values() for c in k.get("counters", {})})
     if counters:
-        lines += ["", "Instruction mix per busy dispatch (SQ counters, summed over the device):", "",
+        lines += ["", "Counters per busy dispatch (summed over the device; SQ_* per wave instruction or cycle, TA_/TCP_ summed over CUs):", "",
                   "| kernel | " + " | ".join(counters) + " |", "| --- |" + " --- |" * len(counters)]
         for name, k in sorted(summary["kernels"].items(), key=lambda kv: -kv[1]["total_ms"]):
             if "counters" in k:
