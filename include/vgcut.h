@@ -138,9 +138,19 @@ int vg_mesh_add(vg_scene* scene, const vg_mesh_desc* mesh, uint32_t* slot);
  * Instances of a mesh (GeometryPool.writeInstances + computeCell, GeometryPool.ts:484-545; Morton order
  * VirtualMesh.ts:171-174, GeometryPool.ts:1263-1311). `matrices`: 16 f32 per instance; `colors`: 4 f32
  * per instance or NULL. `gpu_to_input` (n entries, may be NULL) receives the caller instance of each slot.
+ * A new count moves the mesh to a fresh instance range; the columns are repacked (every mesh's range moves,
+ * in slot order) once more than half of them would be dead.
  */
 int vg_instances_set(vg_scene* scene, uint32_t slot, const float* matrices, const float* colors, uint32_t count,
                      uint32_t* gpu_to_input);
+/*
+ * GeometryPool.remove (GeometryPool.ts:357-363), as VirtualGeometry.unregister calls it (VirtualGeometry.ts:370-376):
+ * the slot is disabled and freed for the next vg_mesh_add. Its instances and cells are dropped (the reference
+ * leaves its cells pointing at the disabled slot, which a later add into the slot would enable again); the
+ * instance columns are repacked once more than half of them are dead. Its geometry stays in the pool until
+ * the scene is destroyed, as the reference keeps it until a repack it does not implement.
+ */
+int vg_mesh_remove(vg_scene* scene, uint32_t slot);
 /* GeometryPool.writeFlags (GeometryPool.ts:463-477). max_draw_distance >= 1e30: none. */
 int vg_mesh_flags(vg_scene* scene, uint32_t slot, float max_draw_distance, float min_pixel_radius, uint32_t enabled,
                   uint32_t shadow, uint32_t occlusion);

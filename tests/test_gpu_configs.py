@@ -7,12 +7,15 @@
   first, so the requested-triangle counts and the retry's scale depend on the append order (in the
   reference too);
 - the 3M-instance world with its terrain as a mesh (config 2): every fourth frame of the 16-frame seeded fly-through
-  (4 frames; the oracle's budget histogram takes about 20 s per frame) at 1080p with an 8M-triangle budget
-  (histogram pick; this world's cut exceeds 8M triangles even at 3 px, so the pick is clamped to the last
-  threshold of thresholdRange, as the reference clamps) and two-phase occlusion against the previous
-  frame's visible set: the first phase's HZB is the occluder raster of the previous frame's cut (both
-  phases) with this frame's camera, the second phase's the raster of the first phase's clusters, each
-  HZB equal to the oracle's raster of the oracle's own cut; the controller state carried over.
+  (4 frames; the oracle's budget histogram takes about 20 s per frame; bench.py --plain-configs times all
+  16) at 1080p with an 8M-triangle budget (histogram pick; this world's cut exceeds 8M triangles even at
+  3 px, so the pick is clamped to the last threshold of thresholdRange, as the reference clamps) and
+  two-phase occlusion against the previous frame's visible set: the first phase's HZB is the occluder
+  raster of the previous frame's cut (both phases) with this frame's camera, the second phase's the raster
+  of the first phase's clusters, each HZB equal to the oracle's raster of the oracle's own cut; the
+  controller state carried over; the second phases must recover clusters. Then frame 0's camera again
+  with a budget of the plain cut's triangles at the middle threshold, so the histogram's pick lands inside
+  the table at full size;
 
 - the same world cut for its 64 mixed views (config 3: main cameras at 1080p and 2160p with terrain
   occlusion, orthographic shadow cascades, cube-map faces, 720p viewers) in one graph-replayed launch,
@@ -111,6 +114,7 @@ def test_world_flythrough_budget_two_phase(gpu_available, world_spec):
     prev_draws, prev_n = None, 0
     added = hidden = 0
     w, h = 960, 540
+    first = None  # frame 0's view and histogram, for the interior pick below
     for f in range(0, 16, 4):
         eye, target = path[f]
         camera = cam.perspective_camera(60, 16 / 9, 0.1, 20000).look_at(eye, target)
@@ -125,7 +129,9 @@ def test_world_flythrough_budget_two_phase(gpu_available, world_spec):
             hzb1 = oracle.hzb_build(oracle.raster(pool, prev_draws, prev_n, vp, w, h), 1024)
             assert np.array_equal(scene.hzb(0)[:len(hzb1)], hzb1), f"frame {f}: first-phase HZB differs"
         view = make_view(camera, 1080, occlusion=2, triangle_budget=budget, threshold_range=(0.5, 3.0))
-        b, _, chosen = oracle.budget_pick(pool, view, budget, thresholds, state=state, hzb=hzb1)
+        b, tris, chosen = oracle.budget_pick(pool, view, budget, thresholds, state=state, hzb=hzb1)
+        if first is None:
+            first = (camera, tris)
         ref1 = oracle.cut(pool, chosen, state=state, hzb=hzb1)
         print(f"world frame {f}: budget bin {b}", flush=True)
         scene.cut([view])
@@ -152,7 +158,30 @@ def test_world_flythrough_budget_two_phase(gpu_available, world_spec):
         hidden += int(ref1.counters[1])
         state = ref.first.state
         prev_draws, prev_n = ref.draws[0], n2
+    print(f"world fly-through: {hidden} instances hidden by the first phases, {added} clusters added by the second", flush=True)
     assert hidden > 0, "the previous frames' visible sets must hide something"
+    assert added > 0, "the second phases must recover clusters the moving camera uncovers"
+
+    # An interior pick at full size: frame 0's camera from the controller at rest with a budget of the plain
+    # cut's triangles at the middle threshold, so the pick lands strictly inside the table.
+    camera, tris = first
+    bins = len(thresholds)
+    mid = int(tris[bins // 2])
+    expect = int(np.nonzero(tris <= np.uint64(mid))[0][0])
+    assert 0 < expect < bins - 1 and tris[0] > mid, (expect, tris[0], mid)
+    far = np.ones((h, w), dtype=np.float32)
+    scene.set_hzb(0, far)
+    scene.set_state(0, oracle.initial_state())
+    view = make_view(camera, 1080, occlusion=1, triangle_budget=mid, threshold_range=(0.5, 3.0))
+    b, _, chosen = oracle.budget_pick(pool, view, mid, thresholds, state=oracle.initial_state(), hzb=oracle.hzb_build(far, 1024))
+    assert b == expect
+    ref = oracle.cut(pool, chosen, state=oracle.initial_state(), hzb=oracle.hzb_build(far, 1024))
+    scene.cut([view])
+    scene.sync()
+    gpu = scene.read(0)
+    assert gpu[3].budget_bin == b, (gpu[3].budget_bin, b)
+    assert not state_order_dependent(ref), "the interior pick's draw list overflowed"
+    compare_cut(gpu, ref, mm, "world frame 0 interior budget pick")
 
 
 def compact(counters, args, draws, metrics, mm):

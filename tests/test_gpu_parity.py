@@ -323,3 +323,48 @@ def test_graph_replay_after_instance_changes(gpu_available):
     scene.set_instances(1, more)
     drawn.append(check("mesh 1 with 140 more instances"))
     assert drawn[0] != drawn[1] and drawn[1] != drawn[2], drawn
+
+
+def test_cut_after_mesh_removals(gpu_available):
+    """VirtualGeometry.unregister (GeometryPool.remove): after a mesh is removed, its slot refilled by another
+    mesh and instance ranges repacked, graph and eager cuts equal the oracle's cut of the pool as it stands,
+    and the removed mesh's instances are no longer drawn."""
+    from vgcut import scenes
+
+    spec = small_ruins(seed=59, instances=600, meshes=3)
+    loaded = load_gpu(spec)
+    scene = loaded.scene
+    view = make_view(ground_camera((-25.0, 6.0, -30.0), (10.0, 0.0, 25.0)), 1080)
+    rng = np.random.default_rng(59)
+
+    def check(label):
+        scene.set_state(0, oracle.initial_state())
+        scene.cut([view], graph=True)
+        scene.sync()
+        graph = scene.read(0)
+        scene.set_state(0, oracle.initial_state())
+        scene.cut([view])
+        scene.sync()
+        eager = scene.read(0)
+        pool = oracle.pool_from_scene(scene, loaded.ctx.config.max_meshlets, 2_000_000, 1_000_000)
+        ref = oracle.cut(pool, view)
+        compare_cut(graph, ref, loaded.ctx.config.max_meshlets, f"{label}: graph")
+        compare_cut(eager, ref, loaded.ctx.config.max_meshlets, f"{label}: eager")
+        return graph
+
+    check("as loaded")
+    scene.remove_mesh(1)
+    after = check("mesh 1 removed")
+    n = int(after[0][2])
+    slots = scene.pool(native.VG_POOL_INSTANCES).reshape(-1, 20)[:, 19]
+    assert not np.any(slots[after[2][0][:n, 0]] == 1), "a removed mesh's instance was drawn"
+    slot = scene.add_mesh(spec.meshes[2])
+    assert slot == 1
+    m = 500
+    mats = scenes.yaw_scale_translate(rng.uniform(0, 6.28, m), rng.uniform(0.5, 2.0, m),
+                                      np.stack([rng.uniform(-30, 30, m), np.zeros(m), rng.uniform(-30, 30, m)], 1))
+    scene.set_instances(slot, mats)
+    check("slot 1 refilled")
+    scene.remove_mesh(0)
+    scene.remove_mesh(2)
+    check("only the refilled slot left")

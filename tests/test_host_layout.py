@@ -97,3 +97,48 @@ def test_invalid_inputs_are_rejected():
     with pytest.raises(native.VgError) as e:
         scene.cut([native.View()])
     assert e.value.code == native.VG_E_DEVICE  # host-only context: no cut
+
+
+def test_mesh_remove_reuses_slots_and_repacks_instances():
+    """100 add / set-instances / remove cycles on a 4-slot pool (GeometryPool.add takes the first free slot,
+    GeometryPool.ts:303-304; remove frees it, :357-363): every add lands in the lowest free slot, the cells
+    are exactly the live meshes' cells (in slot order, over their current instance ranges), each live mesh's
+    instances sit at its range in Morton order with its slot, and the instance columns stay within twice the
+    live instances (dead ranges are repacked away). A removed slot refuses further instance and flag calls."""
+    rng = np.random.default_rng(17)
+    kinds = [scenes.prop_mesh(rng, 1_500, 300 + k, kind=k % 3) for k in range(3)]
+    ctx = native.Context(device=-1, mesh_slots=4)
+    scene = native.Scene(ctx)
+    live = {}  # slot -> (mesh data, matrices in GPU order)
+    for cycle in range(100):
+        if len(live) == 4 or (live and rng.random() < 0.4):
+            slot = int(rng.choice(sorted(live)))
+            scene.remove_mesh(slot)
+            del live[slot]
+            with pytest.raises(native.VgError):
+                scene.set_flags(slot, 1e30, 0.0, True, True, False)
+            continue
+        data = kinds[cycle % 3]
+        slot = scene.add_mesh(data)
+        assert slot == min(set(range(4)) - set(live)), (cycle, slot, sorted(live))
+        n = int(rng.integers(1, 400))
+        m = scenes.yaw_scale_translate(rng.uniform(0, 6.3, n), rng.uniform(0.5, 2, n), rng.uniform(-200, 200, (n, 3)))
+        order = scene.set_instances(slot, m)
+        if rng.random() < 0.3:  # a second call with a new count moves the mesh to a fresh range
+            n = int(rng.integers(1, 400))
+            m = scenes.yaw_scale_translate(rng.uniform(0, 6.3, n), rng.uniform(0.5, 2, n), rng.uniform(-200, 200, (n, 3)))
+            order = scene.set_instances(slot, m)
+        live[slot] = (data, m[order])
+        inst = scene.pool(native.VG_POOL_INSTANCES).reshape(-1, 20)
+        total_live = sum(len(g) for _, g in live.values())
+        assert len(inst) <= 2 * total_live + 400, (cycle, len(inst), total_live)
+        want = []
+        for s in sorted(live):
+            d, g = live[s]
+            base, count = scene.instance_base(s)
+            assert count == len(g)
+            assert np.array_equal(inst[base:base + count, :16], g) and np.all(inst[base:base + count, 19] == s)
+            want.append(pool_ref.compute_cells(g, d.boundingSphere, s, base))
+        assert np.array_equal(scene.pool(native.VG_POOL_CELLS), np.concatenate(want)), cycle
+    scene.close()
+    ctx.close()

@@ -223,3 +223,50 @@ def test_shortest_first_and_fifo_order(gpu_available):
             assert time.time() - t0 < 30
         assert (queued[tc] < queued[tb]) == light_first, (policy, queued[ta], queued[tb], queued[tc])
         server.close()
+
+
+def test_full_size_reruns_wait_for_hbm_budget(gpu_available):
+    """A batch whose optimistic work list overflows runs again at full size only when that fits the HBM
+    budget: two overflowing requests whose full-size transients do not fit together run one after the
+    other (same results as alone), and one whose full size cannot fit even alone completes with
+    VG_E_NOMEM instead of failing an allocation."""
+    specs = [small_ruins(seed=97, instances=2500, meshes=3), small_ruins(seed=98, instances=2500, meshes=3)]
+    ctx = native.Context(max_triangles=2_000_000, max_shadow_triangles=1_000_000, max_meshlets=MAX_MESHLETS, max_views=4, device=0)
+    resident = make_scenes(ctx, specs)
+    rng = np.random.default_rng(9)
+    reqs = [request_views(rng, 4), request_views(rng, 4)]
+    # Alone, with room to spare: each request's full-size transient bytes (the peak after its rerun).
+    alone, full = [], []
+    for k in range(2):
+        probe = native.Server(ctx, lanes=1, max_batch_views=4, first_work_items=64)
+        ids = [probe.add_scene(s) for s in resident]
+        t = probe.submit(ids[k], reqs[k])
+        assert wait_all(probe, [t])[t][0] == native.VG_OK
+        alone.append(results(probe, t, 4))
+        st = probe.stats()
+        assert st.reruns == 1
+        full.append(int(st.peak_inflight_bytes))
+        base = int(st.resident_bytes)
+        probe.close()
+    # Room for either full-size rerun, not both.
+    budget = base + max(full) + min(full) // 2
+    server = native.Server(ctx, hbm_budget=budget, lanes=2, max_batch_views=4, first_work_items=64)
+    ids = [server.add_scene(s) for s in resident]
+    tickets = [server.submit(ids[k], reqs[k]) for k in range(2)]
+    done = wait_all(server, tickets)
+    for k, t in enumerate(tickets):
+        assert done[t][0] == native.VG_OK
+        diff = differences(alone[k], results(server, t, 4))
+        assert not diff, f"request {k} after a deferred rerun: {diff}"
+        server.release(t)
+    st = server.stats()
+    assert st.reruns == 2 and st.resident_bytes + st.peak_inflight_bytes <= budget
+    server.close()
+    # No room for the full size at all.
+    server = native.Server(ctx, hbm_budget=base + min(full) - 1, lanes=1, max_batch_views=4, first_work_items=64)
+    ids = [server.add_scene(s) for s in resident]
+    k = int(np.argmin(full))
+    t = server.submit(ids[k], reqs[k])
+    assert wait_all(server, [t])[t][0] == native.VG_E_NOMEM
+    server.release(t)
+    server.close()

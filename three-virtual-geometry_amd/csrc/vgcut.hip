@@ -1084,6 +1084,64 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
   return VG_OK;
 }
 
+namespace {
+
+// Instances of the used meshes other than `skip`.
+size_t live_instances(const vg_scene* s, uint32_t skip) {
+  size_t live = 0;
+  for (uint32_t k = 0; k < s->slots; k++)
+    if (s->meshes[k].used && k != skip) live += s->meshes[k].instance_count;
+  return live;
+}
+
+// Moves every used mesh's instance range (but `skip`'s, which is dropped) to the front of the columns, in slot
+// order, and rebases its cells' first instances (exact: instance indices are below 2^24).
+void repack_instances(vg_scene* s, uint32_t skip) {
+  std::vector<float4> col[4];
+  std::vector<float> islot;
+  const size_t live = live_instances(s, skip);
+  for (int k = 0; k < 4; k++) col[k].reserve(live);
+  islot.reserve(live);
+  for (uint32_t slot = 0; slot < s->slots; slot++) {
+    MeshHost& m = s->meshes[slot];
+    if (!m.used || slot == skip) continue;
+    const uint32_t base = static_cast<uint32_t>(islot.size());
+    for (int k = 0; k < 4; k++) col[k].insert(col[k].end(), s->i_col[k].begin() + m.instance_base, s->i_col[k].begin() + m.instance_base + m.instance_count);
+    islot.insert(islot.end(), m.instance_count, static_cast<float>(slot));
+    for (float4& rec : m.cell_rec) rec.z = static_cast<float>(static_cast<uint32_t>(rec.z) - m.instance_base + base);
+    m.instance_base = base;
+  }
+  for (int k = 0; k < 4; k++) s->i_col[k].swap(col[k]);
+  s->i_slot.swap(islot);
+  if (skip < s->slots) {
+    s->meshes[skip].instance_base = 0;
+    s->meshes[skip].instance_count = 0;
+    s->meshes[skip].cell_sphere.clear();
+    s->meshes[skip].cell_rec.clear();
+  }
+  s->dirty_instances = s->dirty_cells = true;
+}
+
+}  // namespace
+
+int vg_mesh_remove(vg_scene* s, uint32_t slot) {
+  if (!s || slot >= s->slots || !s->meshes[slot].used) return VG_E_INVALID;
+  std::fill(s->phase1_ready.begin(), s->phase1_ready.end(), 0);  // instance slots may move
+  MeshHost& m = s->meshes[slot];
+  const size_t live = live_instances(s, slot);
+  if (s->i_col[0].size() > 2 * live) {
+    repack_instances(s, slot);
+  } else {  // its range stays dead until a later repack
+    m.instance_count = 0;
+    m.cell_sphere.clear();
+    m.cell_rec.clear();
+  }
+  m.used = false;
+  s->mesh_table[slot * kMeshStride + 5] = make_float4(0, 0, 0, 0);  // writeFlags({enabled: 0})
+  s->dirty_table = s->dirty_cells = true;
+  return VG_OK;
+}
+
 int vg_instances_set(vg_scene* s, uint32_t slot, const float* matrices, const float* colors, uint32_t count,
                      uint32_t* gpu_to_input) {
   (void)colors;  // tints are raster-side data (GeometryPool.ts:490-491); the cut does not read them
@@ -1091,8 +1149,11 @@ int vg_instances_set(vg_scene* s, uint32_t slot, const float* matrices, const fl
   std::fill(s->phase1_ready.begin(), s->phase1_ready.end(), 0);  // instance slots move: recorded rejects no longer apply
   vg_ctx* ctx = s->ctx;
   MeshHost& m = s->meshes[slot];
-  const size_t total = s->i_col[0].size();
   if (count != m.instance_count || m.instance_count == 0) {
+    // The mesh's old range dies; repack first when the dead part would pass half of the columns.
+    const size_t live = live_instances(s, slot);
+    if (s->i_col[0].size() + count > 2 * (live + count) || s->i_col[0].size() + count > kMaxInstancesPerScene) repack_instances(s, slot);
+    const size_t total = s->i_col[0].size();
     if (total + count > kMaxInstancesPerScene) return fail(ctx, VG_E_CAPACITY, "vg_instances_set: more than 2^24 instances in a scene");
     m.instance_base = static_cast<uint32_t>(total);
     for (int k = 0; k < 4; k++) s->i_col[k].resize(total + count);

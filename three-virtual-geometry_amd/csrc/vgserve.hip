@@ -103,6 +103,7 @@ struct vg_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
   std::deque<uint64_t> completed;
   std::vector<Batch> running;
+  std::deque<Batch> deferred;  // reruns at full size waiting for HBM budget (they keep their view slots)
   uint64_t resident = 0, inflight = 0, peak_inflight = 0, batches = 0, requests_done = 0, max_batch_seen = 0, reruns = 0;
   uint64_t next_ticket = 1;
   uint64_t generation = 0;  // bumped by submit and release: the worker waits for it when it cannot progress
@@ -144,7 +145,8 @@ struct vg_server {
         }
       }
       if (status == VG_E_CAPACITY && !b.full) {
-        // The optimistic work list overflowed: run the same batch (same slots) again at full size.
+        // The optimistic work list overflowed: run the same batch (same slots) again at full size, on this
+        // lane now if its full-size transients fit the budget, else once they do (before new batches).
         Batch again;
         again.scene = b.scene;
         again.views = b.views;
@@ -156,7 +158,8 @@ struct vg_server {
         for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
         lane_batch[lane] = -1;
         reruns++;
-        launch(again, static_cast<int>(lane));
+        if (fits_now(again)) launch(again, static_cast<int>(lane));
+        else deferred.push_back(std::move(again));
         continue;
       }
       const double t = now_us();
@@ -172,6 +175,39 @@ struct vg_server {
       running.erase(running.begin() + static_cast<std::ptrdiff_t>(i));
       for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
       cv_done.notify_all();
+    }
+  }
+
+  // Whether a batch's transients fit the HBM budget next to the resident scenes and the batches in flight.
+  bool fits_now(const Batch& b) const {
+    return resident + inflight + transient_bytes(scenes[b.scene], b.views, b.full, cfg.first_work_items) <= cfg.hbm_budget;
+  }
+
+  // Launches deferred reruns that fit now, oldest first; one that cannot fit even with nothing in flight
+  // fails its requests with VG_E_NOMEM. With the mutex held.
+  void launch_deferred() {
+    while (!deferred.empty()) {
+      const int lane = free_lane();
+      if (lane < 0) return;
+      Batch& b = deferred.front();
+      if (!fits_now(b)) {
+        if (!running.empty()) return;  // wait for memory in flight
+        set_error("vg_server: a request's full-size work list does not fit the HBM budget");
+        const double t = now_us();
+        for (Request* r : b.requests) {
+          r->status = VG_E_NOMEM;
+          r->done = true;
+          r->t_done = t;
+          completed.push_back(r->ticket);
+          requests_done++;
+        }
+        deferred.pop_front();
+        cv_done.notify_all();
+        continue;
+      }
+      Batch next = std::move(b);
+      deferred.pop_front();
+      launch(next, lane);
     }
   }
 
@@ -349,8 +385,10 @@ struct vg_server {
     uint64_t seen = generation;
     while (!stop) {
       complete();
-      bool launched = false;
-      for (int lane = free_lane(); lane >= 0 && !queue.empty(); lane = free_lane()) {
+      const size_t waiting = deferred.size();
+      launch_deferred();
+      bool launched = deferred.size() != waiting;
+      for (int lane = free_lane(); lane >= 0 && deferred.empty() && !queue.empty(); lane = free_lane()) {
         Batch b;
         if (!pick(b)) break;
         launch(b, lane);

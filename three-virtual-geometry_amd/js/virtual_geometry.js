@@ -437,6 +437,15 @@ class VirtualGeometry {
     return mesh;
   }
 
+  /** VirtualGeometry.unregister (VirtualGeometry.ts:370-376): the mesh's pool slot is freed for later meshes. */
+  unregister(mesh) {
+    const i = this.meshes.indexOf(mesh);
+    if (i < 0) return;
+    this.addon.removeMesh(this.scene, mesh.slot);
+    this.meshes.splice(i, 1);
+    this.cutDirty = true;
+  }
+
   /** The depth of last frame's occluders for a view slot (OcclusionCulling.beginFrame + the HZB build). */
   setHzb(depth, width, height, slot) {
     this.addon.setHzb(this.scene, slot || 0, depth, width, height);

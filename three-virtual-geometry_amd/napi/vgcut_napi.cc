@@ -275,6 +275,19 @@ static napi_value SetFlags(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// removeMesh(scene, slot) (GeometryPool.remove, as VirtualGeometry.unregister calls it): the slot is freed
+static napi_value RemoveMesh(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  uint32_t slot = 0;
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
+  CHECK_NAPI(napi_get_value_uint32(env, argv[1], &slot));
+  if (int rc = vg_mesh_remove(s->scene, slot)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
 // setHzb(scene, viewSlot, depth: Float32Array, width, height): the slot's HZB from a depth image (row 0 at
 // the top), as OcclusionCulling's copy and reduce nodes build it
 static napi_value SetHzb(napi_env env, napi_callback_info info) {
@@ -629,6 +642,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"addMesh", nullptr, AddMesh, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setInstances", nullptr, SetInstances, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setFlags", nullptr, SetFlags, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"removeMesh", nullptr, RemoveMesh, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cut", nullptr, Cut, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"read", nullptr, Read, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"poolRead", nullptr, PoolRead, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

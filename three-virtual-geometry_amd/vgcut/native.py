@@ -35,7 +35,7 @@ LIBRARY = _native.library_path("libvgcut.so")
 # Every entry point include/vgcut.h declares (the ABI test checks the library exports them).
 EXPORTS = (
     "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
-    "vg_instances_set", "vg_mesh_flags", "vg_hzb_set", "vg_hzb_from_cut", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
+    "vg_instances_set", "vg_mesh_flags", "vg_mesh_remove", "vg_hzb_set", "vg_hzb_from_cut", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
@@ -196,6 +196,7 @@ def load():
         "vg_mesh_add": ([P, ctypes.POINTER(MeshDesc), ctypes.POINTER(U32)], ctypes.c_int),
         "vg_instances_set": ([P, U32, P, P, U32, P], ctypes.c_int),
         "vg_mesh_flags": ([P, U32, ctypes.c_float, ctypes.c_float, U32, U32, U32], ctypes.c_int),
+        "vg_mesh_remove": ([P, U32], ctypes.c_int),
         "vg_hzb_set": ([P, U32, P, U32, U32], ctypes.c_int),
         "vg_hzb_from_cut": ([P, U32, P, U32, U32, U32], ctypes.c_int),
         "vg_hzb_invalidate": ([P, U32], ctypes.c_int),
@@ -341,6 +342,10 @@ class Scene:
         order = np.empty(len(m), dtype=np.uint32)
         self.ctx.check(self.lib.vg_instances_set(self.handle, slot, _ptr(m), _ptr(c), len(m), _ptr(order)), "vg_instances_set")
         return order
+
+    def remove_mesh(self, slot: int):
+        """GeometryPool.remove (vg_mesh_remove): frees the slot; its instances and cells are dropped."""
+        self.ctx.check(self.lib.vg_mesh_remove(self.handle, slot), "vg_mesh_remove")
 
     def set_flags(self, slot: int, max_draw_distance=1e30, min_pixel_radius=0.0, enabled=True, shadow=True, occlusion=False):
         self.ctx.check(self.lib.vg_mesh_flags(self.handle, slot, float(max_draw_distance), float(min_pixel_radius), int(bool(enabled)),
