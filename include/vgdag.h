@@ -6,14 +6,15 @@
  *
  * Per level: partition the current meshlets into groups of ~group_size that share boundary edges,
  * merge each group, simplify it to simplify_ratio of its triangles with the group border locked
- * (quadric half-edge collapses onto existing vertices, so every level indexes the input vertex
- * buffer), split the result back into meshlets of <= 128 triangles and <= 128 vertices. Groups that
- * keep more than max_group_keep_ratio of their triangles stay roots. Once the level is small
- * (<= 1024 triangles) or stalls, the tail merges every root and keeps halving with borders free.
- *
- * Not built: the reference's voxel proxies and piece-by-piece aggregate simplification
- * (voxelProxy.ts, aggregate.ts; options voxelLods, aggregateLods), which serve foliage made of
- * separate small pieces. The builder behaves as the reference does with both switched off.
+ * (quadric half-edge collapses onto existing vertices), split the result back into meshlets of
+ * <= 128 triangles and <= 128 vertices. A group that stalls (separate pieces such as leaves) is
+ * reduced piece by piece instead (aggregate_lods: pieces simplified on their own, neighbouring
+ * pieces thinned in pairs into widened copies; aggregate.ts). Groups that keep more than
+ * max_group_keep_ratio of their triangles stay roots. Once the level is small (<= 1024 triangles)
+ * or stalls, the tail merges every root and keeps halving with borders free; with voxel_lods a
+ * simplified voxel proxy of the full-detail mesh (surface nets over its closed occupancy;
+ * voxelProxy.ts) replaces a tail level when it is more accurate. Thinned pieces and voxel levels
+ * add vertices after the input ones; their attributes are returned with the positions.
  */
 #pragma once
 #include <stdint.h>
@@ -33,14 +34,17 @@ typedef struct vgd_options {
   int32_t tail_simplify;        /* keep halving below one meshlet once the roots cover the mesh (1) */
   uint32_t min_root_triangles;  /* the tail stops at this many triangles (4) */
   uint32_t threads;             /* worker threads for the groups of a level; 0: hardware concurrency */
+  int32_t voxel_lods;           /* voxel proxies may replace tail levels (1) */
+  uint32_t voxel_resolution;    /* finest proxy grid, cells along the longest side (64) */
+  int32_t aggregate_lods;       /* stalled groups are reduced piece by piece (1) */
 } vgd_options;
 
 void vgd_default_options(vgd_options* options);
 
-/* The built mesh: buildVirtualMesh.ts:75-138 VirtualMeshData (normals, colors, uvs are the caller's). */
+/* The built mesh: buildVirtualMesh.ts:75-138 VirtualMeshData. */
 typedef struct vgd_mesh_view {
   uint32_t vertex_count;
-  const float* positions;            /* 3 per vertex: the input positions */
+  const float* positions;            /* 3 per vertex: the input positions, then the vertices levels added */
   uint64_t index_count;
   const uint32_t* indices;           /* concatenated meshlet triangle lists, indexing positions */
   uint32_t meshlet_count;
@@ -65,7 +69,23 @@ typedef struct vgd_mesh_view {
   uint32_t root_triangles;
   uint32_t root_meshlets;
   uint32_t lod_levels;               /* stats.lodLevels */
+  const float* normals;              /* 3 per vertex, NULL when the source had none */
+  const float* colors;               /* 3 per vertex, NULL when the source had none */
+  const float* uvs;                  /* 2 per vertex, NULL when the source had none */
+  uint32_t voxel_levels;             /* stats.voxelLevels */
+  uint32_t aggregate_groups;         /* stats.aggregateGroups */
 } vgd_mesh_view;
+
+/* buildVirtualMesh.ts VirtualMeshSource: normals, colors and uvs may be NULL. */
+typedef struct vgd_source {
+  uint32_t vertex_count;
+  const float* positions; /* 3 per vertex */
+  const float* normals;   /* 3 per vertex */
+  const float* colors;    /* 3 per vertex */
+  const float* uvs;       /* 2 per vertex */
+  uint64_t index_count;
+  const uint32_t* indices;
+} vgd_source;
 
 typedef struct vgd_mesh vgd_mesh;
 
@@ -78,8 +98,32 @@ typedef struct vgd_mesh vgd_mesh;
  */
 int vgd_build(const float* positions, uint32_t vertex_count, const uint32_t* indices, uint64_t index_count,
               const vgd_options* options, vgd_mesh** out);
+/* The same over a source with vertex attributes (buildVirtualMesh(src, options) with src.normals,
+ * src.colors, src.uvs): the attributes of added vertices are copied from the piece they widen or
+ * computed by the voxel proxy (normal from the field's gradient, mean colour, first sampled UV). */
+int vgd_build_source(const vgd_source* source, const vgd_options* options, vgd_mesh** out);
 void vgd_mesh_get(const vgd_mesh* mesh, vgd_mesh_view* view);
 void vgd_mesh_free(vgd_mesh* mesh);
+
+/* A voxel proxy (voxelProxy.ts:33 buildVoxelProxy, which the reference's tests call directly,
+ * tests/import.test.ts:63-70): the closed surface-nets surface around the voxelized triangles at
+ * `resolution` cells along the longest side. colors / uvs may be NULL. Returns 0, 1 when there is no
+ * proxy (no triangles, a flat extent or an empty surface; *out stays NULL), or -1 for invalid input. */
+typedef struct vgd_proxy vgd_proxy;
+typedef struct vgd_proxy_view {
+  uint32_t vertex_count;
+  const float* positions; /* 3 per vertex */
+  const float* normals;   /* 3 per vertex */
+  const float* colors;    /* 3 per vertex, NULL without source colors */
+  const float* uvs;       /* 2 per vertex, NULL without source uvs */
+  uint64_t index_count;
+  const uint32_t* indices;
+  double cell_size;
+} vgd_proxy_view;
+int vgd_voxel_proxy(const float* positions, uint32_t vertex_count, const float* colors, const float* uvs,
+                    const uint32_t* indices, uint64_t index_count, uint32_t resolution, vgd_proxy** out);
+void vgd_proxy_get(const vgd_proxy* proxy, vgd_proxy_view* view);
+void vgd_proxy_free(vgd_proxy* proxy);
 
 /*
  * Replaces partitionMeshlets(meshlets, groupSize) (partition.ts:16-97), exported by the reference's

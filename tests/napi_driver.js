@@ -20,12 +20,22 @@ for (const m of spec.meshes) {
   let data;
   if (m.source) {
     // A raw triangle mesh: the addon's DAG builder makes the VirtualMeshData, as the host's preprocess would.
-    data = vg.buildVirtualMesh(fromHex(m.source.positions, Float32Array), fromHex(m.source.indices, Uint32Array), m.source.options || {});
+    const positions = fromHex(m.source.positions, Float32Array);
+    const indices = fromHex(m.source.indices, Uint32Array);
+    if (m.source.normals || m.source.uvs) {
+      // buildVirtualMesh(src, options) with vertex attributes (VirtualMeshSource)
+      const src = { positions, indices };
+      if (m.source.normals) src.normals = fromHex(m.source.normals, Float32Array);
+      if (m.source.uvs) src.uvs = fromHex(m.source.uvs, Float32Array);
+      data = vg.buildVirtualMesh(src, m.source.options || {});
+    } else {
+      data = vg.buildVirtualMesh(positions, indices, m.source.options || {});
+    }
     const fields = ['meshletBounds', 'meshletInfo', 'meshletVertices', 'meshletTriangles', 'levelRanges', 'levelErrors',
       'levelGuardErrors', 'replacementGroup', 'replacementStart', 'replacementIndices', 'indices', 'positions', 'boundingSphere',
-      'lodBoundsSphere', 'cullBoundsSphere'];
+      'lodBoundsSphere', 'cullBoundsSphere', 'normals', 'colors', 'uvs'];
     const b = { meshletCount: data.meshletCount, stats: data.stats };
-    for (const f of fields) b[f] = hex(data[f].buffer);
+    for (const f of fields) b[f] = data[f] ? hex(data[f].buffer) : null;
     out.built.push(b);
   } else {
     data = {

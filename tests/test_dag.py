@@ -268,5 +268,9 @@ def test_invalid_input_is_rejected():
     with pytest.raises(ValueError):
         dag.build_virtual_mesh(pos, np.array([0, 1], dtype=np.uint32))
     with pytest.raises(ValueError):
-        dag.build_virtual_mesh(pos, np.array([0, 1, 2], dtype=np.uint32), voxelLods=True)
+        dag.build_virtual_mesh(pos, np.array([0, 1, 2], dtype=np.uint32), voxelLods=True, voxelResolution=0)
+    with pytest.raises(ValueError):
+        dag.build_virtual_mesh(pos, np.array([0, 1, 2], dtype=np.uint32), normals=np.zeros(6, np.float32))
+    with pytest.raises(TypeError):
+        dag.build_virtual_mesh(pos, np.array([0, 1, 2], dtype=np.uint32), voxelLod=True)
     assert math.isfinite(float(dag.build_virtual_mesh(pos + np.eye(3, dtype=np.float32), np.array([0, 1, 2], dtype=np.uint32)).boundingSphere[3]))

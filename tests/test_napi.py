@@ -84,6 +84,26 @@ def test_node_host_builds_the_same_pool(tmp_path):
 
 
 @needs_addon
+def test_node_builds_foliage_with_attributes(tmp_path):
+    """buildVirtualMesh(src, options) through the addon with normals and uvs on the leaf cards (aggregate and voxel
+    levels, added vertices) equals the Python binding's build bit for bit, attributes and stats included."""
+    import foliage
+
+    pos, uv, idx, nrm = foliage.leaf_cards(fronds=6)
+    mesh = {"source": {"positions": pos.tobytes().hex(), "indices": idx.tobytes().hex(), "normals": nrm.tobytes().hex(),
+                       "uvs": uv.tobytes().hex(), "options": {}},
+            "matrices": np.eye(4, dtype=np.float32).ravel().tolist(), "flags": FLAGS}
+    got = run_node(tmp_path, {"config": {"maxViews": 2, "meshSlots": 8, "device": -1}, "meshes": [mesh]})
+    d = dag.build_virtual_mesh(pos, idx, normals=nrm, uvs=uv)
+    b = got["built"][0]
+    assert d.aggregateGroups > 0 and d.voxelLevels > 0
+    assert b["stats"]["aggregateGroups"] == d.aggregateGroups and b["stats"]["voxelLevels"] == d.voxelLevels
+    assert b["colors"] is None and d.colors is None
+    for f in ("meshletBounds", "meshletInfo", "meshletVertices", "meshletTriangles", "indices", "positions", "normals", "uvs"):
+        assert bytes.fromhex(b[f]) == np.ascontiguousarray(getattr(d, f)).tobytes(), f
+
+
+@needs_addon
 def test_node_host_builds_the_dag(tmp_path):
     """buildVirtualMesh through the addon (buildVirtualMesh.ts:185) equals the Python binding's build bit for bit,
     and the pool a Node host fills from it equals the Python host's."""

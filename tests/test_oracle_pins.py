@@ -18,6 +18,7 @@ formula again:
 Each check is then shown to catch a plausible mistake: the oracle compiled with one planted mistake
 (-DOR_MUTANT=k, on the CPU) fails at least one known answer, and so do mistaken view-math functions.
 """
+import functools
 import math
 import os
 from concurrent.futures import ThreadPoolExecutor
@@ -114,12 +115,17 @@ def test_prefix_known_answers():
 
 # ------------------------------------------------------------------ distance and size culls
 
+@functools.lru_cache(maxsize=None)
+def small_sphere():
+    return scenegen.icosphere(triangles=2_000, radius=1.0, seed=3, amplitude=0.0)
+
+
 class CullScene:
     """One small mesh (bounding sphere (c, r)) with instances translated along -z in front of a camera at
     the origin looking down -z (so the view matrix is the identity), on a host-only context."""
 
     def __init__(self, positions, max_draw=1e30, min_px=0.0):
-        self.data = scenegen.icosphere(triangles=2_000, radius=1.0, seed=3, amplitude=0.0)
+        self.data = small_sphere()
         mats = np.zeros((len(positions), 16), dtype=np.float32)
         mats[:, 0] = mats[:, 5] = mats[:, 10] = mats[:, 15] = 1.0
         mats[:, 12:15] = positions

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <queue>
@@ -312,6 +313,116 @@ void merge_fragments(const std::vector<uint32_t>& idx, const std::vector<uint32_
   ms.resize(w);
 }
 
+// Nearest free triangle centroid to a point: a static kd-tree over the centroids (median splits, leaves of
+// up to 8) whose nodes count their free triangles, so emptied subtrees are skipped. Ties go to the lower
+// triangle id.
+class CentroidTree {
+ public:
+  CentroidTree(const std::vector<float>& cen, const std::vector<uint8_t>& used) : cen_(cen) {
+    const uint32_t nt = static_cast<uint32_t>(cen.size() / 3);
+    items_.resize(nt);
+    std::iota(items_.begin(), items_.end(), 0u);
+    leaf_of_.assign(nt, 0);
+    build(0, nt, -1);
+    for (uint32_t t = 0; t < nt; t++)
+      if (used[t]) remove(t);
+  }
+  void remove(uint32_t t) {
+    for (int32_t n = leaf_of_[t]; n >= 0; n = nodes_[n].parent) nodes_[n].live--;
+  }
+  // Returns -1 when every triangle is used.
+  int64_t nearest(const double* p, const std::vector<uint8_t>& used) const {
+    best_ = -1;
+    best_d_ = INFINITY;
+    if (!nodes_.empty() && nodes_[0].live) visit(0, p, used);
+    return best_;
+  }
+
+ private:
+  struct Node {
+    float lo[3], hi[3];
+    uint32_t begin, end, live;
+    int32_t left = -1, right = -1, parent = -1;
+  };
+  const std::vector<float>& cen_;
+  std::vector<uint32_t> items_;
+  std::vector<int32_t> leaf_of_;
+  std::vector<Node> nodes_;
+  mutable int64_t best_ = -1;
+  mutable double best_d_ = INFINITY;
+
+  int32_t build(uint32_t begin, uint32_t end, int32_t parent) {
+    const int32_t id = static_cast<int32_t>(nodes_.size());
+    nodes_.push_back(Node{});
+    Node nd;
+    nd.begin = begin;
+    nd.end = end;
+    nd.live = end - begin;
+    nd.parent = parent;
+    for (int k = 0; k < 3; k++) {
+      nd.lo[k] = INFINITY;
+      nd.hi[k] = -INFINITY;
+    }
+    for (uint32_t i = begin; i < end; i++)
+      for (int k = 0; k < 3; k++) {
+        nd.lo[k] = std::min(nd.lo[k], cen_[items_[i] * 3 + k]);
+        nd.hi[k] = std::max(nd.hi[k], cen_[items_[i] * 3 + k]);
+      }
+    if (end - begin > 8) {
+      int axis = 0;
+      for (int k = 1; k < 3; k++)
+        if (nd.hi[k] - nd.lo[k] > nd.hi[axis] - nd.lo[axis]) axis = k;
+      const uint32_t mid = begin + (end - begin) / 2;
+      std::nth_element(items_.begin() + begin, items_.begin() + mid, items_.begin() + end, [&](uint32_t a, uint32_t b) {
+        const float ca = cen_[a * 3 + axis], cb = cen_[b * 3 + axis];
+        return ca < cb || (ca == cb && a < b);
+      });
+      nodes_[id] = nd;
+      const int32_t l = build(begin, mid, id);
+      const int32_t r = build(mid, end, id);
+      nodes_[id].left = l;
+      nodes_[id].right = r;
+    } else {
+      nodes_[id] = nd;
+      for (uint32_t i = begin; i < end; i++) leaf_of_[items_[i]] = id;
+    }
+    return id;
+  }
+  double box_d2(const Node& nd, const double* p) const {
+    double d = 0;
+    for (int k = 0; k < 3; k++) {
+      const double e = p[k] < nd.lo[k] ? nd.lo[k] - p[k] : p[k] > nd.hi[k] ? p[k] - nd.hi[k] : 0.0;
+      d += e * e;
+    }
+    return d;
+  }
+  void visit(int32_t n, const double* p, const std::vector<uint8_t>& used) const {
+    const Node& nd = nodes_[n];
+    if (!nd.live || box_d2(nd, p) > best_d_) return;
+    if (nd.left < 0) {
+      for (uint32_t i = nd.begin; i < nd.end; i++) {
+        const uint32_t t = items_[i];
+        if (used[t]) continue;
+        const double dx = cen_[t * 3] - p[0], dy = cen_[t * 3 + 1] - p[1], dz = cen_[t * 3 + 2] - p[2];
+        const double d = dx * dx + dy * dy + dz * dz;
+        if (d < best_d_ || (d == best_d_ && t < best_)) {
+          best_d_ = d;
+          best_ = t;
+        }
+      }
+      return;
+    }
+    const int32_t a = nd.left, b = nd.right;
+    if (box_d2(nodes_[a], p) <= box_d2(nodes_[b], p)) {
+      visit(a, p, used);
+      visit(b, p, used);
+    } else {
+      visit(b, p, used);
+      visit(a, p, used);
+    }
+  }
+};
+
 std::vector<Built> build_meshlets(const std::vector<uint32_t>& idx, const std::vector<float>& pos) {
   const uint32_t nv = static_cast<uint32_t>(pos.size() / 3);
   const uint32_t nt = static_cast<uint32_t>(idx.size() / 3);
@@ -334,6 +445,7 @@ std::vector<Built> build_meshlets(const std::vector<uint32_t>& idx, const std::v
   std::vector<uint32_t> stamp(nt, ~0u);
   std::vector<uint32_t> cand;
   std::vector<Built> out;
+  std::unique_ptr<CentroidTree> tree;  // built when a meshlet first runs out of adjacent triangles
   uint32_t scan = 0;
   double last[3] = {0, 0, 0};
   auto dist2 = [&](uint32_t t, const double* c) {
@@ -366,6 +478,7 @@ std::vector<Built> build_meshlets(const std::vector<uint32_t>& idx, const std::v
     uint32_t count = 0;
     auto add = [&](uint32_t t) {
       used[t] = 1;
+      if (tree) tree->remove(t);
       for (int k = 0; k < 3; k++) {
         const uint32_t v = idx[t * 3 + k];
         live[v]--;
@@ -412,6 +525,12 @@ std::vector<Built> build_meshlets(const std::vector<uint32_t>& idx, const std::v
         }
       }
       cand.resize(w);
+      if (best < 0 && w == 0 && b.verts.size() + 3 <= kMaxVertices) {
+        // No free triangle touches the meshlet (a separate piece is done): continue with the free triangle
+        // nearest the centroid, as meshoptimizer's builder does, so pieces share meshlets.
+        if (!tree) tree = std::make_unique<CentroidTree>(cen, used);
+        best = tree->nearest(c, used);
+      }
       if (best < 0) break;
       add(static_cast<uint32_t>(best));
     }
@@ -468,10 +587,22 @@ struct Simplified {
 
 enum : uint8_t { kFree = 0, kBorder = 1, kLocked = 2 };
 
+// The collapses of one simplification run, in order, so that the result for any target can be replayed:
+// the collapse order does not depend on the target (a run to a smaller target passes through the state
+// where a run to a larger one stops), so the result for target T is the input triangles with the
+// collapses applied up to the first point where at most T indices are live.
+struct SimplifyLog {
+  std::vector<uint32_t> tri;  // the input triangles without zero-area ones
+  std::vector<std::pair<uint32_t, uint32_t>> collapses;  // u -> v
+  std::vector<size_t> live_after;
+  std::vector<double> max_cost_after;
+  double extent = 0;
+};
+
 // Quadric half-edge collapses down to `target` indices (see the file comment). Indices are local ids.
-// `cls` maps each vertex to the first vertex with the same position.
+// `cls` maps each vertex to the first vertex with the same position. With `log`, the run is recorded.
 Simplified simplify(const std::vector<uint32_t>& in, const std::vector<float>& pos_f, const std::vector<uint32_t>& cls,
-                    size_t target, bool lock_border, bool prune) {
+                    size_t target, bool lock_border, bool prune, SimplifyLog* log = nullptr) {
   const uint32_t nv = static_cast<uint32_t>(pos_f.size() / 3);
   Simplified res;
   std::vector<uint8_t> seam(nv, 0);
@@ -486,6 +617,7 @@ Simplified simplify(const std::vector<uint32_t>& in, const std::vector<float>& p
     tri.insert(tri.end(), {a, b, c});
   }
   const uint32_t ntri = static_cast<uint32_t>(tri.size() / 3);
+  if (log) log->tri = tri;
   if (tri.size() <= target) {
     res.indices = tri;
     return res;
@@ -498,6 +630,7 @@ Simplified simplify(const std::vector<uint32_t>& in, const std::vector<float>& p
       hi[k] = std::max(hi[k], static_cast<double>(pos_f[v * 3 + k]));
     }
   const double extent = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], std::max(hi[2] - lo[2], 1e-30)));
+  if (log) log->extent = extent;
   std::vector<double> p(static_cast<size_t>(nv) * 3);
   for (uint32_t v = 0; v < nv; v++)
     for (int k = 0; k < 3; k++) p[v * 3 + k] = (pos_f[v * 3 + k] - lo[k]) / extent;
@@ -758,6 +891,11 @@ Simplified simplify(const std::vector<uint32_t>& in, const std::vector<float>& p
       max_cost = std::max(max_cost, e.cost);
       collapse(e.u, e.v);
       done++;
+      if (log) {
+        log->collapses.emplace_back(e.u, e.v);
+        log->live_after.push_back(live_tris);
+        log->max_cost_after.push_back(max_cost);
+      }
     }
     if (done == 0) break;
   }
@@ -766,6 +904,39 @@ Simplified simplify(const std::vector<uint32_t>& in, const std::vector<float>& p
   for (uint32_t t = 0; t < ntri; t++)
     if (tri_live[t]) res.indices.insert(res.indices.end(), {tri[t * 3], tri[t * 3 + 1], tri[t * 3 + 2]});
   res.error = std::sqrt(max_cost) * extent;
+  return res;
+}
+
+// The run a SimplifyLog recorded, stopped where a run to `target` indices stops.
+Simplified replay(const SimplifyLog& log, size_t target) {
+  Simplified res;
+  const size_t ntri = log.tri.size() / 3;
+  size_t k = 0;  // collapses applied
+  if (ntri * 3 > target) {
+    while (k < log.collapses.size() && log.live_after[k] * 3 > target) k++;
+    if (k < log.collapses.size()) k++;  // the collapse that reached the target
+  }
+  uint32_t nv = 0;
+  for (uint32_t v : log.tri) nv = std::max(nv, v + 1);
+  for (size_t i = 0; i < k; i++) nv = std::max(nv, std::max(log.collapses[i].first, log.collapses[i].second) + 1);
+  std::vector<uint32_t> rep(nv);
+  std::iota(rep.begin(), rep.end(), 0u);
+  for (size_t i = 0; i < k; i++) rep[log.collapses[i].first] = log.collapses[i].second;
+  auto resolve = [&](uint32_t v) {
+    uint32_t r = v;
+    while (rep[r] != r) r = rep[r];
+    while (rep[v] != r) {
+      const uint32_t next = rep[v];
+      rep[v] = r;
+      v = next;
+    }
+    return r;
+  };
+  for (size_t t = 0; t < ntri; t++) {
+    const uint32_t a = resolve(log.tri[t * 3]), b = resolve(log.tri[t * 3 + 1]), c = resolve(log.tri[t * 3 + 2]);
+    if (a != b && b != c && a != c) res.indices.insert(res.indices.end(), {a, b, c});
+  }
+  res.error = k ? std::sqrt(log.max_cost_after[k - 1]) * log.extent : 0.0;
   return res;
 }
 
@@ -863,6 +1034,585 @@ Simplified simplify_sloppy(const std::vector<uint32_t>& in, const std::vector<fl
   Simplified res;
   run(lo_g, &res);
   return res;
+}
+
+// ------------------------------------------------------------------ voxel proxies (voxelProxy.ts)
+
+constexpr double kVoxelErrorCells = 2;  // VOXEL_ERROR_CELLS, voxelProxy.ts:15
+
+// Vertex attributes that travel with positions when levels add vertices (buildVirtualMesh.ts:388-407
+// appendVertices): normals 3, colors 3, uvs 2 per vertex; an empty vector is an absent attribute.
+struct Attributes {
+  std::vector<float> normals, colors, uvs;
+};
+
+struct VoxelProxy {
+  std::vector<float> positions, normals, colors, uvs;  // colors / uvs empty when the source has none
+  std::vector<uint32_t> indices;
+  double cell = 0;
+};
+
+// voxelProxy.ts:333-354 fillMissing: NaN entries (`size` floats per vertex) take the mean of their
+// triangles' known neighbours, over at most 8 passes; what is still unknown becomes 0.5.
+void fill_missing(std::vector<float>& values, const std::vector<uint32_t>& indices, int size) {
+  const size_t count = values.size() / size;
+  for (int pass = 0; pass < 8; pass++) {
+    size_t missing = 0;
+    std::vector<float> sum(count * size, 0.0f);
+    std::vector<uint32_t> hits(count, 0);
+    for (size_t t = 0; t + 2 < indices.size(); t += 3)
+      for (int k = 0; k < 3; k++) {
+        const uint32_t v = indices[t + k];
+        if (!std::isnan(values[v * size])) continue;
+        for (int m = 1; m < 3; m++) {
+          const uint32_t u = indices[t + (k + m) % 3];
+          if (std::isnan(values[u * size])) continue;
+          for (int c = 0; c < size; c++) sum[v * size + c] = static_cast<float>(double(sum[v * size + c]) + values[u * size + c]);
+          hits[v]++;
+        }
+      }
+    for (size_t v = 0; v < count; v++) {
+      if (!std::isnan(values[v * size])) continue;
+      if (hits[v]) {
+        for (int c = 0; c < size; c++) values[v * size + c] = static_cast<float>(double(sum[v * size + c]) / hits[v]);
+      } else {
+        missing++;
+      }
+    }
+    if (!missing) return;
+  }
+  for (float& x : values)
+    if (std::isnan(x)) x = 0.5f;
+}
+
+// voxelProxy.ts:33-281 buildVoxelProxy: occupancy from samples at half-cell spacing over every triangle,
+// a 3x3x3 closing, a separable 3-tap box blur of the occupancy, and a surface-nets surface at level 0.5
+// (one vertex per cell the level crosses, at the mean of its edge crossings; one quad per crossing sample
+// edge). JavaScript numbers are doubles and its typed arrays round to binary32 on store: the occupancy
+// field, colour sums and sampled UVs are kept in float here and the arithmetic around them in double, as
+// there. Returns false for an empty or flat source.
+bool build_voxel_proxy(const float* positions, const float* colors, const float* uvs, const std::vector<uint32_t>& indices,
+                       int resolution, VoxelProxy& out) {
+  if (indices.empty()) return false;
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i : indices)
+    for (int k = 0; k < 3; k++) {
+      mn[k] = std::min(mn[k], static_cast<double>(positions[static_cast<size_t>(i) * 3 + k]));
+      mx[k] = std::max(mx[k], static_cast<double>(positions[static_cast<size_t>(i) * 3 + k]));
+    }
+  const double extent = std::max(mx[0] - mn[0], std::max(mx[1] - mn[1], mx[2] - mn[2]));
+  if (!(extent > 0)) return false;
+  const double cell = extent / resolution;
+  constexpr int kPad = 3;  // cells of empty border around the samples
+  int64_t n[3];
+  double origin[3];
+  for (int k = 0; k < 3; k++) {
+    n[k] = static_cast<int64_t>(std::ceil((mx[k] - mn[k]) / cell)) + 1 + kPad * 2;
+    origin[k] = mn[k] - kPad * cell;
+  }
+  const int64_t nx = n[0], ny = n[1], nz = n[2];
+  const int64_t sx = 1, sy = nx, sz = nx * ny, total = nx * ny * nz;
+
+  // 1. Occupancy (+ colour sums, first UV) from barycentric samples.
+  std::vector<uint8_t> occupied(total, 0);
+  std::vector<float> color_sum(colors ? total * 3 : 0, 0.0f);
+  std::vector<uint32_t> color_count(colors ? total : 0, 0);
+  std::vector<float> voxel_uv(uvs ? total * 2 : 0, NAN);
+  auto hyp = [&](size_t a, size_t b) {
+    return std::hypot(double(positions[a]) - positions[b], double(positions[a + 1]) - positions[b + 1],
+                      double(positions[a + 2]) - positions[b + 2]);
+  };
+  for (size_t t = 0; t + 2 < indices.size(); t += 3) {
+    const size_t a = static_cast<size_t>(indices[t]) * 3, b = static_cast<size_t>(indices[t + 1]) * 3,
+                 c = static_cast<size_t>(indices[t + 2]) * 3;
+    const double longest = std::max(hyp(a, b), std::max(hyp(b, c), hyp(c, a)));
+    const int64_t steps = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(longest / (cell * 0.5))));
+    for (int64_t i = 0; i <= steps; i++)
+      for (int64_t j = 0; j <= steps - i; j++) {
+        const double u = double(i) / steps, v = double(j) / steps, w = 1 - u - v;
+        double p[3];
+        for (int k = 0; k < 3; k++) p[k] = positions[a + k] * w + positions[b + k] * u + positions[c + k] * v;
+        const int64_t vox = static_cast<int64_t>(std::floor((p[0] - origin[0]) / cell)) * sx +
+                            static_cast<int64_t>(std::floor((p[1] - origin[1]) / cell)) * sy +
+                            static_cast<int64_t>(std::floor((p[2] - origin[2]) / cell)) * sz;
+        occupied[vox] = 1;
+        if (uvs && std::isnan(voxel_uv[vox * 2])) {
+          const size_t ta = static_cast<size_t>(indices[t]) * 2, tb = static_cast<size_t>(indices[t + 1]) * 2,
+                       tc = static_cast<size_t>(indices[t + 2]) * 2;
+          voxel_uv[vox * 2] = static_cast<float>(uvs[ta] * w + uvs[tb] * u + uvs[tc] * v);
+          voxel_uv[vox * 2 + 1] = static_cast<float>(uvs[ta + 1] * w + uvs[tb + 1] * u + uvs[tc + 1] * v);
+        }
+        if (colors) {
+          for (int k = 0; k < 3; k++) {
+            const double col = colors[a + k] * w + colors[b + k] * u + colors[c + k] * v;
+            color_sum[vox * 3 + k] = static_cast<float>(double(color_sum[vox * 3 + k]) + col);
+          }
+          color_count[vox]++;
+        }
+      }
+  }
+
+  // 2. Closing: 26-neighbourhood dilation, then erosion, each separable per axis with empty borders.
+  auto morph = [&](const std::vector<uint8_t>& src, int axis, bool grow) {
+    std::vector<uint8_t> dst(src.size());
+    const int64_t stride = axis == 0 ? sx : axis == 1 ? sy : sz, size = n[axis];
+    for (int64_t z = 0; z < nz; z++)
+      for (int64_t y = 0; y < ny; y++)
+        for (int64_t x = 0; x < nx; x++) {
+          const int64_t i = x + y * nx + z * nx * ny, c = axis == 0 ? x : axis == 1 ? y : z;
+          const uint8_t prev = c > 0 ? src[i - stride] : 0, next = c < size - 1 ? src[i + stride] : 0;
+          dst[i] = grow ? (src[i] | prev | next) : (src[i] & prev & next);
+        }
+    return dst;
+  };
+  std::vector<uint8_t> closed = occupied;
+  for (int axis = 0; axis < 3; axis++) closed = morph(closed, axis, true);
+  for (int axis = 0; axis < 3; axis++) closed = morph(closed, axis, false);
+
+  // 3. Field: the closed occupancy blurred by a 3-tap box along each axis in turn.
+  std::vector<float> field(closed.begin(), closed.end());
+  for (int axis = 0; axis < 3; axis++) {
+    const int64_t stride = axis == 0 ? sx : axis == 1 ? sy : sz, size = n[axis];
+    std::vector<float> line(size);
+    const int64_t lines = total / size;
+    for (int64_t l = 0; l < lines; l++) {
+      const int64_t start = axis == 0 ? l * nx : axis == 1 ? (l % nx) + (l / nx) * nx * ny : l;
+      for (int64_t k = 0; k < size; k++) line[k] = field[start + k * stride];
+      for (int64_t k = 0; k < size; k++) {
+        const double lo = k > 0 ? line[k - 1] : 0.0, hi = k < size - 1 ? line[k + 1] : 0.0;
+        field[start + k * stride] = static_cast<float>((lo + line[k] + hi) / 3);
+      }
+    }
+  }
+  constexpr double kIso = 0.5;
+
+  // 4. Surface nets.
+  const int64_t cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  std::vector<int32_t> cell_vertex(static_cast<size_t>(cx * cy * cz), -1);
+  std::vector<double> pos;  // doubles until the end: the quads are oriented with them
+  std::vector<float> nrm, col, uv;
+  auto corner_index = [&](int64_t base, int c) { return base + (c & 1) * sx + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz; };
+  for (int64_t z = 0; z < cz; z++)
+    for (int64_t y = 0; y < cy; y++)
+      for (int64_t x = 0; x < cx; x++) {
+        const int64_t base = x * sx + y * sy + z * sz;
+        double corner[8];
+        int inside = 0;
+        for (int c = 0; c < 8; c++) {
+          corner[c] = field[corner_index(base, c)];
+          if (corner[c] >= kIso) inside++;
+        }
+        if (inside == 0 || inside == 8) continue;
+        double px = 0, py = 0, pz = 0;
+        int crossings = 0;
+        for (int c = 0; c < 8; c++)
+          for (int bit : {1, 2, 4}) {
+            if (c & bit) continue;
+            const double f0 = corner[c], f1 = corner[c | bit];
+            if ((f0 >= kIso) == (f1 >= kIso)) continue;
+            const double s = (kIso - f0) / (f1 - f0);
+            px += (c & 1) + (bit == 1 ? s : 0);
+            py += ((c >> 1) & 1) + (bit == 2 ? s : 0);
+            pz += ((c >> 2) & 1) + (bit == 4 ? s : 0);
+            crossings++;
+          }
+        cell_vertex[x + y * cx + z * cx * cy] = static_cast<int32_t>(pos.size() / 3);
+        pos.push_back(origin[0] + (x + 0.5 + px / crossings) * cell);
+        pos.push_back(origin[1] + (y + 0.5 + py / crossings) * cell);
+        pos.push_back(origin[2] + (z + 0.5 + pz / crossings) * cell);
+        // The field falls outward: the outward normal is minus its gradient.
+        const double gx = corner[1] + corner[3] + corner[5] + corner[7] - corner[0] - corner[2] - corner[4] - corner[6];
+        const double gy = corner[2] + corner[3] + corner[6] + corner[7] - corner[0] - corner[1] - corner[4] - corner[5];
+        const double gz = corner[4] + corner[5] + corner[6] + corner[7] - corner[0] - corner[1] - corner[2] - corner[3];
+        double len = std::hypot(gx, gy, gz);
+        if (len == 0 || std::isnan(len)) len = 1;
+        nrm.insert(nrm.end(), {static_cast<float>(-gx / len), static_cast<float>(-gy / len), static_cast<float>(-gz / len)});
+        if (colors) {
+          double r = 0, g = 0, bl = 0;
+          uint64_t count = 0;
+          for (int c = 0; c < 8; c++) {
+            const int64_t i = corner_index(base, c);
+            if (!color_count[i]) continue;
+            r += color_sum[i * 3];
+            g += color_sum[i * 3 + 1];
+            bl += color_sum[i * 3 + 2];
+            count += color_count[i];
+          }
+          if (count) col.insert(col.end(), {static_cast<float>(r / count), static_cast<float>(g / count), static_cast<float>(bl / count)});
+          else col.insert(col.end(), {NAN, NAN, NAN});  // filled from neighbours below
+        }
+        if (uvs) {
+          float u = NAN, v = NAN;
+          for (int c = 0; c < 8 && std::isnan(u); c++) {
+            const int64_t i = corner_index(base, c);
+            u = voxel_uv[i * 2];
+            v = voxel_uv[i * 2 + 1];
+          }
+          uv.insert(uv.end(), {u, v});
+        }
+      }
+  std::vector<uint32_t> tris;
+  auto cell_at = [&](int64_t x, int64_t y, int64_t z) { return static_cast<uint32_t>(cell_vertex[x + y * cx + z * cx * cy]); };
+  auto quad = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d, double ox, double oy, double oz) {
+    // Orient the two triangles so their normal agrees with the crossing edge's outward direction.
+    const double ux = pos[c * 3] - pos[a * 3], uy = pos[c * 3 + 1] - pos[a * 3 + 1], uz = pos[c * 3 + 2] - pos[a * 3 + 2];
+    const double vx = pos[d * 3] - pos[b * 3], vy = pos[d * 3 + 1] - pos[b * 3 + 1], vz = pos[d * 3 + 2] - pos[b * 3 + 2];
+    const double qx = uy * vz - uz * vy, qy = uz * vx - ux * vz, qz = ux * vy - uy * vx;
+    if (qx * ox + qy * oy + qz * oz >= 0) tris.insert(tris.end(), {a, b, c, a, c, d});
+    else tris.insert(tris.end(), {a, c, b, a, d, c});
+  };
+  for (int64_t z = 1; z < nz - 1; z++)
+    for (int64_t y = 1; y < ny - 1; y++)
+      for (int64_t x = 1; x < nx - 1; x++) {
+        const int64_t i = x * sx + y * sy + z * sz;
+        const bool here = field[i] >= kIso;
+        const double s = here ? 1 : -1;
+        if (here != (field[i + sx] >= kIso))
+          quad(cell_at(x, y - 1, z - 1), cell_at(x, y, z - 1), cell_at(x, y, z), cell_at(x, y - 1, z), s, 0, 0);
+        if (here != (field[i + sy] >= kIso))
+          quad(cell_at(x - 1, y, z - 1), cell_at(x, y, z - 1), cell_at(x, y, z), cell_at(x - 1, y, z), 0, s, 0);
+        if (here != (field[i + sz] >= kIso))
+          quad(cell_at(x - 1, y - 1, z), cell_at(x, y - 1, z), cell_at(x, y, z), cell_at(x - 1, y, z), 0, 0, s);
+      }
+  if (tris.empty()) return false;
+  out.positions.assign(pos.begin(), pos.end());
+  out.normals = std::move(nrm);
+  out.colors = std::move(col);
+  out.uvs = std::move(uv);
+  if (colors) fill_missing(out.colors, tris, 3);
+  if (uvs) fill_missing(out.uvs, tris, 2);
+  out.indices = std::move(tris);
+  out.cell = cell;
+  return true;
+}
+
+// ------------------------------------------------------------------ aggregate geometry (aggregate.ts)
+
+constexpr int kPairWindow = 24;  // PAIR_WINDOW, aggregate.ts:34
+
+struct Aggregate {
+  std::vector<uint32_t> indices;  // ids >= first_new refer to new_positions
+  double error = 0;
+  std::vector<double> new_positions;  // 3 per new vertex (binary32 once appended)
+  std::vector<uint32_t> new_sources;  // the vertex each new one takes its other attributes from
+  uint32_t first_new = 0;
+};
+
+struct Piece {
+  double area = 0;
+  double center[3] = {0, 0, 0};
+  double axes[3][3] = {};  // unit axes, longest first
+  double extent[3] = {0, 0, 0};
+};
+
+// aggregate.ts:291-338 eigenvectors: cyclic Jacobi rotations (at most 24 sweeps) of a symmetric 3x3 matrix;
+// the columns of the accumulated rotation, normalised, by decreasing eigenvalue (stable for ties).
+void jacobi_axes(const double* m, double axes[3][3]) {
+  double a[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::memcpy(a, m, sizeof a);
+  for (int sweep = 0; sweep < 24; sweep++) {
+    const double off = std::fabs(a[1]) + std::fabs(a[2]) + std::fabs(a[5]);
+    if (off < 1e-12 * (std::fabs(a[0]) + std::fabs(a[4]) + std::fabs(a[8]) + 1e-30)) break;
+    for (const auto& pq : {std::pair<int, int>{0, 1}, {0, 2}, {1, 2}}) {
+      const int p = pq.first, q = pq.second;
+      const double apq = a[p * 3 + q];
+      if (std::fabs(apq) < 1e-30) continue;
+      const double theta = (a[q * 3 + q] - a[p * 3 + p]) / (2 * apq);
+      const double sgn = theta > 0 ? 1.0 : theta < 0 ? -1.0 : 1.0;
+      const double t = sgn / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+      const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+      for (int k = 0; k < 3; k++) {
+        const double akp = a[k * 3 + p], akq = a[k * 3 + q];
+        a[k * 3 + p] = c * akp - s * akq;
+        a[k * 3 + q] = s * akp + c * akq;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double apk = a[p * 3 + k], aqk = a[q * 3 + k];
+        a[p * 3 + k] = c * apk - s * aqk;
+        a[q * 3 + k] = s * apk + c * aqk;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double vkp = v[k * 3 + p], vkq = v[k * 3 + q];
+        v[k * 3 + p] = c * vkp - s * vkq;
+        v[k * 3 + q] = s * vkp + c * vkq;
+      }
+    }
+  }
+  int order[3] = {0, 1, 2};
+  std::stable_sort(order, order + 3, [&](int i, int j) { return a[j * 3 + j] - a[i * 3 + i] < 0; });
+  for (int r = 0; r < 3; r++) {
+    const int i = order[r];
+    const double x = v[i], y = v[3 + i], z = v[6 + i];
+    double l = std::hypot(x, y, z);
+    if (l == 0) l = 1;
+    axes[r][0] = x / l;
+    axes[r][1] = y / l;
+    axes[r][2] = z / l;
+  }
+}
+
+// aggregate.ts:256-289 describe: area, vertex centroid, principal axes and extents along them.
+Piece describe_piece(const std::vector<uint32_t>& idx, const std::vector<float>& positions) {
+  Piece p;
+  std::vector<uint32_t> unique;  // first-occurrence order, as a JavaScript Set keeps it
+  {
+    std::unordered_map<uint32_t, uint8_t> once;
+    once.reserve(idx.size() * 2);
+    for (uint32_t v : idx)
+      if (once.emplace(v, 1).second) unique.push_back(v);
+  }
+  const double count = static_cast<double>(unique.size());
+  for (uint32_t v : unique)
+    for (int k = 0; k < 3; k++) p.center[k] += positions[static_cast<size_t>(v) * 3 + k] / count;
+  double cov[9] = {};
+  for (uint32_t v : unique) {
+    double d[3];
+    for (int k = 0; k < 3; k++) d[k] = positions[static_cast<size_t>(v) * 3 + k] - p.center[k];
+    for (int r = 0; r < 3; r++)
+      for (int q = 0; q < 3; q++) cov[r * 3 + q] += d[r] * d[q];
+  }
+  jacobi_axes(cov, p.axes);
+  for (int a = 0; a < 3; a++) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (uint32_t v : unique) {
+      const float* q = &positions[static_cast<size_t>(v) * 3];
+      const double along = (q[0] - p.center[0]) * p.axes[a][0] + (q[1] - p.center[1]) * p.axes[a][1] + (q[2] - p.center[2]) * p.axes[a][2];
+      lo = std::min(lo, along);
+      hi = std::max(hi, along);
+    }
+    p.extent[a] = hi - lo;
+  }
+  for (size_t t = 0; t + 2 < idx.size(); t += 3) {
+    const float* A = &positions[static_cast<size_t>(idx[t]) * 3];
+    const float* B = &positions[static_cast<size_t>(idx[t + 1]) * 3];
+    const float* C = &positions[static_cast<size_t>(idx[t + 2]) * 3];
+    const double u[3] = {double(B[0]) - A[0], double(B[1]) - A[1], double(B[2]) - A[2]};
+    const double w[3] = {double(C[0]) - A[0], double(C[1]) - A[1], double(C[2]) - A[2]};
+    p.area += 0.5 * std::hypot(u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]);
+  }
+  return p;
+}
+
+// aggregate.ts:349-358 morton: 10 bits per axis of a point in [0, 1]^3 (clamped).
+uint32_t morton10(const double* p) {
+  auto spread = [](double x) {
+    uint32_t v = static_cast<uint32_t>(std::min(1023.0, std::max(0.0, std::floor(x * 1024)))) & 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+  };
+  return spread(p[0]) | (spread(p[1]) << 1) | (spread(p[2]) << 2);
+}
+
+// aggregate.ts:225-254 widen: a copy of a piece's vertices spread along its first two axes (across its width
+// when it is long and thin, evenly in its plane otherwise) to scale its area, and moved by `shift`. New ids
+// start at `first_id`; returns the growth (how far an edge moved).
+double widen_piece(std::vector<uint32_t>& idx, const std::vector<float>& positions, const Piece& piece, double factor,
+                   const double* shift, uint32_t first_id, Aggregate& out) {
+  const bool elongated = piece.extent[0] > piece.extent[1] * 2;
+  const double scale[2] = {elongated ? 1.0 : std::sqrt(factor), elongated ? factor : std::sqrt(factor)};
+  const double growth = std::max((scale[0] - 1) * piece.extent[0], (scale[1] - 1) * piece.extent[1]) / 2;
+  std::unordered_map<uint32_t, uint32_t> copy;
+  for (uint32_t& v : idx) {
+    auto it = copy.find(v);
+    if (it == copy.end()) {
+      const uint32_t id = first_id + static_cast<uint32_t>(copy.size());
+      it = copy.emplace(v, id).first;
+      const float* q = &positions[static_cast<size_t>(v) * 3];
+      const double d[3] = {q[0] - piece.center[0], q[1] - piece.center[1], q[2] - piece.center[2]};
+      double p[3] = {q[0] + shift[0], q[1] + shift[1], q[2] + shift[2]};
+      for (int a = 0; a < 2; a++) {
+        const double* axis = piece.axes[a];
+        const double along = d[0] * axis[0] + d[1] * axis[1] + d[2] * axis[2];
+        for (int k = 0; k < 3; k++) p[k] += (scale[a] - 1) * along * axis[k];
+      }
+      out.new_positions.insert(out.new_positions.end(), {p[0], p[1], p[2]});
+      out.new_sources.push_back(v);
+    }
+    v = it->second;
+  }
+  return growth;
+}
+
+// aggregate.ts:47-223 simplifyAggregate: a group made of separate pieces (connected components over welded
+// vertices) reduced piece by piece toward `target_triangles`. Pieces whose every vertex belongs to this group
+// alone (owner[weld[v]] == group) are free: each may be simplified on its own with open edges free, and
+// neighbouring free pieces (in Morton order of their centres, within PAIR_WINDOW places and twice the median
+// spacing) are thinned in pairs, the larger piece widened over both areas and moved to their area-weighted
+// centre. Both orders (simplify first, or thin only) are tried; the more accurate of those that reach 1.25 x
+// the target wins, else the one with fewest triangles. New vertex ids start at `first_new`.
+bool simplify_aggregate(const std::vector<uint32_t>& merged, const std::vector<float>& positions, const std::vector<uint32_t>& weld,
+                        const std::vector<int32_t>& owner, int32_t group, double target_triangles, uint32_t first_new,
+                        Aggregate& result) {
+  const size_t triangle_count = merged.size() / 3;
+  // Pieces: components over welded vertices, listed in order of their first triangle.
+  std::unordered_map<uint32_t, uint32_t> parent;
+  parent.reserve(merged.size());
+  auto find = [&](uint32_t x) {
+    uint32_t r = x;
+    while (parent[r] != r) r = parent[r];
+    while (parent[x] != r) {
+      const uint32_t next = parent[x];
+      parent[x] = r;
+      x = next;
+    }
+    return r;
+  };
+  for (uint32_t v : merged) parent.emplace(weld[v], weld[v]);
+  for (size_t t = 0; t < merged.size(); t += 3) {
+    const uint32_t a = find(weld[merged[t]]);
+    for (int k = 1; k < 3; k++) {
+      const uint32_t b = find(weld[merged[t + k]]);
+      if (a != b) parent[b] = a;
+    }
+  }
+  std::unordered_map<uint32_t, uint32_t> piece_of_root;
+  std::vector<std::vector<uint32_t>> piece_tris;
+  for (size_t t = 0; t < triangle_count; t++) {
+    const uint32_t root = find(weld[merged[t * 3]]);
+    auto it = piece_of_root.emplace(root, static_cast<uint32_t>(piece_tris.size())).first;
+    if (it->second == piece_tris.size()) piece_tris.emplace_back();
+    piece_tris[it->second].push_back(static_cast<uint32_t>(t));
+  }
+  if (piece_tris.size() < 2) return false;
+  struct Part {
+    std::vector<uint32_t> indices;
+    bool free = true;
+  };
+  std::vector<Part> initial(piece_tris.size());
+  for (size_t p = 0; p < piece_tris.size(); p++) {
+    for (uint32_t t : piece_tris[p])
+      for (int k = 0; k < 3; k++) {
+        const uint32_t v = merged[t * 3 + k];
+        initial[p].indices.push_back(v);
+        if (owner[weld[v]] != group) initial[p].free = false;
+      }
+  }
+
+  auto reduce = [&](bool simplify_first, Aggregate& out) {
+    double error = 0;
+    std::vector<Part> pieces = initial;
+    const double ratio = target_triangles / static_cast<double>(triangle_count);
+    double total = 0;
+    std::vector<int32_t> scratch;
+    for (Part& piece : pieces) {
+      const size_t target = std::max<size_t>(6, static_cast<size_t>(std::floor(piece.indices.size() * ratio / 3)) * 3);
+      if (simplify_first && piece.free && piece.indices.size() > target) {
+        // compactLocal (aggregate.ts:340-358): local ids in first-use order
+        std::unordered_map<uint32_t, uint32_t> local_of;
+        std::vector<uint32_t> to_global, local(piece.indices.size());
+        for (size_t i = 0; i < piece.indices.size(); i++) {
+          auto it = local_of.emplace(piece.indices[i], static_cast<uint32_t>(to_global.size())).first;
+          if (it->second == to_global.size()) to_global.push_back(piece.indices[i]);
+          local[i] = it->second;
+        }
+        std::vector<float> lp(to_global.size() * 3);
+        for (size_t i = 0; i < to_global.size(); i++)
+          std::memcpy(&lp[i * 3], &positions[static_cast<size_t>(to_global[i]) * 3], 12);
+        std::vector<uint32_t> cls = position_remap(lp.data(), static_cast<uint32_t>(to_global.size()));
+        Simplified s = simplify(local, lp, cls, target, false, false);
+        if (!s.indices.empty() && s.indices.size() < piece.indices.size()) {
+          piece.indices.resize(s.indices.size());
+          for (size_t i = 0; i < s.indices.size(); i++) piece.indices[i] = to_global[s.indices[i]];
+          error = std::max(error, s.error);
+        }
+      }
+      total += piece.indices.size() / 3;
+    }
+    out.first_new = first_new;
+    if (total > target_triangles) {
+      std::vector<uint32_t> order;
+      std::unordered_map<uint32_t, Piece> info;
+      for (uint32_t i = 0; i < pieces.size(); i++)
+        if (pieces[i].free) {
+          order.push_back(i);
+          info.emplace(i, describe_piece(pieces[i].indices, positions));
+        }
+      double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t i : order)
+        for (int k = 0; k < 3; k++) {
+          b[k] = std::min(b[k], info[i].center[k]);
+          b[k + 3] = std::max(b[k + 3], info[i].center[k]);
+        }
+      const double span = std::max(std::max(b[3] - b[0], b[4] - b[1]), std::max(b[5] - b[2], 1e-9));
+      auto key = [&](uint32_t i) {
+        const double q[3] = {(info[i].center[0] - b[0]) / span, (info[i].center[1] - b[1]) / span, (info[i].center[2] - b[2]) / span};
+        return morton10(q);
+      };
+      std::vector<uint32_t> keys(pieces.size(), 0);
+      for (uint32_t i : order) keys[i] = key(i);
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return keys[x] < keys[y]; });
+      auto dist = [&](uint32_t x, uint32_t y) {
+        const double* p = info[x].center;
+        const double* q = info[y].center;
+        return std::hypot(p[0] - q[0], p[1] - q[1], p[2] - q[2]);
+      };
+      const int64_t count = static_cast<int64_t>(order.size());
+      std::vector<double> nearest(count, INFINITY);
+      for (int64_t n = 0; n < count; n++)
+        for (int64_t m = std::max<int64_t>(0, n - kPairWindow); m < std::min<int64_t>(count, n + 1 + kPairWindow); m++)
+          if (m != n) nearest[n] = std::min(nearest[n], dist(order[n], order[m]));
+      std::vector<double> sorted_nearest = nearest;
+      std::sort(sorted_nearest.begin(), sorted_nearest.end());
+      const double spacing = count ? sorted_nearest[count / 2] : INFINITY;
+      std::vector<uint8_t> taken(pieces.size(), 0);
+      for (int64_t n = 0; n < count && total > target_triangles; n++) {
+        const uint32_t a = order[n];
+        if (taken[a]) continue;
+        int64_t best = -1;
+        double best_distance = spacing * 2;
+        for (int64_t m = n + 1; m < std::min<int64_t>(count, n + 1 + kPairWindow); m++) {
+          const uint32_t c = order[m];
+          if (taken[c]) continue;
+          const double d = dist(a, c);
+          if (d <= best_distance) {
+            best_distance = d;
+            best = c;
+          }
+        }
+        if (best < 0) continue;
+        taken[a] = taken[best] = 1;
+        const uint32_t keep = info[a].area >= info[static_cast<uint32_t>(best)].area ? a : static_cast<uint32_t>(best);
+        const uint32_t drop = keep == a ? static_cast<uint32_t>(best) : a;
+        const Piece& k = info[keep];
+        const Piece& dropped = info[drop];
+        const double pair_area = std::max(k.area + dropped.area, 1e-12);
+        const double factor = pair_area / std::max(k.area, 1e-12);
+        const double w = dropped.area / pair_area;
+        const double shift[3] = {(dropped.center[0] - k.center[0]) * w, (dropped.center[1] - k.center[1]) * w,
+                                 (dropped.center[2] - k.center[2]) * w};
+        const uint32_t first_id = first_new + static_cast<uint32_t>(out.new_sources.size());
+        const double growth = widen_piece(pieces[keep].indices, positions, k, factor, shift, first_id, out);
+        error = std::max(error, std::max(best_distance * std::max(w, 1 - w), growth));
+        total -= pieces[drop].indices.size() / 3;
+        pieces[drop].indices.clear();
+      }
+    }
+    out.indices.clear();
+    for (const Part& p : pieces) out.indices.insert(out.indices.end(), p.indices.begin(), p.indices.end());
+    if (out.indices.size() >= merged.size()) return false;
+    out.error = error;
+    return true;
+  };
+
+  Aggregate first, thin;
+  const bool have_first = reduce(true, first);
+  const bool have_thin = reduce(false, thin);
+  std::vector<Aggregate*> candidates;
+  if (have_first) candidates.push_back(&first);
+  if (have_thin) candidates.push_back(&thin);
+  if (candidates.empty()) return false;
+  std::vector<Aggregate*> reaches;
+  for (Aggregate* c : candidates)
+    if (c->indices.size() / 3.0 <= target_triangles * 1.25) reaches.push_back(c);
+  std::vector<Aggregate*>& pool = reaches.empty() ? candidates : reaches;
+  std::stable_sort(pool.begin(), pool.end(), [&](const Aggregate* x, const Aggregate* y) {
+    return reaches.empty() ? x->indices.size() < y->indices.size() : x->error < y->error;
+  });
+  result = std::move(*pool[0]);
+  return true;
 }
 
 // ------------------------------------------------------------------ partition (partition.ts:16-136)
@@ -1031,15 +1781,111 @@ std::vector<uint32_t> local_classes(const Local& l, const uint32_t* weld, std::v
   return cls;
 }
 
+// buildVirtualMesh.ts:177-181 voxelResolutions: from the finest grid down, halving, to 4 cells.
+std::vector<int> voxel_resolutions(uint32_t finest) {
+  std::vector<int> list;
+  for (int r = std::max(4, static_cast<int>(finest)); r >= 4; r /= 2) list.push_back(r);
+  return list;
+}
+
 struct Build {
   vgd_options opt;
   unsigned threads;
   std::vector<float> positions;
+  Attributes attr;  // grows with positions
   std::vector<uint32_t> weld;
   std::vector<Meshlet> all;
   std::vector<std::vector<uint32_t>> lists;  // replacement lists (indices into `all`)
   std::vector<std::vector<int32_t>> scratch; // per worker, compact()'s map
   int32_t groups_linked = 0;
+  uint32_t voxel_levels = 0;      // stats.voxelLevels
+  uint32_t aggregate_groups = 0;  // stats.aggregateGroups
+  // The full-detail source, for voxel proxies.
+  const float* src_positions = nullptr;
+  const float* src_colors = nullptr;
+  const float* src_uvs = nullptr;
+  std::vector<uint32_t> src_indices;
+  std::map<int, std::unique_ptr<VoxelProxy>> proxies;  // by resolution; null: none
+  std::map<int, SimplifyLog> proxy_runs;                          // by resolution
+  std::map<std::pair<int, size_t>, Simplified> proxy_simplified;  // by (resolution, target)
+
+  // buildVirtualMesh.ts:410-414 voxelProxy: the source's proxy at `resolution`, built on first use.
+  const VoxelProxy* voxel_proxy(int resolution) {
+    auto it = proxies.find(resolution);
+    if (it == proxies.end()) {
+      auto p = std::make_unique<VoxelProxy>();
+      if (!build_voxel_proxy(src_positions, src_colors, src_uvs, src_indices, resolution, *p)) p.reset();
+      it = proxies.emplace(resolution, std::move(p)).first;
+    }
+    return it->second.get();
+  }
+
+  // A proxy simplified to `target` indices with borders free (meshopt ErrorAbsolute). One complete run per
+  // proxy is recorded and replayed to each target the level loop and the tail ask for.
+  const Simplified& simplified_proxy(int resolution, const VoxelProxy& proxy, size_t target) {
+    auto key = std::make_pair(resolution, target);
+    auto it = proxy_simplified.find(key);
+    if (it == proxy_simplified.end()) {
+      auto run = proxy_runs.find(resolution);
+      if (run == proxy_runs.end()) {
+        std::vector<uint32_t> cls = position_remap(proxy.positions.data(), static_cast<uint32_t>(proxy.positions.size() / 3));
+        run = proxy_runs.emplace(resolution, SimplifyLog{}).first;
+        simplify(proxy.indices, proxy.positions, cls, 0, false, false, &run->second);
+      }
+      it = proxy_simplified.emplace(key, replay(run->second, target)).first;
+    }
+    return it->second;
+  }
+
+  // The cell size a proxy at `resolution` has (build_voxel_proxy's own arithmetic): a proxy's error is at
+  // least VOXEL_ERROR_CELLS cells, so proxies that cannot beat a known error are neither built nor simplified.
+  double src_extent = 0;
+  double cell_of(int resolution) const { return src_extent / resolution; }
+
+  // buildVirtualMesh.ts:298-309 voxelError: the smallest error of a simplified voxel proxy with at most half
+  // of `triangles` (infinite when none gets there). Exact whenever it is at most `cap`; above it, some value
+  // above `cap` (proxies whose least possible error exceeds both `cap` and the best so far are skipped).
+  double voxel_error(size_t triangles, double cap) {
+    const size_t target = std::max<size_t>(3, (triangles / 2) * 3);
+    double best = INFINITY;
+    for (int resolution : voxel_resolutions(opt.voxel_resolution)) {
+      if (kVoxelErrorCells * cell_of(resolution) > std::min(best, cap)) continue;
+      const VoxelProxy* proxy = voxel_proxy(resolution);
+      if (!proxy) continue;
+      const Simplified& s = simplified_proxy(resolution, *proxy, std::min(target, proxy->indices.size()));
+      if (!s.indices.empty() && s.indices.size() <= target) best = std::min(best, kVoxelErrorCells * proxy->cell + s.error);
+    }
+    return best;
+  }
+
+  // buildVirtualMesh.ts:388-407 appendVertices: positions (rounded to binary32) and their attributes join
+  // the shared buffers; each new vertex is its own weld class. Returns the first new id.
+  uint32_t append_vertices(const std::vector<double>& p, const float* normals, const float* colors, const float* uvs) {
+    const uint32_t first = static_cast<uint32_t>(positions.size() / 3);
+    const size_t count = p.size() / 3;
+    for (double x : p) positions.push_back(static_cast<float>(x));
+    if (!attr.normals.empty()) attr.normals.insert(attr.normals.end(), normals, normals + count * 3);
+    if (!attr.colors.empty()) {
+      if (colors) attr.colors.insert(attr.colors.end(), colors, colors + count * 3);
+      else attr.colors.insert(attr.colors.end(), count * 3, 1.0f);
+    }
+    if (!attr.uvs.empty()) {
+      if (uvs) attr.uvs.insert(attr.uvs.end(), uvs, uvs + count * 2);
+      else attr.uvs.insert(attr.uvs.end(), count * 2, 0.0f);
+    }
+    for (size_t i = 0; i < count; i++) weld.push_back(first + static_cast<uint32_t>(i));
+    for (auto& sc : scratch) sc.resize(positions.size() / 3, -1);
+    return first;
+  }
+
+  // Attributes of existing vertices, gathered (for the copies a thinned piece adds).
+  static std::vector<float> gather(const std::vector<float>& a, const std::vector<uint32_t>& ids, int size) {
+    std::vector<float> out;
+    if (a.empty()) return out;
+    out.reserve(ids.size() * size);
+    for (uint32_t v : ids) out.insert(out.end(), a.begin() + static_cast<size_t>(v) * size, a.begin() + static_cast<size_t>(v + 1) * size);
+    return out;
+  }
 
   // buildVirtualMesh.ts:453-482 splitIntoMeshlets: meshlets of a triangle list, not yet added to `all`.
   std::vector<Meshlet> split(const std::vector<uint32_t>& indices, uint32_t level, const Sphere* lod, double error,
@@ -1100,11 +1946,34 @@ struct Build {
     for (uint32_t c : children) all[c].replacements = list;
   }
 
-  // buildVirtualMesh.ts:211-294: the level loop; returns lodLevels.
+  // buildVirtualMesh.ts:326-339 groupOwners: per welded vertex, the group whose meshlets use it, -2 when
+  // meshlets of several groups do, -1 when unused.
+  std::vector<int32_t> group_owners(const std::vector<std::vector<uint32_t>>& groups, const std::vector<uint32_t>& current) const {
+    std::vector<int32_t> owner(weld.size(), -1);
+    for (size_t g = 0; g < groups.size(); g++)
+      for (uint32_t i : groups[g])
+        for (uint32_t v : all[current[i]].indices) {
+          int32_t& o = owner[weld[v]];
+          if (o == -1) o = static_cast<int32_t>(g);
+          else if (o != static_cast<int32_t>(g)) o = -2;
+        }
+    return owner;
+  }
+
+  // buildVirtualMesh.ts:211-294: the level loop; returns lodLevels. Groups are simplified in parallel; what
+  // depends on their order (vertices a thinned group appends, the voxel limit) is settled after, in group order.
   uint32_t levels(std::vector<uint32_t>& current, size_t& last_triangles) {
     uint32_t lod_levels = 1;
+    const bool voxels = opt.tail_simplify && opt.voxel_lods;
     for (uint32_t level = 1; level <= opt.max_lod_levels && current.size() > 1; level++) {
       if (opt.tail_simplify && last_triangles <= kTailStartTriangles) break;
+      // Aggregate geometry: once a voxel stand-in of the whole mesh with half this level's triangles is as
+      // accurate as the level, the tail takes over (buildVirtualMesh.ts:215-218, 311-316 voxelBeats).
+      if (voxels && aggregate_groups > 0) {
+        double level_error = 0;
+        for (uint32_t i : current) level_error = std::max(level_error, all[i].error);
+        if (voxel_error(last_triangles, level_error) <= level_error) break;
+      }
       std::vector<const std::vector<uint64_t>*> edges;
       std::vector<std::array<double, 3>> centers;
       for (uint32_t i : current) {
@@ -1112,36 +1981,71 @@ struct Build {
         centers.push_back({all[i].cull.x, all[i].cull.y, all[i].cull.z});
       }
       const auto groups = partition(edges, centers, opt.group_size);
+      std::vector<int32_t> owner;
+      if (opt.aggregate_lods) owner = group_owners(groups, current);
+      const uint32_t level_vertices = static_cast<uint32_t>(positions.size() / 3);
       struct Result {
+        std::vector<uint32_t> children, merged, global;
+        double shape = 0, child_error = 0;
+        size_t simplified = 0;  // index count of the regular simplification
+        bool has_aggregate = false;
+        Aggregate aggregate;
         bool root = true;
         Sphere lod;
         double error = 0;
         std::vector<Meshlet> outputs;
-        std::vector<uint32_t> children;
       };
       std::vector<Result> results(groups.size());
       parallel_for(groups.size(), threads, [&](size_t g, unsigned worker) {
         Result& r = results[g];
         for (uint32_t i : groups[g]) r.children.push_back(current[i]);
-        const std::vector<uint32_t> merged = concat(r.children);
-        Local local = compact(merged.data(), merged.size(), positions.data(), scratch[worker]);
-        const size_t target = std::max<size_t>(3, static_cast<size_t>(std::floor(merged.size() * opt.simplify_ratio / 3)) * 3);
+        r.merged = concat(r.children);
+        Local local = compact(r.merged.data(), r.merged.size(), positions.data(), scratch[worker]);
+        const size_t target = std::max<size_t>(3, static_cast<size_t>(std::floor(r.merged.size() * opt.simplify_ratio / 3)) * 3);
         Simplified s = simplify(local.indices, local.pos, local_classes(local, weld.data(), scratch[worker]), target, true,
                                 opt.prune != 0);
-        std::vector<uint32_t> global(s.indices.size());
-        for (size_t i = 0; i < s.indices.size(); i++) global[i] = local.to_global[s.indices[i]];
-        const double shape = std::max(s.error, extent_shrink(local.indices, s.indices, local.pos.data()));
-        if (global.size() > merged.size() * opt.max_group_keep_ratio) return;  // its meshlets stay roots
-        std::vector<const Sphere*> spheres;
-        double child_error = 0;
-        for (uint32_t c : r.children) {
-          spheres.push_back(&all[c].lod);
-          child_error = std::max(child_error, all[c].error);
+        r.global.resize(s.indices.size());
+        for (size_t i = 0; i < s.indices.size(); i++) r.global[i] = local.to_global[s.indices[i]];
+        r.simplified = s.indices.size();
+        r.shape = std::max(s.error, extent_shrink(local.indices, s.indices, local.pos.data()));
+        for (uint32_t c : r.children) r.child_error = std::max(r.child_error, all[c].error);
+        // Stalled well short of the target, as groups of separate pieces do: piece by piece (aggregate.ts).
+        if (!owner.empty() && s.indices.size() > target * 1.25)
+          r.has_aggregate = simplify_aggregate(r.merged, positions, weld, owner, static_cast<int32_t>(g), target / 3.0,
+                                               level_vertices, r.aggregate);
+      });
+      // The voxel limit is compared with each thinned group's error: it needs to be exact only up to the
+      // largest of them.
+      double voxel_limit = NAN, largest_thinned = 0;
+      for (const Result& r : results)
+        if (r.has_aggregate) largest_thinned = std::max(largest_thinned, r.child_error + r.aggregate.error);
+      for (size_t g = 0; g < groups.size(); g++) {
+        Result& r = results[g];
+        if (!r.has_aggregate) continue;
+        if (std::isnan(voxel_limit)) voxel_limit = voxels ? voxel_error(last_triangles, largest_thinned) : INFINITY;
+        Aggregate& a = r.aggregate;
+        if (!(a.indices.size() < r.simplified && r.child_error + a.error < voxel_limit)) continue;
+        uint32_t first = static_cast<uint32_t>(positions.size() / 3);
+        if (!a.new_sources.empty()) {
+          const std::vector<float> n = gather(attr.normals, a.new_sources, 3), c = gather(attr.colors, a.new_sources, 3),
+                                   u = gather(attr.uvs, a.new_sources, 2);
+          first = append_vertices(a.new_positions, n.data(), c.empty() ? nullptr : c.data(), u.empty() ? nullptr : u.data());
         }
+        for (uint32_t& v : a.indices)
+          if (v >= level_vertices) v = v - level_vertices + first;
+        r.global = std::move(a.indices);
+        r.shape = std::max(a.error, extent_shrink(r.merged, r.global, positions.data()));
+        aggregate_groups++;
+      }
+      parallel_for(groups.size(), threads, [&](size_t g, unsigned worker) {
+        Result& r = results[g];
+        if (r.global.size() > r.merged.size() * opt.max_group_keep_ratio) return;  // its meshlets stay roots
+        std::vector<const Sphere*> spheres;
+        for (uint32_t c : r.children) spheres.push_back(&all[c].lod);
         r.root = false;
         r.lod = enclosing(spheres);
-        r.error = child_error + std::max(shape, 1e-7);
-        if (!global.empty()) r.outputs = split(global, level, &r.lod, r.error, worker);
+        r.error = r.child_error + std::max(r.shape, 1e-7);
+        if (!r.global.empty()) r.outputs = split(r.global, level, &r.lod, r.error, worker);
       });
       std::vector<uint32_t> next;
       for (Result& r : results) {
@@ -1160,7 +2064,9 @@ struct Build {
     return lod_levels;
   }
 
-  // buildVirtualMesh.ts:345-437 simplifyTail without the voxel candidates.
+  // buildVirtualMesh.ts:345-437 simplifyTail: merge every root and keep halving, borders free; a simplified
+  // voxel proxy of the full-detail mesh replaces the triangles when its error (measured against the
+  // original, so it does not accumulate) is smaller.
   uint32_t tail(std::vector<uint32_t> nodes, uint32_t levels) {
     if (nodes.empty()) return levels;
     uint32_t level = levels;
@@ -1178,11 +2084,54 @@ struct Build {
         Simplified sloppy = simplify_sloppy(local.indices, local.pos, target);
         if (sloppy.indices.size() < s.indices.size()) s = std::move(sloppy);
       }
-      if (!(s.indices.size() <= keep_limit)) break;  // parentError stays infinite
-      const double parent_error =
-          child_error + std::max(std::max(s.error, extent_shrink(local.indices, s.indices, local.pos.data())), 1e-7);
-      std::vector<uint32_t> global(s.indices.size());
-      for (size_t i = 0; i < s.indices.size(); i++) global[i] = local.to_global[s.indices[i]];
+      double parent_error = INFINITY;
+      std::vector<uint32_t> global;
+      if (s.indices.size() <= keep_limit) {
+        parent_error =
+            child_error + std::max(std::max(s.error, extent_shrink(local.indices, s.indices, local.pos.data())), 1e-7);
+        global.resize(s.indices.size());
+        for (size_t i = 0; i < s.indices.size(); i++) global[i] = local.to_global[s.indices[i]];
+      }
+      const VoxelProxy* voxel = nullptr;
+      const Simplified* voxel_tris = nullptr;
+      if (opt.voxel_lods) {
+        for (int resolution : voxel_resolutions(opt.voxel_resolution)) {
+          if (kVoxelErrorCells * cell_of(resolution) >= parent_error) continue;  // cannot win
+          const VoxelProxy* proxy = voxel_proxy(resolution);
+          if (!proxy) continue;
+          const Simplified& p = simplified_proxy(resolution, *proxy, std::min(target, proxy->indices.size()));
+          if (p.indices.empty() || p.indices.size() > keep_limit) continue;
+          const double e = std::max(child_error * (1 + 1e-5) + 1e-7, kVoxelErrorCells * proxy->cell + p.error);
+          if (e < parent_error) {
+            parent_error = e;
+            voxel = proxy;
+            voxel_tris = &p;
+          }
+        }
+      }
+      if (!std::isfinite(parent_error)) break;  // parentError stays infinite
+      if (voxel) {
+        // The proxy vertices this level uses join the shared buffers, in first-use order.
+        std::vector<int32_t> map(voxel->positions.size() / 3, -1);
+        std::vector<uint32_t> used;
+        global.resize(voxel_tris->indices.size());
+        for (size_t i = 0; i < voxel_tris->indices.size(); i++) {
+          int32_t& m = map[voxel_tris->indices[i]];
+          if (m < 0) {
+            m = static_cast<int32_t>(used.size());
+            used.push_back(voxel_tris->indices[i]);
+          }
+          global[i] = static_cast<uint32_t>(m);
+        }
+        std::vector<double> p;
+        for (uint32_t v : used)
+          for (int k = 0; k < 3; k++) p.push_back(voxel->positions[static_cast<size_t>(v) * 3 + k]);
+        const std::vector<float> n = gather(voxel->normals, used, 3), c = gather(voxel->colors, used, 3),
+                                 u = gather(voxel->uvs, used, 2);
+        const uint32_t first = append_vertices(p, n.data(), c.empty() ? nullptr : c.data(), u.empty() ? nullptr : u.data());
+        for (uint32_t& v : global) v += first;
+        voxel_levels++;
+      }
       std::vector<const Sphere*> spheres;
       for (uint32_t n : nodes) spheres.push_back(&all[n].lod);
       const Sphere lod = enclosing(spheres);
@@ -1201,7 +2150,8 @@ struct Build {
 }  // namespace
 
 struct vgd_mesh {
-  std::vector<float> positions;
+  std::vector<float> positions, normals, colors, uvs;
+  uint32_t voxel_levels = 0, aggregate_groups = 0;
   std::vector<uint32_t> indices;
   std::vector<float> bounds;
   std::vector<uint32_t> info;
@@ -1346,28 +2296,47 @@ extern "C" void vgd_default_options(vgd_options* o) {
   o->tail_simplify = 1;
   o->min_root_triangles = 4;
   o->threads = 0;
+  o->voxel_lods = 1;
+  o->voxel_resolution = 64;
+  o->aggregate_lods = 1;
 }
 
-extern "C" int vgd_build(const float* positions, uint32_t vertex_count, const uint32_t* indices, uint64_t index_count,
-                         const vgd_options* options, vgd_mesh** out) {
+extern "C" int vgd_build_source(const vgd_source* src, const vgd_options* options, vgd_mesh** out) {
   *out = nullptr;
-  if (!positions || !indices || index_count == 0 || index_count % 3 != 0 || vertex_count == 0 ||
-      vertex_count >= (1u << 26))
+  if (!src || !src->positions || !src->indices || src->index_count == 0 || src->index_count % 3 != 0 ||
+      src->vertex_count == 0 || src->vertex_count >= (1u << 26))
     return -1;
-  for (uint64_t i = 0; i < index_count; i++)
-    if (indices[i] >= vertex_count) return -1;
+  const uint32_t vertex_count = src->vertex_count;
+  for (uint64_t i = 0; i < src->index_count; i++)
+    if (src->indices[i] >= vertex_count) return -1;
   try {
     Build b;
     vgd_default_options(&b.opt);
     if (options) b.opt = *options;
-    if (b.opt.group_size < 2) return -1;
+    if (b.opt.group_size < 2 || (b.opt.voxel_lods && b.opt.voxel_resolution == 0)) return -1;
     b.threads = b.opt.threads ? b.opt.threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    b.positions.assign(positions, positions + static_cast<size_t>(vertex_count) * 3);
-    b.weld = position_remap(positions, vertex_count);
+    const size_t nv = vertex_count;
+    b.positions.assign(src->positions, src->positions + nv * 3);
+    if (src->normals) b.attr.normals.assign(src->normals, src->normals + nv * 3);
+    if (src->colors) b.attr.colors.assign(src->colors, src->colors + nv * 3);
+    if (src->uvs) b.attr.uvs.assign(src->uvs, src->uvs + nv * 2);
+    b.weld = position_remap(src->positions, vertex_count);
     b.scratch.assign(b.threads, std::vector<int32_t>(vertex_count, -1));
+    b.src_positions = src->positions;
+    b.src_colors = src->colors;
+    b.src_uvs = src->uvs;
+    b.src_indices.assign(src->indices, src->indices + src->index_count);
+    {
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t i : b.src_indices)
+        for (int k = 0; k < 3; k++) {
+          lo[k] = std::min(lo[k], static_cast<double>(src->positions[static_cast<size_t>(i) * 3 + k]));
+          hi[k] = std::max(hi[k], static_cast<double>(src->positions[static_cast<size_t>(i) * 3 + k]));
+        }
+      b.src_extent = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    }
     // buildVirtualMesh.ts:207-210: the leaf level.
-    std::vector<uint32_t> src(indices, indices + index_count);
-    std::vector<uint32_t> current = b.append(b.split(src, 0, nullptr, 0.0, 0));
+    std::vector<uint32_t> current = b.append(b.split(b.src_indices, 0, nullptr, 0.0, 0));
     size_t last_triangles = Build::count_triangles(b.all, current);
     uint32_t lod_levels = b.levels(current, last_triangles);
     if (b.opt.tail_simplify) {
@@ -1378,12 +2347,23 @@ extern "C" int vgd_build(const float* positions, uint32_t vertex_count, const ui
     }
     auto m = std::make_unique<vgd_mesh>();
     m->positions = std::move(b.positions);
+    m->normals = std::move(b.attr.normals);
+    m->colors = std::move(b.attr.colors);
+    m->uvs = std::move(b.attr.uvs);
+    m->voxel_levels = b.voxel_levels;
+    m->aggregate_groups = b.aggregate_groups;
     pack(b, lod_levels, *m);
     *out = m.release();
     return 0;
   } catch (...) {
     return -2;
   }
+}
+
+extern "C" int vgd_build(const float* positions, uint32_t vertex_count, const uint32_t* indices, uint64_t index_count,
+                         const vgd_options* options, vgd_mesh** out) {
+  vgd_source src{vertex_count, positions, nullptr, nullptr, nullptr, index_count, indices};
+  return vgd_build_source(&src, options, out);
 }
 
 extern "C" void vgd_mesh_get(const vgd_mesh* m, vgd_mesh_view* v) {
@@ -1413,9 +2393,48 @@ extern "C" void vgd_mesh_get(const vgd_mesh* m, vgd_mesh_view* v) {
   v->root_triangles = m->root_tris;
   v->root_meshlets = m->root_meshlets;
   v->lod_levels = m->lod_levels;
+  v->normals = m->normals.empty() ? nullptr : m->normals.data();
+  v->colors = m->colors.empty() ? nullptr : m->colors.data();
+  v->uvs = m->uvs.empty() ? nullptr : m->uvs.data();
+  v->voxel_levels = m->voxel_levels;
+  v->aggregate_groups = m->aggregate_groups;
 }
 
 extern "C" void vgd_mesh_free(vgd_mesh* m) { delete m; }
+
+struct vgd_proxy {
+  VoxelProxy p;
+};
+
+extern "C" int vgd_voxel_proxy(const float* positions, uint32_t vertex_count, const float* colors, const float* uvs,
+                               const uint32_t* indices, uint64_t index_count, uint32_t resolution, vgd_proxy** out) {
+  *out = nullptr;
+  if (!positions || (index_count && !indices) || index_count % 3 != 0 || resolution == 0 || resolution > 4096) return -1;
+  for (uint64_t i = 0; i < index_count; i++)
+    if (indices[i] >= vertex_count) return -1;
+  try {
+    auto m = std::make_unique<vgd_proxy>();
+    std::vector<uint32_t> idx(indices, indices + index_count);
+    if (!build_voxel_proxy(positions, colors, uvs, idx, static_cast<int>(resolution), m->p)) return 1;
+    *out = m.release();
+    return 0;
+  } catch (...) {
+    return -2;
+  }
+}
+
+extern "C" void vgd_proxy_get(const vgd_proxy* m, vgd_proxy_view* v) {
+  v->vertex_count = static_cast<uint32_t>(m->p.positions.size() / 3);
+  v->positions = m->p.positions.data();
+  v->normals = m->p.normals.data();
+  v->colors = m->p.colors.empty() ? nullptr : m->p.colors.data();
+  v->uvs = m->p.uvs.empty() ? nullptr : m->p.uvs.data();
+  v->index_count = m->p.indices.size();
+  v->indices = m->p.indices.data();
+  v->cell_size = m->p.cell;
+}
+
+extern "C" void vgd_proxy_free(vgd_proxy* m) { delete m; }
 
 extern "C" uint32_t vgd_partition(uint32_t count, const uint64_t* edges, const uint32_t* edge_offset,
                                   const uint32_t* edge_count, const double* centers, uint32_t group_size,

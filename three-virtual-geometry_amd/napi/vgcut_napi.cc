@@ -538,39 +538,84 @@ static napi_value copy_typed(napi_env env, napi_typedarray_type type, const void
   return out;
 }
 
-// buildVirtualMesh(positions: Float32Array, indices: Uint32Array, options?) -> VirtualMeshData
-// (buildVirtualMesh.ts:185; options with the reference's names, voxelLods / aggregateLods only false)
+// buildVirtualMesh(src, options?) -> VirtualMeshData, src = {positions, indices, normals?, colors?, uvs?}
+// (buildVirtualMesh.ts:185 with VirtualMeshSource), or buildVirtualMesh(positions, indices, options?);
+// options take the reference's names.
 static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) {
+    napi_throw_type_error(env, nullptr, "buildVirtualMesh: expected a source");
+    return nullptr;
+  }
+  bool is_typed = false;
+  CHECK_NAPI(napi_is_typedarray(env, argv[0], &is_typed));
+  napi_value src_obj = nullptr, options = nullptr;
+  napi_value pos_v = argv[0], idx_v = argc > 1 ? argv[1] : nullptr;
+  if (!is_typed) {
+    napi_valuetype t;
+    CHECK_NAPI(napi_typeof(env, argv[0], &t));
+    if (t != napi_object) {
+      napi_throw_type_error(env, nullptr, "buildVirtualMesh: expected a source object or typed arrays");
+      return nullptr;
+    }
+    src_obj = argv[0];
+    pos_v = prop(env, src_obj, "positions");
+    idx_v = prop(env, src_obj, "indices");
+    if (argc > 1) options = argv[1];
+  } else if (argc > 2) {
+    options = argv[2];
+  }
+  if (!idx_v) {
+    napi_throw_type_error(env, nullptr, "buildVirtualMesh: indices missing");
+    return nullptr;
+  }
   size_t np = 0, ni = 0;
   bool bad = false;
-  const float* positions = typed<float>(env, argv[0], &np, &bad, "positions");
+  const float* positions = typed<float>(env, pos_v, &np, &bad, "positions");
   if (bad) return nullptr;
-  const uint32_t* indices = typed<uint32_t>(env, argv[1], &ni, &bad, "indices");
+  const uint32_t* indices = typed<uint32_t>(env, idx_v, &ni, &bad, "indices");
   if (bad) return nullptr;
   if (!positions || !indices || np % 3 || ni % 3) {
     napi_throw_type_error(env, nullptr, "buildVirtualMesh: positions must be a Float32Array of 3 per vertex, indices a Uint32Array of 3 per triangle");
     return nullptr;
   }
+  const size_t nv = np / 3;
+  const float* attr[3] = {nullptr, nullptr, nullptr};
+  if (src_obj) {
+    const char* names[3] = {"normals", "colors", "uvs"};
+    const size_t per[3] = {3, 3, 2};
+    for (int a = 0; a < 3; a++) {
+      napi_value v = prop(env, src_obj, names[a]);
+      napi_valuetype t;
+      if (!v || napi_typeof(env, v, &t) != napi_ok || t == napi_undefined || t == napi_null) continue;
+      size_t n = 0;
+      attr[a] = typed<float>(env, v, &n, &bad, names[a]);
+      if (bad) return nullptr;
+      if (!attr[a] || n != nv * per[a]) {
+        napi_throw_range_error(env, nullptr, "buildVirtualMesh: an attribute does not cover every vertex");
+        return nullptr;
+      }
+    }
+  }
   vgd_options o;
   vgd_default_options(&o);
-  if (argc > 2) {
+  if (options) {
     napi_valuetype t;
-    CHECK_NAPI(napi_typeof(env, argv[2], &t));
+    CHECK_NAPI(napi_typeof(env, options, &t));
     if (t == napi_object) {
       auto u32 = [&](const char* name, uint32_t* out) {
         uint32_t x;
-        if (napi_get_value_uint32(env, prop(env, argv[2], name), &x) == napi_ok) *out = x;
+        if (napi_get_value_uint32(env, prop(env, options, name), &x) == napi_ok) *out = x;
       };
       auto f64 = [&](const char* name, double* out) {
         double x;
-        if (napi_get_value_double(env, prop(env, argv[2], name), &x) == napi_ok) *out = x;
+        if (napi_get_value_double(env, prop(env, options, name), &x) == napi_ok) *out = x;
       };
       auto flag = [&](const char* name, int32_t* out) {
         bool x;
-        if (napi_get_value_bool(env, prop(env, argv[2], name), &x) == napi_ok) *out = x ? 1 : 0;
+        if (napi_get_value_bool(env, prop(env, options, name), &x) == napi_ok) *out = x ? 1 : 0;
       };
       u32("groupSize", &o.group_size);
       u32("maxLodLevels", &o.max_lod_levels);
@@ -581,17 +626,14 @@ static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
       flag("tailSimplify", &o.tail_simplify);
       u32("minRootTriangles", &o.min_root_triangles);
       u32("threads", &o.threads);
-      for (const char* unbuilt : {"voxelLods", "aggregateLods"}) {
-        bool on = false;
-        if (napi_get_value_bool(env, prop(env, argv[2], unbuilt), &on) == napi_ok && on) {
-          napi_throw_error(env, nullptr, "buildVirtualMesh: voxel and aggregate levels are not supported");
-          return nullptr;
-        }
-      }
+      flag("voxelLods", &o.voxel_lods);
+      u32("voxelResolution", &o.voxel_resolution);
+      flag("aggregateLods", &o.aggregate_lods);
     }
   }
+  vgd_source src{static_cast<uint32_t>(nv), positions, attr[0], attr[1], attr[2], ni, indices};
   vgd_mesh* mesh = nullptr;
-  if (int rc = vgd_build(positions, static_cast<uint32_t>(np / 3), indices, ni, &o, &mesh)) {
+  if (int rc = vgd_build_source(&src, &o, &mesh)) {
     napi_throw_error(env, nullptr, rc == -1 ? "buildVirtualMesh: invalid mesh" : "buildVirtualMesh: build failed");
     return nullptr;
   }
@@ -602,6 +644,11 @@ static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
   auto put = [&](napi_value obj, const char* name, napi_value val) { napi_set_named_property(env, obj, name, val); };
   const size_t n = v.meshlet_count;
   put(out, "positions", copy_typed(env, napi_float32_array, v.positions, static_cast<size_t>(v.vertex_count) * 3, 4));
+  napi_value null_value;
+  napi_get_null(env, &null_value);
+  put(out, "normals", v.normals ? copy_typed(env, napi_float32_array, v.normals, static_cast<size_t>(v.vertex_count) * 3, 4) : null_value);
+  put(out, "colors", v.colors ? copy_typed(env, napi_float32_array, v.colors, static_cast<size_t>(v.vertex_count) * 3, 4) : null_value);
+  put(out, "uvs", v.uvs ? copy_typed(env, napi_float32_array, v.uvs, static_cast<size_t>(v.vertex_count) * 2, 4) : null_value);
   put(out, "indices", copy_typed(env, napi_uint32_array, v.indices, v.index_count, 4));
   napi_create_uint32(env, v.meshlet_count, &x);
   put(out, "meshletCount", x);
@@ -630,6 +677,10 @@ static napi_value BuildVirtualMesh(napi_env env, napi_callback_info info) {
   put(stats, "rootMeshlets", x);
   napi_create_uint32(env, v.lod_levels, &x);
   put(stats, "lodLevels", x);
+  napi_create_uint32(env, v.voxel_levels, &x);
+  put(stats, "voxelLevels", x);
+  napi_create_uint32(env, v.aggregate_groups, &x);
+  put(stats, "aggregateGroups", x);
   put(out, "stats", stats);
   vgd_mesh_free(mesh);
   return out;

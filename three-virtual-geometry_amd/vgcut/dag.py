@@ -2,8 +2,9 @@
 
 Python face of the reference's ``buildVirtualMesh(src, options)`` (src/core/preprocess/buildVirtualMesh.ts:185)
 and ``partitionMeshlets(meshlets, groupSize)`` (src/core/preprocess/partition.ts:16), with the reference's
-option names. The builder needs no meshoptimizer; its voxel-proxy and piece-by-piece aggregate levels
-(options ``voxelLods``, ``aggregateLods``) are not built, so it behaves as the reference with both off.
+option names and defaults, including the voxel-proxy tail levels (``voxelLods``, ``voxelResolution``;
+voxelProxy.ts) and the piece-by-piece reduction of groups made of separate pieces (``aggregateLods``;
+aggregate.ts). The builder needs no meshoptimizer.
 """
 from __future__ import annotations
 
@@ -27,6 +28,21 @@ class _Options(ctypes.Structure):
         ("tail_simplify", ctypes.c_int32),
         ("min_root_triangles", ctypes.c_uint32),
         ("threads", ctypes.c_uint32),
+        ("voxel_lods", ctypes.c_int32),
+        ("voxel_resolution", ctypes.c_uint32),
+        ("aggregate_lods", ctypes.c_int32),
+    ]
+
+
+class _Source(ctypes.Structure):
+    _fields_ = [
+        ("vertex_count", ctypes.c_uint32),
+        ("positions", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("colors", ctypes.c_void_p),
+        ("uvs", ctypes.c_void_p),
+        ("index_count", ctypes.c_uint64),
+        ("indices", ctypes.c_void_p),
     ]
 
 
@@ -58,11 +74,30 @@ class _View(ctypes.Structure):
         ("root_triangles", ctypes.c_uint32),
         ("root_meshlets", ctypes.c_uint32),
         ("lod_levels", ctypes.c_uint32),
+        ("normals", ctypes.POINTER(ctypes.c_float)),
+        ("colors", ctypes.POINTER(ctypes.c_float)),
+        ("uvs", ctypes.POINTER(ctypes.c_float)),
+        ("voxel_levels", ctypes.c_uint32),
+        ("aggregate_groups", ctypes.c_uint32),
+    ]
+
+
+class _ProxyView(ctypes.Structure):
+    _fields_ = [
+        ("vertex_count", ctypes.c_uint32),
+        ("positions", ctypes.POINTER(ctypes.c_float)),
+        ("normals", ctypes.POINTER(ctypes.c_float)),
+        ("colors", ctypes.POINTER(ctypes.c_float)),
+        ("uvs", ctypes.POINTER(ctypes.c_float)),
+        ("index_count", ctypes.c_uint64),
+        ("indices", ctypes.POINTER(ctypes.c_uint32)),
+        ("cell_size", ctypes.c_double),
     ]
 
 
 _lib = None
-EXPORTS = ("vgd_default_options", "vgd_build", "vgd_mesh_get", "vgd_mesh_free", "vgd_partition")
+EXPORTS = ("vgd_default_options", "vgd_build", "vgd_build_source", "vgd_mesh_get", "vgd_mesh_free", "vgd_partition",
+           "vgd_voxel_proxy", "vgd_proxy_get", "vgd_proxy_free")
 
 
 def _load():
@@ -74,12 +109,21 @@ def _load():
         lib.vgd_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Options),
                                   ctypes.POINTER(ctypes.c_void_p)]
         lib.vgd_build.restype = ctypes.c_int
+        lib.vgd_build_source.argtypes = [ctypes.POINTER(_Source), ctypes.POINTER(_Options), ctypes.POINTER(ctypes.c_void_p)]
+        lib.vgd_build_source.restype = ctypes.c_int
         lib.vgd_mesh_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(_View)]
         lib.vgd_mesh_get.restype = None
         lib.vgd_mesh_free.argtypes = [ctypes.c_void_p]
         lib.vgd_mesh_free.restype = None
         lib.vgd_partition.argtypes = [ctypes.c_uint32] + [ctypes.c_void_p] * 4 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
         lib.vgd_partition.restype = ctypes.c_uint32
+        lib.vgd_voxel_proxy.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+        lib.vgd_voxel_proxy.restype = ctypes.c_int
+        lib.vgd_proxy_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(_ProxyView)]
+        lib.vgd_proxy_get.restype = None
+        lib.vgd_proxy_free.argtypes = [ctypes.c_void_p]
+        lib.vgd_proxy_free.restype = None
         _lib = lib
     return _lib
 
@@ -100,39 +144,57 @@ _OPTION_NAMES = {
     "tailSimplify": "tail_simplify",
     "minRootTriangles": "min_root_triangles",
     "threads": "threads",
+    "voxelLods": "voxel_lods",
+    "voxelResolution": "voxel_resolution",
+    "aggregateLods": "aggregate_lods",
 }
-_UNBUILT = ("voxelLods", "aggregateLods", "voxelResolution")
+_FLAGS = ("prune", "tail_simplify", "voxel_lods", "aggregate_lods")
 
 
-def build_virtual_mesh(positions, indices, **options) -> VirtualMeshData:
+def _attribute(values, vertices: int, size: int, name: str):
+    if values is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(-1))
+    if len(a) != vertices * size:
+        raise ValueError(f"{name} must hold {size} floats per vertex")
+    return a
+
+
+def build_virtual_mesh(positions, indices, normals=None, colors=None, uvs=None, **options) -> VirtualMeshData:
     """buildVirtualMesh (buildVirtualMesh.ts:185) over float32 xyz positions and uint32 triangle indices.
 
-    Options take the reference's names (``groupSize``, ``maxLodLevels``, ``maxGroupKeepRatio``,
-    ``simplifyRatio``, ``minLevelReduction``, ``prune``, ``tailSimplify``, ``minRootTriangles``) plus
-    ``threads``. ``voxelLods`` / ``aggregateLods`` are accepted only as False.
+    ``normals`` / ``colors`` (3 floats per vertex) and ``uvs`` (2) are optional; when given, the result
+    carries them for every vertex, including those that thinned pieces and voxel levels add. Options take
+    the reference's names (``groupSize``, ``maxLodLevels``, ``maxGroupKeepRatio``, ``simplifyRatio``,
+    ``minLevelReduction``, ``prune``, ``tailSimplify``, ``minRootTriangles``, ``voxelLods``,
+    ``voxelResolution``, ``aggregateLods``) plus ``threads``.
     """
     lib = _load()
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float32).reshape(-1))
     idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+    nv = len(pos) // 3
+    nrm = _attribute(normals, nv, 3, "normals")
+    col = _attribute(colors, nv, 3, "colors")
+    uv = _attribute(uvs, nv, 2, "uvs")
     opt = _Options()
     lib.vgd_default_options(ctypes.byref(opt))
     for k, v in options.items():
-        if k in _UNBUILT:
-            if v:
-                raise ValueError(f"{k} is not supported by this builder")
-            continue
         if k not in _OPTION_NAMES:
             raise TypeError(f"unknown option {k}")
         field = _OPTION_NAMES[k]
-        setattr(opt, field, int(v) if field in ("prune", "tail_simplify") else v)
+        setattr(opt, field, int(bool(v)) if field in _FLAGS else v)
+    src = _Source(nv, pos.ctypes.data, nrm.ctypes.data if nrm is not None else None,
+                  col.ctypes.data if col is not None else None, uv.ctypes.data if uv is not None else None,
+                  len(idx), idx.ctypes.data)
     handle = ctypes.c_void_p()
-    rc = lib.vgd_build(pos.ctypes.data, len(pos) // 3, idx.ctypes.data, len(idx), ctypes.byref(opt), ctypes.byref(handle))
+    rc = lib.vgd_build_source(ctypes.byref(src), ctypes.byref(opt), ctypes.byref(handle))
     if rc != 0:
         raise ValueError(f"vgd_build failed ({rc})")
     try:
         v = _View()
         lib.vgd_mesh_get(handle, ctypes.byref(v))
         n, levels, groups = v.meshlet_count, v.level_count, v.group_count
+        nv_out = v.vertex_count
         start = _arr(v.replacement_start, groups + 1, np.uint32)
         return VirtualMeshData(
             meshletCount=n,
@@ -156,9 +218,47 @@ def build_virtual_mesh(positions, indices, **options) -> VirtualMeshData:
             indices=_arr(v.indices, int(v.index_count), np.uint32),
             meshletVertices=_arr(v.meshlet_vertices, int(v.meshlet_vertex_count), np.uint32),
             lodLevels=int(v.lod_levels),
+            normals=_arr(v.normals, nv_out * 3, np.float32) if v.normals else None,
+            colors=_arr(v.colors, nv_out * 3, np.float32) if v.colors else None,
+            uvs=_arr(v.uvs, nv_out * 2, np.float32) if v.uvs else None,
+            voxelLevels=int(v.voxel_levels),
+            aggregateGroups=int(v.aggregate_groups),
         )
     finally:
         lib.vgd_mesh_free(handle)
+
+
+def build_voxel_proxy(positions, colors, indices, resolution: int, uvs=None):
+    """buildVoxelProxy (voxelProxy.ts:33): a dict of positions, normals, colors, uvs, indices and cellSize, or None
+    when there is no proxy."""
+    lib = _load()
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float32).reshape(-1))
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+    nv = len(pos) // 3
+    col = _attribute(colors, nv, 3, "colors")
+    uv = _attribute(uvs, nv, 2, "uvs")
+    handle = ctypes.c_void_p()
+    rc = lib.vgd_voxel_proxy(pos.ctypes.data, nv, col.ctypes.data if col is not None else None,
+                             uv.ctypes.data if uv is not None else None, idx.ctypes.data, len(idx), int(resolution),
+                             ctypes.byref(handle))
+    if rc == 1:
+        return None
+    if rc != 0:
+        raise ValueError(f"vgd_voxel_proxy failed ({rc})")
+    try:
+        v = _ProxyView()
+        lib.vgd_proxy_get(handle, ctypes.byref(v))
+        n = v.vertex_count
+        return {
+            "positions": _arr(v.positions, n * 3, np.float32),
+            "normals": _arr(v.normals, n * 3, np.float32),
+            "colors": _arr(v.colors, n * 3, np.float32) if v.colors else None,
+            "uvs": _arr(v.uvs, n * 2, np.float32) if v.uvs else None,
+            "indices": _arr(v.indices, int(v.index_count), np.uint32),
+            "cellSize": float(v.cell_size),
+        }
+    finally:
+        lib.vgd_proxy_free(handle)
 
 
 def edge_key(a: int, b: int) -> int:

@@ -98,7 +98,8 @@ class VirtualMeshData:
 
     Mirrors src/core/preprocess/buildVirtualMesh.ts:75-138. ``meshletTriangles`` (the local vertex ids
     the expand pass writes as indices) is always produced; positions, the flat index list and meshlet
-    vertex lists come from the DAG builder (``vgcut.dag``) only. Normals, colors and uvs are the caller's.
+    vertex lists come from the DAG builder (``vgcut.dag``) only, and with them the normals, colors and uvs
+    the builder was given (extended over the vertices its levels add).
     """
 
     meshletCount: int
@@ -124,6 +125,11 @@ class VirtualMeshData:
     indices: Optional[np.ndarray] = None  # uint32, meshletInfo[1] is each meshlet's first
     meshletVertices: Optional[np.ndarray] = None  # uint32
     lodLevels: int = 0
+    normals: Optional[np.ndarray] = None  # float32 [vertices * 3]
+    colors: Optional[np.ndarray] = None  # float32 [vertices * 3]
+    uvs: Optional[np.ndarray] = None  # float32 [vertices * 2]
+    voxelLevels: int = 0  # stats.voxelLevels
+    aggregateGroups: int = 0  # stats.aggregateGroups
 
     @property
     def levelCount(self) -> int:
