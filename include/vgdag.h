@@ -17,6 +17,7 @@
  * add vertices after the input ones; their attributes are returned with the positions.
  */
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -124,6 +125,29 @@ int vgd_voxel_proxy(const float* positions, uint32_t vertex_count, const float* 
                     const uint32_t* indices, uint64_t index_count, uint32_t resolution, vgd_proxy** out);
 void vgd_proxy_get(const vgd_proxy* proxy, vgd_proxy_view* view);
 void vgd_proxy_free(vgd_proxy* proxy);
+
+/*
+ * meshoptimizer's vertex and index buffer codecs (three-virtual-geometry_amd/csrc/vgcodec.cpp), the
+ * encodings 'meshopt-vertex' and 'meshlet-triangles' of the reference's .vgeo files
+ * (src/core/io/serialize.ts:232-341, 353-381, through meshoptimizer 1.3.0's MeshoptEncoder /
+ * MeshoptDecoder). Decoders read the format's versions 0 and 1 and apply its checks; encoders write
+ * version 1 with choices of their own (valid streams, not meshoptimizer's exact bytes).
+ *
+ * vgd_vertex_decode replaces MeshoptDecoder.decodeVertexBuffer(target, count, size, source): 0, or -1
+ * (unsupported header / size), -2 (truncated or malformed), -3 (trailing bytes). vertex_size is a
+ * multiple of 4 up to 256.
+ * vgd_vertex_encode replaces MeshoptEncoder.encodeVertexBufferLevel(source, count, size, level, 1):
+ * the bytes written to `out`, 0 when `capacity` is below vgd_vertex_encode_bound.
+ * vgd_index_decode replaces MeshoptDecoder.decodeIndexBuffer(target, count, 4, source) (same codes);
+ * vgd_index_encode replaces MeshoptEncoder.encodeIndexBuffer(source, count, 4): the bytes written, 0
+ * when `capacity` is below vgd_index_encode_bound. A decoded triangle may have its corners rotated.
+ */
+size_t vgd_vertex_encode_bound(size_t vertex_count, size_t vertex_size);
+size_t vgd_vertex_encode(uint8_t* out, size_t capacity, const void* vertices, size_t vertex_count, size_t vertex_size);
+int vgd_vertex_decode(void* out, size_t vertex_count, size_t vertex_size, const uint8_t* buffer, size_t size);
+size_t vgd_index_encode_bound(size_t index_count, size_t vertex_count);
+size_t vgd_index_encode(uint8_t* out, size_t capacity, const uint32_t* indices, size_t index_count);
+int vgd_index_decode(uint32_t* out, size_t index_count, const uint8_t* buffer, size_t size);
 
 /*
  * Replaces partitionMeshlets(meshlets, groupSize) (partition.ts:16-97), exported by the reference's

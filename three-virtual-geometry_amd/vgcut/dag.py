@@ -97,7 +97,8 @@ class _ProxyView(ctypes.Structure):
 
 _lib = None
 EXPORTS = ("vgd_default_options", "vgd_build", "vgd_build_source", "vgd_mesh_get", "vgd_mesh_free", "vgd_partition",
-           "vgd_voxel_proxy", "vgd_proxy_get", "vgd_proxy_free")
+           "vgd_voxel_proxy", "vgd_proxy_get", "vgd_proxy_free", "vgd_vertex_encode_bound", "vgd_vertex_encode",
+           "vgd_vertex_decode", "vgd_index_encode_bound", "vgd_index_encode", "vgd_index_decode")
 
 
 def _load():
@@ -124,6 +125,19 @@ def _load():
         lib.vgd_proxy_get.restype = None
         lib.vgd_proxy_free.argtypes = [ctypes.c_void_p]
         lib.vgd_proxy_free.restype = None
+        sz, vp = ctypes.c_size_t, ctypes.c_void_p
+        lib.vgd_vertex_encode_bound.argtypes = [sz, sz]
+        lib.vgd_vertex_encode_bound.restype = sz
+        lib.vgd_vertex_encode.argtypes = [vp, sz, vp, sz, sz]
+        lib.vgd_vertex_encode.restype = sz
+        lib.vgd_vertex_decode.argtypes = [vp, sz, sz, vp, sz]
+        lib.vgd_vertex_decode.restype = ctypes.c_int
+        lib.vgd_index_encode_bound.argtypes = [sz, sz]
+        lib.vgd_index_encode_bound.restype = sz
+        lib.vgd_index_encode.argtypes = [vp, sz, vp, sz]
+        lib.vgd_index_encode.restype = sz
+        lib.vgd_index_decode.argtypes = [vp, sz, vp, sz]
+        lib.vgd_index_decode.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -259,6 +273,56 @@ def build_voxel_proxy(positions, colors, indices, resolution: int, uvs=None):
         }
     finally:
         lib.vgd_proxy_free(handle)
+
+
+def encode_vertex_buffer(raw: bytes, count: int, stride: int) -> bytes:
+    """MeshoptEncoder.encodeVertexBufferLevel(raw, count, stride, 2, 1): a version-1 vertex-codec stream."""
+    lib = _load()
+    if len(raw) != count * stride:
+        raise ValueError("raw size is not count x stride")
+    cap = lib.vgd_vertex_encode_bound(count, stride)
+    if cap == 0:
+        raise ValueError(f"unsupported vertex size {stride}")
+    out = ctypes.create_string_buffer(cap)
+    src = ctypes.create_string_buffer(bytes(raw), max(len(raw), 1))
+    n = lib.vgd_vertex_encode(out, cap, src, count, stride)
+    if n == 0:
+        raise ValueError("vertex encoding failed")
+    return out.raw[:n]
+
+
+def decode_vertex_buffer(blob: bytes, count: int, stride: int) -> bytes:
+    """MeshoptDecoder.decodeVertexBuffer: `count` x `stride` bytes; raises ValueError with the codec's code."""
+    lib = _load()
+    out = ctypes.create_string_buffer(max(count * stride, 1))
+    src = ctypes.create_string_buffer(bytes(blob), max(len(blob), 1))
+    rc = lib.vgd_vertex_decode(out, count, stride, src, len(blob))
+    if rc != 0:
+        raise ValueError(f"vertex codec error {rc}")
+    return out.raw[:count * stride]
+
+
+def encode_index_buffer(indices) -> bytes:
+    """MeshoptEncoder.encodeIndexBuffer(indices, count, 4): a version-1 index-codec stream."""
+    lib = _load()
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+    cap = lib.vgd_index_encode_bound(len(idx), 0)
+    out = ctypes.create_string_buffer(cap)
+    n = lib.vgd_index_encode(out, cap, idx.ctypes.data, len(idx))
+    if n == 0:
+        raise ValueError("index encoding failed")
+    return out.raw[:n]
+
+
+def decode_index_buffer(blob: bytes, count: int) -> np.ndarray:
+    """MeshoptDecoder.decodeIndexBuffer(target, count, 4, blob): uint32 indices (corners possibly rotated)."""
+    lib = _load()
+    out = np.zeros(max(count, 1), dtype=np.uint32)
+    src = ctypes.create_string_buffer(bytes(blob), max(len(blob), 1))
+    rc = lib.vgd_index_decode(out.ctypes.data, count, src, len(blob))
+    if rc != 0:
+        raise ValueError(f"index codec error {rc}")
+    return out[:count]
 
 
 def edge_key(a: int, b: int) -> int:

@@ -6,11 +6,15 @@ checksum of everything after the 16-byte prefix), the UTF-8 JSON header space-pa
 fields (name, array type, length, encoding, payload offset and size) and every other value as JSON, with
 non-finite numbers written as {"$number": "Infinity"} (serialize.ts:454-468).
 
-Encodings read here: 'raw', 'raw' with `deflate` (raw deflate stream, serialize.ts:442-452), and
-'meshlet-indices' (`indices` rebuilt from meshletInfo / meshletVertices / meshletTriangles,
-serialize.ts:383-404). The meshoptimizer codecs ('meshopt-vertex', 'meshlet-triangles') are not built:
-a file using them is refused with a VirtualMeshFormatError that says so. The writer emits 'raw' (deflated
-where that is smaller) and 'meshlet-indices', so every file it writes is one the reference reads.
+Encodings: 'raw', any of them with `deflate` (raw deflate stream, serialize.ts:442-452), 'meshlet-indices'
+(`indices` rebuilt from meshletInfo / meshletVertices / meshletTriangles, serialize.ts:383-404),
+'meshopt-vertex' (meshoptimizer's vertex codec over `stride`-byte elements, serialize.ts:244-245, 338-341) and
+'meshlet-triangles' (meshletTriangles as one index buffer over meshlet-vertex slots through meshoptimizer's
+index codec, plus 2 bits per triangle of corner rotation, serialize.ts:257-296, 353-381). The codecs are
+csrc/vgcodec.cpp (through vgcut.dag). The writer follows encodeField (serialize.ts:232-262): with `compress`,
+each field's candidates (raw, the vertex codec at the field's hinted strides, or meshlet-triangles) are
+ordered by size, the best one (and raw, where hinted) is also tried deflated, and the first candidate that
+decodes back to the same bytes is stored.
 """
 from __future__ import annotations
 
@@ -145,18 +149,170 @@ def decode_virtual_mesh(data: Union[bytes, bytearray, memoryview]) -> Dict[str, 
             if any(a is None or a.dtype != np.uint32 for a in need):
                 raise VirtualMeshFormatError(f'geometry file is corrupt ("{f["name"]}" needs the meshlet arrays)')
             arrays[f["name"]] = rebuild_indices(f["length"], *need)
-        elif enc in ("meshopt-vertex", "meshlet-triangles"):
-            raise VirtualMeshFormatError(f'field "{f["name"]}" uses the meshoptimizer codec "{enc}", which this reader does not '
-                                         "decode; re-bake the file with compress: false")
+        elif enc == "meshopt-vertex":
+            stride = f.get("stride") or 0
+            nbytes = f["length"] * dtype.itemsize
+            if not isinstance(stride, int) or stride <= 0 or stride % 4 or nbytes % stride:
+                raise VirtualMeshFormatError(f'geometry file is corrupt (field "{f["name"]}" stride)')
+            try:
+                raw = _dag().decode_vertex_buffer(blob, nbytes // stride, stride)
+            except ValueError as e:
+                raise VirtualMeshFormatError(f'geometry file is corrupt (field "{f["name"]}": {e})')
+            arrays[f["name"]] = np.frombuffer(raw, dtype=dtype).astype(dtype.newbyteorder("="), copy=True)
+        elif enc == "meshlet-triangles":
+            info = arrays.get("meshletInfo")
+            if info is None or info.dtype != np.uint32:
+                raise VirtualMeshFormatError(f'geometry file is corrupt ("{f["name"]}" needs "meshletInfo")')
+            try:
+                arrays[f["name"]] = decode_meshlet_triangles(blob, f["length"], info)
+            except ValueError as e:
+                raise VirtualMeshFormatError(f'geometry file is corrupt (field "{f["name"]}": {e})')
         else:
             raise VirtualMeshFormatError(f'unknown encoding "{enc}" for field "{f["name"]}"')
     return {k: (arrays[k] if k in arrays else header["values"].get(k)) for k in header["keys"]}
 
 
-def encode_virtual_mesh(record: Dict[str, object], deflate: bool = True, keep_indices: bool = False) -> bytes:
-    """encodeVirtualMesh (serialize.ts:109-158) without the meshoptimizer codecs: each typed array (numpy array)
-    is stored raw, or deflated where that is smaller; `indices` becomes 'meshlet-indices' (nothing stored)
-    when the meshlet arrays rebuild it exactly, unless `keep_indices`. Other values go into the JSON header."""
+def _dag():
+    from . import dag
+
+    return dag
+
+
+# serialize.ts:92-100 FIELD_HINTS: vertex-codec strides to try, and whether raw data is also deflated.
+FIELD_HINTS = {
+    "positions": {"strides": [12]}, "normals": {"strides": [12]}, "colors": {"strides": [12]}, "uvs": {"strides": [8]},
+    "tangents": {"strides": [16]}, "meshletBounds": {"strides": [64], "deflateRaw": True}, "meshletInfo": {"strides": [16]},
+}
+
+
+def encode_meshlet_triangles(triangles: np.ndarray, info) -> Union[bytes, None]:
+    """serialize.ts:257-296 encodeMeshletTriangles: u32 codec length, the index codec over meshlet-vertex slots
+    (padded to 4 bytes), then each triangle's corner rotation in 2 bits; None when the meshlets do not cover every
+    triangle exactly once with local ids below 256."""
+    if not isinstance(info, np.ndarray) or info.dtype != np.uint32:
+        return None
+    tri = np.asarray(triangles, dtype=np.uint32)
+    n = len(tri)
+    slots = np.zeros(n * 3, dtype=np.uint32)
+    covered = np.zeros(n, dtype=np.uint8)
+    for count, first_index, _, voff in info.reshape(-1, MESHLET_INFO_STRIDE).astype(np.int64):
+        first = first_index // 3
+        if first + count > n:
+            return None
+        packed = tri[first:first + count].astype(np.int64)
+        if np.any(covered[first:first + count]) or np.any(packed >> 24):
+            return None
+        covered[first:first + count] = 1
+        s = slots[first * 3:(first + count) * 3].reshape(-1, 3)
+        s[:, 0] = voff + (packed & 255)
+        s[:, 1] = voff + ((packed >> 8) & 255)
+        s[:, 2] = voff + ((packed >> 16) & 255)
+    if not np.all(covered == 1):
+        return None
+    codec = _dag().encode_index_buffer(slots)
+    decoded = _dag().decode_index_buffer(codec, len(slots)).reshape(-1, 3).astype(np.int64)
+    a, b, c = (slots.reshape(-1, 3)[:, k].astype(np.int64) for k in range(3))
+    x, y, z = decoded[:, 0], decoded[:, 1], decoded[:, 2]
+    r0 = (a == x) & (b == y) & (c == z)
+    r1 = (a == y) & (b == z) & (c == x)
+    r2 = (a == z) & (b == x) & (c == y)
+    if not np.all(r0 | r1 | r2):
+        return None
+    rot = np.where(r0, 0, np.where(r1, 1, 2)).astype(np.uint8)
+    rotations = np.zeros((n + 3) // 4, dtype=np.uint8)
+    for k in range(4):
+        part = rot[k::4]
+        rotations[:len(part)] |= (part << (2 * k)).astype(np.uint8)
+    pad = align4(len(codec)) - len(codec)
+    return struct.pack("<I", len(codec)) + codec + b"\0" * pad + rotations.tobytes()
+
+
+def decode_meshlet_triangles(blob: bytes, length: int, info: np.ndarray) -> np.ndarray:
+    """serialize.ts:353-381 decodeMeshletTriangles."""
+    if len(blob) < 4:
+        raise ValueError("truncated triangle data")
+    codec_length = struct.unpack_from("<I", blob, 0)[0]
+    start = 4 + align4(codec_length)
+    if start + (length + 3) // 4 > len(blob):
+        raise ValueError("truncated triangle data")
+    slots = _dag().decode_index_buffer(blob[4:4 + codec_length], length * 3).astype(np.int64)
+    rotations = np.frombuffer(blob, dtype=np.uint8, offset=start)
+    out = np.zeros(length, dtype=np.uint32)
+    for count, first_index, _, voff in info.reshape(-1, MESHLET_INFO_STRIDE).astype(np.int64):
+        first = first_index // 3
+        if first + count > length:
+            raise ValueError("meshlet range out of bounds")
+        t = np.arange(first, first + count)
+        r = (rotations[t >> 2] >> ((t & 3) * 2)) & 3
+        xyz = slots[first * 3:(first + count) * 3].reshape(-1, 3) - voff
+        x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+        a = np.where(r == 0, x, np.where(r == 1, y, z))
+        b = np.where(r == 0, y, np.where(r == 1, z, x))
+        c = np.where(r == 0, z, np.where(r == 1, x, y))
+        if np.any(((a | b | c) >> 8) != 0):
+            raise ValueError("meshlet-local vertex id out of range")
+        out[first:first + count] = (a | (b << 8) | (c << 16)).astype(np.uint32)
+    return out
+
+
+def _deflate(raw: bytes) -> bytes:
+    c = zlib.compressobj(9, zlib.DEFLATED, -15)
+    return c.compress(raw) + c.flush()
+
+
+def _decode_blob(entry, blob: bytes, info) -> Union[bytes, None]:
+    """The raw bytes an encoded candidate decodes to (None if it does not decode)."""
+    try:
+        if entry.get("deflate"):
+            blob = zlib.decompress(blob, -15)
+        enc = entry["encoding"]
+        itemsize = np.dtype(TYPED[entry["type"]]).itemsize
+        if enc == "raw":
+            return blob
+        if enc == "meshopt-vertex":
+            n = entry["length"] * itemsize
+            return _dag().decode_vertex_buffer(blob, n // entry["stride"], entry["stride"])
+        if enc == "meshlet-triangles":
+            return decode_meshlet_triangles(blob, entry["length"], info).astype("<u4").tobytes()
+    except (ValueError, zlib.error):
+        return None
+    return None
+
+
+def _encode_field(name: str, type_name: str, arr: np.ndarray, record, use_deflate: bool):
+    """serialize.ts:232-262 encodeField: the smallest candidate that decodes back exactly."""
+    raw = arr.astype(arr.dtype.newbyteorder("<"), copy=False).tobytes()
+    base = {"name": name, "type": type_name, "length": int(arr.size)}
+    raw_candidate = ({**base, "encoding": "raw"}, raw)
+    candidates = [raw_candidate]
+    info = record.get("meshletInfo")
+    tri = encode_meshlet_triangles(arr, info) if name == "meshletTriangles" and arr.dtype == np.uint32 else None
+    if tri is not None:
+        candidates.append(({**base, "encoding": "meshlet-triangles"}, tri))
+    elif len(raw) % 4 == 0 and len(raw) > 0:
+        hint = FIELD_HINTS.get(name, {})
+        strides = [s for s in hint.get("strides", [max(4, arr.dtype.itemsize)]) if s % 4 == 0 and s <= 256 and len(raw) % s == 0]
+        for stride in strides:
+            blob = _dag().encode_vertex_buffer(raw, len(raw) // stride, stride)
+            candidates.append(({**base, "encoding": "meshopt-vertex", "stride": stride}, blob))
+    candidates.sort(key=lambda c: len(c[1]))
+    if use_deflate and len(raw) >= 64:
+        hint = FIELD_HINTS.get(name, {})
+        to_deflate = [candidates[0], raw_candidate] if hint.get("deflateRaw") and candidates[0] is not raw_candidate else [candidates[0]]
+        for entry, blob in to_deflate:
+            candidates.append(({**entry, "deflate": True}, _deflate(blob)))
+        candidates.sort(key=lambda c: len(c[1]))
+    for entry, blob in candidates:
+        if _decode_blob(entry, blob, info) == raw:
+            return entry, blob
+    return raw_candidate
+
+
+def encode_virtual_mesh(record: Dict[str, object], compress: bool = True, deflate: bool = True,
+                        keep_indices: bool = False) -> bytes:
+    """encodeVirtualMesh (serialize.ts:109-158): each typed array (numpy array) through encodeField when
+    `compress` (raw otherwise); `indices` becomes 'meshlet-indices' (nothing stored) when the meshlet arrays
+    rebuild it exactly, unless `keep_indices`. Other values go into the JSON header."""
     keys = list(record.keys())  # None (null) values are kept, as the reference keeps null
     fields, blobs, values = [], [], {}
     offset = 0
@@ -173,15 +329,12 @@ def encode_virtual_mesh(record: Dict[str, object], deflate: bool = True, keep_in
         if name == "indices" and not keep_indices and _indices_match(record):
             entry["encoding"] = "meshlet-indices"
             blob = b""
+        elif compress:
+            entry, blob = _encode_field(name, entry["type"], arr, record, deflate)
+            entry = dict(entry)
         else:
             entry["encoding"] = "raw"
             blob = raw
-            if deflate and len(raw) >= 64:
-                c = zlib.compressobj(9, zlib.DEFLATED, -15)
-                packed = c.compress(raw) + c.flush()
-                if len(packed) < len(raw) and zlib.decompress(packed, -15) == raw:
-                    entry["deflate"] = True
-                    blob = packed
         entry["offset"] = offset
         entry["byteLength"] = len(blob)
         fields.append(entry)
@@ -223,9 +376,9 @@ def record_of(data) -> Dict[str, object]:
     """A builder result (vgcut.dag / scenegen VirtualMeshData) as the reference's VirtualMeshData record:
     typed arrays as numpy arrays, spheres as plain lists (buildVirtualMesh.ts:75-138 key names)."""
     out: Dict[str, object] = {}
-    for k in ("positions", "indices", "meshletCount", "meshletBounds", "meshletInfo", "meshletVertices", "meshletTriangles",
-              "levelRanges", "levelErrors", "levelGuardErrors", "lodBoundsSphere", "cullBoundsSphere", "replacementGroup",
-              "replacementStart", "replacementIndices", "boundingSphere"):
+    for k in ("positions", "normals", "colors", "uvs", "indices", "meshletCount", "meshletBounds", "meshletInfo",
+              "meshletVertices", "meshletTriangles", "levelRanges", "levelErrors", "levelGuardErrors", "lodBoundsSphere",
+              "cullBoundsSphere", "replacementGroup", "replacementStart", "replacementIndices", "boundingSphere"):
         v = getattr(data, k, None)
         if v is None:
             continue
@@ -236,7 +389,8 @@ def record_of(data) -> Dict[str, object]:
         out[k] = v
     out["stats"] = {"leafTriangles": int(data.leafTriangles), "leafMeshlets": int(data.leafMeshlets),
                     "rootTriangles": int(data.rootTriangles), "rootMeshlets": int(data.rootMeshlets),
-                    "lodLevels": int(getattr(data, "lodLevels", 0) or 0)}
+                    "lodLevels": int(getattr(data, "lodLevels", 0) or 0), "voxelLevels": int(getattr(data, "voxelLevels", 0) or 0),
+                    "aggregateGroups": int(getattr(data, "aggregateGroups", 0) or 0)}
     return out
 
 
@@ -257,4 +411,6 @@ def mesh_of(record: Dict[str, object]):
         leafTriangles=int(stats.get("leafTriangles", 0)), leafMeshlets=int(stats.get("leafMeshlets", 0)),
         rootTriangles=int(stats.get("rootTriangles", 0)), rootMeshlets=int(stats.get("rootMeshlets", 0)),
         meshletTriangles=record.get("meshletTriangles"), positions=record.get("positions"), indices=record.get("indices"),
-        meshletVertices=record.get("meshletVertices"), lodLevels=int(stats.get("lodLevels", 0)))
+        meshletVertices=record.get("meshletVertices"), lodLevels=int(stats.get("lodLevels", 0)), normals=record.get("normals"),
+        colors=record.get("colors"), uvs=record.get("uvs"), voxelLevels=int(stats.get("voxelLevels", 0)),
+        aggregateGroups=int(stats.get("aggregateGroups", 0)))
