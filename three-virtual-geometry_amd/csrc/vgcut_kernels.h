@@ -415,6 +415,17 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
+// The largest value over the 64 lanes (wave-uniform): the same DPP pattern with max (shifted-in lanes read 0).
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, dpp<0x111, 0xf, true>(v));
+  v = max(v, dpp<0x112, 0xf, true>(v));
+  v = max(v, dpp<0x114, 0xf, true>(v));
+  v = max(v, dpp<0x118, 0xf, true>(v));
+  v = max(v, dpp<0x142, 0xa, false>(v));
+  v = max(v, dpp<0x143, 0xc, false>(v));
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 // The sum over the 64 lanes (wave-uniform).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_scan(v)), 63));

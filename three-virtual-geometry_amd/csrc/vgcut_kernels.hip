@@ -349,13 +349,16 @@ __device__ __forceinline__ LevelResult select_band(const Params& p, uint32_t lev
 // error over `low` somewhere in [pmin, pmax], guaranteed if that holds with margin at both ends.
 // `levels`: the mesh's level table (2 float4 per level) staged in the wave's LDS; every lane reads the
 // same entry (a broadcast read).
+//
+// Only levels in [first, last] are visited: outside it no lane's level can be possible (the caller's
+// level_window).
 template <bool kBoth, typename V>
-__device__ __forceinline__ void select_levels(const float4* levels, const V& v, uint32_t level_count, float pmin, float pmax,
-                                              LevelResult (&r)[2]) {
+__device__ __forceinline__ void select_levels(const float4* levels, const V& v, uint32_t first, uint32_t last, float pmin,
+                                              float pmax, LevelResult (&r)[2]) {
   const float low0 = v.low[0], high0 = v.high[0], low1 = v.low[1], high1 = v.high[1];
   r[0] = r[1] = LevelResult{0xffffffffu, 0u, 0u, 0u};
 #pragma unroll 2
-  for (uint32_t i = 0; i < level_count; i++) {
+  for (uint32_t i = first; i <= last; i++) {
     const float4 l0 = levels[i * 2];
     const float4 l1 = levels[i * 2 + 1];
     const uint32_t level_first = static_cast<uint32_t>(l0.x);
@@ -383,6 +386,30 @@ __device__ __forceinline__ void select_levels(const float4* levels, const V& v, 
       }
     }
   }
+}
+
+// The levels a wave's select_levels must visit, as [first, last] (first > last: none). Lane i tests level i
+// against wave-wide bounds: a level is possible for a lane only if err * pmax <= high and parent * pmin > low;
+// with pmax >= the wave's least pmax, pmin <= its largest pmin, high <= the larger band top and low >= the
+// smaller band bottom, and binary32 products monotone in each factor, a level failing the test with those
+// bounds fails it for every lane and both cuts. `active`: lanes whose instance is in a cut.
+template <typename V>
+__device__ __forceinline__ uint2 level_window(const float4* levels, const V& v, uint32_t level_count, bool any1, bool active,
+                                              float pmin, float pmax) {
+  // pmin, pmax > 0: their bit patterns order as the values do.
+  const float least_pmax = __uint_as_float(~wave_max_u32(active ? ~__float_as_uint(pmax) : 0u));
+  const float most_pmin = __uint_as_float(wave_max_u32(active ? __float_as_uint(pmin) : 0u));
+  const float high = any1 ? fmaxf(v.high[0], v.high[1]) : v.high[0];
+  const float low = any1 ? fminf(v.low[0], v.low[1]) : v.low[0];
+  const uint32_t lane = lane_id();
+  bool maybe = false;
+  if (lane < level_count) {
+    const float4 l0 = levels[lane * 2];
+    maybe = l0.z * least_pmax <= high && (l0.w >= VG_INF_HALF || l0.w * most_pmin > low);
+  }
+  const uint64_t m = __ballot(maybe);
+  if (!m) return make_uint2(1u, 0u);
+  return make_uint2(static_cast<uint32_t>(__builtin_ctzll(m)), 63u - static_cast<uint32_t>(__builtin_clzll(m)));
 }
 
 // Sets bits [first, last) of a bitmap (measurement only).
@@ -576,8 +603,9 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
       const float pmax = (scale * v.proj_scale) / fmaxf(d + rr, v.near_plane);
       LevelResult lr[2];
-      if (any1) select_levels<true>(level_table[wib], v, level_count, pmin, pmax, lr);
-      else select_levels<false>(level_table[wib], v, level_count, pmin, pmax, lr);
+      const uint2 window = level_window(level_table[wib], v, level_count, any1, in_cut[0] || in_cut[1], pmin, pmax);
+      if (any1) select_levels<true>(level_table[wib], v, window.x, window.y, pmin, pmax, lr);
+      else select_levels<false>(level_table[wib], v, window.x, window.y, pmin, pmax, lr);
       const float3 cc = make_float3(d_cc[r][0], d_cc[r][1], d_cc[r][2]);
       const float cull_radius = cull_bounds.w * scale;
       bool work[2];
