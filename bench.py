@@ -194,6 +194,10 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate,
     batched, batched_ids = server_for(True)
     fifo, fifo_ids = server_for(True, native.VG_SERVE_FIFO)
     legs = (("batched", batched, batched_ids), ("batched_fifo", fifo, fifo_ids), ("plain", plain, plain_ids))
+    # Warm-up: every pool request once through each server, so its lanes hold the buffers the mix needs and
+    # its per-scene run-time estimates exist (a server in service is past this; the plain one is already).
+    for name, sv, ids in legs[:2]:
+        serve_stream(sv, ids, pool, capacity, len(pool), seed=0)
     loads = sorted({int(capacity * f) for f in (0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0)} | set(rates))
     for rate in loads:
         # about `requests_per_rate` seconds of arrivals, and at most about 2 s of work at plain capacity

@@ -338,6 +338,8 @@ typedef struct vg_request_done {
 } vg_request_done;
 
 typedef struct vg_server_stats {
+  /* inflight_bytes: the lanes' transient buffers (cell queues, work lists), kept between batches;
+     peak_inflight_bytes: the most they held. resident_bytes + inflight_bytes <= hbm_budget. */
   uint64_t hbm_budget, resident_bytes, inflight_bytes, peak_inflight_bytes;
   uint64_t batches, requests_done, max_batch_requests_seen, reruns;
   uint32_t queued, running;

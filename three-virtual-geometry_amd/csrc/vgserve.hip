@@ -3,6 +3,11 @@
 // fired, then launches new batches on free lanes, each a group of queued requests of one scene that
 // fits the view limit, the scene's free view slots and the HBM budget. Client threads only touch the
 // queue and the completed list under the mutex; reads of finished requests use a stream of their own.
+//
+// Each lane owns its transient buffers (the (view, cell) queue and the work list) and keeps them between
+// batches, growing them when a batch needs more: allocating per batch (a stream-ordered pool) cost
+// 0.1-0.9 ms of host time per batch, more than a light request's whole cut. The lanes' buffers count
+// against the HBM budget; idle lanes give theirs back when a batch cannot otherwise fit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +61,7 @@ struct Batch {
   uint32_t lane = 0;
   uint32_t views = 0;
   std::vector<Request*> requests;
-  uint64_t bytes = 0;
+  uint64_t bytes = 0;  // transient bytes the batch needs (its lane holds at least these)
   bool full = false;  // work list at its worst-case size (after an overflow of the optimistic one)
   double t_launch = 0;
 };
@@ -95,7 +100,6 @@ struct vg_server {
   std::vector<hipEvent_t> lane_done;
   std::vector<uint32_t*> lane_flags;  // pinned: the lane's first 8 flag words after its batch
   std::vector<int> lane_batch;        // index into `running` or -1
-  hipMemPool_t pool = nullptr;
   hipStream_t read_stream = nullptr;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -104,7 +108,9 @@ struct vg_server {
   std::deque<uint64_t> completed;
   std::vector<Batch> running;
   std::deque<Batch> deferred;  // reruns at full size waiting for HBM budget (they keep their view slots)
-  uint64_t resident = 0, inflight = 0, peak_inflight = 0, batches = 0, requests_done = 0, max_batch_seen = 0, reruns = 0;
+  // `reserved`: bytes of the lanes' transient buffers (lane_bytes per lane), held between batches.
+  uint64_t resident = 0, reserved = 0, peak_reserved = 0, batches = 0, requests_done = 0, max_batch_seen = 0, reruns = 0;
+  std::vector<uint64_t> lane_bytes;
   uint64_t next_ticket = 1;
   uint64_t generation = 0;  // bumped by submit and release: the worker waits for it when it cannot progress
   bool stop = false;
@@ -153,13 +159,14 @@ struct vg_server {
         again.requests = b.requests;
         again.full = true;
         const uint32_t lane = b.lane;
-        inflight -= b.bytes;
         running.erase(running.begin() + static_cast<std::ptrdiff_t>(i));
         for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
         lane_batch[lane] = -1;
         reruns++;
-        if (fits_now(again)) launch(again, static_cast<int>(lane));
-        else deferred.push_back(std::move(again));
+        if (make_room(lane, transient_bytes(scenes[again.scene], again.views, true, cfg.first_work_items), false))
+          launch(again, static_cast<int>(lane));
+        else
+          deferred.push_back(std::move(again));
         continue;
       }
       const double t = now_us();
@@ -170,7 +177,6 @@ struct vg_server {
         completed.push_back(r->ticket);
         requests_done++;
       }
-      inflight -= b.bytes;
       lane_batch[b.lane] = -1;
       running.erase(running.begin() + static_cast<std::ptrdiff_t>(i));
       for (size_t k = 0; k < running.size(); k++) lane_batch[running[k].lane] = static_cast<int>(k);
@@ -178,20 +184,66 @@ struct vg_server {
     }
   }
 
-  // Whether a batch's transients fit the HBM budget next to the resident scenes and the batches in flight.
-  bool fits_now(const Batch& b) const {
-    return resident + inflight + transient_bytes(scenes[b.scene], b.views, b.full, cfg.first_work_items) <= cfg.hbm_budget;
+  // Frees a lane's transient buffers (the lane is idle: its stream has finished its last batch).
+  void shrink(uint32_t lane) {
+    Lane* l = lanes[lane].get();
+    (void)hipStreamSynchronize(l->stream);
+    for (DevBuf* b : {&l->d_cell_queue, &l->d_work, &l->d_work_start, &l->d_work_scale}) {
+      if (b->ptr) (void)hipFree(b->ptr);
+      *b = DevBuf{};
+    }
+    l->cell_queue_capacity = 0;
+    l->work_capacity = 0;
+    reserved -= lane_bytes[lane];
+    lane_bytes[lane] = 0;
   }
 
-  // Launches deferred reruns that fit now, oldest first; one that cannot fit even with nothing in flight
-  // fails its requests with VG_E_NOMEM. With the mutex held.
+  // Whether `lane` can hold `need` transient bytes within the HBM budget: it already does, or it may grow
+  // into what the other lanes leave; with `shrink_idle`, idle lanes give their buffers back first.
+  bool make_room(uint32_t lane, uint64_t need, bool shrink_idle) {
+    if (lane_bytes[lane] >= need) return true;
+    auto fits = [&] { return resident + (reserved - lane_bytes[lane]) + need <= cfg.hbm_budget; };
+    if (fits()) return true;
+    if (!shrink_idle) return false;
+    for (uint32_t l = 0; l < lanes.size() && !fits(); l++)
+      if (l != lane && lane_batch[l] < 0 && lane_bytes[l]) shrink(l);
+    return fits();
+  }
+
+  // Gives `lane` transient buffers for `queue_bytes` of cell queue and `work_items` work-list items (more
+  // when the lane already holds more). The caller made room within the budget. Returns VG_OK or VG_E_NOMEM.
+  int grow(uint32_t lane, size_t queue_bytes, size_t work_items) {
+    Lane* l = lanes[lane].get();
+    const bool queue_ok = l->d_cell_queue.bytes >= queue_bytes, work_ok = l->d_work.bytes >= sizeof(uint4) * work_items;
+    if (queue_ok && work_ok) return VG_OK;
+    (void)hipStreamSynchronize(l->stream);
+    auto realloc = [&](DevBuf& b, size_t bytes) {
+      if (b.bytes >= bytes) return true;
+      if (b.ptr) (void)hipFree(b.ptr);
+      b = DevBuf{};
+      void* ptr = nullptr;
+      if (hipMalloc(&ptr, std::max<size_t>(bytes, 256)) != hipSuccess) return false;
+      b = DevBuf{ptr, std::max<size_t>(bytes, 256)};
+      return true;
+    };
+    const bool ok = realloc(l->d_cell_queue, queue_bytes) && realloc(l->d_work, sizeof(uint4) * work_items) &&
+                    realloc(l->d_work_start, sizeof(unsigned long long) * work_items) && realloc(l->d_work_scale, sizeof(float) * work_items);
+    const uint64_t now = l->d_cell_queue.bytes + l->d_work.bytes + l->d_work_start.bytes + l->d_work_scale.bytes;
+    reserved = reserved - lane_bytes[lane] + now;
+    lane_bytes[lane] = now;
+    peak_reserved = std::max(peak_reserved, reserved);
+    return ok ? VG_OK : VG_E_NOMEM;
+  }
+
+  // Launches deferred reruns that fit now, oldest first (idle lanes give their buffers back for them); one
+  // that cannot fit even with every other lane idle fails its requests with VG_E_NOMEM. With the mutex held.
   void launch_deferred() {
     while (!deferred.empty()) {
       const int lane = free_lane();
       if (lane < 0) return;
       Batch& b = deferred.front();
-      if (!fits_now(b)) {
-        if (!running.empty()) return;  // wait for memory in flight
+      if (!make_room(static_cast<uint32_t>(lane), transient_bytes(scenes[b.scene], b.views, true, cfg.first_work_items), true)) {
+        if (!running.empty()) return;  // wait for the lanes in use
         set_error("vg_server: a request's full-size work list does not fit the HBM budget");
         const double t = now_us();
         for (Request* r : b.requests) {
@@ -224,8 +276,14 @@ struct vg_server {
   // else the queued request of least estimated run time that fits; the batch is then filled with the
   // head scene's other requests, least estimated run time first. An overdue head that does not fit
   // stops the pick: nothing overtakes it. With the mutex held.
-  bool pick_shortest(Batch& b) {
-    const uint64_t budget_left = cfg.hbm_budget > resident + inflight ? cfg.hbm_budget - resident - inflight : 0;
+  // What a lane's transient buffers may hold: the budget left by the resident scenes and the other lanes.
+  uint64_t lane_room(uint32_t lane) const {
+    const uint64_t others = resident + reserved - lane_bytes[lane];
+    return cfg.hbm_budget > others ? cfg.hbm_budget - others : 0;
+  }
+
+  bool pick_shortest(Batch& b, uint32_t lane) {
+    const uint64_t budget_left = lane_room(lane);
     const double t = now_us();
     auto fits = [&](const Request* r, uint32_t views_before) {
       const uint32_t n = views_before + static_cast<uint32_t>(r->views.size());
@@ -264,11 +322,11 @@ struct vg_server {
     return true;
   }
 
-  bool pick(Batch& b) { return cfg.policy == VG_SERVE_FIFO ? pick_fifo(b) : pick_shortest(b); }
+  bool pick(Batch& b, uint32_t lane) { return cfg.policy == VG_SERVE_FIFO ? pick_fifo(b, lane) : pick_shortest(b, lane); }
 
   // Picks the next batch from the queue (FIFO, one scene, fits views, slots and memory). With the mutex held.
-  bool pick_fifo(Batch& b) {
-    const uint64_t budget_left = cfg.hbm_budget > resident + inflight ? cfg.hbm_budget - resident - inflight : 0;
+  bool pick_fifo(Batch& b, uint32_t lane) {
+    const uint64_t budget_left = lane_room(lane);
     const double t = now_us();
     std::vector<uint32_t> slots_left(scenes.size());
     for (size_t k = 0; k < scenes.size(); k++) slots_left[k] = static_cast<uint32_t>(scenes[k].free_slots.size());
@@ -328,30 +386,21 @@ struct vg_server {
     b.bytes = transient_bytes(se, b.views, b.full, cfg.first_work_items);
     b.lane = static_cast<uint32_t>(lane_index);
     b.t_launch = t;
-    void* ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
-    const size_t sizes[4] = {queue_bytes, sizeof(uint4) * work_items, sizeof(unsigned long long) * work_items, sizeof(float) * work_items};
-    int rc = VG_OK;
-    for (int k = 0; k < 4 && rc == VG_OK; k++)
-      if (hipMallocFromPoolAsync(&ptrs[k], sizes[k], pool, lane->stream) != hipSuccess) rc = VG_E_NOMEM;
+    int rc = grow(static_cast<uint32_t>(lane_index), queue_bytes, work_items);
     if (rc == VG_OK) {
+      // The whole work list the lane holds (whole partitions): a batch runs with at least what it asked for.
       lane->borrowed = true;
-      lane->d_cell_queue = DevBuf{ptrs[0], sizes[0]};
-      lane->d_work = DevBuf{ptrs[1], sizes[1]};
-      lane->d_work_start = DevBuf{ptrs[2], sizes[2]};
-      lane->d_work_scale = DevBuf{ptrs[3], sizes[3]};
-      lane->cell_queue_capacity = queue_bytes / sizeof(uint2);
-      lane->work_capacity = static_cast<uint32_t>(work_items);
+      lane->cell_queue_capacity = lane->d_cell_queue.bytes / sizeof(uint2);
+      const size_t items = lane->d_work.bytes / sizeof(uint4);
+      size_t full_queue = 0, full_items = 0;
+      transient_sizes(s, b.views, &full_queue, &full_items);
+      lane->work_capacity = static_cast<uint32_t>(std::min(items, full_items) / vg::kWorkParts * vg::kWorkParts);
       rc = prepare_cut(s, lane, views.data(), b.views, true);
       if (rc == VG_OK) rc = enqueue_cut(s, lane, b.views, false);
       if (rc == VG_OK && hipMemcpyAsync(lane_flags[lane_index], lane->d_flags.ptr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                         lane->stream) != hipSuccess)
         rc = VG_E_DEVICE;
     }
-    for (int k = 0; k < 4; k++)
-      if (ptrs[k]) (void)hipFreeAsync(ptrs[k], lane->stream);
-    lane->d_cell_queue = lane->d_work = lane->d_work_start = lane->d_work_scale = DevBuf{};
-    lane->work_capacity = 0;
-    lane->cell_queue_capacity = 0;
     if (rc == VG_OK && hipEventRecord(lane_done[lane_index], lane->stream) != hipSuccess) rc = VG_E_DEVICE;
     if (rc != VG_OK) {
       set_error("batch launch failed: " + ctx->error);
@@ -367,8 +416,6 @@ struct vg_server {
       cv_done.notify_all();
       return rc;
     }
-    inflight += b.bytes;
-    peak_inflight = std::max(peak_inflight, inflight);
     batches++;
     max_batch_seen = std::max<uint64_t>(max_batch_seen, b.requests.size());
     running.push_back(std::move(b));
@@ -390,7 +437,7 @@ struct vg_server {
       bool launched = deferred.size() != waiting;
       for (int lane = free_lane(); lane >= 0 && deferred.empty() && !queue.empty(); lane = free_lane()) {
         Batch b;
-        if (!pick(b)) break;
+        if (!pick(b, static_cast<uint32_t>(lane))) break;
         launch(b, lane);
         launched = true;
       }
@@ -434,16 +481,6 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
   sv->cfg = c;
   int rc = VG_OK;
   auto init = [&]() -> int {
-    if (hipDeviceGetDefaultMemPool(&sv->pool, ctx->cfg.device) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipDeviceGetDefaultMemPool");
-    uint64_t keep = UINT64_MAX;  // keep freed transient memory reserved in the pool between batches
-    if (hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
-      return fail(ctx, VG_E_DEVICE, "hipMemPoolSetAttribute");
-    // Memory a batch frees is reused by later batches of the same lane (stream order makes that safe);
-    // no reuse across lanes, where it would rest on the allocator's own cross-stream tracking.
-    int no = 0;
-    if (hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowOpportunistic, &no) != hipSuccess ||
-        hipMemPoolSetAttribute(sv->pool, hipMemPoolReuseAllowInternalDependencies, &no) != hipSuccess)
-      return fail(ctx, VG_E_DEVICE, "hipMemPoolSetAttribute (reuse)");
     if (hipStreamCreateWithFlags(&sv->read_stream, hipStreamNonBlocking) != hipSuccess) return fail(ctx, VG_E_DEVICE, "hipStreamCreate");
     for (uint32_t l = 0; l < c.lanes; l++) {
       sv->lanes.emplace_back(new Lane());
@@ -458,6 +495,7 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
       std::memset(flags, 0, 8 * sizeof(uint32_t));
       sv->lane_flags.push_back(flags);
       sv->lane_batch.push_back(-1);
+      sv->lane_bytes.push_back(0);
     }
     return VG_OK;
   };
@@ -481,7 +519,10 @@ void vg_server_destroy(vg_server* sv) {
   if (sv->worker.joinable()) sv->worker.join();
   for (auto& lane : sv->lanes) {
     if (lane->stream) (void)hipStreamSynchronize(lane->stream);
-    lane->borrowed = true;  // its transient buffers went back to the pool with each batch
+    for (DevBuf* b : {&lane->d_cell_queue, &lane->d_work, &lane->d_work_start, &lane->d_work_scale})
+      if (b->ptr) (void)hipFree(b->ptr);
+    lane->d_cell_queue = lane->d_work = lane->d_work_start = lane->d_work_scale = DevBuf{};
+    lane->borrowed = true;  // the server freed its transient buffers above
     lane_destroy(lane.get());
   }
   for (hipEvent_t e : sv->lane_done) (void)hipEventDestroy(e);
@@ -612,8 +653,8 @@ int vg_server_stats_get(const vg_server* sv, vg_server_stats* st) {
   std::lock_guard<std::mutex> lk(m->mu);
   st->hbm_budget = sv->cfg.hbm_budget;
   st->resident_bytes = sv->resident;
-  st->inflight_bytes = sv->inflight;
-  st->peak_inflight_bytes = sv->peak_inflight;
+  st->inflight_bytes = sv->reserved;
+  st->peak_inflight_bytes = sv->peak_reserved;
   st->batches = sv->batches;
   st->requests_done = sv->requests_done;
   st->max_batch_requests_seen = sv->max_batch_seen;
