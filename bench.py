@@ -161,8 +161,10 @@ def serve_bench(spec_meshes, make_view, cam, scenegen, rates, requests_per_rate,
         f"schedules tuned ({time.time() - t0:.1f} s)")
 
     def server_for(batched, policy=native.VG_SERVE_SHORTEST):
+        # Batched: 4 lanes whose launches take at most 4 persistent blocks per CU, leaving room for the other
+        # lanes' kernels (light requests then wait less behind heavy ones). Plain: one request per launch.
         sv = native.Server(ctx, lanes=4 if batched else 1, max_batch_views=48, max_batch_requests=0 if batched else 1,
-                           policy=policy if batched else native.VG_SERVE_FIFO)
+                           policy=policy if batched else native.VG_SERVE_FIFO, max_blocks_per_cu=4 if batched else 0)
         return sv, [sv.add_scene(sc) for sc, _, _ in resident]
 
     # request classes: views (light 1-4, heavy 5-16) x scene (small <= 152k instances, large >= 500k)

@@ -320,6 +320,8 @@ typedef struct vg_server_config {
   uint32_t max_bypass_us;       /* how long later requests may overtake one that does not fit (default 2000) */
   uint32_t first_work_items;    /* work-list items of a batch's first launch (default 1 << 22) */
   uint32_t policy;              /* VG_SERVE_SHORTEST (default) or VG_SERVE_FIFO */
+  uint32_t max_blocks_per_cu;   /* cap on the persistent blocks per CU of a batch's launches, so batches on
+                                   other lanes find room beside it (0: the scene's tuned schedule) */
 } vg_server_config;
 
 enum { VG_SERVE_SHORTEST = 0, VG_SERVE_FIFO = 1 };

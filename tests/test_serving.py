@@ -141,6 +141,11 @@ def test_each_request_is_the_same_alone_and_in_the_mix(gpu_available):
     ids = [server.add_scene(s) for s in resident]
     run_mix(server, ids, reqs, alone)
     server.close()
+    # launches capped at 2 persistent blocks per CU (room for the other lanes' kernels): same results
+    server = native.Server(ctx, lanes=3, max_batch_views=16, max_blocks_per_cu=2)
+    ids = [server.add_scene(s) for s in resident]
+    run_mix(server, ids, reqs, alone)
+    server.close()
     # a first work list far too small: every batch overflows it and runs again at full size
     server = native.Server(ctx, lanes=2, max_batch_views=16, first_work_items=64)
     ids = [server.add_scene(s) for s in resident]

@@ -493,8 +493,9 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   vg_ctx* ctx = s->ctx;
   hipStream_t st = lane->stream;
   Params p = make_params(s, lane, view_count);
-  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
-  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
+  const uint32_t cap = lane->max_blocks_per_cu ? lane->max_blocks_per_cu : UINT32_MAX;
+  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * std::min(s->schedule.instance_blocks_per_cu, cap);
+  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * std::min(s->schedule.meshlet_blocks_per_cu, cap);
   const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd, affine_scene(s));
   std::vector<hipEvent_t> ev;
   const bool timed = s->profile && !capturing && lane == &s->lane;
@@ -555,8 +556,9 @@ int enqueue_phase2(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.counters1 = static_cast<const uint32_t*>(s->d_counters.ptr);
   p.counters = static_cast<uint32_t*>(s->d_counters2.ptr);
   p.args = static_cast<uint32_t*>(s->d_args2.ptr);
-  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * s->schedule.instance_blocks_per_cu;
-  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * s->schedule.meshlet_blocks_per_cu;
+  const uint32_t cap = lane->max_blocks_per_cu ? lane->max_blocks_per_cu : UINT32_MAX;
+  const uint32_t persistent_instances = static_cast<uint32_t>(ctx->cus) * std::min(s->schedule.instance_blocks_per_cu, cap);
+  const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * std::min(s->schedule.meshlet_blocks_per_cu, cap);
   const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd, affine_scene(s));
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);

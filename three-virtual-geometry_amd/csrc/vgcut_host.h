@@ -40,6 +40,9 @@ struct ViewRecord;  // the device view record (vg::ViewDev), opaque here
 struct Lane {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
+  // Most persistent blocks per CU its launches use (0: the scene's schedule as is): a server lane launched
+  // beside others leaves room on each CU for their kernels.
+  uint32_t max_blocks_per_cu = 0;
   DevBuf d_views, d_cell_queue, d_cell_queue_count, d_work, d_work_start, d_work_scale, d_work_counter, d_flags;
   DevBuf d_budget_thresholds, d_budget_hist;
   DevBuf d_expand_plan;  // per (view, cut): first draw-list chunk of the expand pass; then the total

@@ -422,31 +422,52 @@ __device__ void mark_range(uint32_t* bits, uint32_t first, uint32_t last) {
   }
 }
 
+// Per-view counters of the instance pass, summed in a block's LDS when a launch has at most this many views
+// (the queue is cell-major, so consecutive entries change view every time, while a block's entries touch
+// few distinct views): flushed to memory once per block.
+constexpr uint32_t kBlockViews = 64;
+enum : uint32_t { A_INSTANCES, A_OCC_TESTS, A_OCCLUDED, A_SURVIVED0, A_SURVIVED1, A_WORK0, A_WORK1, A_CLUSTERS0, A_CLUSTERS1, kAccWords };
+
 // One wave per queued (view, cell), the cell's 128 instances in two rounds of 64 lanes. Each
 // (instance, cut) that survives the culls becomes at most one work item: a range of LOD runs for the
 // meshlet pass, or, on the fast path, the instance's <= kFastPathMax guaranteed meshlets flagged to be
 // drawn without cluster tests. The cell's items are appended with one atomic, lane-major, so item order
-// and first-task order agree. Counters are summed in registers over consecutive cells of one view.
+// and first-task order agree. Counters are summed in registers while the view stays the same, then in
+// the block's LDS (or directly in memory when a launch has more than kBlockViews views).
 __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
   __shared__ float4 level_table[4][2 * kMaxLevels];  // per wave: the current mesh's level table
-  if (retry_round && !p.flags[retry_round - 1]) return;
+  __shared__ unsigned long long view_acc[kBlockViews][kAccWords];
+  if (retry_round && !p.flags[retry_round - 1]) return;  // grid-uniform
+  const bool block_acc = p.view_count <= kBlockViews;
+  if (block_acc)
+    for (uint32_t i = threadIdx.x; i < kBlockViews * kAccWords; i += blockDim.x) (&view_acc[0][0])[i] = 0ull;
+  __syncthreads();
   const uint32_t lane = lane_id(), wib = threadIdx.x / 64;
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), rank = xcd_wave_rank();
   const uint32_t q_begin = static_cast<uint32_t>(queue_count * rank / waves);
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (rank + 1) / waves);
-  if (q_begin >= q_end) return;
+  if (q_begin < q_end) {
   // This wave's work-list partition: ranks are split evenly, so each partition takes the items of an
   // even share of the queue.
   const uint32_t part = static_cast<uint32_t>(rank * kWorkParts / waves);
   const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
 
   // Counters of the current view (wave-uniform), flushed when the view changes.
-  uint32_t acc_slot = 0xffffffffu;
+  uint32_t acc_slot = 0xffffffffu, acc_vi = 0;
   uint32_t acc_instances = 0, acc_occ_tests = 0, acc_occluded = 0, acc_survived[2] = {0, 0}, acc_work[2] = {0, 0};
   uint64_t acc_clusters[2] = {0, 0};
   auto flush = [&]() {
     if (acc_slot == 0xffffffffu || lane != 0) return;
+    if (block_acc) {
+      unsigned long long* a = view_acc[acc_vi];
+      const unsigned long long v[kAccWords] = {acc_instances, acc_occ_tests, acc_occluded, acc_survived[0], acc_survived[1], acc_work[0],
+                                               acc_work[1], acc_clusters[0], acc_clusters[1]};
+#pragma unroll
+      for (uint32_t k = 0; k < kAccWords; k++)
+        if (v[k]) atomicAdd(&a[k], v[k]);
+      return;
+    }
     if (acc_instances) atomicAdd(metric_at(p, acc_slot, M_INSTANCES), static_cast<unsigned long long>(acc_instances));
     if (acc_occ_tests) atomicAdd(metric_at(p, acc_slot, M_OCCLUSION), static_cast<unsigned long long>(acc_occ_tests));
     if (acc_occluded) atomicAdd(view_counters(p, acc_slot) + 1, acc_occluded);
@@ -494,6 +515,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
     if (vslot != acc_slot) {
       flush();
       acc_slot = vslot;
+      acc_vi = vi;
       acc_instances = acc_occ_tests = acc_occluded = 0;
       acc_survived[0] = acc_survived[1] = acc_work[0] = acc_work[1] = 0;
       acc_clusters[0] = acc_clusters[1] = 0;
@@ -684,6 +706,28 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
   }
   flush();
   if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
+  }
+  if (!block_acc) return;  // block-uniform
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < p.view_count * kAccWords; i += blockDim.x) {
+    const unsigned long long wide = (&view_acc[0][0])[i];
+    if (!wide) continue;
+    const uint32_t value = static_cast<uint32_t>(wide);  // the 32-bit counters' words (their sums fit 32 bits)
+    const uint32_t vi = i / kAccWords, k = i % kAccWords, slot = p.views[vi].slot;
+    switch (k) {
+      case A_INSTANCES: atomicAdd(metric_at(p, slot, M_INSTANCES), wide); break;
+      case A_OCC_TESTS: atomicAdd(metric_at(p, slot, M_OCCLUSION), wide); break;
+      case A_OCCLUDED: atomicAdd(view_counters(p, slot) + 1, value); break;
+      case A_SURVIVED0: atomicAdd(counter_at(p, slot, 0, kSurvived), value); break;
+      case A_SURVIVED1: atomicAdd(counter_at(p, slot, 1, kSurvived), value); break;
+      case A_WORK0:
+      case A_WORK1:
+        atomicAdd(counter_at(p, slot, k - A_WORK0, kWork), value);
+        atomicAdd(metric_at(p, slot, M_WORK0 + (k - A_WORK0)), wide);
+        break;
+      default: atomicAdd(metric_at(p, slot, M_CLUSTERS0 + (k - A_CLUSTERS0)), wide); break;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ 3. meshlets (meshletPass, GeometryPool.ts:926-968)

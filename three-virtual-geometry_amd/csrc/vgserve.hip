@@ -496,6 +496,7 @@ int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** ou
       sv->lane_flags.push_back(flags);
       sv->lane_batch.push_back(-1);
       sv->lane_bytes.push_back(0);
+      sv->lanes.back()->max_blocks_per_cu = c.max_blocks_per_cu;
     }
     return VG_OK;
   };

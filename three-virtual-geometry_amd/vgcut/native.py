@@ -130,6 +130,7 @@ class ServerConfig(ctypes.Structure):
         ("max_bypass_us", ctypes.c_uint32),
         ("first_work_items", ctypes.c_uint32),
         ("policy", ctypes.c_uint32),
+        ("max_blocks_per_cu", ctypes.c_uint32),
     ]
 
 
@@ -521,11 +522,12 @@ class Server:
     """Request server over resident scenes (vg_server_*): continuous batching on `lanes` HIP streams."""
 
     def __init__(self, ctx: "Context", hbm_budget: int = 0, lanes: int = 4, max_batch_views: int = 0, max_batch_requests: int = 0,
-                 max_bypass_us: int = 0, first_work_items: int = 0, policy: int = 0):
+                 max_bypass_us: int = 0, first_work_items: int = 0, policy: int = 0, max_blocks_per_cu: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         self.scenes = []  # keeps the Scene objects alive while the server uses them
-        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us, first_work_items, policy)
+        cfg = ServerConfig(hbm_budget, lanes, max_batch_views, max_batch_requests, max_bypass_us, first_work_items, policy,
+                           max_blocks_per_cu)
         handle = ctypes.c_void_p()
         ctx.check(self.lib.vg_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(handle)), "vg_server_create")
         self.handle = handle
