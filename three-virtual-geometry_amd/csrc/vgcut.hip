@@ -511,7 +511,7 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   mark();
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
   mark();
-  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
+  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 63) / 64, (view_count + 63) / 64), dim3(256), 0, st, p);
   if (lane->launch_budget) {
     hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
     void (*const budget_meshlets)(Params) = affine_scene(s) ? k_budget_meshlets<true> : k_budget_meshlets<false>;
@@ -561,7 +561,7 @@ int enqueue_phase2(vg_scene* s, Lane* lane, uint32_t view_count) {
   const uint32_t persistent_meshlets = static_cast<uint32_t>(ctx->cus) * std::min(s->schedule.meshlet_blocks_per_cu, cap);
   const MeshletKernel k_meshlets = meshlet_kernel(s->schedule.meshlet_waves_per_simd, affine_scene(s));
   hipLaunchKernelGGL(k_begin, dim3(view_count), dim3(256), 0, st, p);
-  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 255) / 256, (view_count + 63) / 64), dim3(256), 0, st, p);
+  if (p.cell_count) hipLaunchKernelGGL(k_cells, dim3((p.cell_count + 63) / 64, (view_count + 63) / 64), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_instances, dim3(persistent_instances), dim3(256), 0, st, p, 0u);
   hipLaunchKernelGGL(k_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
   hipLaunchKernelGGL(k_retest_plan, dim3(1), dim3(64), 0, st, p);

@@ -2,7 +2,7 @@
 //
 // One launch sequence serves any number of views (the plain path is one view):
 //   k_begin      zero the views' counters, derive the threshold bands from the controller state
-//   k_cells      one thread per (view, cell): cell culls, wave-compacted append to a (view, cell) queue
+//   k_cells      one thread per (cell, 16 views): cell culls, wave-compacted append to a (view, cell) queue
 //   k_instances  one wave per queued (view, cell), 2 instances per lane: instance culls, occlusion,
 //                level ranges, fast path; work items appended with one 64-bit atomic per wave that
 //                returns both the item slot and the item's first cluster test, so the work list is
@@ -266,12 +266,21 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
 // are wave-uniform (scalar loads), the cell's data is read once for all of them. The (view, cell) pairs
 // that pass are appended cell-major with one atomic per wave, so the instance pass meets the views of
 // one cell one after another and reuses the cell's instance data.
+// A block takes 64 cells and up to 64 views: each of its 4 waves tests the 64 cells (one per lane) against 16
+// of the views, the waves' view bits are OR-ed per cell in LDS, and the first wave appends the cells' visible
+// (view, cell) pairs cell-major (views ascending within a cell) with one atomic, as one wave per 64 cells
+// testing every view in turn would; the per-thread view loop is 4 times shorter.
 __global__ __launch_bounds__(256) void k_cells(Params p) {
   __shared__ uint32_t visible[64];
-  const uint32_t lane = lane_id();
-  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned long long cell_mask[64];
+  const uint32_t lane = lane_id(), group = threadIdx.x >> 6;
+  const uint32_t cell = blockIdx.x * 64 + lane;
   const uint32_t v_first = blockIdx.y * 64, v_end = min(v_first + 64, p.view_count);
-  if (threadIdx.x < 64) visible[threadIdx.x] = 0;
+  const uint32_t g_first = v_first + group * 16, g_end = min(g_first + 16, v_end);
+  if (threadIdx.x < 64) {
+    visible[threadIdx.x] = 0;
+    cell_mask[threadIdx.x] = 0;
+  }
   __syncthreads();
   const bool valid = cell < p.cell_count;
   float4 sphere = make_float4(0, 0, 0, 0), record = sphere, flags = sphere, levers = sphere;
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
   const float radius = sphere.w, max_radius = record.x;
   uint64_t mask = 0;
   if (__ballot(enabled)) {
-    for (uint32_t vi = v_first; vi < v_end; vi++) {
+    for (uint32_t vi = g_first; vi < g_end; vi++) {
       const auto& v = scalar(p.views)[vi];
       bool keep = enabled && !(v.frustum == 1 && frustum_distance(v, sphere.x, sphere.y, sphere.z, 2, sweep) < -radius);
       const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
@@ -303,14 +312,18 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
       if (lane == 0 && n) atomicAdd(&visible[vi - v_first], n);
     }
   }
-  const uint32_t n = static_cast<uint32_t>(__popcll(mask));
-  const uint32_t incl = wave_scan(n);
-  const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-  uint32_t base = 0;
-  if (lane == 0 && total) base = atomicAdd(p.cell_queue_count, total);
-  uint32_t at = uniform(base) + incl - n;
-  for (uint64_t m = mask; m; m &= m - 1) p.cell_queue[at++] = make_uint2(v_first + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1), cell);
+  if (mask) atomicOr(&cell_mask[lane], mask);
   __syncthreads();
+  if (group == 0) {
+    const uint64_t m_all = cell_mask[lane];
+    const uint32_t n = static_cast<uint32_t>(__popcll(m_all));
+    const uint32_t incl = wave_scan(n);
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(p.cell_queue_count, total);
+    uint32_t at = uniform(base) + incl - n;
+    for (uint64_t m = m_all; m; m &= m - 1) p.cell_queue[at++] = make_uint2(v_first + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1), cell);
+  }
   if (threadIdx.x < v_end - v_first && visible[threadIdx.x]) {
     const uint32_t slot = scalar(p.views)[v_first + threadIdx.x].slot;
     atomicAdd(view_counters(p, slot), visible[threadIdx.x]);  // counters[0]: visible cells
