@@ -15,6 +15,7 @@
 #include "vgcut_kernels.hip"
 #include "vgraster_kernels.hip"
 #include "vgcut_blocks.h"
+#include "vgcut_cells.h"
 #include "vgcut_host.h"
 
 using namespace vg;
@@ -164,19 +165,6 @@ int upload(vg_scene* s, DevBuf& b, const std::vector<T>& v) {
   return 0;
 }
 
-// instanceSphere, GeometryPool.ts:1171-1180 (fp64; column lengths as sqrt of the sum of squares).
-void instance_sphere(const float* m, const float* sphere, double out[4]) {
-  const double sx = sphere[0], sy = sphere[1], sz = sphere[2], sr = sphere[3];
-  out[0] = static_cast<double>(m[0]) * sx + static_cast<double>(m[4]) * sy + static_cast<double>(m[8]) * sz + m[12];
-  out[1] = static_cast<double>(m[1]) * sx + static_cast<double>(m[5]) * sy + static_cast<double>(m[9]) * sz + m[13];
-  out[2] = static_cast<double>(m[2]) * sx + static_cast<double>(m[6]) * sy + static_cast<double>(m[10]) * sz + m[14];
-  auto col = [&](int k) {
-    const double x = m[k * 4], y = m[k * 4 + 1], z = m[k * 4 + 2];
-    return std::sqrt(x * x + y * y + z * z);
-  };
-  out[3] = sr * std::max(col(0), std::max(col(1), col(2)));
-}
-
 uint32_t spread_bits10(uint32_t v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -227,32 +215,6 @@ std::vector<uint32_t> morton_order(const float* matrices, uint32_t n, const floa
     order.swap(next);
   }
   return order;
-}
-
-// computeCell, GeometryPool.ts:519-545 (fp64, stored as f32).
-void compute_cell(const std::vector<float>& gpu_matrices, uint32_t first, uint32_t end, const float* sphere, uint32_t slot,
-                  uint32_t instance_base, float4& out_sphere, float4& out_rec) {
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, s[4];
-  double max_radius = 0;
-  for (uint32_t g = first; g < end; g++) {
-    instance_sphere(&gpu_matrices[static_cast<size_t>(g) * 16], sphere, s);
-    for (int a = 0; a < 3; a++) {
-      lo[a] = std::min(lo[a], s[a] - s[3]);
-      hi[a] = std::max(hi[a], s[a] + s[3]);
-    }
-    max_radius = std::max(max_radius, s[3]);
-  }
-  const double cx = (lo[0] + hi[0]) / 2, cy = (lo[1] + hi[1]) / 2, cz = (lo[2] + hi[2]) / 2;
-  double radius = 0;
-  for (uint32_t g = first; g < end; g++) {
-    instance_sphere(&gpu_matrices[static_cast<size_t>(g) * 16], sphere, s);
-    const double dx = s[0] - cx, dy = s[1] - cy, dz = s[2] - cz;
-    radius = std::max(radius, std::sqrt(dx * dx + dy * dy + dz * dz) + s[3]);
-  }
-  out_sphere = make_float4(static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz),
-                           static_cast<float>(radius * (1 + 1e-4) + 1e-3));
-  out_rec = make_float4(static_cast<float>(max_radius * (1 + 1e-4) + 1e-4), static_cast<float>(slot),
-                        static_cast<float>(instance_base + first), static_cast<float>(end - first));
 }
 
 void write_mesh_record(vg_scene* s, uint32_t slot) {
@@ -309,12 +271,15 @@ int upload_scene(vg_scene* s) {
   if (s->dirty_cells) {
     s->c_sphere.clear();
     s->c_rec.clear();
+    s->c_bound.clear();
     for (const MeshHost& m : s->meshes) {
       if (!m.used) continue;
       s->c_sphere.insert(s->c_sphere.end(), m.cell_sphere.begin(), m.cell_sphere.end());
       s->c_rec.insert(s->c_rec.end(), m.cell_rec.begin(), m.cell_rec.end());
+      s->c_bound.insert(s->c_bound.end(), m.cell_bound.begin(), m.cell_bound.end());
     }
-    if ((rc = upload(s, s->d_csphere, s->c_sphere)) || (rc = upload(s, s->d_crec, s->c_rec))) return rc;
+    if ((rc = upload(s, s->d_csphere, s->c_sphere)) || (rc = upload(s, s->d_crec, s->c_rec)) || (rc = upload(s, s->d_cbound, s->c_bound)))
+      return rc;
     s->dirty_cells = false;
   }
   return 0;
@@ -397,6 +362,7 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.i_col3 = static_cast<const float4*>(s->d_col[3].ptr);
   p.c_sphere = static_cast<const float4*>(s->d_csphere.ptr);
   p.c_rec = static_cast<const float4*>(s->d_crec.ptr);
+  p.c_bound = static_cast<const float4*>(s->d_cbound.ptr);
   p.cell_count = static_cast<uint32_t>(s->c_sphere.size());
   p.mesh_slots = s->mesh_slots;
   p.slots = s->slots;
@@ -907,7 +873,7 @@ void vg_scene_destroy(vg_scene* s) {
   lane_destroy(&s->lane);
   DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_rblock, &s->d_bspan, &s->d_blod, &s->d_bpar,
                     &s->d_berr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
-                    &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_counters, &s->d_args, &s->d_metrics,
+                    &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_cbound, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
                     &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
                     &s->d_reject_meshlets, &s->d_phase, &s->d_vpos, &s->d_vlist, &s->d_raster_chunks,
@@ -1120,6 +1086,7 @@ void repack_instances(vg_scene* s, uint32_t skip) {
     s->meshes[skip].instance_count = 0;
     s->meshes[skip].cell_sphere.clear();
     s->meshes[skip].cell_rec.clear();
+    s->meshes[skip].cell_bound.clear();
   }
   s->dirty_instances = s->dirty_cells = true;
 }
@@ -1137,6 +1104,7 @@ int vg_mesh_remove(vg_scene* s, uint32_t slot) {
     m.instance_count = 0;
     m.cell_sphere.clear();
     m.cell_rec.clear();
+    m.cell_bound.clear();
   }
   m.used = false;
   s->mesh_table[slot * kMeshStride + 5] = make_float4(0, 0, 0, 0);  // writeFlags({enabled: 0})
@@ -1173,9 +1141,12 @@ int vg_instances_set(vg_scene* s, uint32_t slot, const float* matrices, const fl
   const uint32_t cells = (count + kCellSize - 1) / kCellSize;
   m.cell_sphere.resize(cells);
   m.cell_rec.resize(cells);
-  for (uint32_t c = 0; c < cells; c++)
-    compute_cell(gpu, c * kCellSize, std::min(count, (c + 1) * kCellSize), m.bounding, slot, m.instance_base, m.cell_sphere[c],
-                 m.cell_rec[c]);
+  m.cell_bound.resize(cells);
+  for (uint32_t c = 0; c < cells; c++) {
+    const uint32_t first = c * kCellSize, end = std::min(count, (c + 1) * kCellSize);
+    compute_cell(gpu, first, end, m.bounding, slot, m.instance_base, m.cell_sphere[c], m.cell_rec[c]);
+    m.cell_bound[c] = cell_bound(gpu, first, end, m.bounding, m.cull_bounds, m.cell_sphere[c]);
+  }
   s->dirty_instances = s->dirty_cells = true;
   return VG_OK;
 }

@@ -28,7 +28,7 @@ struct MeshHost {
   uint32_t level_base = 0, level_count = 0;  // level_base in float4 units of the mesh table
   float bounding[4], lod_bounds[4], cull_bounds[4];
   uint32_t instance_base = 0, instance_count = 0;
-  std::vector<float4> cell_sphere, cell_rec;
+  std::vector<float4> cell_sphere, cell_rec, cell_bound;
 };
 
 struct ViewRecord;  // the device view record (vg::ViewDev), opaque here
@@ -115,13 +115,13 @@ struct vg_scene {
   uint64_t index_used = 0;  // indices of the meshes added so far
   std::vector<float4> i_col[4];
   std::vector<float> i_slot;  // mesh slot of each instance
-  std::vector<float4> c_sphere, c_rec;
+  std::vector<float4> c_sphere, c_rec, c_bound;
   bool dirty_table = true, dirty_meshlets = true, dirty_instances = true, dirty_cells = true;
   // Every instance matrix affine (last row 0, 0, 0, 1) with finite entries / every run's LOD spheres finite:
   // together they select the meshlet pass that takes the transformed w as 1.
   bool affine_instances = false, finite_runs = false;
   // device scene
-  vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec;
+  vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec, d_cbound;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
   vgi::DevBuf d_rblock, d_bspan, d_blod, d_bpar, d_berr;
   vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs

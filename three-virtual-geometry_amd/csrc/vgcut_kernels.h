@@ -102,6 +102,7 @@ struct ViewDev {
   // Bounds on the view matrix for the block test: its 3 x 3 part's spectral norm (Gershgorin on V^T V) and
   // largest row sum of magnitudes, and its largest translation magnitude.
   float norm_bound, row_bound, move_bound;
+  float frustum_n2, frustum_c;  // bounds on the frustum planes (frustum_bounds)
 };
 
 struct Params {
@@ -128,6 +129,7 @@ struct Params {
   const float4* __restrict__ i_col3;
   const float4* __restrict__ c_sphere;    // per cell: bounding sphere
   const float4* __restrict__ c_rec;       // (maxRadius, slot, firstInstance, count)
+  const float4* __restrict__ c_bound;     // (A, R_in, R_cull, 0): bounds of the cell's instances (cell_inside_bits)
   uint32_t cell_count;
   uint32_t mesh_slots;   // highest used slot + 1
   uint32_t slots;        // slot capacity
@@ -214,8 +216,78 @@ __host__ __device__ inline float4 mat_mul16(P m, float x, float y, float z, floa
 // llvm.sqrt.f32 and fdiv to IEEE-exact sequences on gfx950 (checked against the host by a GPU test).
 __host__ __device__ inline float len3(float x, float y, float z) { return __builtin_sqrtf((x * x + y * y) + z * z); }
 template <typename P>
-__device__ __forceinline__ float dot3(P n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
+__host__ __device__ inline float dot3(P n, float x, float y, float z) { return (n[0] * x + n[1] * y) + n[2] * z; }
 __host__ __device__ inline float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// frustumDistance, GeometryPool.ts:690-698. mode 0: plain, 1: swept (x 1), 2: swept x select(flag, 1, 0).
+template <typename V>
+__host__ __device__ inline float frustum_distance(const V& v, float cx, float cy, float cz, int mode, bool flag) {
+  float best = 1e30f;
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    const auto n = &v.planes[p * 4];
+    float d = dot3(n, cx, cy, cz) + n[3];
+    // sweep_reach[p] = fmaxf(n . sweep, 0), computed once per view by k_begin (x 1.0f is exact).
+    if (mode == 1) d = d + v.sweep_reach[p];
+    if (mode == 2) d = d + v.sweep_reach[p] * (flag ? 1.0f : 0.0f);
+    best = fminf(best, d);
+  }
+  return best;
+}
+
+// Bounds on a view's frustum planes for the inside test below: n2 = the largest |n|_2 of a plane normal
+// (at least 1; its binary32 square root, a few ulp, covered by 1 + 1e-6), c = the largest |constant| plus n2
+// times |sweep|_1 (so at least any plane's |constant| + |sweep_reach|). Computed once per view by k_begin.
+template <typename V>
+__host__ __device__ inline void frustum_bounds(V& v) {
+  float n2 = 1.0f, c = 0.0f;
+  for (int p = 0; p < 6; p++) {
+    const float* n = &v.planes[p * 4];
+    n2 = fmaxf(n2, __builtin_sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]) * 1.000001f);
+    c = fmaxf(c, fabsf(n[3]));
+  }
+  v.frustum_n2 = n2;
+  v.frustum_c = c + n2 * ((fabsf(v.sweep[0]) + fabsf(v.sweep[1])) + fabsf(v.sweep[2]));
+}
+
+// Which whole-cell answers of the instance pass's frustum tests a cell settles for one view (queue-entry
+// bits kQ*): kQInCamera when every instance's bounding sphere certainly passes the camera test
+// (frustum_distance(centre, 0) >= -radius), kQInShadow the same for the swept test (mode 1), kQCullCamera /
+// kQCullShadow when every instance's culling sphere is certainly inside (frustum_distance(centre, mode) >=
+// cull radius, the fast path's condition). `bound` (per cell, built on the host in binary64 from the binary32
+// matrices): x = A, the largest |c0_k x| + |c1_k y| + |c2_k z| + |c3_k| over instances, components and both
+// sphere centres (what the kernel's mat_mul rounds against); y = R_in >= |c_i - C| and z = R_cull >=
+// |cc_i - C| + cr_i around the cell centre C (binary32, as stored), for the exact (real) centres c_i, cc_i of the
+// instances' bounding and culling spheres and the culling spheres' radii cr_i.
+// With n2 >= |n|_2 (n2 >= 1) and D = the exact plane distance of C: the exact plane distance of c_i is at least
+// D - |n| |c_i - C| >= D - n2 R_in, and the kernel's binary32 value differs from the exact one by at most
+// sqrt(3) gamma_4 n2 A (rounding of the centre) + gamma_5 (|n|_1 |c_i|_inf + c) (the dot product), as D's
+// computed value here by gamma_5 (|n|_1 |C|_inf + c), with |n|_1 <= sqrt(3) n2 and gamma_5 < 3.1e-7: in all
+// under 1.1e-6 n2 (|C|_inf + R + A) + 6.2e-7 c. So a computed D >= n2 (1 + 1e-5) R_in + err, err = 1e-5
+// (2 n2 (|C|_inf + R + A) + c), leaves every instance's computed value >= 0 >= -r_i (radii are >= 0). For
+// the culling spheres, D >= n2 (1 + 1e-5) R_cull + err leaves each computed value above (1 + 1e-5) cr_i,
+// which bounds the kernel's binary32 cull radius (a product and a square root: a few ulp). NaN anywhere
+// gives no bits.
+constexpr uint32_t kQueueViewMask = (1u << 28) - 1u;
+constexpr uint32_t kQInCamera = 1u << 28, kQInShadow = 1u << 29, kQCullCamera = 1u << 30, kQCullShadow = 1u << 31;
+
+template <typename V>
+__host__ __device__ inline uint32_t cell_inside_bits(const V& v, const float4& sphere, const float4& bound, bool sweep) {
+  const float mag = fmaxf(fabsf(sphere.x), fmaxf(fabsf(sphere.y), fabsf(sphere.z))) + fmaxf(bound.y, bound.z);
+  const float err = 1e-5f * (2.0f * v.frustum_n2 * (mag + bound.x) + v.frustum_c);
+  const float reach = v.frustum_n2 * 1.00001f;
+  const float in_r = reach * bound.y + err, cull_r = reach * bound.z + err;
+  uint32_t bits = 0;
+  const float d0 = frustum_distance(v, sphere.x, sphere.y, sphere.z, 0, false);
+  if (d0 >= in_r) bits |= kQInCamera;
+  if (d0 >= cull_r) bits |= kQCullCamera;
+  if (sweep) {
+    const float d1 = frustum_distance(v, sphere.x, sphere.y, sphere.z, 1, false);
+    if (d1 >= in_r) bits |= kQInShadow;
+    if (d1 >= cull_r) bits |= kQCullShadow;
+  }
+  return bits;
+}
 
 // fdlibm e_logf.c (Sun Microsystems, 1993): x = 2^k (1 + f), log(1 + f) = f - s (f - R(z)),
 // s = f / (2 + f). Basic operations only, so the host and device give the same bits.

@@ -94,22 +94,6 @@ __device__ void load_bands(ViewDev& v, const uint32_t* state) {
   v.high[1] = blending ? t1 * width : t1;
 }
 
-// frustumDistance, GeometryPool.ts:690-698. mode 0: plain, 1: swept (x 1), 2: swept x select(flag, 1, 0).
-template <typename V>
-__device__ __forceinline__ float frustum_distance(const V& v, float cx, float cy, float cz, int mode, bool flag) {
-  float best = 1e30f;
-#pragma unroll
-  for (int p = 0; p < 6; p++) {
-    const auto n = &v.planes[p * 4];
-    float d = dot3(n, cx, cy, cz) + n[3];
-    // sweep_reach[p] = fmaxf(n . sweep, 0), computed once per view by k_begin (x 1.0f is exact).
-    if (mode == 1) d = d + v.sweep_reach[p];
-    if (mode == 2) d = d + v.sweep_reach[p] * (flag ? 1.0f : 0.0f);
-    best = fminf(best, d);
-  }
-  return best;
-}
-
 
 __device__ __forceinline__ uint32_t ceil_log2(uint32_t n) { return n <= 1 ? 0u : 32u - __clz(n - 1u); }
 
@@ -206,6 +190,7 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
   ViewDev& v = p.views[blockIdx.x];
   if (threadIdx.x < 6) v.sweep_reach[threadIdx.x] = fmaxf(dot3(&v.planes[threadIdx.x * 4], v.sweep[0], v.sweep[1], v.sweep[2]), 0.0f);
   if (threadIdx.x == 6) view_bounds(v);
+  if (threadIdx.x == 7) frustum_bounds(v);
   uint32_t* c = view_counters(p, v.slot);
   for (uint32_t i = threadIdx.x; i < kPerMeshCounters + 2 * p.slots * 4; i += blockDim.x) c[i] = 0;
   if (p.phase == 2) {
@@ -272,21 +257,20 @@ __global__ __launch_bounds__(256) void k_begin(Params p) {
 // testing every view in turn would; the per-thread view loop is 4 times shorter.
 __global__ __launch_bounds__(256) void k_cells(Params p) {
   __shared__ uint32_t visible[64];
-  __shared__ unsigned long long cell_mask[64];
+  __shared__ unsigned long long cell_mask[5][64];  // kept views, then the views of each kQ* bit
   const uint32_t lane = lane_id(), group = threadIdx.x >> 6;
   const uint32_t cell = blockIdx.x * 64 + lane;
   const uint32_t v_first = blockIdx.y * 64, v_end = min(v_first + 64, p.view_count);
   const uint32_t g_first = v_first + group * 16, g_end = min(g_first + 16, v_end);
-  if (threadIdx.x < 64) {
-    visible[threadIdx.x] = 0;
-    cell_mask[threadIdx.x] = 0;
-  }
+  if (threadIdx.x < 64) visible[threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < 5 * 64; i += blockDim.x) (&cell_mask[0][0])[i] = 0;
   __syncthreads();
   const bool valid = cell < p.cell_count;
-  float4 sphere = make_float4(0, 0, 0, 0), record = sphere, flags = sphere, levers = sphere;
+  float4 sphere = make_float4(0, 0, 0, 0), record = sphere, flags = sphere, levers = sphere, bound = sphere;
   if (valid) {
     sphere = p.c_sphere[cell];
     record = p.c_rec[cell];
+    bound = p.c_bound[cell];
     const uint32_t slot = static_cast<uint32_t>(record.y);
     flags = p.mesh_table[slot * kMeshStride + 5];
     levers = p.mesh_table[slot * kMeshStride + 3];
@@ -294,7 +278,7 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
   const bool enabled = valid && flags.x != 0.0f;
   const bool sweep = flags.y > 0.5f;
   const float radius = sphere.w, max_radius = record.x;
-  uint64_t mask = 0;
+  uint64_t mask = 0, inside[4] = {0, 0, 0, 0};
   if (__ballot(enabled)) {
     for (uint32_t vi = g_first; vi < g_end; vi++) {
       const auto& v = scalar(p.views)[vi];
@@ -308,21 +292,38 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
       // second phase: only cells holding an instance the cut rejected by occlusion
       if (p.phase == 2 && keep) keep = any_rejected(p, v.slot, static_cast<uint32_t>(record.z), min(static_cast<uint32_t>(record.w), kCellSize));
       if (keep) mask |= 1ull << (vi - v_first);
+      // Frustum answers the whole cell settles for the instance pass (bounds on the cell's instances).
+      if (v.frustum == 1 && __ballot(keep)) {
+        const uint32_t bits = keep ? cell_inside_bits(v, sphere, bound, sweep) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+          if (bits & (kQInCamera << k)) inside[k] |= 1ull << (vi - v_first);
+      }
       const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(keep)));
       if (lane == 0 && n) atomicAdd(&visible[vi - v_first], n);
     }
   }
-  if (mask) atomicOr(&cell_mask[lane], mask);
+  if (mask) atomicOr(&cell_mask[0][lane], mask);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++)
+    if (inside[k]) atomicOr(&cell_mask[1 + k][lane], inside[k]);
   __syncthreads();
   if (group == 0) {
-    const uint64_t m_all = cell_mask[lane];
+    const uint64_t m_all = cell_mask[0][lane];
+    const uint64_t m_in[4] = {cell_mask[1][lane], cell_mask[2][lane], cell_mask[3][lane], cell_mask[4][lane]};
     const uint32_t n = static_cast<uint32_t>(__popcll(m_all));
     const uint32_t incl = wave_scan(n);
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
     uint32_t base = 0;
     if (lane == 0 && total) base = atomicAdd(p.cell_queue_count, total);
     uint32_t at = uniform(base) + incl - n;
-    for (uint64_t m = m_all; m; m &= m - 1) p.cell_queue[at++] = make_uint2(v_first + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1), cell);
+    for (uint64_t m = m_all; m; m &= m - 1) {
+      const uint32_t k = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+      uint32_t bits = 0;
+#pragma unroll
+      for (uint32_t b = 0; b < 4; b++) bits |= ((m_in[b] >> k) & 1ull) ? (kQInCamera << b) : 0u;
+      p.cell_queue[at++] = make_uint2((v_first + k) | bits, cell);
+    }
   }
   if (threadIdx.x < v_end - v_first && visible[threadIdx.x]) {
     const uint32_t slot = scalar(p.views)[v_first + threadIdx.x].slot;
@@ -513,7 +514,8 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
   uint32_t high_water = 0;
   uint2 entry = scalar_u2(p.cell_queue, q_begin);
   for (uint32_t q = q_begin; q < q_end; q++) {
-    const uint32_t vi = uniform(entry.x);
+    const uint32_t vi = uniform(entry.x) & kQueueViewMask;
+    const uint32_t qbits = uniform(entry.x) & ~kQueueViewMask;  // frustum answers settled for the whole cell
     const uint32_t cell = uniform(entry.y);
     if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next entry, loaded while this one runs
     const auto& v = scalar(p.views)[vi];
@@ -609,9 +611,10 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
       const float3 center = make_float3(d_center[r][0], d_center[r][1], d_center[r][2]);
       const float radius = object_sphere.w * scale;
       const bool no_frustum = v.frustum == 0;
-      bool in_camera = valid && redo_camera && (no_frustum || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
+      bool in_camera = valid && redo_camera &&
+                       (no_frustum || (qbits & kQInCamera) || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
       const bool in_shadow = valid && redo_shadow && shadow_on &&
-                             (no_frustum || frustum_distance(v, center.x, center.y, center.z, 1, false) >= -radius);
+                             (no_frustum || (qbits & kQInShadow) || frustum_distance(v, center.x, center.y, center.z, 1, false) >= -radius);
       const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
       const float view_distance = len3(vc.x, vc.y, vc.z);
       const bool too_small = view_distance > radius && (radius * v.proj_scale) / view_distance < min_pixels;
@@ -649,7 +652,8 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
         const bool survived = in_cut[cut] && lr[cut].last > lr[cut].first;
         const bool fast = survived && lr[cut].possible == lr[cut].last - lr[cut].first && lr[cut].guaranteed == lr[cut].possible &&
                           lr[cut].possible <= kFastPathMax &&
-                          (no_frustum || frustum_distance(v, cc.x, cc.y, cc.z, cut == 1 ? 1 : 0, false) >= cull_radius);
+                          (no_frustum || (qbits & (cut == 1 ? kQCullShadow : kQCullCamera)) ||
+                           frustum_distance(v, cc.x, cc.y, cc.z, cut == 1 ? 1 : 0, false) >= cull_radius);
         work[cut] = survived && !fast;
         acc_survived[cut] += static_cast<uint32_t>(__popcll(__ballot(survived)));
         acc_work[cut] += static_cast<uint32_t>(__popcll(__ballot(work[cut])));
@@ -1142,7 +1146,8 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   uint32_t high_water = 0;  // the most items the partition held after one of this wave's appends
   for (uint32_t q = static_cast<uint32_t>(queue_count * wrank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
-    const uint32_t vi = uniform(entry.x);
+    const uint32_t vi = uniform(entry.x) & kQueueViewMask;
+    const bool cell_in_camera = (uniform(entry.x) & kQInCamera) != 0;  // settled for the whole cell by k_cells
     const uint32_t cell = uniform(entry.y);
     const ViewDev& v = p.views[vi];
     if (!v.budget) continue;
@@ -1171,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
       const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
       const float radius = object_sphere.w * scale;
-      bool keep = valid && (v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
+      bool keep = valid && (v.frustum == 0 || cell_in_camera || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
       const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
       const float view_distance = len3(vc.x, vc.y, vc.z);
       keep = keep && !(view_distance > radius && (radius * v.proj_scale) / view_distance < min_pixels);
