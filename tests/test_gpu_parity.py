@@ -256,19 +256,24 @@ def test_graph_replays_match_eager(gpu_available):
         compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"graph frame {f}")
 
 
-def test_triangle_budget_pick_and_cut(gpu_available):
+@pytest.mark.parametrize("copies", [1, 6])
+def test_triangle_budget_pick_and_cut(gpu_available, copies):
     """Budget views pick the oracle's threshold on the GPU and cut exactly as the oracle does at it; a view
-    without a budget in the same launch is unaffected."""
+    without a budget in the same launch is unaffected. 3 views keep every view's histogram in the block's
+    LDS; 18 views are past that limit, and each wave keeps its current view's histogram."""
     spec = small_ruins(seed=61, shadows=True)
-    loaded = load_gpu(spec, max_views=3, max_triangles=20_000_000, max_shadow=10_000_000)
+    loaded = load_gpu(spec, max_views=3 * copies, max_triangles=20_000_000, max_shadow=10_000_000)
     bins = loaded.ctx.config.budget_bins
     t = native.budget_thresholds(0.5, 3.0, bins)
     views = []
-    for slot, (eye, target, budget) in enumerate([((0.0, 4.0, -60.0), (0.0, 0.0, 40.0), 300_000),
-                                                  ((30.0, 15.0, 20.0), (-40.0, 0.0, -10.0), 50_000),
-                                                  ((-20.0, 3.0, 0.0), (40.0, 0.0, 5.0), 0)]):
+    poses = [((0.0, 4.0, -60.0), (0.0, 0.0, 40.0), 300_000), ((30.0, 15.0, 20.0), (-40.0, 0.0, -10.0), 50_000),
+             ((-20.0, 3.0, 0.0), (40.0, 0.0, 5.0), 0)]
+    for slot in range(3 * copies):
+        eye, target, budget = poses[slot % 3]
+        k = slot // 3
+        eye = (eye[0] + 3.0 * k, eye[1], eye[2] - 2.0 * k)
         camera = ground_camera(eye, target)
-        views.append(make_view(camera, 1080, slot=slot, triangle_budget=budget, threshold_range=(0.5, 3.0),
+        views.append(make_view(camera, 1080, slot=slot, triangle_budget=budget * (1 + k) // 1, threshold_range=(0.5, 3.0),
                                shadow_sweep=(10.0, -30.0, 0.0)))
     for graph in (False, True):
         for v in views:

@@ -389,6 +389,7 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.budget_thresholds = static_cast<const float*>(lane->d_budget_thresholds.ptr);
   p.budget_hist = static_cast<unsigned long long*>(lane->d_budget_hist.ptr);
   p.budget_bins = ctx->cfg.budget_bins;
+  p.budget_block_hist = view_count * (ctx->cfg.budget_bins + 1) <= kBudgetBlockHistWords ? 1u : 0u;
   p.i_slot = static_cast<const uint32_t*>(s->d_islot.ptr);
   p.m_first = static_cast<const uint32_t*>(s->d_first.ptr);
   p.m_tri = static_cast<const uint32_t*>(s->d_tri.ptr);
@@ -481,7 +482,10 @@ int enqueue_cut(vg_scene* s, Lane* lane, uint32_t view_count, bool capturing) {
   if (lane->launch_budget) {
     hipLaunchKernelGGL(k_budget_instances, dim3(persistent_instances), dim3(256), 0, st, p);
     void (*const budget_meshlets)(Params) = affine_scene(s) ? k_budget_meshlets<true> : k_budget_meshlets<false>;
-    hipLaunchKernelGGL(budget_meshlets, dim3(persistent_meshlets), dim3(256), 0, st, p);
+    // Dynamic LDS: one 32-bit histogram per launched view for the block, or a 64-bit one per wave.
+    const size_t lds = p.budget_block_hist ? sizeof(uint32_t) * view_count * (p.budget_bins + 1)
+                                           : sizeof(unsigned long long) * 4 * (p.budget_bins + 1);
+    hipLaunchKernelGGL(budget_meshlets, dim3(persistent_meshlets), dim3(256), lds, st, p);
     hipLaunchKernelGGL(k_budget_pick, dim3(view_count), dim3(64), 0, st, p);
   }
   mark();

@@ -39,6 +39,13 @@ enum : uint32_t {
   M_BUDGET_TESTS = 7, M_THRESHOLD = 8, M_BUDGET_BIN = 9
 };
 constexpr uint32_t kMaxBudgetBins = 1024;
+// k_budget_meshlets keeps one histogram per launched view in its block's LDS (32-bit partial sums, flushed by
+// exchange) when they fit in this many words; otherwise each wave keeps its current view's histogram.
+constexpr uint32_t kBudgetBlockHistWords = 4352;  // 17 KB: 16 views at the default 256 bins
+// A wave flushes its block's partial sums after adding this many meshlets (of at most 255 triangles each):
+// between two flushes, each of the four waves adds at most 2^20 + 64 meshlets, so any one word moves by less
+// than 4 * (2^20 + 64) * 255 < 2^31.
+constexpr uint32_t kBudgetFlushMeshlets = 1u << 20;
 // A LOD run: consecutive meshlets of one level whose LOD test has the same inputs (same LOD and parent
 // spheres and errors; a zero error projects to 0 and an error >= 5e29 to 1e30 whatever the sphere). The
 // meshlet pass tests each run once. Runs hold at most kMaxRun meshlets.
@@ -162,6 +169,7 @@ struct Params {
   const float* budget_thresholds;  // bins, ascending
   unsigned long long* budget_hist; // per view: bins + 1
   uint32_t budget_bins;
+  uint32_t budget_block_hist;      // 1: k_budget_meshlets keeps every launched view's histogram in its block's LDS
   // expand pass (GeometryPool.ts:997-1021)
   const uint32_t* __restrict__ i_slot;   // mesh slot of each instance
   const uint32_t* __restrict__ m_first;  // meshletInfo.y: first index of each meshlet's triangles (pool)

@@ -1225,36 +1225,63 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   if (lane == 0 && high_water) atomicMax(&p.flags[4], high_water);
 }
 
-// The budget histogram over LOD blocks, in stages per wave as the meshlet pass: a lane per block of the
+// The budget histogram over LOD blocks, in stages per wave as the meshlet pass: (1) a lane per block of the
 // budget items; on affine scenes the block test against the band [T[0], T[B-1]] times the camera scale
 // drops blocks none of whose runs can add to a bin (every run's own error above the band, so its first
 // bin is B, or every parent's at or below it, so its last is 0); the runs of the other blocks are queued
-// per wave in LDS and binned 64 at a time: a run's own and parent errors give the bins [a, e) of the
-// thresholds where its meshlets are in the plain cut, and each visible meshlet of a run with a < e adds
-// its triangles there.
+// per wave in LDS. (2) A lane per queued run, 64 at a time: its own and parent errors give the bins [a, e)
+// of the thresholds where its meshlets are in the plain cut; the meshlets of runs with a < e go to a
+// per-wave LDS ring. (3) A lane per ring meshlet, 64 at a time: each visible one adds its triangles to bins
+// [a, e) (its inputs loaded before the per-view loop, the view's HZB level rows copied to LDS).
 template <bool kAffine>
 __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
-  // Per-wave LDS histogram of the wave's current view, flushed to the view's global histogram when
-  // the view changes and at the end; lanes of another view add to global memory directly.
-  __shared__ unsigned long long hist[4][kMaxBudgetBins + 1];
+  // The LDS histograms (dynamic LDS). With p.budget_block_hist: one per launched view, shared by the block's
+  // waves, 32-bit two's-complement partial sums added with LDS atomics and moved to the global histograms by
+  // exchange (any wave, any time: no add is lost), after every kBudgetFlushMeshlets meshlets a wave adds and
+  // at the end. Otherwise, per wave: the histogram of the wave's current view (bins + 1 64-bit words),
+  // flushed when the view changes and at the end; lanes of another view add to global memory directly.
+  extern __shared__ unsigned long long budget_lds[];
+  // Per wave: meshlets of runs that add to some bin, (instance, meshlet, tag, instance scale bits), and their
+  // bins a | e << 16.
+  __shared__ uint4 ring[4][kQueue];
+  __shared__ uint32_t ring_bins[4][kQueue];
+  __shared__ float4 hzb_rows[4][kHzbMaxLevels];
   // Per wave: blocks that passed and the position of each one's first run in the stream of queued runs
   // (as in the meshlet pass).
   __shared__ uint4 bq[4][kBlockQueue];
   __shared__ uint32_t bq_start[4][kBlockQueue];
   const uint32_t wib = threadIdx.x / 64, lane = lane_id();
   const uint32_t bins = p.budget_bins;
-  for (uint32_t i = lane; i <= bins; i += 64) hist[wib][i] = 0;
+  const bool block_hist = p.budget_block_hist != 0;  // grid-uniform
+  unsigned long long* hist = budget_lds + static_cast<size_t>(wib) * (bins + 1);
+  uint32_t* block_words = reinterpret_cast<uint32_t*>(budget_lds);
+  const uint32_t block_count = p.view_count * (bins + 1);
+  if (block_hist) {
+    for (uint32_t i = threadIdx.x; i < block_count; i += blockDim.x) block_words[i] = 0;
+    __syncthreads();
+  } else {
+    for (uint32_t i = lane; i <= bins; i += 64) hist[i] = 0;
+  }
+  uint32_t added = 0;  // meshlets this wave added to the block histograms since its last flush
+  auto block_flush = [&]() __attribute__((always_inline)) {
+    for (uint32_t i = lane; i < block_count; i += 64) {
+      const int32_t h = static_cast<int32_t>(atomicExch(&block_words[i], 0u));
+      if (h) atomicAdd(&p.budget_hist[i], static_cast<unsigned long long>(static_cast<int64_t>(h)));
+    }
+    added = 0;
+  };
   uint32_t current = 0xffffffffu;
   auto flush = [&]() __attribute__((always_inline)) {
     if (current != 0xffffffffu) {
       unsigned long long* g = p.budget_hist + static_cast<size_t>(current) * (bins + 1);
       for (uint32_t i = lane; i <= bins; i += 64) {
-        const unsigned long long h = hist[wib][i];
+        const unsigned long long h = hist[i];
         if (h) atomicAdd(&g[i], h);
-        hist[wib][i] = 0;
+        hist[i] = 0;
       }
     }
   };
+  uint32_t head = 0, tail = 0;  // ring positions (wave-uniform, running freely)
   uint32_t bq_head = 0, bq_tail = 1, rq_pos = 0, rq_end = 0;
   if (lane == 0) {
     bq[wib][0] = make_uint4(0, 0, 0, 0);
@@ -1266,7 +1293,80 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  // Bins the next n (<= 64) queued runs, one per lane.
+  // Tests the next n (<= 64) ring meshlets, one per lane, and adds the visible ones' triangles to their bins.
+  auto drain = [&](uint32_t n) __attribute__((always_inline)) {
+    const bool active = lane < n;
+    const uint4 c = active ? ring[wib][(head + lane) & (kQueue - 1)] : make_uint4(0, 0, 0, 0);
+    const uint32_t ab = active ? ring_bins[wib][(head + lane) & (kQueue - 1)] : 0u;
+    const uint32_t a = ab & 0xffffu, e = ab >> 16;
+    const uint32_t instance = c.x, meshlet = c.y, tag = c.z, vi = tag >> kTagViewShift;
+    const float scale = __uint_as_float(c.w);
+    const bool occ_on = (tag & kTagOcclusion) != 0;
+    uint32_t meshlet_tris = 0;
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, cs = c0;
+    if (active) {
+      meshlet_tris = ld32(p.m_tris, meshlet);
+      c0 = ld32(p.i_col0, instance);
+      c1 = ld32(p.i_col1, instance);
+      c2 = ld32(p.i_col2, instance);
+      c3 = ld32(p.i_col3, instance);
+      cs = ld32(p.m_cull, meshlet);
+    }
+    bool visible = false;
+    for (uint64_t pending = __ballot(active); pending;) {  // one launched view at a time (uniform constants)
+      const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
+      const bool mine = active && vi == vu;
+      const auto& v = scalar(p.views)[uniform(vu)];
+      if (v.hzb_valid && __ballot(mine && occ_on)) {
+        reinterpret_cast<float*>(hzb_rows[wib])[lane] = scalar(p.hzbs)[v.slot][lane];
+        wave_sync();
+      }
+      if (mine) {
+        const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
+        visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -(cs.w * scale);
+        if (visible && occ_on && v.hzb_valid) {
+          const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
+          const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
+          visible = !occluded_by(v, scalar(p.hzbs)[v.slot], HzbLevelsLds{hzb_rows[wib]}, vc.x, vc.y, vc.z, cs.w * scale + margin);
+        }
+      }
+      pending &= ~__ballot(mine);
+    }
+    head += n;
+    const uint32_t tris = visible ? meshlet_tris : 0u;
+    const bool add = tris != 0;
+    if (block_hist) {
+      if (add) {
+        atomicAdd(&block_words[vi * (bins + 1) + a], tris);
+        atomicAdd(&block_words[vi * (bins + 1) + e], 0u - tris);
+      }
+      added += n;
+      if (added >= kBudgetFlushMeshlets) block_flush();
+      return;
+    }
+    // Switch the wave's LDS histogram to the first contributing view when none of the lanes adds to the current one.
+    const uint64_t contrib = __ballot(add);
+    if (contrib) {
+      const uint32_t lead_view = __shfl(vi, __ffsll(static_cast<unsigned long long>(contrib)) - 1, 64);
+      if (__ballot(add && vi == current) == 0ull && lead_view != current) {
+        flush();
+        current = lead_view;
+      }
+    }
+    if (add) {
+      if (vi == current) {
+        atomicAdd(&hist[a], static_cast<unsigned long long>(tris));
+        atomicAdd(&hist[e], static_cast<unsigned long long>(0) - tris);
+      } else {
+        unsigned long long* g = p.budget_hist + static_cast<size_t>(vi) * (bins + 1);
+        atomicAdd(&g[a], static_cast<unsigned long long>(tris));
+        atomicAdd(&g[e], static_cast<unsigned long long>(0) - tris);
+      }
+    }
+  };
+
+  // Bins the next n (<= 64) queued runs, one per lane; the meshlets of runs that add to a bin go to the ring
+  // (the lanes' k-th meshlets together), which is drained whenever 64 wait (at most 63 + 64 are queued).
   auto run_stage = [&](uint32_t n) __attribute__((always_inline)) {
     wave_sync();
     const uint32_t e_next = bq_head + 1 + lane;
@@ -1285,7 +1385,6 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
     const uint32_t instance = q.x & 0xffffffu, tag = q.z, run = q.y + (rq_pos + lane - first_pos);
     const float scale = __uint_as_float(q.w);
     const uint32_t vi = tag >> kTagViewShift;
-    const bool occ_on = (tag & kTagOcclusion) != 0;
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0, c3 = c0, lod_sphere = c0, parent_sphere = c0;
     float2 err = make_float2(0.0f, 0.0f);
     uint2 span = make_uint2(0, 0);
@@ -1299,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
       parent_sphere = ld32(p.r_parent, run);
       span = ld32(p.r_span, run);
     }
-    uint32_t a = 0, e = 0, tris = 0;
+    uint32_t a = 0, e = 0;
     for (uint64_t pending = __ballot(valid); pending;) {  // one launched view at a time (uniform constants)
       const uint32_t vu = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vi), __ffsll(static_cast<unsigned long long>(pending)) - 1));
       const bool mine = valid && vi == vu;
@@ -1310,20 +1409,6 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
         const float* t = p.budget_thresholds + static_cast<size_t>(uniform(vu)) * bins;
         a = bin_of(t, bins, v.budget_scale, v.budget_inv, own);
         e = bin_of(t, bins, v.budget_scale, v.budget_inv, parent);
-        if (a < e) {
-          for (uint32_t k = 0; k < span.y; k++) {
-            const uint32_t meshlet = span.x + k;
-            const float4 cs = p.m_cull[meshlet];
-            const float4 center = mat_mul(c0, c1, c2, c3, cs.x, cs.y, cs.z, 1.0f);
-            bool visible = v.frustum == 0 || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -(cs.w * scale);
-            if (visible && occ_on && v.hzb_valid) {
-              const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
-              const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
-              visible = !occluded(v, scalar(p.hzbs)[v.slot], vc.x, vc.y, vc.z, cs.w * scale + margin);
-            }
-            if (visible) tris += p.m_tris[meshlet];
-          }
-        }
       }
       pending &= ~__ballot(mine);
     }
@@ -1332,24 +1417,21 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
     const bool at_end = __ballot(dist == 64) != 0 && n >= 64;
     bq_head += static_cast<uint32_t>(__popcll(starts & upto_n)) + (at_end ? 1u : 0u);
     rq_pos += n;
-    const bool add = tris != 0;
-    // Switch the wave's LDS histogram to the chunk's first contributing view when needed.
-    const uint64_t contrib = __ballot(add);
-    if (contrib) {
-      const uint32_t lead_view = __shfl(vi, __ffsll(static_cast<unsigned long long>(contrib)) - 1, 64);
-      if (__ballot(add && vi == current) == 0ull && lead_view != current) {
-        flush();
-        current = lead_view;
+    const uint32_t count = valid && a < e ? span.y : 0u;
+    for (uint32_t k = 0; k < kMaxRun; k++) {
+      const bool has = k < count;
+      const uint64_t round = __ballot(has);
+      if (!round) break;
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(round >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(round), 0u));
+      if (has) {
+        ring[wib][(tail + below) & (kQueue - 1)] = make_uint4(instance, span.x + k, tag, __float_as_uint(scale));
+        ring_bins[wib][(tail + below) & (kQueue - 1)] = a | (e << 16);
       }
-    }
-    if (add) {
-      if (vi == current) {
-        atomicAdd(&hist[wib][a], static_cast<unsigned long long>(tris));
-        atomicAdd(&hist[wib][e], static_cast<unsigned long long>(0) - tris);
-      } else {
-        unsigned long long* g = p.budget_hist + static_cast<size_t>(vi) * (bins + 1);
-        atomicAdd(&g[a], static_cast<unsigned long long>(tris));
-        atomicAdd(&g[e], static_cast<unsigned long long>(0) - tris);
+      tail += static_cast<uint32_t>(__popcll(round));
+      if (tail - head >= 64) {
+        wave_sync();
+        drain(64);
+        wave_sync();
       }
     }
   };
@@ -1403,7 +1485,19 @@ __global__ __launch_bounds__(256) void k_budget_meshlets(Params p) {
     }
   });
   if (rq_end != rq_pos) run_stage(rq_end - rq_pos);
-  flush();
+  if (tail != head) {
+    wave_sync();
+    drain(tail - head);
+  }
+  if (block_hist) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < block_count; i += blockDim.x) {
+      const int32_t h = static_cast<int32_t>(block_words[i]);
+      if (h) atomicAdd(&p.budget_hist[i], static_cast<unsigned long long>(static_cast<int64_t>(h)));
+    }
+  } else {
+    flush();
+  }
 }
 
 // One block per view with a budget: prefix over the bins, pick the threshold, redo the bands.
