@@ -235,6 +235,7 @@ int upload_scene(vg_scene* s) {
         (rc = upload(s, s->d_rparent, s->r_parent)) || (rc = upload(s, s->d_rerr, s->r_err)) ||
         (rc = upload(s, s->d_rblock, s->r_block)) || (rc = upload(s, s->d_bspan, s->b_span)) ||
         (rc = upload(s, s->d_blod, s->b_lod)) || (rc = upload(s, s->d_bpar, s->b_par)) || (rc = upload(s, s->d_berr, s->b_err)) ||
+        (rc = upload(s, s->d_level_blocks, s->level_blocks)) ||
         (rc = upload(s, s->d_first, s->m_first)) || (rc = upload(s, s->d_tri, s->m_tri)) ||
         (rc = upload(s, s->d_vlist, s->m_vlist)))
       return rc;
@@ -349,6 +350,7 @@ Params make_params(vg_scene* s, Lane* lane, uint32_t view_count) {
   p.m_run = static_cast<const uint32_t*>(s->d_run.ptr);
   p.r_span = static_cast<const uint2*>(s->d_rspan.ptr);
   p.r_block = static_cast<const uint32_t*>(s->d_rblock.ptr);
+  p.level_blocks = static_cast<const uint2*>(s->d_level_blocks.ptr);
   p.b_span = static_cast<const uint2*>(s->d_bspan.ptr);
   p.b_lod = static_cast<const float4*>(s->d_blod.ptr);
   p.b_par = static_cast<const float4*>(s->d_bpar.ptr);
@@ -875,7 +877,7 @@ void vg_scene_destroy(vg_scene* s) {
   if (!s) return;
   if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
   lane_destroy(&s->lane);
-  DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_rblock, &s->d_bspan, &s->d_blod, &s->d_bpar,
+  DevBuf* bufs[] = {&s->d_run, &s->d_rspan, &s->d_rlod, &s->d_rparent, &s->d_rerr, &s->d_rblock, &s->d_level_blocks, &s->d_bspan, &s->d_blod, &s->d_bpar,
                     &s->d_berr, &s->d_mesh_table, &s->d_cull, &s->d_lod, &s->d_parent, &s->d_err, &s->d_tris, &s->d_col[0], &s->d_col[1],
                     &s->d_col[2], &s->d_col[3], &s->d_csphere, &s->d_crec, &s->d_cbound, &s->d_counters, &s->d_args, &s->d_metrics,
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
@@ -1040,11 +1042,19 @@ int vg_mesh_add(vg_scene* s, const vg_mesh_desc* mesh, uint32_t* slot_out) {
   m.level_base = s->slots * kMeshStride + s->level_used;
   s->level_used += L * 2;
   s->mesh_table.resize(static_cast<size_t>(s->slots) * kMeshStride + s->level_used);
+  // Per level, the LOD blocks of its first meshlet and of its last one (the instance pass's work items cover
+  // the blocks from the first possible level's first meshlet to the last possible level's last one).
+  s->level_blocks.resize(s->level_used / 2);
   for (uint32_t l = 0; l < L; l++) {
     s->mesh_table[m.level_base + l * 2] =
         make_float4(static_cast<float>(mesh->level_ranges[l * 2] + m.meshlet_base), static_cast<float>(mesh->level_ranges[l * 2 + 1]),
                     mesh->level_errors[l * 2], mesh->level_errors[l * 2 + 1]);
     s->mesh_table[m.level_base + l * 2 + 1] = make_float4(mesh->level_guard_errors[l * 2], mesh->level_guard_errors[l * 2 + 1], 0, 0);
+    const uint64_t first = static_cast<uint64_t>(mesh->level_ranges[l * 2]) + m.meshlet_base, end = first + mesh->level_ranges[l * 2 + 1];
+    uint2 blocks = make_uint2(0, 0);
+    if (first < s->m_run.size()) blocks.x = s->r_block[s->m_run[first]];
+    if (end >= 1 && end - 1 < s->m_run.size()) blocks.y = s->r_block[s->m_run[end - 1]] + 1;
+    s->level_blocks[(m.level_base - s->slots * kMeshStride) / 2 + l] = blocks;
   }
   write_mesh_record(s, slot);
   // writeMeshRecord defaults (GeometryPool.ts:350): no draw distance, min pixel radius 0, enabled, shadow, no occlusion.

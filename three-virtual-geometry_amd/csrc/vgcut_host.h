@@ -110,6 +110,7 @@ struct vg_scene {
   std::vector<float4> r_lod, r_parent;
   std::vector<float2> r_err;
   std::vector<uint32_t> r_block;  // LOD blocks (see kBlockRuns)
+  std::vector<uint2> level_blocks;  // per level-table entry pair: the level's first block, last block + 1
   std::vector<uint2> b_span;
   std::vector<float4> b_lod, b_par, b_err;
   uint64_t index_used = 0;  // indices of the meshes added so far
@@ -123,7 +124,7 @@ struct vg_scene {
   // device scene
   vgi::DevBuf d_mesh_table, d_cull, d_lod, d_parent, d_err, d_tris, d_col[4], d_csphere, d_crec, d_cbound;
   vgi::DevBuf d_run, d_rspan, d_rlod, d_rparent, d_rerr;
-  vgi::DevBuf d_rblock, d_bspan, d_blod, d_bpar, d_berr;
+  vgi::DevBuf d_rblock, d_bspan, d_blod, d_bpar, d_berr, d_level_blocks;
   vgi::DevBuf d_first, d_tri, d_islot;  // expand pass inputs
   // Occluder raster inputs: each meshlet's vertex positions in its vertex-list order (x, y, z, 0), so a
   // meshlet's are contiguous; the vertex lists' (offset, count).

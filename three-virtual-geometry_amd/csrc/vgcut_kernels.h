@@ -126,6 +126,7 @@ struct Params {
   const float4* __restrict__ r_parent;    // parent LOD sphere
   const float2* __restrict__ r_err;       // (error, parentError)
   const uint32_t* __restrict__ r_block;   // block of each run
+  const uint2* __restrict__ level_blocks; // per level (mesh-table level entry pair): block of its first meshlet, block of its last + 1
   const uint2* __restrict__ b_span;       // per block: (first run, run count)
   const float4* __restrict__ b_lod;       // per block: centre and radius of a sphere holding its runs' LOD-sphere centres
   const float4* __restrict__ b_par;       // the same for their parent spheres' centres

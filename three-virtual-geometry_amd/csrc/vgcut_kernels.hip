@@ -335,12 +335,13 @@ __global__ __launch_bounds__(256) void k_cells(Params p) {
 
 struct LevelResult {
   uint32_t first, last, possible, guaranteed;
+  uint32_t levels;  // select_levels: the level giving `first` | the level giving `last` << 8
 };
 
 // selectLevels (GeometryPool.ts:771-800) for one band [low, high].
 __device__ __forceinline__ LevelResult select_band(const Params& p, uint32_t level_base, uint32_t level_count, float low, float high,
                                                    float pmin, float pmax) {
-  LevelResult r{0xffffffffu, 0u, 0u, 0u};
+  LevelResult r{0xffffffffu, 0u, 0u, 0u, 0u};
   for (uint32_t i = 0; i < level_count; i++) {
     const float4 l0 = p.mesh_table[level_base + i * 2];
     const float4 l1 = p.mesh_table[level_base + i * 2 + 1];
@@ -370,7 +371,9 @@ template <bool kBoth, typename V>
 __device__ __forceinline__ void select_levels(const float4* levels, const V& v, uint32_t first, uint32_t last, float pmin,
                                               float pmax, LevelResult (&r)[2]) {
   const float low0 = v.low[0], high0 = v.high[0], low1 = v.low[1], high1 = v.high[1];
-  r[0] = r[1] = LevelResult{0xffffffffu, 0u, 0u, 0u};
+  r[0] = r[1] = LevelResult{0xffffffffu, 0u, 0u, 0u, 0u};
+  // first / last as min / max over the possible levels; `levels` records which level gave each (ties give
+  // equal values, so either level's blocks are the same).
 #pragma unroll 2
   for (uint32_t i = first; i <= last; i++) {
     const float4 l0 = levels[i * 2];
@@ -383,6 +386,8 @@ __device__ __forceinline__ void select_levels(const float4* levels, const V& v, 
       const bool possible = err_max <= high0 && (open_parent || parent_min > low0);
       const bool sure = guard_min <= low0 * VG_MARGIN_LO && (open_guard || guard_max > high0 * VG_MARGIN_HI);
       if (possible) {
+        if (level_first < r[0].first) r[0].levels = (r[0].levels & ~255u) | i;
+        if (level_first + size > r[0].last) r[0].levels = (r[0].levels & 255u) | (i << 8);
         r[0].first = min(r[0].first, level_first);
         r[0].last = max(r[0].last, level_first + size);
         r[0].possible += size;
@@ -393,6 +398,8 @@ __device__ __forceinline__ void select_levels(const float4* levels, const V& v, 
       const bool possible = err_max <= high1 && (open_parent || parent_min > low1);
       const bool sure = guard_min <= low1 * VG_MARGIN_LO && (open_guard || guard_max > high1 * VG_MARGIN_HI);
       if (possible) {
+        if (level_first < r[1].first) r[1].levels = (r[1].levels & ~255u) | i;
+        if (level_first + size > r[1].last) r[1].levels = (r[1].levels & 255u) | (i << 8);
         r[1].first = min(r[1].first, level_first);
         r[1].last = max(r[1].last, level_first + size);
         r[1].possible += size;
@@ -448,8 +455,9 @@ enum : uint32_t { A_INSTANCES, A_OCC_TESTS, A_OCCLUDED, A_SURVIVED0, A_SURVIVED1
 // drawn without cluster tests. The cell's items are appended with one atomic, lane-major, so item order
 // and first-task order agree. Counters are summed in registers while the view stays the same, then in
 // the block's LDS (or directly in memory when a launch has more than kBlockViews views).
-__global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_round) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_instances(Params p, uint32_t retry_round) {
   __shared__ float4 level_table[4][2 * kMaxLevels];  // per wave: the current mesh's level table
+  __shared__ uint2 level_blk[4][kMaxLevels];         // per wave: its levels' first block, last block + 1
   __shared__ unsigned long long view_acc[kBlockViews][kAccWords];
   if (retry_round && !p.flags[retry_round - 1]) return;  // grid-uniform
   const bool block_acc = p.view_count <= kBlockViews;
@@ -554,6 +562,7 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         for (uint32_t i = lane; i < 2 * level_count; i += 64) level_table[wib][i] = p.mesh_table[level_base + i];
+        for (uint32_t i = lane; i < level_count; i += 64) level_blk[wib][i] = p.level_blocks[(level_base - p.slots * kMeshStride) / 2 + i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -666,12 +675,13 @@ __global__ __launch_bounds__(256) void k_instances(Params p, uint32_t retry_roun
         }
       }
       // Work items cover the LOD blocks of [first, last) (level ranges hold whole runs, runs and blocks
-      // never cross a level's bounds).
+      // never cross a level's bounds): from the block of the first level's first meshlet to that of the last
+      // level's last one.
 #pragma unroll
       for (uint32_t cut = 0; cut < 2; cut++) {
         if (work[cut]) {
-          item_lo[r][cut] = p.r_block[p.m_run[lr[cut].first]];
-          item_hi[r][cut] = p.r_block[p.m_run[lr[cut].last - 1]] + 1;
+          item_lo[r][cut] = level_blk[wib][lr[cut].levels & 255u].x;
+          item_hi[r][cut] = level_blk[wib][lr[cut].levels >> 8].y;
         }
       }
     }
@@ -1137,7 +1147,8 @@ __device__ __forceinline__ uint32_t bin_of(const float* __restrict__ t, uint32_t
 }
 
 __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
-  const uint32_t lane = lane_id();
+  __shared__ float4 hzb_rows[4][kHzbMaxLevels];  // per wave: the current view's HZB level rows (occlusion tests)
+  const uint32_t lane = lane_id(), wib = threadIdx.x / 64;
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), wrank = xcd_wave_rank();
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (wrank + 1) / waves);
@@ -1163,6 +1174,12 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
     const float4 flags = p.mesh_table[slot * kMeshStride + 5];
     const bool occlusion_on = flags.z > 0.5f;
     const float min_pixels = fmaxf(v.min_pixel_radius, levers.y);
+    if (occlusion_on && v.hzb_valid) {  // wave-uniform: the level table into the wave's LDS rows
+      reinterpret_cast<float*>(hzb_rows[wib])[lane] = p.hzbs[v.slot][lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     for (uint32_t r = 0; r < kCellSize; r += 64) {
       const bool valid = r + lane < count;
       const uint32_t instance = first_instance + r + lane;
@@ -1183,7 +1200,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       keep = keep && !((view_distance - radius) > levers.x);
       if (keep && occlusion_on && v.hzb_valid) {
         const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
-        keep = !occluded(v, p.hzbs[v.slot], vc.x, vc.y, vc.z, radius + margin);
+        keep = !occluded_by(v, p.hzbs[v.slot], HzbLevelsLds{hzb_rows[wib]}, vc.x, vc.y, vc.z, radius + margin);
       }
       const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
       const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
