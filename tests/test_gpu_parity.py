@@ -256,11 +256,11 @@ def test_graph_replays_match_eager(gpu_available):
         compare_cut(gpu, ref, loaded.ctx.config.max_meshlets, f"graph frame {f}")
 
 
-@pytest.mark.parametrize("copies", [1, 6])
+@pytest.mark.parametrize("copies", [1, 6, 14])
 def test_triangle_budget_pick_and_cut(gpu_available, copies):
     """Budget views pick the oracle's threshold on the GPU and cut exactly as the oracle does at it; a view
-    without a budget in the same launch is unaffected. 3 views keep every view's histogram in the block's
-    LDS; 18 views are past that limit, and each wave keeps its current view's histogram."""
+    without a budget in the same launch is unaffected. 3 and 18 views keep every view's histogram in the
+    block's LDS; 42 views are past that limit, and each wave keeps its current view's histogram."""
     spec = small_ruins(seed=61, shadows=True)
     loaded = load_gpu(spec, max_views=3 * copies, max_triangles=20_000_000, max_shadow=10_000_000)
     bins = loaded.ctx.config.budget_bins

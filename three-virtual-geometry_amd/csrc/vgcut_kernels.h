@@ -41,7 +41,7 @@ enum : uint32_t {
 constexpr uint32_t kMaxBudgetBins = 1024;
 // k_budget_meshlets keeps one histogram per launched view in its block's LDS (32-bit partial sums, flushed by
 // exchange) when they fit in this many words; otherwise each wave keeps its current view's histogram.
-constexpr uint32_t kBudgetBlockHistWords = 4352;  // 17 KB: 16 views at the default 256 bins
+constexpr uint32_t kBudgetBlockHistWords = 10240;  // 40 KB: 39 views at the default 256 bins (with the static LDS, under 64 KB)
 // A wave flushes its block's partial sums after adding this many meshlets (of at most 255 triangles each):
 // between two flushes, each of the four waves adds at most 2^20 + 64 meshlets, so any one word moves by less
 // than 4 * (2^20 + 64) * 255 < 2^31.
