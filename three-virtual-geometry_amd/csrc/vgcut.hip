@@ -139,13 +139,17 @@ void lane_destroy(Lane* lane) {
 
 void transient_sizes(const vg_scene* s, uint32_t count, size_t* queue_bytes, size_t* work_items) {
   // Every (view, cell) can be queued once. A partition of the work list takes the items of at most
-  // ceil(entries / kWorkParts) + 2 queue entries, each at most 2 cuts x 128 instances, nor more than all
-  // (view, instance, cut) pairs.
+  // ceil(entries / kWorkParts) + 2 queue entries (partitions by wave rank, launches of fewer than kViewPartsMin
+  // views) or of the entries of ceil(count / kWorkParts) views (partitions by view), each entry at most
+  // 2 cuts x 128 instances, nor more than all of those views' (instance, cut) pairs.
   const size_t cells = std::max<size_t>(s->c_sphere.size(), 1);
   *queue_bytes = sizeof(uint2) * cells * count;
   const uint64_t entries = static_cast<uint64_t>(cells) * count;
-  const uint64_t per_part = std::min<uint64_t>(((entries + kWorkParts - 1) / kWorkParts + 2) * 2 * kCellSize,
-                                               static_cast<uint64_t>(s->i_col[0].size()) * 2 * count + 64);
+  const bool by_view = count >= kViewPartsMin;
+  const uint64_t views_per_part = by_view ? (count + kWorkParts - 1) / kWorkParts : count;
+  const uint64_t entries_per_part = by_view ? static_cast<uint64_t>(cells) * views_per_part : (entries + kWorkParts - 1) / kWorkParts + 2;
+  const uint64_t per_part = std::min<uint64_t>(entries_per_part * 2 * kCellSize,
+                                               static_cast<uint64_t>(s->i_col[0].size()) * 2 * views_per_part + 64);
   *work_items = std::min<uint64_t>(per_part, kMaxWorkItems / kWorkParts) * kWorkParts;
 }
 

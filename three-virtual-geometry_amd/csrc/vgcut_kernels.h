@@ -31,6 +31,8 @@ constexpr uint32_t kMaxWorkItems = (1u << 28) - 1;
 // which spreads the appends over kWorkParts atomics instead of one. The meshlet pass walks the
 // partitions' task ranges one after another as one flat task space.
 constexpr uint32_t kWorkParts = 16;
+// Launches of at least this many views split the work list by view (partition = view % kWorkParts).
+constexpr uint32_t kViewPartsMin = kWorkParts;
 constexpr uint32_t kWorkCounterSpacing = 32;
 // Metrics words (u64) per view slot.
 constexpr uint32_t kMetricWords = 16;

@@ -449,6 +449,12 @@ __device__ void mark_range(uint32_t* bits, uint32_t first, uint32_t last) {
 constexpr uint32_t kBlockViews = 64;
 enum : uint32_t { A_INSTANCES, A_OCC_TESTS, A_OCCLUDED, A_SURVIVED0, A_SURVIVED1, A_WORK0, A_WORK1, A_CLUSTERS0, A_CLUSTERS1, kAccWords };
 
+// The work-list partition of a queue entry's items (wave-uniform): by view when a launch has at least
+// kViewPartsMin views, by wave rank otherwise (transient_sizes sizes the partitions for either).
+__device__ __forceinline__ uint32_t work_part(const Params& p, uint32_t vi, uint64_t rank, uint64_t waves) {
+  return p.view_count >= kViewPartsMin ? vi % kWorkParts : static_cast<uint32_t>(rank * kWorkParts / waves);
+}
+
 // One wave per queued (view, cell), the cell's 128 instances in two rounds of 64 lanes. Each
 // (instance, cut) that survives the culls becomes at most one work item: a range of LOD runs for the
 // meshlet pass, or, on the fast path, the instance's <= kFastPathMax guaranteed meshlets flagged to be
@@ -470,11 +476,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   const uint32_t q_begin = static_cast<uint32_t>(queue_count * rank / waves);
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (rank + 1) / waves);
   if (q_begin < q_end) {
-  // This wave's work-list partition: ranks are split evenly, so each partition takes the items of an
-  // even share of the queue.
-  const uint32_t part = static_cast<uint32_t>(rank * kWorkParts / waves);
-  const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
-
   // Counters of the current view (wave-uniform), flushed when the view changes.
   uint32_t acc_slot = 0xffffffffu, acc_vi = 0;
   uint32_t acc_instances = 0, acc_occ_tests = 0, acc_occluded = 0, acc_survived[2] = {0, 0}, acc_work[2] = {0, 0};
@@ -525,6 +526,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint32_t vi = uniform(entry.x) & kQueueViewMask;
     const uint32_t qbits = uniform(entry.x) & ~kQueueViewMask;  // frustum answers settled for the whole cell
     const uint32_t cell = uniform(entry.y);
+    // The work-list partition: with at least kViewPartsMin views, the entry's view's (a partition then holds the
+    // items of few views, and the meshlet pass, which takes 64 consecutive tasks at a time and runs its
+    // per-view loops once per view present, meets few views per chunk); else the wave's (ranks split evenly).
+    const uint32_t part = work_part(p, vi, rank, waves);
+    const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
     if (q + 1 < q_end) entry = scalar_u2(p.cell_queue, q + 1);  // next entry, loaded while this one runs
     const auto& v = scalar(p.views)[vi];
     bool redo_camera = true, redo_shadow = p.phase != 2;  // the second phase has no shadow cut
@@ -1152,14 +1158,14 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   const uint32_t queue_count = *p.cell_queue_count;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), wrank = xcd_wave_rank();
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (wrank + 1) / waves);
-  const uint32_t part = static_cast<uint32_t>(wrank * kWorkParts / waves);  // this wave's work-list partition
-  const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
   uint32_t high_water = 0;  // the most items the partition held after one of this wave's appends
   for (uint32_t q = static_cast<uint32_t>(queue_count * wrank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
     const uint32_t vi = uniform(entry.x) & kQueueViewMask;
     const bool cell_in_camera = (uniform(entry.x) & kQInCamera) != 0;  // settled for the whole cell by k_cells
     const uint32_t cell = uniform(entry.y);
+    const uint32_t part = work_part(p, vi, wrank, waves);  // as in k_instances
+    const size_t part_base = static_cast<size_t>(part) * p.work_capacity;
     const ViewDev& v = p.views[vi];
     if (!v.budget) continue;
     const float4 record = p.c_rec[cell];
