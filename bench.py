@@ -387,16 +387,14 @@ def two_phase_bench(scene, vd, steps):
             scene.sync()
             t[i] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        if f > 0:
-            for slot, vp, w, h in rasters:
-                scene.hzb_from_cut(slot, vp, w, h, with_phase2=True)
+        if f > 0:  # every occluded view's raster in one launch sequence (vg_hzb_from_cuts)
+            scene.hzb_from_cuts([(slot, vp, w, h, True) for slot, vp, w, h in rasters])
         mark(0, t0)
         t0 = time.perf_counter()
         scene.cut(views, graph=True)
         mark(1, t0)
         t0 = time.perf_counter()
-        for slot, vp, w, h in rasters:
-            scene.hzb_from_cut(slot, vp, w, h)
+        scene.hzb_from_cuts([(slot, vp, w, h, False) for slot, vp, w, h in rasters])
         mark(2, t0)
         t0 = time.perf_counter()
         scene.cut_phase2(two)
@@ -456,7 +454,7 @@ def two_phase_bench(scene, vd, steps):
         if t is None:
             traffic, source = None, [src]
             break
-        traffic += t * 2 * len(rasters)  # per dispatch; two rasters per view per frame
+        traffic += t * 2  # per dispatch; one launch sequence per phase draws every view's raster, two per frame
         source.append(src)
     out = {"views_two_phase": len(two), "camera_step_m_per_frame": speed, "ms_per_frame": sum(ms), "raster_previous_ms": ms[0],
            "cut_ms_with_recording": ms[1], "raster_first_phase_ms": ms[2], "phase2_ms": ms[3], "instances_rejected": rejected_instances,

@@ -172,6 +172,17 @@ int vg_hzb_invalidate(vg_scene* scene, uint32_t view_slot);
  */
 int vg_hzb_from_cut(vg_scene* scene, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height,
                     uint32_t with_phase2);
+/* One raster of vg_hzb_from_cuts: the arguments of a vg_hzb_from_cut. */
+typedef struct vg_raster {
+  uint32_t view_slot;
+  uint32_t width, height;
+  uint32_t with_phase2;
+  float view_proj[16];
+} vg_raster;
+/* vg_hzb_from_cut for `count` distinct view slots (at most 256) in one launch sequence: the same depth
+ * images and HZBs as `count` vg_hzb_from_cut calls. The reference's occluder pass draws one camera at a
+ * time (OcclusionCulling.ts:244-273); this batched form has no counterpart there. */
+int vg_hzb_from_cuts(vg_scene* scene, const vg_raster* rasters, uint32_t count);
 /* The slot's HZB buffer (OcclusionCulling hzbAttribute): `size_floats` receives its length; dst may be NULL.
  * Each level has a region sized for hzb_max_size; its texels outside the level's width x height are 0. */
 int vg_hzb_read(vg_scene* scene, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats);

@@ -132,3 +132,34 @@ def test_chunk_queue_overflow_and_a_smaller_image(gpu_available):
     capacity = 1 << 20
     assert chunks[0] > capacity and chunks[0] % (2 * 16 * 16) == 0 and capacity % (16 * 16) == 0
     assert chunks[1] > capacity and capacity % (16 * 9) != 0
+
+
+def test_batched_rasters_equal_single_ones(gpu_available):
+    """vg_hzb_from_cuts: three slots of different image sizes and cameras drawn in one launch sequence give
+    each slot the HZB that its own vg_hzb_from_cut and the oracle's raster give, whole buffers compared; a
+    slot named twice is refused."""
+    spec = small_ruins(seed=41, instances=3000)
+    loaded = load_gpu(spec, max_views=3)
+    scene, pool = loaded.scene, loaded.pool
+    cams = [ground_camera((0.0, 2.0, -70.0), (0.0, 0.0, 60.0)), ground_camera((50.0, 2.0, 10.0), (-60.0, 2.0, 0.0)),
+            ground_camera((-40.0, 6.0, 30.0), (30.0, 0.0, -20.0))]
+    sizes = [(480, 270), (320, 200), (640, 360)]
+    views = [make_view(c, 540, slot=i) for i, c in enumerate(cams)]
+    scene.cut(views)
+    scene.sync()
+    rasters = [(v.slot, vp_of(cams[(i + 1) % 3]), w, h, False) for i, (v, (w, h)) in enumerate(zip(views, sizes))]
+    print("batched raster of 3 slots", flush=True)
+    scene.hzb_from_cuts(rasters)
+    batched = [scene.hzb(v.slot) for v in views]
+    for (slot, vp, w, h, _), got in zip(rasters, batched):
+        counters, _, draws, _ = scene.read(slot)
+        n = min(int(counters[2]), loaded.ctx.config.max_meshlets)
+        depth = oracle.raster(pool, draws[0], n, vp, w, h)
+        want = oracle.hzb_build(depth, loaded.ctx.config.hzb_max_size)
+        assert np.array_equal(got[:len(want)], want), f"slot {slot}: batched raster HZB differs from the oracle"
+        assert float((depth < 1).mean()) > 0.02
+        print(f"single raster of slot {slot}", flush=True)
+        scene.hzb_from_cut(slot, vp, w, h)
+        assert np.array_equal(scene.hzb(slot), got), f"slot {slot}: batched and single rasters differ"
+    with pytest.raises(native.VgError):
+        scene.hzb_from_cuts([rasters[0], rasters[0]])

@@ -887,7 +887,7 @@ void vg_scene_destroy(vg_scene* s) {
                     &s->d_draw_ptrs, &s->d_hzb_ptrs, &s->d_hzb_stage, &s->d_fp_instances, &s->d_fp_meshlets, &s->d_fp_counts,
                     &s->d_first, &s->d_tri, &s->d_islot, &s->d_index_ptrs, &s->d_counters2, &s->d_args2, &s->d_reject_bits,
                     &s->d_reject_meshlets, &s->d_phase, &s->d_vpos, &s->d_vlist, &s->d_raster_chunks,
-                    &s->d_raster_count};
+                    &s->d_raster_count, &s->d_raster_views};
   for (DevBuf* b : bufs) release(s, *b);
   for (DevBuf& b : s->d_draws) release(s, b);
   for (DevBuf& b : s->d_indices) release(s, b);
@@ -1218,8 +1218,8 @@ int vg_hzb_set(vg_scene* s, uint32_t view_slot, const float* depth, uint32_t wid
                                hipMemcpyHostToDevice, ctx->stream));
   }
   float* hzb = static_cast<float*>(s->d_hzbs[view_slot].ptr);
-  hipLaunchKernelGGL(k_hzb_clear, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, 4096u)), dim3(256), 0, ctx->stream,
-                     reinterpret_cast<uint32_t*>(hzb + kHzbHeader), 0u, total - kHzbHeader, nullptr);
+  hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, 4096u)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<uint32_t*>(hzb + kHzbHeader), total - kHzbHeader);
   VG_HIP(ctx, hipMemcpyAsync(hzb, header.data(), sizeof(float) * kHzbHeader, hipMemcpyHostToDevice, ctx->stream));
   VG_HIP(ctx, hipMemcpyAsync(hzb + offsets[0], depth, sizeof(float) * width * height, hipMemcpyHostToDevice, ctx->stream));
   for (uint32_t base = 0; base + 1 < levels; base += 5) {
@@ -1232,45 +1232,71 @@ int vg_hzb_set(vg_scene* s, uint32_t view_slot, const float* depth, uint32_t wid
   return VG_OK;
 }
 
-int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height, uint32_t with_phase2) {
-  if (!s || !view_proj || view_slot >= s->ctx->cfg.max_views) return VG_E_INVALID;
+int vg_hzb_from_cuts(vg_scene* s, const vg_raster* rasters, uint32_t count) {
+  if (!s || (!rasters && count)) return VG_E_INVALID;
   vg_ctx* ctx = s->ctx;
-  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_hzb_from_cut: host-only context");
+  if (!ctx->stream) return fail(ctx, VG_E_DEVICE, "vg_hzb_from_cuts: host-only context");
+  if (count == 0) return VG_OK;
   const uint32_t max_size = ctx->cfg.hzb_max_size;
-  if (width == 0 || height == 0 || width > max_size || height > max_size)
-    return fail(ctx, VG_E_INVALID, "vg_hzb_from_cut: depth size must be within [1, hzb_max_size]");
-  if (!s->d_draws[view_slot].ptr) return fail(ctx, VG_E_INVALID, "vg_hzb_from_cut: the slot has no cut yet");
+  if (count > kMaxRasters) return fail(ctx, VG_E_INVALID, "vg_hzb_from_cuts: more than 256 rasters in one call");
+  std::vector<uint8_t> seen(ctx->cfg.max_views, 0);
+  for (uint32_t i = 0; i < count; i++) {
+    const vg_raster& q = rasters[i];
+    if (q.view_slot >= ctx->cfg.max_views) return VG_E_INVALID;
+    if (q.width == 0 || q.height == 0 || q.width > max_size || q.height > max_size)
+      return fail(ctx, VG_E_INVALID, "vg_hzb_from_cuts: depth size must be within [1, hzb_max_size]");
+    if (!s->d_draws[q.view_slot].ptr) return fail(ctx, VG_E_INVALID, "vg_hzb_from_cuts: the slot has no cut yet");
+    if (seen[q.view_slot]++) return fail(ctx, VG_E_INVALID, "vg_hzb_from_cuts: a view slot appears twice");
+  }
   if (int rc = upload_scene(s)) return rc;
   uint32_t total = kHzbHeader;
   for (uint32_t dim = max_size;; dim = (dim + 1) / 2) {
     total += dim * dim;
     if (dim == 1) break;
   }
-  if (!s->d_hzbs[view_slot].ptr) {
-    if (int rc = ensure(s, s->d_hzbs[view_slot], sizeof(float) * total)) return rc;
-    s->h_hzb_ptrs[view_slot] = static_cast<float*>(s->d_hzbs[view_slot].ptr);
-    VG_HIP(ctx, hipMemcpyAsync(static_cast<float**>(s->d_hzb_ptrs.ptr) + view_slot, &s->h_hzb_ptrs[view_slot], sizeof(float*),
-                               hipMemcpyHostToDevice, ctx->stream));
+  // The rasters' records: camera, draw list, entry count (the cut's, or with its second phase's), HZB.
+  std::vector<RasterView> views(count);
+  uint32_t max_w = 0, max_h = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    const vg_raster& q = rasters[i];
+    const uint32_t slot = q.view_slot;
+    if (!s->d_hzbs[slot].ptr) {
+      if (int rc = ensure(s, s->d_hzbs[slot], sizeof(float) * total)) return rc;
+      s->h_hzb_ptrs[slot] = static_cast<float*>(s->d_hzbs[slot].ptr);
+      VG_HIP(ctx, hipMemcpyAsync(static_cast<float**>(s->d_hzb_ptrs.ptr) + slot, &s->h_hzb_ptrs[slot], sizeof(float*),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    }
+    RasterView& v = views[i];
+    std::memset(&v, 0, sizeof(v));
+    std::memcpy(v.vp, q.view_proj, sizeof(v.vp));
+    v.draws = static_cast<const uint4*>(s->d_draws[slot].ptr);
+    // Camera entries: [0, drawn) of the cut, or [0, min(selected, max)) when its second phase appended more.
+    const bool both = q.with_phase2 && s->phase2_ran[slot] && s->d_counters2.ptr;
+    const uint32_t* counters = static_cast<const uint32_t*>(both ? s->d_counters2.ptr : s->d_counters.ptr) +
+                               static_cast<size_t>(slot) * s->counter_stride;
+    v.count = counters + 2 + (both ? kSelected : kDrawn);
+    v.hzb = static_cast<float*>(s->d_hzbs[slot].ptr);
+    v.width = q.width;
+    v.height = q.height;
+    max_w = std::max(max_w, q.width);
+    max_h = std::max(max_h, q.height);
   }
-  float* hzb = static_cast<float*>(s->d_hzbs[view_slot].ptr);
+  if (int rc = ensure(s, s->d_raster_views, sizeof(RasterView) * kMaxRasters)) return rc;
+  // From pageable memory: the copy has taken the records when it returns.
+  VG_HIP(ctx, hipMemcpyAsync(s->d_raster_views.ptr, views.data(), sizeof(RasterView) * count, hipMemcpyHostToDevice, ctx->stream));
   RasterParams r;
+  std::memset(&r, 0, sizeof(r));
   for (int k = 0; k < 4; k++) r.col[k] = static_cast<const float4*>(s->d_col[k].ptr);
   r.first = static_cast<const uint32_t*>(s->d_first.ptr);
   r.tris = static_cast<const uint32_t*>(s->d_tris.ptr);
   r.vlist = static_cast<const uint2*>(s->d_vlist.ptr);
   r.tri = static_cast<const uint32_t*>(s->d_tri.ptr);
   r.vpos = static_cast<const float4*>(s->d_vpos.ptr);
-  r.draws = static_cast<const uint4*>(s->d_draws[view_slot].ptr);
-  // Camera entries: [0, drawn) of the cut, or [0, min(selected, max)) when its second phase appended more.
-  const bool both = with_phase2 && s->phase2_ran[view_slot] && s->d_counters2.ptr;
-  const uint32_t* counters = static_cast<const uint32_t*>(both ? s->d_counters2.ptr : s->d_counters.ptr) +
-                             static_cast<size_t>(view_slot) * s->counter_stride;
-  r.count = counters + 2 + (both ? kSelected : kDrawn);
+  r.views = static_cast<const RasterView*>(s->d_raster_views.ptr);
+  r.rasters = count;
   r.clamp = ctx->cfg.max_meshlets;
-  r.width = width;
-  r.height = height;
-  r.depth = reinterpret_cast<uint32_t*>(hzb + kHzbHeader);
-  std::memcpy(r.vp, view_proj, sizeof(r.vp));
+  r.max_size = max_size;
+  r.clear_texels = total - kHzbHeader;
   constexpr uint32_t kChunkCapacity = 1u << 20;  // 32 MB of queued rectangles; beyond it the waves fill them
   if (!s->d_raster_chunks.ptr) {
     if (int rc = ensure(s, s->d_raster_chunks, sizeof(RasterChunk) * kChunkCapacity)) return rc;
@@ -1279,9 +1305,9 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
   r.chunks = static_cast<RasterChunk*>(s->d_raster_chunks.ptr);
   r.chunk_count = static_cast<uint32_t*>(s->d_raster_count.ptr);
   r.chunk_capacity = kChunkCapacity;
-  hipLaunchKernelGGL(k_hzb_header, dim3(1), dim3(64), 0, ctx->stream, hzb, max_size, width, height);
-  hipLaunchKernelGGL(k_hzb_clear, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, 4096u)), dim3(256), 0, ctx->stream, r.depth,
-                     width * height, total - kHzbHeader, r.chunk_count);
+  hipLaunchKernelGGL(k_hzb_header, dim3(count), dim3(64), 0, ctx->stream, r);
+  hipLaunchKernelGGL(k_hzb_clear, dim3(std::min<uint32_t>((total - kHzbHeader + 255) / 256, std::max<uint32_t>(4096u / count, 64u)), count),
+                     dim3(256), 0, ctx->stream, r);
   if (!s->m_verts.empty()) {
     const dim3 grid(static_cast<uint32_t>(ctx->cus) * kRasterBlocksPerCu);
     std::vector<hipEvent_t> ev;
@@ -1298,7 +1324,8 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
       s->pending.push_back(ev);
     }
   }
-  uint32_t lw = width, lh = height, levels = 0;
+  // The pyramid, 5 levels per pass, every raster in one launch per pass (grids sized for the largest image).
+  uint32_t lw = max_w, lh = max_h, levels = 0;
   std::vector<uint32_t> ws, hs;
   for (uint32_t dim = max_size;; dim = (dim + 1) / 2) {
     ws.push_back(lw);
@@ -1309,12 +1336,23 @@ int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uin
     lh = lh > 1 ? (lh + 1) / 2 : 1;
   }
   for (uint32_t base = 0; base + 1 < levels; base += 5) {
-    dim3 grid((ws[base] + 31) / 32, (hs[base] + 31) / 32);
-    hipLaunchKernelGGL(k_hzb_pass, grid, dim3(256), 0, ctx->stream, hzb, base);
+    dim3 grid((ws[base] + 31) / 32, (hs[base] + 31) / 32, count);
+    hipLaunchKernelGGL(k_hzb_pass_rasters, grid, dim3(256), 0, ctx->stream, r, base);
   }
   VG_HIP(ctx, hipGetLastError());
-  s->hzb_valid[view_slot] = 1;
+  for (uint32_t i = 0; i < count; i++) s->hzb_valid[rasters[i].view_slot] = 1;
   return VG_OK;
+}
+
+int vg_hzb_from_cut(vg_scene* s, uint32_t view_slot, const float* view_proj, uint32_t width, uint32_t height, uint32_t with_phase2) {
+  if (!s || !view_proj) return VG_E_INVALID;
+  vg_raster q;
+  q.view_slot = view_slot;
+  q.width = width;
+  q.height = height;
+  q.with_phase2 = with_phase2;
+  std::memcpy(q.view_proj, view_proj, sizeof(q.view_proj));
+  return vg_hzb_from_cuts(s, &q, 1);
 }
 
 int vg_hzb_read(vg_scene* s, uint32_t view_slot, float* dst, uint64_t capacity_floats, uint64_t* size_floats) {

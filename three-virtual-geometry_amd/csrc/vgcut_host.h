@@ -130,6 +130,7 @@ struct vg_scene {
   // meshlet's are contiguous; the vertex lists' (offset, count).
   vgi::DevBuf d_vpos, d_vlist;
   vgi::DevBuf d_raster_chunks, d_raster_count;  // the raster's queue of large-triangle rectangles
+  vgi::DevBuf d_raster_views;  // the raster records of the last vg_hzb_from_cuts
   // per view slot
   vgi::DevBuf d_counters, d_args, d_metrics, d_draw_ptrs, d_hzb_ptrs, d_index_ptrs;
   std::vector<vgi::DevBuf> d_draws, d_hzbs;

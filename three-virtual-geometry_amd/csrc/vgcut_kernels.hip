@@ -1865,12 +1865,15 @@ __global__ __launch_bounds__(256) void k_expand(Params p) {
 // ------------------------------------------------------------------ HZB (OcclusionCulling.ts:302-350)
 
 // Reduces levels base+1 .. base+5 (those that exist) over 32 x 32 texel tiles of level `base`.
-__global__ __launch_bounds__(256) void k_hzb_pass(float* hzb, uint32_t base) {
+// One block per 32 x 32 tile of level `base` of `hzb`; a block outside the level (a grid sized for the largest
+// of several HZBs) returns at once.
+__device__ __forceinline__ void hzb_pass(float* hzb, uint32_t base) {
   __shared__ float tile[2][32 * 32];
   const uint32_t levels = static_cast<uint32_t>(hzb[kHzbLevelCountSlot]);
   const uint32_t W = static_cast<uint32_t>(hzb[base * 4 + 1]), H = static_cast<uint32_t>(hzb[base * 4 + 2]);
   const uint32_t src_offset = static_cast<uint32_t>(hzb[base * 4]);
   const uint32_t tx = blockIdx.x * 32, ty = blockIdx.y * 32;
+  if (base >= levels || tx >= W || ty >= H) return;  // block-uniform (header rows past the last level are 0)
   for (uint32_t i = threadIdx.x; i < 32 * 32; i += blockDim.x) {
     const uint32_t x = min(tx + (i & 31), W - 1), y = min(ty + (i >> 5), H - 1);
     tile[0][i] = hzb[src_offset + y * W + x];
@@ -1901,6 +1904,13 @@ __global__ __launch_bounds__(256) void k_hzb_pass(float* hzb, uint32_t base) {
     __syncthreads();
     cur ^= 1;
   }
+}
+
+__global__ __launch_bounds__(256) void k_hzb_pass(float* hzb, uint32_t base) { hzb_pass(hzb, base); }
+
+// words[0, n) <- 0 (an HZB's texel area before vg_hzb_set writes its image).
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* words, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) words[i] = 0u;
 }
 
 // Population count of a bitmap into *out (measurement only).

@@ -2,6 +2,8 @@
 // entries of a view slot's draw list are drawn depth-only into level 0 of the slot's HZB buffer, which
 // k_hzb_pass then reduces. The reference draws the occluders' previous cuts with the hardware raster
 // (three.js, depth-only material); here the same draw list is read where the cut left it on the device.
+// One launch sequence draws any number of rasters (view slots, each with its camera and image size): the
+// kernels take a table of RasterView records and walk all their entries as one task space.
 //
 //   k_hzb_header  writes the HZB header (level offsets and sizes) for a width x height level 0
 //   k_hzb_clear   level 0 <- 1.0 (WebGPU far depth), every other texel of the buffer <- 0
@@ -25,6 +27,19 @@ constexpr uint32_t kRasterBlocksPerCu = 8;
 constexpr uint32_t kRasterSub = 4;  // meshlets per wave
 constexpr int32_t kChunkSide = 64;    // large triangles are filled in rectangles of this side, one block each
 
+// One raster of a launch: a view slot's camera entries drawn into its HZB.
+struct RasterView {
+  float vp[16];           // column-major clip = vp * world
+  const uint4* draws;     // the slot's draw list; camera entries from position 0
+  const uint32_t* count;  // entries to draw: min(*count, clamp)
+  float* hzb;             // the slot's HZB buffer (header, then level 0 at kHzbHeader: float bits, row 0 at the top)
+  uint32_t width, height;
+  uint32_t pad[2];
+};
+
+// Most rasters of one launch (k_raster keeps their task ends in LDS).
+constexpr uint32_t kMaxRasters = 256;
+
 struct RasterParams {
   const float4* col[4];   // instance matrix columns
   const uint32_t* first;  // per meshlet: firstIndex (pool-rebased); its triangles are at first / 3
@@ -32,16 +47,17 @@ struct RasterParams {
   const uint2* vlist;     // per meshlet: (offset into vpos, vertex count; 0 = no positions)
   const uint32_t* tri;    // packed local vertex ids a | b << 8 | c << 16
   const float4* vpos;     // per meshlet vertex, in vertex-list order: position (x, y, z, 0)
-  const uint4* draws;     // the slot's draw list; camera entries from position 0
-  const uint32_t* count;  // entries to draw: min(*count, clamp)
+  const RasterView* views;
+  uint32_t rasters;       // RasterView records (<= kMaxRasters)
   uint32_t clamp;
-  uint32_t width, height;
-  uint32_t* depth;        // level 0 of the HZB, float bits, row 0 at the top
+  uint32_t max_size;      // hzb_max_size: every HZB buffer has the same layout
+  uint32_t clear_texels;  // texels of an HZB buffer after its header
   struct RasterChunk* chunks;  // large triangles, cut into pixel rectangles of at most kChunkSide^2
   uint32_t* chunk_count;
   uint32_t chunk_capacity;
-  float vp[16];           // column-major clip = vp * world
 };
+
+__device__ __forceinline__ uint32_t* raster_depth(const RasterView& v) { return reinterpret_cast<uint32_t*>(v.hzb + kHzbHeader); }
 
 struct RasterVertex {
   int32_t x, y;  // screen position in 1/256 pixel (y down); x == kRasterNoVertex: not drawable
@@ -49,7 +65,7 @@ struct RasterVertex {
 };
 
 // Transforms an object-space position to a snapped screen vertex; false when it cannot be drawn.
-__device__ __forceinline__ RasterVertex raster_vertex(const RasterParams& r, float4 c0, float4 c1, float4 c2, float4 c3, float4 p) {
+__device__ __forceinline__ RasterVertex raster_vertex(const RasterView& r, float4 c0, float4 c1, float4 c2, float4 c3, float4 p) {
   const float wx = ((c0.x * p.x + c1.x * p.y) + c2.x * p.z) + c3.x;
   const float wy = ((c0.y * p.x + c1.y * p.y) + c2.y * p.z) + c3.y;
   const float wz = ((c0.z * p.x + c1.z * p.y) + c2.z * p.z) + c3.z;
@@ -151,6 +167,7 @@ struct RasterChunk {
   RasterVertex a, b, c;
   int32_t area_lo, area_hi;
   int32_t x0, x1, y0, y1;
+  uint32_t raster;  // its RasterView
 };
 
 __device__ __forceinline__ uint32_t raster_box_pixels(const RasterTriangle& t) {
@@ -158,7 +175,11 @@ __device__ __forceinline__ uint32_t raster_box_pixels(const RasterTriangle& t) {
   return static_cast<uint32_t>(t.x1 - t.x0 + 1) * static_cast<uint32_t>(t.y1 - t.y0 + 1);
 }
 
-__global__ __launch_bounds__(64) void k_hzb_header(float* hzb, uint32_t max_size, uint32_t width, uint32_t height) {
+// One block per raster.
+__global__ __launch_bounds__(64) void k_hzb_header(RasterParams r) {
+  const RasterView& v = r.views[blockIdx.x];
+  float* hzb = v.hzb;
+  const uint32_t max_size = r.max_size, width = v.width, height = v.height;
   for (uint32_t i = threadIdx.x; i < kHzbHeader; i += blockDim.x) hzb[i] = 0.0f;
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -178,11 +199,14 @@ __global__ __launch_bounds__(64) void k_hzb_header(float* hzb, uint32_t max_size
   hzb[kHzbLevelCountSlot] = static_cast<float>(levels);
 }
 
-// Clears an HZB's texel area: the first `far` texels (a depth image about to be rasterized) to 1.0 and the
-// rest of the `n` texels (every level's area, used or not) to 0, so the buffer holds no earlier contents
-// whatever the image size. Also resets the chunk queue when one is given.
-__global__ __launch_bounds__(256) void k_hzb_clear(uint32_t* texels, uint32_t far, uint32_t n, uint32_t* chunk_count) {
-  if (chunk_count && blockIdx.x == 0 && threadIdx.x == 0) *chunk_count = 0;
+// Clears each raster's HZB texel area: the first width x height texels (the depth image about to be drawn)
+// to 1.0 and the rest of its clear_texels (every level's area, used or not) to 0, so the buffer holds no
+// earlier contents whatever the image size. blockIdx.y is the raster. Also resets the chunk queue.
+__global__ __launch_bounds__(256) void k_hzb_clear(RasterParams r) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *r.chunk_count = 0;
+  const RasterView& v = r.views[blockIdx.y];
+  uint32_t* texels = raster_depth(v);
+  const uint32_t far = v.width * v.height, n = r.clear_texels;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) texels[i] = i < far ? 0x3f800000u : 0u;
 }
 
@@ -190,25 +214,47 @@ __global__ __launch_bounds__(256) void k_hzb_clear(uint32_t* texels, uint32_t fa
 // positions, contiguous in vertex-list order, and packed triangles) is issued before any is used, and the
 // groups' meshlets are in flight together, so the dependent chain entry -> meshlet record -> positions costs
 // one wait per kSub meshlets.
+// A wave-task is kSub consecutive entries of one raster (so the raster's camera and image are wave-uniform):
+// the tasks of all rasters form one space, walked by the persistent waves; task_end[k] (in LDS) is the
+// end of raster k's tasks.
 template <uint32_t kSub>
 __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
   constexpr uint32_t kLanes = 64 / kSub, kPer = 128 / kLanes;
   __shared__ RasterVertex lds[4][kSub][128];
+  __shared__ uint32_t task_end[kMaxRasters];
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, group = lane / kLanes, gl = lane % kLanes;
   RasterVertex* V = lds[wave][group];
   const uint64_t group_mask = (kSub == 1 ? ~0ull : ((1ull << kLanes) - 1ull)) << (group * kLanes);
-  const uint32_t n = min(*r.count, r.clamp);
-  const uint32_t slots = gridDim.x * (blockDim.x >> 6) * kSub;
+  if (wave == 0) {  // the rasters' task ends: a prefix over ceil(min(count, clamp) / kSub)
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < r.rasters; b0 += 64) {
+      const uint32_t k = b0 + lane;
+      const uint32_t n = k < r.rasters ? min(*r.views[k].count, r.clamp) : 0u;
+      const uint32_t incl = carry + wave_scan((n + kSub - 1) / kSub);
+      if (k < r.rasters) task_end[k] = incl;
+      carry = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    }
+  }
+  __syncthreads();
+  const uint32_t total = task_end[r.rasters - 1];
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   auto wave_sync = [&]() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  for (uint32_t base = xcd_wave_rank() * kSub; base < n; base += slots) {
-    const uint32_t e = base + group;
+  uint32_t rk = 0;  // the raster of the current task (tasks only increase, so the search moves forward)
+  for (uint32_t t = xcd_wave_rank(); t < total; t += waves) {
+    while (task_end[rk] <= t) rk++;  // rk < rasters: t < total = task_end[rasters - 1]
+    const uint32_t k = uniform(rk);
+    const RasterView& rv = r.views[k];
+    const uint32_t n = min(*rv.count, r.clamp);
+    const uint32_t e = (t - (k ? task_end[k - 1] : 0u)) * kSub + group;
+    uint32_t* depth = raster_depth(rv);
+    const uint32_t width = rv.width, height = rv.height;
     uint32_t nv = 0, tris = 0, first = 0, voff = 0, instance = 0;
     if (e < n) {
-      const uint4 entry = r.draws[e];
+      const uint4 entry = rv.draws[e];
       instance = entry.x;
       const uint2 vl = r.vlist[entry.y];
       voff = vl.x;
@@ -231,7 +277,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
       const float4 c0 = r.col[0][instance], c1 = r.col[1][instance], c2 = r.col[2][instance], c3 = r.col[3][instance];
 #pragma unroll
       for (uint32_t j = 0; j < kPer; j++)
-        if (gl + j * kLanes < nv) V[gl + j * kLanes] = raster_vertex(r, c0, c1, c2, c3, pos[j]);
+        if (gl + j * kLanes < nv) V[gl + j * kLanes] = raster_vertex(rv, c0, c1, c2, c3, pos[j]);
     }
     wave_sync();
     bool big[kPer];
@@ -240,7 +286,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
       big[j] = false;
       if (gl + j * kLanes >= tris) continue;
       const uint32_t p = packed[j];
-      const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], r.width, r.height);
+      const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], width, height);
       const uint32_t pixels = raster_box_pixels(tr);
       if (pixels > kRasterLanePixels) {
         // Large: its kChunkSide^2 rectangles go to the chunk queue (filled by every block of k_raster_chunks);
@@ -270,11 +316,12 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
           c.y0 = tr.y0 + static_cast<int32_t>(q / cw) * kChunkSide;
           c.x1 = min(c.x0 + kChunkSide - 1, tr.x1);
           c.y1 = min(c.y0 + kChunkSide - 1, tr.y1);
+          c.raster = k;
           r.chunks[at + q] = c;
         }
         continue;
       }
-      raster_box(r.depth, r.width, tr);
+      raster_box(depth, width, tr);
     }
     // Large triangles the queue had no room for: each lane group strides over its meshlet's, one at a time.
 #pragma unroll
@@ -284,10 +331,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterParams r) {
         const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
         mask &= mask - 1;
         const uint32_t p = static_cast<uint32_t>(__shfl(static_cast<int>(packed[j]), static_cast<int>(src)));
-        const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], r.width, r.height);
+        const RasterTriangle tr = raster_setup(V[p & 255u], V[(p >> 8) & 255u], V[(p >> 16) & 255u], width, height);
         const uint32_t bw = static_cast<uint32_t>(tr.x1 - tr.x0 + 1), pixels = raster_box_pixels(tr);
         for (uint32_t i = gl; i < pixels; i += kLanes)
-          raster_pixel(r.depth, r.width, tr, tr.x0 + static_cast<int32_t>(i % bw), tr.y0 + static_cast<int32_t>(i / bw));
+          raster_pixel(depth, width, tr, tr.x0 + static_cast<int32_t>(i % bw), tr.y0 + static_cast<int32_t>(i / bw));
       }
     }
     wave_sync();
@@ -305,12 +352,18 @@ __global__ __launch_bounds__(256) void k_raster_chunks(RasterParams r) {
     t.b = c.b;
     t.c = c.c;
     t.area = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(c.area_hi)) << 32) | static_cast<uint32_t>(c.area_lo));
+    const RasterView& rv = r.views[c.raster];
+    uint32_t* depth = raster_depth(rv);
+    const uint32_t width = rv.width;
     for (int32_t y0 = c.y0; y0 <= c.y1; y0 += 16)
       for (int32_t x0 = c.x0; x0 <= c.x1; x0 += 16) {
         const int32_t x = x0 + tx, y = y0 + ty;
-        if (x <= c.x1 && y <= c.y1) raster_pixel(r.depth, r.width, t, x, y);
+        if (x <= c.x1 && y <= c.y1) raster_pixel(depth, width, t, x, y);
       }
   }
 }
+
+// k_hzb_pass over the launch's rasters: blockIdx.z is the raster, the grid is sized for the largest level.
+__global__ __launch_bounds__(256) void k_hzb_pass_rasters(RasterParams r, uint32_t base) { hzb_pass(r.views[blockIdx.z].hzb, base); }
 
 }  // namespace vg

@@ -35,7 +35,7 @@ LIBRARY = _native.library_path("libvgcut.so")
 # Every entry point include/vgcut.h declares (the ABI test checks the library exports them).
 EXPORTS = (
     "vg_create", "vg_destroy", "vg_last_error", "vg_scene_create", "vg_scene_destroy", "vg_scene_bytes", "vg_mesh_add",
-    "vg_instances_set", "vg_mesh_flags", "vg_mesh_remove", "vg_hzb_set", "vg_hzb_from_cut", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
+    "vg_instances_set", "vg_mesh_flags", "vg_mesh_remove", "vg_hzb_set", "vg_hzb_from_cut", "vg_hzb_from_cuts", "vg_hzb_invalidate", "vg_hzb_read", "vg_cut", "vg_cut_graph", "vg_sync", "vg_read",
     "vg_state_get", "vg_state_set", "vg_scene_pool_read", "vg_scene_mesh_slots", "vg_mesh_instance_base",
     "vg_counter_words", "vg_draw_args_words", "vg_profile", "vg_profile_read", "vg_footprint", "vg_budget_thresholds",
     "vg_schedule_get", "vg_schedule_set", "vg_tune", "vg_index_read", "vg_index_buffer",
@@ -106,6 +106,12 @@ class View(ctypes.Structure):
         ("threshold_range", ctypes.c_float * 2),
         ("max_error_threshold", ctypes.c_float),
     ]
+
+
+class Raster(ctypes.Structure):
+    """vg_raster: one raster of vg_hzb_from_cuts."""
+    _fields_ = [("view_slot", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("with_phase2", ctypes.c_uint32), ("view_proj", ctypes.c_float * 16)]
 
 
 class Schedule(ctypes.Structure):
@@ -200,6 +206,7 @@ def load():
         "vg_mesh_remove": ([P, U32], ctypes.c_int),
         "vg_hzb_set": ([P, U32, P, U32, U32], ctypes.c_int),
         "vg_hzb_from_cut": ([P, U32, P, U32, U32, U32], ctypes.c_int),
+        "vg_hzb_from_cuts": ([P, P, U32], ctypes.c_int),
         "vg_hzb_invalidate": ([P, U32], ctypes.c_int),
         "vg_hzb_read": ([P, U32, P, U64, ctypes.POINTER(U64)], ctypes.c_int),
         "vg_cut": ([P, P, U32], ctypes.c_int),
@@ -363,6 +370,15 @@ class Scene:
         vp = np.ascontiguousarray(np.asarray(view_proj, dtype=np.float32).reshape(16))
         self.ctx.check(self.lib.vg_hzb_from_cut(self.handle, view_slot, vp.ctypes.data, width, height, int(bool(with_phase2))),
                        "vg_hzb_from_cut")
+
+    def hzb_from_cuts(self, rasters):
+        """vg_hzb_from_cuts: [(view_slot, view_proj, width, height, with_phase2)] drawn in one launch sequence."""
+        arr = (Raster * len(rasters))()
+        for i, (slot, vp, w, h, phase2) in enumerate(rasters):
+            arr[i].view_slot, arr[i].width, arr[i].height, arr[i].with_phase2 = slot, w, h, int(bool(phase2))
+            for k, x in enumerate(np.asarray(vp, dtype=np.float32).reshape(16)):
+                arr[i].view_proj[k] = float(x)
+        self.ctx.check(self.lib.vg_hzb_from_cuts(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(rasters)), "vg_hzb_from_cuts")
 
     def hzb(self, view_slot: int) -> np.ndarray:
         n = ctypes.c_uint64()
