@@ -276,6 +276,9 @@ const words = vg.counterWords(ctx), argWords = vg.drawArgsWords(ctx);
 attempt('countersAsUint8', () => vg.read(scene, 0, new Uint8Array(words * 4), new Uint32Array(argWords), new Uint32Array(8)));
 attempt('argsAsInt16', () => vg.read(scene, 0, new Uint32Array(words), new Int16Array(argWords * 2), new Uint32Array(8)));
 attempt('viewProjAsFloat64', () => vg.hzbFromCut(scene, 0, new Float64Array(16), 8, 8));
+attempt('rastersNotArray', () => vg.hzbFromCuts(scene, { slot: 0 }));
+attempt('rasterViewProjShort', () => vg.hzbFromCuts(scene, [{ slot: 0, viewProj: new Float32Array(15), width: 8, height: 8 }]));
+attempt('rastersOnHostOnly', () => vg.hzbFromCuts(scene, [{ slot: 0, viewProj: new Float32Array(16), width: 8, height: 8 }]));
 // A cut runs on a worker thread: until its promise settles every call on the context throws.
 const promise = vg.cut(scene, new ArrayBuffer(292), 1);
 attempt('readWhileCutting', () => vg.read(scene, 0, new Uint32Array(words), new Uint32Array(argWords), new Uint32Array(8)));
@@ -302,8 +305,10 @@ def test_addon_checks_arrays_and_pending_cuts(tmp_path):
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["shortBounds"].startswith("RangeError"), out
     assert out["shortErrors"].startswith("RangeError"), out
-    for k in ("boundsAsFloat64", "matricesAsUint8", "countersAsUint8", "argsAsInt16", "viewProjAsFloat64"):
+    for k in ("boundsAsFloat64", "matricesAsUint8", "countersAsUint8", "argsAsInt16", "viewProjAsFloat64", "rastersNotArray",
+              "rasterViewProjShort"):
         assert out[k].startswith("TypeError"), (k, out[k])
+    assert "host-only" in out["rastersOnHostOnly"], out
     assert out["matricesRagged"].startswith("RangeError"), out
     for k in ("readWhileCutting", "setFlagsWhileCutting", "secondCutWhileCutting", "sceneWhileCutting"):
         assert "VG_BUSY" in out[k], (k, out[k])

@@ -528,6 +528,53 @@ static napi_value HzbFromCut(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// hzbFromCuts(scene, rasters: [{slot, viewProj: Float32Array(16), width, height, withPhase2?}]) -- hzbFromCut for
+// several view slots in one launch sequence (vg_hzb_from_cuts)
+static napi_value HzbFromCuts(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  Scene* s = scene_of(env, argv[0]);
+  if (!s) return nullptr;
+  bool is_array = false;
+  if (argc < 2 || napi_is_array(env, argv[1], &is_array) != napi_ok || !is_array) {
+    napi_throw_type_error(env, nullptr, "hzbFromCuts: rasters must be an array");
+    return nullptr;
+  }
+  uint32_t count = 0;
+  CHECK_NAPI(napi_get_array_length(env, argv[1], &count));
+  std::vector<vg_raster> rasters(count);
+  for (uint32_t i = 0; i < count; i++) {
+    napi_value item, v;
+    CHECK_NAPI(napi_get_element(env, argv[1], i, &item));
+    vg_raster& q = rasters[i];
+    std::memset(&q, 0, sizeof(q));
+    CHECK_NAPI(napi_get_named_property(env, item, "slot", &v));
+    CHECK_NAPI(napi_get_value_uint32(env, v, &q.view_slot));
+    CHECK_NAPI(napi_get_named_property(env, item, "width", &v));
+    CHECK_NAPI(napi_get_value_uint32(env, v, &q.width));
+    CHECK_NAPI(napi_get_named_property(env, item, "height", &v));
+    CHECK_NAPI(napi_get_value_uint32(env, v, &q.height));
+    bool has_phase2 = false, with_phase2 = false;
+    if (napi_has_named_property(env, item, "withPhase2", &has_phase2) == napi_ok && has_phase2 &&
+        napi_get_named_property(env, item, "withPhase2", &v) == napi_ok)
+      napi_get_value_bool(env, v, &with_phase2);
+    q.with_phase2 = with_phase2 ? 1u : 0u;
+    CHECK_NAPI(napi_get_named_property(env, item, "viewProj", &v));
+    size_t n = 0;
+    bool bad = false;
+    const float* vp = typed<float>(env, v, &n, &bad, "viewProj");
+    if (bad) return nullptr;
+    if (!vp || n != 16) {
+      napi_throw_type_error(env, nullptr, "hzbFromCuts: viewProj must be a Float32Array of 16");
+      return nullptr;
+    }
+    std::memcpy(q.view_proj, vp, sizeof(q.view_proj));
+  }
+  if (int rc = vg_hzb_from_cuts(s->scene, rasters.data(), count)) return throw_vg(env, rc, s->ctx);
+  return nullptr;
+}
+
 // A new typed array holding a copy of `n` elements at `src`.
 static napi_value copy_typed(napi_env env, napi_typedarray_type type, const void* src, size_t n, size_t elem) {
   void* data = nullptr;
@@ -705,6 +752,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"drawArgsWords", nullptr, DrawArgsWords, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"buildVirtualMesh", nullptr, BuildVirtualMesh, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"hzbFromCut", nullptr, HzbFromCut, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"hzbFromCuts", nullptr, HzbFromCuts, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
   return exports;
