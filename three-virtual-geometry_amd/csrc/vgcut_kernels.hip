@@ -1159,6 +1159,10 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64), wrank = xcd_wave_rank();
   const uint32_t q_end = static_cast<uint32_t>(queue_count * (wrank + 1) / waves);
   uint32_t high_water = 0;  // the most items the partition held after one of this wave's appends
+  // The current cell's per-instance values (scale, bounding-sphere centre, LOD-sphere centre), per round of
+  // 64 instances, kept while consecutive queue entries (the cell's views) name the same cell.
+  uint32_t cur_cell = 0xffffffffu;
+  float d_scale[2] = {0.0f, 0.0f}, d_center[2][3] = {{0, 0, 0}, {0, 0, 0}}, d_lq[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   for (uint32_t q = static_cast<uint32_t>(queue_count * wrank / waves); q < q_end; q++) {
     uint2 entry = p.cell_queue[q];
     const uint32_t vi = uniform(entry.x) & kQueueViewMask;
@@ -1186,18 +1190,38 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    for (uint32_t r = 0; r < kCellSize; r += 64) {
-      const bool valid = r + lane < count;
-      const uint32_t instance = first_instance + r + lane;
-      float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = make_float4(0, 0, 0, 1);
-      if (valid) {
-        c0 = p.i_col0[instance];
-        c1 = p.i_col1[instance];
-        c2 = p.i_col2[instance];
-        c3 = p.i_col3[instance];
+    if (cell != cur_cell) {  // wave-uniform: the cell's instance values, as k_instances derives them
+      cur_cell = cell;
+#pragma unroll
+      for (uint32_t r = 0; r < 2; r++) {
+        const bool valid = r * 64 + lane < count;
+        const uint32_t instance = first_instance + r * 64 + lane;
+        float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = make_float4(0, 0, 0, 1);
+        if (valid) {
+          c0 = p.i_col0[instance];
+          c1 = p.i_col1[instance];
+          c2 = p.i_col2[instance];
+          c3 = p.i_col3[instance];
+        }
+        d_scale[r] = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
+        const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+        d_center[r][0] = center.x;
+        d_center[r][1] = center.y;
+        d_center[r][2] = center.z;
+        const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
+        d_lq[r][0] = lq.x;
+        d_lq[r][1] = lq.y;
+        d_lq[r][2] = lq.z;
+        d_lq[r][3] = lq.w;
       }
-      const float scale = fmaxf(len3(c0.x, c0.y, c0.z), fmaxf(len3(c1.x, c1.y, c1.z), len3(c2.x, c2.y, c2.z)));
-      const float4 center = mat_mul(c0, c1, c2, c3, object_sphere.x, object_sphere.y, object_sphere.z, 1.0f);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 2; r++) {
+      if (r * 64 >= count) break;  // wave-uniform
+      const bool valid = r * 64 + lane < count;
+      const uint32_t instance = first_instance + r * 64 + lane;
+      const float scale = d_scale[r];
+      const float4 center = make_float4(d_center[r][0], d_center[r][1], d_center[r][2], 1.0f);
       const float radius = object_sphere.w * scale;
       bool keep = valid && (v.frustum == 0 || cell_in_camera || frustum_distance(v, center.x, center.y, center.z, 0, false) >= -radius);
       const float4 vc = mat_mul16(v.view, center.x, center.y, center.z, 1.0f);
@@ -1208,8 +1232,7 @@ __global__ __launch_bounds__(256) void k_budget_instances(Params p) {
         const float margin = (fabsf(vc.z) * v.budget_high) / v.proj_scale;
         keep = !occluded_by(v, p.hzbs[v.slot], HzbLevelsLds{hzb_rows[wib]}, vc.x, vc.y, vc.z, radius + margin);
       }
-      const float4 lq = mat_mul(c0, c1, c2, c3, lod_bounds.x, lod_bounds.y, lod_bounds.z, 1.0f);
-      const float4 lv = mat_mul16(v.view, lq.x, lq.y, lq.z, lq.w);
+      const float4 lv = mat_mul16(v.view, d_lq[r][0], d_lq[r][1], d_lq[r][2], d_lq[r][3]);
       const float d = len3(lv.x, lv.y, lv.z);
       const float rr = lod_bounds.w * scale;
       const float pmin = (scale * v.proj_scale) / fmaxf(d - rr, v.near_plane);
