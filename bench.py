@@ -622,6 +622,13 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda:0")
     if args.plain_configs:
+        if args.profile_only:
+            # A plain 1 GiB device copy first: its FETCH_SIZE / WRITE_SIZE calibrate the counters' units.
+            src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
         print(json.dumps({"plain_configs": plain_configs_bench(args.cpu_seconds)}))
         return
     t0 = time.time()
